@@ -1,0 +1,293 @@
+"""flens_amd -- MI355X-native dense GEMM and LU solve behind FLENS's call surface.
+
+The compute lives in ``libflx.so`` (hand-written HIP for gfx950, built in-tree
+by ``flens_amd.build``); this module is a thin ctypes mirror of its C ABI
+(``include/flx.h``) with the reference's names:
+
+* ``blas.mm``       -- flens::blas::mm        (flens/blas/level3/mm.tcc:380-479)
+* ``lapack.trf``    -- flens::lapack::trf     (flens/lapack/ge/trf.tcc:183-261)
+* ``lapack.trs``    -- flens::lapack::trs     (flens/lapack/ge/trs.tcc:151-252)
+* ``lapack.sv``     -- flens::lapack::sv      (flens/lapack/ge/sv.tcc:115-219)
+* ``lapack.laswp``  -- flens::lapack::laswp   (flens/lapack/la/laswp.tcc:126-186)
+
+Two data paths, exactly as in the C ABI:
+
+* host numpy arrays (Fortran order) go through the FLENS plugin seams
+  (``cblas_dgemm``, ``dgetrf_``, ``dgetrs_``, ``dgesv_``, ``dlaswp_``);
+* device arrays (:class:`DeviceMatrix`, column-major views of torch CUDA
+  storage) go through the asynchronous ``flx_*`` entry points.
+
+There is no CPU fallback: importing works anywhere, but every compute call
+raises if ``libflx.so`` is missing or no GPU is present.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+from dataclasses import dataclass
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libflx.so")
+
+# every symbol include/flx.h declares
+EXPORTS = [
+    "flx_create", "flx_destroy", "flx_set_stream", "flx_sync", "flx_status_string",
+    "flx_set_lu_blocking", "flx_dgemm", "flx_dgetrf", "flx_dgetrs", "flx_dgesv",
+    "flx_dlaswp", "flx_fill_uniform", "flx_profile_start", "flx_profile_stop",
+    "cblas_dgemm", "cblas_dtrsm", "dgetrf_", "dgetrs_", "dgesv_", "dlaswp_", "xerbla_",
+]
+
+_dp = C.POINTER(C.c_double)
+_ip = C.POINTER(C.c_int)
+_i, _d, _vp, _u64 = C.c_int, C.c_double, C.c_void_p, C.c_uint64
+
+_lib = None
+
+
+class FlxError(RuntimeError):
+    pass
+
+
+def lib():
+    """Load libflx.so (raises if it was not built)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise FlxError(f"{LIB_PATH} is missing: run `python -m flens_amd.build` first")
+    L = C.CDLL(LIB_PATH)
+    L.flx_create.argtypes = [C.POINTER(_vp)]
+    L.flx_destroy.argtypes = [_vp]
+    L.flx_set_stream.argtypes = [_vp, _vp]
+    L.flx_sync.argtypes = [_vp]
+    L.flx_status_string.argtypes = [_i]
+    L.flx_status_string.restype = C.c_char_p
+    L.flx_set_lu_blocking.argtypes = [_vp, _i, _i]
+    L.flx_dgemm.argtypes = [_vp, _i, _i, _i, _i, _i, _d, _vp, _i, _vp, _i, _d, _vp, _i]
+    L.flx_dgetrf.argtypes = [_vp, _i, _i, _vp, _i, _vp, _vp]
+    L.flx_dgetrs.argtypes = [_vp, _i, _i, _i, _vp, _i, _vp, _vp, _i]
+    L.flx_dgesv.argtypes = [_vp, _i, _i, _vp, _i, _vp, _vp, _i, _ip]
+    L.flx_dlaswp.argtypes = [_vp, _i, _vp, _i, _i, _i, _vp, _i]
+    L.flx_fill_uniform.argtypes = [_vp, _vp, _i, _i, _i, _u64, _u64]
+    L.flx_profile_start.argtypes = [_vp]
+    L.flx_profile_stop.argtypes = [_vp, _dp]
+    L.cblas_dgemm.argtypes = [_i, _i, _i, _i, _i, _i, _d, _dp, _i, _dp, _i, _d, _dp, _i]
+    L.cblas_dgemm.restype = None
+    L.cblas_dtrsm.argtypes = [_i, _i, _i, _i, _i, _i, _i, _d, _dp, _i, _dp, _i]
+    L.cblas_dtrsm.restype = None
+    L.dgetrf_.argtypes = [_ip, _ip, _dp, _ip, _ip, _ip]
+    L.dgetrf_.restype = None
+    L.dgetrs_.argtypes = [C.c_char_p, _ip, _ip, _dp, _ip, _ip, _dp, _ip, _ip]
+    L.dgetrs_.restype = None
+    L.dgesv_.argtypes = [_ip, _ip, _dp, _ip, _ip, _dp, _ip, _ip]
+    L.dgesv_.restype = None
+    L.dlaswp_.argtypes = [_ip, _dp, _ip, _ip, _ip, _ip, _ip]
+    L.dlaswp_.restype = None
+    _lib = L
+    return L
+
+
+def _check(status: int, what: str):
+    if status != 0:
+        msg = lib().flx_status_string(status).decode()
+        raise FlxError(f"{what}: {msg} (status {status})")
+
+
+class Handle:
+    """flx handle bound to a HIP stream (torch's current stream by default)."""
+
+    def __init__(self, stream=None):
+        self._h = _vp()
+        _check(lib().flx_create(C.byref(self._h)), "flx_create")
+        if stream is not None:
+            self.set_stream(stream)
+
+    def set_stream(self, stream):
+        ptr = getattr(stream, "cuda_stream", stream)
+        _check(lib().flx_set_stream(self._h, _vp(ptr)), "flx_set_stream")
+
+    def set_lu_blocking(self, nb: int, ib: int = 32):
+        _check(lib().flx_set_lu_blocking(self._h, nb, ib), "flx_set_lu_blocking")
+
+    def sync(self):
+        _check(lib().flx_sync(self._h), "flx_sync")
+
+    PROF_CLASSES = ["gemm", "panel", "rowperm", "trsm_rm", "transpose", "trs", "other"]
+
+    def profile_start(self):
+        _check(lib().flx_profile_start(self._h), "flx_profile_start")
+
+    def profile_stop(self) -> dict:
+        out = (C.c_double * (4 * len(self.PROF_CLASSES)))()
+        _check(lib().flx_profile_stop(self._h, out), "flx_profile_stop")
+        res = {}
+        for i, name in enumerate(self.PROF_CLASSES):
+            cnt, ms, fl, by = out[4 * i: 4 * i + 4]
+            res[name] = dict(launches=int(cnt), ms=ms, flops=fl, bytes=by)
+        return res
+
+    @property
+    def ptr(self):
+        return self._h
+
+    def close(self):
+        if self._h:
+            lib().flx_destroy(self._h)
+            self._h = _vp()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+@dataclass
+class DeviceMatrix:
+    """Column-major m x n view (leading dimension ld) of device memory.
+
+    ``storage`` keeps the owning torch tensor alive."""
+    ptr: int
+    m: int
+    n: int
+    ld: int
+    storage: object = None
+
+    @staticmethod
+    def empty(m: int, n: int, dtype="float64", device="cuda"):
+        import torch
+        t = torch.empty((max(n, 1), max(m, 1)), dtype=getattr(torch, dtype), device=device)
+        return DeviceMatrix(t.data_ptr(), m, n, max(m, 1), t)
+
+    @staticmethod
+    def from_numpy(a: np.ndarray, device="cuda"):
+        import torch
+        a = np.asfortranarray(a)
+        m, n = a.shape
+        t = torch.from_numpy(np.ascontiguousarray(a.T)).to(device)
+        return DeviceMatrix(t.data_ptr(), m, n, max(m, 1), t)
+
+    def numpy(self) -> np.ndarray:
+        t = self.storage
+        assert t is not None and self.ld == self.m
+        return np.asfortranarray(t.cpu().numpy().T[: self.m, : self.n])
+
+
+def _int_buffer(n: int, device="cuda"):
+    import torch
+    return torch.zeros(max(n, 1), dtype=torch.int32, device=device)
+
+
+# ---------------------------------------------------------------------------
+# blas / lapack mirrors
+# ---------------------------------------------------------------------------
+class blas:
+    NoTrans, Trans = 0, 1
+
+    @staticmethod
+    def mm(transA, transB, alpha, A, B, beta, C, handle: Handle | None = None):
+        """C <- beta*C + alpha*op(A)*op(B).
+
+        numpy (Fortran order) operands go through cblas_dgemm (synchronous,
+        BLAS beta=0 convention); DeviceMatrix operands through flx_dgemm."""
+        ta, tb = int(bool(transA)), int(bool(transB))
+        if isinstance(C, np.ndarray):
+            for x in (A, B, C):
+                assert x.dtype == np.float64 and x.flags.f_contiguous
+            m, n = C.shape
+            k = A.shape[0] if ta else A.shape[1]
+            lib().cblas_dgemm(102, 112 if ta else 111, 112 if tb else 111, m, n, k,
+                              float(alpha), A.ctypes.data_as(_dp), max(1, A.shape[0]),
+                              B.ctypes.data_as(_dp), max(1, B.shape[0]), float(beta),
+                              C.ctypes.data_as(_dp), max(1, m))
+            return C
+        h = handle or default_handle()
+        k = A.m if ta else A.n
+        _check(lib().flx_dgemm(h.ptr, ta, tb, C.m, C.n, k, float(alpha), _vp(A.ptr), A.ld,
+                               _vp(B.ptr), B.ld, float(beta), _vp(C.ptr), C.ld), "flx_dgemm")
+        return C
+
+
+class lapack:
+    @staticmethod
+    def trf(A, piv=None, handle: Handle | None = None):
+        """LU with partial pivoting in place.  Returns (piv, info): numpy
+        int32 1-based pivots for host A; a device int32 tensor for device A
+        (info is then a device tensor of one int)."""
+        if isinstance(A, np.ndarray):
+            assert A.dtype == np.float64 and A.flags.f_contiguous
+            m, n = A.shape
+            piv = np.zeros(max(1, min(m, n)), dtype=np.int32)
+            info = C.c_int(0)
+            lib().dgetrf_(C.byref(C.c_int(m)), C.byref(C.c_int(n)), A.ctypes.data_as(_dp),
+                          C.byref(C.c_int(max(1, m))), piv.ctypes.data_as(_ip), C.byref(info))
+            return piv[: min(m, n)], info.value
+        h = handle or default_handle()
+        piv = _int_buffer(min(A.m, A.n)) if piv is None else piv
+        info = _int_buffer(1)
+        _check(lib().flx_dgetrf(h.ptr, A.m, A.n, _vp(A.ptr), A.ld, _vp(piv.data_ptr()),
+                                _vp(info.data_ptr())), "flx_dgetrf")
+        return piv, info
+
+    @staticmethod
+    def trs(trans, A, piv, B, handle: Handle | None = None):
+        if isinstance(B, np.ndarray):
+            n = A.shape[0]
+            nrhs = B.shape[1]
+            piv = np.ascontiguousarray(piv, dtype=np.int32)
+            info = C.c_int(0)
+            lib().dgetrs_(b"T" if trans else b"N", C.byref(C.c_int(n)), C.byref(C.c_int(nrhs)),
+                          A.ctypes.data_as(_dp), C.byref(C.c_int(max(1, n))),
+                          piv.ctypes.data_as(_ip), B.ctypes.data_as(_dp),
+                          C.byref(C.c_int(max(1, n))), C.byref(info))
+            if info.value != 0:
+                raise FlxError(f"dgetrs_ info={info.value}")
+            return B
+        h = handle or default_handle()
+        _check(lib().flx_dgetrs(h.ptr, int(bool(trans)), A.n, B.n, _vp(A.ptr), A.ld,
+                                _vp(piv.data_ptr()), _vp(B.ptr), B.ld), "flx_dgetrs")
+        return B
+
+    @staticmethod
+    def sv(A, B, handle: Handle | None = None):
+        """Solve A X = B; returns (piv, info).  B is overwritten with X."""
+        if isinstance(A, np.ndarray):
+            n = A.shape[0]
+            piv = np.zeros(max(1, n), dtype=np.int32)
+            info = C.c_int(0)
+            lib().dgesv_(C.byref(C.c_int(n)), C.byref(C.c_int(B.shape[1])),
+                         A.ctypes.data_as(_dp), C.byref(C.c_int(max(1, n))),
+                         piv.ctypes.data_as(_ip), B.ctypes.data_as(_dp),
+                         C.byref(C.c_int(max(1, n))), C.byref(info))
+            return piv[:n], info.value
+        h = handle or default_handle()
+        piv = _int_buffer(A.n)
+        info = C.c_int(0)
+        _check(lib().flx_dgesv(h.ptr, A.n, B.n, _vp(A.ptr), A.ld, _vp(piv.data_ptr()),
+                               _vp(B.ptr), B.ld, C.byref(info)), "flx_dgesv")
+        return piv, info.value
+
+    @staticmethod
+    def laswp(A: np.ndarray, k1: int, k2: int, piv, incx: int = 1):
+        piv = np.ascontiguousarray(piv, dtype=np.int32)
+        m, n = A.shape
+        lib().dlaswp_(C.byref(C.c_int(n)), A.ctypes.data_as(_dp), C.byref(C.c_int(max(1, m))),
+                      C.byref(C.c_int(k1)), C.byref(C.c_int(k2)), piv.ctypes.data_as(_ip),
+                      C.byref(C.c_int(incx)))
+        return A
+
+
+_default = None
+
+
+def default_handle() -> Handle:
+    """Process-wide handle on torch's current stream."""
+    global _default
+    if _default is None:
+        import torch
+        if not torch.cuda.is_available():
+            raise FlxError("no GPU: flens_amd has no CPU path")
+        _default = Handle(torch.cuda.current_stream())
+    return _default

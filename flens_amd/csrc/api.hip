@@ -1,0 +1,626 @@
+// api.hip -- the C ABI of libflx (include/flx.h): handles, device-pointer
+// entry points and the FLENS plugin seams (CBLAS + Fortran LAPACK symbols).
+#include <climits>
+#include <cstdio>
+#include <cstring>
+#include <mutex>
+#include <new>
+#include "../../include/flx.h"
+#include "flx_kernels.h"
+
+using namespace flx;
+
+// ---------------------------------------------------------------------------
+// handle
+// ---------------------------------------------------------------------------
+struct DevBuf {
+    void *p = nullptr;
+    size_t cap = 0;
+};
+
+struct flx_context {
+    hipStream_t stream = nullptr;
+    int device = 0;
+    LuWork wk;
+    LuOptions opt;
+    DevBuf W, A, B, C, X, piv0, ipiv, info, perm;
+    int last_error = 0;
+    Profiler *prof = nullptr;   // non-null while profiling
+};
+
+// binds the calling handle's profiler for the duration of one entry point
+struct ProfBind {
+    ProfBind(flx_context *h) { tl_prof = h ? h->prof : nullptr; if (h) h->wk.prof = h->prof; }
+    ~ProfBind() { tl_prof = nullptr; }
+};
+
+static int hip_status(hipError_t e)
+{
+    if (e == hipSuccess) return FLX_SUCCESS;
+    if (e == hipErrorOutOfMemory) return FLX_ALLOC_FAILED;
+    if (e == hipErrorInvalidValue) return FLX_UNSUPPORTED;
+    return FLX_HIP_ERROR;
+}
+
+#define FLX_TRY(x)                                                            \
+    do {                                                                      \
+        hipError_t e_ = (x);                                                  \
+        if (e_ != hipSuccess) return hip_status(e_);                          \
+    } while (0)
+
+// grow-only device buffer; waits for queued work before releasing memory
+static hipError_t ensure(flx_context *h, DevBuf &b, size_t bytes)
+{
+    if (b.cap >= bytes) return hipSuccess;
+    if (b.p) {
+        hipError_t e = hipStreamSynchronize(h->stream);
+        if (e != hipSuccess) return e;
+        hipFree(b.p);
+        b.p = nullptr;
+        b.cap = 0;
+    }
+    size_t sz = bytes < 256 ? 256 : bytes;
+    hipError_t e = hipMalloc(&b.p, sz);
+    if (e != hipSuccess) { b.p = nullptr; return e; }
+    b.cap = sz;
+    return hipSuccess;
+}
+
+static hipError_t init_work(flx_context *h)
+{
+    if (h->wk.exch) return hipSuccess;
+    const size_t words = lu_exch_words();
+    hipError_t e = hipMalloc(&h->wk.exch, words * sizeof(uint64_t));
+    if (e != hipSuccess) return e;
+    h->wk.exch_words = words;
+    e = hipMemsetAsync(h->wk.exch, 0, words * sizeof(uint64_t), h->stream);
+    if (e != hipSuccess) return e;
+    e = hipMalloc(&h->wk.list, sizeof(int) * (1 + 2 * 2 * 64 + 16));
+    if (e != hipSuccess) return e;
+    e = hipMalloc(&h->wk.flags, sizeof(int) * 4);
+    if (e != hipSuccess) return e;
+    e = hipMalloc(&h->wk.moved, sizeof(unsigned long long));
+    if (e != hipSuccess) return e;
+    e = hipMemsetAsync(h->wk.moved, 0, sizeof(unsigned long long), h->stream);
+    if (e != hipSuccess) return e;
+    return hipMemsetAsync(h->wk.flags, 0, sizeof(int) * 4, h->stream);
+}
+
+extern "C" int flx_create(flx_handle *out)
+{
+    if (!out) return FLX_INVALID_ARG;
+    flx_context *h = new (std::nothrow) flx_context();
+    if (!h) return FLX_ALLOC_FAILED;
+    if (hipGetDevice(&h->device) != hipSuccess) { delete h; return FLX_HIP_ERROR; }
+    *out = h;
+    return FLX_SUCCESS;
+}
+
+static void free_buf(DevBuf &b)
+{
+    if (b.p) hipFree(b.p);
+    b.p = nullptr;
+    b.cap = 0;
+}
+
+extern "C" int flx_destroy(flx_handle h)
+{
+    if (!h) return FLX_INVALID_ARG;
+    hipStreamSynchronize(h->stream);
+    free_buf(h->W); free_buf(h->A); free_buf(h->B); free_buf(h->C); free_buf(h->X);
+    free_buf(h->piv0); free_buf(h->ipiv); free_buf(h->info); free_buf(h->perm);
+    if (h->wk.exch) hipFree(h->wk.exch);
+    if (h->wk.list) hipFree(h->wk.list);
+    if (h->wk.flags) hipFree(h->wk.flags);
+    if (h->wk.moved) hipFree(h->wk.moved);
+    prof_delete(h->prof);
+    delete h;
+    return FLX_SUCCESS;
+}
+
+extern "C" int flx_set_stream(flx_handle h, void *stream)
+{
+    if (!h) return FLX_INVALID_ARG;
+    h->stream = (hipStream_t)stream;
+    return FLX_SUCCESS;
+}
+
+extern "C" int flx_set_lu_blocking(flx_handle h, int nb, int ib)
+{
+    if (!h || nb < 1 || ib < 1 || ib > 64 || nb > 256) return FLX_INVALID_ARG;
+    h->opt.nb = nb;
+    h->opt.ib = ib;
+    return FLX_SUCCESS;
+}
+
+extern "C" int flx_sync(flx_handle h)
+{
+    if (!h) return FLX_INVALID_ARG;
+    FLX_TRY(hipStreamSynchronize(h->stream));
+    if (h->wk.flags) {
+        int f = 0;
+        FLX_TRY(hipMemcpy(&f, h->wk.flags, sizeof(int), hipMemcpyDeviceToHost));
+        if (f) {
+            int z = 0;
+            hipMemcpy(h->wk.flags, &z, sizeof(int), hipMemcpyHostToDevice);
+            return FLX_SYNC_TIMEOUT;
+        }
+    }
+    return FLX_SUCCESS;
+}
+
+extern "C" int flx_profile_start(flx_handle h)
+{
+    if (!h) return FLX_INVALID_ARG;
+    FLX_TRY(init_work(h));
+    if (!h->prof) h->prof = prof_new();
+    FLX_TRY(hipStreamSynchronize(h->stream));
+    prof_reset(h->prof);
+    FLX_TRY(hipMemsetAsync(h->wk.moved, 0, sizeof(unsigned long long), h->stream));
+    return FLX_SUCCESS;
+}
+
+extern "C" int flx_profile_stop(flx_handle h, double *out)
+{
+    if (!h || !h->prof || !out) return FLX_INVALID_ARG;
+    FLX_TRY(hipStreamSynchronize(h->stream));
+    prof_collect(h->prof, out);
+    unsigned long long moved = 0;
+    FLX_TRY(hipMemcpy(&moved, h->wk.moved, sizeof(moved), hipMemcpyDeviceToHost));
+    out[PROF_ROWPERM * 4 + 3] = 16.0 * (double)moved;   // each element read + written
+    h->wk.prof = nullptr;
+    Profiler *p = h->prof;
+    h->prof = nullptr;
+    prof_delete(p);
+    return FLX_SUCCESS;
+}
+
+extern "C" const char *flx_status_string(int s)
+{
+    switch (s) {
+    case FLX_SUCCESS: return "success";
+    case FLX_INVALID_ARG: return "invalid argument";
+    case FLX_HIP_ERROR: return "HIP runtime error";
+    case FLX_ALLOC_FAILED: return "device allocation failed";
+    case FLX_SYNC_TIMEOUT: return "device-side wait timed out";
+    case FLX_UNSUPPORTED: return "unsupported shape or option";
+    default: return "unknown status";
+    }
+}
+
+// ---------------------------------------------------------------------------
+// device-pointer entries
+// ---------------------------------------------------------------------------
+extern "C" int flx_dgemm(flx_handle h, int transa, int transb, int m, int n, int k,
+                         double alpha, const double *A, int lda, const double *B, int ldb,
+                         double beta, double *C, int ldc)
+{
+    ProfBind pb_(h);
+    if (!h || m < 0 || n < 0 || k < 0) return FLX_INVALID_ARG;
+    const int arows = transa ? k : m, brows = transb ? n : k;
+    if (lda < (arows > 1 ? arows : 1) || ldb < (brows > 1 ? brows : 1) ||
+        ldc < (m > 1 ? m : 1))
+        return FLX_INVALID_ARG;
+    if (m == 0 || n == 0) return FLX_SUCCESS;
+    ProfScope ps(h->prof, h->stream, PROF_GEMM, 2.0 * m * n * k,
+                 8.0 * ((double)m * k + (double)k * n + 2.0 * m * n));
+    FLX_TRY(gemm_f64(h->stream, transa, transb, m, n, k, alpha, A, lda, B, ldb, beta, C, ldc,
+                     0));
+    return FLX_SUCCESS;
+}
+
+static long round_ld(long n) { return ((n + 31) / 32) * 32; }
+
+// LU of device column-major A via the row-major working copy
+static int dgetrf_dev(flx_context *h, int m, int n, double *A, int lda, int *ipiv, int *info,
+                      bool keep_perm)
+{
+    const int mn = m < n ? m : n;
+    FLX_TRY(init_work(h));
+    const long ldw = round_ld(n);
+    FLX_TRY(ensure(h, h->W, sizeof(double) * (size_t)ldw * (size_t)(m > 0 ? m : 1)));
+    FLX_TRY(ensure(h, h->piv0, sizeof(int) * (size_t)(mn > 0 ? mn : 1)));
+    if (keep_perm) {
+        FLX_TRY(ensure(h, h->perm, sizeof(int) * (size_t)(m > 0 ? m : 1)));
+        h->wk.perm = (int *)h->perm.p;
+    } else {
+        h->wk.perm = nullptr;
+    }
+    double *W = (double *)h->W.p;
+    FLX_TRY(fill_int(h->stream, info, 1, INT_MAX));
+    FLX_TRY(transpose_f64(h->stream, A, lda, W, ldw, m, n));
+    FLX_TRY(getrf_rm_f64(h->stream, h->wk, W, ldw, m, n, (int *)h->piv0.p, info, h->opt));
+    FLX_TRY(transpose_f64(h->stream, W, ldw, A, lda, n, m));
+    FLX_TRY(piv0_to_ipiv(h->stream, (int *)h->piv0.p, ipiv, mn, info));
+    return FLX_SUCCESS;
+}
+
+extern "C" int flx_dgetrf(flx_handle h, int m, int n, double *A, int lda, int *ipiv, int *info)
+{
+    ProfBind pb_(h);
+    if (!h || m < 0 || n < 0 || lda < (m > 1 ? m : 1)) return FLX_INVALID_ARG;
+    if (m == 0 || n == 0) {
+        FLX_TRY(fill_int(h->stream, info, 1, 0));
+        return FLX_SUCCESS;
+    }
+    if (m > 16384) return FLX_UNSUPPORTED;
+    return dgetrf_dev(h, m, n, A, lda, ipiv, info, false);
+}
+
+// NoTrans: X = U^{-1} L^{-1} P B ; Trans: X = P^T L^{-T} U^{-T} B
+static int dgetrs_dev(flx_context *h, int trans, int n, int nrhs, const double *A, int lda,
+                      const int *ipiv, const int *perm_fwd, double *B, int ldb)
+{
+    const size_t bytes = sizeof(double) * (size_t)n * (size_t)nrhs;
+    FLX_TRY(ensure(h, h->B, bytes));
+    FLX_TRY(ensure(h, h->X, bytes));
+    double *T1 = (double *)h->B.p, *T2 = (double *)h->X.p;
+    if (!trans) {
+        const int *perm = perm_fwd;
+        if (!perm) {
+            FLX_TRY(ensure(h, h->perm, sizeof(int) * (size_t)n));
+            FLX_TRY(ipiv_to_perm(h->stream, ipiv, (int *)h->perm.p, n, 0));
+            perm = (const int *)h->perm.p;
+        }
+        FLX_TRY(gather_rows_cm(h->stream, B, ldb, T1, n, perm, n, nrhs));
+        FLX_TRY(trsm_cm_f64(h->stream, 0, 0, 1, n, nrhs, A, lda, T1, n, T2, n));
+        FLX_TRY(trsm_cm_f64(h->stream, 1, 0, 0, n, nrhs, A, lda, T2, n, B, ldb));
+    } else {
+        FLX_TRY(copy_cm(h->stream, B, ldb, T1, n, n, nrhs));
+        FLX_TRY(trsm_cm_f64(h->stream, 1, 1, 0, n, nrhs, A, lda, T1, n, T2, n));
+        FLX_TRY(trsm_cm_f64(h->stream, 0, 1, 1, n, nrhs, A, lda, T2, n, T1, n));
+        // apply the interchanges last to first: X = P^T Y
+        FLX_TRY(ensure(h, h->perm, sizeof(int) * (size_t)n));
+        FLX_TRY(ipiv_to_perm(h->stream, ipiv, (int *)h->perm.p, n, 1));
+        FLX_TRY(gather_rows_cm(h->stream, T1, n, B, ldb, (const int *)h->perm.p, n, nrhs));
+    }
+    return FLX_SUCCESS;
+}
+
+extern "C" int flx_dgetrs(flx_handle h, int trans, int n, int nrhs, const double *A, int lda,
+                          const int *ipiv, double *B, int ldb)
+{
+    ProfBind pb_(h);
+    if (!h || n < 0 || nrhs < 0 || lda < (n > 1 ? n : 1) || ldb < (n > 1 ? n : 1))
+        return FLX_INVALID_ARG;
+    if (n == 0 || nrhs == 0) return FLX_SUCCESS;
+    return dgetrs_dev(h, trans ? 1 : 0, n, nrhs, A, lda, ipiv, nullptr, B, ldb);
+}
+
+extern "C" int flx_dgesv(flx_handle h, int n, int nrhs, double *A, int lda, int *ipiv,
+                         double *B, int ldb, int *info_host)
+{
+    ProfBind pb_(h);
+    if (!h || n < 0 || nrhs < 0 || lda < (n > 1 ? n : 1) || ldb < (n > 1 ? n : 1))
+        return FLX_INVALID_ARG;
+    if (n == 0) { if (info_host) *info_host = 0; return FLX_SUCCESS; }
+    if (n > 16384) return FLX_UNSUPPORTED;
+    FLX_TRY(ensure(h, h->info, sizeof(int)));
+    int *dinfo = (int *)h->info.p;
+    int st = dgetrf_dev(h, n, n, A, lda, ipiv, dinfo, true);
+    if (st) return st;
+    int info = 0;
+    FLX_TRY(hipMemcpyAsync(&info, dinfo, sizeof(int), hipMemcpyDeviceToHost, h->stream));
+    FLX_TRY(hipStreamSynchronize(h->stream));
+    if (info_host) *info_host = info;
+    if (info == 0 && nrhs > 0)
+        return dgetrs_dev(h, 0, n, nrhs, A, lda, ipiv, (const int *)h->perm.p, B, ldb);
+    return FLX_SUCCESS;
+}
+
+extern "C" int flx_dlaswp(flx_handle h, int n, double *A, int lda, int k1, int k2,
+                          const int *ipiv, int incx)
+{
+    if (!h || n < 0 || k1 < 1 || k2 < k1 - 1 || (incx != 1 && incx != -1))
+        return FLX_INVALID_ARG;
+    FLX_TRY(laswp_cm_f64(h->stream, A, lda, n, k1, k2, ipiv, incx));
+    return FLX_SUCCESS;
+}
+
+extern "C" int flx_fill_uniform(flx_handle h, double *A, int m, int n, int lda, uint64_t seed,
+                                uint64_t offset)
+{
+    if (!h || m < 0 || n < 0 || lda < (m > 1 ? m : 1)) return FLX_INVALID_ARG;
+    FLX_TRY(fill_uniform_f64(h->stream, A, m, n, lda, seed, offset));
+    return FLX_SUCCESS;
+}
+
+// ---------------------------------------------------------------------------
+// FLENS plugin seams (host pointers, synchronous)
+// ---------------------------------------------------------------------------
+extern "C" __attribute__((weak)) void xerbla_(const char *srname, const int *info, int len)
+{
+    fprintf(stderr, " ** On entry to %.*s parameter number %d had an illegal value\n",
+            len, srname, *info);
+}
+
+static flx_context *default_handle()
+{
+    // one handle per host thread, created on first use
+    static thread_local flx_context *h = nullptr;
+    if (!h) {
+        if (flx_create(&h) != FLX_SUCCESS) h = nullptr;
+    }
+    return h;
+}
+
+static void report(const char *who, int status)
+{
+    if (status != FLX_SUCCESS)
+        fprintf(stderr, "libflx: %s failed: %s\n", who, flx_status_string(status));
+}
+
+// host (ld) <-> device (compact, ld = rows) copies
+static hipError_t h2d(void *dst, const double *src, int rows, int cols, int ld)
+{
+    if (rows <= 0 || cols <= 0) return hipSuccess;
+    return hipMemcpy2D(dst, sizeof(double) * rows, src, sizeof(double) * ld,
+                       sizeof(double) * rows, cols, hipMemcpyHostToDevice);
+}
+static hipError_t d2h(double *dst, const void *src, int rows, int cols, int ld)
+{
+    if (rows <= 0 || cols <= 0) return hipSuccess;
+    return hipMemcpy2D(dst, sizeof(double) * ld, src, sizeof(double) * rows,
+                       sizeof(double) * rows, cols, hipMemcpyDeviceToHost);
+}
+
+static int host_gemm(int ta, int tb, int M, int N, int K, double alpha, const double *A, int lda,
+                     const double *B, int ldb, double beta, double *C, int ldc)
+{
+    flx_context *h = default_handle();
+    if (!h) return FLX_HIP_ERROR;
+    const int ar = ta ? K : M, ac = ta ? M : K;
+    const int br = tb ? N : K, bc = tb ? K : N;
+    FLX_TRY(ensure(h, h->A, sizeof(double) * (size_t)(ar > 0 ? ar : 1) * (ac > 0 ? ac : 1)));
+    FLX_TRY(ensure(h, h->B, sizeof(double) * (size_t)(br > 0 ? br : 1) * (bc > 0 ? bc : 1)));
+    FLX_TRY(ensure(h, h->C, sizeof(double) * (size_t)M * N));
+    const bool need_ab = alpha != 0.0 && K > 0;
+    if (need_ab) {
+        FLX_TRY(h2d(h->A.p, A, ar, ac, lda));
+        FLX_TRY(h2d(h->B.p, B, br, bc, ldb));
+    }
+    if (beta != 0.0) FLX_TRY(h2d(h->C.p, C, M, N, ldc));
+    FLX_TRY(gemm_f64(h->stream, ta, tb, M, N, need_ab ? K : 0, alpha, (const double *)h->A.p,
+                     ar > 1 ? ar : 1, (const double *)h->B.p, br > 1 ? br : 1, beta,
+                     (double *)h->C.p, M > 1 ? M : 1, 1));
+    FLX_TRY(hipStreamSynchronize(h->stream));
+    FLX_TRY(d2h(C, h->C.p, M, N, ldc));
+    return FLX_SUCCESS;
+}
+
+extern "C" void cblas_dgemm(int order, int transA, int transB, int M, int N, int K,
+                            double alpha, const double *A, int lda, const double *B, int ldb,
+                            double beta, double *C, int ldc)
+{
+    // CBLAS enums (cxxblas/drivers/cblas.h:22-27): 101 RowMajor, 102 ColMajor,
+    // 111 NoTrans, 112 Trans, 113 ConjTrans (= Trans for real data).
+    if (order != 101 && order != 102) { fprintf(stderr, "cblas_dgemm: parameter 1 illegal\n"); return; }
+    int ta = (transA == 112 || transA == 113) ? 1 : (transA == 111 ? 0 : -1);
+    int tb = (transB == 112 || transB == 113) ? 1 : (transB == 111 ? 0 : -1);
+    if (ta < 0) { fprintf(stderr, "cblas_dgemm: parameter 2 illegal\n"); return; }
+    if (tb < 0) { fprintf(stderr, "cblas_dgemm: parameter 3 illegal\n"); return; }
+    if (order == 101) {
+        // row-major C = op(A) op(B)  <=>  column-major C^T = op(B)^T op(A)^T
+        std::swap(ta, tb);
+        std::swap(M, N);
+        std::swap(A, B);
+        std::swap(lda, ldb);
+    }
+    const int arows = ta ? K : M, brows = tb ? N : K;
+    int bad = 0;
+    if (M < 0) bad = 4;
+    else if (N < 0) bad = 5;
+    else if (K < 0) bad = 6;
+    else if (lda < (arows > 1 ? arows : 1)) bad = 9;
+    else if (ldb < (brows > 1 ? brows : 1)) bad = 11;
+    else if (ldc < (M > 1 ? M : 1)) bad = 14;
+    if (bad) { fprintf(stderr, "cblas_dgemm: parameter %d illegal\n", bad); return; }
+    if (M == 0 || N == 0) return;
+    report("cblas_dgemm", host_gemm(ta, tb, M, N, K, alpha, A, lda, B, ldb, beta, C, ldc));
+}
+
+static int host_trsm(int right, int upper, int trans, int unit, int M, int N, double alpha,
+                     const double *A, int lda, double *B, int ldb)
+{
+    flx_context *h = default_handle();
+    if (!h) return FLX_HIP_ERROR;
+    const int na = right ? N : M;
+    FLX_TRY(ensure(h, h->A, sizeof(double) * (size_t)na * na));
+    FLX_TRY(ensure(h, h->C, sizeof(double) * (size_t)M * N));
+    FLX_TRY(ensure(h, h->W, sizeof(double) * (size_t)M * N));
+    FLX_TRY(h2d(h->A.p, A, na, na, lda));
+    FLX_TRY(h2d(h->C.p, B, M, N, ldb));
+    double *dB = (double *)h->C.p, *dT = (double *)h->W.p;
+    if (alpha != 1.0)
+        FLX_TRY(gemm_f64(h->stream, 0, 0, M, N, 0, 0.0, nullptr, 1, nullptr, 1, alpha, dB, M, 0));
+    if (!right) {
+        // X = op(A)^{-1} B : solve consumes dT copy, writes dB
+        FLX_TRY(copy_cm(h->stream, dB, M, dT, M, M, N));
+        FLX_TRY(trsm_cm_f64(h->stream, upper, trans, unit, M, N, (const double *)h->A.p, na,
+                            dT, M, dB, M));
+    } else {
+        // X op(A) = B  <=>  op(A)^T X^T = B^T
+        FLX_TRY(transpose_f64(h->stream, dB, M, dT, N, M, N));       // dT = B^T (N x M)
+        FLX_TRY(trsm_cm_f64(h->stream, upper, !trans, unit, N, M, (const double *)h->A.p, na,
+                            dT, N, dB, N));                           // dB = X^T (N x M)
+        FLX_TRY(transpose_f64(h->stream, dB, N, dT, M, N, M));       // dT = X (M x N)
+        FLX_TRY(copy_cm(h->stream, dT, M, dB, M, M, N));
+    }
+    FLX_TRY(hipStreamSynchronize(h->stream));
+    FLX_TRY(d2h(B, h->C.p, M, N, ldb));
+    return FLX_SUCCESS;
+}
+
+extern "C" void cblas_dtrsm(int order, int side, int uplo, int transA, int diag, int M, int N,
+                            double alpha, const double *A, int lda, double *B, int ldb)
+{
+    if (order != 101 && order != 102) { fprintf(stderr, "cblas_dtrsm: parameter 1 illegal\n"); return; }
+    int right = side == 142 ? 1 : (side == 141 ? 0 : -1);
+    int upper = uplo == 121 ? 1 : (uplo == 122 ? 0 : -1);
+    int trans = (transA == 112 || transA == 113) ? 1 : (transA == 111 ? 0 : -1);
+    int unit = diag == 132 ? 1 : (diag == 131 ? 0 : -1);
+    if (right < 0 || upper < 0 || trans < 0 || unit < 0) {
+        fprintf(stderr, "cblas_dtrsm: illegal enum parameter\n");
+        return;
+    }
+    if (order == 101) {
+        // row-major B (M x N) is column-major B^T (N x M): swap side and uplo
+        right = !right;
+        upper = !upper;
+        std::swap(M, N);
+    }
+    const int na = right ? N : M;
+    if (M < 0 || N < 0 || lda < (na > 1 ? na : 1) || ldb < (M > 1 ? M : 1)) {
+        fprintf(stderr, "cblas_dtrsm: illegal size parameter\n");
+        return;
+    }
+    if (M == 0 || N == 0) return;
+    report("cblas_dtrsm", host_trsm(right, upper, trans, unit, M, N, alpha, A, lda, B, ldb));
+}
+
+extern "C" void dgetrf_(const int *M, const int *N, double *A, const int *LDA, int *IPIV,
+                        int *INFO)
+{
+    // argument checks of flens/lapack/interface/src/getrf.cc:22-35
+    *INFO = 0;
+    if (*M < 0) *INFO = -1;
+    else if (*N < 0) *INFO = -2;
+    else if (*LDA < (*M > 1 ? *M : 1)) *INFO = -4;
+    if (*INFO != 0) {
+        int i = -*INFO;
+        xerbla_("DGETRF", &i, 6);
+        return;
+    }
+    const int m = *M, n = *N, mn = m < n ? m : n;
+    if (m == 0 || n == 0) return;
+    flx_context *h = default_handle();
+    int st = h ? FLX_SUCCESS : FLX_HIP_ERROR;
+    if (h && m > 16384) st = FLX_UNSUPPORTED;
+    if (st == FLX_SUCCESS) {
+        auto run = [&]() -> int {
+            FLX_TRY(ensure(h, h->A, sizeof(double) * (size_t)m * n));
+            FLX_TRY(ensure(h, h->ipiv, sizeof(int) * (size_t)mn));
+            FLX_TRY(ensure(h, h->info, sizeof(int)));
+            FLX_TRY(h2d(h->A.p, A, m, n, *LDA));
+            int s = dgetrf_dev(h, m, n, (double *)h->A.p, m, (int *)h->ipiv.p, (int *)h->info.p,
+                               false);
+            if (s) return s;
+            s = flx_sync(h);
+            if (s) return s;
+            FLX_TRY(d2h(A, h->A.p, m, n, *LDA));
+            FLX_TRY(hipMemcpy(IPIV, h->ipiv.p, sizeof(int) * mn, hipMemcpyDeviceToHost));
+            FLX_TRY(hipMemcpy(INFO, h->info.p, sizeof(int), hipMemcpyDeviceToHost));
+            return FLX_SUCCESS;
+        };
+        st = run();
+    }
+    report("dgetrf_", st);
+}
+
+extern "C" void dgetrs_(const char *TRANS, const int *N, const int *NRHS, const double *A,
+                        const int *LDA, const int *IPIV, double *B, const int *LDB, int *INFO)
+{
+    const char t = TRANS ? (char)(TRANS[0] & ~0x20) : 'N';
+    *INFO = 0;
+    if (t != 'N' && t != 'T' && t != 'C') *INFO = -1;
+    else if (*N < 0) *INFO = -2;
+    else if (*NRHS < 0) *INFO = -3;
+    else if (*LDA < (*N > 1 ? *N : 1)) *INFO = -5;
+    else if (*LDB < (*N > 1 ? *N : 1)) *INFO = -8;
+    if (*INFO != 0) {
+        int i = -*INFO;
+        xerbla_("DGETRS", &i, 6);
+        return;
+    }
+    const int n = *N, nrhs = *NRHS;
+    if (n == 0 || nrhs == 0) return;
+    flx_context *h = default_handle();
+    int st = h ? FLX_SUCCESS : FLX_HIP_ERROR;
+    if (st == FLX_SUCCESS) {
+        auto run = [&]() -> int {
+            FLX_TRY(ensure(h, h->A, sizeof(double) * (size_t)n * n));
+            FLX_TRY(ensure(h, h->C, sizeof(double) * (size_t)n * nrhs));
+            FLX_TRY(ensure(h, h->ipiv, sizeof(int) * (size_t)n));
+            FLX_TRY(h2d(h->A.p, A, n, n, *LDA));
+            FLX_TRY(h2d(h->C.p, B, n, nrhs, *LDB));
+            FLX_TRY(hipMemcpy(h->ipiv.p, IPIV, sizeof(int) * n, hipMemcpyHostToDevice));
+            int s = dgetrs_dev(h, t != 'N', n, nrhs, (const double *)h->A.p, n,
+                               (const int *)h->ipiv.p, nullptr, (double *)h->C.p, n);
+            if (s) return s;
+            FLX_TRY(hipStreamSynchronize(h->stream));
+            FLX_TRY(d2h(B, h->C.p, n, nrhs, *LDB));
+            return FLX_SUCCESS;
+        };
+        st = run();
+    }
+    report("dgetrs_", st);
+}
+
+extern "C" void dgesv_(const int *N, const int *NRHS, double *A, const int *LDA, int *IPIV,
+                       double *B, const int *LDB, int *INFO)
+{
+    *INFO = 0;
+    if (*N < 0) *INFO = -1;
+    else if (*NRHS < 0) *INFO = -2;
+    else if (*LDA < (*N > 1 ? *N : 1)) *INFO = -4;
+    else if (*LDB < (*N > 1 ? *N : 1)) *INFO = -7;
+    if (*INFO != 0) {
+        int i = -*INFO;
+        xerbla_("DGESV ", &i, 6);
+        return;
+    }
+    const int n = *N, nrhs = *NRHS;
+    if (n == 0) return;
+    flx_context *h = default_handle();
+    int st = h ? FLX_SUCCESS : FLX_HIP_ERROR;
+    if (h && n > 16384) st = FLX_UNSUPPORTED;
+    if (st == FLX_SUCCESS) {
+        auto run = [&]() -> int {
+            FLX_TRY(ensure(h, h->A, sizeof(double) * (size_t)n * n));
+            FLX_TRY(ensure(h, h->C, sizeof(double) * (size_t)n * (nrhs > 0 ? nrhs : 1)));
+            FLX_TRY(ensure(h, h->ipiv, sizeof(int) * (size_t)n));
+            FLX_TRY(h2d(h->A.p, A, n, n, *LDA));
+            FLX_TRY(h2d(h->C.p, B, n, nrhs, *LDB));
+            int info = 0;
+            int s = flx_dgesv(h, n, nrhs, (double *)h->A.p, n, (int *)h->ipiv.p,
+                              (double *)h->C.p, n, &info);
+            if (s) return s;
+            s = flx_sync(h);
+            if (s) return s;
+            *INFO = info;
+            FLX_TRY(d2h(A, h->A.p, n, n, *LDA));
+            FLX_TRY(hipMemcpy(IPIV, h->ipiv.p, sizeof(int) * n, hipMemcpyDeviceToHost));
+            if (info == 0) FLX_TRY(d2h(B, h->C.p, n, nrhs, *LDB));
+            return FLX_SUCCESS;
+        };
+        st = run();
+    }
+    report("dgesv_", st);
+}
+
+extern "C" void dlaswp_(const int *N, double *A, const int *LDA, const int *K1, const int *K2,
+                        const int *IPIV, const int *INCX)
+{
+    const int n = *N, k1 = *K1, k2 = *K2, incx = *INCX;
+    if (n <= 0 || incx == 0 || k2 < k1) return;
+    // rows touched: 1..max(k2, max ipiv)
+    int rows = k2;
+    for (int i = k1 - 1; i < k2; ++i) rows = IPIV[i] > rows ? IPIV[i] : rows;
+    flx_context *h = default_handle();
+    int st = h ? FLX_SUCCESS : FLX_HIP_ERROR;
+    if (st == FLX_SUCCESS) {
+        auto run = [&]() -> int {
+            FLX_TRY(ensure(h, h->A, sizeof(double) * (size_t)rows * n));
+            FLX_TRY(ensure(h, h->ipiv, sizeof(int) * (size_t)k2));
+            FLX_TRY(h2d(h->A.p, A, rows, n, *LDA));
+            FLX_TRY(hipMemcpy(h->ipiv.p, IPIV, sizeof(int) * k2, hipMemcpyHostToDevice));
+            FLX_TRY(laswp_cm_f64(h->stream, (double *)h->A.p, rows, n, k1, k2,
+                                 (const int *)h->ipiv.p, incx > 0 ? 1 : -1));
+            FLX_TRY(hipStreamSynchronize(h->stream));
+            FLX_TRY(d2h(A, h->A.p, rows, n, *LDA));
+            return FLX_SUCCESS;
+        };
+        st = run();
+    }
+    report("dlaswp_", st);
+}

@@ -1,0 +1,68 @@
+// flx_common.h -- shared definitions for the flx HIP kernels (gfx950 only).
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#define FLX_DEV static __device__ __forceinline__
+
+typedef double double4_t __attribute__((ext_vector_type(4)));
+typedef float float4_t __attribute__((ext_vector_type(4)));
+
+// Status codes returned by every flx_* entry point.
+enum flx_status_internal {
+    FLX_OK = 0,
+    FLX_ERR_ARG = 1,        // invalid argument (negative size, bad ld, ...)
+    FLX_ERR_HIP = 2,        // a HIP runtime call failed
+    FLX_ERR_ALLOC = 3,      // device allocation failed
+    FLX_ERR_SYNC = 4,       // a bounded device-side wait gave up
+    FLX_ERR_UNSUPPORTED = 5 // shape or option outside what the kernels cover
+};
+
+// Agent-scope relaxed atomics: coherent across the XCDs of one device
+// without fences (each 64-bit word is single-copy atomic).
+FLX_DEV uint64_t ld_agent(const uint64_t *p)
+{
+    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+FLX_DEV void st_agent(uint64_t *p, uint64_t v)
+{
+    __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// A double posted as two 64-bit words, each tagged with a 32-bit sequence
+// number in its high half, so a reader can verify every word it reads
+// belongs to the step it expects without any fence or flag ordering.
+FLX_DEV void post_double(uint64_t *w, double v, uint32_t seq)
+{
+    uint64_t b = __double_as_longlong(v);
+    st_agent(w, ((uint64_t)seq << 32) | (b >> 32));
+    st_agent(w + 1, ((uint64_t)seq << 32) | (b & 0xffffffffull));
+}
+FLX_DEV double join_double(uint64_t hi, uint64_t lo)
+{
+    return __longlong_as_double((long long)(((hi & 0xffffffffull) << 32) |
+                                            (lo & 0xffffffffull)));
+}
+FLX_DEV uint32_t seq_of(uint64_t w) { return (uint32_t)(w >> 32); }
+
+// Wave-level (64 lanes) reductions through cross-lane shuffles.
+FLX_DEV double wave_sum(double v)
+{
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+    return v;
+}
+FLX_DEV double wave_max(double v)
+{
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v = fmax(v, __shfl_xor(v, o, 64));
+    return v;
+}
+
+// Pivot candidate ordering: larger |value| wins, ties go to the lower row
+// (cxxblas/level1/iamax.tcc:42-58 keeps the first maximum).
+FLX_DEV bool cand_better(double a, int ia, double b, int ib)
+{
+    return (a > b) || (a == b && ia < ib);
+}

@@ -1,0 +1,238 @@
+// getrs.hip -- solves with the LU factors (FP64, column-major).
+//
+// Replaces flens::lapack::trs (flens/lapack/ge/trs.tcc:59-109): NoTrans =
+// laswp(B,piv), L-unit solve, U solve; Trans = U' solve, L' solve,
+// laswp(B, piv.reverse()).  The two triangular solves are the
+// blas::sm -> cxxblas::trsm -> trsv chain of cxxblas/level3/trsm.tcc:40-68.
+//
+// Row interchanges are applied as one gather with the composed permutation
+// (perm[i] = original row that ends up in row i), and the triangular solves
+// are blocked by 64 rows: one launch per diagonal block solves the block in
+// LDS and folds it into the remaining rows (GEMM-shaped, coalesced reads of
+// the factor columns).
+#include <climits>
+#include "flx_common.h"
+#include "flx_kernels.h"
+
+namespace flx {
+
+// ---------------------------------------------------------------------------
+// perm from LAPACK ipiv (1-based, row i swapped with ipiv[i]); reverse=1
+// composes the swaps last to first.  Sequential by nature; one lane walks
+// the swaps in LDS (n <= 16384) or in global memory.
+// ---------------------------------------------------------------------------
+constexpr int PERM_LDS = 16384;
+
+__global__ void ipiv_to_perm_kernel(const int *__restrict__ ipiv, int *__restrict__ perm,
+                                    int n, int reverse)
+{
+    __shared__ int s[PERM_LDS];
+    const bool lds = n <= PERM_LDS;
+    int *p = lds ? s : perm;
+    for (int i = threadIdx.x; i < n; i += blockDim.x) p[i] = i;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        if (!reverse) {
+            for (int i = 0; i < n; ++i) {
+                int q = ipiv[i] - 1;
+                if (q != i && q >= 0 && q < n) { int t = p[i]; p[i] = p[q]; p[q] = t; }
+            }
+        } else {
+            for (int i = n - 1; i >= 0; --i) {
+                int q = ipiv[i] - 1;
+                if (q != i && q >= 0 && q < n) { int t = p[i]; p[i] = p[q]; p[q] = t; }
+            }
+        }
+    }
+    __syncthreads();
+    if (lds)
+        for (int i = threadIdx.x; i < n; i += blockDim.x) perm[i] = s[i];
+}
+
+hipError_t ipiv_to_perm(hipStream_t s, const int *ipiv, int *perm, int n, int reverse)
+{
+    if (n <= 0) return hipSuccess;
+    hipLaunchKernelGGL(ipiv_to_perm_kernel, dim3(1), dim3(256), 0, s, ipiv, perm, n, reverse);
+    return hipGetLastError();
+}
+
+__global__ void gather_rows_kernel(const double *__restrict__ B, long ldb,
+                                   double *__restrict__ out, long ldo,
+                                   const int *__restrict__ perm, int n, int nrhs)
+{
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    int c = blockIdx.y;
+    if (i < n) out[(long)c * ldo + i] = B[(long)c * ldb + perm[i]];
+}
+
+hipError_t gather_rows_cm(hipStream_t s, const double *B, long ldb, double *out, long ldo,
+                          const int *perm, int n, int nrhs)
+{
+    if (n <= 0 || nrhs <= 0) return hipSuccess;
+    hipLaunchKernelGGL(gather_rows_kernel, dim3((n + 255) / 256, nrhs), dim3(256), 0, s, B,
+                       ldb, out, ldo, perm, n, nrhs);
+    return hipGetLastError();
+}
+
+__global__ void copy_cm_kernel(const double *__restrict__ in, long ldi,
+                               double *__restrict__ out, long ldo, int m)
+{
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    int c = blockIdx.y;
+    if (i < m) out[(long)c * ldo + i] = in[(long)c * ldi + i];
+}
+
+hipError_t copy_cm(hipStream_t s, const double *in, long ldi, double *out, long ldo, int m,
+                   int n)
+{
+    if (m <= 0 || n <= 0) return hipSuccess;
+    hipLaunchKernelGGL(copy_cm_kernel, dim3((m + 255) / 256, n), dim3(256), 0, s, in, ldi, out,
+                       ldo, m);
+    return hipGetLastError();
+}
+
+// ---------------------------------------------------------------------------
+// Blocked triangular solve.  T(i,q) is the triangular operand:
+//   FWD (forward order, T lower): L (unit, NoTrans) or U^T (non-unit, Trans)
+//   BWD (backward order, T upper): U (non-unit, NoTrans) or L^T (unit, Trans)
+// TR selects the element address: NoTrans T(i,q) = LU[i + q*lda],
+// Trans T(i,q) = LU[q + i*lda].
+// One launch per 64-row diagonal block [kb, kb+bs): every workgroup solves
+// the block redundantly in LDS (right-looking, bs barriers), block 0 writes
+// the solution rows to X, and every workgroup updates its 256 trailing rows
+// of B in place.  The solution goes to a separate array so no workgroup can
+// read a block row another one already overwrote.
+// ---------------------------------------------------------------------------
+constexpr int TS_BS = 64;
+constexpr int TS_CG = 16;   // right-hand sides per pass
+
+template <int TR, int UNIT, int FWD>
+__global__ __launch_bounds__(256) void trs_block_kernel(const double *__restrict__ LU, long lda,
+                                                        double *__restrict__ B, long ldb,
+                                                        double *__restrict__ X, long ldx,
+                                                        int n, int nrhs, int kb, int bs)
+{
+    __shared__ double xs[TS_BS][TS_CG + 1];
+    const int tid = threadIdx.x;
+    auto T = [&](int i, int q) -> double {
+        return TR ? LU[(long)i * lda + q] : LU[(long)q * lda + i];
+    };
+    // trailing rows: FWD -> [kb+bs, n), BWD -> [0, kb)
+    const int t0 = FWD ? kb + bs : 0;
+    const int tn = FWD ? n - (kb + bs) : kb;
+    const int r = t0 + blockIdx.x * 256 + tid;
+    const bool has_row = (blockIdx.x * 256 + tid) < tn;
+
+    for (int cg = 0; cg < nrhs; cg += TS_CG) {
+        const int nc = min(TS_CG, nrhs - cg);
+        for (int q = tid; q < bs * TS_CG; q += 256) {
+            int i = q / TS_CG, c = q % TS_CG;
+            xs[i][c] = (c < nc) ? B[(long)(cg + c) * ldb + kb + i] : 0.0;
+        }
+        __syncthreads();
+        // solve the diagonal block
+        for (int s = 0; s < bs; ++s) {
+            const int i = FWD ? s : bs - 1 - s;
+            if (!UNIT) {
+                if (tid < TS_CG) xs[i][tid] = xs[i][tid] / T(kb + i, kb + i);
+                __syncthreads();
+            }
+            // fold x_i into the remaining rows of the block
+            const int rem = FWD ? bs - 1 - i : i;
+            for (int q = tid; q < rem * TS_CG; q += 256) {
+                int ii = (FWD ? i + 1 : 0) + q / TS_CG, c = q % TS_CG;
+                xs[ii][c] = fma(-T(kb + ii, kb + i), xs[i][c], xs[ii][c]);
+            }
+            __syncthreads();
+        }
+        if (blockIdx.x == 0)
+            for (int q = tid; q < bs * TS_CG; q += 256) {
+                int i = q / TS_CG, c = q % TS_CG;
+                if (c < nc) X[(long)(cg + c) * ldx + kb + i] = xs[i][c];
+            }
+        if (has_row) {
+            double acc[TS_CG];
+#pragma unroll
+            for (int c = 0; c < TS_CG; ++c)
+                acc[c] = (c < nc) ? B[(long)(cg + c) * ldb + r] : 0.0;
+            for (int q = 0; q < bs; ++q) {
+                const double t = T(r, kb + q);
+#pragma unroll
+                for (int c = 0; c < TS_CG; ++c) acc[c] = fma(-t, xs[q][c], acc[c]);
+            }
+#pragma unroll
+            for (int c = 0; c < TS_CG; ++c)
+                if (c < nc) B[(long)(cg + c) * ldb + r] = acc[c];
+        }
+        __syncthreads();
+    }
+}
+
+hipError_t trsm_cm_f64(hipStream_t s, int upper, int trans, int unit, int n, int nrhs,
+                       const double *LU, long lda, double *B, long ldb, double *X,
+                       long ldx)
+{
+    if (n <= 0 || nrhs <= 0) return hipSuccess;
+    // forward order when the operand is lower triangular
+    const int fwd = (!upper && !trans) || (upper && trans);
+    const int nblk = (n + TS_BS - 1) / TS_BS;
+    for (int b = 0; b < nblk; ++b) {
+        const int blk = fwd ? b : nblk - 1 - b;
+        const int kb = blk * TS_BS;
+        const int bs = min(TS_BS, n - kb);
+        const int tn = fwd ? n - (kb + bs) : kb;
+        const int grid = tn > 0 ? (tn + 255) / 256 : 1;
+        // solve of the block + fold into tn rows: factor block and rows read once
+        ProfScope ps(tl_prof, s, PROF_TRS, (double)nrhs * (bs * bs + 2.0 * bs * tn),
+                     8.0 * ((double)bs * bs / 2 + (double)bs * tn + 2.0 * nrhs * (bs + tn)));
+#define FLX_TS(tr_, un_, fw_)                                                  \
+    hipLaunchKernelGGL((trs_block_kernel<tr_, un_, fw_>), dim3(grid), dim3(256), 0, s, LU, \
+                       lda, B, ldb, X, ldx, n, nrhs, kb, bs)
+        if (!trans && unit && fwd) FLX_TS(0, 1, 1);
+        else if (!trans && !unit && !fwd) FLX_TS(0, 0, 0);
+        else if (trans && !unit && fwd) FLX_TS(1, 0, 1);
+        else if (trans && unit && !fwd) FLX_TS(1, 1, 0);
+        else if (!trans && unit && !fwd) FLX_TS(0, 1, 0);
+        else if (!trans && !unit && fwd) FLX_TS(0, 0, 1);
+        else if (trans && unit && fwd) FLX_TS(1, 1, 1);
+        else FLX_TS(1, 0, 0);
+#undef FLX_TS
+        hipError_t e = hipGetLastError();
+        if (e != hipSuccess) return e;
+    }
+    return hipSuccess;
+}
+
+// ---------------------------------------------------------------------------
+// dlaswp on a column-major matrix: rows k1..k2 (1-based), incx = +1 forward
+// / -1 backward (flens/lapack/la/laswp.tcc:54-93).  One thread per column.
+// ---------------------------------------------------------------------------
+__global__ void laswp_cm_kernel(double *__restrict__ A, long lda, int ncols, int k1, int k2,
+                                const int *__restrict__ ipiv, int incx)
+{
+    int c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= ncols) return;
+    double *col = A + (long)c * lda;
+    if (incx > 0) {
+        for (int i = k1 - 1; i < k2; ++i) {
+            int q = ipiv[i] - 1;
+            if (q != i) { double t = col[i]; col[i] = col[q]; col[q] = t; }
+        }
+    } else {
+        for (int i = k2 - 1; i >= k1 - 1; --i) {
+            int q = ipiv[i] - 1;
+            if (q != i) { double t = col[i]; col[i] = col[q]; col[q] = t; }
+        }
+    }
+}
+
+hipError_t laswp_cm_f64(hipStream_t s, double *A, long lda, int ncols, int k1, int k2,
+                        const int *ipiv, int incx)
+{
+    if (ncols <= 0 || k2 < k1) return hipSuccess;
+    hipLaunchKernelGGL(laswp_cm_kernel, dim3((ncols + 63) / 64), dim3(64), 0, s, A, lda, ncols,
+                       k1, k2, ipiv, incx);
+    return hipGetLastError();
+}
+
+} // namespace flx

@@ -1,0 +1,159 @@
+/*
+ * flx.h -- C ABI of libflx, an MI355X-native (gfx950, HIP) dense GEMM and
+ * LU-solve library that is a drop-in for FLENS's GEMM / LU path.
+ *
+ * Two layers, both plain C (pointers and sizes, no C++ or torch types):
+ *
+ *  1. The FLENS plugin seams.  FLENS reaches an external BLAS through the
+ *     CBLAS symbols (cxxblas/drivers/cblas.h) when HAVE_CBLAS is set, and an
+ *     external LAPACK through the Fortran symbols (cxxlapack/netlib/
+ *     interface/lapack.in.h) when USE_CXXLAPACK + ALWAYS_USE_CXXLAPACK are
+ *     set (flens/lapack/lapack.h:48-55).  libflx exports exactly those
+ *     symbols with the same signatures, so unmodified FLENS code runs its
+ *     GEMM, trf, trs and sv on the GPU by linking libflx instead of a CPU
+ *     BLAS/LAPACK.  They take HOST pointers, stage through device memory,
+ *     and are synchronous, like the CPU routines they replace.
+ *
+ *  2. flx_* device-pointer entry points: the same operations on data that
+ *     already lives in device memory (column-major, caller-owned), queued
+ *     on the handle's HIP stream.  This is the fast path.
+ *
+ * Conventions (identical to the reference): column-major storage with a
+ * leading dimension, 1-based int32 pivot vectors ("row i was interchanged
+ * with row ipiv[i]"), A overwritten by L\U with L's unit diagonal not
+ * stored, B overwritten by X, INFO = 0 success, INFO = j > 0 first exact
+ * zero pivot U(j,j) (the factorisation completes, dgesv_ skips the solve),
+ * INFO = -i invalid i-th argument after calling xerbla_.
+ */
+#ifndef FLX_H
+#define FLX_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ------------------------------------------------------------------ status */
+enum flx_status {
+    FLX_SUCCESS = 0,
+    FLX_INVALID_ARG = 1,
+    FLX_HIP_ERROR = 2,
+    FLX_ALLOC_FAILED = 3,
+    FLX_SYNC_TIMEOUT = 4,     /* a bounded device-side wait gave up */
+    FLX_UNSUPPORTED = 5
+};
+
+typedef struct flx_context *flx_handle;
+
+/* Handle with its own scratch memory; bound to the current device and to
+ * the null stream until flx_set_stream.  One handle per host thread. */
+int flx_create(flx_handle *h);
+int flx_destroy(flx_handle h);
+int flx_set_stream(flx_handle h, void *hip_stream);
+int flx_sync(flx_handle h);                /* waits, then reports device-side errors */
+const char *flx_status_string(int status);
+/* LU blocking: nb = outer block (trailing-update K), ib = panel width
+ * (<= 64).  The reference's equivalent knob is ilaenv(1,"GETRF") = 64,
+ * flens/lapack/la/ilaenv.tcc:91-99. */
+int flx_set_lu_blocking(flx_handle h, int nb, int ib);
+
+/* Launch profiler (no counterpart in the reference, which has no timers:
+ * flens/debug only logs calls).  Between start and stop every library
+ * launch of a profiled kernel class is bracketed by HIP events on the
+ * stream it runs on.  stop() synchronises and writes FLX_PROF_NCLS x 4
+ * doubles: {launches, milliseconds, algorithmic flops, algorithmic bytes}
+ * per class, classes in the order of enum flx_prof_class. */
+enum flx_prof_class {
+    FLX_PROF_GEMM = 0, FLX_PROF_PANEL = 1, FLX_PROF_ROWPERM = 2,
+    FLX_PROF_TRSM_RM = 3, FLX_PROF_TRANSPOSE = 4, FLX_PROF_TRS = 5,
+    FLX_PROF_OTHER = 6, FLX_PROF_NCLS = 7
+};
+int flx_profile_start(flx_handle h);
+int flx_profile_stop(flx_handle h, double *out);
+
+/* ------------------------------------------------- device-pointer entries */
+/* C <- beta*C + alpha*op(A)*op(B), trans = 0 NoTrans / 1 Trans.
+ * Replaces cxxblas::gemm (cxxblas/level3/gemm.tcc:149-299, reached from
+ * flens::blas::mm, flens/blas/level3/mm.tcc:380-479).  Like the reference's
+ * generic path, beta is applied to C first, so beta = 0 computes 0*C. */
+int flx_dgemm(flx_handle h, int transa, int transb, int m, int n, int k,
+              double alpha, const double *A, int lda, const double *B,
+              int ldb, double beta, double *C, int ldc);
+
+/* LU with partial pivoting in place; ipiv (min(m,n) ints) and info (one
+ * int) are device pointers.  Replaces flens::lapack::trf
+ * (flens/lapack/ge/trf.tcc:183-261). */
+int flx_dgetrf(flx_handle h, int m, int n, double *A, int lda, int *ipiv,
+               int *info);
+
+/* Solve op(A) X = B with the factors of flx_dgetrf; trans 0/1.  Replaces
+ * flens::lapack::trs (flens/lapack/ge/trs.tcc:151-252). */
+int flx_dgetrs(flx_handle h, int trans, int n, int nrhs, const double *A,
+               int lda, const int *ipiv, double *B, int ldb);
+
+/* trf + trs; *info_host (host int) receives INFO and the solve is skipped
+ * when INFO > 0, as in flens::lapack::sv (flens/lapack/ge/sv.tcc:58-77).
+ * This entry synchronises the stream to read INFO. */
+int flx_dgesv(flx_handle h, int n, int nrhs, double *A, int lda, int *ipiv,
+              double *B, int ldb, int *info_host);
+
+/* Row interchanges k1..k2 (1-based), incx = +1 / -1.  Replaces
+ * flens::lapack::laswp (flens/lapack/la/laswp.tcc:126-186). */
+int flx_dlaswp(flx_handle h, int n, double *A, int lda, int k1, int k2,
+               const int *ipiv, int incx);
+
+/* Fill an m x n column-major block with the library's seeded uniform
+ * [-1,1) stream (element (i,j) = stream[offset + j*m + i]). */
+int flx_fill_uniform(flx_handle h, double *A, int m, int n, int lda,
+                     uint64_t seed, uint64_t offset);
+
+/* --------------------------------------------------- FLENS plugin seams */
+/* CBLAS seam, cxxblas/drivers/cblas.h:798-805; enum values :22-27
+ * (Order 101/102, Trans 111/112/113).  Caller: cxxblas::gemm<int> for
+ * double, cxxblas/level3/gemm.tcc:327-349.  BLAS convention: beta = 0
+ * does not read C. */
+void cblas_dgemm(int order, int transA, int transB, int M, int N, int K,
+                 double alpha, const double *A, int lda, const double *B,
+                 int ldb, double beta, double *C, int ldc);
+
+/* CBLAS seam, cxxblas/drivers/cblas.h:1044-1049 (Side 141/142, Uplo
+ * 121/122, Diag 131/132).  Caller: cxxblas::trsm, cxxblas/level3/
+ * trsm.tcc:107-126, reached from blas::sm (flens/blas/level3/sm.tcc:63). */
+void cblas_dtrsm(int order, int side, int uplo, int transA, int diag, int M,
+                 int N, double alpha, const double *A, int lda, double *B,
+                 int ldb);
+
+/* Fortran LAPACK seam, cxxlapack/netlib/interface/lapack.in.h:6751-6756.
+ * Caller: cxxlapack::getrf (cxxlapack/interface/getrf.tcc:68-90) from
+ * flens::lapack::external::trf_impl (flens/lapack/ge/trf.tcc:164-173);
+ * same shape as FLENS's own export flens/lapack/interface/src/getrf.cc:9-45. */
+void dgetrf_(const int *M, const int *N, double *A, const int *LDA, int *IPIV,
+             int *INFO);
+
+/* lapack.in.h:6770-6778; caller external::trs_impl (trs.tcc:124-140);
+ * FLENS export flens/lapack/interface/src/getrs.cc:12-53. */
+void dgetrs_(const char *TRANS, const int *N, const int *NRHS, const double *A,
+             const int *LDA, const int *IPIV, double *B, const int *LDB,
+             int *INFO);
+
+/* lapack.in.h:6633-6640; caller external::sv_impl (sv.tcc:92-105);
+ * FLENS export flens/lapack/interface/src/gesv.cc:12-49. */
+void dgesv_(const int *N, const int *NRHS, double *A, const int *LDA,
+            int *IPIV, double *B, const int *LDB, int *INFO);
+
+/* lapack.in.h:9333-9339; caller external::laswp_impl
+ * (flens/lapack/la/laswp.tcc:105-118). */
+void dlaswp_(const int *N, double *A, const int *LDA, const int *K1,
+             const int *K2, const int *IPIV, const int *INCX);
+
+/* Error hook called before a negative INFO is returned, as the reference's
+ * LAPACK_ERROR (flens/lapack/interface/include/config.h:40-45).  libflx
+ * provides a weak default that prints the LAPACK message to stderr. */
+void xerbla_(const char *SRNAME, const int *INFO, int SRNAME_LEN);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* FLX_H */

@@ -1,0 +1,264 @@
+"""CPU oracle bindings -- TEST INFRASTRUCTURE ONLY.
+
+ctypes front-end over ``oracle/liboracle.so`` (the plain-C restatement of the
+FLENS generic GEMM / LU path in ``flens_oracle.c``) and, where it was built,
+``oracle/_ref/libflensref.so`` (the reference itself, compiled from its own
+headers).  Only tests/, ``__graft_entry__.smoke()`` and bench.py's
+``cpu_baseline`` leg import this module; the product package never does.
+
+All matrices are numpy float64 arrays in Fortran (column-major) order;
+pivot vectors are 1-based int32, as in LAPACK/FLENS.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+_ORC = os.path.join(HERE, "liboracle.so")
+_REF = os.path.join(HERE, "_ref", "libflensref.so")
+
+_dp = C.POINTER(C.c_double)
+_fp = C.POINTER(C.c_float)
+_ip = C.POINTER(C.c_int)
+_i, _d, _u64 = C.c_int, C.c_double, C.c_uint64
+
+
+def build() -> None:
+    """Compile liboracle.so (and _ref where the reference is present)."""
+    import subprocess
+    subprocess.run(["make", "-s", "-C", HERE], check=True)
+
+
+def _load(path: str):
+    if not os.path.exists(path):
+        return None
+    return C.CDLL(path)
+
+
+_lib = None
+_ref = None
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(_ORC):
+            build()
+        _lib = C.CDLL(_ORC)
+        _lib.orc_fill_uniform.argtypes = [_dp, _i, _i, _i, _u64, _u64]
+        _lib.orc_dgemm.argtypes = [_i, _i, _i, _i, _i, _d, _dp, _i, _dp, _i, _d, _dp, _i]
+        _lib.orc_dgetrf_nb.argtypes = [_i, _i, _dp, _i, _ip, _i]
+        _lib.orc_dgetrf_nb.restype = _i
+        _lib.orc_dgetrf_gaps.argtypes = [_i, _i, _dp, _i, _ip, _i, _dp]
+        _lib.orc_dgetrf_gaps.restype = _i
+        _lib.orc_sgetrf_nb.argtypes = [_i, _i, _fp, _i, _ip, _i]
+        _lib.orc_sgetrf_nb.restype = _i
+        _lib.orc_dgetrs_c.argtypes = [_i, _i, _i, _dp, _i, _ip, _dp, _i]
+        _lib.orc_sgetrs_c.argtypes = [_i, _i, _i, _fp, _i, _ip, _fp, _i]
+        _lib.orc_dgesv_nb.argtypes = [_i, _i, _dp, _i, _ip, _dp, _i, _i]
+        _lib.orc_dgesv_nb.restype = _i
+        _lib.orc_dlaswp.argtypes = [_i, _dp, _i, _i, _i, _ip, _i]
+        _lib.orc_dget01.argtypes = [_i, _i, _dp, _i, _dp, _i, _ip]
+        _lib.orc_dget01.restype = _d
+        _lib.orc_dget02.argtypes = [_i, _i, _dp, _i, _dp, _i, _dp, _i]
+        _lib.orc_dget02.restype = _d
+        _lib.orc_dmmch.argtypes = [_i, _i, _i, _i, _i, _d, _dp, _i, _dp, _i, _d, _dp, _dp, _i]
+        _lib.orc_dmmch.restype = _d
+        _lib.orc_dsgesv.argtypes = [_i, _i, _dp, _i, _ip, _dp, _i, _dp, _i, _i, _ip]
+        _lib.orc_dsgesv.restype = _i
+        _lib.orc_dgetrf_steps.argtypes = [_i, _i, _dp, _i, _i, _i]
+    return _lib
+
+
+def ref():
+    """The compiled reference (None where it was not built)."""
+    global _ref
+    if _ref is None:
+        _ref = _load(_REF)
+        if _ref is None:
+            return None
+        _ref.ref_dgemm.argtypes = [_i, _i, _i, _i, _i, _d, _dp, _i, _dp, _i, _d, _dp, _i]
+        _ref.ref_dgetrf.argtypes = [_i, _i, _dp, _i, _ip, _i]
+        _ref.ref_dgetrf.restype = _i
+        _ref.ref_dgetrs.argtypes = [_i, _i, _i, _dp, _i, _ip, _dp, _i]
+        _ref.ref_dgesv.argtypes = [_i, _i, _dp, _i, _ip, _dp, _i, _i]
+        _ref.ref_dgesv.restype = _i
+        _ref.ref_dlaswp.argtypes = [_i, _i, _dp, _i, _i, _i, _ip, _i]
+    return _ref
+
+
+def _p(a):
+    if a.dtype == np.float64:
+        return a.ctypes.data_as(_dp)
+    if a.dtype == np.float32:
+        return a.ctypes.data_as(_fp)
+    if a.dtype == np.int32:
+        return a.ctypes.data_as(_ip)
+    raise TypeError(a.dtype)
+
+
+def _ld(a):
+    """Leading dimension of a Fortran-ordered 2-D array."""
+    assert a.flags.f_contiguous or a.ndim == 1
+    return max(1, a.shape[0])
+
+
+# ------------------------------------------------------------------- RNG ---
+_GOLDEN = np.uint64(0x9E3779B97F4A7C15)
+
+
+def _mix64(z):
+    z = (z ^ (z >> np.uint64(30))) * np.uint64(0xBF58476D1CE4E5B9)
+    z = (z ^ (z >> np.uint64(27))) * np.uint64(0x94D049BB133111EB)
+    return z ^ (z >> np.uint64(31))
+
+
+def uniform(m: int, n: int, seed: int, offset: int = 0) -> np.ndarray:
+    """m x n column-major matrix, element (i,j) = stream[offset + j*m + i],
+    exact dyadic values in [-1,1).  Vectorised numpy twin of
+    orc_fill_uniform and of the library's device generator."""
+    idx = np.arange(m * n, dtype=np.uint64) + np.uint64(offset)
+    with np.errstate(over="ignore"):
+        z = _mix64(np.uint64(seed) + (idx + np.uint64(1)) * _GOLDEN)
+    k = (z >> np.uint64(11)).astype(np.int64) - (np.int64(1) << np.int64(52))
+    v = k.astype(np.float64) * 2.0 ** -52
+    return np.asfortranarray(v.reshape(n, m).T)
+
+
+# ------------------------------------------------------------ oracle API ---
+def dgemm(transa, transb, alpha, a, b, beta, c, use_ref=False):
+    """C <- beta*C + alpha*op(A)*op(B) in place (gemm_generic order)."""
+    L = ref() if use_ref else lib()
+    m, n = c.shape
+    k = a.shape[0] if transa else a.shape[1]
+    L_fn = L.ref_dgemm if use_ref else L.orc_dgemm
+    L_fn(int(transa), int(transb), m, n, k, float(alpha), _p(a), _ld(a), _p(b), _ld(b),
+         float(beta), _p(c), _ld(c))
+    return c
+
+
+def dgetrf(a, nb=64, use_ref=False):
+    """In-place LU (trf); returns (ipiv 1-based int32, info)."""
+    m, n = a.shape
+    ipiv = np.zeros(max(1, min(m, n)), dtype=np.int32)
+    if use_ref:
+        info = ref().ref_dgetrf(m, n, _p(a), _ld(a), _p(ipiv), nb)
+    else:
+        info = lib().orc_dgetrf_nb(m, n, _p(a), _ld(a), _p(ipiv), nb)
+    return ipiv[: min(m, n)], info
+
+
+def dgetrf_steps(a, nb, nsteps):
+    """Run only the first nsteps block steps of the blocked getrf."""
+    m, n = a.shape
+    lib().orc_dgetrf_steps(m, n, _p(a), _ld(a), nb, nsteps)
+    return a
+
+
+def getrf_steps_flops(m: int, n: int, nb: int, nsteps: int) -> float:
+    """Exact flop count of the first nsteps block steps (panel getf2 with
+    one reciprocal-multiply per subdiagonal entry, unit-lower trsm, GEMM)."""
+    mn = min(m, n)
+    fl = 0.0
+    j = 0
+    for _ in range(nsteps):
+        if j >= mn:
+            break
+        jb = min(nb, mn - j)
+        mp = m - j
+        for c in range(jb):
+            below = mp - c - 1
+            fl += below + 2.0 * below * (jb - c - 1)
+        nr = n - j - jb
+        if nr > 0:
+            fl += float(jb) * (jb - 1) * nr
+            fl += 2.0 * (m - j - jb) * nr * jb
+        j += jb
+    return fl
+
+
+def dgetrf_gaps(a, nb=64):
+    m, n = a.shape
+    ipiv = np.zeros(max(1, min(m, n)), dtype=np.int32)
+    gap = np.zeros(max(1, min(m, n)), dtype=np.float64)
+    info = lib().orc_dgetrf_gaps(m, n, _p(a), _ld(a), _p(ipiv), nb, _p(gap))
+    return ipiv[: min(m, n)], info, gap[: min(m, n)]
+
+
+def sgetrf(a, nb=64):
+    m, n = a.shape
+    ipiv = np.zeros(max(1, min(m, n)), dtype=np.int32)
+    info = lib().orc_sgetrf_nb(m, n, _p(a), _ld(a), _p(ipiv), nb)
+    return ipiv[: min(m, n)], info
+
+
+def dgetrs(trans, a, ipiv, b, use_ref=False):
+    n = a.shape[0]
+    nrhs = b.shape[1]
+    ipiv = np.ascontiguousarray(ipiv, dtype=np.int32)
+    if use_ref:
+        ref().ref_dgetrs(int(trans), n, nrhs, _p(a), _ld(a), _p(ipiv), _p(b), _ld(b))
+    else:
+        lib().orc_dgetrs_c(int(trans), n, nrhs, _p(a), _ld(a), _p(ipiv), _p(b), _ld(b))
+    return b
+
+
+def sgetrs(trans, a, ipiv, b):
+    n = a.shape[0]
+    nrhs = b.shape[1]
+    ipiv = np.ascontiguousarray(ipiv, dtype=np.int32)
+    lib().orc_sgetrs_c(int(trans), n, nrhs, _p(a), _ld(a), _p(ipiv), _p(b), _ld(b))
+    return b
+
+
+def dgesv(a, b, nb=64, use_ref=False):
+    n = a.shape[0]
+    ipiv = np.zeros(max(1, n), dtype=np.int32)
+    if use_ref:
+        info = ref().ref_dgesv(n, b.shape[1], _p(a), _ld(a), _p(ipiv), _p(b), _ld(b), nb)
+    else:
+        info = lib().orc_dgesv_nb(n, b.shape[1], _p(a), _ld(a), _p(ipiv), _p(b), _ld(b), nb)
+    return ipiv[:n], info
+
+
+def dlaswp(a, k1, k2, ipiv, incx, use_ref=False):
+    ipiv = np.ascontiguousarray(ipiv, dtype=np.int32)
+    m, n = a.shape
+    if use_ref:
+        ref().ref_dlaswp(m, n, _p(a), _ld(a), k1, k2, _p(ipiv), incx)
+    else:
+        lib().orc_dlaswp(n, _p(a), _ld(a), k1, k2, _p(ipiv), incx)
+    return a
+
+
+def dget01(a0, lu, ipiv):
+    m, n = a0.shape
+    ipiv = np.ascontiguousarray(ipiv, dtype=np.int32)
+    return lib().orc_dget01(m, n, _p(a0), _ld(a0), _p(lu), _ld(lu), _p(ipiv))
+
+
+def dget02(a, x, b):
+    n = a.shape[0]
+    return lib().orc_dget02(n, x.shape[1], _p(a), _ld(a), _p(x), _ld(x), _p(b), _ld(b))
+
+
+def dmmch(transa, transb, alpha, a, b, beta, c0, c):
+    m, n = c.shape
+    k = a.shape[0] if transa else a.shape[1]
+    return lib().orc_dmmch(int(transa), int(transb), m, n, k, float(alpha), _p(a), _ld(a),
+                           _p(b), _ld(b), float(beta), _p(c0), _p(c), _ld(c))
+
+
+def dsgesv(a, b, itermax=30):
+    """netlib dsgesv semantics; returns (x, ipiv, info, iter)."""
+    n = a.shape[0]
+    nrhs = b.shape[1]
+    x = np.zeros((n, nrhs), order="F")
+    ipiv = np.zeros(max(1, n), dtype=np.int32)
+    it = C.c_int(0)
+    info = lib().orc_dsgesv(n, nrhs, _p(a), _ld(a), _p(ipiv), _p(b), _ld(b), _p(x), _ld(x),
+                            itermax, C.byref(it))
+    return x, ipiv[:n], info, it.value

@@ -1,0 +1,101 @@
+// ref_driver.cc -- TEST INFRASTRUCTURE ONLY.
+//
+// A thin C entry layer over the reference's own generic FLENS path, compiled
+// directly from the reference's headers (g++ -I<reference root>) into
+// oracle/_ref/libflensref.so by oracle/Makefile.  Nothing of the reference is
+// copied into this repository; this file only calls it.  It exists so the
+// C restatement in flens_oracle.c can be checked bit for bit against the
+// reference itself, and so golden fixtures can be generated from it
+// (tests/golden/make_golden.py).  The product library never links it.
+//
+// The GETRF block size is swept through the reference's own ilaenv hook
+// (flens/lapack/la/ilaenv.tcc:453-509: USE_ILAENV_WITH_UNDERSCORE plus a
+// caller-supplied ilaenv_), the same hook the netlib test suite uses.
+
+#define USE_ILAENV_WITH_UNDERSCORE 1
+#define USE_NATIVE_ILAENV 1
+#define INTEGER int
+#include <flens/flens.cxx>
+
+static int g_getrf_nb = 64;
+
+extern "C" int ilaenv_(const int *spec, const char *, const char *,
+                       const int *, const int *, const int *, const int *,
+                       int, int)
+{
+    if (*spec == 1) return g_getrf_nb;   // optimal block size (GETRF)
+    if (*spec == 2) return 2;            // minimal block size
+    return 1;                            // crossover
+}
+
+using namespace flens;
+
+typedef FullStorageView<double, ColMajor>       DView;
+typedef ConstFullStorageView<double, ColMajor>  DConstView;
+typedef GeMatrix<DView>                          DGeView;
+typedef GeMatrix<DConstView>                     DGeConstView;
+typedef DenseVector<ArrayView<int> >             IView;
+
+extern "C" {
+
+// flens::blas::mm (flens/blas/level3/mm.tcc:380-479) -> cxxblas::gemm
+// generic (cxxblas/level3/gemm.tcc:149-299).  trans: 0 NoTrans, 1 Trans.
+void ref_dgemm(int transa, int transb, int m, int n, int k, double alpha,
+               const double *a, int lda, const double *b, int ldb,
+               double beta, double *c, int ldc)
+{
+    int ar = transa ? k : m, ac = transa ? m : k;
+    int br = transb ? n : k, bc = transb ? k : n;
+    DGeConstView A = DConstView(ar, ac, a, lda);
+    DGeConstView B = DConstView(br, bc, b, ldb);
+    DGeView      C = DView(m, n, c, ldc);
+    blas::mm(transa ? Trans : NoTrans, transb ? Trans : NoTrans,
+             alpha, A, B, beta, C);
+}
+
+// flens::lapack::trf (flens/lapack/ge/trf.tcc:183-261); ipiv 1-based.
+int ref_dgetrf(int m, int n, double *a, int lda, int *ipiv, int nb)
+{
+    g_getrf_nb = nb;
+    DGeView A = DView(m, n, a, lda);
+    IView piv = ArrayView<int>(std::min(m, n), ipiv, 1);
+    return lapack::trf(A, piv);
+}
+
+// flens::lapack::trs (flens/lapack/ge/trs.tcc:151-252).
+void ref_dgetrs(int trans, int n, int nrhs, const double *a, int lda,
+                const int *ipiv, double *b, int ldb)
+{
+    DGeConstView A = DConstView(n, n, a, lda);
+    IView piv = ArrayView<int>(n, const_cast<int *>(ipiv), 1);
+    DGeView B = DView(n, nrhs, b, ldb);
+    lapack::trs(trans ? Trans : NoTrans, A, piv, B);
+}
+
+// flens::lapack::sv (flens/lapack/ge/sv.tcc:115-219).
+int ref_dgesv(int n, int nrhs, double *a, int lda, int *ipiv, double *b,
+              int ldb, int nb)
+{
+    g_getrf_nb = nb;
+    DGeView A = DView(n, n, a, lda);
+    IView piv = ArrayView<int>(n, ipiv, 1);
+    DGeView B = DView(n, nrhs, b, ldb);
+    return lapack::sv(A, piv, B);
+}
+
+// flens::lapack::laswp (flens/lapack/la/laswp.tcc:126-186) over rows
+// k1..k2 (1-based), forward for incx>0, piv.reverse() for incx<0.
+void ref_dlaswp(int m, int n, double *a, int lda, int k1, int k2,
+                const int *ipiv, int incx)
+{
+    DGeView A = DView(m, n, a, lda);
+    int len = k2 - k1 + 1;
+    IView piv = ArrayView<int>(len, const_cast<int *>(ipiv) + (k1 - 1), 1, k1);
+    if (incx > 0) {
+        lapack::laswp(A, piv);
+    } else {
+        lapack::laswp(A, piv.reverse());
+    }
+}
+
+} // extern "C"

@@ -1,0 +1,51 @@
+"""CPU tests of the C-ABI boundary: libflx.so loads without a GPU and exports
+every function include/flx.h declares (no compute calls here)."""
+import os
+import re
+import subprocess
+
+import flens_amd as F
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def declared_functions():
+    text = open(os.path.join(ROOT, "include", "flx.h")).read()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    names = re.findall(r"^\s*(?:const\s+)?[A-Za-z_][A-Za-z0-9_]*\s*\**\s*([A-Za-z_][A-Za-z0-9_]*)\s*\(",
+                       text, flags=re.M)
+    return sorted(set(n for n in names if n not in ("if", "while", "return")))
+
+
+def test_header_declares_expected_symbols():
+    names = declared_functions()
+    assert set(F.EXPORTS) == set(names), set(F.EXPORTS) ^ set(names)
+
+
+def test_library_exports_every_declared_symbol():
+    L = F.lib()
+    for name in declared_functions():
+        assert hasattr(L, name), name
+    out = subprocess.run(["nm", "-D", "--defined-only", F.LIB_PATH], capture_output=True,
+                         text=True, check=True).stdout
+    exported = {line.split()[-1] for line in out.splitlines() if line.strip()}
+    for name in declared_functions():
+        assert name in exported, name
+
+
+def test_no_cpu_fallback_without_gpu():
+    import torch
+    if torch.cuda.is_available():
+        return
+    try:
+        F.default_handle()
+    except F.FlxError:
+        return
+    raise AssertionError("device path must refuse to run without a GPU")
+
+
+def test_oracle_not_linked_into_product():
+    out = subprocess.run(["nm", "-D", F.LIB_PATH], capture_output=True, text=True, check=True).stdout
+    assert "orc_" not in out and "ref_d" not in out
+    src = open(os.path.join(ROOT, "flens_amd", "__init__.py")).read()
+    assert "oracle" not in src.split('"""', 2)[2]

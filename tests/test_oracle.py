@@ -1,0 +1,242 @@
+"""CPU tests: pin the oracle (the C restatement of the FLENS generic path).
+
+Each check is chosen so a plausible restatement mistake (dropped term, wrong
+sign/index, transposed operand, wrong loop order) fails one of them:
+  * bit-exact SHA-256 agreement with fixtures produced by the compiled
+    reference (tests/golden/small.json, lu8192_nb*.json),
+  * the 4x4 known answer of flens/examples/lapack-getrf.cc:43-46,
+  * exact-zero / rank-deficient / tie cases (INFO and pivot conventions),
+  * closed forms: permutation matrices (pivots = inverse permutation),
+    diagonal matrices (identity pivots), integer GEMM (exact),
+  * brute force against a textbook partial-pivot LU in long double (n<=8),
+  * the checkers themselves reject a planted error.
+"""
+import hashlib
+import json
+import os
+
+import numpy as np
+import pytest
+
+from oracle import oracle as O
+
+GOLD = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+
+
+def sha(a):
+    return hashlib.sha256(np.ascontiguousarray(a).tobytes()).hexdigest()
+
+
+@pytest.fixture(scope="module")
+def small():
+    with open(os.path.join(GOLD, "small.json")) as f:
+        return json.load(f)
+
+
+def test_rng_numpy_matches_c():
+    import ctypes as C
+    a = O.uniform(37, 11, 1234, offset=5)
+    b = np.zeros((37, 11), order="F")
+    O.lib().orc_fill_uniform(b.ctypes.data_as(C.POINTER(C.c_double)), 37, 11, 37, 1234, 5)
+    assert np.array_equal(a, b)
+    assert a.min() >= -1.0 and a.max() < 1.0
+    # values are on the 2^-52 grid
+    assert np.array_equal(a * 2.0 ** 52, np.round(a * 2.0 ** 52))
+
+
+def test_gemm_grid_bit_exact(small):
+    for c in small["gemm"]:
+        ta, tb, m, n, k = c["ta"], c["tb"], c["m"], c["n"], c["k"]
+        s = c["seed"]
+        a = O.uniform(k if ta else m, m if ta else k, s)
+        b = O.uniform(n if tb else k, k if tb else n, s + 1)
+        cc = O.uniform(m, n, s + 2)
+        O.dgemm(ta, tb, c["alpha"], a, b, c["beta"], cc)
+        assert sha(cc) == c["sha"], c
+
+
+def test_lu_grid_bit_exact(small):
+    for c in small["lu"]:
+        a = O.uniform(c["m"], c["n"], c["seed"])
+        piv, info = O.dgetrf(a, nb=c["nb"])
+        assert info == c["info"] and piv.tolist() == c["piv"], c
+        assert sha(a) == c["sha_lu"], c
+        if c["m"] == c["n"]:
+            for nrhs in (1, 2, 15, 16):
+                for trans in (0, 1):
+                    b = O.uniform(c["n"], nrhs, c["seed"] + 1)
+                    O.dgetrs(trans, a, piv, b)
+                    assert sha(b) == c[f"sha_x_{nrhs}_{trans}"], (c["m"], nrhs, trans)
+
+
+def test_known_answer_4x4(small):
+    # flens/examples/lapack-getrf.cc:43-46: trf on [A|b], then U solve
+    ab = np.asfortranarray(np.array([[2, 3, -1, 0, 20], [-6, -5, 0, 2, -33],
+                                     [2, -5, 6, -6, -43], [4, 6, 2, -3, 49]], float))
+    piv, info = O.dgetrf(ab)
+    assert piv.tolist() == [2, 3, 4, 4] and info == 0
+    assert piv.tolist() == small["special"]["kat_getrf"]["piv"]
+    assert ab.T.ravel().tolist() == small["special"]["kat_getrf"]["lu"]
+    u = np.triu(ab[:, :4])
+    x = np.linalg.solve(u, ab[:, 4])
+    assert np.allclose(x, [1, 9, 9, 9])
+
+
+def test_exact_zero_and_tie_cases(small):
+    sp = small["special"]
+    for key in ("zero_col", "rank_def", "zero_row", "tie"):
+        a = np.asfortranarray(np.array(sp[key]["a"], float).reshape(-1, int(len(sp[key]["a"]) ** 0.5)).T)
+        piv, info = O.dgetrf(a)
+        assert piv.tolist() == sp[key]["piv"] and info == sp[key]["info"], key
+    # tie: |2| vs |-2| in column 1 -> the first (row 2 here since a11=1) wins
+    assert sp["tie"]["piv"][0] == 2
+    # the generic GEMM computes 0*C for beta = 0 (NaN propagates)
+    assert sp["beta0_nan"]["all_nan"]
+    a, b = O.uniform(3, 3, 1), O.uniform(3, 3, 2)
+    c = np.asfortranarray(np.full((3, 3), np.nan))
+    O.dgemm(0, 0, 1.0, a, b, 0.0, c)
+    assert np.isnan(c).all()
+
+
+def test_permutation_and_diagonal_closed_forms():
+    rng = np.random.default_rng(3)
+    for n in (1, 5, 64, 130):
+        p = rng.permutation(n)
+        a = np.asfortranarray(np.eye(n)[p])           # row i of A = e_{p[i]}
+        piv, info = O.dgetrf(a, nb=32)
+        # apply the interchanges to the identity ordering -> P A = I
+        perm = np.arange(n)
+        for i, q in enumerate(piv - 1):
+            perm[[i, q]] = perm[[q, i]]
+        assert info == 0
+        assert np.array_equal(np.eye(n)[p][perm], np.eye(n))
+        assert np.array_equal(a, np.eye(n))           # L = U = I
+        d = np.asfortranarray(np.diag(rng.uniform(1, 2, n)))
+        piv, info = O.dgetrf(d.copy(order="F"), nb=32)
+        assert np.array_equal(piv, np.arange(1, n + 1))
+
+
+def test_integer_gemm_exact():
+    rng = np.random.default_rng(1)
+    for (ta, tb) in [(0, 0), (0, 1), (1, 0), (1, 1)]:
+        m, n, k = 17, 23, 31
+        a = np.asfortranarray(rng.integers(-8, 9, (k, m) if ta else (m, k)).astype(float))
+        b = np.asfortranarray(rng.integers(-8, 9, (n, k) if tb else (k, n)).astype(float))
+        c = np.asfortranarray(rng.integers(-8, 9, (m, n)).astype(float))
+        want = 2.0 * (a.T if ta else a) @ (b.T if tb else b) - 3.0 * c
+        O.dgemm(ta, tb, 2.0, a, b, -3.0, c)
+        assert np.array_equal(c, want)
+
+
+def _textbook_lu(a):
+    """Partial-pivot LU (Golub & Van Loan alg. 3.4.1) in long double."""
+    a = a.astype(np.longdouble).copy()
+    n = a.shape[0]
+    piv = np.zeros(n, int)
+    for j in range(n):
+        p = j + int(np.argmax(np.abs(a[j:, j])))  # argmax keeps the first max
+        piv[j] = p + 1
+        a[[j, p]] = a[[p, j]]
+        if a[j, j] != 0:
+            a[j + 1:, j] /= a[j, j]
+        a[j + 1:, j + 1:] -= np.outer(a[j + 1:, j], a[j, j + 1:])
+    return a, piv
+
+
+def test_brute_force_small():
+    for n in range(1, 9):
+        for seed in range(20):
+            a0 = O.uniform(n, n, 900 + 31 * n + seed)
+            lu_ref, piv_ref = _textbook_lu(a0)
+            a = a0.copy(order="F")
+            piv, info = O.dgetrf(a, nb=3)
+            assert piv.tolist() == piv_ref.tolist()
+            assert np.allclose(a, lu_ref.astype(float), rtol=1e-12, atol=1e-12)
+            assert O.dget01(a0, a, piv) < 30
+
+
+def test_checkers_reject_planted_error():
+    a0 = O.uniform(40, 40, 77)
+    a = a0.copy(order="F")
+    piv, _ = O.dgetrf(a)
+    assert O.dget01(a0, a, piv) < 30
+    bad = a.copy(order="F")
+    bad[17, 5] += 1e-9                                 # a dropped-term sized error
+    assert O.dget01(a0, bad, piv) > 30
+    wrong_piv = piv.copy()
+    wrong_piv[3] = 4 if wrong_piv[3] != 4 else 5
+    assert O.dget01(a0, a, wrong_piv) > 30
+    b0 = O.uniform(40, 3, 78)
+    x = b0.copy(order="F")
+    O.dgetrs(0, a, piv, x)
+    assert O.dget02(a0, x, b0) < 30
+    x[0, 0] += 1e-9
+    assert O.dget02(a0, x, b0) > 30
+    # DMMCH
+    m = 20
+    aa, bb, c0 = O.uniform(m, m, 1), O.uniform(m, m, 2), O.uniform(m, m, 3)
+    c = c0.copy(order="F")
+    O.dgemm(0, 0, 0.7, aa, bb, 1.3, c)
+    assert O.dmmch(0, 0, 0.7, aa, bb, 1.3, c0, c) < 16
+    c[3, 4] += 1e-12
+    assert O.dmmch(0, 0, 0.7, aa, bb, 1.3, c0, c) > 16
+
+
+def test_laswp_forward_backward_inverse():
+    a0 = O.uniform(30, 7, 5)
+    piv = np.array([3, 5, 3, 30, 5, 6, 7, 29], np.int32)
+    a = a0.copy(order="F")
+    O.dlaswp(a, 1, 8, piv, 1)
+    O.dlaswp(a, 1, 8, piv, -1)
+    assert np.array_equal(a, a0)
+
+
+def test_dsgesv_semantics():
+    n = 60
+    rng = np.random.default_rng(0)
+    a = np.asfortranarray(O.uniform(n, n, 3) + n * np.eye(n))
+    b = O.uniform(n, 2, 4)
+    x, piv, info, it = O.dsgesv(a.copy(order="F"), b)
+    assert info == 0 and 0 <= it <= 30
+    assert np.linalg.norm(a @ x - b, np.inf) <= n * 2.0 ** -53 * np.linalg.norm(a, np.inf) * np.linalg.norm(x, np.inf) * 10
+    # no refinement allowed and a badly scaled system -> fallback code -1
+    q, _ = np.linalg.qr(rng.standard_normal((n, n)))
+    ill = np.asfortranarray(q @ np.diag(np.logspace(0, -12, n)) @ q.T)
+    x, piv, info, it = O.dsgesv(ill.copy(order="F"), b, itermax=0)
+    assert it == -1
+    # overflow in the single-precision copy -> -2
+    big = a.copy(order="F")
+    big[0, 0] = 1e300
+    x, piv, info, it = O.dsgesv(big, b)
+    assert it == -2
+
+
+@pytest.mark.skipif(O.ref() is None, reason="reference sources not built here")
+def test_oracle_vs_compiled_reference_random():
+    rng = np.random.default_rng(11)
+    for _ in range(6):
+        m, n = (int(x) for x in rng.integers(1, 200, 2))
+        nb = int(rng.choice([1, 2, 8, 32, 64]))
+        a0 = O.uniform(m, n, int(rng.integers(1 << 30)))
+        a1, a2 = a0.copy(order="F"), a0.copy(order="F")
+        p1, i1 = O.dgetrf(a1, nb=nb)
+        p2, i2 = O.dgetrf(a2, nb=nb, use_ref=True)
+        assert i1 == i2 and np.array_equal(p1, p2) and sha(a1) == sha(a2)
+
+
+def test_big_lu_fixture_consistency():
+    """The n=8192 fixtures were produced by the reference; the oracle's own
+    run of the same input was compared bit for bit when they were made."""
+    found = 0
+    for nb in (64, 128, 256):
+        path = os.path.join(GOLD, f"lu8192_nb{nb}.json")
+        if not os.path.exists(path):
+            continue
+        found += 1
+        meta = json.load(open(path))
+        z = np.load(os.path.join(GOLD, f"lu8192_nb{nb}.npz"))
+        assert meta["oracle_bit_equal"]
+        piv = z["piv"]
+        assert piv.shape == (8192,) and piv.min() >= 1 and piv.max() <= 8192
+        assert np.all(piv >= np.arange(1, 8193))
+    assert found >= 1
