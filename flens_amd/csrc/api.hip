@@ -75,7 +75,7 @@ static hipError_t init_work(flx_context *h)
     h->wk.exch_words = words;
     e = hipMemsetAsync(h->wk.exch, 0, words * sizeof(uint64_t), h->stream);
     if (e != hipSuccess) return e;
-    e = hipMalloc(&h->wk.list, sizeof(int) * (1 + 2 * 2 * 64 + 16));
+    e = hipMalloc(&h->wk.list, sizeof(int) * PANEL_LIST_SLICE * PANEL_GROUPS_MAX);
     if (e != hipSuccess) return e;
     e = hipMalloc(&h->wk.flags, sizeof(int) * 4);
     if (e != hipSuccess) return e;

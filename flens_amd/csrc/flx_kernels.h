@@ -68,6 +68,17 @@ struct LuOptions {
 };
 
 size_t lu_exch_words();
+// Panel kernel geometry: 512 rows per workgroup, at most 32 workgroups; the
+// interchange list has one slice of 1 + 2*64 ints per workgroup:
+// [count, dst_0, src_0, dst_1, src_1, ...] in absolute rows.
+constexpr int PANEL_ROWS_PER_WG = 512;
+constexpr int PANEL_LIST_SLICE = 1 + 2 * 64;
+constexpr int PANEL_GROUPS_MAX = 32;
+inline int panel_groups(int mp) { return (mp + PANEL_ROWS_PER_WG - 1) / PANEL_ROWS_PER_WG; }
+// Panel factorisation of the mp x w row-major panel P (w <= 32); piv gets
+// 0-based absolute pivot rows, the composed interchanges go to wk.list.
+hipError_t launch_panel(hipStream_t s, LuWork &wk, double *P, long ldw, int mp, int w,
+                        int *piv, int piv_base, int *info, int ib);
 // Factor the m x n row-major matrix W (leading dimension ldw) in place.
 // piv0 receives 0-based absolute pivot rows (length min(m,n)); info receives
 // the 1-based first zero pivot column (INT_MAX if none; caller initialises).

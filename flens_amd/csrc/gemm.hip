@@ -9,13 +9,16 @@
 // still computes 0*C (NaN/Inf in C propagate, gemm.tcc:175), and alpha=0
 // returns after the beta scaling (gemm.tcc:176-178).
 //
-// Design: 256 threads = 4 wave64s in a 2x2 grid; each wave owns a 64x64
-// sub-tile built from 4x4 blocks of the 16x16x4 MFMA (f64: v_mfma_f64_
-// 16x16x4_f64, f32: v_mfma_f32_16x16x4_f32 -- same operand layout).  A is
-// staged in LDS as [k][m] (m contiguous, the MFMA A-operand lane order), B
-// as [n][k] padded by one element so the 16 lanes that read different n hit
-// distinct banks.  Global->register prefetch of tile t+1 overlaps the MFMAs
-// of tile t; LDS is double-buffered, one barrier per K tile.
+// Design: a 128x128 C tile per 512-thread workgroup = 8 wave64s in a 4x2
+// grid; each wave owns a 32x64 sub-tile built from 2x4 blocks of the
+// 16x16x4 MFMA (f64: v_mfma_f64_16x16x4_f64, f32: v_mfma_f32_16x16x4_f32 --
+// same operand layout).  The accumulators (64 registers per lane) fit the
+// architectural VGPRs, so no accumulator copies and two waves per SIMD.  A
+// is staged in LDS as [k][m] (m contiguous, the MFMA A-operand lane order),
+// B as [n][k] padded by one element so the 16 lanes that read different n
+// hit distinct banks.  Global->register prefetch of tile t+1 overlaps the
+// MFMAs of tile t; LDS is double-buffered, one barrier per K tile; interior
+// tiles run a predicate-free path with 16-byte loads.
 #include "flx_common.h"
 #include "flx_kernels.h"
 
@@ -44,45 +47,124 @@ FLX_DEV float4_t mfma16x16x4(float a, float b, float4_t c)
 }
 
 constexpr int GBM = 128, GBN = 128, GBK = 16;
-constexpr int APAD = 16;   // [k][m] rows shifted by 32 banks
-constexpr int BPAD = 1;    // [n][k] rows of 17 elements
+constexpr int GNT = 512;              // 8 waves
+constexpr int GWM = 4, GWN = 2;       // wave grid: 4 along m, 2 along n
+constexpr int GMI = GBM / GWM / 16;   // 16x16 blocks per wave along m (2)
+constexpr int GNJ = GBN / GWN / 16;   // 16x16 blocks per wave along n (4)
+constexpr int APAD = 16;              // [k][m] rows shifted by 32 banks
+constexpr int BPAD = 1;               // [n][k] rows of 17 elements
+constexpr int GRUN = GBM * GBK / GNT; // elements per thread per operand tile (4)
+static_assert(GRUN == 4, "tile loader assumes 4 elements per thread");
 
-// Loads one thread's 8-element slice of an A or B tile into registers.
-// NT selects which index is contiguous in memory for this operand.
-//  - "row run": 8 consecutive elements along the contiguous dimension.
-template <typename T>
-FLX_DEV void load_run8(T (&r)[8], const T *__restrict__ base, long stride_c,
-                       int i0, int j, int imax, int jmax, bool full, bool vec)
+// 4 consecutive elements base[(i0+q) + j*stride_c], q = 0..3, valid if
+// i0+q < imax and j < jmax (zero otherwise); FAST = in bounds and aligned.
+template <typename T, bool FAST>
+FLX_DEV void load_run4(T (&r)[4], const T *__restrict__ base, long stride_c, int i0, int j,
+                       int imax, int jmax)
 {
-    // elements base[(i0+q) + j*stride_c], q = 0..7, valid if i0+q<imax, j<jmax
     const T *p = base + (long)j * stride_c + i0;
-    if (full && vec) {
+    if (FAST) {
         if constexpr (sizeof(T) == 8) {
             const double2 *q = reinterpret_cast<const double2 *>(p);
-#pragma unroll
-            for (int u = 0; u < 4; ++u) {
-                double2 v = q[u];
-                r[2 * u] = v.x;
-                r[2 * u + 1] = v.y;
-            }
+            double2 v0 = q[0], v1 = q[1];
+            r[0] = v0.x; r[1] = v0.y; r[2] = v1.x; r[3] = v1.y;
         } else {
-            const float4 *q = reinterpret_cast<const float4 *>(p);
-#pragma unroll
-            for (int u = 0; u < 2; ++u) {
-                float4 v = q[u];
-                r[4 * u] = v.x; r[4 * u + 1] = v.y;
-                r[4 * u + 2] = v.z; r[4 * u + 3] = v.w;
-            }
+            float4 v = *reinterpret_cast<const float4 *>(p);
+            r[0] = v.x; r[1] = v.y; r[2] = v.z; r[3] = v.w;
         }
     } else {
 #pragma unroll
-        for (int q = 0; q < 8; ++q)
-            r[q] = (j < jmax && i0 + q < imax) ? p[q] : T(0);
+        for (int q = 0; q < 4; ++q) r[q] = (j < jmax && i0 + q < imax) ? p[q] : T(0);
+    }
+}
+
+// Per-thread slice of an A or B tile (4 consecutive elements along the
+// operand's contiguous dimension):
+//   A NoTrans: k = tid/32, m = (tid%32)*4      A Trans: m = tid/4, k = (tid%4)*4
+//   B NoTrans: n = tid/4,  k = (tid%4)*4       B Trans: k = tid/32, n = (tid%32)*4
+template <typename T, int TA, int TB, bool FAST>
+struct TileIO {
+    T ra[GRUN], rb[GRUN];
+
+    __device__ __forceinline__ void load(const T *__restrict__ A, long lda,
+                                         const T *__restrict__ B, long ldb, int M, int N,
+                                         int K, int m0, int n0, int k0, int tid)
+    {
+        if (!TA) load_run4<T, FAST>(ra, A + m0, lda, (tid & 31) * 4, k0 + (tid >> 5), M - m0, K);
+        else     load_run4<T, FAST>(ra, A + k0, lda, (tid & 3) * 4, m0 + (tid >> 2), K - k0, M);
+        if (!TB) load_run4<T, FAST>(rb, B + k0, ldb, (tid & 3) * 4, n0 + (tid >> 2), K - k0, N);
+        else     load_run4<T, FAST>(rb, B + n0, ldb, (tid & 31) * 4, k0 + (tid >> 5), N - n0, K);
+    }
+
+    __device__ __forceinline__ void store(T (*As)[GBM + APAD], T (*Bs)[GBK + BPAD], int tid)
+    {
+        if (!TA) {
+            const int kk = tid >> 5, mm = (tid & 31) * 4;
+#pragma unroll
+            for (int q = 0; q < 4; ++q) As[kk][mm + q] = ra[q];
+        } else {
+            const int mm = tid >> 2, kk = (tid & 3) * 4;
+#pragma unroll
+            for (int q = 0; q < 4; ++q) As[kk + q][mm] = ra[q];
+        }
+        if (!TB) {
+            const int nn = tid >> 2, kk = (tid & 3) * 4;
+#pragma unroll
+            for (int q = 0; q < 4; ++q) Bs[nn][kk + q] = rb[q];
+        } else {
+            const int kk = tid >> 5, nn = (tid & 31) * 4;
+#pragma unroll
+            for (int q = 0; q < 4; ++q) Bs[nn + q][kk] = rb[q];
+        }
+    }
+};
+
+// GMI x GNJ blocks of 16x16x4 MFMAs over one K tile held in LDS.
+template <typename T, typename acc_t>
+FLX_DEV void mma_tile(acc_t (&acc)[GMI][GNJ], const T (*As)[GBM + APAD],
+                      const T (*Bs)[GBK + BPAD], int wm, int wn, int lrow, int lk)
+{
+#pragma unroll
+    for (int s = 0; s < GBK / 4; ++s) {
+        T af[GMI], bf[GNJ];
+#pragma unroll
+        for (int i = 0; i < GMI; ++i) af[i] = As[s * 4 + lk][wm * (GMI * 16) + i * 16 + lrow];
+#pragma unroll
+        for (int j = 0; j < GNJ; ++j) bf[j] = Bs[wn * (GNJ * 16) + j * 16 + lrow][s * 4 + lk];
+#pragma unroll
+        for (int i = 0; i < GMI; ++i)
+#pragma unroll
+            for (int j = 0; j < GNJ; ++j) acc[i][j] = mfma16x16x4(af[i], bf[j], acc[i][j]);
+    }
+}
+
+// K loop: register prefetch of tile t+1 overlaps the MFMAs of tile t; LDS
+// double buffer; one barrier per tile.  FAST = interior block with 16-byte
+// aligned operands and K a multiple of the tile (no predicates at all).
+template <typename T, int TA, int TB, bool FAST, typename acc_t>
+FLX_DEV void gemm_mainloop(acc_t (&acc)[GMI][GNJ], T (*As)[GBK][GBM + APAD],
+                           T (*Bs)[GBN][GBK + BPAD], const T *__restrict__ A, long lda,
+                           const T *__restrict__ B, long ldb, int M, int N, int K, int m0,
+                           int n0, int nkt, int tid, int wm, int wn, int lrow, int lk)
+{
+    TileIO<T, TA, TB, FAST> io;
+    io.load(A, lda, B, ldb, M, N, K, m0, n0, 0, tid);
+    io.store(As[0], Bs[0], tid);
+    __syncthreads();
+    for (int kt = 0; kt < nkt; ++kt) {
+        const int buf = kt & 1;
+        const bool more = kt + 1 < nkt;
+        if (more) io.load(A, lda, B, ldb, M, N, K, m0, n0, (kt + 1) * GBK, tid);
+        mma_tile<T>(acc, As[buf], Bs[buf], wm, wn, lrow, lk);
+        if (more) {
+            io.store(As[buf ^ 1], Bs[buf ^ 1], tid);
+            __syncthreads();
+        }
     }
 }
 
 template <typename T, int TA, int TB>
-__global__ __launch_bounds__(256) void gemm_mfma_kernel(
+__global__ __launch_bounds__(GNT) void gemm_mfma_kernel(
     int M, int N, int K, T alpha, const T *__restrict__ A, long lda,
     const T *__restrict__ B, long ldb, T beta, T *__restrict__ C, long ldc,
     int vecA, int vecB, int blas_beta0)
@@ -94,108 +176,39 @@ __global__ __launch_bounds__(256) void gemm_mfma_kernel(
     const int tid = threadIdx.x;
     const int lane = tid & 63;
     const int wave = tid >> 6;
-    const int wm = wave & 1, wn = wave >> 1;
+    const int wm = wave % GWM, wn = wave / GWM;
     const int m0 = blockIdx.x * GBM;
     const int n0 = blockIdx.y * GBN;
+    const int lrow = lane & 15, lk = lane >> 4;
 
-    acc_t acc[4][4];
+    acc_t acc[GMI][GNJ];
 #pragma unroll
-    for (int i = 0; i < 4; ++i)
+    for (int i = 0; i < GMI; ++i)
 #pragma unroll
-        for (int j = 0; j < 4; ++j) acc[i][j] = acc_t{0, 0, 0, 0};
-
-    const bool mfull = (m0 + GBM <= M);
-    const bool nfull = (n0 + GBN <= N);
-
-    // per-thread slice coordinates
-    // A NoTrans: run along m (8 elems), one k column per thread
-    //   thread -> k = tid/16, m = (tid%16)*8
-    // A Trans  : run along k (8 elems), one m per thread
-    //   thread -> m = tid/2, k = (tid%2)*8
-    // B NoTrans: run along k, one n per thread: n = tid/2, k = (tid%2)*8
-    // B Trans  : run along n, one k per thread: k = tid/16, n = (tid%16)*8
-    T ra[8], rb[8];
-
-    auto load_tiles = [&](int k0) {
-        const bool kfull = (k0 + GBK <= K);
-        if (!TA) {
-            int kk = tid >> 4, mm = (tid & 15) * 8;
-            load_run8(ra, A + m0, lda, mm, k0 + kk, M - m0, K, mfull && kfull, vecA);
-        } else {
-            int mm = tid >> 1, kk = (tid & 1) * 8;
-            load_run8(ra, A + k0, lda, kk, m0 + mm, K - k0, M, mfull && kfull, vecA);
-        }
-        if (!TB) {
-            int nn = tid >> 1, kk = (tid & 1) * 8;
-            load_run8(rb, B + k0, ldb, kk, n0 + nn, K - k0, N, nfull && kfull, vecB);
-        } else {
-            int kk = tid >> 4, nn = (tid & 15) * 8;
-            load_run8(rb, B + n0, ldb, nn, k0 + kk, N - n0, K, nfull && kfull, vecB);
-        }
-    };
-    auto store_tiles = [&](int buf) {
-        if (!TA) {
-            int kk = tid >> 4, mm = (tid & 15) * 8;
-#pragma unroll
-            for (int q = 0; q < 8; ++q) As[buf][kk][mm + q] = ra[q];
-        } else {
-            int mm = tid >> 1, kk = (tid & 1) * 8;
-#pragma unroll
-            for (int q = 0; q < 8; ++q) As[buf][kk + q][mm] = ra[q];
-        }
-        if (!TB) {
-            int nn = tid >> 1, kk = (tid & 1) * 8;
-#pragma unroll
-            for (int q = 0; q < 8; ++q) Bs[buf][nn][kk + q] = rb[q];
-        } else {
-            int kk = tid >> 4, nn = (tid & 15) * 8;
-#pragma unroll
-            for (int q = 0; q < 8; ++q) Bs[buf][nn + q][kk] = rb[q];
-        }
-    };
+        for (int j = 0; j < GNJ; ++j) acc[i][j] = acc_t{0, 0, 0, 0};
 
     const int nkt = (alpha == T(0)) ? 0 : (K + GBK - 1) / GBK;
     if (nkt > 0) {
-        load_tiles(0);
-        store_tiles(0);
-        __syncthreads();
-    }
-    const int lrow = lane & 15, lk = lane >> 4;
-    for (int kt = 0; kt < nkt; ++kt) {
-        const int buf = kt & 1;
-        if (kt + 1 < nkt) load_tiles((kt + 1) * GBK);
-#pragma unroll
-        for (int s = 0; s < GBK / 4; ++s) {
-            T af[4], bf[4];
-#pragma unroll
-            for (int i = 0; i < 4; ++i)
-                af[i] = As[buf][s * 4 + lk][wm * 64 + i * 16 + lrow];
-#pragma unroll
-            for (int j = 0; j < 4; ++j)
-                bf[j] = Bs[buf][wn * 64 + j * 16 + lrow][s * 4 + lk];
-#pragma unroll
-            for (int i = 0; i < 4; ++i)
-#pragma unroll
-                for (int j = 0; j < 4; ++j)
-                    acc[i][j] = mfma16x16x4(af[i], bf[j], acc[i][j]);
-        }
-        if (kt + 1 < nkt) {
-            store_tiles(buf ^ 1);
-            __syncthreads();
-        }
+        const bool fast = (m0 + GBM <= M) && (n0 + GBN <= N) && (K % GBK == 0) && vecA && vecB;
+        if (fast)
+            gemm_mainloop<T, TA, TB, true>(acc, As, Bs, A, lda, B, ldb, M, N, K, m0, n0, nkt,
+                                           tid, wm, wn, lrow, lk);
+        else
+            gemm_mainloop<T, TA, TB, false>(acc, As, Bs, A, lda, B, ldb, M, N, K, m0, n0, nkt,
+                                            tid, wm, wn, lrow, lk);
     }
 
     // Epilogue: see MfmaAcc for which row each accumulator item holds.
 #pragma unroll
-    for (int i = 0; i < 4; ++i)
+    for (int i = 0; i < GMI; ++i)
 #pragma unroll
-        for (int j = 0; j < 4; ++j) {
-            const int col = n0 + wn * 64 + j * 16 + lrow;
+        for (int j = 0; j < GNJ; ++j) {
+            const int col = n0 + wn * (GNJ * 16) + j * 16 + lrow;
             if (col >= N) continue;
             T *cp = C + (long)col * ldc;
 #pragma unroll
             for (int q = 0; q < 4; ++q) {
-                const int row = m0 + wm * 64 + i * 16 + MfmaAcc<T>::row(lk, q);
+                const int row = m0 + wm * (GMI * 16) + i * 16 + MfmaAcc<T>::row(lk, q);
                 if (row < M) {
                     // generic FLENS order: C*beta first (0*C for beta=0);
                     // BLAS convention (blas_beta0): beta=0 never reads C.
@@ -207,19 +220,18 @@ __global__ __launch_bounds__(256) void gemm_mfma_kernel(
 }
 
 template <typename T>
-static hipError_t launch_gemm(hipStream_t s, int ta, int tb, int M, int N,
-                              int K, T alpha, const T *A, long lda, const T *B,
-                              long ldb, T beta, T *C, long ldc, int blas_beta0)
+static hipError_t launch_gemm(hipStream_t s, int ta, int tb, int M, int N, int K, T alpha,
+                              const T *A, long lda, const T *B, long ldb, T beta, T *C,
+                              long ldc, int blas_beta0)
 {
     if (M <= 0 || N <= 0) return hipSuccess;
     dim3 grid((M + GBM - 1) / GBM, (N + GBN - 1) / GBN);
     const uintptr_t al = 16;
     int vecA = ((uintptr_t)A % al == 0) && (lda * (long)sizeof(T)) % al == 0;
     int vecB = ((uintptr_t)B % al == 0) && (ldb * (long)sizeof(T)) % al == 0;
-#define FLX_LG(a_, b_)                                                        \
-    hipLaunchKernelGGL((gemm_mfma_kernel<T, a_, b_>), grid, dim3(256), 0, s,  \
-                       M, N, K, alpha, A, lda, B, ldb, beta, C, ldc, vecA, vecB, \
-                       blas_beta0)
+#define FLX_LG(a_, b_)                                                                   \
+    hipLaunchKernelGGL((gemm_mfma_kernel<T, a_, b_>), grid, dim3(GNT), 0, s, M, N, K,    \
+                       alpha, A, lda, B, ldb, beta, C, ldc, vecA, vecB, blas_beta0)
     if (!ta && !tb) FLX_LG(0, 0);
     else if (!ta && tb) FLX_LG(0, 1);
     else if (ta && !tb) FLX_LG(1, 0);
@@ -228,20 +240,20 @@ static hipError_t launch_gemm(hipStream_t s, int ta, int tb, int M, int N,
     return hipGetLastError();
 }
 
-hipError_t gemm_f64(hipStream_t s, int ta, int tb, int M, int N, int K,
-                    double alpha, const double *A, long lda, const double *B,
-                    long ldb, double beta, double *C, long ldc, int blas_beta0)
+hipError_t gemm_f64(hipStream_t s, int ta, int tb, int M, int N, int K, double alpha,
+                    const double *A, long lda, const double *B, long ldb, double beta,
+                    double *C, long ldc, int blas_beta0)
 {
-    return launch_gemm<double>(s, ta, tb, M, N, K, alpha, A, lda, B, ldb, beta,
-                               C, ldc, blas_beta0);
+    return launch_gemm<double>(s, ta, tb, M, N, K, alpha, A, lda, B, ldb, beta, C, ldc,
+                               blas_beta0);
 }
 
-hipError_t gemm_f32(hipStream_t s, int ta, int tb, int M, int N, int K,
-                    float alpha, const float *A, long lda, const float *B,
-                    long ldb, float beta, float *C, long ldc, int blas_beta0)
+hipError_t gemm_f32(hipStream_t s, int ta, int tb, int M, int N, int K, float alpha,
+                    const float *A, long lda, const float *B, long ldb, float beta, float *C,
+                    long ldc, int blas_beta0)
 {
-    return launch_gemm<float>(s, ta, tb, M, N, K, alpha, A, lda, B, ldb, beta,
-                              C, ldc, blas_beta0);
+    return launch_gemm<float>(s, ta, tb, M, N, K, alpha, A, lda, B, ldb, beta, C, ldc,
+                              blas_beta0);
 }
 
 } // namespace flx

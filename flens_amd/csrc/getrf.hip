@@ -39,249 +39,9 @@ namespace flx {
 // overwrite a slot after every workgroup has posted the next column, which
 // each does only after it finished reading the previous one.
 // ---------------------------------------------------------------------------
-constexpr int PANEL_T = 256;
-constexpr int PANEL_GMAX = 64;          // panel height <= 64*256 = 16384
+constexpr int PANEL_W = 32;             // panel width of the cooperative kernel
+constexpr int PANEL_ROWS_MAX = 16384;   // panel height limit
 constexpr int PANEL_WMAX = 64;
-constexpr uint32_t SPIN_LIMIT = 1u << 20;   // ~1 s of polling per word
-
-// words per parity: headers [GMAX][4], rows [GMAX][2*WMAX], diag [2*WMAX]
-constexpr size_t EX_HDR = 0;
-constexpr size_t EX_ROWS = PANEL_GMAX * 4;
-constexpr size_t EX_DIAG = EX_ROWS + (size_t)PANEL_GMAX * 2 * PANEL_WMAX;
-constexpr size_t EX_PAR = EX_DIAG + 2 * PANEL_WMAX;
-
-// Compile-time loop: the body sees its index as a constant, so arrays
-// indexed by it stay in registers.
-template <int I, int N, typename F>
-FLX_DEV void static_for(F &&f)
-{
-    if constexpr (I < N) {
-        f(std::integral_constant<int, I>{});
-        static_for<I + 1, N>(f);
-    }
-}
-
-size_t lu_exch_words() { return 2 * EX_PAR; }
-
-template <int W>
-__global__ __launch_bounds__(PANEL_T) void panel_kernel(
-    double *__restrict__ P, long ldw, int mp, int w, uint64_t *__restrict__ ex,
-    uint32_t gen, int *__restrict__ piv, int piv_base, int *__restrict__ list,
-    int *__restrict__ info, int info_base, int *__restrict__ err)
-{
-    const int G = gridDim.x;
-    const int g = blockIdx.x;
-    const int tid = threadIdx.x;
-    const int lane = tid & 63, wave = tid >> 6;
-    const int r = g * PANEL_T + tid;
-    const bool valid = r < mp;
-    const int mn = mp < w ? mp : w;
-
-    __shared__ double s_cand[PANEL_GMAX][W];
-    __shared__ double s_diag[W];
-    __shared__ double s_habs[PANEL_GMAX];
-    __shared__ int s_hidx[PANEL_GMAX];
-    __shared__ double s_wabs[4];
-    __shared__ int s_widx[4];
-    __shared__ int s_top[W];
-    __shared__ int s_orow[W];
-    __shared__ int s_osrc[W];
-    __shared__ int s_ocnt;
-    __shared__ int s_fail;
-
-    double row[W];
-#pragma unroll
-    for (int c = 0; c < W; ++c)
-        row[c] = (valid && c < w) ? P[(long)r * ldw + c] : 0.0;
-
-    if (g == 0)
-        for (int i = tid; i < W; i += PANEL_T) s_top[i] = i;
-    if (tid == 0) { s_ocnt = 0; s_fail = 0; }
-    int first_zero = 0;
-    __syncthreads();
-
-    static_for<0, W>([&](auto J) {
-        constexpr int j = decltype(J)::value;
-        if (j >= mn) return;
-        // ---- 1. local pivot candidate --------------------------------------
-        const bool act = valid && r >= j;
-        double a = act ? fabs(row[j]) : -1.0;
-        int ia = act ? r : INT_MAX;
-#pragma unroll
-        for (int o = 32; o > 0; o >>= 1) {
-            double a2 = __shfl_xor(a, o, 64);
-            int i2 = __shfl_xor(ia, o, 64);
-            if (cand_better(a2, i2, a, ia)) { a = a2; ia = i2; }
-        }
-        if (lane == 0) { s_wabs[wave] = a; s_widx[wave] = ia; }
-        __syncthreads();
-        double ba = s_wabs[0];
-        int bi = s_widx[0];
-#pragma unroll
-        for (int q = 1; q < 4; ++q)
-            if (cand_better(s_wabs[q], s_widx[q], ba, bi)) { ba = s_wabs[q]; bi = s_widx[q]; }
-        if (r == bi) {
-#pragma unroll
-            for (int c = 0; c < W; ++c) s_cand[g][c] = row[c];
-        }
-        if (r == j) {
-#pragma unroll
-            for (int c = 0; c < W; ++c) s_diag[c] = row[c];
-        }
-        __syncthreads();
-
-        // ---- 2./3. all-gather across workgroups ---------------------------
-        int win = bi, wg = g;
-        if (G > 1) {
-            const uint32_t seq = (gen << 8) | (uint32_t)(j + 1);
-            uint64_t *base = ex + (size_t)(j & 1) * EX_PAR;
-            const int diag_owner = j / PANEL_T;
-            if (tid < 2 * W) {
-                const int c = tid >> 1;
-                uint64_t b = (uint64_t)__double_as_longlong(s_cand[g][c]);
-                uint64_t half = (tid & 1) ? (b & 0xffffffffull) : (b >> 32);
-                st_agent(base + EX_ROWS + (size_t)g * 2 * PANEL_WMAX + tid,
-                         ((uint64_t)seq << 32) | half);
-                if (g == diag_owner) {
-                    uint64_t d = (uint64_t)__double_as_longlong(s_diag[c]);
-                    uint64_t dh = (tid & 1) ? (d & 0xffffffffull) : (d >> 32);
-                    st_agent(base + EX_DIAG + tid, ((uint64_t)seq << 32) | dh);
-                }
-            }
-            if (tid == 0) {
-                uint64_t b = (uint64_t)__double_as_longlong(ba);
-                st_agent(base + EX_HDR + g * 4 + 0, ((uint64_t)seq << 32) | (b >> 32));
-                st_agent(base + EX_HDR + g * 4 + 1, ((uint64_t)seq << 32) | (b & 0xffffffffull));
-                st_agent(base + EX_HDR + g * 4 + 2, ((uint64_t)seq << 32) | (uint32_t)bi);
-            }
-            // read every post: units = G headers + G*W row values + W diag
-            const int units = G + G * W + W;
-            for (int u = tid; u < units; u += PANEL_T) {
-                const uint64_t *p0;
-                int nw;
-                if (u < G) {
-                    p0 = base + EX_HDR + u * 4; nw = 3;
-                } else if (u < G + G * W) {
-                    int gg = (u - G) / W, c = (u - G) % W;
-                    if (gg == g) continue;
-                    p0 = base + EX_ROWS + (size_t)gg * 2 * PANEL_WMAX + 2 * c; nw = 2;
-                } else {
-                    if (g == diag_owner) continue;
-                    p0 = base + EX_DIAG + 2 * (u - G - G * W); nw = 2;
-                }
-                uint64_t v[3];
-                for (int q = 0; q < nw; ++q) {
-                    uint32_t spins = 0;
-                    uint64_t x = ld_agent(p0 + q);
-                    while (seq_of(x) != seq) {
-                        // give up for good once any wait of this workgroup failed
-                        if (*(volatile int *)&s_fail || ++spins > SPIN_LIMIT) { s_fail = 1; break; }
-                        __builtin_amdgcn_s_sleep(1);
-                        x = ld_agent(p0 + q);
-                    }
-                    v[q] = x;
-                }
-                if (u < G) {
-                    s_habs[u] = join_double(v[0], v[1]);
-                    s_hidx[u] = (int)(uint32_t)(v[2] & 0xffffffffull);
-                } else if (u < G + G * W) {
-                    int gg = (u - G) / W, c = (u - G) % W;
-                    s_cand[gg][c] = join_double(v[0], v[1]);
-                } else {
-                    s_diag[u - G - G * W] = join_double(v[0], v[1]);
-                }
-            }
-            __syncthreads();
-            double wa = s_habs[0];
-            win = s_hidx[0];
-            for (int q = 1; q < G; ++q)
-                if (cand_better(s_habs[q], s_hidx[q], wa, win)) { wa = s_habs[q]; win = s_hidx[q]; }
-            // a winner outside the panel can only come from a failed wait:
-            // keep every later index in bounds and report the failure
-            if (win < j || win >= mp) { win = j; s_fail = 1; }
-            wg = win / PANEL_T;
-        }
-
-        // ---- swap, scale, rank-1 update ------------------------------------
-        const double *u = s_cand[wg];
-        const double pv = u[j];
-        if (win != j) {
-            if (r == j) {
-#pragma unroll
-                for (int c = 0; c < W; ++c) row[c] = u[c];
-            } else if (r == win) {
-#pragma unroll
-                for (int c = 0; c < W; ++c) row[c] = s_diag[c];
-            }
-        }
-        if (valid && r > j) {
-            double l = row[j];
-            if (pv != 0.0) {
-                l = (fabs(pv) >= DBL_MIN) ? l * (1.0 / pv) : l / pv;
-                row[j] = l;
-            }
-#pragma unroll
-            for (int c = j + 1; c < W; ++c) row[c] = fma(-l, u[c], row[c]);
-        }
-        if (g == 0) {
-            if (tid == 0) {
-                piv[j] = piv_base + win;
-                if (pv == 0.0 && first_zero == 0) first_zero = j + 1;
-            }
-            if (wave == 1 && win != j) {
-                // track the composed permutation of this panel's swaps
-                if (win < w) {
-                    if (lane == 0) {
-                        int t = s_top[j]; s_top[j] = s_top[win]; s_top[win] = t;
-                    }
-                } else {
-                    const int cnt = s_ocnt;
-                    int found = -1;
-                    for (int q0 = 0; q0 < cnt; q0 += 64) {
-                        int q = q0 + lane;
-                        unsigned long long m = __ballot(q < cnt && s_orow[q] == win);
-                        if (m) { found = q0 + __ffsll(m) - 1; break; }
-                    }
-                    if (lane == 0) {
-                        if (found < 0) {
-                            found = cnt;
-                            s_orow[cnt] = win;
-                            s_osrc[cnt] = win;
-                            s_ocnt = cnt + 1;
-                        }
-                        int t = s_top[j]; s_top[j] = s_osrc[found]; s_osrc[found] = t;
-                    }
-                }
-            }
-        }
-        __syncthreads();
-    });
-
-    if (valid) {
-#pragma unroll
-        for (int c = 0; c < W; ++c)
-            if (c < w) P[(long)r * ldw + c] = row[c];
-    }
-    if (g == 0 && tid == 0) {
-        // list = [count, dst_0, src_0, dst_1, src_1, ...] in absolute rows
-        int cnt = 0;
-        for (int k = 0; k < mn; ++k)
-            if (s_top[k] != k) {
-                list[1 + 2 * cnt] = piv_base + k;
-                list[2 + 2 * cnt] = piv_base + s_top[k];
-                ++cnt;
-            }
-        for (int q = 0; q < s_ocnt; ++q)
-            if (s_osrc[q] != s_orow[q]) {
-                list[1 + 2 * cnt] = piv_base + s_orow[q];
-                list[2 + 2 * cnt] = piv_base + s_osrc[q];
-                ++cnt;
-            }
-        list[0] = cnt;
-        if (first_zero) atomicMin(info, info_base + first_zero);
-    }
-    if (tid == 0 && s_fail) atomicExch(err, 1);
-}
 
 // ---------------------------------------------------------------------------
 // Row permutation of a row-major matrix outside one panel:
@@ -294,29 +54,50 @@ __global__ __launch_bounds__(PANEL_T) void panel_kernel(
 constexpr int RP_CW = 64;
 constexpr int RP_MAXE = 2 * PANEL_WMAX;
 
-__global__ __launch_bounds__(256) void rowperm_kernel(double *__restrict__ Wm,
-                                                      long ldw, int m, int c1, int c2,
-                                                      int n,
-                                                      const int *__restrict__ list,
+// gathers the per-workgroup slices of a panel's interchange list into LDS
+FLX_DEV int load_list(const int *__restrict__ list, int G, int m, int *s_dst, int *s_src,
+                      int *s_off)
+{
+    const int t = threadIdx.x;
+    if (t < G) s_off[t] = min(list[(size_t)t * PANEL_LIST_SLICE], 64);
+    __syncthreads();
+    if (t == 0) {
+        int acc = 0;
+        for (int q = 0; q < G; ++q) { int c = s_off[q]; s_off[q] = acc; acc += c; }
+        s_off[PANEL_GROUPS_MAX] = min(acc, RP_MAXE);
+    }
+    __syncthreads();
+    if (t < G) {
+        const int *sl = list + (size_t)t * PANEL_LIST_SLICE;
+        const int c = min(sl[0], 64);
+        for (int e = 0; e < c; ++e) {
+            const int o = s_off[t] + e;
+            if (o >= RP_MAXE) break;
+            int d = sl[1 + 2 * e], sr = sl[2 + 2 * e];
+            bool ok = d >= 0 && d < m && sr >= 0 && sr < m;
+            s_dst[o] = ok ? d : 0;   // an invalid entry degenerates to row 0 -> row 0
+            s_src[o] = ok ? sr : 0;
+        }
+    }
+    __syncthreads();
+    return s_off[PANEL_GROUPS_MAX];
+}
+
+__global__ __launch_bounds__(256) void rowperm_kernel(double *__restrict__ Wm, long ldw, int m,
+                                                      int c1, int c2, int n,
+                                                      const int *__restrict__ list, int G,
                                                       int *__restrict__ perm,
                                                       unsigned long long *__restrict__ moved)
 {
     __shared__ double tile[RP_MAXE][RP_CW];
-    __shared__ int s_dst[RP_MAXE], s_src[RP_MAXE];
-    const int cnt = min(list[0], RP_MAXE);
+    __shared__ int s_dst[RP_MAXE], s_src[RP_MAXE], s_off[PANEL_GROUPS_MAX + 1];
+    const int cnt = load_list(list, G, m, s_dst, s_src, s_off);
     if (cnt <= 0) return;
     const int nleft = (c1 + RP_CW - 1) / RP_CW;
     int chunk = blockIdx.x;
     int cbeg, cend;
     if (chunk < nleft) { cbeg = chunk * RP_CW; cend = min(c1, cbeg + RP_CW); }
     else { cbeg = c2 + (chunk - nleft) * RP_CW; cend = min(n, cbeg + RP_CW); }
-    for (int e = threadIdx.x; e < cnt; e += 256) {
-        int d = list[1 + 2 * e], sr = list[2 + 2 * e];
-        bool ok = d >= 0 && d < m && sr >= 0 && sr < m;
-        s_dst[e] = ok ? d : 0;
-        s_src[e] = ok ? sr : 0;   // an invalid entry degenerates to row 0 -> row 0
-    }
-    __syncthreads();
     const int cw = cend - cbeg;
     if (moved && threadIdx.x == 0) atomicAdd(moved, (unsigned long long)cnt * cw);
     for (int q = threadIdx.x; q < cnt * RP_CW; q += 256) {
@@ -331,27 +112,23 @@ __global__ __launch_bounds__(256) void rowperm_kernel(double *__restrict__ Wm,
     // block 0 also composes the global row permutation
     if (blockIdx.x == 0 && perm) {
         __syncthreads();
-        int v[2] = {0, 0};
-        for (int e = threadIdx.x, k = 0; e < cnt; e += 256, ++k) v[k] = perm[s_src[e]];
+        int v = 0;
+        if (threadIdx.x < cnt) v = perm[s_src[threadIdx.x]];
         __syncthreads();
-        for (int e = threadIdx.x, k = 0; e < cnt; e += 256, ++k) perm[s_dst[e]] = v[k];
+        if (threadIdx.x < cnt) perm[s_dst[threadIdx.x]] = v;
     }
 }
 
 // permutation-only variant, used when no column lies outside the panel
-__global__ void perm_compose_kernel(const int *__restrict__ list, int *__restrict__ perm, int m)
+__global__ __launch_bounds__(256) void perm_compose_kernel(const int *__restrict__ list, int G,
+                                                           int *__restrict__ perm, int m)
 {
-    const int cnt = min(list[0], RP_MAXE);
-    int v[2] = {0, 0};
-    for (int e = threadIdx.x, k = 0; e < cnt; e += 256, ++k) {
-        int sr = list[2 + 2 * e];
-        v[k] = (sr >= 0 && sr < m) ? perm[sr] : 0;
-    }
+    __shared__ int s_dst[RP_MAXE], s_src[RP_MAXE], s_off[PANEL_GROUPS_MAX + 1];
+    const int cnt = load_list(list, G, m, s_dst, s_src, s_off);
+    int v = 0;
+    if (threadIdx.x < cnt) v = perm[s_src[threadIdx.x]];
     __syncthreads();
-    for (int e = threadIdx.x, k = 0; e < cnt; e += 256, ++k) {
-        int d = list[1 + 2 * e];
-        if (d >= 0 && d < m) perm[d] = v[k];
-    }
+    if (threadIdx.x < cnt) perm[s_dst[threadIdx.x]] = v;
 }
 
 // ---------------------------------------------------------------------------
@@ -362,44 +139,56 @@ __global__ void perm_compose_kernel(const int *__restrict__ list, int *__restric
 // ---------------------------------------------------------------------------
 constexpr int TR_CW = 32;
 constexpr int TR_KMAX = 256;
+constexpr int TR_BS = 16;
 
 __global__ __launch_bounds__(256) void trsm_rm_llu_kernel(const double *__restrict__ L,
                                                           long ldl, double *__restrict__ X,
                                                           long ldx, int k, int ncols)
 {
     __shared__ double xs[TR_KMAX][TR_CW + 1];
+    __shared__ double ls[TR_KMAX][TR_BS + 1];   // L(kb.., kb..kb+15) strip
     const int c0 = blockIdx.x * TR_CW;
     const int cw = min(TR_CW, ncols - c0);
     const int tid = threadIdx.x;
-    const int tc = tid % TR_CW, tg = tid / TR_CW;   // 8 row groups
+    const int tc = tid % TR_CW, tg = tid / TR_CW;   // 32 columns x 8 row groups
     for (int q = tid; q < k * TR_CW; q += 256) {
         int rr = q / TR_CW, c = q % TR_CW;
         xs[rr][c] = (c < cw) ? X[(long)rr * ldx + c0 + c] : 0.0;
     }
-    __syncthreads();
-    for (int kb = 0; kb < k; kb += 16) {
-        const int bs = min(16, k - kb);
-        // diagonal block: one thread per column, sequential rows
-        if (tid < TR_CW) {
-            for (int i = 1; i < bs; ++i) {
-                double s = xs[kb + i][tc];
-                const double *lr = L + (long)(kb + i) * ldl + kb;
-                for (int q = 0; q < i; ++q) s = fma(-lr[q], xs[kb + q][tc], s);
-                xs[kb + i][tc] = s;
-            }
+    for (int kb = 0; kb < k; kb += TR_BS) {
+        const int bs = min(TR_BS, k - kb);
+        // stage the strip of L below (and including) the diagonal block
+        for (int q = tid; q < (k - kb) * TR_BS; q += 256) {
+            int rr = q / TR_BS, cc = q % TR_BS;
+            ls[rr][cc] = (cc < bs) ? L[(long)(kb + rr) * ldl + kb + cc] : 0.0;
         }
         __syncthreads();
-        // rows below
-        double xb[16];
+        // diagonal block: column tc held in registers by threads tg == 0
+        if (tg == 0) {
+            double xb[TR_BS];
 #pragma unroll
-        for (int q = 0; q < 16; ++q) xb[q] = (q < bs) ? xs[kb + q][tc] : 0.0;
+            for (int i = 0; i < TR_BS; ++i) xb[i] = (i < bs) ? xs[kb + i][tc] : 0.0;
+#pragma unroll
+            for (int i = 1; i < TR_BS; ++i) {
+                double v = xb[i];
+#pragma unroll
+                for (int q = 0; q < i; ++q) v = fma(-ls[i][q], xb[q], v);
+                xb[i] = v;
+            }
+#pragma unroll
+            for (int i = 0; i < TR_BS; ++i)
+                if (i < bs) xs[kb + i][tc] = xb[i];
+        }
+        __syncthreads();
+        // fold the solved block into the rows below
+        double xb[TR_BS];
+#pragma unroll
+        for (int q = 0; q < TR_BS; ++q) xb[q] = xs[kb + q < k ? kb + q : kb][tc];
         for (int rr = kb + bs + tg; rr < k; rr += 8) {
-            const double *lr = L + (long)rr * ldl + kb;
-            double s = xs[rr][tc];
+            double v = xs[rr][tc];
 #pragma unroll
-            for (int q = 0; q < 16; ++q)
-                if (q < bs) s = fma(-lr[q], xb[q], s);
-            xs[rr][tc] = s;
+            for (int q = 0; q < TR_BS; ++q) v = fma(-ls[rr - kb][q], xb[q], v);
+            xs[rr][tc] = v;
         }
         __syncthreads();
     }
@@ -475,11 +264,6 @@ hipError_t piv0_to_ipiv(hipStream_t s, const int *piv0, int *ipiv, int n, int *i
     return hipGetLastError();
 }
 
-__global__ void zero_u64_kernel(uint64_t *p, size_t n)
-{
-    size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < n) p[i] = 0;
-}
 
 // ---------------------------------------------------------------------------
 // Host driver.
@@ -489,33 +273,6 @@ __global__ void zero_u64_kernel(uint64_t *p, size_t n)
         hipError_t e_ = (x);                                                  \
         if (e_ != hipSuccess) return e_;                                      \
     } while (0)
-
-static hipError_t launch_panel(hipStream_t s, LuWork &wk, double *P, long ldw,
-                               int mp, int w, int *piv, int piv_base, int *info,
-                               int ib)
-{
-    int G = (mp + PANEL_T - 1) / PANEL_T;
-    if (G > PANEL_GMAX) return hipErrorInvalidValue;
-    wk.gen = (wk.gen + 1) & 0xffffffu;
-    if (wk.gen == 0) {
-        // sequence space wrapped: clear the slots so no stale tag can match
-        hipLaunchKernelGGL(zero_u64_kernel, dim3((wk.exch_words + 255) / 256), dim3(256), 0, s,
-                           wk.exch, wk.exch_words);
-        wk.gen = 1;
-    }
-    // algorithmic panel traffic: the mp x w panel read and written once
-    const double pf = (double)w * w * mp - (double)w * w * w / 3.0;
-    ProfScope ps(wk.prof, s, PROF_PANEL, pf, 16.0 * mp * w);
-    if (ib <= 32)
-        hipLaunchKernelGGL(panel_kernel<32>, dim3(G), dim3(PANEL_T), 0, s, P, ldw, mp, w,
-                           wk.exch, wk.gen, piv, piv_base, wk.list, info, piv_base,
-                           wk.flags);
-    else
-        hipLaunchKernelGGL(panel_kernel<64>, dim3(G), dim3(PANEL_T), 0, s, P, ldw, mp, w,
-                           wk.exch, wk.gen, piv, piv_base, wk.list, info, piv_base,
-                           wk.flags);
-    return hipGetLastError();
-}
 
 static hipError_t launch_trsm_rm(hipStream_t s, Profiler *prof, const double *L, long ldl,
                                  double *X, long ldx, int k, int ncols)
@@ -546,12 +303,12 @@ hipError_t getrf_rm_f64(hipStream_t s, LuWork &wk, double *W, long ldw, int m, i
     const int mn = m < n ? m : n;
     if (m <= 0 || n <= 0) return hipSuccess;
     int nb = opt.nb, ib = opt.ib;
-    if (ib > PANEL_WMAX) ib = PANEL_WMAX;
-    if (ib < 1) ib = 32;
+    if (ib > PANEL_W) ib = PANEL_W;
+    if (ib < 1) ib = PANEL_W;
     if (nb < ib) nb = ib;
     if (nb > TR_KMAX) nb = TR_KMAX;
     nb = (nb / ib) * ib;
-    if (m > PANEL_GMAX * PANEL_T) return hipErrorInvalidValue;
+    if (m > PANEL_ROWS_MAX) return hipErrorInvalidValue;
     // the composed permutation, when requested, starts as the identity
     if (wk.perm) {
         hipLaunchKernelGGL(iota_kernel, dim3((m + 255) / 256), dim3(256), 0, s, wk.perm, m);
@@ -568,10 +325,11 @@ hipError_t getrf_rm_f64(hipStream_t s, LuWork &wk, double *W, long ldw, int m, i
             if (nleft + nright > 0) {
                 ProfScope ps(wk.prof, s, PROF_ROWPERM, 0.0, 0.0);
                 hipLaunchKernelGGL(rowperm_kernel, dim3(nleft + nright), dim3(256), 0, s, W,
-                                   ldw, m, j, j + w, n, wk.list, wk.perm, wk.moved);
+                                   ldw, m, j, j + w, n, wk.list, panel_groups(m - j), wk.perm,
+                                   wk.moved);
             } else if (wk.perm) {
                 hipLaunchKernelGGL(perm_compose_kernel, dim3(1), dim3(256), 0, s, wk.list,
-                                   wk.perm, m);
+                                   panel_groups(m - j), wk.perm, m);
             }
             FLX_CHECK(hipGetLastError());
             // inner update of the rest of the outer block

@@ -85,7 +85,7 @@ def test_known_answer_4x4(small):
 def test_exact_zero_and_tie_cases(small):
     sp = small["special"]
     for key in ("zero_col", "rank_def", "zero_row", "tie"):
-        a = np.asfortranarray(np.array(sp[key]["a"], float).reshape(-1, int(len(sp[key]["a"]) ** 0.5)).T)
+        a = np.array(np.array(sp[key]["a"], float).reshape(-1, int(len(sp[key]["a"]) ** 0.5)).T, order="F")
         piv, info = O.dgetrf(a)
         assert piv.tolist() == sp[key]["piv"] and info == sp[key]["info"], key
     # tie: |2| vs |-2| in column 1 -> the first (row 2 here since a11=1) wins
