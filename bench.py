@@ -46,15 +46,33 @@ def cpu_model() -> str:
     return platform.processor()
 
 
-def cpu_baseline(n: int, nrhs: int, nb: int = 64):
-    """Oracle (restatement of the reference's generic trf, 1 core) on the
-    first block step of the same factorisation; projected to solves/s."""
+def progress(msg: str):
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
+def cpu_baseline(n: int, nrhs: int, nb: int = 64, target_s: float = 15.0):
+    """Oracle (restatement of the reference's generic trf, 1 core) on a
+    bounded sample of the same factorisation: the first nb-wide block step
+    (panel, interchanges, trsm, trailing GEMM) restricted to the first c
+    columns of the n x n matrix, leading dimension n.  In a right-looking
+    step each trailing column depends only on the panel, so this is that
+    step's work for c columns at the workload's memory layout.  c doubles
+    until the sample takes about target_s; the rate is projected to the
+    full trf+trs flop count."""
     from oracle import oracle as O
-    a = O.uniform(n, n, 2)
-    fl = O.getrf_steps_flops(n, n, nb, 1)
-    t = time.perf_counter()
-    O.dgetrf_steps(a, nb, 1)
-    dt = time.perf_counter() - t
+    a_full = O.uniform(n, n, 2)
+    c, fl, dt = nb * 2, 0.0, 0.0
+    while True:
+        a = a_full[:, :c]          # column slice of a Fortran array: same lda = n
+        fl = O.getrf_steps_flops(n, c, nb, 1)
+        t = time.perf_counter()
+        O.dgetrf_steps(a, nb, 1)
+        dt = time.perf_counter() - t
+        progress(f"CPU sample: {c} columns, {fl / 1e9:.2f} GFLOP in {dt:.1f} s")
+        if dt * 2.2 > target_s or c * 2 > n:
+            break
+        c *= 2
+        a_full = O.uniform(n, n, 2)
     rate = fl / dt
     total = lu_flops(n, nrhs)
     return {
@@ -63,9 +81,10 @@ def cpu_baseline(n: int, nrhs: int, nb: int = 64):
         "cores": 1,
         "kind": "port",
         "sample": (f"first nb={nb} block step of the n={n} getrf (panel + laswp + trsm + "
-                   f"trailing GEMM, {fl / 1e9:.2f} GFLOP) in {dt:.1f} s = {rate / 1e9:.3f} GFLOP/s "
-                   f"on 1 core of {cpu_model()} ({os.cpu_count()} visible); projected to the "
-                   f"{total / 1e9:.0f} GFLOP trf+trs workload"),
+                   f"trailing GEMM) on the first {c} columns, lda={n}: {fl / 1e9:.2f} GFLOP in "
+                   f"{dt:.1f} s = {rate / 1e9:.3f} GFLOP/s on 1 core of {cpu_model()} "
+                   f"({os.cpu_count()} visible); projected to the {total / 1e9:.0f} GFLOP "
+                   f"trf+trs workload"),
     }
 
 
@@ -108,8 +127,10 @@ def main():
         if info != 0:
             raise RuntimeError(f"dgesv info={info}")
 
+    progress(f"inputs resident (n={n}, nrhs={nrhs}); warmup {args.warmup} steps")
     for _ in range(args.warmup):
         step()
+    progress(f"timing {args.steps} steps")
     torch.cuda.synchronize()
     h.profile_start()
     t0 = time.perf_counter()
@@ -119,6 +140,7 @@ def main():
     t1 = time.perf_counter()
     prof = h.profile_stop()
     h.sync()
+    progress(f"timed region done: {1e3 * (t1 - t0) / args.steps:.2f} ms per step; checking the solution")
     elapsed = t1 - t0
     ms = 1e3 * elapsed / args.steps
 
@@ -166,6 +188,7 @@ def main():
         "breakdown": breakdown,
     }
     if not args.no_cpu:
+        progress("CPU baseline: oracle getrf, first block step, 1 core")
         out["cpu_baseline"] = cpu_baseline(n, nrhs)
     print(json.dumps(out))
 

@@ -23,7 +23,7 @@ struct flx_context {
     int device = 0;
     LuWork wk;
     LuOptions opt;
-    DevBuf W, A, B, C, X, piv0, ipiv, info, perm;
+    DevBuf W, A, B, C, X, piv0, ipiv, info, perm, tinv;
     int last_error = 0;
     Profiler *prof = nullptr;   // non-null while profiling
 };
@@ -109,6 +109,7 @@ extern "C" int flx_destroy(flx_handle h)
     hipStreamSynchronize(h->stream);
     free_buf(h->W); free_buf(h->A); free_buf(h->B); free_buf(h->C); free_buf(h->X);
     free_buf(h->piv0); free_buf(h->ipiv); free_buf(h->info); free_buf(h->perm);
+    free_buf(h->tinv);
     if (h->wk.exch) hipFree(h->wk.exch);
     if (h->wk.list) hipFree(h->wk.list);
     if (h->wk.flags) hipFree(h->wk.flags);
@@ -254,7 +255,9 @@ static int dgetrs_dev(flx_context *h, int trans, int n, int nrhs, const double *
     const size_t bytes = sizeof(double) * (size_t)n * (size_t)nrhs;
     FLX_TRY(ensure(h, h->B, bytes));
     FLX_TRY(ensure(h, h->X, bytes));
+    FLX_TRY(ensure(h, h->tinv, sizeof(double) * trs_inv_doubles(n)));
     double *T1 = (double *)h->B.p, *T2 = (double *)h->X.p;
+    double *ti = (double *)h->tinv.p;
     if (!trans) {
         const int *perm = perm_fwd;
         if (!perm) {
@@ -263,12 +266,12 @@ static int dgetrs_dev(flx_context *h, int trans, int n, int nrhs, const double *
             perm = (const int *)h->perm.p;
         }
         FLX_TRY(gather_rows_cm(h->stream, B, ldb, T1, n, perm, n, nrhs));
-        FLX_TRY(trsm_cm_f64(h->stream, 0, 0, 1, n, nrhs, A, lda, T1, n, T2, n));
-        FLX_TRY(trsm_cm_f64(h->stream, 1, 0, 0, n, nrhs, A, lda, T2, n, B, ldb));
+        FLX_TRY(trsm_cm_f64(h->stream, 0, 0, 1, n, nrhs, A, lda, T1, n, T2, n, ti));
+        FLX_TRY(trsm_cm_f64(h->stream, 1, 0, 0, n, nrhs, A, lda, T2, n, B, ldb, ti));
     } else {
         FLX_TRY(copy_cm(h->stream, B, ldb, T1, n, n, nrhs));
-        FLX_TRY(trsm_cm_f64(h->stream, 1, 1, 0, n, nrhs, A, lda, T1, n, T2, n));
-        FLX_TRY(trsm_cm_f64(h->stream, 0, 1, 1, n, nrhs, A, lda, T2, n, T1, n));
+        FLX_TRY(trsm_cm_f64(h->stream, 1, 1, 0, n, nrhs, A, lda, T1, n, T2, n, ti));
+        FLX_TRY(trsm_cm_f64(h->stream, 0, 1, 1, n, nrhs, A, lda, T2, n, T1, n, ti));
         // apply the interchanges last to first: X = P^T Y
         FLX_TRY(ensure(h, h->perm, sizeof(int) * (size_t)n));
         FLX_TRY(ipiv_to_perm(h->stream, ipiv, (int *)h->perm.p, n, 1));
@@ -428,6 +431,8 @@ static int host_trsm(int right, int upper, int trans, int unit, int M, int N, do
     FLX_TRY(ensure(h, h->A, sizeof(double) * (size_t)na * na));
     FLX_TRY(ensure(h, h->C, sizeof(double) * (size_t)M * N));
     FLX_TRY(ensure(h, h->W, sizeof(double) * (size_t)M * N));
+    FLX_TRY(ensure(h, h->tinv, sizeof(double) * trs_inv_doubles(na)));
+    double *ti = (double *)h->tinv.p;
     FLX_TRY(h2d(h->A.p, A, na, na, lda));
     FLX_TRY(h2d(h->C.p, B, M, N, ldb));
     double *dB = (double *)h->C.p, *dT = (double *)h->W.p;
@@ -437,12 +442,12 @@ static int host_trsm(int right, int upper, int trans, int unit, int M, int N, do
         // X = op(A)^{-1} B : solve consumes dT copy, writes dB
         FLX_TRY(copy_cm(h->stream, dB, M, dT, M, M, N));
         FLX_TRY(trsm_cm_f64(h->stream, upper, trans, unit, M, N, (const double *)h->A.p, na,
-                            dT, M, dB, M));
+                            dT, M, dB, M, ti));
     } else {
         // X op(A) = B  <=>  op(A)^T X^T = B^T
         FLX_TRY(transpose_f64(h->stream, dB, M, dT, N, M, N));       // dT = B^T (N x M)
         FLX_TRY(trsm_cm_f64(h->stream, upper, !trans, unit, N, M, (const double *)h->A.p, na,
-                            dT, N, dB, N));                           // dB = X^T (N x M)
+                            dT, N, dB, N, ti));                       // dB = X^T (N x M)
         FLX_TRY(transpose_f64(h->stream, dB, N, dT, M, N, M));       // dT = X (M x N)
         FLX_TRY(copy_cm(h->stream, dT, M, dB, M, M, N));
     }

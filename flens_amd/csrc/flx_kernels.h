@@ -99,10 +99,12 @@ hipError_t gather_rows_cm(hipStream_t s, const double *B, long ldb,
 hipError_t copy_cm(hipStream_t s, const double *in, long ldi, double *out,
                    long ldo, int m, int n);
 // Triangular solve with the factors in LU; B (n x nrhs) is consumed (its
-// rows are updated in place), the solution is written to X.
+// rows are updated in place), the solution is written to X.  inv is device
+// scratch of trs_inv_doubles(n) doubles (inverted diagonal blocks).
+size_t trs_inv_doubles(int n);
 hipError_t trsm_cm_f64(hipStream_t s, int uplo_upper, int trans, int unit,
                        int n, int nrhs, const double *LU, long lda, double *B,
-                       long ldb, double *X, long ldx);
+                       long ldb, double *X, long ldx, double *inv);
 hipError_t laswp_cm_f64(hipStream_t s, double *A, long lda, int ncols, int k1,
                         int k2, const int *ipiv, int incx);
 

@@ -97,106 +97,170 @@ hipError_t copy_cm(hipStream_t s, const double *in, long ldi, double *out, long 
 //   BWD (backward order, T upper): U (non-unit, NoTrans) or L^T (unit, Trans)
 // TR selects the element address: NoTrans T(i,q) = LU[i + q*lda],
 // Trans T(i,q) = LU[q + i*lda].
-// One launch per 64-row diagonal block [kb, kb+bs): every workgroup solves
-// the block redundantly in LDS (right-looking, bs barriers), block 0 writes
-// the solution rows to X, and every workgroup updates its 256 trailing rows
-// of B in place.  The solution goes to a separate array so no workgroup can
-// read a block row another one already overwrote.
+//
+// 1. tri_inv_kernel inverts every 64x64 diagonal block of T at once (one
+//    workgroup per block, one column of the inverse per thread, registers).
+// 2. One launch per diagonal block: X_k = inv(T_kk) * B_k (no sequential
+//    chain), block 0 stores X_k, and every workgroup folds X_k into its 256
+//    trailing rows of B (coalesced reads of the factor's columns).
+// The solution goes to a separate array so no workgroup can read a block
+// row another one already overwrote.
 // ---------------------------------------------------------------------------
 constexpr int TS_BS = 64;
 constexpr int TS_CG = 16;   // right-hand sides per pass
 
 template <int TR, int UNIT, int FWD>
+__global__ __launch_bounds__(64) void tri_inv_kernel(const double *__restrict__ LU, long lda,
+                                                     int n, double *__restrict__ inv)
+{
+    __shared__ double ts[TS_BS][TS_BS + 1];
+    const int b = blockIdx.x, kb = b * TS_BS, bs = min(TS_BS, n - kb);
+    const int t = threadIdx.x;
+    // T(i,q) for the block, identity outside the matrix
+    for (int i = 0; i < TS_BS; ++i) {
+        double v;
+        if (i < bs && t < bs) v = TR ? LU[(long)(kb + i) * lda + kb + t] : LU[(long)(kb + t) * lda + kb + i];
+        else v = (i == t) ? 1.0 : 0.0;
+        ts[i][t] = v;
+    }
+    __syncthreads();
+    // column t of inv(T): T x = e_t
+    double x[TS_BS];
+#pragma unroll
+    for (int s = 0; s < TS_BS; ++s) {
+        const int i = FWD ? s : TS_BS - 1 - s;
+        double v = (i == t) ? 1.0 : 0.0;
+        if (FWD) {
+#pragma unroll
+            for (int q = 0; q < i; ++q) v = fma(-ts[i][q], x[q], v);
+        } else {
+#pragma unroll
+            for (int q = i + 1; q < TS_BS; ++q) v = fma(-ts[i][q], x[q], v);
+        }
+        x[i] = UNIT ? v : v / ts[i][i];
+    }
+    double *o = inv + (size_t)b * TS_BS * TS_BS;   // row-major 64x64
+#pragma unroll
+    for (int i = 0; i < TS_BS; ++i) o[i * TS_BS + t] = x[i];
+}
+
+template <int TR, int FWD>
 __global__ __launch_bounds__(256) void trs_block_kernel(const double *__restrict__ LU, long lda,
+                                                        const double *__restrict__ inv,
                                                         double *__restrict__ B, long ldb,
                                                         double *__restrict__ X, long ldx,
                                                         int n, int nrhs, int kb, int bs)
 {
     __shared__ double xs[TS_BS][TS_CG + 1];
+    __shared__ double is[TS_BS][TS_BS + 1];   // inv(T_kk)
+    __shared__ double ys[TS_BS][TS_CG + 1];   // X_k
     const int tid = threadIdx.x;
+    // thread -> (row ri of 64, right-hand sides 4*cq .. 4*cq+3): a wave
+    // covers 64 consecutive rows, so factor-column reads coalesce
+    const int ri = tid & 63, cq = tid >> 6;
     auto T = [&](int i, int q) -> double {
         return TR ? LU[(long)i * lda + q] : LU[(long)q * lda + i];
     };
+    const double *ib = inv + (size_t)(kb / TS_BS) * TS_BS * TS_BS;
+    for (int q = tid; q < TS_BS * TS_BS; q += 256) is[q / TS_BS][q % TS_BS] = ib[q];
     // trailing rows: FWD -> [kb+bs, n), BWD -> [0, kb)
     const int t0 = FWD ? kb + bs : 0;
     const int tn = FWD ? n - (kb + bs) : kb;
-    const int r = t0 + blockIdx.x * 256 + tid;
-    const bool has_row = (blockIdx.x * 256 + tid) < tn;
+    const int r = t0 + blockIdx.x * 64 + ri;
+    const bool has_row = (blockIdx.x * 64 + ri) < tn;
 
     for (int cg = 0; cg < nrhs; cg += TS_CG) {
         const int nc = min(TS_CG, nrhs - cg);
-        for (int q = tid; q < bs * TS_CG; q += 256) {
+        for (int q = tid; q < TS_BS * TS_CG; q += 256) {
             int i = q / TS_CG, c = q % TS_CG;
-            xs[i][c] = (c < nc) ? B[(long)(cg + c) * ldb + kb + i] : 0.0;
+            xs[i][c] = (c < nc && i < bs) ? B[(long)(cg + c) * ldb + kb + i] : 0.0;
         }
         __syncthreads();
-        // solve the diagonal block
-        for (int s = 0; s < bs; ++s) {
-            const int i = FWD ? s : bs - 1 - s;
-            if (!UNIT) {
-                if (tid < TS_CG) xs[i][tid] = xs[i][tid] / T(kb + i, kb + i);
-                __syncthreads();
+        // X_k = inv(T_kk) B_k
+        {
+            double acc[4] = {0, 0, 0, 0};
+#pragma unroll 8
+            for (int q = 0; q < TS_BS; ++q) {
+                const double tv = is[ri][q];
+#pragma unroll
+                for (int c = 0; c < 4; ++c) acc[c] = fma(tv, xs[q][4 * cq + c], acc[c]);
             }
-            // fold x_i into the remaining rows of the block
-            const int rem = FWD ? bs - 1 - i : i;
-            for (int q = tid; q < rem * TS_CG; q += 256) {
-                int ii = (FWD ? i + 1 : 0) + q / TS_CG, c = q % TS_CG;
-                xs[ii][c] = fma(-T(kb + ii, kb + i), xs[i][c], xs[ii][c]);
-            }
-            __syncthreads();
+#pragma unroll
+            for (int c = 0; c < 4; ++c) ys[ri][4 * cq + c] = acc[c];
         }
+        __syncthreads();
         if (blockIdx.x == 0)
             for (int q = tid; q < bs * TS_CG; q += 256) {
                 int i = q / TS_CG, c = q % TS_CG;
-                if (c < nc) X[(long)(cg + c) * ldx + kb + i] = xs[i][c];
+                if (c < nc) X[(long)(cg + c) * ldx + kb + i] = ys[i][c];
             }
         if (has_row) {
-            double acc[TS_CG];
+            double acc[4];
 #pragma unroll
-            for (int c = 0; c < TS_CG; ++c)
-                acc[c] = (c < nc) ? B[(long)(cg + c) * ldb + r] : 0.0;
-            for (int q = 0; q < bs; ++q) {
-                const double t = T(r, kb + q);
+            for (int c = 0; c < 4; ++c)
+                acc[c] = (4 * cq + c < nc) ? B[(long)(cg + 4 * cq + c) * ldb + r] : 0.0;
+            if (bs == TS_BS) {
+#pragma unroll 16
+                for (int q = 0; q < TS_BS; ++q) {
+                    const double t = T(r, kb + q);
 #pragma unroll
-                for (int c = 0; c < TS_CG; ++c) acc[c] = fma(-t, xs[q][c], acc[c]);
+                    for (int c = 0; c < 4; ++c) acc[c] = fma(-t, ys[q][4 * cq + c], acc[c]);
+                }
+            } else {
+                for (int q = 0; q < bs; ++q) {
+                    const double t = T(r, kb + q);
+#pragma unroll
+                    for (int c = 0; c < 4; ++c) acc[c] = fma(-t, ys[q][4 * cq + c], acc[c]);
+                }
             }
 #pragma unroll
-            for (int c = 0; c < TS_CG; ++c)
-                if (c < nc) B[(long)(cg + c) * ldb + r] = acc[c];
+            for (int c = 0; c < 4; ++c)
+                if (4 * cq + c < nc) B[(long)(cg + 4 * cq + c) * ldb + r] = acc[c];
         }
         __syncthreads();
     }
 }
 
+size_t trs_inv_doubles(int n) { return (size_t)((n + TS_BS - 1) / TS_BS) * TS_BS * TS_BS; }
+
 hipError_t trsm_cm_f64(hipStream_t s, int upper, int trans, int unit, int n, int nrhs,
                        const double *LU, long lda, double *B, long ldb, double *X,
-                       long ldx)
+                       long ldx, double *inv)
 {
     if (n <= 0 || nrhs <= 0) return hipSuccess;
     // forward order when the operand is lower triangular
     const int fwd = (!upper && !trans) || (upper && trans);
     const int nblk = (n + TS_BS - 1) / TS_BS;
+    {
+        ProfScope ps(tl_prof, s, PROF_TRS, (double)nblk * TS_BS * TS_BS * TS_BS / 3.0,
+                     8.0 * nblk * TS_BS * TS_BS * 1.5);
+#define FLX_TI(tr_, un_, fw_)                                                                \
+    hipLaunchKernelGGL((tri_inv_kernel<tr_, un_, fw_>), dim3(nblk), dim3(64), 0, s, LU, lda, n, inv)
+        if (!trans && unit && fwd) FLX_TI(0, 1, 1);
+        else if (!trans && !unit && !fwd) FLX_TI(0, 0, 0);
+        else if (trans && !unit && fwd) FLX_TI(1, 0, 1);
+        else if (trans && unit && !fwd) FLX_TI(1, 1, 0);
+        else if (!trans && unit && !fwd) FLX_TI(0, 1, 0);
+        else if (!trans && !unit && fwd) FLX_TI(0, 0, 1);
+        else if (trans && unit && fwd) FLX_TI(1, 1, 1);
+        else FLX_TI(1, 0, 0);
+#undef FLX_TI
+        hipError_t e = hipGetLastError();
+        if (e != hipSuccess) return e;
+    }
     for (int b = 0; b < nblk; ++b) {
         const int blk = fwd ? b : nblk - 1 - b;
         const int kb = blk * TS_BS;
         const int bs = min(TS_BS, n - kb);
         const int tn = fwd ? n - (kb + bs) : kb;
-        const int grid = tn > 0 ? (tn + 255) / 256 : 1;
-        // solve of the block + fold into tn rows: factor block and rows read once
-        ProfScope ps(tl_prof, s, PROF_TRS, (double)nrhs * (bs * bs + 2.0 * bs * tn),
-                     8.0 * ((double)bs * bs / 2 + (double)bs * tn + 2.0 * nrhs * (bs + tn)));
-#define FLX_TS(tr_, un_, fw_)                                                  \
-    hipLaunchKernelGGL((trs_block_kernel<tr_, un_, fw_>), dim3(grid), dim3(256), 0, s, LU, \
-                       lda, B, ldb, X, ldx, n, nrhs, kb, bs)
-        if (!trans && unit && fwd) FLX_TS(0, 1, 1);
-        else if (!trans && !unit && !fwd) FLX_TS(0, 0, 0);
-        else if (trans && !unit && fwd) FLX_TS(1, 0, 1);
-        else if (trans && unit && !fwd) FLX_TS(1, 1, 0);
-        else if (!trans && unit && !fwd) FLX_TS(0, 1, 0);
-        else if (!trans && !unit && fwd) FLX_TS(0, 0, 1);
-        else if (trans && unit && fwd) FLX_TS(1, 1, 1);
-        else FLX_TS(1, 0, 0);
-#undef FLX_TS
+        const int grid = tn > 0 ? (tn + 63) / 64 : 1;
+        // block product + fold into tn rows: factor block and rows read once
+        ProfScope ps(tl_prof, s, PROF_TRS, (double)nrhs * (2.0 * bs * bs + 2.0 * bs * tn),
+                     8.0 * ((double)bs * bs + (double)bs * tn + 2.0 * nrhs * (bs + tn)));
+        if (!trans && fwd) hipLaunchKernelGGL((trs_block_kernel<0, 1>), dim3(grid), dim3(256), 0, s, LU, lda, inv, B, ldb, X, ldx, n, nrhs, kb, bs);
+        else if (!trans) hipLaunchKernelGGL((trs_block_kernel<0, 0>), dim3(grid), dim3(256), 0, s, LU, lda, inv, B, ldb, X, ldx, n, nrhs, kb, bs);
+        else if (fwd) hipLaunchKernelGGL((trs_block_kernel<1, 1>), dim3(grid), dim3(256), 0, s, LU, lda, inv, B, ldb, X, ldx, n, nrhs, kb, bs);
+        else hipLaunchKernelGGL((trs_block_kernel<1, 0>), dim3(grid), dim3(256), 0, s, LU, lda, inv, B, ldb, X, ldx, n, nrhs, kb, bs);
         hipError_t e = hipGetLastError();
         if (e != hipSuccess) return e;
     }
