@@ -2,6 +2,7 @@
 // entry points and the FLENS plugin seams (CBLAS + Fortran LAPACK symbols).
 #include <climits>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <mutex>
 #include <new>
@@ -347,10 +348,16 @@ static flx_context *default_handle()
     return h;
 }
 
+// The Fortran/CBLAS seams have no error channel for device failures (INFO
+// > 0 means a singular factor); returning with INFO = 0 would hand the caller
+// unfactored data.  A device failure therefore ends the program the way the
+// reference's failed ASSERT does (flens/auxiliary/macros.h:63-66).
 static void report(const char *who, int status)
 {
-    if (status != FLX_SUCCESS)
+    if (status != FLX_SUCCESS) {
         fprintf(stderr, "libflx: %s failed: %s\n", who, flx_status_string(status));
+        abort();
+    }
 }
 
 // host (ld) <-> device (compact, ld = rows) copies

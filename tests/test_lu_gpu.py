@@ -206,3 +206,16 @@ def test_lu8192_vs_reference_fixture(gpu, nb):
     a1 = a0.abs().sum(0).max().item()
     ratio = r1 / (n * a1 * 2.0 ** -53)
     assert ratio < 30, ratio
+
+
+def test_flens_program_runs_on_libflx(gpu):
+    """The FLENS program of tests/dropin (FLENS API only, external
+    BLAS/LAPACK switches, linked to libflx) runs its mm / trf / trs / sv on
+    the GPU and passes its own checks (KAT pivots, GEMM bound, dget02)."""
+    import subprocess
+    exe = os.path.join(os.path.dirname(GOLD), "..", "oracle", "_ref", "flens_dropin")
+    if not os.path.exists(exe):
+        pytest.skip("drop-in program not built (needs the reference headers)")
+    r = subprocess.run([exe], capture_output=True, text=True, timeout=120)
+    print(r.stdout)
+    assert r.returncode == 0 and "DROPIN PASSED" in r.stdout, r.stdout + r.stderr
