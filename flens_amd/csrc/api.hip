@@ -6,36 +6,11 @@
 #include <cstring>
 #include <mutex>
 #include <new>
-#include "../../include/flx.h"
-#include "flx_kernels.h"
+#include "context.h"
 
 using namespace flx;
 
-// ---------------------------------------------------------------------------
-// handle
-// ---------------------------------------------------------------------------
-struct DevBuf {
-    void *p = nullptr;
-    size_t cap = 0;
-};
-
-struct flx_context {
-    hipStream_t stream = nullptr;
-    int device = 0;
-    LuWork wk;
-    LuOptions opt;
-    DevBuf W, A, B, C, X, piv0, ipiv, info, perm, tinv;
-    int last_error = 0;
-    Profiler *prof = nullptr;   // non-null while profiling
-};
-
-// binds the calling handle's profiler for the duration of one entry point
-struct ProfBind {
-    ProfBind(flx_context *h) { tl_prof = h ? h->prof : nullptr; if (h) h->wk.prof = h->prof; }
-    ~ProfBind() { tl_prof = nullptr; }
-};
-
-static int hip_status(hipError_t e)
+int hip_status(hipError_t e)
 {
     if (e == hipSuccess) return FLX_SUCCESS;
     if (e == hipErrorOutOfMemory) return FLX_ALLOC_FAILED;
@@ -43,14 +18,7 @@ static int hip_status(hipError_t e)
     return FLX_HIP_ERROR;
 }
 
-#define FLX_TRY(x)                                                            \
-    do {                                                                      \
-        hipError_t e_ = (x);                                                  \
-        if (e_ != hipSuccess) return hip_status(e_);                          \
-    } while (0)
-
-// grow-only device buffer; waits for queued work before releasing memory
-static hipError_t ensure(flx_context *h, DevBuf &b, size_t bytes)
+hipError_t ensure(flx_context *h, DevBuf &b, size_t bytes)
 {
     if (b.cap >= bytes) return hipSuccess;
     if (b.p) {
@@ -67,7 +35,15 @@ static hipError_t ensure(flx_context *h, DevBuf &b, size_t bytes)
     return hipSuccess;
 }
 
-static hipError_t init_work(flx_context *h)
+hipError_t ensure_zeroed(flx_context *h, DevBuf &b, size_t bytes)
+{
+    if (b.cap >= bytes) return hipSuccess;
+    hipError_t e = ensure(h, b, bytes);
+    if (e != hipSuccess) return e;
+    return hipMemsetAsync(b.p, 0, b.cap, h->stream);
+}
+
+hipError_t init_work(flx_context *h)
 {
     if (h->wk.exch) return hipSuccess;
     const size_t words = lu_exch_words();
@@ -97,7 +73,7 @@ extern "C" int flx_create(flx_handle *out)
     return FLX_SUCCESS;
 }
 
-static void free_buf(DevBuf &b)
+void free_buf(DevBuf &b)
 {
     if (b.p) hipFree(b.p);
     b.p = nullptr;
@@ -110,7 +86,13 @@ extern "C" int flx_destroy(flx_handle h)
     hipStreamSynchronize(h->stream);
     free_buf(h->W); free_buf(h->A); free_buf(h->B); free_buf(h->C); free_buf(h->X);
     free_buf(h->piv0); free_buf(h->ipiv); free_buf(h->info); free_buf(h->perm);
-    free_buf(h->tinv);
+    free_buf(h->tinv); free_buf(h->sb); free_buf(h->sx);
+    MixBufs &x = h->mix;
+    for (DevBuf *b : {&x.W32, &x.perm, &x.cand, &x.ccount, &x.counters, &x.u11, &x.win, &x.list,
+                      &x.zero, &x.lpmax, &x.la, &x.ub, &x.inv_l, &x.inv_u, &x.y32, &x.flags,
+                      &x.err, &x.dr, &x.dc, &x.part, &x.scal, &x.R, &x.V, &x.w, &x.t, &x.zeros,
+                      &x.mpart, &x.hbuf})
+        free_buf(*b);
     if (h->wk.exch) hipFree(h->wk.exch);
     if (h->wk.list) hipFree(h->wk.list);
     if (h->wk.flags) hipFree(h->wk.flags);
@@ -211,10 +193,8 @@ extern "C" int flx_dgemm(flx_handle h, int transa, int transb, int m, int n, int
     return FLX_SUCCESS;
 }
 
-static long round_ld(long n) { return ((n + 31) / 32) * 32; }
-
 // LU of device column-major A via the row-major working copy
-static int dgetrf_dev(flx_context *h, int m, int n, double *A, int lda, int *ipiv, int *info,
+int dgetrf_dev(flx_context *h, int m, int n, double *A, int lda, int *ipiv, int *info,
                       bool keep_perm)
 {
     const int mn = m < n ? m : n;
@@ -250,7 +230,7 @@ extern "C" int flx_dgetrf(flx_handle h, int m, int n, double *A, int lda, int *i
 }
 
 // NoTrans: X = U^{-1} L^{-1} P B ; Trans: X = P^T L^{-T} U^{-T} B
-static int dgetrs_dev(flx_context *h, int trans, int n, int nrhs, const double *A, int lda,
+int dgetrs_dev(flx_context *h, int trans, int n, int nrhs, const double *A, int lda,
                       const int *ipiv, const int *perm_fwd, double *B, int ldb)
 {
     const size_t bytes = sizeof(double) * (size_t)n * (size_t)nrhs;
@@ -608,6 +588,57 @@ extern "C" void dgesv_(const int *N, const int *NRHS, double *A, const int *LDA,
         st = run();
     }
     report("dgesv_", st);
+}
+
+extern "C" void dsgesv_(const int *N, const int *NRHS, double *A, const int *LDA, int *IPIV,
+                        double *B, const int *LDB, double *X, const int *LDX, double *WORK,
+                        float *SWORK, int *ITER, int *INFO)
+{
+    (void)WORK;
+    (void)SWORK;
+    *INFO = 0;
+    *ITER = 0;
+    if (*N < 0) *INFO = -1;
+    else if (*NRHS < 0) *INFO = -2;
+    else if (*LDA < (*N > 1 ? *N : 1)) *INFO = -4;
+    else if (*LDB < (*N > 1 ? *N : 1)) *INFO = -7;
+    else if (*LDX < (*N > 1 ? *N : 1)) *INFO = -9;
+    if (*INFO != 0) {
+        int i = -*INFO;
+        xerbla_("DSGESV", &i, 6);
+        return;
+    }
+    const int n = *N, nrhs = *NRHS;
+    if (n == 0) return;
+    flx_context *h = default_handle();
+    int st = h ? FLX_SUCCESS : FLX_HIP_ERROR;
+    if (h && n > 16384) st = FLX_UNSUPPORTED;
+    if (st == FLX_SUCCESS) {
+        auto run = [&]() -> int {
+            const size_t rb = sizeof(double) * (size_t)n * (nrhs > 0 ? nrhs : 1);
+            FLX_TRY(ensure(h, h->A, sizeof(double) * (size_t)n * n));
+            FLX_TRY(ensure(h, h->sb, rb));
+            FLX_TRY(ensure(h, h->sx, rb));
+            FLX_TRY(ensure(h, h->ipiv, sizeof(int) * (size_t)n));
+            FLX_TRY(h2d(h->A.p, A, n, n, *LDA));
+            FLX_TRY(h2d(h->sb.p, B, n, nrhs, *LDB));
+            int iter = 0, info = 0;
+            int s = flx_dsgesv(h, n, nrhs, (double *)h->A.p, n, (int *)h->ipiv.p,
+                               (const double *)h->sb.p, n, (double *)h->sx.p, n, &iter, &info);
+            if (s) return s;
+            s = flx_sync(h);
+            if (s) return s;
+            *ITER = iter;
+            *INFO = info;
+            // A is overwritten only when the FP64 fallback factorised it
+            if (iter < 0) FLX_TRY(d2h(A, h->A.p, n, n, *LDA));
+            FLX_TRY(hipMemcpy(IPIV, h->ipiv.p, sizeof(int) * n, hipMemcpyDeviceToHost));
+            if (info == 0) FLX_TRY(d2h(X, h->sx.p, n, nrhs, *LDX));
+            return FLX_SUCCESS;
+        };
+        st = run();
+    }
+    report("dsgesv_", st);
 }
 
 extern "C" void dlaswp_(const int *N, double *A, const int *LDA, const int *K1, const int *K2,

@@ -1,0 +1,57 @@
+// context.h -- the handle behind flx_handle and its device buffers (internal).
+#pragma once
+#include <climits>
+#include "../../include/flx.h"
+#include "flx_kernels.h"
+
+struct DevBuf {
+    void *p = nullptr;
+    size_t cap = 0;
+};
+
+// buffers of the mixed-precision solver
+struct MixBufs {
+    DevBuf W32, perm, cand, ccount, counters, u11, win, list, zero, lpmax, la, ub;
+    DevBuf inv_l, inv_u, y32, flags, err, dr, dc, part, scal, R, V, w, t, zeros, mpart, hbuf;
+    unsigned epoch = 0;
+};
+
+struct flx_context {
+    hipStream_t stream = nullptr;
+    int device = 0;
+    flx::LuWork wk;
+    flx::LuOptions opt;
+    DevBuf W, A, B, C, X, piv0, ipiv, info, perm, tinv;
+    DevBuf sb, sx;   // staging of the dsgesv_ seam
+    MixBufs mix;
+    int last_error = 0;
+    flx::Profiler *prof = nullptr;   // non-null while profiling
+};
+
+// binds the calling handle's profiler for the duration of one entry point
+struct ProfBind {
+    ProfBind(flx_context *h) { flx::tl_prof = h ? h->prof : nullptr; if (h) h->wk.prof = h->prof; }
+    ~ProfBind() { flx::tl_prof = nullptr; }
+};
+
+int hip_status(hipError_t e);
+
+#define FLX_TRY(x)                                                            \
+    do {                                                                      \
+        hipError_t e_ = (x);                                                  \
+        if (e_ != hipSuccess) return hip_status(e_);                          \
+    } while (0)
+
+// grow-only device buffer; waits for queued work before releasing memory
+hipError_t ensure(flx_context *h, DevBuf &b, size_t bytes);
+// as ensure, and zero-filled whenever it is (re)allocated
+hipError_t ensure_zeroed(flx_context *h, DevBuf &b, size_t bytes);
+hipError_t init_work(flx_context *h);
+void free_buf(DevBuf &b);
+inline long round_ld(long n) { return ((n + 31) / 32) * 32; }
+
+// FP64 LU / solve on device column-major data (api.hip)
+int dgetrf_dev(flx_context *h, int m, int n, double *A, int lda, int *ipiv, int *info,
+               bool keep_perm);
+int dgetrs_dev(flx_context *h, int trans, int n, int nrhs, const double *A, int lda,
+               const int *ipiv, const int *perm_fwd, double *B, int ldb);

@@ -3,6 +3,12 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#define FLX_CHECK(x)                                                          \
+    do {                                                                      \
+        hipError_t e_ = (x);                                                  \
+        if (e_ != hipSuccess) return e_;                                      \
+    } while (0)
+
 namespace flx {
 
 // ---- launch profiler -------------------------------------------------------
@@ -84,6 +90,18 @@ hipError_t launch_panel(hipStream_t s, LuWork &wk, double *P, long ldw, int mp, 
 // the 1-based first zero pivot column (INT_MAX if none; caller initialises).
 hipError_t getrf_rm_f64(hipStream_t s, LuWork &wk, double *W, long ldw, int m,
                         int n, int *piv0, int *info, const LuOptions &opt);
+// Row-major building blocks shared by the FP64 and the mixed-precision LU.
+// X <- L^{-1} X with L the k x k unit lower triangle at L (k <= 256).
+template <typename T>
+hipError_t trsm_rm_llu(hipStream_t s, Profiler *prof, const T *L, long ldl, T *X, long ldx,
+                       int k, int ncols);
+// Applies an interchange list (per-workgroup slices of PANEL_LIST_SLICE ints,
+// G slices) to the columns [0, c1) and [c2, n) of the row-major W, and
+// composes it into perm (when not null).
+template <typename T>
+hipError_t rowperm_rm(hipStream_t s, Profiler *prof, T *W, long ldw, int m, int c1, int c2,
+                      int n, const int *list, int G, int *perm, unsigned long long *moved);
+hipError_t iota(hipStream_t s, int *p, int n);
 hipError_t transpose_f64(hipStream_t s, const double *in, long ldi, double *out,
                          long ldo, int rows_in, int cols_in);
 hipError_t fill_int(hipStream_t s, int *p, int n, int v);
@@ -107,6 +125,67 @@ hipError_t trsm_cm_f64(hipStream_t s, int uplo_upper, int trans, int unit,
                        long ldb, double *X, long ldx, double *inv);
 hipError_t laswp_cm_f64(hipStream_t s, double *A, long lda, int ncols, int k1,
                         int k2, const int *ipiv, int incx);
+
+// ---- mixed-precision LU (mixed.hip) ----------------------------------------
+enum MixPrec { MIX_FP32 = 0, MIX_BF16 = 1, MIX_FP16 = 2 };
+// Device scratch of the mixed-precision factorisation (owned by a handle).
+struct MixWork {
+    int *perm = nullptr;          // n: original row at each position
+    int *cand = nullptr;          // tslu_nodes(n) x 32 candidate positions
+    int *ccount = nullptr;        // tslu_nodes(n)
+    unsigned *counters = nullptr; // tslu_nodes(n), zero between launches
+    float *u11 = nullptr;         // 32 x 32
+    int *win = nullptr;           // 32
+    int *list = nullptr;          // PANEL_LIST_SLICE
+    int *zero = nullptr;          // first zero pivot column (INT_MAX: none)
+    unsigned *lpmax = nullptr;    // 2 (FP16 operand maxima)
+    void *la = nullptr, *ub = nullptr;   // lp_pack_elems(n, nb) 16-bit each
+    Profiler *prof = nullptr;
+};
+size_t tslu_nodes(int mp);
+size_t lp_pack_elems(int n, int nb);
+// Factor the n x n row-major FP32 W in place (tournament-pivoted panels,
+// trailing update in prec); mw.perm receives the row order.
+hipError_t getrf_rm_mixed(hipStream_t s, const MixWork &mw, float *W, long ldw, int n,
+                          int prec, int nb);
+// W[i][j] = float(A(i,j) dr[i] dc[j]) (dr, dc optional); *flag |= 1 on overflow
+hipError_t conv_d2s_rm(hipStream_t s, const double *A, long lda, int n, const double *dr,
+                       const double *dc, float *W, long ldw, int *flag);
+
+// ---- refinement (ir.hip) -------------------------------------------------------
+// The FP32 factor as a preconditioner: M^{-1} v = D_c U^{-1} L^{-1} P D_r v.
+struct PrecFactor {
+    int n = 0;
+    const float *W = nullptr;  // row-major L\U
+    long ldw = 0;
+    const int *perm = nullptr;
+    const double *dr = nullptr, *dc = nullptr;   // optional power-of-two scales
+    const float *inv_l = nullptr, *inv_u = nullptr;   // trsv_inv_floats(n) each
+    float *y32 = nullptr;      // n
+    unsigned *flags = nullptr; // ceil(n/64), zero-initialised once
+    unsigned *epoch = nullptr; // host counter paired with flags
+    int *err = nullptr;        // device error flag
+};
+size_t ir_part_doubles(int n);
+size_t trsv_inv_floats(int n);
+size_t mdot_part_doubles(int n, int kmax);
+hipError_t gemv_resid(hipStream_t s, const double *A, long lda, int n, const double *x,
+                      const double *b, double *r, double *part, unsigned long long *nrm);
+hipError_t norm_inf(hipStream_t s, const double *A, long lda, int n, double *part,
+                    unsigned long long *nrm);
+hipError_t equilibrate(hipStream_t s, const double *A, long lda, int n, double *part,
+                       double *dr, double *dc);
+hipError_t tri_inv_rm(hipStream_t s, const float *W, long ldw, int n, float *inv_l, float *inv_u);
+hipError_t trsv_rm(hipStream_t s, int upper, const float *W, long ldw, int n, const float *inv,
+                   float *y, unsigned *flags, unsigned epoch, int *err);
+hipError_t zero_slots(hipStream_t s, unsigned long long *p, int k);
+hipError_t vmax(hipStream_t s, const double *v, int n, unsigned long long *nrm);
+hipError_t prec_apply(hipStream_t s, const PrecFactor &f, const double *v, double *z, int add,
+                      unsigned long long *vnrm, unsigned long long *znrm);
+hipError_t mdot(hipStream_t s, const double *V, long ldv, int n, int k, const double *w,
+                double *part, double *out);
+hipError_t maxpy(hipStream_t s, const double *V, long ldv, int n, int k, const double *h, double *w);
+hipError_t vscale_copy(hipStream_t s, const double *src, double alpha, double *dst, int n);
 
 // ---- utilities (util.hip) --------------------------------------------------
 hipError_t fill_uniform_f64(hipStream_t s, double *A, int m, int n, long lda,

@@ -83,13 +83,14 @@ FLX_DEV int load_list(const int *__restrict__ list, int G, int m, int *s_dst, in
     return s_off[PANEL_GROUPS_MAX];
 }
 
-__global__ __launch_bounds__(256) void rowperm_kernel(double *__restrict__ Wm, long ldw, int m,
+template <typename T>
+__global__ __launch_bounds__(256) void rowperm_kernel(T *__restrict__ Wm, long ldw, int m,
                                                       int c1, int c2, int n,
                                                       const int *__restrict__ list, int G,
                                                       int *__restrict__ perm,
                                                       unsigned long long *__restrict__ moved)
 {
-    __shared__ double tile[RP_MAXE][RP_CW];
+    __shared__ T tile[RP_MAXE][RP_CW];
     __shared__ int s_dst[RP_MAXE], s_src[RP_MAXE], s_off[PANEL_GROUPS_MAX + 1];
     const int cnt = load_list(list, G, m, s_dst, s_src, s_off);
     if (cnt <= 0) return;
@@ -141,36 +142,37 @@ constexpr int TR_CW = 32;
 constexpr int TR_KMAX = 256;
 constexpr int TR_BS = 16;
 
-__global__ __launch_bounds__(256) void trsm_rm_llu_kernel(const double *__restrict__ L,
-                                                          long ldl, double *__restrict__ X,
+template <typename T>
+__global__ __launch_bounds__(256) void trsm_rm_llu_kernel(const T *__restrict__ L,
+                                                          long ldl, T *__restrict__ X,
                                                           long ldx, int k, int ncols)
 {
-    __shared__ double xs[TR_KMAX][TR_CW + 1];
-    __shared__ double ls[TR_KMAX][TR_BS + 1];   // L(kb.., kb..kb+15) strip
+    __shared__ T xs[TR_KMAX][TR_CW + 1];
+    __shared__ T ls[TR_KMAX][TR_BS + 1];   // L(kb.., kb..kb+15) strip
     const int c0 = blockIdx.x * TR_CW;
     const int cw = min(TR_CW, ncols - c0);
     const int tid = threadIdx.x;
     const int tc = tid % TR_CW, tg = tid / TR_CW;   // 32 columns x 8 row groups
     for (int q = tid; q < k * TR_CW; q += 256) {
         int rr = q / TR_CW, c = q % TR_CW;
-        xs[rr][c] = (c < cw) ? X[(long)rr * ldx + c0 + c] : 0.0;
+        xs[rr][c] = (c < cw) ? X[(long)rr * ldx + c0 + c] : T(0);
     }
     for (int kb = 0; kb < k; kb += TR_BS) {
         const int bs = min(TR_BS, k - kb);
         // stage the strip of L below (and including) the diagonal block
         for (int q = tid; q < (k - kb) * TR_BS; q += 256) {
             int rr = q / TR_BS, cc = q % TR_BS;
-            ls[rr][cc] = (cc < bs) ? L[(long)(kb + rr) * ldl + kb + cc] : 0.0;
+            ls[rr][cc] = (cc < bs) ? L[(long)(kb + rr) * ldl + kb + cc] : T(0);
         }
         __syncthreads();
         // diagonal block: column tc held in registers by threads tg == 0
         if (tg == 0) {
-            double xb[TR_BS];
+            T xb[TR_BS];
 #pragma unroll
-            for (int i = 0; i < TR_BS; ++i) xb[i] = (i < bs) ? xs[kb + i][tc] : 0.0;
+            for (int i = 0; i < TR_BS; ++i) xb[i] = (i < bs) ? xs[kb + i][tc] : T(0);
 #pragma unroll
             for (int i = 1; i < TR_BS; ++i) {
-                double v = xb[i];
+                T v = xb[i];
 #pragma unroll
                 for (int q = 0; q < i; ++q) v = fma(-ls[i][q], xb[q], v);
                 xb[i] = v;
@@ -181,11 +183,11 @@ __global__ __launch_bounds__(256) void trsm_rm_llu_kernel(const double *__restri
         }
         __syncthreads();
         // fold the solved block into the rows below
-        double xb[TR_BS];
+        T xb[TR_BS];
 #pragma unroll
         for (int q = 0; q < TR_BS; ++q) xb[q] = xs[kb + q < k ? kb + q : kb][tc];
         for (int rr = kb + bs + tg; rr < k; rr += 8) {
-            double v = xs[rr][tc];
+            T v = xs[rr][tc];
 #pragma unroll
             for (int q = 0; q < TR_BS; ++q) v = fma(-ls[rr - kb][q], xb[q], v);
             xs[rr][tc] = v;
@@ -268,21 +270,48 @@ hipError_t piv0_to_ipiv(hipStream_t s, const int *piv0, int *ipiv, int n, int *i
 // ---------------------------------------------------------------------------
 // Host driver.
 // ---------------------------------------------------------------------------
-#define FLX_CHECK(x)                                                          \
-    do {                                                                      \
-        hipError_t e_ = (x);                                                  \
-        if (e_ != hipSuccess) return e_;                                      \
-    } while (0)
 
-static hipError_t launch_trsm_rm(hipStream_t s, Profiler *prof, const double *L, long ldl,
-                                 double *X, long ldx, int k, int ncols)
+template <typename T>
+hipError_t trsm_rm_llu(hipStream_t s, Profiler *prof, const T *L, long ldl, T *X, long ldx,
+                       int k, int ncols)
 {
     if (ncols <= 0 || k <= 0) return hipSuccess;
     if (k > TR_KMAX) return hipErrorInvalidValue;
     ProfScope ps(prof, s, PROF_TRSM_RM, (double)k * k * ncols,
-                 8.0 * (2.0 * k * ncols + 0.5 * k * k));
-    hipLaunchKernelGGL(trsm_rm_llu_kernel, dim3((ncols + TR_CW - 1) / TR_CW), dim3(256), 0,
+                 sizeof(T) * (2.0 * k * ncols + 0.5 * k * k));
+    hipLaunchKernelGGL(trsm_rm_llu_kernel<T>, dim3((ncols + TR_CW - 1) / TR_CW), dim3(256), 0,
                        s, L, ldl, X, ldx, k, ncols);
+    return hipGetLastError();
+}
+template hipError_t trsm_rm_llu<double>(hipStream_t, Profiler *, const double *, long, double *,
+                                        long, int, int);
+template hipError_t trsm_rm_llu<float>(hipStream_t, Profiler *, const float *, long, float *,
+                                       long, int, int);
+
+template <typename T>
+hipError_t rowperm_rm(hipStream_t s, Profiler *prof, T *W, long ldw, int m, int c1, int c2,
+                      int n, const int *list, int G, int *perm, unsigned long long *moved)
+{
+    const int nleft = (c1 + RP_CW - 1) / RP_CW;
+    const int nright = (n - c2 + RP_CW - 1) / RP_CW;
+    if (nleft + nright > 0) {
+        ProfScope ps(prof, s, PROF_ROWPERM, 0.0, 0.0);
+        hipLaunchKernelGGL(rowperm_kernel<T>, dim3(nleft + nright), dim3(256), 0, s, W, ldw, m,
+                           c1, c2, n, list, G, perm, moved);
+    } else if (perm) {
+        hipLaunchKernelGGL(perm_compose_kernel, dim3(1), dim3(256), 0, s, list, G, perm, m);
+    }
+    return hipGetLastError();
+}
+template hipError_t rowperm_rm<double>(hipStream_t, Profiler *, double *, long, int, int, int,
+                                       int, const int *, int, int *, unsigned long long *);
+template hipError_t rowperm_rm<float>(hipStream_t, Profiler *, float *, long, int, int, int, int,
+                                      const int *, int, int *, unsigned long long *);
+
+hipError_t iota(hipStream_t s, int *p, int n)
+{
+    if (n <= 0) return hipSuccess;
+    hipLaunchKernelGGL(iota_kernel, dim3((n + 255) / 256), dim3(256), 0, s, p, n);
     return hipGetLastError();
 }
 
@@ -310,9 +339,7 @@ hipError_t getrf_rm_f64(hipStream_t s, LuWork &wk, double *W, long ldw, int m, i
     nb = (nb / ib) * ib;
     if (m > PANEL_ROWS_MAX) return hipErrorInvalidValue;
     // the composed permutation, when requested, starts as the identity
-    if (wk.perm) {
-        hipLaunchKernelGGL(iota_kernel, dim3((m + 255) / 256), dim3(256), 0, s, wk.perm, m);
-    }
+    if (wk.perm) FLX_CHECK(iota(s, wk.perm, m));
     for (int J = 0; J < mn; J += nb) {
         const int jb = (mn - J < nb) ? mn - J : nb;
         for (int j = J; j < J + jb; j += ib) {
@@ -320,30 +347,20 @@ hipError_t getrf_rm_f64(hipStream_t s, LuWork &wk, double *W, long ldw, int m, i
             FLX_CHECK(launch_panel(s, wk, W + (long)j * ldw + j, ldw, m - j, w, piv0 + j, j,
                                    info, ib));
             // apply this sub-panel's interchanges to every other column
-            const int nleft = (j + RP_CW - 1) / RP_CW;
-            const int nright = (n - (j + w) + RP_CW - 1) / RP_CW;
-            if (nleft + nright > 0) {
-                ProfScope ps(wk.prof, s, PROF_ROWPERM, 0.0, 0.0);
-                hipLaunchKernelGGL(rowperm_kernel, dim3(nleft + nright), dim3(256), 0, s, W,
-                                   ldw, m, j, j + w, n, wk.list, panel_groups(m - j), wk.perm,
-                                   wk.moved);
-            } else if (wk.perm) {
-                hipLaunchKernelGGL(perm_compose_kernel, dim3(1), dim3(256), 0, s, wk.list,
-                                   panel_groups(m - j), wk.perm, m);
-            }
-            FLX_CHECK(hipGetLastError());
+            FLX_CHECK(rowperm_rm(s, wk.prof, W, ldw, m, j, j + w, n, wk.list,
+                                 panel_groups(m - j), wk.perm, wk.moved));
             // inner update of the rest of the outer block
             const int cr = J + jb - (j + w);
             if (cr > 0) {
-                FLX_CHECK(launch_trsm_rm(s, wk.prof, W + (long)j * ldw + j, ldw,
-                                         W + (long)j * ldw + j + w, ldw, w, cr));
+                FLX_CHECK(trsm_rm_llu(s, wk.prof, (const double *)W + (long)j * ldw + j, ldw,
+                                      W + (long)j * ldw + j + w, ldw, w, cr));
                 FLX_CHECK(rm_update(s, wk.prof, W, ldw, j + w, m - j - w, j + w, cr, j, w));
             }
         }
         const int nr = n - (J + jb);
         if (nr > 0) {
-            FLX_CHECK(launch_trsm_rm(s, wk.prof, W + (long)J * ldw + J, ldw,
-                                     W + (long)J * ldw + J + jb, ldw, jb, nr));
+            FLX_CHECK(trsm_rm_llu(s, wk.prof, (const double *)W + (long)J * ldw + J, ldw,
+                                  W + (long)J * ldw + J + jb, ldw, jb, nr));
             FLX_CHECK(rm_update(s, wk.prof, W, ldw, J + jb, m - J - jb, J + jb, nr, J, jb));
         }
     }

@@ -1,0 +1,619 @@
+// mixed.hip -- low-precision LU factorisation for iterative refinement.
+//
+// The factor of A (FP64, column-major, caller's) is computed in a row-major
+// FP32 working copy W: FP32 panels and triangular solves, and a trailing
+// update in FP32 (v_mfma_f32_16x16x4_f32) or in BF16 / FP16 with FP32
+// accumulation (v_mfma_f32_16x16x32_bf16 / _f16).  It is the factorisation
+// step of a mixed-precision solver in the style of LAPACK's dsgesv (netlib
+// dsgesv.f:131-337, reached from FLENS as cxxlapack::sgesv -> dsgesv_,
+// cxxlapack/netlib/interface/lapack.in.h:10611-10623).  Refinement (FP64
+// residuals, classic and GMRES) lives in ir.hip.
+//
+// Pivoting: tournament pivoting ("CALU", Grigori, Demmel & Xiang 2011).
+// The pivot rows of a 32-column panel are chosen by partial-pivoting
+// eliminations on blocks of 256 rows (leaves), whose 32 winners each go up
+// a reduction tree of fan-in 8 where the same elimination on the stacked
+// candidates' original rows picks 32 again.  One kernel launch per panel:
+// the last workgroup to finish a tree node's children (an atomic arrival
+// count, no spinning) climbs to that node.  The root's elimination yields
+// L11\U11 of the panel directly; L21 = A21 U11^-1 is one independent
+// substitution per row.  This replaces the per-column grid-wide pivot
+// exchange of partial pivoting (one device round trip per column) with
+// about one per panel.  The factor is only a preconditioner for FP64
+// refinement, so it does not have to reproduce the reference's FP32 pivot
+// sequence; its quality is checked by its backward error.
+#include <cfloat>
+#include <climits>
+#include "flx_common.h"
+#include "flx_kernels.h"
+
+namespace flx {
+
+typedef __bf16 bf16x8_t __attribute__((ext_vector_type(8)));
+typedef _Float16 f16x8_t __attribute__((ext_vector_type(8)));
+
+// ---------------------------------------------------------------------------
+// Tournament panel
+// ---------------------------------------------------------------------------
+constexpr int TS_B = 32;     // panel width (max)
+constexpr int TS_R = 256;    // rows per leaf = threads per workgroup
+constexpr int TS_F = 8;      // tree fan-in: 8 children x 32 candidates = 256 rows
+
+// Selection key: larger |v| first, then the lower panel position; 0 = none.
+FLX_DEV uint64_t ts_key(float v, int pos)
+{
+    const uint32_t a = __float_as_uint(fabsf(v)) + 1u;
+    return ((uint64_t)a << 32) | (uint64_t)(~(uint32_t)pos);
+}
+
+FLX_DEV uint64_t wave_max_u64(uint64_t v)
+{
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) {
+        uint64_t u = __shfl_xor(v, o, 64);
+        v = u > v ? u : v;
+    }
+    return v;
+}
+
+struct TsShared {
+    float4 row[2][4][TS_B / 4];   // per-wave candidate rows, double-buffered
+    uint64_t key[2][4];
+    int win[TS_B];                // selected positions in order
+    int nsel;
+    int last;
+};
+
+// Partial-pivoting elimination over the 256 rows held by the workgroup (one
+// per thread, in registers).  Selects up to w rows; when u11 is given (the
+// root) the selected rows' eliminated values -- L multipliers left of the
+// diagonal, U entries from it -- are written to u11 in selection order, and
+// the first column whose pivot is exactly zero is reported through zero_col.
+FLX_DEV void ts_select(float (&r)[TS_B], bool act, int pos, int w, TsShared &sh,
+                       float *__restrict__ u11, int *zero_col, int zbase)
+{
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    int nsel = 0;
+#pragma unroll
+    for (int j = 0; j < TS_B; ++j) {
+        if (j < w) {
+            const int par = j & 1;
+            const uint64_t key = act ? ts_key(r[j], pos) : 0ull;
+            const uint64_t wk = wave_max_u64(key);
+            if (key != 0ull && key == wk) {
+#pragma unroll
+                for (int c = 0; c < TS_B / 4; ++c)
+                    sh.row[par][wave][c] = make_float4(r[4 * c], r[4 * c + 1], r[4 * c + 2],
+                                                       r[4 * c + 3]);
+            }
+            if (lane == 0) sh.key[par][wave] = wk;
+            __syncthreads();
+            uint64_t best = sh.key[par][0];
+            int bw = 0;
+#pragma unroll
+            for (int q = 1; q < 4; ++q) {
+                const uint64_t kq = sh.key[par][q];
+                if (kq > best) { best = kq; bw = q; }
+            }
+            if (best != 0ull) {
+                const float *pr = reinterpret_cast<const float *>(sh.row[par][bw]);
+                if (key == best) {
+                    act = false;
+                    if (u11) {
+#pragma unroll
+                        for (int c = 0; c < TS_B; ++c) u11[j * TS_B + c] = r[c];
+                    }
+                }
+                if (threadIdx.x == 0) {
+                    sh.win[j] = (int)(~(uint32_t)(best & 0xffffffffull));
+                    if (zero_col && (best >> 32) == 1ull) atomicMin(zero_col, zbase + j);
+                }
+                nsel = j + 1;
+                if (act) {
+                    const float p = pr[j];
+                    const float l = (p != 0.0f) ? r[j] / p : 0.0f;
+                    r[j] = l;
+#pragma unroll
+                    for (int c = j + 1; c < TS_B; ++c) r[c] = fmaf(-l, pr[c], r[c]);
+                }
+            }
+        }
+    }
+    if (threadIdx.x == 0) sh.nsel = nsel;
+    __syncthreads();
+}
+
+FLX_DEV int ld_agent_i(const int *p)
+{
+    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+FLX_DEV void st_agent_i(int *p, int v)
+{
+    __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+struct TsTree {
+    int nlev;        // levels, 0 = leaves
+    int count[6];    // nodes per level
+    int off[6];      // first node id of each level
+};
+
+// P: panel top-left in W (row-major, ld ldw), mp rows x w (<= 32) columns.
+// Outputs: u11 (32 x 32, selection order), win (selected positions,
+// relative to the panel top), list (one interchange slice, absolute rows
+// r0 + position), zero (first zero-pivot column, INT_MAX if none).
+__global__ __launch_bounds__(TS_R) void tslu_kernel(const float *__restrict__ P, long ldw, int mp,
+                                                    int w, int r0, TsTree tree,
+                                                    int *__restrict__ cand,
+                                                    int *__restrict__ ccount,
+                                                    unsigned *__restrict__ counters,
+                                                    float *__restrict__ u11,
+                                                    int *__restrict__ win,
+                                                    int *__restrict__ list,
+                                                    int *__restrict__ zero)
+{
+    __shared__ TsShared sh;
+    __shared__ int s_cpos[TS_F * TS_B];
+    const int t = threadIdx.x;
+    int node = blockIdx.x;     // index within the level
+    int lev = 0;
+    float r[TS_B];
+    // leaf: rows [node*256, +256)
+    int pos = node * TS_R + t;
+    bool act = pos < mp;
+#pragma unroll
+    for (int c = 0; c < TS_B; ++c) r[c] = (act && c < w) ? P[(long)pos * ldw + c] : 0.0f;
+    for (;;) {
+        const bool root = (lev == tree.nlev - 1);
+        ts_select(r, act, pos, w, sh, root ? u11 : nullptr, root ? zero : nullptr, r0);
+        const int nsel = sh.nsel;
+        if (root) break;
+        const int id = tree.off[lev] + node;
+        if (t < nsel) st_agent_i(&cand[(long)id * TS_B + t], sh.win[t]);
+        if (t == 0) st_agent_i(&ccount[id], nsel);
+        __threadfence();
+        __syncthreads();
+        const int parent = node / TS_F;
+        const int c0 = parent * TS_F;
+        const int nchild = min(TS_F, tree.count[lev] - c0);
+        const int pid = tree.off[lev + 1] + parent;
+        if (t == 0) {
+            const unsigned old = atomicAdd(&counters[pid], 1u);
+            sh.last = (old == (unsigned)(nchild - 1));
+            if (sh.last) atomicExch(&counters[pid], 0u);   // ready for the next launch
+        }
+        __syncthreads();
+        if (!sh.last) return;
+        __threadfence();
+        // gather the children's candidates: thread t -> child t/32, slot t%32
+        const int ch = t / TS_B, k = t % TS_B;
+        int p = -1;
+        if (ch < nchild) {
+            const int cid = tree.off[lev] + c0 + ch;
+            if (k < ld_agent_i(&ccount[cid])) p = ld_agent_i(&cand[(long)cid * TS_B + k]);
+        }
+        act = (p >= 0 && p < mp);
+        pos = act ? p : INT_MAX;
+#pragma unroll
+        for (int c = 0; c < TS_B; ++c) r[c] = (act && c < w) ? P[(long)pos * ldw + c] : 0.0f;
+        node = parent;
+        ++lev;
+    }
+    // root: publish the winners and the interchange list.  Rows k < nsel
+    // receive winner k; the displaced top rows (not winners) fill the
+    // winners' vacated slots (>= nsel), both in ascending order.
+    __shared__ int s_D[TS_B], s_V[TS_B], s_n[2];
+    const int nsel = sh.nsel;
+    const int k = t;
+    const int wkp = (k < nsel) ? sh.win[k] : -1;
+    bool inD = false, inV = false;
+    if (t < 64) {
+        if (k < nsel) win[k] = wkp;
+        bool top_is_win = false;
+        if (k < nsel)
+            for (int q = 0; q < nsel; ++q) top_is_win |= (sh.win[q] == k);
+        inD = (k < nsel) && !top_is_win;
+        inV = (k < nsel) && (wkp >= nsel);
+        const uint64_t bD = __ballot(inD);
+        const int rankD = __popcll(bD & ((1ull << k) - 1ull));
+        int rankV = 0;
+        if (inV)
+            for (int q = 0; q < nsel; ++q) rankV += (sh.win[q] >= nsel && sh.win[q] < wkp);
+        const bool mv1 = (k < nsel) && (wkp != k);
+        const uint64_t b1 = __ballot(mv1);
+        const int slot1 = __popcll(b1 & ((1ull << k) - 1ull));
+        if (mv1) {
+            list[1 + 2 * slot1] = r0 + k;
+            list[2 + 2 * slot1] = r0 + wkp;
+        }
+        if (inD) s_D[rankD] = k;
+        if (inV) s_V[rankV] = wkp;
+        if (k == 0) { s_n[0] = __popcll(b1); s_n[1] = __popcll(bD); }
+    }
+    __syncthreads();
+    if (t < 64) {
+        const int n1 = s_n[0], nD = s_n[1];
+        if (k < nD) {
+            list[1 + 2 * (n1 + k)] = r0 + s_V[k];
+            list[2 + 2 * (n1 + k)] = r0 + s_D[k];
+        }
+        if (k == 0) list[0] = n1 + nD;
+    }
+}
+
+// Writes the panel's factor in place, before the rows move: selected rows
+// get their L11\U11 row, every other row its L21 row  x U11 = a.
+__global__ __launch_bounds__(256) void tslu_finish_kernel(float *__restrict__ P, long ldw, int mp,
+                                                          int w, const float *__restrict__ u11,
+                                                          const int *__restrict__ win, int nsel)
+{
+    __shared__ float su[TS_B][TS_B + 1];
+    __shared__ int sw[TS_B];
+    const int t = threadIdx.x;
+    for (int q = t; q < TS_B * TS_B; q += 256) su[q / TS_B][q % TS_B] = u11[q];
+    if (t < TS_B) sw[t] = (t < nsel) ? win[t] : -1;
+    __syncthreads();
+    const int pos = blockIdx.x * 256 + t;
+    if (pos >= mp) return;
+    int k = -1;
+    for (int q = 0; q < nsel; ++q) k = (sw[q] == pos) ? q : k;
+    float *row = P + (long)pos * ldw;
+    if (k >= 0) {
+        for (int c = 0; c < w; ++c) row[c] = su[k][c];
+        return;
+    }
+    float x[TS_B];
+#pragma unroll
+    for (int c = 0; c < TS_B; ++c) x[c] = (c < w) ? row[c] : 0.0f;
+#pragma unroll
+    for (int j = 0; j < TS_B; ++j) {
+        if (j < nsel) {
+            float v = x[j];
+#pragma unroll
+            for (int q = 0; q < j; ++q) v = fmaf(-x[q], su[q][j], v);
+            const float d = su[j][j];
+            x[j] = (d != 0.0f) ? v / d : v;
+        }
+    }
+#pragma unroll
+    for (int c = 0; c < TS_B; ++c)
+        if (c < w) row[c] = x[c];
+}
+
+size_t tslu_nodes(int mp)
+{
+    size_t total = 0;
+    int c = (mp + TS_R - 1) / TS_R;
+    for (;;) {
+        total += c;
+        if (c <= 1) break;
+        c = (c + TS_F - 1) / TS_F;
+    }
+    return total;
+}
+
+static TsTree tslu_tree(int mp)
+{
+    TsTree t{};
+    int c = (mp + TS_R - 1) / TS_R, off = 0;
+    t.nlev = 0;
+    for (;;) {
+        t.count[t.nlev] = c;
+        t.off[t.nlev] = off;
+        off += c;
+        ++t.nlev;
+        if (c <= 1) break;
+        c = (c + TS_F - 1) / TS_F;
+    }
+    return t;
+}
+
+// ---------------------------------------------------------------------------
+// Low-precision trailing update  C (nr x nc, row-major FP32) += La * Ub^T
+// with La (nr_pad x Kp) and Ub (nc_pad x Kp) packed 16-bit operands, k
+// contiguous, so each MFMA operand fragment is one 16-byte load:
+//   A operand: lane l holds La[i0 + l%16][k0 + 8(l/16) .. +7]
+//   B operand: lane l holds Ub[j0 + l%16][k0 + 8(l/16) .. +7]
+//   result item q of lane l: C[i0 + 4(l/16) + q][j0 + l%16]
+// 256 threads = 4 waves in 2 x 2, a 64 x 64 sub-tile each (4 x 4 MFMAs).
+// The update is bound by streaming C (8 bytes per element per step), not by
+// the matrix cores, so the operands come straight from global memory/L2.
+// ---------------------------------------------------------------------------
+template <typename V>
+FLX_DEV float4_t mfma_lp(V a, V b, float4_t c);
+template <>
+FLX_DEV float4_t mfma_lp<bf16x8_t>(bf16x8_t a, bf16x8_t b, float4_t c)
+{
+    return __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, c, 0, 0, 0);
+}
+template <>
+FLX_DEV float4_t mfma_lp<f16x8_t>(f16x8_t a, f16x8_t b, float4_t c)
+{
+    return __builtin_amdgcn_mfma_f32_16x16x32_f16(a, b, c, 0, 0, 0);
+}
+
+// operand scale: 2^(14 - e) with max = f * 2^e, f in [0.5, 1): the largest
+// magnitude lands in [2^13, 2^14), well inside FP16's range (65504).
+FLX_DEV float lp_scale(const unsigned *maxbits)
+{
+    if (!maxbits) return 1.0f;
+    const float m = __uint_as_float(*maxbits);
+    if (!(m > 0.0f) || !(m <= FLT_MAX)) return 1.0f;
+    int e;
+    frexpf(m, &e);
+    return ldexpf(1.0f, 14 - e);
+}
+
+template <typename V>
+__global__ __launch_bounds__(256) void gemm_lp_kernel(const V *__restrict__ La,
+                                                      const V *__restrict__ Ub, int Kp,
+                                                      float *__restrict__ C, long ldc, int nr,
+                                                      int nc, const unsigned *__restrict__ maxes)
+{
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int i0 = blockIdx.y * 128 + (wave >> 1) * 64;
+    const int j0 = blockIdx.x * 128 + (wave & 1) * 64;
+    const int lr = lane & 15, lk = lane >> 4;
+    float4_t acc[4][4];
+#pragma unroll
+    for (int a = 0; a < 4; ++a)
+#pragma unroll
+        for (int b = 0; b < 4; ++b) acc[a][b] = float4_t{0.f, 0.f, 0.f, 0.f};
+    // operands are indexed in 8-element vectors: a row holds Kp / 8
+    const int kv = Kp / 8;
+    const V *pa = La + (long)(i0 + lr) * kv + lk;
+    const V *pb = Ub + (long)(j0 + lr) * kv + lk;
+    const long sa = 16L * kv;   // next 16-row fragment
+    V fa[4], fb[4];
+#pragma unroll
+    for (int a = 0; a < 4; ++a) fa[a] = pa[a * sa];
+#pragma unroll
+    for (int b = 0; b < 4; ++b) fb[b] = pb[b * sa];
+    for (int k = 0; k < kv; k += 4) {
+        V na[4], nb[4];
+        const bool more = k + 4 < kv;
+        if (more) {
+#pragma unroll
+            for (int a = 0; a < 4; ++a) na[a] = pa[a * sa + k + 4];
+#pragma unroll
+            for (int b = 0; b < 4; ++b) nb[b] = pb[b * sa + k + 4];
+        }
+#pragma unroll
+        for (int a = 0; a < 4; ++a)
+#pragma unroll
+            for (int b = 0; b < 4; ++b) acc[a][b] = mfma_lp<V>(fa[a], fb[b], acc[a][b]);
+        if (more) {
+#pragma unroll
+            for (int a = 0; a < 4; ++a) { fa[a] = na[a]; fb[a] = nb[a]; }
+        }
+    }
+    float inv = 1.0f;
+    if (maxes) inv = 1.0f / (lp_scale(maxes) * lp_scale(maxes + 1));
+#pragma unroll
+    for (int a = 0; a < 4; ++a)
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            const int row = i0 + a * 16 + 4 * lk + q;
+            if (row < nr) {
+                float *cr = C + (long)row * ldc;
+#pragma unroll
+                for (int b = 0; b < 4; ++b) {
+                    const int col = j0 + b * 16 + lr;
+                    if (col < nc) cr[col] = fmaf(acc[a][b][q], inv, cr[col]);
+                }
+            }
+        }
+}
+
+// max |x| over an nr x nc row-major block into *out (float bits, >= 0)
+__global__ __launch_bounds__(256) void absmax_rm_kernel(const float *__restrict__ X, long ldx,
+                                                        int nr, int nc, unsigned *out)
+{
+    float m = 0.0f;
+    const long total = (long)nr * nc;
+    for (long q = (long)blockIdx.x * 256 + threadIdx.x; q < total; q += (long)gridDim.x * 256) {
+        const int i = (int)(q / nc), j = (int)(q % nc);
+        m = fmaxf(m, fabsf(X[(long)i * ldx + j]));
+    }
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) m = fmaxf(m, __shfl_xor(m, o, 64));
+    if ((threadIdx.x & 63) == 0) atomicMax(out, __float_as_uint(m));
+}
+
+template <typename E> FLX_DEV E to_lp(float v);
+template <> FLX_DEV __bf16 to_lp<__bf16>(float v) { return (__bf16)v; }
+template <> FLX_DEV _Float16 to_lp<_Float16>(float v) { return (_Float16)v; }
+
+// La[i][k] = -L[i][k] * s  (rows i < nr_pad, k < Kp; zero outside nr x kk)
+template <typename E>
+__global__ __launch_bounds__(256) void pack_l_kernel(const float *__restrict__ L, long ldl, int nr,
+                                                     int kk, E *__restrict__ La, int nr_pad,
+                                                     int Kp, const unsigned *__restrict__ maxes)
+{
+    const float s = -lp_scale(maxes);
+    const long total = (long)nr_pad * Kp;
+    for (long q = (long)blockIdx.x * 256 + threadIdx.x; q < total; q += (long)gridDim.x * 256) {
+        const int i = (int)(q / Kp), k = (int)(q % Kp);
+        const float v = (i < nr && k < kk) ? L[(long)i * ldl + k] * s : 0.0f;
+        La[q] = to_lp<E>(v);
+    }
+}
+
+// Ub[j][k] = U[k][j] * s  (j < nc_pad, k < Kp; zero outside kk x nc);
+// 32 x 32 tiles transposed through LDS.
+template <typename E>
+__global__ __launch_bounds__(256) void pack_u_kernel(const float *__restrict__ U, long ldu, int kk,
+                                                     int nc, E *__restrict__ Ub, int nc_pad,
+                                                     int Kp, const unsigned *__restrict__ maxes)
+{
+    __shared__ float tile[32][33];
+    const float s = lp_scale(maxes);
+    const int k0 = blockIdx.y * 32, j0 = blockIdx.x * 32;
+    const int tx = threadIdx.x & 31, ty = threadIdx.x >> 5;
+    for (int q = ty; q < 32; q += 8) {
+        const int k = k0 + q, j = j0 + tx;
+        tile[q][tx] = (k < kk && j < nc) ? U[(long)k * ldu + j] * s : 0.0f;
+    }
+    __syncthreads();
+    for (int q = ty; q < 32; q += 8) {
+        const int j = j0 + q, k = k0 + tx;
+        if (j < nc_pad && k < Kp) Ub[(long)j * Kp + k] = to_lp<E>(tile[tx][q]);
+    }
+}
+
+__global__ void zero_u32_kernel(unsigned *p, int n)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = 0u;
+}
+
+size_t lp_pack_elems(int n, int nb)
+{
+    const int Kp = ((nb + 31) / 32) * 32;
+    const int np = ((n + 127) / 128) * 128;
+    return (size_t)np * Kp;
+}
+
+// C[r0.., c0..] -= W[r0.., k0..k0+kk) * W[k0..k0+kk, c0..) in precision prec
+static hipError_t lp_update(hipStream_t s, const MixWork &mw, int prec, float *W, long ldw,
+                            int r0, int nr, int c0, int nc, int k0, int kk)
+{
+    if (nr <= 0 || nc <= 0 || kk <= 0) return hipSuccess;
+    ProfScope ps(mw.prof, s, PROF_GEMM, 2.0 * nr * nc * kk,
+                 4.0 * (2.0 * nr * nc + (double)nr * kk + (double)kk * nc));
+    if (prec == MIX_FP32)
+        return gemm_f32(s, 0, 0, nc, nr, kk, -1.0f, W + (long)k0 * ldw + c0, ldw,
+                        W + (long)r0 * ldw + k0, ldw, 1.0f, W + (long)r0 * ldw + c0, ldw, 0);
+    const int Kp = ((kk + 31) / 32) * 32;
+    const int nr_pad = ((nr + 127) / 128) * 128, nc_pad = ((nc + 127) / 128) * 128;
+    unsigned *mx = nullptr;
+    const float *L = W + (long)r0 * ldw + k0;
+    const float *U = W + (long)k0 * ldw + c0;
+    if (prec == MIX_FP16) {
+        mx = mw.lpmax;
+        hipLaunchKernelGGL(zero_u32_kernel, dim3(1), dim3(64), 0, s, mx, 2);
+        hipLaunchKernelGGL(absmax_rm_kernel, dim3(512), dim3(256), 0, s, L, ldw, nr, kk, mx);
+        hipLaunchKernelGGL(absmax_rm_kernel, dim3(512), dim3(256), 0, s, U, ldw, kk, nc, mx + 1);
+    }
+    const long la_total = (long)nr_pad * Kp;
+    const int gl = (int)((la_total + 255) / 256 < 8192 ? (la_total + 255) / 256 : 8192);
+    dim3 gu((nc_pad + 31) / 32, (Kp + 31) / 32);
+    dim3 gg(nc_pad / 128, nr_pad / 128);
+    if (prec == MIX_BF16) {
+        __bf16 *La = reinterpret_cast<__bf16 *>(mw.la), *Ub = reinterpret_cast<__bf16 *>(mw.ub);
+        hipLaunchKernelGGL(pack_l_kernel<__bf16>, dim3(gl), dim3(256), 0, s, L, ldw, nr, kk, La,
+                           nr_pad, Kp, (const unsigned *)nullptr);
+        hipLaunchKernelGGL(pack_u_kernel<__bf16>, gu, dim3(256), 0, s, U, ldw, kk, nc, Ub, nc_pad,
+                           Kp, (const unsigned *)nullptr);
+        hipLaunchKernelGGL(gemm_lp_kernel<bf16x8_t>, gg, dim3(256), 0, s,
+                           reinterpret_cast<const bf16x8_t *>(La),
+                           reinterpret_cast<const bf16x8_t *>(Ub), Kp,
+                           W + (long)r0 * ldw + c0, ldw, nr, nc, (const unsigned *)nullptr);
+    } else {
+        _Float16 *La = reinterpret_cast<_Float16 *>(mw.la), *Ub = reinterpret_cast<_Float16 *>(mw.ub);
+        hipLaunchKernelGGL(pack_l_kernel<_Float16>, dim3(gl), dim3(256), 0, s, L, ldw, nr, kk,
+                           La, nr_pad, Kp, (const unsigned *)mx);
+        hipLaunchKernelGGL(pack_u_kernel<_Float16>, gu, dim3(256), 0, s, U, ldw, kk, nc, Ub,
+                           nc_pad, Kp, (const unsigned *)(mx + 1));
+        hipLaunchKernelGGL(gemm_lp_kernel<f16x8_t>, gg, dim3(256), 0, s,
+                           reinterpret_cast<const f16x8_t *>(La),
+                           reinterpret_cast<const f16x8_t *>(Ub), Kp,
+                           W + (long)r0 * ldw + c0, ldw, nr, nc, (const unsigned *)mx);
+    }
+    return hipGetLastError();
+}
+
+// ---------------------------------------------------------------------------
+// Driver: factor the n x n row-major FP32 W in place; perm[i] = original row
+// now at position i; *zero = first zero-pivot column (0-based, INT_MAX if
+// none; caller initialises).
+// ---------------------------------------------------------------------------
+hipError_t getrf_rm_mixed(hipStream_t s, const MixWork &mw, float *W, long ldw, int n,
+                          int prec, int nb)
+{
+    if (n <= 0) return hipSuccess;
+    const int ib = TS_B;
+    if (nb < ib) nb = ib;
+    if (nb > 256) nb = 256;
+    nb = (nb / ib) * ib;
+    FLX_CHECK(iota(s, mw.perm, n));
+    for (int J = 0; J < n; J += nb) {
+        const int jb = (n - J < nb) ? n - J : nb;
+        for (int j = J; j < J + jb; j += ib) {
+            const int w = (J + jb - j < ib) ? J + jb - j : ib;
+            const int mp = n - j;
+            const TsTree tree = tslu_tree(mp);
+            {
+                ProfScope ps(mw.prof, s, PROF_PANEL, 0.0, 8.0 * mp * w);
+                hipLaunchKernelGGL(tslu_kernel, dim3(tree.count[0]), dim3(TS_R), 0, s,
+                                   (const float *)(W + (long)j * ldw + j), ldw, mp, w, j, tree,
+                                   mw.cand, mw.ccount, mw.counters, mw.u11, mw.win, mw.list,
+                                   mw.zero);
+                hipLaunchKernelGGL(tslu_finish_kernel, dim3((mp + 255) / 256), dim3(256), 0, s,
+                                   W + (long)j * ldw + j, ldw, mp, w, (const float *)mw.u11,
+                                   (const int *)mw.win, w < mp ? w : mp);
+                FLX_CHECK(hipGetLastError());
+            }
+            FLX_CHECK(rowperm_rm<float>(s, mw.prof, W, ldw, n, n, n, n, mw.list, 1, mw.perm,
+                                        nullptr));
+            const int cr = J + jb - (j + w);
+            if (cr > 0) {
+                FLX_CHECK(trsm_rm_llu<float>(s, mw.prof, W + (long)j * ldw + j, ldw,
+                                             W + (long)j * ldw + j + w, ldw, w, cr));
+                FLX_CHECK(lp_update(s, mw, MIX_FP32, W, ldw, j + w, n - j - w, j + w, cr, j, w));
+            }
+        }
+        const int nr = n - (J + jb);
+        if (nr > 0) {
+            FLX_CHECK(trsm_rm_llu<float>(s, mw.prof, W + (long)J * ldw + J, ldw,
+                                         W + (long)J * ldw + J + jb, ldw, jb, nr));
+            FLX_CHECK(lp_update(s, mw, prec, W, ldw, J + jb, nr, J + jb, nr, J, jb));
+        }
+    }
+    return hipSuccess;
+}
+
+// ---------------------------------------------------------------------------
+// FP64 column-major A -> FP32 row-major W, W[i][j] = A(i,j) * dr[i] * dc[j]
+// (scales optional).  flag[0] is set when a finite value exceeds FP32's
+// range (the overflow test of LAPACK's dlag2s).
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void conv_d2s_rm_kernel(const double *__restrict__ A, long lda,
+                                                          int n, const double *__restrict__ dr,
+                                                          const double *__restrict__ dc,
+                                                          float *__restrict__ W, long ldw,
+                                                          int *flag)
+{
+    __shared__ float t[32][33];
+    const int bi = blockIdx.x * 32, bj = blockIdx.y * 32;
+    const int tx = threadIdx.x & 31, ty = threadIdx.x >> 5;
+    bool ovf = false;
+    for (int q = ty; q < 32; q += 8) {
+        const int i = bi + tx, j = bj + q;
+        if (i < n && j < n) {
+            double v = A[(long)j * lda + i];
+            if (dr) v *= dr[i];
+            if (dc) v *= dc[j];
+            ovf |= fabs(v) > (double)FLT_MAX;
+            t[q][tx] = (float)v;
+        }
+    }
+    __syncthreads();
+    for (int q = ty; q < 32; q += 8) {
+        const int i = bi + q, j = bj + tx;
+        if (i < n && j < n) W[(long)i * ldw + j] = t[tx][q];
+    }
+    if (ovf) atomicOr(flag, 1);
+}
+
+hipError_t conv_d2s_rm(hipStream_t s, const double *A, long lda, int n, const double *dr,
+                       const double *dc, float *W, long ldw, int *flag)
+{
+    if (n <= 0) return hipSuccess;
+    ProfScope ps(tl_prof, s, PROF_TRANSPOSE, 0.0, 12.0 * n * n);
+    dim3 g((n + 31) / 32, (n + 31) / 32);
+    hipLaunchKernelGGL(conv_d2s_rm_kernel, g, dim3(256), 0, s, A, lda, n, dr, dc, W, ldw, flag);
+    return hipGetLastError();
+}
+
+} // namespace flx

@@ -1,0 +1,483 @@
+// solve_mixed.hip -- mixed-precision solver: low-precision LU + FP64
+// refinement, with FP64 LU as the fallback.
+//
+//   flx_dsgesv / dsgesv_  netlib dsgesv semantics (dsgesv.f:131-337, the
+//                         routine FLENS reaches as cxxlapack::sgesv,
+//                         cxxlapack/netlib/interface/lapack.in.h:10611-10623):
+//                         FP32 factor, classic refinement, ITERMAX = 30,
+//                         stop when ||r||_inf <= ||x||_inf ||A||_inf eps sqrt(n)
+//                         for every column, ITER < 0 codes for the fallback.
+//   flx_dgesv_mixed       the same with a choice of factor precision (FP32,
+//                         BF16 or FP16 trailing updates) and of refinement
+//                         (classic or GMRES-IR), stopping on the normwise
+//                         backward error ||r|| / (||A|| ||x|| + ||b||) <= tol.
+//
+// FP64 appears only where the refinement needs it: residuals r = b - A x on
+// the caller's A, solution updates and norms.  The factor is applied as the
+// preconditioner M^{-1} (ir.hip).
+#include <chrono>
+#include <cstring>
+#include <cmath>
+#include <vector>
+#include "context.h"
+
+using namespace flx;
+
+#define FLX_RET(x)                                                            \
+    do {                                                                      \
+        int r_ = (x);                                                         \
+        if (r_) return r_;                                                    \
+    } while (0)
+
+namespace {
+
+constexpr double EPS64 = 1.1102230246251565e-16;   // 2^-53, lamch(Eps)
+constexpr int SLOT_ANRM = 0, SLOT_VN = 1, SLOT_T = 2, SLOT_COL = 4;   // scalar slots
+
+struct Opts {
+    int prec = MIX_FP32, refine = FLX_REFINE_CLASSIC, itermax = 30, gmres_m = 40;
+    double tol = 0.0, gmres_tol = 1e-10;
+    int fallback = 1, nb = 256;
+    bool dsgesv = false;
+};
+
+double bits_to_double(unsigned long long b)
+{
+    double d;
+    memcpy(&d, &b, sizeof d);
+    return d;
+}
+
+struct Solver {
+    flx_context *h;
+    hipStream_t s;
+    int n, nrhs;
+    const double *A;
+    int lda;
+    const double *B;
+    int ldb;
+    double *X;
+    int ldx;
+    Opts o;
+    PrecFactor f;
+    MixWork mw;
+    long ldw = 0;
+    unsigned long long *scal = nullptr;
+    std::vector<unsigned long long> hs;   // host copy of the scalar slots
+
+    unsigned long long *slot(int k) { return scal + k; }
+    unsigned long long *col_slot(int c, int q) { return scal + SLOT_COL + 3 * c + q; }
+
+    hipError_t read_slots()
+    {
+        hipError_t e = hipMemcpyAsync(hs.data(), scal, sizeof(unsigned long long) * hs.size(),
+                                      hipMemcpyDeviceToHost, s);
+        if (e != hipSuccess) return e;
+        return hipStreamSynchronize(s);
+    }
+    double hslot(int k) const { return bits_to_double(hs[k]); }
+    double hcol(int c, int q) const { return bits_to_double(hs[SLOT_COL + 3 * c + q]); }
+
+    int alloc()
+    {
+        MixBufs &m = h->mix;
+        ldw = round_ld(n);
+        const size_t nodes = tslu_nodes(n);
+        FLX_TRY(ensure(h, m.W32, sizeof(float) * (size_t)ldw * n));
+        FLX_TRY(ensure(h, m.perm, sizeof(int) * n));
+        FLX_TRY(ensure(h, m.cand, sizeof(int) * nodes * 32));
+        FLX_TRY(ensure(h, m.ccount, sizeof(int) * nodes));
+        FLX_TRY(ensure_zeroed(h, m.counters, sizeof(unsigned) * nodes));
+        FLX_TRY(ensure(h, m.u11, sizeof(float) * 32 * 32));
+        FLX_TRY(ensure(h, m.win, sizeof(int) * 32));
+        FLX_TRY(ensure(h, m.list, sizeof(int) * PANEL_LIST_SLICE));
+        FLX_TRY(ensure(h, m.zero, sizeof(int) * 4));
+        FLX_TRY(ensure(h, m.lpmax, sizeof(unsigned) * 4));
+        if (o.prec != MIX_FP32) {
+            FLX_TRY(ensure(h, m.la, 2 * lp_pack_elems(n, o.nb)));
+            FLX_TRY(ensure(h, m.ub, 2 * lp_pack_elems(n, o.nb)));
+        }
+        FLX_TRY(ensure(h, m.inv_l, sizeof(float) * trsv_inv_floats(n)));
+        FLX_TRY(ensure(h, m.inv_u, sizeof(float) * trsv_inv_floats(n)));
+        FLX_TRY(ensure(h, m.y32, sizeof(float) * n));
+        const size_t nflags = (size_t)(n + 63) / 64;
+        if (m.flags.cap < sizeof(unsigned) * nflags) m.epoch = 0;
+        FLX_TRY(ensure_zeroed(h, m.flags, sizeof(unsigned) * nflags));
+        FLX_TRY(ensure(h, m.err, sizeof(int) * 4));
+        FLX_TRY(ensure(h, m.dr, sizeof(double) * n));
+        FLX_TRY(ensure(h, m.dc, sizeof(double) * n));
+        FLX_TRY(ensure(h, m.part, sizeof(double) * ir_part_doubles(n)));
+        const int nslots = SLOT_COL + 3 * nrhs;
+        FLX_TRY(ensure(h, m.scal, sizeof(unsigned long long) * nslots));
+        FLX_TRY(ensure(h, m.R, sizeof(double) * (size_t)n * nrhs));
+        if (o.refine == FLX_REFINE_GMRES) {
+            FLX_TRY(ensure(h, m.V, sizeof(double) * (size_t)n * (o.gmres_m + 1)));
+            FLX_TRY(ensure(h, m.w, sizeof(double) * n));
+            FLX_TRY(ensure(h, m.t, sizeof(double) * n));
+            FLX_TRY(ensure(h, m.zeros, sizeof(double) * n));
+            FLX_TRY(ensure(h, m.mpart, sizeof(double) * mdot_part_doubles(n, o.gmres_m + 1)));
+            FLX_TRY(ensure(h, m.hbuf, sizeof(double) * 4 * (o.gmres_m + 2)));
+        }
+        scal = (unsigned long long *)m.scal.p;
+        hs.assign(nslots, 0ull);
+        mw.perm = (int *)m.perm.p;
+        mw.cand = (int *)m.cand.p;
+        mw.ccount = (int *)m.ccount.p;
+        mw.counters = (unsigned *)m.counters.p;
+        mw.u11 = (float *)m.u11.p;
+        mw.win = (int *)m.win.p;
+        mw.list = (int *)m.list.p;
+        mw.zero = (int *)m.zero.p;
+        mw.lpmax = (unsigned *)m.lpmax.p;
+        mw.la = m.la.p;
+        mw.ub = m.ub.p;
+        mw.prof = h->prof;
+        f.n = n;
+        f.W = (const float *)m.W32.p;
+        f.ldw = ldw;
+        f.perm = (const int *)m.perm.p;
+        f.dr = (o.prec == MIX_FP16) ? (const double *)m.dr.p : nullptr;
+        f.dc = (o.prec == MIX_FP16) ? (const double *)m.dc.p : nullptr;
+        f.inv_l = (const float *)m.inv_l.p;
+        f.inv_u = (const float *)m.inv_u.p;
+        f.y32 = (float *)m.y32.p;
+        f.flags = (unsigned *)m.flags.p;
+        f.epoch = &m.epoch;
+        f.err = (int *)m.err.p;
+        return FLX_SUCCESS;
+    }
+
+    double *R(int c) { return (double *)h->mix.R.p + (size_t)c * n; }
+    const double *Bc(int c) { return B + (size_t)c * ldb; }
+    double *Xc(int c) { return X + (size_t)c * ldx; }
+
+    // factor: 0 ok, -2 FP32 overflow in A or B, -3 exact zero pivot
+    int factor(int *code)
+    {
+        MixBufs &m = h->mix;
+        FLX_TRY(zero_slots(s, scal, SLOT_COL + 3 * nrhs));
+        FLX_TRY(hipMemsetAsync(m.err.p, 0, sizeof(int) * 4, s));
+        FLX_TRY(fill_int(s, mw.zero, 1, INT_MAX));
+        FLX_TRY(norm_inf(s, A, lda, n, (double *)m.part.p, slot(SLOT_ANRM)));
+        for (int c = 0; c < nrhs; ++c) FLX_TRY(vmax(s, Bc(c), n, col_slot(c, 2)));
+        if (o.prec == MIX_FP16)
+            FLX_TRY(equilibrate(s, A, lda, n, (double *)m.part.p, (double *)m.dr.p,
+                                (double *)m.dc.p));
+        FLX_TRY(conv_d2s_rm(s, A, lda, n, f.dr, f.dc, (float *)m.W32.p, ldw, (int *)m.err.p + 1));
+        FLX_TRY(getrf_rm_mixed(s, mw, (float *)m.W32.p, ldw, n, o.prec, o.nb));
+        FLX_TRY(tri_inv_rm(s, (const float *)m.W32.p, ldw, n, (float *)m.inv_l.p,
+                           (float *)m.inv_u.p));
+        int flags[4] = {0, 0, 0, 0}, zero = INT_MAX;
+        FLX_TRY(hipMemcpyAsync(flags, m.err.p, sizeof flags, hipMemcpyDeviceToHost, s));
+        FLX_TRY(hipMemcpyAsync(&zero, mw.zero, sizeof(int), hipMemcpyDeviceToHost, s));
+        FLX_TRY(read_slots());
+        *code = 0;
+        bool bovf = false;
+        for (int c = 0; c < nrhs; ++c) bovf |= hcol(c, 2) > 3.4028234663852886e38;
+        if (flags[1] || bovf) *code = -2;
+        else if (zero != INT_MAX) *code = -3;
+        return FLX_SUCCESS;
+    }
+
+    // residuals and norms of every column; returns 1 when all converged
+    int check(bool *conv, double *eta_max)
+    {
+        FLX_TRY(zero_slots(s, scal + SLOT_COL, 3 * nrhs));
+        for (int c = 0; c < nrhs; ++c) {
+            FLX_TRY(gemv_resid(s, A, lda, n, Xc(c), Bc(c), R(c), (double *)h->mix.part.p,
+                               col_slot(c, 0)));
+            FLX_TRY(vmax(s, Xc(c), n, col_slot(c, 1)));
+            FLX_TRY(vmax(s, Bc(c), n, col_slot(c, 2)));
+        }
+        FLX_TRY(read_slots());
+        const double anrm = hslot(SLOT_ANRM);
+        const double cte = anrm * EPS64 * std::sqrt((double)n);
+        bool all = true;
+        double em = 0.0;
+        for (int c = 0; c < nrhs; ++c) {
+            const double rn = hcol(c, 0), xn = hcol(c, 1), bn = hcol(c, 2);
+            const double denom = anrm * xn + bn;
+            const double eta = denom > 0.0 ? rn / denom : (rn > 0.0 ? INFINITY : 0.0);
+            em = std::isnan(eta) || std::isnan(em) ? NAN : std::max(em, eta);
+            const bool ok = o.dsgesv ? !(rn > xn * cte) && !std::isnan(rn) : eta <= o.tol;
+            all = all && ok;
+        }
+        *conv = all;
+        *eta_max = em;
+        return FLX_SUCCESS;
+    }
+
+    int precond(const double *v, double *z, int add)
+    {
+        FLX_TRY(zero_slots(s, slot(SLOT_VN), 1));
+        FLX_TRY(prec_apply(s, f, v, z, add, slot(SLOT_VN), nullptr));
+        return FLX_SUCCESS;
+    }
+
+    // classic refinement: returns iterations (>= 0) or -itermax-1
+    int classic(int *iters, double *eta)
+    {
+        bool conv = false;
+        for (int c = 0; c < nrhs; ++c) {
+            int st = precond(Bc(c), Xc(c), 0);
+            if (st) return st;
+        }
+        int st = check(&conv, eta);
+        if (st) return st;
+        if (conv) { *iters = 0; return FLX_SUCCESS; }
+        for (int it = 1; it <= o.itermax; ++it) {
+            for (int c = 0; c < nrhs; ++c) {
+                st = precond(R(c), Xc(c), 1);
+                if (st) return st;
+            }
+            st = check(&conv, eta);
+            if (st) return st;
+            if (conv) { *iters = it; return FLX_SUCCESS; }
+        }
+        *iters = -o.itermax - 1;
+        return FLX_SUCCESS;
+    }
+
+    // left-preconditioned GMRES on M^{-1} A d = M^{-1} r for column c; x_c += d
+    int gmres(int c, int *inner)
+    {
+        MixBufs &m = h->mix;
+        const int mm = o.gmres_m;
+        double *V = (double *)m.V.p, *w = (double *)m.w.p, *t = (double *)m.t.p;
+        double *hb = (double *)m.hbuf.p;   // [0..mm+1) h1, [mm+1..) h2, then norms
+        double *mp = (double *)m.mpart.p;
+        std::vector<double> H((size_t)(mm + 1) * mm, 0.0), g(mm + 1, 0.0), cs(mm), sn(mm);
+        std::vector<double> hh(2 * (mm + 1) + 2);
+        FLX_RET(precond(R(c), V, 0));
+        FLX_TRY(mdot(s, V, n, n, 1, V, mp, hb + 2 * (mm + 1)));
+        FLX_TRY(hipMemcpyAsync(hh.data(), hb + 2 * (mm + 1), sizeof(double), hipMemcpyDeviceToHost, s));
+        FLX_TRY(hipStreamSynchronize(s));
+        const double beta = std::sqrt(hh[0]);
+        if (!(beta > 0.0) || !std::isfinite(beta)) { *inner = 0; return FLX_SUCCESS; }
+        FLX_TRY(vscale_copy(s, V, 1.0 / beta, V, n));
+        g[0] = beta;
+        int k = 0;
+        for (; k < mm; ++k) {
+            double *vk = V + (size_t)k * n;
+            FLX_TRY(gemv_resid(s, A, lda, n, vk, (const double *)m.zeros.p, t,
+                               (double *)m.part.p, slot(SLOT_T)));       // t = -A v_k
+            FLX_RET(precond(t, w, 0));                                    // w = -M^{-1} A v_k
+            FLX_TRY(vscale_copy(s, w, -1.0, w, n));
+            // classical Gram-Schmidt, twice
+            FLX_TRY(mdot(s, V, n, n, k + 1, w, mp, hb));
+            FLX_TRY(maxpy(s, V, n, n, k + 1, hb, w));
+            FLX_TRY(mdot(s, V, n, n, k + 1, w, mp, hb + (mm + 1)));
+            FLX_TRY(maxpy(s, V, n, n, k + 1, hb + (mm + 1), w));
+            FLX_TRY(mdot(s, w, n, n, 1, w, mp, hb + 2 * (mm + 1)));
+            FLX_TRY(hipMemcpyAsync(hh.data(), hb, sizeof(double) * (2 * (mm + 1) + 1),
+                                   hipMemcpyDeviceToHost, s));
+            FLX_TRY(hipStreamSynchronize(s));
+            double *hk = &H[(size_t)k * (mm + 1)];
+            for (int i = 0; i <= k; ++i) hk[i] = hh[i] + hh[mm + 1 + i];
+            const double hn = std::sqrt(hh[2 * (mm + 1)]);
+            hk[k + 1] = hn;
+            if (hn > 0.0 && std::isfinite(hn))
+                FLX_TRY(vscale_copy(s, w, 1.0 / hn, V + (size_t)(k + 1) * n, n));
+            // Givens rotations
+            for (int i = 0; i < k; ++i) {
+                const double a = hk[i], b = hk[i + 1];
+                hk[i] = cs[i] * a + sn[i] * b;
+                hk[i + 1] = -sn[i] * a + cs[i] * b;
+            }
+            const double a = hk[k], b = hk[k + 1];
+            const double r = std::hypot(a, b);
+            cs[k] = r > 0.0 ? a / r : 1.0;
+            sn[k] = r > 0.0 ? b / r : 0.0;
+            hk[k] = r;
+            hk[k + 1] = 0.0;
+            g[k + 1] = -sn[k] * g[k];
+            g[k] = cs[k] * g[k];
+            if (!std::isfinite(g[k + 1])) return FLX_SUCCESS;   // leaves x unchanged
+            if (std::fabs(g[k + 1]) <= o.gmres_tol * beta || !(hn > 0.0)) { ++k; break; }
+        }
+        // y = H^{-1} g, x_c += V y
+        std::vector<double> y(k, 0.0);
+        for (int i = k - 1; i >= 0; --i) {
+            double v = g[i];
+            for (int j = i + 1; j < k; ++j) v -= H[(size_t)j * (mm + 1) + i] * y[j];
+            const double d = H[(size_t)i * (mm + 1) + i];
+            y[i] = d != 0.0 ? v / d : 0.0;
+            y[i] = -y[i];   // maxpy subtracts
+        }
+        FLX_TRY(hipMemcpyAsync(hb, y.data(), sizeof(double) * k, hipMemcpyHostToDevice, s));
+        FLX_TRY(maxpy(s, V, n, n, k, hb, Xc(c)));
+        *inner = k;
+        return FLX_SUCCESS;
+    }
+
+    int gmres_ir(int *iters, int *inner_total, double *eta)
+    {
+        MixBufs &m = h->mix;
+        FLX_TRY(hipMemsetAsync(m.zeros.p, 0, sizeof(double) * n, s));
+        for (int c = 0; c < nrhs; ++c) {
+            int st = precond(Bc(c), Xc(c), 0);
+            if (st) return st;
+        }
+        bool conv = false;
+        *inner_total = 0;
+        for (int it = 0; it <= o.itermax; ++it) {
+            int st = check(&conv, eta);
+            if (st) return st;
+            if (conv) { *iters = it; return FLX_SUCCESS; }
+            if (it == o.itermax || std::isnan(*eta)) break;
+            for (int c = 0; c < nrhs; ++c) {
+                int inner = 0;
+                st = gmres(c, &inner);
+                if (st) return st;
+                *inner_total += inner;
+            }
+        }
+        *iters = -o.itermax - 1;
+        return FLX_SUCCESS;
+    }
+
+    // FP64 LU + solve (A overwritten, X = solution); returns LAPACK info
+    int fallback64(int *info, int *ipiv)
+    {
+        FLX_TRY(copy_cm(s, B, ldb, X, ldx, n, nrhs));
+        FLX_TRY(ensure(h, h->info, sizeof(int)));
+        FLX_TRY(ensure(h, h->ipiv, sizeof(int) * n));
+        int *piv = ipiv ? ipiv : (int *)h->ipiv.p;
+        int st = dgetrf_dev(h, n, n, const_cast<double *>(A), lda, piv, (int *)h->info.p, true);
+        if (st) return st;
+        FLX_TRY(hipMemcpyAsync(info, h->info.p, sizeof(int), hipMemcpyDeviceToHost, s));
+        FLX_TRY(hipStreamSynchronize(s));
+        if (*info == 0) return dgetrs_dev(h, 0, n, nrhs, A, lda, piv, (const int *)h->perm.p, X, ldx);
+        return FLX_SUCCESS;
+    }
+
+    // LAPACK-style interchanges from the factor's row order (host, O(n))
+    int perm_to_ipiv(int *ipiv_dev)
+    {
+        std::vector<int> perm(n), ip(n), cur(n), where(n);
+        FLX_TRY(hipMemcpyAsync(perm.data(), mw.perm, sizeof(int) * n, hipMemcpyDeviceToHost, s));
+        FLX_TRY(hipStreamSynchronize(s));
+        for (int i = 0; i < n; ++i) { cur[i] = i; where[i] = i; }
+        for (int i = 0; i < n; ++i) {
+            const int p = where[perm[i]];
+            ip[i] = p + 1;
+            const int ri = cur[i], rp = cur[p];
+            cur[i] = rp; cur[p] = ri;
+            where[rp] = i; where[ri] = p;
+        }
+        FLX_TRY(hipMemcpyAsync(ipiv_dev, ip.data(), sizeof(int) * n, hipMemcpyHostToDevice, s));
+        FLX_TRY(hipStreamSynchronize(s));
+        return FLX_SUCCESS;
+    }
+};
+
+bool valid_dims(int n, int nrhs, int lda, int ldb, int ldx)
+{
+    const int m1 = n > 1 ? n : 1;
+    return n >= 0 && nrhs >= 0 && lda >= m1 && ldb >= m1 && ldx >= m1;
+}
+
+} // namespace
+
+static int run_mixed(flx_context *h, int n, int nrhs, double *A, int lda, const double *B,
+                     int ldb, double *X, int ldx, const Opts &o, flx_mixed_stats *st,
+                     int *ipiv, int *iter_out, int *info_out)
+{
+    using clk = std::chrono::steady_clock;
+    Solver S{h, h->stream, n, nrhs, A, lda, B, ldb, X, ldx, o, {}, {}};
+    if (o.tol <= 0.0) S.o.tol = n * EPS64;
+    int r = S.alloc();
+    if (r) return r;
+    const auto t0 = clk::now();
+    int code = 0;
+    r = S.factor(&code);
+    if (r) return r;
+    const auto t1 = clk::now();
+    int iters = 0, inner = 0;
+    double eta = NAN;
+    if (code == 0) {
+        r = (o.refine == FLX_REFINE_GMRES) ? S.gmres_ir(&iters, &inner, &eta)
+                                           : S.classic(&iters, &eta);
+        if (r) return r;
+        int terr[4] = {0, 0, 0, 0};
+        FLX_TRY(hipMemcpy(terr, h->mix.err.p, sizeof terr, hipMemcpyDeviceToHost));
+        if (terr[0]) return FLX_SYNC_TIMEOUT;
+        code = iters;
+    }
+    const auto t2 = clk::now();
+    int info = 0, fell = 0;
+    if (code < 0 && (o.dsgesv || o.fallback)) {
+        r = S.fallback64(&info, ipiv);
+        if (r) return r;
+        fell = 1;
+        if (info == 0) {
+            bool conv = false;
+            Opts keep = S.o;
+            S.o.dsgesv = false;
+            r = S.check(&conv, &eta);
+            S.o = keep;
+            if (r) return r;
+        }
+    } else if (code >= 0 && ipiv) {
+        r = S.perm_to_ipiv(ipiv);
+        if (r) return r;
+    }
+    const auto t3 = clk::now();
+    if (iter_out) *iter_out = code;
+    if (info_out) *info_out = info;
+    if (st) {
+        st->iter = code;
+        st->inner_iters = inner;
+        st->fallback = fell;
+        st->info = info;
+        st->backward_error = eta;
+        st->anorm = S.hslot(SLOT_ANRM);
+        st->factor_ms = std::chrono::duration<double, std::milli>(t1 - t0).count();
+        st->refine_ms = std::chrono::duration<double, std::milli>(t2 - t1).count();
+        st->fallback_ms = std::chrono::duration<double, std::milli>(t3 - t2).count();
+    }
+    return FLX_SUCCESS;
+}
+
+extern "C" int flx_dgesv_mixed(flx_handle h, int n, int nrhs, double *A, int lda,
+                               const double *B, int ldb, double *X, int ldx,
+                               const flx_mixed_opts *opt, flx_mixed_stats *st)
+{
+    ProfBind pb_(h);
+    if (!h || !valid_dims(n, nrhs, lda, ldb, ldx)) return FLX_INVALID_ARG;
+    Opts o;
+    if (opt) {
+        if (opt->prec < 0 || opt->prec > 2 || opt->refine < 0 || opt->refine > 1 ||
+            opt->itermax < 0 || opt->gmres_m < 0 || opt->gmres_m > 200)
+            return FLX_INVALID_ARG;
+        o.prec = opt->prec;
+        o.refine = opt->refine;
+        o.itermax = opt->itermax;
+        o.gmres_m = opt->gmres_m > 0 ? opt->gmres_m : 40;
+        o.tol = opt->tol;
+        o.gmres_tol = opt->gmres_tol > 0.0 ? opt->gmres_tol : 1e-10;
+        o.fallback = opt->fallback;
+        o.nb = opt->nb > 0 ? opt->nb : 256;
+    }
+    if (st) memset(st, 0, sizeof *st);
+    if (n == 0 || nrhs == 0) return FLX_SUCCESS;
+    if (n > 65536) return FLX_UNSUPPORTED;
+    if (o.fallback && n > 16384) return FLX_UNSUPPORTED;
+    return run_mixed(h, n, nrhs, A, lda, B, ldb, X, ldx, o, st, nullptr, nullptr, nullptr);
+}
+
+extern "C" int flx_dsgesv(flx_handle h, int n, int nrhs, double *A, int lda, int *ipiv,
+                          const double *B, int ldb, double *X, int ldx, int *iter, int *info)
+{
+    ProfBind pb_(h);
+    if (!h || !valid_dims(n, nrhs, lda, ldb, ldx)) return FLX_INVALID_ARG;
+    if (iter) *iter = 0;
+    if (info) *info = 0;
+    if (n == 0) return FLX_SUCCESS;
+    if (n > 16384) return FLX_UNSUPPORTED;
+    Opts o;
+    o.dsgesv = true;
+    o.itermax = 30;
+    o.nb = h->opt.nb;
+    return run_mixed(h, n, nrhs, A, lda, B, ldb, X, ldx, o, nullptr, ipiv, iter, info);
+}

@@ -108,6 +108,51 @@ int flx_dlaswp(flx_handle h, int n, double *A, int lda, int k1, int k2,
 int flx_fill_uniform(flx_handle h, double *A, int m, int n, int lda,
                      uint64_t seed, uint64_t offset);
 
+/* ------------------------------------------- mixed-precision solvers */
+/* Mixed-precision solve of A X = B (device pointers, column-major; A and B
+ * are not modified unless the FP64 fallback runs, which overwrites A with
+ * its FP64 factors).  The factor is computed in FP32 with tournament
+ * pivoting and a trailing update in the chosen precision; FP64 is used for
+ * residuals, updates and norms.  Replaces the mixed-precision route of the
+ * reference, cxxlapack::sgesv -> dsgesv_ (cxxlapack/netlib/interface/
+ * lapack.in.h:10611-10623, netlib dsgesv.f:131-337), and extends it with
+ * BF16/FP16 factors and GMRES-IR. */
+enum flx_mixed_prec { FLX_MIXED_FP32 = 0, FLX_MIXED_BF16 = 1, FLX_MIXED_FP16 = 2 };
+enum flx_refine { FLX_REFINE_CLASSIC = 0, FLX_REFINE_GMRES = 1 };
+typedef struct flx_mixed_opts {
+    int prec;          /* enum flx_mixed_prec (FP16 adds power-of-two row and
+                          column equilibration and per-step operand scaling) */
+    int refine;        /* enum flx_refine */
+    int itermax;       /* outer refinement steps (30 in dsgesv) */
+    int gmres_m;       /* Krylov vectors per GMRES solve (0: 40) */
+    double tol;        /* stop at ||r||/(||A|| ||x|| + ||b||) <= tol (0: n*2^-53) */
+    double gmres_tol;  /* relative GMRES residual (0: 1e-10) */
+    int fallback;      /* 1: FP64 LU + solve when refinement fails */
+    int nb;            /* outer block of the factorisation (0: 256) */
+} flx_mixed_opts;
+typedef struct flx_mixed_stats {
+    int iter;              /* refinement steps; < 0: failed (dsgesv ITER codes) */
+    int inner_iters;       /* GMRES iterations in total */
+    int fallback;          /* 1 when the FP64 fallback produced X */
+    int info;              /* INFO of the FP64 fallback factorisation */
+    double backward_error; /* max over columns, inf-norm, at exit */
+    double anorm;          /* ||A||_inf */
+    double factor_ms, refine_ms, fallback_ms;
+} flx_mixed_stats;
+int flx_dgesv_mixed(flx_handle h, int n, int nrhs, double *A, int lda,
+                    const double *B, int ldb, double *X, int ldx,
+                    const flx_mixed_opts *opt, flx_mixed_stats *stats);
+
+/* dsgesv semantics on device pointers: FP32 factor + classic refinement
+ * (ITERMAX 30, stop when ||r||_inf <= ||x||_inf ||A||_inf eps sqrt(n)),
+ * *iter / *info (host ints) as LAPACK's ITER / INFO: ITER = -2 FP32
+ * overflow, -3 zero pivot in the FP32 factor, -31 no convergence, each
+ * followed by the FP64 LU + solve.  ipiv (device, n ints) receives the
+ * interchanges of the factor that produced X. */
+int flx_dsgesv(flx_handle h, int n, int nrhs, double *A, int lda, int *ipiv,
+               const double *B, int ldb, double *X, int ldx, int *iter,
+               int *info);
+
 /* --------------------------------------------------- FLENS plugin seams */
 /* CBLAS seam, cxxblas/drivers/cblas.h:798-805; enum values :22-27
  * (Order 101/102, Trans 111/112/113).  Caller: cxxblas::gemm<int> for
@@ -146,6 +191,13 @@ void dgesv_(const int *N, const int *NRHS, double *A, const int *LDA,
  * (flens/lapack/la/laswp.tcc:105-118). */
 void dlaswp_(const int *N, double *A, const int *LDA, const int *K1,
              const int *K2, const int *IPIV, const int *INCX);
+
+/* Fortran-LAPACK seam for the mixed-precision solver,
+ * cxxlapack/netlib/interface/lapack.in.h:10611-10623.  WORK and SWORK are
+ * accepted and unused (the library keeps its own device workspace). */
+void dsgesv_(const int *N, const int *NRHS, double *A, const int *LDA,
+             int *IPIV, double *B, const int *LDB, double *X, const int *LDX,
+             double *WORK, float *SWORK, int *ITER, int *INFO);
 
 /* Error hook called before a negative INFO is returned, as the reference's
  * LAPACK_ERROR (flens/lapack/interface/include/config.h:40-45).  libflx
