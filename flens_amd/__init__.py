@@ -9,6 +9,10 @@ by ``flens_amd.build``); this module is a thin ctypes mirror of its C ABI
 * ``lapack.trs``    -- flens::lapack::trs     (flens/lapack/ge/trs.tcc:151-252)
 * ``lapack.sv``     -- flens::lapack::sv      (flens/lapack/ge/sv.tcc:115-219)
 * ``lapack.laswp``  -- flens::lapack::laswp   (flens/lapack/la/laswp.tcc:126-186)
+* ``lapack.dsgesv`` -- cxxlapack::sgesv -> dsgesv_ (cxxlapack/netlib/interface/
+  lapack.in.h:10611-10623): FP32 factor + FP64 refinement, LAPACK semantics
+* ``lapack.sv_mixed`` -- the mixed-precision solver with a choice of factor
+  precision (FP32 / BF16 / FP16-scaled) and refinement (classic / GMRES-IR)
 
 Two data paths, exactly as in the C ABI:
 
@@ -36,8 +40,29 @@ EXPORTS = [
     "flx_create", "flx_destroy", "flx_set_stream", "flx_sync", "flx_status_string",
     "flx_set_lu_blocking", "flx_dgemm", "flx_dgetrf", "flx_dgetrs", "flx_dgesv",
     "flx_dlaswp", "flx_fill_uniform", "flx_profile_start", "flx_profile_stop",
-    "cblas_dgemm", "cblas_dtrsm", "dgetrf_", "dgetrs_", "dgesv_", "dlaswp_", "xerbla_",
+    "flx_dgesv_mixed", "flx_dsgesv",
+    "cblas_dgemm", "cblas_dtrsm", "dgetrf_", "dgetrs_", "dgesv_", "dlaswp_", "dsgesv_",
+    "xerbla_",
 ]
+
+PREC = {"fp32": 0, "bf16": 1, "fp16": 2}
+REFINE = {"classic": 0, "gmres": 1}
+
+
+class MixedOpts(C.Structure):
+    _fields_ = [("prec", C.c_int), ("refine", C.c_int), ("itermax", C.c_int),
+                ("gmres_m", C.c_int), ("tol", C.c_double), ("gmres_tol", C.c_double),
+                ("fallback", C.c_int), ("nb", C.c_int), ("graph", C.c_int)]
+
+
+class MixedStats(C.Structure):
+    _fields_ = [("iter", C.c_int), ("inner_iters", C.c_int), ("fallback", C.c_int),
+                ("info", C.c_int), ("backward_error", C.c_double), ("anorm", C.c_double),
+                ("factor_ms", C.c_double), ("refine_ms", C.c_double),
+                ("fallback_ms", C.c_double)]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
 
 _dp = C.POINTER(C.c_double)
 _ip = C.POINTER(C.c_int)
@@ -85,6 +110,12 @@ def lib():
     L.dgesv_.restype = None
     L.dlaswp_.argtypes = [_ip, _dp, _ip, _ip, _ip, _ip, _ip]
     L.dlaswp_.restype = None
+    L.flx_dgesv_mixed.argtypes = [_vp, _i, _i, _vp, _i, _vp, _i, _vp, _i,
+                                  C.POINTER(MixedOpts), C.POINTER(MixedStats)]
+    L.flx_dsgesv.argtypes = [_vp, _i, _i, _vp, _i, _vp, _vp, _i, _vp, _i, _ip, _ip]
+    L.dsgesv_.argtypes = [_ip, _ip, _dp, _ip, _ip, _dp, _ip, _dp, _ip, _dp,
+                          C.POINTER(C.c_float), _ip, _ip]
+    L.dsgesv_.restype = None
     _lib = L
     return L
 
@@ -268,6 +299,44 @@ class lapack:
         _check(lib().flx_dgesv(h.ptr, A.n, B.n, _vp(A.ptr), A.ld, _vp(piv.data_ptr()),
                                _vp(B.ptr), B.ld, C.byref(info)), "flx_dgesv")
         return piv, info.value
+
+    @staticmethod
+    def dsgesv(A, B, X, handle: Handle | None = None):
+        """LAPACK dsgesv: X = A^{-1} B by an FP32 factor and FP64 classic
+        refinement, FP64 LU as the fallback (A is overwritten only then).
+        Returns (piv, iter, info)."""
+        if isinstance(A, np.ndarray):
+            n, nrhs = A.shape[0], B.shape[1]
+            piv = np.zeros(max(1, n), dtype=np.int32)
+            it, info = C.c_int(0), C.c_int(0)
+            lib().dsgesv_(C.byref(C.c_int(n)), C.byref(C.c_int(nrhs)), A.ctypes.data_as(_dp),
+                          C.byref(C.c_int(max(1, n))), piv.ctypes.data_as(_ip),
+                          B.ctypes.data_as(_dp), C.byref(C.c_int(max(1, n))),
+                          X.ctypes.data_as(_dp), C.byref(C.c_int(max(1, n))), None, None,
+                          C.byref(it), C.byref(info))
+            return piv[:n], it.value, info.value
+        h = handle or default_handle()
+        piv = _int_buffer(A.n)
+        it, info = C.c_int(0), C.c_int(0)
+        _check(lib().flx_dsgesv(h.ptr, A.n, B.n, _vp(A.ptr), A.ld, _vp(piv.data_ptr()),
+                                _vp(B.ptr), B.ld, _vp(X.ptr), X.ld, C.byref(it), C.byref(info)),
+               "flx_dsgesv")
+        return piv, it.value, info.value
+
+    @staticmethod
+    def sv_mixed(A, B, X, prec="fp32", refine="classic", itermax=30, gmres_m=40, tol=0.0,
+                 gmres_tol=0.0, fallback=True, nb=256, graph=True,
+                 handle: Handle | None = None) -> dict:
+        """Mixed-precision solve of A X = B on device matrices; returns the
+        solver's statistics (iterations, fallback, backward error, times)."""
+        h = handle or default_handle()
+        o = MixedOpts(PREC[prec], REFINE[refine], itermax, gmres_m, tol, gmres_tol,
+                      int(bool(fallback)), nb, int(bool(graph)))
+        st = MixedStats()
+        _check(lib().flx_dgesv_mixed(h.ptr, A.n, B.n, _vp(A.ptr), A.ld, _vp(B.ptr), B.ld,
+                                     _vp(X.ptr), X.ld, C.byref(o), C.byref(st)),
+               "flx_dgesv_mixed")
+        return st.as_dict()
 
     @staticmethod
     def laswp(A: np.ndarray, k1: int, k2: int, piv, incx: int = 1):

@@ -93,6 +93,10 @@ extern "C" int flx_destroy(flx_handle h)
                       &x.err, &x.dr, &x.dc, &x.part, &x.scal, &x.R, &x.V, &x.w, &x.t, &x.zeros,
                       &x.mpart, &x.hbuf})
         free_buf(*b);
+    if (x.graph) hipGraphExecDestroy(x.graph);
+    if (x.gstream) hipStreamDestroy(x.gstream);
+    if (x.ev0) hipEventDestroy(x.ev0);
+    if (x.ev1) hipEventDestroy(x.ev1);
     if (h->wk.exch) hipFree(h->wk.exch);
     if (h->wk.list) hipFree(h->wk.list);
     if (h->wk.flags) hipFree(h->wk.flags);

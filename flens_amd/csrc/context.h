@@ -14,6 +14,11 @@ struct MixBufs {
     DevBuf W32, perm, cand, ccount, counters, u11, win, list, zero, lpmax, la, ub;
     DevBuf inv_l, inv_u, y32, flags, err, dr, dc, part, scal, R, V, w, t, zeros, mpart, hbuf;
     unsigned epoch = 0;
+    // captured factorisation: valid while the key matches
+    hipGraphExec_t graph = nullptr;
+    hipStream_t gstream = nullptr;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    size_t gkey[8] = {0, 0, 0, 0, 0, 0, 0, 0};
 };
 
 struct flx_context {

@@ -160,14 +160,13 @@ struct PrecFactor {
     long ldw = 0;
     const int *perm = nullptr;
     const double *dr = nullptr, *dc = nullptr;   // optional power-of-two scales
-    const float *inv_l = nullptr, *inv_u = nullptr;   // trsv_inv_floats(n) each
     float *y32 = nullptr;      // n
-    unsigned *flags = nullptr; // ceil(n/64), zero-initialised once
-    unsigned *epoch = nullptr; // host counter paired with flags
+    uint64_t *xt = nullptr;    // trsv_words(n) tagged words, zero-initialised once
+    unsigned *epoch = nullptr; // host counter paired with xt
     int *err = nullptr;        // device error flag
 };
 size_t ir_part_doubles(int n);
-size_t trsv_inv_floats(int n);
+size_t trsv_words(int n);
 size_t mdot_part_doubles(int n, int kmax);
 hipError_t gemv_resid(hipStream_t s, const double *A, long lda, int n, const double *x,
                       const double *b, double *r, double *part, unsigned long long *nrm);
@@ -175,9 +174,8 @@ hipError_t norm_inf(hipStream_t s, const double *A, long lda, int n, double *par
                     unsigned long long *nrm);
 hipError_t equilibrate(hipStream_t s, const double *A, long lda, int n, double *part,
                        double *dr, double *dc);
-hipError_t tri_inv_rm(hipStream_t s, const float *W, long ldw, int n, float *inv_l, float *inv_u);
-hipError_t trsv_rm(hipStream_t s, int upper, const float *W, long ldw, int n, const float *inv,
-                   float *y, unsigned *flags, unsigned epoch, int *err);
+hipError_t trsv_rm(hipStream_t s, int upper, const float *W, long ldw, int n, float *y,
+                   uint64_t *xt, unsigned epoch, int *err);
 hipError_t zero_slots(hipStream_t s, unsigned long long *p, int k);
 hipError_t vmax(hipStream_t s, const double *v, int n, unsigned long long *nrm);
 hipError_t prec_apply(hipStream_t s, const PrecFactor &f, const double *v, double *z, int add,

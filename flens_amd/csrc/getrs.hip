@@ -114,6 +114,7 @@ __global__ __launch_bounds__(64) void tri_inv_kernel(const double *__restrict__ 
                                                      int n, double *__restrict__ inv)
 {
     __shared__ double ts[TS_BS][TS_BS + 1];
+    __shared__ double xs[TS_BS][TS_BS + 1];   // xs[i][t]: column t of inv(T)
     const int b = blockIdx.x, kb = b * TS_BS, bs = min(TS_BS, n - kb);
     const int t = threadIdx.x;
     // T(i,q) for the block, identity outside the matrix
@@ -124,24 +125,19 @@ __global__ __launch_bounds__(64) void tri_inv_kernel(const double *__restrict__ 
         ts[i][t] = v;
     }
     __syncthreads();
-    // column t of inv(T): T x = e_t
-    double x[TS_BS];
-#pragma unroll
+    // column t of inv(T): T x = e_t, by substitution (rows only this thread touches)
     for (int s = 0; s < TS_BS; ++s) {
         const int i = FWD ? s : TS_BS - 1 - s;
         double v = (i == t) ? 1.0 : 0.0;
         if (FWD) {
-#pragma unroll
-            for (int q = 0; q < i; ++q) v = fma(-ts[i][q], x[q], v);
+            for (int q = 0; q < i; ++q) v = fma(-ts[i][q], xs[q][t], v);
         } else {
-#pragma unroll
-            for (int q = i + 1; q < TS_BS; ++q) v = fma(-ts[i][q], x[q], v);
+            for (int q = i + 1; q < TS_BS; ++q) v = fma(-ts[i][q], xs[q][t], v);
         }
-        x[i] = UNIT ? v : v / ts[i][i];
+        xs[i][t] = UNIT ? v : v / ts[i][i];
     }
     double *o = inv + (size_t)b * TS_BS * TS_BS;   // row-major 64x64
-#pragma unroll
-    for (int i = 0; i < TS_BS; ++i) o[i * TS_BS + t] = x[i];
+    for (int i = 0; i < TS_BS; ++i) o[i * TS_BS + t] = xs[i][t];
 }
 
 template <int TR, int FWD>

@@ -9,9 +9,9 @@
 //     M = D_r^{-1} P^T L U D_c^{-1}: gather + scale into FP32, forward and
 //     backward substitution, scale back into FP64;
 //   * the substitutions: one launch per triangle, one workgroup per 64-row
-//     block, each waiting (bounded) on per-block ready flags of the blocks
-//     it depends on, which it consumes as they appear; the diagonal block
-//     is applied through its precomputed inverse;
+//     block, each consuming (with bounded waits) the blocks it depends on
+//     as they are published, and solving its diagonal block by wave-level
+//     substitution;
 //   * vector kernels for GMRES (several dot products at once, w -= V h).
 // The host-side loops (classic refinement with dsgesv's stopping rule,
 // netlib dsgesv.f:199-296, and GMRES-IR) are in api.hip.
@@ -44,7 +44,7 @@ __global__ __launch_bounds__(256) void rowred_kernel(const double *__restrict__ 
     const int i = blockIdx.x * 512 + 2 * threadIdx.x;
     const int j0 = blockIdx.y * RW, j1 = min(n, j0 + RW);
     double a0 = 0.0, a1 = 0.0;
-    if (i + 1 < m && (lda & 1) == 0) {
+    if (i + 1 < m && (lda & 1) == 0 && ((uintptr_t)A & 15) == 0) {
         const double *p = A + (long)j0 * lda + i;
         for (int j = j0; j < j1; ++j, p += lda) {
             const double2 v = *reinterpret_cast<const double2 *>(p);
@@ -183,92 +183,41 @@ hipError_t equilibrate(hipStream_t s, const double *A, long lda, int n, double *
 }
 
 // ---------------------------------------------------------------------------
-// Inverses of the 64 x 64 diagonal blocks of the FP32 factor (row-major W):
-// inv_l[b] = L_bb^{-1} (unit lower), inv_u[b] = U_bb^{-1}; a partial last
-// block is completed by the identity.  One workgroup (64 threads) per block
-// and triangle, thread c computes column c.
+// y <- T^{-1} y for T the unit-lower (UPPER = 0) or upper triangle of the
+// row-major FP32 factor W.  One launch per triangle; workgroup g owns the
+// 64-row block bi = g (lower) or nbk-1-g (upper), so every block it waits
+// for belongs to a workgroup dispatched before it.
+//   * Finished blocks are published as 64-bit words (epoch << 32 | x bits)
+//     in xt: a word is its own ready flag, so one agent-scope load both
+//     tests readiness and delivers the value.  Dependencies are consumed in
+//     groups of 8 blocks; a block counts once all 64 of its words carry this
+//     launch's epoch; waits are bounded (err |= 2).
+//   * Thread (rr, q) = (tid / 4, tid % 4) accumulates row rr of its block
+//     over columns q*16..+15 of every finished block.
+//   * The diagonal block is solved by wave 0 by substitution, one row per
+//     lane, the solved entry broadcast by readlane.
 // ---------------------------------------------------------------------------
+FLX_DEV float readlane_f(float v, int lane)
+{
+    return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), lane));
+}
+
 constexpr int TV = 64;
-
-template <int UPPER>
-__global__ __launch_bounds__(64) void tri_inv_rm_kernel(const float *__restrict__ W, long ldw,
-                                                        int n, float *__restrict__ inv)
-{
-    __shared__ float a[TV][TV + 1];
-    __shared__ float x[TV][TV + 1];   // x[r][c]: column c of the inverse
-    const int b = blockIdx.x, r0 = b * TV, rows = min(TV, n - r0);
-    const int c = threadIdx.x;
-    for (int r = 0; r < TV; ++r)
-        a[r][c] = (r < rows && c < rows) ? W[(long)(r0 + r) * ldw + r0 + c] : (r == c ? 1.0f : 0.0f);
-    __syncthreads();
-    if (!UPPER) {
-        for (int r = 0; r < TV; ++r) x[r][c] = 0.0f;
-        x[c][c] = 1.0f;
-        for (int r = c + 1; r < TV; ++r) {
-            float v = 0.0f;
-            for (int q = c; q < r; ++q) v = fmaf(-a[r][q], x[q][c], v);
-            x[r][c] = v;
-        }
-    } else {
-        for (int r = 0; r < TV; ++r) x[r][c] = 0.0f;
-        x[c][c] = 1.0f / a[c][c];
-        for (int r = c - 1; r >= 0; --r) {
-            float v = 0.0f;
-            for (int q = r + 1; q <= c; ++q) v = fmaf(-a[r][q], x[q][c], v);
-            x[r][c] = v / a[r][r];
-        }
-    }
-    __syncthreads();
-    float *o = inv + (long)b * TV * TV;
-    for (int r = 0; r < TV; ++r) o[r * TV + c] = x[r][c];
-}
-
-hipError_t tri_inv_rm(hipStream_t s, const float *W, long ldw, int n, float *inv_l, float *inv_u)
-{
-    const int nbk = (n + TV - 1) / TV;
-    if (nbk <= 0) return hipSuccess;
-    hipLaunchKernelGGL(tri_inv_rm_kernel<0>, dim3(nbk), dim3(64), 0, s, W, ldw, n, inv_l);
-    hipLaunchKernelGGL(tri_inv_rm_kernel<1>, dim3(nbk), dim3(64), 0, s, W, ldw, n, inv_u);
-    return hipGetLastError();
-}
-
-size_t trsv_inv_floats(int n) { return (size_t)((n + TV - 1) / TV) * TV * TV; }
-
-// ---------------------------------------------------------------------------
-// y <- T^{-1} y for T the unit-lower (UPPER = 0) or upper triangle of W.
-// Workgroup g owns block row bi = g (lower) or nbk-1-g (upper), so every
-// block it waits for belongs to a workgroup launched before it.  Thread
-// (rr, q) = (tid / 4, tid % 4) accumulates row rr over columns q*16..+15 of
-// each finished block.  A finished block publishes its 64 values and then
-// flags[bi] = epoch (agent scope); waits are bounded (err[0] |= 2 on give-up).
-// ---------------------------------------------------------------------------
+constexpr int TV_FETCH = 8;
 constexpr int TV_SPIN_LIMIT = 1 << 22;
 
-FLX_DEV float ld_agent_f(const float *p)
+FLX_DEV uint64_t tv_word(unsigned epoch, float x)
 {
-    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-}
-FLX_DEV void st_agent_f(float *p, float v)
-{
-    __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-}
-FLX_DEV unsigned ld_agent_u(const unsigned *p)
-{
-    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-}
-FLX_DEV void st_agent_u(unsigned *p, unsigned v)
-{
-    __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    return ((uint64_t)epoch << 32) | (uint64_t)__float_as_uint(x);
 }
 
 template <int UPPER>
 __global__ __launch_bounds__(256) void trsv_rm_kernel(const float *__restrict__ W, long ldw, int n,
-                                                      const float *__restrict__ inv,
                                                       float *__restrict__ y,
-                                                      unsigned *__restrict__ flags,
+                                                      uint64_t *__restrict__ xt,
                                                       unsigned epoch, int *__restrict__ err)
 {
-    __shared__ int s_ready[2];
+    __shared__ float s_x[2][TV_FETCH][TV];
     __shared__ float s_rhs[TV];
     const int nbk = (n + TV - 1) / TV;
     const int g = blockIdx.x;
@@ -279,90 +228,107 @@ __global__ __launch_bounds__(256) void trsv_rm_kernel(const float *__restrict__ 
     const int ndep = UPPER ? nbk - 1 - bi : bi;
     const bool row_ok = rr < rows;
     const float *wrow = W + (long)(r0 + (row_ok ? rr : 0)) * ldw;
-    // the inverse of the diagonal block: row rr, columns q*16..+15
-    float iv[16];
-    {
-        const float4 *pi = reinterpret_cast<const float4 *>(inv + (long)bi * TV * TV + rr * TV + q * 16);
-#pragma unroll
-        for (int c = 0; c < 4; ++c) {
-            const float4 v = pi[c];
-            iv[4 * c] = v.x; iv[4 * c + 1] = v.y; iv[4 * c + 2] = v.z; iv[4 * c + 3] = v.w;
-        }
-    }
     const float rhs0 = row_ok ? y[r0 + rr] : 0.0f;
+    // wave 0: row `lane` of the diagonal block (identity outside the matrix)
+    float drow[TV];
+    if (wave == 0) {
+        const bool lok = lane < rows;
+        const float *dr = W + (long)(r0 + (lok ? lane : 0)) * ldw + r0;
+#pragma unroll
+        for (int c = 0; c < TV; ++c)
+            drow[c] = (lok && c < rows) ? dr[c] : (lane == c ? 1.0f : 0.0f);
+    }
+    // dependency blocks in groups of TV_FETCH: the group's factor entries are
+    // requested first, then wave 0 waits until every block of the group is
+    // published, so the two latencies overlap
     float acc = 0.0f;
-    int done = 0, it = 0, spins = 0;
-    while (done < ndep) {
+    int par = 0, spins = 0;
+    for (int d0 = 0; d0 < ndep; d0 += TV_FETCH, par ^= 1) {
+        const int cnt = min(TV_FETCH, ndep - d0);
+        float wv[TV_FETCH][16];
+#pragma unroll
+        for (int b = 0; b < TV_FETCH; ++b) {
+            const int d = d0 + b;
+            const int jb = UPPER ? nbk - 1 - d : d;
+            const int c0 = jb * TV + q * 16;
+#pragma unroll
+            for (int c = 0; c < 16; ++c)
+                wv[b][c] = (b < cnt && row_ok && c0 + c < n) ? wrow[c0 + c] : 0.0f;
+        }
         if (wave == 0) {
-            const int d = done + lane;
-            bool ready = false;
-            if (d < ndep) {
-                const int jb = UPPER ? nbk - 1 - d : d;
-                ready = ld_agent_u(&flags[jb]) == epoch;
-            }
-            // finished blocks form a prefix of the dependency order
-            const uint64_t bal = __ballot(ready);
-            int take = (bal == ~0ull) ? 64 : (int)__builtin_ctzll(~bal);
-            take = min(take, ndep - done);
-            if (lane == 0) {
-                if (take == 0 && ++spins > TV_SPIN_LIMIT) {
-                    atomicOr(err, 2);
-                    take = ndep - done;   // give up: finish with what is there
+            for (;;) {
+                uint64_t v[TV_FETCH];
+#pragma unroll
+                for (int b = 0; b < TV_FETCH; ++b) {
+                    const int d = d0 + b;
+                    const int jb = UPPER ? nbk - 1 - d : d;
+                    v[b] = (b < cnt) ? __hip_atomic_load(&xt[(long)jb * TV + lane], __ATOMIC_RELAXED,
+                                                         __HIP_MEMORY_SCOPE_AGENT)
+                                     : 0ull;
                 }
-                s_ready[it & 1] = take;
+                bool all = true;
+#pragma unroll
+                for (int b = 0; b < TV_FETCH; ++b)
+                    if (b < cnt) all = all && (__ballot((unsigned)(v[b] >> 32) == epoch) == ~0ull);
+                if (all || spins > TV_SPIN_LIMIT) {
+                    if (!all && lane == 0) atomicOr(err, 2);   // gave up: go on with what is there
+#pragma unroll
+                    for (int b = 0; b < TV_FETCH; ++b)
+                        s_x[par][b][lane] = __uint_as_float((unsigned)(v[b] & 0xffffffffull));
+                    break;
+                }
+                ++spins;
+                __builtin_amdgcn_s_sleep(1);
             }
         }
         __syncthreads();
-        const int take = s_ready[it & 1];
-        ++it;
-        if (take == 0) {
-            __builtin_amdgcn_s_sleep(2);
-            continue;
-        }
-        for (int d = done; d < done + take; ++d) {
-            const int jb = UPPER ? nbk - 1 - d : d;
-            const int c0 = jb * TV + q * 16;
-            float xv[16], wv[16];
 #pragma unroll
-            for (int c = 0; c < 16; ++c) {
-                const bool ok = c0 + c < n;
-                xv[c] = ok ? ld_agent_f(&y[c0 + c]) : 0.0f;
-                wv[c] = (ok && row_ok) ? wrow[c0 + c] : 0.0f;
-            }
+        for (int b = 0; b < TV_FETCH; ++b)
 #pragma unroll
-            for (int c = 0; c < 16; ++c) acc = fmaf(wv[c], xv[c], acc);
-        }
-        done += take;
+            for (int c = 0; c < 16; ++c) acc = fmaf(wv[b][c], s_x[par][b][q * 16 + c], acc);
     }
     acc += __shfl_xor(acc, 1, 64);
     acc += __shfl_xor(acc, 2, 64);
     if (q == 0) s_rhs[rr] = row_ok ? rhs0 - acc : 0.0f;
     __syncthreads();
-    float v = 0.0f;
+    if (wave == 0) {
+        float bv = s_rhs[lane];
+        if (!UPPER) {
 #pragma unroll
-    for (int c = 0; c < 16; ++c) v = fmaf(iv[c], s_rhs[q * 16 + c], v);
-    v += __shfl_xor(v, 1, 64);
-    v += __shfl_xor(v, 2, 64);
-    if (q == 0 && row_ok) st_agent_f(&y[r0 + rr], v);
-    __threadfence();
-    __syncthreads();
-    if (tid == 0) st_agent_u(&flags[bi], epoch);
+            for (int c = 0; c < TV; ++c) {
+                const float xc = readlane_f(bv, c);
+                if (lane > c) bv = fmaf(-drow[c], xc, bv);
+            }
+        } else {
+#pragma unroll
+            for (int c = TV - 1; c >= 0; --c) {
+                if (lane == c) bv = bv / drow[c];
+                const float xc = readlane_f(bv, c);
+                if (lane < c) bv = fmaf(-drow[c], xc, bv);
+            }
+        }
+        if (lane < rows) y[r0 + lane] = bv;
+        __hip_atomic_store(&xt[(long)bi * TV + lane], tv_word(epoch, lane < rows ? bv : 0.0f),
+                           __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    }
 }
 
-hipError_t trsv_rm(hipStream_t s, int upper, const float *W, long ldw, int n, const float *inv,
-                   float *y, unsigned *flags, unsigned epoch, int *err)
+hipError_t trsv_rm(hipStream_t s, int upper, const float *W, long ldw, int n, float *y,
+                   uint64_t *xt, unsigned epoch, int *err)
 {
     const int nbk = (n + TV - 1) / TV;
     if (nbk <= 0) return hipSuccess;
     ProfScope ps(tl_prof, s, PROF_TRS, (double)n * n, 2.0 * n * (double)n);
     if (upper)
-        hipLaunchKernelGGL(trsv_rm_kernel<1>, dim3(nbk), dim3(256), 0, s, W, ldw, n, inv, y, flags,
-                           epoch, err);
+        hipLaunchKernelGGL(trsv_rm_kernel<1>, dim3(nbk), dim3(256), 0, s, W, ldw, n, y, xt, epoch,
+                           err);
     else
-        hipLaunchKernelGGL(trsv_rm_kernel<0>, dim3(nbk), dim3(256), 0, s, W, ldw, n, inv, y, flags,
-                           epoch, err);
+        hipLaunchKernelGGL(trsv_rm_kernel<0>, dim3(nbk), dim3(256), 0, s, W, ldw, n, y, xt, epoch,
+                           err);
     return hipGetLastError();
 }
+
+size_t trsv_words(int n) { return (size_t)((n + TV - 1) / TV) * TV; }
 
 // ---------------------------------------------------------------------------
 // Preconditioner glue and vector kernels (FP64 vectors of length n)
@@ -432,18 +398,18 @@ hipError_t vmax(hipStream_t s, const double *v, int n, unsigned long long *nrm)
     return hipGetLastError();
 }
 
-__global__ void zero_u32_n_kernel(unsigned *p, int n)
+__global__ void zero_u64_n_kernel(uint64_t *p, int n)
 {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < n) p[i] = 0u;
+    if (i < n) p[i] = 0ull;
 }
 
-// a fresh flag value per substitution launch; flags are cleared on wrap
+// a fresh tag per substitution launch; the words are cleared on wrap
 static unsigned next_epoch(hipStream_t s, const PrecFactor &f)
 {
     if (++*f.epoch == 0u) {
-        const int nbk = (f.n + TV - 1) / TV;
-        hipLaunchKernelGGL(zero_u32_n_kernel, dim3((nbk + 255) / 256), dim3(256), 0, s, f.flags, nbk);
+        const int nw = (int)trsv_words(f.n);
+        hipLaunchKernelGGL(zero_u64_n_kernel, dim3((nw + 255) / 256), dim3(256), 0, s, f.xt, nw);
         *f.epoch = 1u;
     }
     return *f.epoch;
@@ -457,8 +423,8 @@ hipError_t prec_apply(hipStream_t s, const PrecFactor &f, const double *v, doubl
     FLX_CHECK(vmax(s, v, n, vnrm));
     hipLaunchKernelGGL(prec_in_kernel, dim3((n + 255) / 256), dim3(256), 0, s, v,
                        (const int *)f.perm, f.dr, (const unsigned long long *)vnrm, f.y32, n);
-    FLX_CHECK(trsv_rm(s, 0, f.W, f.ldw, n, f.inv_l, f.y32, f.flags, next_epoch(s, f), f.err));
-    FLX_CHECK(trsv_rm(s, 1, f.W, f.ldw, n, f.inv_u, f.y32, f.flags, next_epoch(s, f), f.err));
+    FLX_CHECK(trsv_rm(s, 0, f.W, f.ldw, n, f.y32, f.xt, next_epoch(s, f), f.err));
+    FLX_CHECK(trsv_rm(s, 1, f.W, f.ldw, n, f.y32, f.xt, next_epoch(s, f), f.err));
     hipLaunchKernelGGL(prec_out_kernel, dim3((n + 255) / 256), dim3(256), 0, s,
                        (const float *)f.y32, f.dc, (const unsigned long long *)vnrm, z, add, n,
                        znrm);

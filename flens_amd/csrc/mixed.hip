@@ -56,57 +56,69 @@ FLX_DEV uint64_t wave_max_u64(uint64_t v)
     return v;
 }
 
+// 256 threads, one row each.  Selection key (32 bits): |v| with its low 8
+// mantissa bits replaced by 255 - thread, so the wave maximum is one DPP
+// reduction and ties go to the lower thread.  Dropping those bits lets a
+// pivot within a factor 1 - 2^-15 of the column maximum win, which is
+// immaterial for the stability of the factor.
 struct TsShared {
     float4 row[2][4][TS_B / 4];   // per-wave candidate rows, double-buffered
-    uint64_t key[2][4];
+    uint32_t key[2][4];
     int win[TS_B];                // selected positions in order
-    int nsel;
-    int last;
 };
 
-// Partial-pivoting elimination over the 256 rows held by the workgroup (one
-// per thread, in registers).  Selects up to w rows; when u11 is given (the
-// root) the selected rows' eliminated values -- L multipliers left of the
-// diagonal, U entries from it -- are written to u11 in selection order, and
-// the first column whose pivot is exactly zero is reported through zero_col.
-FLX_DEV void ts_select(float (&r)[TS_B], bool act, int pos, int w, TsShared &sh,
-                       float *__restrict__ u11, int *zero_col, int zbase)
+FLX_DEV uint32_t ts_key32(float v, int t)
 {
-    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    int nsel = 0;
+    uint32_t a = __float_as_uint(fabsf(v)) & ~0xffu;
+    a = a < 0x100u ? 0x100u : a;           // nonzero for every live row
+    return a | (uint32_t)(255 - t);
+}
+
+// Partial-pivoting elimination over the workgroup's rows (one per thread,
+// fully unrolled column loop so every register index is static, one
+// barrier per column).  Selects up to w rows into sh.win (positions, in
+// order) and returns how many.  When u11 is given (the root) every
+// selected row's eliminated values -- L multipliers left of the diagonal,
+// U entries from it -- are written to u11 in selection order, and the
+// first exactly-zero pivot column (zbase + j) goes to zero_col.
+FLX_DEV int ts_select(float (&r)[TS_B], bool act, int pos, int w, TsShared &sh,
+                      float *__restrict__ u11, int *zero_col, int zbase)
+{
+    const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
+    int nsel = 0, sel = -1;
 #pragma unroll
     for (int j = 0; j < TS_B; ++j) {
         if (j < w) {
             const int par = j & 1;
-            const uint64_t key = act ? ts_key(r[j], pos) : 0ull;
-            const uint64_t wk = wave_max_u64(key);
-            if (key != 0ull && key == wk) {
+            const uint32_t key = act ? ts_key32(r[j], t) : 0u;
+            const uint32_t wk = wave_umax(key);
+            if (key != 0u && key == wk) {
 #pragma unroll
-                for (int c = 0; c < TS_B / 4; ++c)
+                for (int c = j / 4; c < TS_B / 4; ++c)
                     sh.row[par][wave][c] = make_float4(r[4 * c], r[4 * c + 1], r[4 * c + 2],
                                                        r[4 * c + 3]);
             }
             if (lane == 0) sh.key[par][wave] = wk;
             __syncthreads();
-            uint64_t best = sh.key[par][0];
+            uint32_t best = sh.key[par][0];
             int bw = 0;
 #pragma unroll
             for (int q = 1; q < 4; ++q) {
-                const uint64_t kq = sh.key[par][q];
+                const uint32_t kq = sh.key[par][q];
                 if (kq > best) { best = kq; bw = q; }
             }
-            if (best != 0ull) {
-                const float *pr = reinterpret_cast<const float *>(sh.row[par][bw]);
+            if (best != 0u) {
+                float pr[TS_B];
+#pragma unroll
+                for (int c = j / 4; c < TS_B / 4; ++c) {
+                    const float4 v = sh.row[par][bw][c];
+                    pr[4 * c] = v.x; pr[4 * c + 1] = v.y; pr[4 * c + 2] = v.z; pr[4 * c + 3] = v.w;
+                }
                 if (key == best) {
                     act = false;
-                    if (u11) {
-#pragma unroll
-                        for (int c = 0; c < TS_B; ++c) u11[j * TS_B + c] = r[c];
-                    }
-                }
-                if (threadIdx.x == 0) {
-                    sh.win[j] = (int)(~(uint32_t)(best & 0xffffffffull));
-                    if (zero_col && (best >> 32) == 1ull) atomicMin(zero_col, zbase + j);
+                    sel = j;
+                    sh.win[j] = pos;
+                    if (zero_col && pr[j] == 0.0f) atomicMin(zero_col, zbase + j);
                 }
                 nsel = j + 1;
                 if (act) {
@@ -119,8 +131,14 @@ FLX_DEV void ts_select(float (&r)[TS_B], bool act, int pos, int w, TsShared &sh,
             }
         }
     }
-    if (threadIdx.x == 0) sh.nsel = nsel;
+    if (u11 && sel >= 0) {
+        float4 *dst = reinterpret_cast<float4 *>(u11 + sel * TS_B);
+#pragma unroll
+        for (int c = 0; c < TS_B / 4; ++c)
+            dst[c] = make_float4(r[4 * c], r[4 * c + 1], r[4 * c + 2], r[4 * c + 3]);
+    }
     __syncthreads();
+    return nsel;
 }
 
 FLX_DEV int ld_agent_i(const int *p)
@@ -138,6 +156,22 @@ struct TsTree {
     int off[6];      // first node id of each level
 };
 
+// one panel row (w <= 32 entries, 128-byte aligned when w == 32) into r
+FLX_DEV void ts_load_row(float (&r)[TS_B], const float *__restrict__ src, bool ok, int w)
+{
+    if (ok && w == TS_B) {
+        const float4 *s4 = reinterpret_cast<const float4 *>(src);
+#pragma unroll
+        for (int c = 0; c < TS_B / 4; ++c) {
+            const float4 v = s4[c];
+            r[4 * c] = v.x; r[4 * c + 1] = v.y; r[4 * c + 2] = v.z; r[4 * c + 3] = v.w;
+        }
+    } else {
+#pragma unroll
+        for (int c = 0; c < TS_B; ++c) r[c] = (ok && c < w) ? src[c] : 0.0f;
+    }
+}
+
 // P: panel top-left in W (row-major, ld ldw), mp rows x w (<= 32) columns.
 // Outputs: u11 (32 x 32, selection order), win (selected positions,
 // relative to the panel top), list (one interchange slice, absolute rows
@@ -153,20 +187,19 @@ __global__ __launch_bounds__(TS_R) void tslu_kernel(const float *__restrict__ P,
                                                     int *__restrict__ zero)
 {
     __shared__ TsShared sh;
-    __shared__ int s_cpos[TS_F * TS_B];
+    __shared__ int s_last;
+    __shared__ int s_D[TS_B], s_V[TS_B];
     const int t = threadIdx.x;
     int node = blockIdx.x;     // index within the level
     int lev = 0;
     float r[TS_B];
-    // leaf: rows [node*256, +256)
     int pos = node * TS_R + t;
     bool act = pos < mp;
-#pragma unroll
-    for (int c = 0; c < TS_B; ++c) r[c] = (act && c < w) ? P[(long)pos * ldw + c] : 0.0f;
+    ts_load_row(r, P + (long)(act ? pos : 0) * ldw, act, w);
+    int nsel;
     for (;;) {
         const bool root = (lev == tree.nlev - 1);
-        ts_select(r, act, pos, w, sh, root ? u11 : nullptr, root ? zero : nullptr, r0);
-        const int nsel = sh.nsel;
+        nsel = ts_select(r, act, pos, w, sh, root ? u11 : nullptr, root ? zero : nullptr, r0);
         if (root) break;
         const int id = tree.off[lev] + node;
         if (t < nsel) st_agent_i(&cand[(long)id * TS_B + t], sh.win[t]);
@@ -179,11 +212,11 @@ __global__ __launch_bounds__(TS_R) void tslu_kernel(const float *__restrict__ P,
         const int pid = tree.off[lev + 1] + parent;
         if (t == 0) {
             const unsigned old = atomicAdd(&counters[pid], 1u);
-            sh.last = (old == (unsigned)(nchild - 1));
-            if (sh.last) atomicExch(&counters[pid], 0u);   // ready for the next launch
+            s_last = (old == (unsigned)(nchild - 1));
+            if (s_last) atomicExch(&counters[pid], 0u);   // ready for the next launch
         }
         __syncthreads();
-        if (!sh.last) return;
+        if (!s_last) return;
         __threadfence();
         // gather the children's candidates: thread t -> child t/32, slot t%32
         const int ch = t / TS_B, k = t % TS_B;
@@ -194,26 +227,22 @@ __global__ __launch_bounds__(TS_R) void tslu_kernel(const float *__restrict__ P,
         }
         act = (p >= 0 && p < mp);
         pos = act ? p : INT_MAX;
-#pragma unroll
-        for (int c = 0; c < TS_B; ++c) r[c] = (act && c < w) ? P[(long)pos * ldw + c] : 0.0f;
+        ts_load_row(r, P + (long)(act ? p : 0) * ldw, act, w);
         node = parent;
         ++lev;
     }
-    // root: publish the winners and the interchange list.  Rows k < nsel
-    // receive winner k; the displaced top rows (not winners) fill the
-    // winners' vacated slots (>= nsel), both in ascending order.
-    __shared__ int s_D[TS_B], s_V[TS_B], s_n[2];
-    const int nsel = sh.nsel;
-    const int k = t;
-    const int wkp = (k < nsel) ? sh.win[k] : -1;
-    bool inD = false, inV = false;
+    // root: publish the winners and the interchange list (wave 0).  Rows
+    // k < nsel receive winner k; the displaced top rows (not winners) fill
+    // the winners' vacated slots (>= nsel), both in ascending order.
     if (t < 64) {
+        const int k = t;
+        const int wkp = (k < nsel) ? sh.win[k] : -1;
         if (k < nsel) win[k] = wkp;
         bool top_is_win = false;
         if (k < nsel)
             for (int q = 0; q < nsel; ++q) top_is_win |= (sh.win[q] == k);
-        inD = (k < nsel) && !top_is_win;
-        inV = (k < nsel) && (wkp >= nsel);
+        const bool inD = (k < nsel) && !top_is_win;
+        const bool inV = (k < nsel) && (wkp >= nsel);
         const uint64_t bD = __ballot(inD);
         const int rankD = __popcll(bD & ((1ull << k) - 1ull));
         int rankV = 0;
@@ -228,11 +257,12 @@ __global__ __launch_bounds__(TS_R) void tslu_kernel(const float *__restrict__ P,
         }
         if (inD) s_D[rankD] = k;
         if (inV) s_V[rankV] = wkp;
-        if (k == 0) { s_n[0] = __popcll(b1); s_n[1] = __popcll(bD); }
+        if (k == 0) { sh.key[0][0] = __popcll(b1); sh.key[0][1] = __popcll(bD); }
     }
     __syncthreads();
     if (t < 64) {
-        const int n1 = s_n[0], nD = s_n[1];
+        const int k = t;
+        const int n1 = (int)sh.key[0][0], nD = (int)sh.key[0][1];
         if (k < nD) {
             list[1 + 2 * (n1 + k)] = r0 + s_V[k];
             list[2 + 2 * (n1 + k)] = r0 + s_D[k];
@@ -258,26 +288,33 @@ __global__ __launch_bounds__(256) void tslu_finish_kernel(float *__restrict__ P,
     int k = -1;
     for (int q = 0; q < nsel; ++q) k = (sw[q] == pos) ? q : k;
     float *row = P + (long)pos * ldw;
-    if (k >= 0) {
-        for (int c = 0; c < w; ++c) row[c] = su[k][c];
-        return;
-    }
     float x[TS_B];
+    if (k >= 0) {
 #pragma unroll
-    for (int c = 0; c < TS_B; ++c) x[c] = (c < w) ? row[c] : 0.0f;
+        for (int c = 0; c < TS_B; ++c) x[c] = su[k][c];
+    } else {
+        ts_load_row(x, row, true, w);
 #pragma unroll
-    for (int j = 0; j < TS_B; ++j) {
-        if (j < nsel) {
-            float v = x[j];
+        for (int j = 0; j < TS_B; ++j) {
+            if (j < nsel) {
+                float v = x[j];
 #pragma unroll
-            for (int q = 0; q < j; ++q) v = fmaf(-x[q], su[q][j], v);
-            const float d = su[j][j];
-            x[j] = (d != 0.0f) ? v / d : v;
+                for (int q = 0; q < j; ++q) v = fmaf(-x[q], su[q][j], v);
+                const float d = su[j][j];
+                x[j] = (d != 0.0f) ? v / d : v;
+            }
         }
     }
+    if (w == TS_B) {
+        float4 *r4 = reinterpret_cast<float4 *>(row);
 #pragma unroll
-    for (int c = 0; c < TS_B; ++c)
-        if (c < w) row[c] = x[c];
+        for (int c = 0; c < TS_B / 4; ++c)
+            r4[c] = make_float4(x[4 * c], x[4 * c + 1], x[4 * c + 2], x[4 * c + 3]);
+    } else {
+#pragma unroll
+        for (int c = 0; c < TS_B; ++c)
+            if (c < w) row[c] = x[c];
+    }
 }
 
 size_t tslu_nodes(int mp)
@@ -359,6 +396,20 @@ __global__ __launch_bounds__(256) void gemm_lp_kernel(const V *__restrict__ La,
     for (int a = 0; a < 4; ++a)
 #pragma unroll
         for (int b = 0; b < 4; ++b) acc[a][b] = float4_t{0.f, 0.f, 0.f, 0.f};
+    // the C tile is requested up front so its HBM latency overlaps the MFMAs
+    float cpre[4][4][4];
+#pragma unroll
+    for (int a = 0; a < 4; ++a)
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            const int row = i0 + a * 16 + 4 * lk + q;
+            const float *cr = C + (long)(row < nr ? row : 0) * ldc;
+#pragma unroll
+            for (int b = 0; b < 4; ++b) {
+                const int col = j0 + b * 16 + lr;
+                cpre[a][b][q] = (row < nr && col < nc) ? cr[col] : 0.0f;
+            }
+        }
     // operands are indexed in 8-element vectors: a row holds Kp / 8
     const int kv = Kp / 8;
     const V *pa = La + (long)(i0 + lr) * kv + lk;
@@ -399,7 +450,7 @@ __global__ __launch_bounds__(256) void gemm_lp_kernel(const V *__restrict__ La,
 #pragma unroll
                 for (int b = 0; b < 4; ++b) {
                     const int col = j0 + b * 16 + lr;
-                    if (col < nc) cr[col] = fmaf(acc[a][b][q], inv, cr[col]);
+                    if (col < nc) cr[col] = fmaf(acc[a][b][q], inv, cpre[a][b][q]);
                 }
             }
         }

@@ -37,7 +37,7 @@ constexpr int SLOT_ANRM = 0, SLOT_VN = 1, SLOT_T = 2, SLOT_COL = 4;   // scalar 
 struct Opts {
     int prec = MIX_FP32, refine = FLX_REFINE_CLASSIC, itermax = 30, gmres_m = 40;
     double tol = 0.0, gmres_tol = 1e-10;
-    int fallback = 1, nb = 256;
+    int fallback = 1, nb = 256, graph = 1;
     bool dsgesv = false;
 };
 
@@ -97,12 +97,9 @@ struct Solver {
             FLX_TRY(ensure(h, m.la, 2 * lp_pack_elems(n, o.nb)));
             FLX_TRY(ensure(h, m.ub, 2 * lp_pack_elems(n, o.nb)));
         }
-        FLX_TRY(ensure(h, m.inv_l, sizeof(float) * trsv_inv_floats(n)));
-        FLX_TRY(ensure(h, m.inv_u, sizeof(float) * trsv_inv_floats(n)));
         FLX_TRY(ensure(h, m.y32, sizeof(float) * n));
-        const size_t nflags = (size_t)(n + 63) / 64;
-        if (m.flags.cap < sizeof(unsigned) * nflags) m.epoch = 0;
-        FLX_TRY(ensure_zeroed(h, m.flags, sizeof(unsigned) * nflags));
+        if (m.flags.cap < sizeof(uint64_t) * trsv_words(n)) m.epoch = 0;
+        FLX_TRY(ensure_zeroed(h, m.flags, sizeof(uint64_t) * trsv_words(n)));
         FLX_TRY(ensure(h, m.err, sizeof(int) * 4));
         FLX_TRY(ensure(h, m.dr, sizeof(double) * n));
         FLX_TRY(ensure(h, m.dc, sizeof(double) * n));
@@ -138,10 +135,8 @@ struct Solver {
         f.perm = (const int *)m.perm.p;
         f.dr = (o.prec == MIX_FP16) ? (const double *)m.dr.p : nullptr;
         f.dc = (o.prec == MIX_FP16) ? (const double *)m.dc.p : nullptr;
-        f.inv_l = (const float *)m.inv_l.p;
-        f.inv_u = (const float *)m.inv_u.p;
         f.y32 = (float *)m.y32.p;
-        f.flags = (unsigned *)m.flags.p;
+        f.xt = (uint64_t *)m.flags.p;
         f.epoch = &m.epoch;
         f.err = (int *)m.err.p;
         return FLX_SUCCESS;
@@ -150,6 +145,49 @@ struct Solver {
     double *R(int c) { return (double *)h->mix.R.p + (size_t)c * n; }
     const double *Bc(int c) { return B + (size_t)c * ldb; }
     double *Xc(int c) { return X + (size_t)c * ldx; }
+
+    // The factorisation's launches depend only on (n, prec, nb) and the
+    // workspace, so they are captured once into a graph and replayed, on a
+    // private stream ordered after / before the caller's by events (the
+    // caller's stream may be the legacy default stream, which cannot be
+    // captured).  Recaptured when the key changes.
+    int factor_graph()
+    {
+        MixBufs &m = h->mix;
+        const size_t key[8] = {(size_t)n, (size_t)ldw, (size_t)o.prec, (size_t)o.nb,
+                               (size_t)m.W32.p, (size_t)m.cand.p, (size_t)m.la.p,
+                               (size_t)m.perm.p};
+        if (!m.gstream) {
+            FLX_TRY(hipStreamCreateWithFlags(&m.gstream, hipStreamNonBlocking));
+            FLX_TRY(hipEventCreateWithFlags(&m.ev0, hipEventDisableTiming));
+            FLX_TRY(hipEventCreateWithFlags(&m.ev1, hipEventDisableTiming));
+        }
+        if (!m.graph || memcmp(key, m.gkey, sizeof key) != 0) {
+            if (m.graph) {
+                FLX_TRY(hipStreamSynchronize(m.gstream));
+                hipGraphExecDestroy(m.graph);
+                m.graph = nullptr;
+            }
+            hipGraph_t g = nullptr;
+            FLX_TRY(hipStreamBeginCapture(m.gstream, hipStreamCaptureModeThreadLocal));
+            hipError_t e = getrf_rm_mixed(m.gstream, mw, (float *)m.W32.p, ldw, n, o.prec, o.nb);
+            hipGraph_t gg = nullptr;
+            hipError_t e2 = hipStreamEndCapture(m.gstream, &gg);
+            if (e != hipSuccess) { if (gg) hipGraphDestroy(gg); return hip_status(e); }
+            if (e2 != hipSuccess) return hip_status(e2);
+            g = gg;
+            e = hipGraphInstantiate(&m.graph, g, nullptr, nullptr, 0);
+            hipGraphDestroy(g);
+            if (e != hipSuccess) { m.graph = nullptr; return hip_status(e); }
+            memcpy(m.gkey, key, sizeof key);
+        }
+        FLX_TRY(hipEventRecord(m.ev0, s));
+        FLX_TRY(hipStreamWaitEvent(m.gstream, m.ev0, 0));
+        FLX_TRY(hipGraphLaunch(m.graph, m.gstream));
+        FLX_TRY(hipEventRecord(m.ev1, m.gstream));
+        FLX_TRY(hipStreamWaitEvent(s, m.ev1, 0));
+        return FLX_SUCCESS;
+    }
 
     // factor: 0 ok, -2 FP32 overflow in A or B, -3 exact zero pivot
     int factor(int *code)
@@ -164,9 +202,11 @@ struct Solver {
             FLX_TRY(equilibrate(s, A, lda, n, (double *)m.part.p, (double *)m.dr.p,
                                 (double *)m.dc.p));
         FLX_TRY(conv_d2s_rm(s, A, lda, n, f.dr, f.dc, (float *)m.W32.p, ldw, (int *)m.err.p + 1));
-        FLX_TRY(getrf_rm_mixed(s, mw, (float *)m.W32.p, ldw, n, o.prec, o.nb));
-        FLX_TRY(tri_inv_rm(s, (const float *)m.W32.p, ldw, n, (float *)m.inv_l.p,
-                           (float *)m.inv_u.p));
+        if (o.graph && !h->prof) {
+            FLX_RET(factor_graph());
+        } else {
+            FLX_TRY(getrf_rm_mixed(s, mw, (float *)m.W32.p, ldw, n, o.prec, o.nb));
+        }
         int flags[4] = {0, 0, 0, 0}, zero = INT_MAX;
         FLX_TRY(hipMemcpyAsync(flags, m.err.p, sizeof flags, hipMemcpyDeviceToHost, s));
         FLX_TRY(hipMemcpyAsync(&zero, mw.zero, sizeof(int), hipMemcpyDeviceToHost, s));
@@ -296,15 +336,16 @@ struct Solver {
             if (std::fabs(g[k + 1]) <= o.gmres_tol * beta || !(hn > 0.0)) { ++k; break; }
         }
         // y = H^{-1} g, x_c += V y
-        std::vector<double> y(k, 0.0);
+        std::vector<double> y(k, 0.0), ny(k, 0.0);
         for (int i = k - 1; i >= 0; --i) {
             double v = g[i];
             for (int j = i + 1; j < k; ++j) v -= H[(size_t)j * (mm + 1) + i] * y[j];
             const double d = H[(size_t)i * (mm + 1) + i];
             y[i] = d != 0.0 ? v / d : 0.0;
-            y[i] = -y[i];   // maxpy subtracts
         }
-        FLX_TRY(hipMemcpyAsync(hb, y.data(), sizeof(double) * k, hipMemcpyHostToDevice, s));
+        for (int i = 0; i < k; ++i) ny[i] = -y[i];   // maxpy subtracts
+        FLX_TRY(hipMemcpyAsync(hb, ny.data(), sizeof(double) * k, hipMemcpyHostToDevice, s));
+        FLX_TRY(hipStreamSynchronize(s));   // ny is a host temporary
         FLX_TRY(maxpy(s, V, n, n, k, hb, Xc(c)));
         *inner = k;
         return FLX_SUCCESS;
@@ -385,7 +426,7 @@ static int run_mixed(flx_context *h, int n, int nrhs, double *A, int lda, const 
 {
     using clk = std::chrono::steady_clock;
     Solver S{h, h->stream, n, nrhs, A, lda, B, ldb, X, ldx, o, {}, {}};
-    if (o.tol <= 0.0) S.o.tol = n * EPS64;
+    if (o.tol <= 0.0) S.o.tol = std::sqrt((double)n) * EPS64;
     int r = S.alloc();
     if (r) return r;
     const auto t0 = clk::now();
@@ -458,6 +499,7 @@ extern "C" int flx_dgesv_mixed(flx_handle h, int n, int nrhs, double *A, int lda
         o.gmres_tol = opt->gmres_tol > 0.0 ? opt->gmres_tol : 1e-10;
         o.fallback = opt->fallback;
         o.nb = opt->nb > 0 ? opt->nb : 256;
+        o.graph = opt->graph;
     }
     if (st) memset(st, 0, sizeof *st);
     if (n == 0 || nrhs == 0) return FLX_SUCCESS;

@@ -125,10 +125,13 @@ typedef struct flx_mixed_opts {
     int refine;        /* enum flx_refine */
     int itermax;       /* outer refinement steps (30 in dsgesv) */
     int gmres_m;       /* Krylov vectors per GMRES solve (0: 40) */
-    double tol;        /* stop at ||r||/(||A|| ||x|| + ||b||) <= tol (0: n*2^-53) */
+    double tol;        /* stop at ||r||/(||A|| ||x|| + ||b||) <= tol
+                          (0: sqrt(n) 2^-53, the level of dsgesv's test) */
     double gmres_tol;  /* relative GMRES residual (0: 1e-10) */
     int fallback;      /* 1: FP64 LU + solve when refinement fails */
     int nb;            /* outer block of the factorisation (0: 256) */
+    int graph;         /* 1: replay the factorisation's launches from a
+                          captured HIP graph (per shape), 0: launch eagerly */
 } flx_mixed_opts;
 typedef struct flx_mixed_stats {
     int iter;              /* refinement steps; < 0: failed (dsgesv ITER codes) */

@@ -1,0 +1,168 @@
+"""GPU tests of the mixed-precision solver (flx_dgesv_mixed, flx_dsgesv and
+the dsgesv_ seam): FP32 factor with tournament pivoting, FP32/BF16/FP16
+trailing updates, classic and GMRES refinement, FP64 fallback.
+
+Bar: the normwise backward error ||b - A x||_inf / (||A||_inf ||x||_inf +
+||b||_inf) <= n * 2^-53 at exit; forward error against x_true within a
+small multiple of what the FP64 oracle's own solve achieves; netlib dsgesv
+ITER / INFO semantics (dsgesv.f:131-337) on the cases that fix them.
+"""
+import numpy as np
+import pytest
+
+from flens_amd import testmat as T
+from oracle import oracle as O
+
+pytestmark = pytest.mark.gpu
+EPS = 2.0 ** -53
+
+
+def backward_error(a, x, b):
+    r = b - a @ x
+    an = np.abs(a).sum(1).max()
+    return max(np.abs(r[:, j]).max() / (an * np.abs(x[:, j]).max() + np.abs(b[:, j]).max())
+               for j in range(b.shape[1]))
+
+
+def fwd_err(x, xt):
+    return np.abs(x - xt).max() / np.abs(xt).max()
+
+
+def run_mixed(gpu, a, b, **kw):
+    A = gpu.DeviceMatrix.from_numpy(a)
+    B = gpu.DeviceMatrix.from_numpy(b)
+    X = gpu.DeviceMatrix.from_numpy(np.zeros_like(b))
+    st = gpu.lapack.sv_mixed(A, B, X, **kw)
+    return X.numpy(), st, A.numpy()
+
+
+@pytest.mark.parametrize("n", [1, 7, 31, 32, 33, 100, 257, 520, 1000, 2100])
+def test_fp32_classic_converges_on_sizes(gpu, n):
+    # tree shapes: one leaf (<= 256 rows), several leaves, two tree levels (2100)
+    a = T.svd_matrix(n, 1e2, 10 + n)
+    b, xt = T.rhs_from_solution(a, 1, n)
+    x, st, a_after = run_mixed(gpu, a, b, prec="fp32", refine="classic")
+    assert st["fallback"] == 0 and 0 <= st["iter"] <= 5, st
+    assert backward_error(a, x, b) <= n * EPS, st
+    assert np.array_equal(a_after, a)      # A untouched without fallback
+    assert fwd_err(x, xt) <= 4.0 * 1e2 * backward_error(a, x, b) + 100 * EPS
+
+
+@pytest.mark.parametrize("prec,refine,kappa", [
+    ("fp32", "classic", 1e6), ("fp32", "gmres", 1e6), ("fp32", "gmres", 1e10),
+    ("bf16", "gmres", 1e2), ("bf16", "gmres", 1e4), ("fp16", "gmres", 1e2),
+    ("fp16", "gmres", 1e4), ("bf16", "classic", 1e2), ("fp16", "classic", 1e2)])
+def test_ladder_reaches_fp64_backward_error(gpu, prec, refine, kappa):
+    n = 1200
+    a = T.svd_matrix(n, kappa, 77)
+    b, xt = T.rhs_from_solution(a, 1, 5)
+    x, st, _ = run_mixed(gpu, a, b, prec=prec, refine=refine, fallback=False, itermax=50)
+    eta = backward_error(a, x, b)
+    assert st["iter"] >= 0 and st["fallback"] == 0, st
+    assert eta <= n * EPS, (eta, st)
+    assert abs(st["backward_error"] - eta) <= 0.5 * eta + 1e-18
+    # forward error: within 10x of the FP64 oracle's, or within what the
+    # achieved backward error allows (kappa * eta)
+    ao, bo = a.copy(order="F"), b.copy(order="F")
+    piv, info = O.dgesv(ao, bo)
+    assert info == 0
+    assert fwd_err(x, xt) <= max(10.0 * fwd_err(bo, xt), 4.0 * kappa * eta) + 100 * EPS
+
+
+def test_several_right_hand_sides(gpu):
+    n = 700
+    a = T.svd_matrix(n, 1e3, 3)
+    b, xt = T.rhs_from_solution(a, 5, 4)
+    for refine in ("classic", "gmres"):
+        x, st, _ = run_mixed(gpu, a, b, prec="fp32", refine=refine)
+        assert st["fallback"] == 0 and backward_error(a, x, b) <= n * EPS, (refine, st)
+
+
+def test_fallback_when_refinement_fails(gpu):
+    # BF16 factor of a kappa = 1e10 matrix with classic refinement cannot converge
+    n = 600
+    a = T.svd_matrix(n, 1e10, 9)
+    b, xt = T.rhs_from_solution(a, 1, 9)
+    x, st, a_after = run_mixed(gpu, a, b, prec="bf16", refine="classic", itermax=5)
+    assert st["iter"] == -6 and st["fallback"] == 1 and st["info"] == 0, st
+    assert backward_error(a, x, b) <= n * EPS
+    assert not np.array_equal(a_after, a)     # the fallback overwrote A with its FP64 factors
+
+
+def test_dsgesv_semantics_device(gpu):
+    n = 300
+    a = O.uniform(n, n, 3) + n * np.eye(n)
+    a = np.asfortranarray(a)
+    b = O.uniform(n, 2, 4)
+    A = gpu.DeviceMatrix.from_numpy(a)
+    B = gpu.DeviceMatrix.from_numpy(b)
+    X = gpu.DeviceMatrix.from_numpy(np.zeros_like(b))
+    piv, it, info = gpu.lapack.dsgesv(A, B, X)
+    assert info == 0 and 0 <= it <= 3
+    x = X.numpy()
+    assert np.array_equal(A.numpy(), a)
+    cte = np.abs(a).sum(1).max() * EPS * np.sqrt(n)
+    r = b - a @ x
+    assert all(np.abs(r[:, j]).max() <= np.abs(x[:, j]).max() * cte * 1.01 for j in range(2))
+    # the returned interchanges reproduce a permutation (1-based, in range)
+    p = piv.cpu().numpy()
+    assert p.min() >= 1 and p.max() <= n and np.all(p >= np.arange(1, n + 1))
+
+
+def test_dsgesv_codes_host_seam(gpu):
+    n = 80
+    b = O.uniform(n, 1, 2)
+    # -2: an entry beyond FP32's range
+    a = np.asfortranarray(O.uniform(n, n, 1) + n * np.eye(n))
+    a[0, 0] = 1e300
+    x = np.zeros_like(b, order="F")
+    piv, it, info = gpu.lapack.dsgesv(a.copy(order="F"), b.copy(order="F"), x)
+    xo, po, io, ito = O.dsgesv(a.copy(order="F"), b)
+    assert it == ito == -2 and info == io == 0
+    assert O.dget02(a, x, b) < 30
+    # -3: exactly singular in FP32 -> FP64 fallback reports INFO
+    a = np.asfortranarray(O.uniform(n, n, 5))
+    a[:, 17] = 0.0
+    x = np.zeros_like(b, order="F")
+    piv, it, info = gpu.lapack.dsgesv(a.copy(order="F"), b.copy(order="F"), x)
+    assert it == -3 and info == 18
+    # -31: no convergence (kappa 1e12) -> FP64 solution
+    a = T.svd_matrix(n, 1e12, 2)
+    bb, xt = T.rhs_from_solution(a, 1, 3)
+    x = np.zeros_like(bb, order="F")
+    ac = a.copy(order="F")
+    piv, it, info = gpu.lapack.dsgesv(ac, bb.copy(order="F"), x)
+    xo, po, io, ito = O.dsgesv(a.copy(order="F"), bb)
+    assert it == ito == -31 and info == 0
+    assert O.dget02(a, x, bb) < 30
+    assert O.dget01(a, ac, piv) < 30          # A holds the FP64 factors after the fallback
+    # converged case: same ITER as the oracle's dsgesv up to one step (different FP32 pivots)
+    a = np.asfortranarray(O.uniform(n, n, 8) + 2 * np.eye(n))
+    x = np.zeros_like(b, order="F")
+    piv, it, info = gpu.lapack.dsgesv(a.copy(order="F"), b.copy(order="F"), x)
+    xo, po, io, ito = O.dsgesv(a.copy(order="F"), b)
+    assert it >= 0 and ito >= 0 and abs(it - ito) <= 1 and info == 0
+    assert O.dget02(a, x, b) < 30
+
+
+def test_graph_replays_match_eager(gpu):
+    """The factorisation replayed from its captured graph gives bit-identical
+    solutions to eager launches, across several replays with other work (a
+    DGEMM, a solve of another size, which recaptures) in between."""
+    n = 900
+    a = T.svd_matrix(n, 1e4, 21)
+    b, xt = T.rhs_from_solution(a, 1, 21)
+    a2 = T.svd_matrix(400, 1e2, 22)
+    b2, _ = T.rhs_from_solution(a2, 1, 22)
+    for prec in ("fp32", "fp16", "bf16"):
+        ref, st0, _ = run_mixed(gpu, a, b, prec=prec, refine="gmres", graph=False)
+        for rep in range(3):
+            x, st, _ = run_mixed(gpu, a, b, prec=prec, refine="gmres", graph=True)
+            assert np.array_equal(x, ref), (prec, rep)
+            assert st["iter"] == st0["iter"] and st["inner_iters"] == st0["inner_iters"]
+            # eager work between replays
+            c = gpu.DeviceMatrix.from_numpy(np.asfortranarray(np.ones((64, 64))))
+            gpu.blas.mm(0, 0, 1.0, c, c, 1.0, c)
+            if rep == 1:
+                x2, st2, _ = run_mixed(gpu, a2, b2, prec=prec, refine="classic", graph=True)
+                assert backward_error(a2, x2, b2) <= 400 * EPS
