@@ -41,6 +41,8 @@ EXPORTS = [
     "flx_set_lu_blocking", "flx_dgemm", "flx_dgetrf", "flx_dgetrs", "flx_dgesv",
     "flx_dlaswp", "flx_fill_uniform", "flx_profile_start", "flx_profile_stop",
     "flx_dgesv_mixed", "flx_dsgesv",
+    "flx_batch_gesv_create", "flx_batch_gemm_create", "flx_batch_execute", "flx_batch_config",
+    "flx_batch_destroy",
     "cblas_dgemm", "cblas_dtrsm", "dgetrf_", "dgetrs_", "dgesv_", "dlaswp_", "dsgesv_",
     "xerbla_",
 ]
@@ -116,6 +118,13 @@ def lib():
     L.dsgesv_.argtypes = [_ip, _ip, _dp, _ip, _ip, _dp, _ip, _dp, _ip, _dp,
                           C.POINTER(C.c_float), _ip, _ip]
     L.dsgesv_.restype = None
+    L.flx_batch_gesv_create.argtypes = [_vp, C.POINTER(_vp), _i, _ip, _ip, _vp, _vp, _vp, _vp,
+                                        _vp, _i, _i, _i]
+    L.flx_batch_gemm_create.argtypes = [_vp, C.POINTER(_vp), _i, _ip, _ip, _ip, _dp, _dp, _vp,
+                                        _vp, _vp, _i]
+    L.flx_batch_execute.argtypes = [_vp]
+    L.flx_batch_config.argtypes = [_vp, _ip, _dp]
+    L.flx_batch_destroy.argtypes = [_vp]
     _lib = L
     return L
 
@@ -346,6 +355,70 @@ class lapack:
                       C.byref(C.c_int(k1)), C.byref(C.c_int(k2)), piv.ctypes.data_as(_ip),
                       C.byref(C.c_int(incx)))
         return A
+
+
+class BatchPlan:
+    """A batch of solves or GEMMs captured as one persistent launch
+    (flx_batch_*).  Keeps the device buffers it was built on alive."""
+
+    def __init__(self, ptr, keep, handle):
+        self._p, self._keep, self._h = ptr, keep, handle
+
+    def execute(self):
+        _check(lib().flx_batch_execute(self._p), "flx_batch_execute")
+
+    def config(self):
+        cfg = (C.c_int * 2)()
+        ms = C.c_double(0.0)
+        _check(lib().flx_batch_config(self._p, cfg, C.byref(ms)), "flx_batch_config")
+        return {"tile_cfg": cfg[0], "grid": cfg[1], "tuned_ms": ms.value}
+
+    def close(self):
+        if self._p:
+            lib().flx_batch_destroy(self._p)
+            self._p = _vp()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class batched:
+    """Packed variable-size batches (see include/flx.h for the layout)."""
+
+    @staticmethod
+    def gesv_plan(n, nrhs, A, B, ipiv, info, iters=None, mode=0, itmax=5, tune=True,
+                  handle: Handle | None = None) -> BatchPlan:
+        """A, B: device float64 tensors (packed); ipiv, info, iters: int32."""
+        h = handle or default_handle()
+        n = np.ascontiguousarray(n, dtype=np.int32)
+        nrhs = np.ascontiguousarray(nrhs, dtype=np.int32)
+        p = _vp()
+        _check(lib().flx_batch_gesv_create(
+            h.ptr, C.byref(p), len(n), n.ctypes.data_as(_ip), nrhs.ctypes.data_as(_ip),
+            _vp(A.data_ptr()), _vp(B.data_ptr()), _vp(ipiv.data_ptr()), _vp(info.data_ptr()),
+            _vp(iters.data_ptr()) if iters is not None else None, mode, itmax, int(bool(tune))),
+            "flx_batch_gesv_create")
+        return BatchPlan(p, (A, B, ipiv, info, iters), h)
+
+    @staticmethod
+    def gemm_plan(m, n, k, alpha, beta, A, B, Cm, tune=True,
+                  handle: Handle | None = None) -> BatchPlan:
+        h = handle or default_handle()
+        m = np.ascontiguousarray(m, dtype=np.int32)
+        n = np.ascontiguousarray(n, dtype=np.int32)
+        k = np.ascontiguousarray(k, dtype=np.int32)
+        alpha = np.ascontiguousarray(alpha, dtype=np.float64)
+        beta = np.ascontiguousarray(beta, dtype=np.float64)
+        p = _vp()
+        _check(lib().flx_batch_gemm_create(
+            h.ptr, C.byref(p), len(m), m.ctypes.data_as(_ip), n.ctypes.data_as(_ip),
+            k.ctypes.data_as(_ip), alpha.ctypes.data_as(_dp), beta.ctypes.data_as(_dp),
+            _vp(A.data_ptr()), _vp(B.data_ptr()), _vp(Cm.data_ptr()), int(bool(tune))),
+            "flx_batch_gemm_create")
+        return BatchPlan(p, (A, B, Cm), h)
 
 
 _default = None

@@ -1,0 +1,673 @@
+// batched.hip -- thousands of independent, uneven LU solves and GEMMs in one
+// persistent launch each.
+//
+// Solves: every system (n <= 512, nrhs <= 8, column-major, ld = n) is handled
+// by one 256-thread workgroup from start to end: blocked right-looking LU
+// with partial pivoting (the algorithm of flens::lapack::trf, flens/lapack/
+// ge/trf.tcc:59-152, with tf2's pivot rule, tf2.tcc:101-126: first maximum,
+// INFO = first exact zero) -- panel of 32 columns held in registers, MFMA
+// trailing update -- then the forward / backward substitution of trs
+// (flens/lapack/ge/trs.tcc:59-109).  Workgroups take systems from an atomic
+// queue ordered by decreasing cost, so the launch needs no host round trip
+// per system and balances itself.
+//   mode 0: FP64 LU in place + solve (B overwritten by X);
+//   mode 1: FP32 LU of a converted copy, FP64 refinement (residuals on the
+//           FP64 A, FP32 corrections), FP64 LU + solve when refinement does
+//           not reach the tolerance within itmax steps (A overwritten then).
+// GEMMs: C_i <- beta_i C_i + alpha_i A_i B_i (FLENS generic semantics: beta
+// scales C first, so beta = 0 computes 0*C) over a flat list of output
+// tiles of all problems, one MFMA tile per workgroup pass.
+#include <cfloat>
+#include <climits>
+#include "flx_common.h"
+#include "flx_kernels.h"
+
+namespace flx {
+
+constexpr int BT = 256;           // threads per workgroup
+constexpr int BPW = 32;           // panel width
+constexpr int BMAX = 512;         // largest n
+constexpr int BSLOT = BMAX / BT;  // panel rows per thread
+constexpr int BRHS = 8;           // largest nrhs
+
+// ---- small helpers ----------------------------------------------------------
+FLX_DEV double readlane_d(double v, int lane)
+{
+    const long long b = __double_as_longlong(v);
+    const int lo = __builtin_amdgcn_readlane((int)(b & 0xffffffffll), lane);
+    const int hi = __builtin_amdgcn_readlane((int)(b >> 32), lane);
+    return __longlong_as_double(((long long)hi << 32) | (unsigned)lo);
+}
+FLX_DEV float readlane_t(float v, int lane)
+{
+    return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), lane));
+}
+FLX_DEV double readlane_t(double v, int lane) { return readlane_d(v, lane); }
+
+// |v| ordered as unsigned bits: (hi, lo) of a double, (bits, 0) of a float
+FLX_DEV void abs_key(double v, uint32_t &hi, uint32_t &lo)
+{
+    const unsigned long long b = (unsigned long long)__double_as_longlong(fabs(v));
+    hi = (uint32_t)(b >> 32);
+    lo = (uint32_t)b;
+}
+FLX_DEV void abs_key(float v, uint32_t &hi, uint32_t &lo)
+{
+    hi = __float_as_uint(fabsf(v));
+    lo = 0u;
+}
+
+FLX_DEV double4_t mfma_t(double a, double b, double4_t c)
+{
+    return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
+}
+FLX_DEV float4_t mfma_t(float a, float b, float4_t c)
+{
+    return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
+}
+template <typename T> struct Acc4;
+template <> struct Acc4<double> {
+    typedef double4_t type;
+    static FLX_DEV int row(int lk, int q) { return lk + 4 * q; }
+};
+template <> struct Acc4<float> {
+    typedef float4_t type;
+    static FLX_DEV int row(int lk, int q) { return 4 * lk + q; }
+};
+
+// shared memory of one workgroup (the solve buffers double as scratch)
+struct BShared {
+    double cand[2][4][BPW];        // per-wave pivot candidate rows (as double)
+    uint32_t khi[2][4], klo[2][4];
+    int kpos[2][4];
+    double l11[BPW][BPW + 1];
+    int perm[BMAX];                // original row now at each position
+    int ptmp[BMAX];
+    int mv_dst[2 * BPW], mv_src[2 * BPW];
+    int nmv;
+    int info;
+    double y[BMAX * BRHS];         // solve right-hand sides (T-typed view)
+    union {
+        double xs[BMAX * BRHS];        // refinement iterate (FP64), between factorisations
+        double mv_buf[2 * BPW][BPW];   // interchange staging, inside a factorisation
+    };
+    unsigned long long red[4 * BRHS];
+    double rs[BRHS], bn[BRHS];
+    int idx;
+};
+
+// ---- LU of one system: A (n x n, ld n, column-major) in place -----------------
+// ipiv (n ints, 1-based, LAPACK convention); sh.perm receives the row order;
+// returns INFO (first exact zero pivot, 1-based, 0 if none).
+template <typename T>
+FLX_DEV int wg_getrf(T *__restrict__ A, int n, int *__restrict__ ipiv, BShared &sh)
+{
+    const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
+    for (int i = t; i < n; i += BT) sh.perm[i] = i;
+    if (t == 0) sh.info = 0;
+    __syncthreads();
+    for (int k = 0; k < n; k += BPW) {
+        const int w = min(BPW, n - k);
+        // ---- panel: rows k..n-1 of columns k..k+w-1, BSLOT rows per thread
+        T r[BSLOT][BPW];
+        int pos[BSLOT], start[BSLOT];
+#pragma unroll
+        for (int s = 0; s < BSLOT; ++s) {
+            start[s] = k + t + BT * s;
+            pos[s] = start[s];
+            const bool ok = start[s] < n;
+#pragma unroll
+            for (int c = 0; c < BPW; ++c)
+                r[s][c] = (ok && c < w) ? A[(long)(k + c) * n + start[s]] : T(0);
+        }
+#pragma unroll
+        for (int j = 0; j < BPW; ++j) {
+            if (j < w) {
+                const int par = j & 1;
+                const int kj = k + j;
+                // best candidate of this thread: largest |v|, then lowest position
+                uint32_t bhi = 0, blo = 0;
+                int bpos = INT_MAX, bs = -1;
+#pragma unroll
+                for (int s = 0; s < BSLOT; ++s) {
+                    if (pos[s] >= kj && pos[s] < n) {
+                        uint32_t hi, lo;
+                        abs_key(r[s][j], hi, lo);
+                        const bool better = bs < 0 || hi > bhi || (hi == bhi && lo > blo) ||
+                                            (hi == bhi && lo == blo && pos[s] < bpos);
+                        if (better) { bhi = hi; blo = lo; bpos = pos[s]; bs = s; }
+                    }
+                }
+                // wave: max hi, then max lo among those, then min position
+                const uint32_t valid = bs >= 0 ? 1u : 0u;
+                const uint32_t mhi = wave_umax(valid ? bhi + 1u : 0u);
+                const bool c1 = valid && bhi + 1u == mhi;
+                const uint32_t mlo = wave_umax(c1 ? blo : 0u);
+                const bool c2 = c1 && blo == mlo;
+                const uint32_t mpos = wave_umin(c2 ? (uint32_t)bpos : 0xffffffffu);
+                if (c2 && (uint32_t)bpos == mpos) {
+                    // the slot's row by blending with exact 0/1 weights (a select
+                    // on bs becomes a dynamically indexed array in scratch)
+                    T wgt[BSLOT];
+#pragma unroll
+                    for (int s = 0; s < BSLOT; ++s) wgt[s] = (bs == s) ? T(1) : T(0);
+#pragma unroll
+                    for (int c = 0; c < BPW; ++c) {
+                        T v = r[0][c] * wgt[0];
+#pragma unroll
+                        for (int s = 1; s < BSLOT; ++s) v = v + r[s][c] * wgt[s];
+                        sh.cand[par][wave][c] = (double)v;
+                    }
+                }
+                if (lane == 0) {
+                    sh.khi[par][wave] = mhi;
+                    sh.klo[par][wave] = mlo;
+                    sh.kpos[par][wave] = (int)mpos;
+                }
+                __syncthreads();
+                int bw = 0;
+#pragma unroll
+                for (int q = 1; q < 4; ++q) {
+                    const uint32_t h = sh.khi[par][q], l = sh.klo[par][q];
+                    const int p = sh.kpos[par][q];
+                    const uint32_t hb = sh.khi[par][bw], lb = sh.klo[par][bw];
+                    const int pb = sh.kpos[par][bw];
+                    if (h > hb || (h == hb && (l > lb || (l == lb && p < pb)))) bw = q;
+                }
+                const int pp = sh.kpos[par][bw];   // pivot row position
+                const T piv = (T)sh.cand[par][bw][j];
+                if (t == 0) {
+                    ipiv[kj] = pp + 1;
+                    const int a = sh.perm[kj];
+                    sh.perm[kj] = sh.perm[pp];
+                    sh.perm[pp] = a;
+                    if (piv == T(0) && sh.info == 0) sh.info = kj + 1;
+                }
+                // swap positions kj <-> pp, eliminate below kj
+#pragma unroll
+                for (int s = 0; s < BSLOT; ++s) {
+                    if (pos[s] == pp) pos[s] = kj;
+                    else if (pos[s] == kj) pos[s] = pp;
+                    if (pos[s] > kj && pos[s] < n) {
+                        const T l = (piv != T(0)) ? r[s][j] / piv : T(0);
+                        r[s][j] = l;
+#pragma unroll
+                        for (int c = j + 1; c < BPW; ++c)
+                            r[s][c] = r[s][c] - l * (T)sh.cand[par][bw][c];
+                    }
+                }
+            }
+        }
+        __syncthreads();
+        // ---- write the panel back at the final positions; L11 to LDS
+#pragma unroll
+        for (int s = 0; s < BSLOT; ++s) {
+            if (start[s] < n) {
+#pragma unroll
+                for (int c = 0; c < BPW; ++c)
+                    if (c < w) A[(long)(k + c) * n + pos[s]] = r[s][c];
+                if (pos[s] < k + w)
+#pragma unroll
+                    for (int c = 0; c < BPW; ++c) sh.l11[pos[s] - k][c] = (double)r[s][c];
+            }
+        }
+        // ---- the panel's row moves, for the columns outside it
+        if (t == 0) sh.nmv = 0;
+        __syncthreads();
+#pragma unroll
+        for (int s = 0; s < BSLOT; ++s) {
+            if (start[s] < n && pos[s] != start[s]) {
+                const int e = atomicAdd(&sh.nmv, 1);
+                if (e < 2 * BPW) { sh.mv_dst[e] = pos[s]; sh.mv_src[e] = start[s]; }
+            }
+        }
+        __syncthreads();
+        const int nmv = min(sh.nmv, 2 * BPW);
+        for (int c0 = 0; c0 < n; c0 += BPW) {
+            if (c0 >= k && c0 < k + w) continue;   // panel columns are already placed
+            for (int q = t; q < nmv * BPW; q += BT) {
+                const int e = q / BPW, cc = q % BPW, c = c0 + cc;
+                if (c < n && !(c >= k && c < k + w))
+                    sh.mv_buf[e][cc] = (double)A[(long)c * n + sh.mv_src[e]];
+            }
+            __syncthreads();
+            for (int q = t; q < nmv * BPW; q += BT) {
+                const int e = q / BPW, cc = q % BPW, c = c0 + cc;
+                if (c < n && !(c >= k && c < k + w))
+                    A[(long)c * n + sh.mv_dst[e]] = (T)sh.mv_buf[e][cc];
+            }
+            __syncthreads();
+        }
+        // ---- U12 = L11^{-1} A12, one column per thread
+        for (int jc = k + w + t; jc < n; jc += BT) {
+            // rows >= w are computed from padding and never stored
+            T x[BPW];
+            T *col = A + (long)jc * n + k;
+#pragma unroll
+            for (int i = 0; i < BPW; ++i) x[i] = (i < w) ? col[i] : T(0);
+#pragma unroll
+            for (int q = 0; q < BPW - 1; ++q) {
+                const T xq = x[q];
+#pragma unroll
+                for (int i = q + 1; i < BPW; ++i) x[i] = x[i] - (T)sh.l11[i][q] * xq;
+            }
+            if (w == BPW) {
+#pragma unroll
+                for (int i = 0; i < BPW; ++i) col[i] = x[i];
+            } else {
+                for (int i = 0; i < w; ++i) col[i] = x[i];
+            }
+        }
+        __syncthreads();
+        // ---- A22 -= L21 U12 on 16x16 MFMA tiles (K = w)
+        const int m2 = n - k - w;
+        if (m2 > 0) {
+            const int nt = (m2 + 15) / 16;
+            const int lr = lane & 15, lk = lane >> 4;
+            for (int tile = wave; tile < nt * nt; tile += 4) {
+                const int ti = tile % nt, tj = tile / nt;
+                const int i0 = k + w + ti * 16, j0 = k + w + tj * 16;
+                typename Acc4<T>::type acc = {T(0), T(0), T(0), T(0)};
+                for (int kk = 0; kk < w; kk += 4) {
+                    const int kq = kk + lk;
+                    const int ar = i0 + lr, bc = j0 + lr;
+                    const T a = (ar < n && kq < w) ? A[(long)(k + kq) * n + ar] : T(0);
+                    const T b = (bc < n && kq < w) ? A[(long)bc * n + k + kq] : T(0);
+                    acc = mfma_t(a, b, acc);
+                }
+                const int col = j0 + lr;
+#pragma unroll
+                for (int q = 0; q < 4; ++q) {
+                    const int row = i0 + Acc4<T>::row(lk, q);
+                    if (row < n && col < n) A[(long)col * n + row] -= acc[q];
+                }
+            }
+        }
+        __syncthreads();
+    }
+    return sh.info;
+}
+
+// ---- solve with the factors: Y (n x nrhs, rows already permuted, in sh.y
+// as T) <- U^{-1} L^{-1} Y.  64-row blocks: wave 0 substitutes within the
+// block (one row per lane, the solved row broadcast by readlane), then all
+// threads fold the block into the rows below (above, for U).
+template <typename T>
+FLX_DEV void wg_trs(const T *__restrict__ A, int n, int nrhs, BShared &sh)
+{
+    const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
+    T *y = reinterpret_cast<T *>(sh.y);   // y[i * BRHS + c]
+    const int nbk = (n + 63) / 64;
+    for (int pass = 0; pass < 2; ++pass) {
+        const bool lower = pass == 0;
+        for (int bb = 0; bb < nbk; ++bb) {
+            const int blk = lower ? bb : nbk - 1 - bb;
+            const int r0 = blk * 64, rows = min(64, n - r0);
+            if (wave == 0) {
+                const int i = r0 + lane;
+                T v[BRHS];
+#pragma unroll
+                for (int c = 0; c < BRHS; ++c) v[c] = (lane < rows && c < nrhs) ? y[i * BRHS + c] : T(0);
+                if (lower) {
+                    for (int q = 0; q < rows; ++q) {
+                        const T lq = (lane < rows && lane > q) ? A[(long)(r0 + q) * n + i] : T(0);
+#pragma unroll
+                        for (int c = 0; c < BRHS; ++c) {
+                            const T xq = readlane_t(v[c], q);
+                            v[c] = v[c] - lq * xq;
+                        }
+                    }
+                } else {
+                    for (int q = rows - 1; q >= 0; --q) {
+                        const T d = A[(long)(r0 + q) * n + r0 + q];
+                        if (lane == q) {
+#pragma unroll
+                            for (int c = 0; c < BRHS; ++c) v[c] = v[c] / d;
+                        }
+                        const T uq = (lane < q) ? A[(long)(r0 + q) * n + i] : T(0);
+#pragma unroll
+                        for (int c = 0; c < BRHS; ++c) {
+                            const T xq = readlane_t(v[c], q);
+                            v[c] = v[c] - uq * xq;
+                        }
+                    }
+                }
+                if (lane < rows)
+#pragma unroll
+                    for (int c = 0; c < BRHS; ++c)
+                        if (c < nrhs) y[i * BRHS + c] = v[c];
+            }
+            __syncthreads();
+            // fold into the remaining rows: below (L) or above (U)
+            const int f0 = lower ? r0 + rows : 0, f1 = lower ? n : r0;
+            for (int i = f0 + t; i < f1; i += BT) {
+                T acc[BRHS];
+#pragma unroll
+                for (int c = 0; c < BRHS; ++c) acc[c] = (c < nrhs) ? y[i * BRHS + c] : T(0);
+                for (int q = 0; q < rows; ++q) {
+                    const T a = A[(long)(r0 + q) * n + i];
+#pragma unroll
+                    for (int c = 0; c < BRHS; ++c) acc[c] = acc[c] - a * y[(r0 + q) * BRHS + c];
+                }
+#pragma unroll
+                for (int c = 0; c < BRHS; ++c)
+                    if (c < nrhs) y[i * BRHS + c] = acc[c];
+            }
+            __syncthreads();
+        }
+    }
+}
+
+// y = (T) (src[perm[i]] * scale_c), src column-major n x nrhs (ld n)
+template <typename T>
+FLX_DEV void wg_load_rhs(const double *__restrict__ src, int n, int nrhs, const double *scale,
+                         BShared &sh)
+{
+    T *y = reinterpret_cast<T *>(sh.y);
+    for (int q = threadIdx.x; q < n * BRHS; q += BT) {
+        const int i = q / BRHS, c = q % BRHS;
+        y[q] = (c < nrhs) ? (T)(src[(long)c * n + sh.perm[i]] * (scale ? scale[c] : 1.0)) : T(0);
+    }
+    __syncthreads();
+}
+
+struct BatchProb {
+    long a;      // offset of A (and of its FP32 copy) in elements
+    long b;      // offset of B / X / R in elements
+    int n, nrhs;
+    int piv;     // offset of ipiv
+    int id;      // caller's index (info / iter outputs)
+};
+
+// block-wide max of a non-negative double per right-hand side into sh.red
+FLX_DEV void wg_colmax_reset(BShared &sh)
+{
+    if (threadIdx.x < 4 * BRHS) sh.red[threadIdx.x] = 0ull;
+    __syncthreads();
+}
+
+template <int MODE>
+__global__ __launch_bounds__(BT) void batched_gesv_kernel(const BatchProb *__restrict__ probs,
+                                                           int nprob, int *__restrict__ counter,
+                                                           double *__restrict__ A,
+                                                           double *__restrict__ B,
+                                                           float *__restrict__ W32,
+                                                           double *__restrict__ R,
+                                                           int *__restrict__ ipiv,
+                                                           int *__restrict__ info,
+                                                           int *__restrict__ iters, int itmax)
+{
+    __shared__ BShared sh;
+    const int t = threadIdx.x;
+    for (;;) {
+        if (t == 0) sh.idx = atomicAdd(counter, 1);
+        __syncthreads();
+        const int idx = sh.idx;
+        __syncthreads();
+        if (idx >= nprob) break;
+        const BatchProb p = probs[idx];
+        const int n = p.n, nrhs = p.nrhs;
+        if (n <= 0 || n > BMAX || nrhs < 0 || nrhs > BRHS) {
+            if (t == 0) { info[p.id] = -1; if (iters) iters[p.id] = 0; }
+            continue;
+        }
+        double *a = A + p.a;
+        double *bx = B + p.b;
+        int *pv = ipiv + p.piv;
+        int it_code = 0;
+        bool need64 = (MODE == 0);
+        if (MODE == 1) {
+            float *w = W32 + p.a;
+            double *r = R + p.b;
+            // FP32 copy (overflow -> fall back, as dsgesv's ITER = -2)
+            int ovf = 0;
+            for (long q = t; q < (long)n * n; q += BT) {
+                const double v = a[q];
+                ovf |= fabs(v) > (double)FLT_MAX;
+                w[q] = (float)v;
+            }
+            ovf = __syncthreads_or(ovf);
+            int inf32 = ovf ? 1 : wg_getrf<float>(w, n, pv, sh);
+            if (ovf) it_code = -2;
+            else if (inf32) it_code = -3;
+            if (!ovf && !inf32) {
+                // ||A||_inf and ||b_c||_inf
+                double an = 0.0;
+                for (int i = t; i < n; i += BT) {
+                    double s = 0.0;
+                    for (int j = 0; j < n; ++j) s += fabs(a[(long)j * n + i]);
+                    an = fmax(an, s);
+                }
+                wg_colmax_reset(sh);
+                atomicMax(&sh.red[0], (unsigned long long)__double_as_longlong(an));
+                for (int q = t; q < n * nrhs; q += BT)
+                    atomicMax(&sh.red[BRHS + q / n],
+                              (unsigned long long)__double_as_longlong(fabs(bx[q])));
+                __syncthreads();
+                const double anrm = __longlong_as_double((long long)sh.red[0]);
+                if (t < BRHS) sh.bn[t] = __longlong_as_double((long long)sh.red[BRHS + t]);
+                __syncthreads();
+                const double tol = sqrt((double)n) * 1.1102230246251565e-16;
+                // x0 = M^{-1} b ; x kept in sh.xs (FP64)
+                wg_load_rhs<float>(bx, n, nrhs, nullptr, sh);
+                wg_trs<float>(w, n, nrhs, sh);
+                for (int q = t; q < n * BRHS; q += BT)
+                    sh.xs[q] = (double)reinterpret_cast<float *>(sh.y)[q];
+                __syncthreads();
+                bool conv = false;
+                int it = 0;
+                for (; it <= itmax; ++it) {
+                    // r = b - A x, and the norms
+                    wg_colmax_reset(sh);
+                    for (int i = t; i < n; i += BT) {
+                        double acc[BRHS];
+#pragma unroll
+                        for (int c = 0; c < BRHS; ++c) acc[c] = (c < nrhs) ? bx[(long)c * n + i] : 0.0;
+                        for (int j = 0; j < n; ++j) {
+                            const double aij = a[(long)j * n + i];
+#pragma unroll
+                            for (int c = 0; c < BRHS; ++c) acc[c] = fma(-aij, sh.xs[j * BRHS + c], acc[c]);
+                        }
+#pragma unroll
+                        for (int c = 0; c < BRHS; ++c) {
+                            if (c < nrhs) {
+                                r[(long)c * n + i] = acc[c];
+                                atomicMax(&sh.red[c], (unsigned long long)__double_as_longlong(fabs(acc[c])));
+                                atomicMax(&sh.red[2 * BRHS + c],
+                                          (unsigned long long)__double_as_longlong(fabs(sh.xs[i * BRHS + c])));
+                            }
+                        }
+                    }
+                    __syncthreads();
+                    bool all = true;
+#pragma unroll
+                    for (int c = 0; c < BRHS; ++c) {
+                        const double rn = __longlong_as_double((long long)sh.red[c]);
+                        const double xn = __longlong_as_double((long long)sh.red[2 * BRHS + c]);
+                        if (c < nrhs) all = all && (rn <= tol * (anrm * xn + sh.bn[c]));
+                    }
+                    if (t < BRHS) {
+                        const double rn = __longlong_as_double((long long)sh.red[t]);
+                        sh.rs[t] = (rn > 0.0 && rn <= DBL_MAX) ? 1.0 / rn : 1.0;
+                    }
+                    __syncthreads();
+                    if (all) { conv = true; break; }
+                    if (it == itmax) break;
+                    // x += M^{-1} r  (r scaled by 1/||r|| into FP32 range and back)
+                    wg_load_rhs<float>(r, n, nrhs, sh.rs, sh);
+                    wg_trs<float>(w, n, nrhs, sh);
+                    for (int q = t; q < n * BRHS; q += BT) {
+                        const int c = q % BRHS;
+                        sh.xs[q] += (double)reinterpret_cast<float *>(sh.y)[q] / sh.rs[c];
+                    }
+                    __syncthreads();
+                }
+                if (conv) {
+                    for (int q = t; q < n * nrhs; q += BT) {
+                        const int c = q / n, i = q % n;
+                        bx[q] = sh.xs[i * BRHS + c];
+                    }
+                    it_code = it;
+                } else {
+                    it_code = -itmax - 1;
+                }
+            }
+            need64 = it_code < 0;
+            __syncthreads();
+        }
+        int inf = 0;
+        if (need64) {
+            inf = wg_getrf<double>(a, n, pv, sh);
+            if (inf == 0 && nrhs > 0) {
+                wg_load_rhs<double>(bx, n, nrhs, nullptr, sh);
+                wg_trs<double>(a, n, nrhs, sh);
+                for (int q = t; q < n * nrhs; q += BT) {
+                    const int c = q / n, i = q % n;
+                    bx[q] = sh.y[i * BRHS + c];
+                }
+            }
+        }
+        if (t == 0) {
+            info[p.id] = inf;
+            if (iters) iters[p.id] = it_code;
+        }
+        __syncthreads();
+    }
+}
+
+// ---- batched GEMM over a flat tile list ---------------------------------------
+struct GemmProb {
+    long a, b, c;        // offsets (elements); column-major, ld = rows
+    int m, n, k;
+    int tile0;           // first tile index of this problem
+    double alpha, beta;
+};
+
+// C tile (TM x TN) per 256-thread workgroup pass; 4 waves of (TM/2 x TN/2),
+// 16x16x4 FP64 MFMAs with operands read from L2.
+template <int TM, int TN>
+__global__ __launch_bounds__(256) void batched_gemm_kernel(const GemmProb *__restrict__ probs,
+                                                           int nprob, int ntiles,
+                                                           int *__restrict__ counter,
+                                                           const double *__restrict__ A,
+                                                           const double *__restrict__ Bm,
+                                                           double *__restrict__ Cm)
+{
+    constexpr int WM = TM / 2, WN = TN / 2, FI = WM / 16, FJ = WN / 16;
+    __shared__ int s_tile;
+    const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
+    const int lr = lane & 15, lk = lane >> 4;
+    for (;;) {
+        if (t == 0) s_tile = atomicAdd(counter, 1);
+        __syncthreads();
+        const int tile = s_tile;
+        __syncthreads();
+        if (tile >= ntiles) break;
+        // the problem owning this tile: last p with tile0 <= tile
+        int lo = 0, hi = nprob - 1;
+        while (lo < hi) {
+            const int mid = (lo + hi + 1) >> 1;
+            if (probs[mid].tile0 <= tile) lo = mid; else hi = mid - 1;
+        }
+        const GemmProb p = probs[lo];
+        const int tm = (p.m + TM - 1) / TM;
+        const int local = tile - p.tile0;
+        const int i0 = (local % tm) * TM + (wave & 1) * WM;
+        const int j0 = (local / tm) * TN + (wave >> 1) * WN;
+        const double *a = A + p.a, *b = Bm + p.b;
+        double *c = Cm + p.c;
+        double4_t acc[FI][FJ];
+#pragma unroll
+        for (int x = 0; x < FI; ++x)
+#pragma unroll
+            for (int y = 0; y < FJ; ++y) acc[x][y] = double4_t{0.0, 0.0, 0.0, 0.0};
+        for (int kk = 0; kk < p.k; kk += 4) {
+            const int kq = kk + lk;
+            double fa[FI], fb[FJ];
+#pragma unroll
+            for (int x = 0; x < FI; ++x) {
+                const int r = i0 + x * 16 + lr;
+                fa[x] = (r < p.m && kq < p.k) ? a[(long)kq * p.m + r] : 0.0;
+            }
+#pragma unroll
+            for (int y = 0; y < FJ; ++y) {
+                const int cc = j0 + y * 16 + lr;
+                fb[y] = (cc < p.n && kq < p.k) ? b[(long)cc * p.k + kq] : 0.0;
+            }
+#pragma unroll
+            for (int x = 0; x < FI; ++x)
+#pragma unroll
+                for (int y = 0; y < FJ; ++y) acc[x][y] = mfma_t(fa[x], fb[y], acc[x][y]);
+        }
+#pragma unroll
+        for (int x = 0; x < FI; ++x)
+#pragma unroll
+            for (int y = 0; y < FJ; ++y) {
+                const int col = j0 + y * 16 + lr;
+#pragma unroll
+                for (int q = 0; q < 4; ++q) {
+                    const int row = i0 + x * 16 + lk + 4 * q;
+                    if (row < p.m && col < p.n) {
+                        double *cp = c + (long)col * p.m + row;
+                        *cp = p.beta * *cp + p.alpha * acc[x][y][q];
+                    }
+                }
+            }
+    }
+}
+
+// ---- launchers ------------------------------------------------------------------
+__global__ void zero_int_kernel(int *p) { *p = 0; }
+
+size_t batched_prob_bytes() { return sizeof(BatchProb); }
+size_t batched_gemm_prob_bytes() { return sizeof(GemmProb); }
+
+hipError_t batched_gesv(hipStream_t s, int mode, const void *probs, int nprob, int *counter,
+                        double *A, double *B, float *W32, double *R, int *ipiv, int *info,
+                        int *iters, int itmax, int grid)
+{
+    if (nprob <= 0) return hipSuccess;
+    hipLaunchKernelGGL(zero_int_kernel, dim3(1), dim3(1), 0, s, counter);
+    if (mode == 0)
+        hipLaunchKernelGGL(batched_gesv_kernel<0>, dim3(grid), dim3(BT), 0, s,
+                           (const BatchProb *)probs, nprob, counter, A, B, W32, R, ipiv, info,
+                           iters, itmax);
+    else
+        hipLaunchKernelGGL(batched_gesv_kernel<1>, dim3(grid), dim3(BT), 0, s,
+                           (const BatchProb *)probs, nprob, counter, A, B, W32, R, ipiv, info,
+                           iters, itmax);
+    return hipGetLastError();
+}
+
+hipError_t batched_gemm(hipStream_t s, int tile_cfg, const void *probs, int nprob, int ntiles,
+                        int *counter, const double *A, const double *B, double *C, int grid)
+{
+    if (nprob <= 0 || ntiles <= 0) return hipSuccess;
+    hipLaunchKernelGGL(zero_int_kernel, dim3(1), dim3(1), 0, s, counter);
+    switch (tile_cfg) {
+    case 0:
+        hipLaunchKernelGGL((batched_gemm_kernel<32, 32>), dim3(grid), dim3(256), 0, s,
+                           (const GemmProb *)probs, nprob, ntiles, counter, A, B, C);
+        break;
+    case 1:
+        hipLaunchKernelGGL((batched_gemm_kernel<64, 64>), dim3(grid), dim3(256), 0, s,
+                           (const GemmProb *)probs, nprob, ntiles, counter, A, B, C);
+        break;
+    default:
+        hipLaunchKernelGGL((batched_gemm_kernel<128, 64>), dim3(grid), dim3(256), 0, s,
+                           (const GemmProb *)probs, nprob, ntiles, counter, A, B, C);
+        break;
+    }
+    return hipGetLastError();
+}
+
+int batched_gemm_tile(int cfg, int *tm, int *tn)
+{
+    static const int TMS[3] = {32, 64, 128}, TNS[3] = {32, 64, 64};
+    if (cfg < 0 || cfg > 2) return 0;
+    *tm = TMS[cfg];
+    *tn = TNS[cfg];
+    return 1;
+}
+
+} // namespace flx

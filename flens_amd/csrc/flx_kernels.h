@@ -185,6 +185,17 @@ hipError_t mdot(hipStream_t s, const double *V, long ldv, int n, int k, const do
 hipError_t maxpy(hipStream_t s, const double *V, long ldv, int n, int k, const double *h, double *w);
 hipError_t vscale_copy(hipStream_t s, const double *src, double alpha, double *dst, int n);
 
+// ---- batched solves and GEMMs (batched.hip) ----------------------------------
+// Problem tables are built by the host in the layouts of batched.hip.
+size_t batched_prob_bytes();
+size_t batched_gemm_prob_bytes();
+hipError_t batched_gesv(hipStream_t s, int mode, const void *probs, int nprob, int *counter,
+                        double *A, double *B, float *W32, double *R, int *ipiv, int *info,
+                        int *iters, int itmax, int grid);
+hipError_t batched_gemm(hipStream_t s, int tile_cfg, const void *probs, int nprob, int ntiles,
+                        int *counter, const double *A, const double *B, double *C, int grid);
+int batched_gemm_tile(int cfg, int *tm, int *tn);
+
 // ---- utilities (util.hip) --------------------------------------------------
 hipError_t fill_uniform_f64(hipStream_t s, double *A, int m, int n, long lda,
                             uint64_t seed, uint64_t offset);

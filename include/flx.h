@@ -156,6 +156,42 @@ int flx_dsgesv(flx_handle h, int n, int nrhs, double *A, int lda, int *ipiv,
                const double *B, int ldb, double *X, int ldx, int *iter,
                int *info);
 
+/* ------------------------------------------------------ batched plans */
+/* Thousands of independent small problems in one persistent launch each
+ * (no counterpart in the reference, whose trf / trs / mm handle one matrix
+ * per call: flens/lapack/ge/trf.tcc:183-261, trs.tcc:151-252,
+ * flens/blas/level3/mm.tcc:380-479; the batch applies those routines'
+ * semantics to every problem).  Inputs are packed column-major, ld = rows:
+ *   solves: A_i (n_i x n_i) at offset sum_{j<i} n_j^2, B_i (n_i x nrhs_i)
+ *           at sum_{j<i} n_j nrhs_j (overwritten by X_i), ipiv_i at
+ *           sum_{j<i} n_j; 1 <= n_i <= 512, 0 <= nrhs_i <= 8; info[i] as
+ *           LAPACK's INFO; mode 0 = FP64 LU, mode 1 = FP32 LU + FP64
+ *           refinement (itmax steps, stop at ||r|| <= sqrt(n) 2^-53 (||A||
+ *           ||x|| + ||b||)) with FP64 LU when it fails; iters[i] (optional)
+ *           = refinement steps or a dsgesv-style negative code.
+ *   GEMMs:  C_i <- beta_i C_i + alpha_i A_i B_i (beta applied to C first,
+ *           as the reference's generic gemm), A_i m_i x k_i at
+ *           sum m_j k_j, B_i k_i x n_i at sum k_j n_j, C_i at sum m_j n_j.
+ * A plan keeps the device pointers it was created with, picks its launch
+ * shape (GEMM tile and persistent grid) by timing the candidates on copies
+ * of the inputs when tune = 1, and captures the batch into a HIP graph;
+ * flx_batch_execute replays it on the handle's stream. */
+typedef struct flx_batch *flx_batch_t;
+int flx_batch_gesv_create(flx_handle h, flx_batch_t *plan, int count,
+                          const int *n, const int *nrhs, double *A,
+                          double *B, int *ipiv, int *info, int *iters,
+                          int mode, int itmax, int tune);
+int flx_batch_gemm_create(flx_handle h, flx_batch_t *plan, int count,
+                          const int *m, const int *n, const int *k,
+                          const double *alpha, const double *beta,
+                          const double *A, const double *B, double *C,
+                          int tune);
+int flx_batch_execute(flx_batch_t plan);
+/* cfg[0] = GEMM tile shape (0: 32x32, 1: 64x64, 2: 128x64), cfg[1] = grid;
+ * *tuned_ms = the winning candidate's time (0 when not tuned). */
+int flx_batch_config(flx_batch_t plan, int *cfg, double *tuned_ms);
+int flx_batch_destroy(flx_batch_t plan);
+
 /* --------------------------------------------------- FLENS plugin seams */
 /* CBLAS seam, cxxblas/drivers/cblas.h:798-805; enum values :22-27
  * (Order 101/102, Trans 111/112/113).  Caller: cxxblas::gemm<int> for

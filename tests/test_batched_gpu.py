@@ -1,0 +1,173 @@
+"""GPU tests of the batched plans (flx_batch_*): thousands of uneven LU
+solves and GEMMs in one persistent, graph-captured launch.
+
+Each problem must meet the single-problem bar of the FP64 path: INFO as
+LAPACK (flens/lapack/ge/tf2.tcc:101-126), pivots equal to the oracle's
+partial pivoting except at near-ties, dget01 / dget02 ratios < 30; GEMMs
+within 4 K eps64 (|alpha||A||B| + |beta||C|) elementwise and exact on
+integer data, beta = 0 computing 0 * C as the reference's generic gemm.
+"""
+import numpy as np
+import pytest
+
+from oracle import oracle as O
+
+pytestmark = pytest.mark.gpu
+EPS = 2.0 ** -53
+TAU = 1e-9
+
+
+def make_solves(count, seed, nmax=512, singular=(), illcond=()):
+    from flens_amd import testmat as T
+    rng = np.random.default_rng(seed)
+    n = np.clip(np.exp(rng.uniform(0.0, np.log(nmax), count)).round(), 1, nmax).astype(np.int32)
+    nrhs = rng.integers(1, 9, count).astype(np.int32)
+    A, B = [], []
+    for i in range(count):
+        a = np.asfortranarray(rng.standard_normal((n[i], n[i])))
+        if i in singular:
+            a[:, min(2, n[i] - 1)] = 0.0
+        if i in illcond and n[i] > 4:
+            a = T.svd_matrix(int(n[i]), 1e13, seed + i)
+        A.append(a)
+        B.append(np.asfortranarray(rng.standard_normal((n[i], nrhs[i]))))
+    return n, nrhs, A, B
+
+
+def pack(mats):
+    return np.concatenate([m.ravel(order="F") for m in mats]) if mats else np.zeros(0)
+
+
+def unpack(vec, shapes):
+    out, o = [], 0
+    for (r, c) in shapes:
+        out.append(np.asfortranarray(vec[o:o + r * c].reshape((c, r)).T))
+        o += r * c
+    return out
+
+
+def run_solves(gpu, n, nrhs, A, B, mode, tune=False, reps=1):
+    import torch
+    dev = "cuda"
+    At = torch.from_numpy(pack(A)).to(dev)
+    Bt = torch.from_numpy(pack(B)).to(dev)
+    A0, B0 = At.clone(), Bt.clone()
+    ipiv = torch.zeros(int(n.sum()), dtype=torch.int32, device=dev)
+    info = torch.zeros(len(n), dtype=torch.int32, device=dev)
+    iters = torch.zeros(len(n), dtype=torch.int32, device=dev)
+    plan = gpu.batched.gesv_plan(n, nrhs, At, Bt, ipiv, info, iters, mode=mode, tune=tune)
+    outs = []
+    for _ in range(reps):
+        At.copy_(A0)
+        Bt.copy_(B0)
+        plan.execute()
+        torch.cuda.synchronize()
+        outs.append((At.cpu().numpy(), Bt.cpu().numpy(), ipiv.cpu().numpy(), info.cpu().numpy(),
+                     iters.cpu().numpy()))
+    cfg = plan.config()
+    plan.close()
+    return outs, cfg
+
+
+def test_batched_fp64_solves_vs_oracle(gpu):
+    count = 300
+    sing = {5, 77, 150}
+    n, nrhs, A, B = make_solves(count, 11, singular=sing)
+    (outs, cfg) = run_solves(gpu, n, nrhs, A, B, mode=0)
+    lu_all, x_all, piv_all, info, _ = outs[0]
+    lus = unpack(lu_all, [(k, k) for k in n])
+    xs = unpack(x_all, [(k, r) for k, r in zip(n, nrhs)])
+    po = np.concatenate([[0], np.cumsum(n)])
+    for i in range(count):
+        a, b, lu, x = A[i], B[i], lus[i], xs[i]
+        piv = piv_all[po[i]:po[i + 1]]
+        ref = a.copy(order="F")
+        rpiv, rinfo, gap = O.dgetrf_gaps(ref)
+        assert info[i] == rinfo, (i, n[i], info[i], rinfo)
+        d = np.nonzero(piv != rpiv)[0]
+        assert d.size == 0 or gap[d[0]] < TAU, (i, n[i], d[:3])
+        if n[i] <= 160:
+            assert O.dget01(a, lu, piv) < 30, i
+        if rinfo == 0:
+            assert O.dget02(a, x, b) < 30, (i, n[i])
+        else:
+            assert np.array_equal(x, b)      # the solve is skipped on INFO > 0
+
+
+def test_batched_mixed_solves_and_fallback(gpu):
+    count = 200
+    ill = {3, 40, 41, 90}
+    n, nrhs, A, B = make_solves(count, 12, illcond=ill)
+    outs, cfg = run_solves(gpu, n, nrhs, A, B, mode=1)
+    lu_all, x_all, piv_all, info, iters = outs[0]
+    xs = unpack(x_all, [(k, r) for k, r in zip(n, nrhs)])
+    conv = 0
+    for i in range(count):
+        a, b, x = A[i], B[i], xs[i]
+        assert info[i] == 0
+        r = b - a @ x
+        an = np.abs(a).sum(1).max()
+        eta = max(np.abs(r[:, j]).max() / (an * np.abs(x[:, j]).max() + np.abs(b[:, j]).max())
+                  for j in range(b.shape[1]))
+        assert eta <= n[i] * EPS, (i, n[i], eta, iters[i])
+        if i in ill and n[i] > 4:
+            assert iters[i] < 0, (i, iters[i])       # refinement cannot reach FP64 accuracy
+        conv += iters[i] >= 0
+    assert conv >= count - len(ill) - 5
+
+
+def test_batched_graph_replays_and_tuning(gpu):
+    n, nrhs, A, B = make_solves(120, 13)
+    outs, cfg = run_solves(gpu, n, nrhs, A, B, mode=0, reps=3)
+    for o in outs[1:]:
+        for u, v in zip(o, outs[0]):
+            assert np.array_equal(u, v)
+    outs_t, cfg_t = run_solves(gpu, n, nrhs, A, B, mode=1, tune=True, reps=2)
+    assert cfg_t["grid"] > 0 and cfg_t["tuned_ms"] > 0
+    assert np.array_equal(outs_t[0][1], outs_t[1][1])
+
+
+def test_batched_gemm(gpu):
+    import torch
+    rng = np.random.default_rng(5)
+    count = 250
+    m, n, k = (np.clip(np.exp(rng.uniform(0, np.log(300), count)).round(), 1, 300).astype(np.int32)
+               for _ in range(3))
+    m[7] = 0
+    k[9] = 0
+    alpha = rng.uniform(-2, 2, count)
+    beta = rng.uniform(-2, 2, count)
+    beta[11] = 0.0
+    integer = set(range(20, 30))
+    As, Bs, Cs = [], [], []
+    for i in range(count):
+        if i in integer:
+            As.append(np.asfortranarray(rng.integers(-8, 9, (m[i], k[i])).astype(float)))
+            Bs.append(np.asfortranarray(rng.integers(-8, 9, (k[i], n[i])).astype(float)))
+            Cs.append(np.asfortranarray(rng.integers(-8, 9, (m[i], n[i])).astype(float)))
+            alpha[i], beta[i] = 2.0, -3.0
+        else:
+            As.append(np.asfortranarray(rng.uniform(-1, 1, (m[i], k[i]))))
+            Bs.append(np.asfortranarray(rng.uniform(-1, 1, (k[i], n[i]))))
+            Cs.append(np.asfortranarray(rng.uniform(-1, 1, (m[i], n[i]))))
+    Cs[11][:] = np.nan
+    At, Bt, Ct = (torch.from_numpy(pack(x)).cuda() for x in (As, Bs, Cs))
+    C0 = Ct.clone()
+    for tune in (False, True):
+        Ct.copy_(C0)
+        plan = gpu.batched.gemm_plan(m, n, k, alpha, beta, At, Bt, Ct, tune=tune)
+        plan.execute()
+        torch.cuda.synchronize()
+        out = unpack(Ct.cpu().numpy(), [(a, b) for a, b in zip(m, n)])
+        plan.close()
+        for i in range(count):
+            want = beta[i] * Cs[i] + alpha[i] * (As[i] @ Bs[i])
+            if i == 11:
+                assert np.isnan(out[i]).all()
+                continue
+            if i in integer:
+                assert np.array_equal(out[i], want), i
+                continue
+            bound = 4 * max(k[i], 1) * EPS * (abs(alpha[i]) * np.abs(As[i]) @ np.abs(Bs[i])
+                                              + abs(beta[i]) * np.abs(Cs[i]))
+            assert np.all(np.abs(out[i] - want) <= bound + 1e-300), (i, tune)
