@@ -42,10 +42,14 @@ EXPORTS = [
     "flx_dlaswp", "flx_fill_uniform", "flx_profile_start", "flx_profile_stop",
     "flx_dgesv_mixed", "flx_dsgesv",
     "flx_batch_gesv_create", "flx_batch_gemm_create", "flx_batch_execute", "flx_batch_config",
-    "flx_batch_destroy",
+    "flx_batch_destroy", "flx_mlp_forward", "flx_mlp_grad", "flx_mlp_adam",
     "cblas_dgemm", "cblas_dtrsm", "dgetrf_", "dgetrs_", "dgesv_", "dlaswp_", "dsgesv_",
     "xerbla_",
 ]
+
+_dp = C.POINTER(C.c_double)
+_ip = C.POINTER(C.c_int)
+_i, _d, _vp, _u64 = C.c_int, C.c_double, C.c_void_p, C.c_uint64
 
 PREC = {"fp32": 0, "bf16": 1, "fp16": 2}
 REFINE = {"classic": 0, "gmres": 1}
@@ -57,6 +61,13 @@ class MixedOpts(C.Structure):
                 ("fallback", C.c_int), ("nb", C.c_int), ("graph", C.c_int)]
 
 
+class BatchExtra(C.Structure):
+    _fields_ = [("selector", _vp), ("features", _vp), ("ticks", _vp), ("choice", _vp)]
+
+
+MLP_IN, MLP_HIDDEN, MLP_OUT, MLP_PARAMS = 5, 64, 4, 4804
+
+
 class MixedStats(C.Structure):
     _fields_ = [("iter", C.c_int), ("inner_iters", C.c_int), ("fallback", C.c_int),
                 ("info", C.c_int), ("backward_error", C.c_double), ("anorm", C.c_double),
@@ -66,9 +77,6 @@ class MixedStats(C.Structure):
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}
 
-_dp = C.POINTER(C.c_double)
-_ip = C.POINTER(C.c_int)
-_i, _d, _vp, _u64 = C.c_int, C.c_double, C.c_void_p, C.c_uint64
 
 _lib = None
 
@@ -119,7 +127,10 @@ def lib():
                           C.POINTER(C.c_float), _ip, _ip]
     L.dsgesv_.restype = None
     L.flx_batch_gesv_create.argtypes = [_vp, C.POINTER(_vp), _i, _ip, _ip, _vp, _vp, _vp, _vp,
-                                        _vp, _i, _i, _i]
+                                        _vp, _i, _i, _i, C.POINTER(BatchExtra)]
+    L.flx_mlp_forward.argtypes = [_vp, _vp, _vp, _i, _vp]
+    L.flx_mlp_grad.argtypes = [_vp, _vp, _vp, _vp, _vp, _i, _vp, _vp]
+    L.flx_mlp_adam.argtypes = [_vp, _vp, _vp, _vp, _vp, _i, C.c_float, C.c_float]
     L.flx_batch_gemm_create.argtypes = [_vp, C.POINTER(_vp), _i, _ip, _ip, _ip, _dp, _dp, _vp,
                                         _vp, _vp, _i]
     L.flx_batch_execute.argtypes = [_vp]
@@ -385,23 +396,59 @@ class BatchPlan:
             pass
 
 
+class mlp:
+    """The selector network's device kernels (flx_mlp_*) on torch tensors."""
+
+    @staticmethod
+    def forward(params, X, handle: Handle | None = None):
+        import torch
+        h = handle or default_handle()
+        out = torch.empty((X.shape[0], MLP_OUT), dtype=torch.float32, device=X.device)
+        _check(lib().flx_mlp_forward(h.ptr, _vp(params.data_ptr()), _vp(X.data_ptr()),
+                                     X.shape[0], _vp(out.data_ptr())), "flx_mlp_forward")
+        return out
+
+    @staticmethod
+    def grad(params, X, label, weight=None, handle: Handle | None = None):
+        import torch
+        h = handle or default_handle()
+        g = torch.empty(MLP_PARAMS, dtype=torch.float32, device=X.device)
+        loss = torch.empty(1, dtype=torch.float32, device=X.device)
+        _check(lib().flx_mlp_grad(h.ptr, _vp(params.data_ptr()), _vp(X.data_ptr()),
+                                  _vp(label.data_ptr()),
+                                  _vp(weight.data_ptr()) if weight is not None else None,
+                                  X.shape[0], _vp(g.data_ptr()), _vp(loss.data_ptr())),
+               "flx_mlp_grad")
+        return g, loss
+
+    @staticmethod
+    def adam(params, grad, m, v, step, lr=1e-3, weight_decay=0.0, handle: Handle | None = None):
+        h = handle or default_handle()
+        _check(lib().flx_mlp_adam(h.ptr, _vp(params.data_ptr()), _vp(grad.data_ptr()),
+                                  _vp(m.data_ptr()), _vp(v.data_ptr()), step, lr, weight_decay),
+               "flx_mlp_adam")
+
+
 class batched:
     """Packed variable-size batches (see include/flx.h for the layout)."""
 
     @staticmethod
     def gesv_plan(n, nrhs, A, B, ipiv, info, iters=None, mode=0, itmax=5, tune=True,
+                  selector=None, features=None, ticks=None, choice=None,
                   handle: Handle | None = None) -> BatchPlan:
-        """A, B: device float64 tensors (packed); ipiv, info, iters: int32."""
+        """A, B: device float64 tensors (packed); ipiv, info, iters, choice:
+        int32; selector, features: float32; ticks: int64 (all on the device)."""
         h = handle or default_handle()
         n = np.ascontiguousarray(n, dtype=np.int32)
         nrhs = np.ascontiguousarray(nrhs, dtype=np.int32)
+        ptr = (lambda t: _vp(t.data_ptr()) if t is not None else None)
+        ex = BatchExtra(ptr(selector), ptr(features), ptr(ticks), ptr(choice))
         p = _vp()
         _check(lib().flx_batch_gesv_create(
             h.ptr, C.byref(p), len(n), n.ctypes.data_as(_ip), nrhs.ctypes.data_as(_ip),
             _vp(A.data_ptr()), _vp(B.data_ptr()), _vp(ipiv.data_ptr()), _vp(info.data_ptr()),
-            _vp(iters.data_ptr()) if iters is not None else None, mode, itmax, int(bool(tune))),
-            "flx_batch_gesv_create")
-        return BatchPlan(p, (A, B, ipiv, info, iters), h)
+            ptr(iters), mode, itmax, int(bool(tune)), C.byref(ex)), "flx_batch_gesv_create")
+        return BatchPlan(p, (A, B, ipiv, info, iters, selector, features, ticks, choice), h)
 
     @staticmethod
     def gemm_plan(m, n, k, alpha, beta, A, B, Cm, tune=True,

@@ -86,7 +86,7 @@ extern "C" int flx_destroy(flx_handle h)
     hipStreamSynchronize(h->stream);
     free_buf(h->W); free_buf(h->A); free_buf(h->B); free_buf(h->C); free_buf(h->X);
     free_buf(h->piv0); free_buf(h->ipiv); free_buf(h->info); free_buf(h->perm);
-    free_buf(h->tinv); free_buf(h->sb); free_buf(h->sx);
+    free_buf(h->tinv); free_buf(h->sb); free_buf(h->sx); free_buf(h->mlp_part);
     MixBufs &x = h->mix;
     for (DevBuf *b : {&x.W32, &x.perm, &x.cand, &x.ccount, &x.counters, &x.u11, &x.win, &x.list,
                       &x.zero, &x.lpmax, &x.la, &x.ub, &x.inv_l, &x.inv_u, &x.y32, &x.flags,
@@ -310,6 +310,37 @@ extern "C" int flx_fill_uniform(flx_handle h, double *A, int m, int n, int lda, 
 {
     if (!h || m < 0 || n < 0 || lda < (m > 1 ? m : 1)) return FLX_INVALID_ARG;
     FLX_TRY(fill_uniform_f64(h->stream, A, m, n, lda, seed, offset));
+    return FLX_SUCCESS;
+}
+
+// ---------------------------------------------------------------------------
+// precision selector network
+// ---------------------------------------------------------------------------
+static_assert(FLX_MLP_PARAMS == MLP_PARAMS && FLX_MLP_IN == MLP_IN && FLX_MLP_OUT == MLP_OUT,
+              "selector layout");
+
+extern "C" int flx_mlp_forward(flx_handle h, const float *params, const float *X, int nb,
+                               float *logits)
+{
+    if (!h || nb < 0 || (nb > 0 && (!params || !X || !logits))) return FLX_INVALID_ARG;
+    FLX_TRY(mlp_forward(h->stream, params, X, nb, logits));
+    return FLX_SUCCESS;
+}
+
+extern "C" int flx_mlp_grad(flx_handle h, const float *params, const float *X, const int *label,
+                            const float *weight, int nb, float *grad, float *loss)
+{
+    if (!h || nb <= 0 || !params || !X || !label || !grad) return FLX_INVALID_ARG;
+    FLX_TRY(ensure(h, h->mlp_part, sizeof(float) * mlp_part_floats(nb)));
+    FLX_TRY(mlp_grad(h->stream, params, X, label, weight, nb, (float *)h->mlp_part.p, grad, loss));
+    return FLX_SUCCESS;
+}
+
+extern "C" int flx_mlp_adam(flx_handle h, float *params, const float *grad, float *m, float *v,
+                            int step, float lr, float weight_decay)
+{
+    if (!h || !params || !grad || !m || !v || step < 1) return FLX_INVALID_ARG;
+    FLX_TRY(mlp_adam(h->stream, params, grad, m, v, step, lr, weight_decay));
     return FLX_SUCCESS;
 }
 

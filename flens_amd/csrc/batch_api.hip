@@ -49,6 +49,7 @@ struct flx_batch {
     // launch shape
     int tile_cfg = 1, grid = 0;
     double tuned_ms = 0.0;
+    BatchOut out;
     // graph
     hipStream_t gs = nullptr;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
@@ -69,7 +70,7 @@ static hipError_t launch(flx_batch *p, hipStream_t s, int tile_cfg, int grid, do
 {
     if (p->kind == 0)
         return batched_gesv(s, p->mode, p->probs, p->count, p->counter, A, B, p->W32, p->R,
-                            p->ipiv, p->info, p->iters, p->itmax, grid);
+                            p->ipiv, p->info, p->iters, p->itmax, grid, p->out);
     return batched_gemm(s, tile_cfg, p->gprobs[tile_cfg], p->count, p->ntiles[tile_cfg],
                         p->counter, p->GA, p->GB, C, grid);
 }
@@ -161,11 +162,13 @@ static int autotune(flx_batch *p, bool tune)
 
 extern "C" int flx_batch_gesv_create(flx_handle h, flx_batch_t *out, int count, const int *n,
                                      const int *nrhs, double *A, double *B, int *ipiv, int *info,
-                                     int *iters, int mode, int itmax, int tune)
+                                     int *iters, int mode, int itmax, int tune,
+                                     const flx_batch_extra *extra)
 {
     if (!h || !out || count < 0 || (count > 0 && (!n || !nrhs || !A || !B || !ipiv || !info)) ||
-        mode < 0 || mode > 1 || itmax < 0 || itmax > 100)
+        mode < 0 || mode > 2 || itmax < 0 || itmax > 100)
         return FLX_INVALID_ARG;
+    if (mode == 2 && (!extra || !extra->selector)) return FLX_INVALID_ARG;
     if (batched_prob_bytes() != sizeof(BatchProbH)) return FLX_HIP_ERROR;
     std::vector<BatchProbH> tab(count);
     long ao = 0, bo = 0;
@@ -192,11 +195,17 @@ extern "C" int flx_batch_gesv_create(flx_handle h, flx_batch_t *out, int count, 
     p->A = A; p->B = B; p->ipiv = ipiv; p->info = info; p->iters = iters;
     p->a_total = ao;
     p->b_total = bo;
+    if (extra) {
+        p->out.sel = extra->selector;
+        p->out.feat = extra->features;
+        p->out.tsys = extra->ticks;
+        p->out.choice = extra->choice;
+    }
     auto fail = [&](int st) { flx_batch_destroy(p); return st; };
     if (hipMalloc(&p->probs, sizeof(BatchProbH) * std::max(1, count)) != hipSuccess ||
         hipMalloc(&p->counter, sizeof(int)) != hipSuccess)
         return fail(FLX_ALLOC_FAILED);
-    if (mode == 1 && (hipMalloc(&p->W32, sizeof(float) * std::max(1L, ao)) != hipSuccess ||
+    if (mode >= 1 && (hipMalloc(&p->W32, sizeof(float) * std::max(1L, ao)) != hipSuccess ||
                       hipMalloc(&p->R, sizeof(double) * std::max(1L, bo)) != hipSuccess))
         return fail(FLX_ALLOC_FAILED);
     if (count > 0 &&

@@ -386,6 +386,36 @@ FLX_DEV void wg_colmax_reset(BShared &sh)
     __syncthreads();
 }
 
+// selector features of one system (see the feature definitions below)
+FLX_DEV float clampf(float v, float lo, float hi) { return fminf(fmaxf(v, lo), hi); }
+
+// In-kernel inference of the selector network (mlp.hip layout) by wave 0:
+// lane j computes hidden unit j; returns the arg-max class (uniform).
+FLX_DEV int wg_select(const float *__restrict__ P, const float (&f)[MLP_IN])
+{
+    constexpr int H = MLP_HIDDEN;
+    constexpr int oB1 = H * MLP_IN, oW2 = oB1 + H, oB2 = oW2 + H * H, oW3 = oB2 + H,
+                  oB3 = oW3 + MLP_OUT * H;
+    const int lane = threadIdx.x & 63;
+    float a = P[oB1 + lane];
+#pragma unroll
+    for (int i = 0; i < MLP_IN; ++i) a = fmaf(P[lane * MLP_IN + i], f[i], a);
+    const float h1 = fmaxf(a, 0.0f);
+    float b = P[oB2 + lane];
+    for (int k = 0; k < H; ++k) b = fmaf(P[oW2 + lane * H + k], readlane_t(h1, k), b);
+    const float h2 = fmaxf(b, 0.0f);
+    int best = 0;
+    float bz = -INFINITY;
+    for (int o = 0; o < MLP_OUT; ++o) {
+        float z = h2 * P[oW3 + o * H + lane];
+#pragma unroll
+        for (int d = 32; d > 0; d >>= 1) z += __shfl_xor(z, d, 64);
+        z += P[oB3 + o];
+        if (z > bz) { bz = z; best = o; }
+    }
+    return best;
+}
+
 template <int MODE>
 __global__ __launch_bounds__(BT) void batched_gesv_kernel(const BatchProb *__restrict__ probs,
                                                            int nprob, int *__restrict__ counter,
@@ -395,9 +425,11 @@ __global__ __launch_bounds__(BT) void batched_gesv_kernel(const BatchProb *__res
                                                            double *__restrict__ R,
                                                            int *__restrict__ ipiv,
                                                            int *__restrict__ info,
-                                                           int *__restrict__ iters, int itmax)
+                                                           int *__restrict__ iters, int itmax,
+                                                           BatchOut out)
 {
     __shared__ BShared sh;
+    __shared__ int s_choice;
     const int t = threadIdx.x;
     for (;;) {
         if (t == 0) sh.idx = atomicAdd(counter, 1);
@@ -407,6 +439,7 @@ __global__ __launch_bounds__(BT) void batched_gesv_kernel(const BatchProb *__res
         if (idx >= nprob) break;
         const BatchProb p = probs[idx];
         const int n = p.n, nrhs = p.nrhs;
+        const long long t_start = wall_clock64();
         if (n <= 0 || n > BMAX || nrhs < 0 || nrhs > BRHS) {
             if (t == 0) { info[p.id] = -1; if (iters) iters[p.id] = 0; }
             continue;
@@ -416,7 +449,7 @@ __global__ __launch_bounds__(BT) void batched_gesv_kernel(const BatchProb *__res
         int *pv = ipiv + p.piv;
         int it_code = 0;
         bool need64 = (MODE == 0);
-        if (MODE == 1) {
+        if (MODE >= 1) {
             float *w = W32 + p.a;
             double *r = R + p.b;
             // FP32 copy (overflow -> fall back, as dsgesv's ITER = -2)
@@ -431,22 +464,75 @@ __global__ __launch_bounds__(BT) void batched_gesv_kernel(const BatchProb *__res
             if (ovf) it_code = -2;
             else if (inf32) it_code = -3;
             if (!ovf && !inf32) {
-                // ||A||_inf and ||b_c||_inf
-                double an = 0.0;
+                // ||A||_inf, ||b_c||_inf, and the row statistics of the features
+                double an = 0.0, amin = DBL_MAX, dmin = DBL_MAX;
                 for (int i = t; i < n; i += BT) {
                     double s = 0.0;
                     for (int j = 0; j < n; ++j) s += fabs(a[(long)j * n + i]);
                     an = fmax(an, s);
+                    amin = fmin(amin, s);
+                    const double d = fabs(a[(long)i * n + i]);
+                    dmin = fmin(dmin, d / fmax(s - d, 1e-300));
                 }
                 wg_colmax_reset(sh);
+                if (t < 2) sh.red[3 * BRHS + 1 + t] = (unsigned long long)__double_as_longlong(DBL_MAX);
+                __syncthreads();
                 atomicMax(&sh.red[0], (unsigned long long)__double_as_longlong(an));
+                atomicMin(&sh.red[3 * BRHS + 1], (unsigned long long)__double_as_longlong(amin));
+                atomicMin(&sh.red[3 * BRHS + 2], (unsigned long long)__double_as_longlong(dmin));
                 for (int q = t; q < n * nrhs; q += BT)
                     atomicMax(&sh.red[BRHS + q / n],
                               (unsigned long long)__double_as_longlong(fabs(bx[q])));
                 __syncthreads();
                 const double anrm = __longlong_as_double((long long)sh.red[0]);
+                const double rmin = __longlong_as_double((long long)sh.red[3 * BRHS + 1]);
+                const double ddom = __longlong_as_double((long long)sh.red[3 * BRHS + 2]);
                 if (t < BRHS) sh.bn[t] = __longlong_as_double((long long)sh.red[BRHS + t]);
                 __syncthreads();
+                bool skip_ir = false;
+                if (MODE == 2 || out.feat) {
+                    // ||A^{-1}||_inf >= ||A^{-1} e||_inf for e = 1 and e = sign(A^{-1} 1)
+                    float *y = reinterpret_cast<float *>(sh.y);
+                    for (int q = t; q < n * BRHS; q += BT) y[q] = (q % BRHS == 0) ? 1.0f : 0.0f;
+                    __syncthreads();
+                    wg_trs<float>(w, n, 1, sh);
+                    wg_colmax_reset(sh);
+                    for (int i = t; i < n; i += BT) {
+                        const float v = y[i * BRHS];
+                        atomicMax(&sh.red[0], (unsigned long long)__double_as_longlong(fabs((double)v)));
+                        sh.xs[i] = (v >= 0.0f) ? 1.0 : -1.0;
+                    }
+                    __syncthreads();
+                    for (int q = t; q < n * BRHS; q += BT)
+                        y[q] = (q % BRHS == 0) ? (float)sh.xs[sh.perm[q / BRHS]] : 0.0f;
+                    __syncthreads();
+                    wg_trs<float>(w, n, 1, sh);
+                    for (int i = t; i < n; i += BT)
+                        atomicMax(&sh.red[0], (unsigned long long)__double_as_longlong(fabs((double)y[i * BRHS])));
+                    __syncthreads();
+                    const double ainv = __longlong_as_double((long long)sh.red[0]);
+                    __syncthreads();
+                    // features: log2 n / 9, nrhs / 8, log10 kappa_inf / 16,
+                    // log10 (diagonal dominance) / 4, log10 (row-norm ratio) / 12
+                    float f[MLP_IN];
+                    f[0] = log2f((float)n) / 9.0f;
+                    f[1] = (float)nrhs / 8.0f;
+                    const double kap = anrm * ainv;
+                    f[2] = clampf((float)log10(fmax(kap, 1.0)), 0.0f, 20.0f) / 16.0f;
+                    f[3] = clampf((float)log10(fmax(ddom, 1e-300)), -4.0f, 4.0f) / 4.0f;
+                    f[4] = clampf((float)log10(anrm / fmax(rmin, 1e-300)), 0.0f, 12.0f) / 12.0f;
+                    if (!(kap <= DBL_MAX)) f[2] = 20.0f / 16.0f;
+                    if (out.feat && t < MLP_IN) out.feat[(long)p.id * MLP_IN + t] = f[t];
+                    if (MODE == 2) {
+                        int c = 1;
+                        if (t < 64) c = wg_select(out.sel, f);
+                        if (t == 0) s_choice = c;
+                        __syncthreads();
+                        c = s_choice;
+                        if (out.choice && t == 0) out.choice[p.id] = c;
+                        skip_ir = (c == 0);
+                    }
+                }
                 const double tol = sqrt((double)n) * 1.1102230246251565e-16;
                 // x0 = M^{-1} b ; x kept in sh.xs (FP64)
                 wg_load_rhs<float>(bx, n, nrhs, nullptr, sh);
@@ -456,7 +542,7 @@ __global__ __launch_bounds__(BT) void batched_gesv_kernel(const BatchProb *__res
                 __syncthreads();
                 bool conv = false;
                 int it = 0;
-                for (; it <= itmax; ++it) {
+                for (; it <= itmax && !skip_ir; ++it) {
                     // r = b - A x, and the norms
                     wg_colmax_reset(sh);
                     for (int i = t; i < n; i += BT) {
@@ -509,7 +595,7 @@ __global__ __launch_bounds__(BT) void batched_gesv_kernel(const BatchProb *__res
                     }
                     it_code = it;
                 } else {
-                    it_code = -itmax - 1;
+                    it_code = skip_ir ? -4 : -itmax - 1;   // -4: the selector chose FP64
                 }
             }
             need64 = it_code < 0;
@@ -530,6 +616,7 @@ __global__ __launch_bounds__(BT) void batched_gesv_kernel(const BatchProb *__res
         if (t == 0) {
             info[p.id] = inf;
             if (iters) iters[p.id] = it_code;
+            if (out.tsys) out.tsys[p.id] = wall_clock64() - t_start;
         }
         __syncthreads();
     }
@@ -624,18 +711,19 @@ size_t batched_gemm_prob_bytes() { return sizeof(GemmProb); }
 
 hipError_t batched_gesv(hipStream_t s, int mode, const void *probs, int nprob, int *counter,
                         double *A, double *B, float *W32, double *R, int *ipiv, int *info,
-                        int *iters, int itmax, int grid)
+                        int *iters, int itmax, int grid, const BatchOut &out)
 {
     if (nprob <= 0) return hipSuccess;
+    if (mode == 2 && !out.sel) return hipErrorInvalidValue;
     hipLaunchKernelGGL(zero_int_kernel, dim3(1), dim3(1), 0, s, counter);
-    if (mode == 0)
-        hipLaunchKernelGGL(batched_gesv_kernel<0>, dim3(grid), dim3(BT), 0, s,
-                           (const BatchProb *)probs, nprob, counter, A, B, W32, R, ipiv, info,
-                           iters, itmax);
-    else
-        hipLaunchKernelGGL(batched_gesv_kernel<1>, dim3(grid), dim3(BT), 0, s,
-                           (const BatchProb *)probs, nprob, counter, A, B, W32, R, ipiv, info,
-                           iters, itmax);
+#define FLX_BG(m_)                                                                          \
+    hipLaunchKernelGGL(batched_gesv_kernel<m_>, dim3(grid), dim3(BT), 0, s,               \
+                       (const BatchProb *)probs, nprob, counter, A, B, W32, R, ipiv, info, \
+                       iters, itmax, out)
+    if (mode == 0) FLX_BG(0);
+    else if (mode == 1) FLX_BG(1);
+    else FLX_BG(2);
+#undef FLX_BG
     return hipGetLastError();
 }
 

@@ -28,6 +28,7 @@ struct flx_context {
     flx::LuOptions opt;
     DevBuf W, A, B, C, X, piv0, ipiv, info, perm, tinv;
     DevBuf sb, sx;   // staging of the dsgesv_ seam
+    DevBuf mlp_part; // per-block gradients of flx_mlp_grad
     MixBufs mix;
     int last_error = 0;
     flx::Profiler *prof = nullptr;   // non-null while profiling

@@ -185,13 +185,33 @@ hipError_t mdot(hipStream_t s, const double *V, long ldv, int n, int k, const do
 hipError_t maxpy(hipStream_t s, const double *V, long ldv, int n, int k, const double *h, double *w);
 hipError_t vscale_copy(hipStream_t s, const double *src, double alpha, double *dst, int n);
 
+// ---- precision selector network (mlp.hip) ----------------------------------
+constexpr int MLP_IN = 5, MLP_HIDDEN = 64, MLP_OUT = 4;
+constexpr int MLP_PARAMS = MLP_HIDDEN * MLP_IN + MLP_HIDDEN + MLP_HIDDEN * MLP_HIDDEN +
+                           MLP_HIDDEN + MLP_OUT * MLP_HIDDEN + MLP_OUT;
+size_t mlp_part_floats(int nb);
+hipError_t mlp_forward(hipStream_t s, const float *P, const float *X, int nb, float *logits);
+hipError_t mlp_grad(hipStream_t s, const float *P, const float *X, const int *label,
+                    const float *weight, int nb, float *part, float *grad, float *loss);
+hipError_t mlp_adam(hipStream_t s, float *P, const float *grad, float *m, float *v, int step,
+                    float lr, float wd);
+
 // ---- batched solves and GEMMs (batched.hip) ----------------------------------
 // Problem tables are built by the host in the layouts of batched.hip.
 size_t batched_prob_bytes();
 size_t batched_gemm_prob_bytes();
+// Optional outputs: feat (count x MLP_IN selector features, modes 1-2),
+// tsys (count, in-kernel time per system in 100 MHz ticks), choice (count,
+// the selector's class, mode 2); sel = selector parameters (mode 2).
+struct BatchOut {
+    float *feat = nullptr;
+    long long *tsys = nullptr;
+    int *choice = nullptr;
+    const float *sel = nullptr;
+};
 hipError_t batched_gesv(hipStream_t s, int mode, const void *probs, int nprob, int *counter,
                         double *A, double *B, float *W32, double *R, int *ipiv, int *info,
-                        int *iters, int itmax, int grid);
+                        int *iters, int itmax, int grid, const BatchOut &out);
 hipError_t batched_gemm(hipStream_t s, int tile_cfg, const void *probs, int nprob, int ntiles,
                         int *counter, const double *A, const double *B, double *C, int grid);
 int batched_gemm_tile(int cfg, int *tm, int *tn);

@@ -177,10 +177,28 @@ int flx_dsgesv(flx_handle h, int n, int nrhs, double *A, int lda, int *ipiv,
  * of the inputs when tune = 1, and captures the batch into a HIP graph;
  * flx_batch_execute replays it on the handle's stream. */
 typedef struct flx_batch *flx_batch_t;
+/* Optional per-system outputs / inputs of a solve plan (device pointers,
+ * any may be NULL): the selector's parameters (mode 2), its input features
+ * (count x FLX_MLP_IN, modes 1-2), the in-kernel time of every system in
+ * 100 MHz ticks, and the selector's class (mode 2). */
+typedef struct flx_batch_extra {
+    const float *selector;
+    float *features;
+    long long *ticks;
+    int *choice;
+} flx_batch_extra;
+/* mode 2 = learned: after the FP32 factor the selector network (FP32 MLP,
+ * flx_mlp_*) reads the system's features -- log2 n / 9, nrhs / 8,
+ * log10 kappa_inf / 16 with kappa_inf = ||A||_inf times a lower bound of
+ * ||A^{-1}||_inf from two solves, log10 of the smallest diagonal-dominance
+ * ratio / 4, log10 of the max/min row-norm ratio / 12 -- and either goes on
+ * with the refinement (classes 1-3) or factors in FP64 at once (class 0);
+ * iters[i] = -4 records that choice. */
 int flx_batch_gesv_create(flx_handle h, flx_batch_t *plan, int count,
                           const int *n, const int *nrhs, double *A,
                           double *B, int *ipiv, int *info, int *iters,
-                          int mode, int itmax, int tune);
+                          int mode, int itmax, int tune,
+                          const flx_batch_extra *extra);
 int flx_batch_gemm_create(flx_handle h, flx_batch_t *plan, int count,
                           const int *m, const int *n, const int *k,
                           const double *alpha, const double *beta,
@@ -191,6 +209,21 @@ int flx_batch_execute(flx_batch_t plan);
  * *tuned_ms = the winning candidate's time (0 when not tuned). */
 int flx_batch_config(flx_batch_t plan, int *cfg, double *tuned_ms);
 int flx_batch_destroy(flx_batch_t plan);
+
+/* ------------------------------------------- precision selector network */
+/* FP32 MLP  in (FLX_MLP_IN) -> 64 ReLU -> 64 ReLU -> FLX_MLP_OUT logits
+ * ({FP64, FP32+IR, BF16+GMRES-IR, FP16-scaled+GMRES-IR}), parameters in one
+ * flat array: W1[64][IN], b1[64], W2[64][64], b2[64], W3[OUT][64], b3[OUT].
+ * Forward, mean softmax cross-entropy gradient (deterministic) and an Adam
+ * step, on device pointers.  No counterpart in the reference. */
+enum { FLX_MLP_IN = 5, FLX_MLP_HIDDEN = 64, FLX_MLP_OUT = 4, FLX_MLP_PARAMS = 4804 };
+int flx_mlp_forward(flx_handle h, const float *params, const float *X, int nb,
+                    float *logits);
+int flx_mlp_grad(flx_handle h, const float *params, const float *X,
+                 const int *label, const float *weight, int nb, float *grad,
+                 float *loss);
+int flx_mlp_adam(flx_handle h, float *params, const float *grad, float *m,
+                 float *v, int step, float lr, float weight_decay);
 
 /* --------------------------------------------------- FLENS plugin seams */
 /* CBLAS seam, cxxblas/drivers/cblas.h:798-805; enum values :22-27

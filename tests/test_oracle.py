@@ -240,3 +240,36 @@ def test_big_lu_fixture_consistency():
         assert piv.shape == (8192,) and piv.min() >= 1 and piv.max() <= 8192
         assert np.all(piv >= np.arange(1, 8193))
     assert found >= 1
+
+
+def test_selector_network_oracle_against_finite_differences():
+    """Pins the CPU restatement of the selector network: its analytic
+    gradient equals central finite differences of its loss."""
+    from oracle import mlp as M
+    rng = np.random.default_rng(0)
+    p = M.init(3).astype(np.float64)
+    p[M.PARAMS - M.OUT:] = rng.standard_normal(M.OUT) * 0.1     # nonzero b3
+    X = rng.standard_normal((9, M.IN))
+    y = rng.integers(0, M.OUT, 9)
+    w = rng.uniform(0.5, 2.0, 9)
+    loss, g = M.loss_and_grad(p, X, y, w)
+    idx = rng.choice(M.PARAMS, 60, replace=False)
+    h = 1e-6
+    for i in idx:
+        e = np.zeros(M.PARAMS)
+        e[i] = h
+        fd = (M.loss_and_grad(p + e, X, y, w)[0] - M.loss_and_grad(p - e, X, y, w)[0]) / (2 * h)
+        assert abs(fd - g[i]) <= 1e-6 * max(1.0, abs(g[i])), (i, fd, g[i])
+    # a planted mistake -- the backward pass without the second ReLU mask --
+    # disagrees with the finite differences
+    d = M.unpack(p)
+    z, (X_, a1, h1, a2, h2) = M.forward(p, X)
+    nb = X.shape[0]
+    e = np.exp(z - z.max(1, keepdims=True))
+    d3 = e / e.sum(1, keepdims=True)
+    d3[np.arange(nb), y] -= 1.0
+    d3 *= w[:, None] / nb
+    bad_d2 = d3 @ d["W3"]                       # mask dropped
+    bad_gb2 = bad_d2.sum(0)
+    o = M.IN * M.H + M.H + M.H * M.H             # offset of b2
+    assert np.abs(bad_gb2 - g[o:o + M.H]).max() > 1e-3 * np.abs(g[o:o + M.H]).max()
