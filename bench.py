@@ -1,15 +1,24 @@
-"""Benchmark: FP64 LU with partial pivoting + solve on one MI355X.
+"""Benchmark: dense LU solves on one MI355X.
 
-Workload (the plain FP64 LU configuration): n = 8192, A i.i.d. U[-1,1)
-(seed 2), nrhs = 16 with B i.i.d. U[-1,1) (seed 3), blocked right-looking
-LU through the library's device entry flx_dgesv (getrf + getrs).  One step
-= restore A and B from resident pristine copies (device-to-device) + one
-solve.  The result of the last step must reach a normwise backward error
-<= n*eps64 or the run fails.
+Primary workload (`value`): plain FP64 LU with partial pivoting + solve,
+n = 8192, A i.i.d. U[-1,1) (seed 2), nrhs = 16 with B i.i.d. U[-1,1)
+(seed 3), blocked right-looking LU through the library's device entry
+flx_dgesv (getrf + getrs).  One step = restore A and B from resident
+pristine copies (device-to-device) + one solve.  The result of the last step
+must reach a normwise backward error <= n*eps64 or the run fails.
 
-Prints ONE JSON line (see the keys below).  `roofline` is for the dominant
-kernel (the MFMA trailing-update GEMM), from HIP events recorded around each
-of its launches on the stream it runs on, over the timed region.
+Also measured in the same run (unless --quick), each in its own JSON key:
+  mixed_vs_fp64  mixed-precision LU + FP64 refinement against plain FP64 LU
+                 at n = 16384, A = U diag(sigma) V^T with kappa in
+                 {1e2, 1e6, 1e10}: times, backward / forward errors, speedups;
+  batched        20,000 uneven solves (FP64, FP32 + refinement) and 20,000
+                 uneven GEMMs, one persistent graph-captured launch each;
+  selector       the learned precision selector against the same batch
+                 solved without it.
+
+Prints ONE JSON line.  `roofline` is for the dominant kernel of the primary
+workload (the MFMA trailing-update GEMM), from HIP events recorded around
+each of its launches on the stream it runs on, over the timed region.
 `cpu_baseline` times the oracle's restatement of the reference's generic
 getrf on one core over a bounded sample (the first block step of the same
 n = 8192 factorisation) and projects the full trf+trs flop count.
@@ -88,6 +97,183 @@ def cpu_baseline(n: int, nrhs: int, nb: int = 64, target_s: float = 15.0):
     }
 
 
+def mixed_section(n: int, kappas=(1e2, 1e6, 1e10), seed: int = 1234):
+    """Mixed-precision LU + FP64 refinement against plain FP64 LU on
+    A = U diag(sigma) V^T (Haar U, V; sigma geometric 1 -> 1/kappa), nrhs = 1,
+    b = A x_true.  Mixed = FP32 factor with the solver's automatic ladder
+    (condition estimate -> classic refinement, GMRES-IR or FP64); the dsgesv
+    method (classic refinement, FP64 fallback) and FP16-factor GMRES-IR are
+    timed beside it.  Times include everything (factor, estimate,
+    refinement, fallback)."""
+    import torch
+    import flens_amd as F
+    from flens_amd import testmat as T
+    t = time.perf_counter()
+    mats = T.svd_matrix_device(n, list(kappas), seed)
+    torch.cuda.synchronize()
+    progress(f"mixed: generated n={n} test matrices in {time.perf_counter() - t:.1f} s")
+    fl = 2.0 / 3.0 * n ** 3 + 2.0 * n * n
+    eps = 2.0 ** -53
+    out = {"n": n, "nrhs": 1, "flops_per_solve": fl, "per_kappa": []}
+    A2 = F.DeviceMatrix.empty(n, n)
+    X = F.DeviceMatrix.empty(n, 1)
+
+    def errors(A, Xt, B):
+        a, x, b = A.storage.T, X.storage.T, B.storage.T
+        r = b - a @ x
+        eta = (r.abs().max() / (a.abs().sum(1).max() * x.abs().max() + b.abs().max())).item()
+        fwd = ((X.storage - Xt.storage).abs().max() / Xt.storage.abs().max()).item()
+        return eta, fwd
+
+    tot64 = totmx = 0.0
+    for kappa, A, Xt, B in mats:
+        rec = {"kappa": kappa}
+        best = 1e30
+        for rep in range(2):
+            A2.storage.copy_(A.storage)
+            X.storage.copy_(B.storage)
+            torch.cuda.synchronize()
+            t = time.perf_counter()
+            piv, info = F.lapack.sv(A2, X)
+            torch.cuda.synchronize()
+            best = min(best, time.perf_counter() - t)
+        eta, fwd = errors(A, Xt, B)
+        rec["fp64"] = {"ms": 1e3 * best, "solves_per_s": 1.0 / best, "tflops": fl / best / 1e12,
+                       "backward_error": eta, "forward_error": fwd}
+        for name, kw in (("fp32_auto", dict(prec="fp32", refine="auto", itermax=30, gmres_m=40)),
+                         ("fp32_classic", dict(prec="fp32", refine="classic", itermax=30)),
+                         ("fp16_gmres", dict(prec="fp16", refine="gmres", itermax=3, gmres_m=40))):
+            bt, st = 1e30, None
+            for rep in range(2 if name == "fp32_auto" else 1):
+                X.storage.zero_()
+                A2.storage.copy_(A.storage)
+                torch.cuda.synchronize()
+                t = time.perf_counter()
+                st = F.lapack.sv_mixed(A2, B, X, fallback=True, **kw)
+                torch.cuda.synchronize()
+                bt = min(bt, time.perf_counter() - t)
+            eta, fwd = errors(A, Xt, B)
+            rec[name] = {"ms": 1e3 * bt, "solves_per_s": 1.0 / bt, "tflops_fp64_equiv": fl / bt / 1e12,
+                         "backward_error": eta, "forward_error": fwd, "iter": st["iter"],
+                         "inner_iters": st["inner_iters"], "fallback": st["fallback"],
+                         "factor_ms": st["factor_ms"], "refine_ms": st["refine_ms"],
+                         "kappa_est": st["kappa_est"],
+                         "meets_n_eps": eta <= n * eps,
+                         "speedup_vs_fp64": rec["fp64"]["ms"] / (1e3 * bt)}
+        tot64 += rec["fp64"]["ms"]
+        totmx += rec["fp32_auto"]["ms"]
+        out["per_kappa"].append(rec)
+        progress(f"mixed: kappa={kappa:.0e} fp64 {rec['fp64']['ms']:.1f} ms, "
+                 f"fp32 auto {rec['fp32_auto']['ms']:.1f} ms (iter {rec['fp32_auto']['iter']}, "
+                 f"inner {rec['fp32_auto']['inner_iters']}), "
+                 f"fp32+IR {rec['fp32_classic']['ms']:.1f} ms (iter {rec['fp32_classic']['iter']}), "
+                 f"fp16+GMRES {rec['fp16_gmres']['ms']:.1f} ms (iter {rec['fp16_gmres']['iter']})")
+    out["job"] = {"fp64_solves_per_s": len(kappas) / (tot64 * 1e-3),
+                  "mixed_solves_per_s": len(kappas) / (totmx * 1e-3),
+                  "speedup": tot64 / totmx}
+    return out
+
+
+def batched_section(count: int = 20000, seed: int = 4):
+    """20,000 systems n ~ logU[8,512], A ~ N(0,1), nrhs ~ U{1..8} (FP64 and
+    FP32 + FP64 refinement), and 20,000 GEMMs M,N,K ~ logU[16,512], each
+    batch one persistent graph-captured launch with an autotuned shape.
+    Roofline per problem: max(flops / 78.6 TF, bytes / 8 TB/s), summed."""
+    import numpy as np
+    import torch
+    import flens_amd as F
+    rng = np.random.default_rng(seed)
+    n = np.clip(np.exp(rng.uniform(np.log(8), np.log(512), count)).round(), 8, 512).astype(np.int32)
+    nrhs = rng.integers(1, 9, count).astype(np.int32)
+    nd = n.astype(np.float64)
+    na = int((n.astype(np.int64) ** 2).sum())
+    nb = int((n.astype(np.int64) * nrhs).sum())
+    g = torch.Generator(device="cuda")
+    g.manual_seed(seed)
+    A = torch.randn(na, dtype=torch.float64, device="cuda", generator=g)
+    B = torch.randn(nb, dtype=torch.float64, device="cuda", generator=g)
+    A0, B0 = A.clone(), B.clone()
+    ipiv = torch.zeros(int(n.sum()), dtype=torch.int32, device="cuda")
+    info = torch.zeros(count, dtype=torch.int32, device="cuda")
+    iters = torch.zeros(count, dtype=torch.int32, device="cuda")
+    fl_i = 2.0 / 3.0 * nd ** 3 + 2.0 * nd ** 2 * nrhs
+    by_i = 8.0 * (2.0 * nd ** 2 + 2.0 * nd * nrhs)
+    roof = float(np.maximum(fl_i / (PEAK_FP64_MFMA * 1e12), by_i / (PEAK_HBM * 1e9)).sum())
+    out = {"count": count, "flops": float(fl_i.sum()), "roofline_ms": 1e3 * roof}
+    for mode, name in ((0, "fp64"), (1, "fp32_ir")):
+        plan = F.batched.gesv_plan(n, nrhs, A, B, ipiv, info, iters, mode=mode, itmax=10, tune=True)
+        ts = []
+        for rep in range(3):
+            A.copy_(A0)
+            B.copy_(B0)
+            torch.cuda.synchronize()
+            t = time.perf_counter()
+            plan.execute()
+            torch.cuda.synchronize()
+            ts.append(time.perf_counter() - t)
+        it = iters.cpu().numpy()
+        dt = min(ts)
+        out[name] = {"ms": 1e3 * dt, "solves_per_s": count / dt, "tflops": float(fl_i.sum()) / dt / 1e12,
+                     "roofline_frac": roof / dt, "config": plan.config(),
+                     "info_nonzero": int((info.cpu().numpy() != 0).sum()),
+                     **({"refinement_failures": int((it < 0).sum()),
+                         "mean_iters": float(it[it >= 0].mean()) if (it >= 0).any() else 0.0}
+                        if mode == 1 else {})}
+        plan.close()
+        progress(f"batched {name}: {1e3 * dt:.1f} ms, {count / dt:.0f} solves/s")
+    del A, B, A0, B0
+    m, nn, k = (np.clip(np.exp(rng.uniform(np.log(16), np.log(512), count)).round(), 16, 512)
+                .astype(np.int32) for _ in range(3))
+    GA = torch.randn(int((m.astype(np.int64) * k).sum()), dtype=torch.float64, device="cuda", generator=g)
+    GB = torch.randn(int((k.astype(np.int64) * nn).sum()), dtype=torch.float64, device="cuda", generator=g)
+    GC = torch.randn(int((m.astype(np.int64) * nn).sum()), dtype=torch.float64, device="cuda", generator=g)
+    gfl_i = 2.0 * m.astype(np.float64) * nn * k
+    gby_i = 8.0 * (m.astype(np.float64) * k + k.astype(np.float64) * nn + 2.0 * m * nn)
+    groof = float(np.maximum(gfl_i / (PEAK_FP64_MFMA * 1e12), gby_i / (PEAK_HBM * 1e9)).sum())
+    plan = F.batched.gemm_plan(m, nn, k, np.full(count, 1.0), np.full(count, 0.5), GA, GB, GC, tune=True)
+    ts = []
+    for rep in range(3):
+        torch.cuda.synchronize()
+        t = time.perf_counter()
+        plan.execute()
+        torch.cuda.synchronize()
+        ts.append(time.perf_counter() - t)
+    dt = min(ts)
+    out["gemm"] = {"count": count, "ms": 1e3 * dt, "tflops": float(gfl_i.sum()) / dt / 1e12,
+                   "roofline_ms": 1e3 * groof, "roofline_frac": groof / dt, "config": plan.config()}
+    plan.close()
+    progress(f"batched gemm: {1e3 * dt:.1f} ms")
+    return out
+
+
+def selector_section(train_count: int = 3000, eval_count: int = 3000):
+    """Train the selector on two self-generated batches and compare the
+    learned dispatch with FP32 + refinement for every system on a held-out
+    batch (summed in-kernel time)."""
+    import numpy as np
+    import torch
+    from flens_amd import selector as S
+    t = time.perf_counter()
+    Xs, ys, ws = [], [], []
+    for seed in (101, 102):
+        n, nrhs, kind, A, B = S.generate(train_count, seed, nmax=256)
+        f, lab, wt, t0, t1, it1 = S.collect(n, nrhs, A, B)
+        Xs.append(f); ys.append(lab); ws.append(wt)
+    params, hist = S.train(torch.cat(Xs), np.concatenate(ys), np.concatenate(ws), epochs=120)
+    n, nrhs, kind, A, B = S.generate(eval_count, 303, nmax=256)
+    _, lab, _, t0, t1, it1 = S.collect(n, nrhs, A, B)
+    r2 = S.run(n, nrhs, A, B, 2, params=params)
+    t2 = r2["ticks"].astype(np.float64)
+    res = {"train_samples": 2 * train_count, "eval_systems": eval_count,
+           "loss_first": hist[0], "loss_last": hist[-1],
+           "accuracy": float((np.minimum(r2["choice"], 1) == lab).mean()),
+           "ticks_fp32_ir": float(t1.sum()), "ticks_learned": float(t2.sum()),
+           "ticks_fp64": float(t0.sum()), "gain_vs_fp32_ir": float(t1.sum() / t2.sum()),
+           "seconds": time.perf_counter() - t}
+    progress(f"selector: accuracy {res['accuracy']:.3f}, gain x{res['gain_vs_fp32_ir']:.2f}")
+    return res
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -98,6 +284,9 @@ def main():
     ap.add_argument("--nb", type=int, default=256)
     ap.add_argument("--ib", type=int, default=32)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline sample")
+    ap.add_argument("--quick", action="store_true",
+                    help="only the FP64 LU workload (no mixed / batched / selector sections)")
+    ap.add_argument("--mixed-n", type=int, default=16384)
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only: --gpus {args.gpus} is not supported "
@@ -187,6 +376,10 @@ def main():
                      "launches": g["launches"], "avg_launch_us": 1e3 * g["ms"] / max(1, g["launches"])},
         "breakdown": breakdown,
     }
+    if not args.quick:
+        out["mixed_vs_fp64"] = mixed_section(args.mixed_n)
+        out["batched"] = batched_section()
+        out["selector"] = selector_section()
     if not args.no_cpu:
         progress("CPU baseline: oracle getrf, first block step, 1 core")
         out["cpu_baseline"] = cpu_baseline(n, nrhs)

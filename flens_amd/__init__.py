@@ -52,7 +52,7 @@ _ip = C.POINTER(C.c_int)
 _i, _d, _vp, _u64 = C.c_int, C.c_double, C.c_void_p, C.c_uint64
 
 PREC = {"fp32": 0, "bf16": 1, "fp16": 2}
-REFINE = {"classic": 0, "gmres": 1}
+REFINE = {"classic": 0, "gmres": 1, "auto": 2}
 
 
 class MixedOpts(C.Structure):
@@ -71,6 +71,7 @@ MLP_IN, MLP_HIDDEN, MLP_OUT, MLP_PARAMS = 5, 64, 4, 4804
 class MixedStats(C.Structure):
     _fields_ = [("iter", C.c_int), ("inner_iters", C.c_int), ("fallback", C.c_int),
                 ("info", C.c_int), ("backward_error", C.c_double), ("anorm", C.c_double),
+                ("kappa_est", C.c_double),
                 ("factor_ms", C.c_double), ("refine_ms", C.c_double),
                 ("fallback_ms", C.c_double)]
 

@@ -12,7 +12,7 @@ struct DevBuf {
 // buffers of the mixed-precision solver
 struct MixBufs {
     DevBuf W32, perm, cand, ccount, counters, u11, win, list, zero, lpmax, la, ub;
-    DevBuf inv_l, inv_u, y32, flags, err, dr, dc, part, scal, R, V, w, t, zeros, mpart, hbuf;
+    DevBuf inv_l, inv_u, y32, y64, flags, err, dr, dc, part, scal, R, V, w, t, zeros, mpart, hbuf;
     unsigned epoch = 0;
     // captured factorisation: valid while the key matches
     hipGraphExec_t graph = nullptr;

@@ -161,6 +161,7 @@ struct PrecFactor {
     const int *perm = nullptr;
     const double *dr = nullptr, *dc = nullptr;   // optional power-of-two scales
     float *y32 = nullptr;      // n
+    double *y64 = nullptr;     // n (FP64 application)
     uint64_t *xt = nullptr;    // trsv_words(n) tagged words, zero-initialised once
     unsigned *epoch = nullptr; // host counter paired with xt
     int *err = nullptr;        // device error flag
@@ -174,16 +175,20 @@ hipError_t norm_inf(hipStream_t s, const double *A, long lda, int n, double *par
                     unsigned long long *nrm);
 hipError_t equilibrate(hipStream_t s, const double *A, long lda, int n, double *part,
                        double *dr, double *dc);
-hipError_t trsv_rm(hipStream_t s, int upper, const float *W, long ldw, int n, float *y,
+template <typename TY>
+hipError_t trsv_rm(hipStream_t s, int upper, const float *W, long ldw, int n, TY *y,
                    uint64_t *xt, unsigned epoch, int *err);
 hipError_t zero_slots(hipStream_t s, unsigned long long *p, int k);
 hipError_t vmax(hipStream_t s, const double *v, int n, unsigned long long *nrm);
 hipError_t prec_apply(hipStream_t s, const PrecFactor &f, const double *v, double *z, int add,
                       unsigned long long *vnrm, unsigned long long *znrm);
+hipError_t prec_apply64(hipStream_t s, const PrecFactor &f, const double *v, double *z, int add);
 hipError_t mdot(hipStream_t s, const double *V, long ldv, int n, int k, const double *w,
                 double *part, double *out);
 hipError_t maxpy(hipStream_t s, const double *V, long ldv, int n, int k, const double *h, double *w);
 hipError_t vscale_copy(hipStream_t s, const double *src, double alpha, double *dst, int n);
+hipError_t fill_const(hipStream_t s, double *v, int n, double c);
+hipError_t sign_of(hipStream_t s, const double *x, double *y, int n);
 
 // ---- precision selector network (mlp.hip) ----------------------------------
 constexpr int MLP_IN = 5, MLP_HIDDEN = 64, MLP_OUT = 4;

@@ -211,14 +211,61 @@ FLX_DEV uint64_t tv_word(unsigned epoch, float x)
     return ((uint64_t)epoch << 32) | (uint64_t)__float_as_uint(x);
 }
 
-template <int UPPER>
+// Published solution words: a float travels as one tagged word, a double
+// as two (high and low halves), each (epoch << 32 | 32 payload bits).
+template <typename TY> struct TvWords;
+template <> struct TvWords<float> {
+    static constexpr int N = 1;
+    static FLX_DEV void put(uint64_t *p, unsigned e, float v)
+    {
+        __hip_atomic_store(p, ((uint64_t)e << 32) | __float_as_uint(v), __ATOMIC_RELAXED,
+                           __HIP_MEMORY_SCOPE_AGENT);
+    }
+    static FLX_DEV bool get(const uint64_t *p, unsigned e, float &v)
+    {
+        const uint64_t w = __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        v = __uint_as_float((unsigned)(w & 0xffffffffull));
+        return (unsigned)(w >> 32) == e;
+    }
+    static FLX_DEV float readlane(float v, int l) { return readlane_f(v, l); }
+};
+template <> struct TvWords<double> {
+    static constexpr int N = 2;
+    static FLX_DEV void put(uint64_t *p, unsigned e, double v)
+    {
+        const uint64_t b = (uint64_t)__double_as_longlong(v);
+        __hip_atomic_store(p, ((uint64_t)e << 32) | (b >> 32), __ATOMIC_RELAXED,
+                           __HIP_MEMORY_SCOPE_AGENT);
+        __hip_atomic_store(p + 1, ((uint64_t)e << 32) | (b & 0xffffffffull), __ATOMIC_RELAXED,
+                           __HIP_MEMORY_SCOPE_AGENT);
+    }
+    static FLX_DEV bool get(const uint64_t *p, unsigned e, double &v)
+    {
+        const uint64_t hi = __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        const uint64_t lo = __hip_atomic_load(p + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        v = __longlong_as_double((long long)(((hi & 0xffffffffull) << 32) | (lo & 0xffffffffull)));
+        return (unsigned)(hi >> 32) == e && (unsigned)(lo >> 32) == e;
+    }
+    static FLX_DEV double readlane(double v, int l)
+    {
+        const long long b = __double_as_longlong(v);
+        const int lo = __builtin_amdgcn_readlane((int)(b & 0xffffffffll), l);
+        const int hi = __builtin_amdgcn_readlane((int)(b >> 32), l);
+        return __longlong_as_double(((long long)hi << 32) | (unsigned)lo);
+    }
+};
+
+// TY = float: FP32 substitution (dsgesv's single-precision solve);
+// TY = double: the FP32 factor applied in FP64 arithmetic (GMRES-IR).
+template <typename TY, int UPPER>
 __global__ __launch_bounds__(256) void trsv_rm_kernel(const float *__restrict__ W, long ldw, int n,
-                                                      float *__restrict__ y,
+                                                      TY *__restrict__ y,
                                                       uint64_t *__restrict__ xt,
                                                       unsigned epoch, int *__restrict__ err)
 {
-    __shared__ float s_x[2][TV_FETCH][TV];
-    __shared__ float s_rhs[TV];
+    typedef TvWords<TY> TW;
+    __shared__ TY s_x[2][TV_FETCH][TV];
+    __shared__ TY s_rhs[TV];
     const int nbk = (n + TV - 1) / TV;
     const int g = blockIdx.x;
     const int bi = UPPER ? nbk - 1 - g : g;
@@ -228,7 +275,7 @@ __global__ __launch_bounds__(256) void trsv_rm_kernel(const float *__restrict__ 
     const int ndep = UPPER ? nbk - 1 - bi : bi;
     const bool row_ok = rr < rows;
     const float *wrow = W + (long)(r0 + (row_ok ? rr : 0)) * ldw;
-    const float rhs0 = row_ok ? y[r0 + rr] : 0.0f;
+    const TY rhs0 = row_ok ? y[r0 + rr] : TY(0);
     // wave 0: row `lane` of the diagonal block (identity outside the matrix)
     float drow[TV];
     if (wave == 0) {
@@ -241,7 +288,7 @@ __global__ __launch_bounds__(256) void trsv_rm_kernel(const float *__restrict__ 
     // dependency blocks in groups of TV_FETCH: the group's factor entries are
     // requested first, then wave 0 waits until every block of the group is
     // published, so the two latencies overlap
-    float acc = 0.0f;
+    TY acc = TY(0);
     int par = 0, spins = 0;
     for (int d0 = 0; d0 < ndep; d0 += TV_FETCH, par ^= 1) {
         const int cnt = min(TV_FETCH, ndep - d0);
@@ -257,24 +304,21 @@ __global__ __launch_bounds__(256) void trsv_rm_kernel(const float *__restrict__ 
         }
         if (wave == 0) {
             for (;;) {
-                uint64_t v[TV_FETCH];
+                TY v[TV_FETCH];
+                bool all = true;
 #pragma unroll
                 for (int b = 0; b < TV_FETCH; ++b) {
                     const int d = d0 + b;
                     const int jb = UPPER ? nbk - 1 - d : d;
-                    v[b] = (b < cnt) ? __hip_atomic_load(&xt[(long)jb * TV + lane], __ATOMIC_RELAXED,
-                                                         __HIP_MEMORY_SCOPE_AGENT)
-                                     : 0ull;
+                    bool ok = true;
+                    v[b] = TY(0);
+                    if (b < cnt) ok = TW::get(&xt[((long)jb * TV + lane) * TW::N], epoch, v[b]);
+                    all = all && (__ballot(ok) == ~0ull);
                 }
-                bool all = true;
-#pragma unroll
-                for (int b = 0; b < TV_FETCH; ++b)
-                    if (b < cnt) all = all && (__ballot((unsigned)(v[b] >> 32) == epoch) == ~0ull);
                 if (all || spins > TV_SPIN_LIMIT) {
                     if (!all && lane == 0) atomicOr(err, 2);   // gave up: go on with what is there
 #pragma unroll
-                    for (int b = 0; b < TV_FETCH; ++b)
-                        s_x[par][b][lane] = __uint_as_float((unsigned)(v[b] & 0xffffffffull));
+                    for (int b = 0; b < TV_FETCH; ++b) s_x[par][b][lane] = v[b];
                     break;
                 }
                 ++spins;
@@ -285,50 +329,54 @@ __global__ __launch_bounds__(256) void trsv_rm_kernel(const float *__restrict__ 
 #pragma unroll
         for (int b = 0; b < TV_FETCH; ++b)
 #pragma unroll
-            for (int c = 0; c < 16; ++c) acc = fmaf(wv[b][c], s_x[par][b][q * 16 + c], acc);
+            for (int c = 0; c < 16; ++c) acc = acc + (TY)wv[b][c] * s_x[par][b][q * 16 + c];
     }
     acc += __shfl_xor(acc, 1, 64);
     acc += __shfl_xor(acc, 2, 64);
-    if (q == 0) s_rhs[rr] = row_ok ? rhs0 - acc : 0.0f;
+    if (q == 0) s_rhs[rr] = row_ok ? rhs0 - acc : TY(0);
     __syncthreads();
     if (wave == 0) {
-        float bv = s_rhs[lane];
+        TY bv = s_rhs[lane];
         if (!UPPER) {
 #pragma unroll
             for (int c = 0; c < TV; ++c) {
-                const float xc = readlane_f(bv, c);
-                if (lane > c) bv = fmaf(-drow[c], xc, bv);
+                const TY xc = TW::readlane(bv, c);
+                if (lane > c) bv = bv - (TY)drow[c] * xc;
             }
         } else {
 #pragma unroll
             for (int c = TV - 1; c >= 0; --c) {
-                if (lane == c) bv = bv / drow[c];
-                const float xc = readlane_f(bv, c);
-                if (lane < c) bv = fmaf(-drow[c], xc, bv);
+                if (lane == c) bv = bv / (TY)drow[c];
+                const TY xc = TW::readlane(bv, c);
+                if (lane < c) bv = bv - (TY)drow[c] * xc;
             }
         }
         if (lane < rows) y[r0 + lane] = bv;
-        __hip_atomic_store(&xt[(long)bi * TV + lane], tv_word(epoch, lane < rows ? bv : 0.0f),
-                           __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        TW::put(&xt[((long)bi * TV + lane) * TW::N], epoch, lane < rows ? bv : TY(0));
     }
 }
 
-hipError_t trsv_rm(hipStream_t s, int upper, const float *W, long ldw, int n, float *y,
-                   uint64_t *xt, unsigned epoch, int *err)
+template <typename TY>
+hipError_t trsv_rm(hipStream_t s, int upper, const float *W, long ldw, int n, TY *y, uint64_t *xt,
+                   unsigned epoch, int *err)
 {
     const int nbk = (n + TV - 1) / TV;
     if (nbk <= 0) return hipSuccess;
     ProfScope ps(tl_prof, s, PROF_TRS, (double)n * n, 2.0 * n * (double)n);
     if (upper)
-        hipLaunchKernelGGL(trsv_rm_kernel<1>, dim3(nbk), dim3(256), 0, s, W, ldw, n, y, xt, epoch,
-                           err);
+        hipLaunchKernelGGL((trsv_rm_kernel<TY, 1>), dim3(nbk), dim3(256), 0, s, W, ldw, n, y, xt,
+                           epoch, err);
     else
-        hipLaunchKernelGGL(trsv_rm_kernel<0>, dim3(nbk), dim3(256), 0, s, W, ldw, n, y, xt, epoch,
-                           err);
+        hipLaunchKernelGGL((trsv_rm_kernel<TY, 0>), dim3(nbk), dim3(256), 0, s, W, ldw, n, y, xt,
+                           epoch, err);
     return hipGetLastError();
 }
+template hipError_t trsv_rm<float>(hipStream_t, int, const float *, long, int, float *, uint64_t *,
+                                   unsigned, int *);
+template hipError_t trsv_rm<double>(hipStream_t, int, const float *, long, int, double *,
+                                    uint64_t *, unsigned, int *);
 
-size_t trsv_words(int n) { return (size_t)((n + TV - 1) / TV) * TV; }
+size_t trsv_words(int n) { return 2 * (size_t)((n + TV - 1) / TV) * TV; }   // room for doubles
 
 // ---------------------------------------------------------------------------
 // Preconditioner glue and vector kernels (FP64 vectors of length n)
@@ -423,11 +471,43 @@ hipError_t prec_apply(hipStream_t s, const PrecFactor &f, const double *v, doubl
     FLX_CHECK(vmax(s, v, n, vnrm));
     hipLaunchKernelGGL(prec_in_kernel, dim3((n + 255) / 256), dim3(256), 0, s, v,
                        (const int *)f.perm, f.dr, (const unsigned long long *)vnrm, f.y32, n);
-    FLX_CHECK(trsv_rm(s, 0, f.W, f.ldw, n, f.y32, f.xt, next_epoch(s, f), f.err));
-    FLX_CHECK(trsv_rm(s, 1, f.W, f.ldw, n, f.y32, f.xt, next_epoch(s, f), f.err));
+    FLX_CHECK(trsv_rm<float>(s, 0, f.W, f.ldw, n, f.y32, f.xt, next_epoch(s, f), f.err));
+    FLX_CHECK(trsv_rm<float>(s, 1, f.W, f.ldw, n, f.y32, f.xt, next_epoch(s, f), f.err));
     hipLaunchKernelGGL(prec_out_kernel, dim3((n + 255) / 256), dim3(256), 0, s,
                        (const float *)f.y32, f.dc, (const unsigned long long *)vnrm, z, add, n,
                        znrm);
+    return hipGetLastError();
+}
+
+// FP64 application of the FP32 factor: z = M^{-1} v without rounding the
+// operand or the solution to FP32 (the factor entries are exact in FP64)
+__global__ void prec_in64_kernel(const double *__restrict__ v, const int *__restrict__ perm,
+                                 const double *__restrict__ dr, double *__restrict__ y, int n)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const int p = perm[i];
+    y[i] = dr ? v[p] * dr[p] : v[p];
+}
+
+__global__ void prec_out64_kernel(const double *__restrict__ y, const double *__restrict__ dc,
+                                  double *__restrict__ out, int add, int n)
+{
+    const int j = blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= n) return;
+    const double t = dc ? y[j] * dc[j] : y[j];
+    out[j] = add ? out[j] + t : t;
+}
+
+hipError_t prec_apply64(hipStream_t s, const PrecFactor &f, const double *v, double *z, int add)
+{
+    const int n = f.n;
+    hipLaunchKernelGGL(prec_in64_kernel, dim3((n + 255) / 256), dim3(256), 0, s, v,
+                       (const int *)f.perm, f.dr, f.y64, n);
+    FLX_CHECK(trsv_rm<double>(s, 0, f.W, f.ldw, n, f.y64, f.xt, next_epoch(s, f), f.err));
+    FLX_CHECK(trsv_rm<double>(s, 1, f.W, f.ldw, n, f.y64, f.xt, next_epoch(s, f), f.err));
+    hipLaunchKernelGGL(prec_out64_kernel, dim3((n + 255) / 256), dim3(256), 0, s,
+                       (const double *)f.y64, f.dc, z, add, n);
     return hipGetLastError();
 }
 
@@ -479,6 +559,27 @@ __global__ void vscale_copy_kernel(const double *__restrict__ src, double alpha,
 {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i < n) dst[i] = alpha * src[i];
+}
+
+__global__ void fill_const_kernel(double *v, int n, double c)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) v[i] = c;
+}
+__global__ void sign_of_kernel(const double *__restrict__ x, double *__restrict__ y, int n)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) y[i] = x[i] >= 0.0 ? 1.0 : -1.0;
+}
+hipError_t fill_const(hipStream_t s, double *v, int n, double c)
+{
+    hipLaunchKernelGGL(fill_const_kernel, dim3((n + 255) / 256), dim3(256), 0, s, v, n, c);
+    return hipGetLastError();
+}
+hipError_t sign_of(hipStream_t s, const double *x, double *y, int n)
+{
+    hipLaunchKernelGGL(sign_of_kernel, dim3((n + 255) / 256), dim3(256), 0, s, x, y, n);
+    return hipGetLastError();
 }
 
 size_t mdot_part_doubles(int n, int kmax) { return (size_t)((n + MD_ROWS - 1) / MD_ROWS) * kmax; }

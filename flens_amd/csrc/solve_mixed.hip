@@ -32,7 +32,7 @@ using namespace flx;
 namespace {
 
 constexpr double EPS64 = 1.1102230246251565e-16;   // 2^-53, lamch(Eps)
-constexpr int SLOT_ANRM = 0, SLOT_VN = 1, SLOT_T = 2, SLOT_COL = 4;   // scalar slots
+constexpr int SLOT_ANRM = 0, SLOT_VN = 1, SLOT_T = 2, SLOT_COL = 5;   // scalar slots
 
 struct Opts {
     int prec = MIX_FP32, refine = FLX_REFINE_CLASSIC, itermax = 30, gmres_m = 40;
@@ -98,6 +98,7 @@ struct Solver {
             FLX_TRY(ensure(h, m.ub, 2 * lp_pack_elems(n, o.nb)));
         }
         FLX_TRY(ensure(h, m.y32, sizeof(float) * n));
+        FLX_TRY(ensure(h, m.y64, sizeof(double) * n));
         if (m.flags.cap < sizeof(uint64_t) * trsv_words(n)) m.epoch = 0;
         FLX_TRY(ensure_zeroed(h, m.flags, sizeof(uint64_t) * trsv_words(n)));
         FLX_TRY(ensure(h, m.err, sizeof(int) * 4));
@@ -107,7 +108,8 @@ struct Solver {
         const int nslots = SLOT_COL + 3 * nrhs;
         FLX_TRY(ensure(h, m.scal, sizeof(unsigned long long) * nslots));
         FLX_TRY(ensure(h, m.R, sizeof(double) * (size_t)n * nrhs));
-        if (o.refine == FLX_REFINE_GMRES) {
+        FLX_TRY(ensure(h, m.t, sizeof(double) * n));
+        if (o.refine != FLX_REFINE_CLASSIC) {
             FLX_TRY(ensure(h, m.V, sizeof(double) * (size_t)n * (o.gmres_m + 1)));
             FLX_TRY(ensure(h, m.w, sizeof(double) * n));
             FLX_TRY(ensure(h, m.t, sizeof(double) * n));
@@ -136,6 +138,7 @@ struct Solver {
         f.dr = (o.prec == MIX_FP16) ? (const double *)m.dr.p : nullptr;
         f.dc = (o.prec == MIX_FP16) ? (const double *)m.dc.p : nullptr;
         f.y32 = (float *)m.y32.p;
+        f.y64 = (double *)m.y64.p;
         f.xt = (uint64_t *)m.flags.p;
         f.epoch = &m.epoch;
         f.err = (int *)m.err.p;
@@ -254,6 +257,26 @@ struct Solver {
         return FLX_SUCCESS;
     }
 
+    // kappa_inf lower-bound estimate from the factor: ||A||_inf times
+    // max(||A^{-1} e||_inf, ||A^{-1} sign(A^{-1} e)||_inf), e = ones
+    int estimate(double *kappa)
+    {
+        MixBufs &m = h->mix;
+        double *v = R(0);   // R holds nothing yet
+        double *w = (double *)m.t.p;
+        FLX_TRY(fill_const(s, v, n, 1.0));
+        FLX_TRY(prec_apply64(s, f, v, w, 0));
+        FLX_TRY(sign_of(s, w, v, n));
+        FLX_TRY(zero_slots(s, slot(SLOT_T), 2));
+        FLX_TRY(vmax(s, w, n, slot(SLOT_T)));
+        FLX_TRY(prec_apply64(s, f, v, w, 0));
+        FLX_TRY(vmax(s, w, n, slot(SLOT_T + 1)));
+        FLX_TRY(read_slots());
+        const double ainv = std::max(hslot(SLOT_T), hslot(SLOT_T + 1));
+        *kappa = hslot(SLOT_ANRM) * ainv;
+        return FLX_SUCCESS;
+    }
+
     // classic refinement: returns iterations (>= 0) or -itermax-1
     int classic(int *iters, double *eta)
     {
@@ -288,7 +311,7 @@ struct Solver {
         double *mp = (double *)m.mpart.p;
         std::vector<double> H((size_t)(mm + 1) * mm, 0.0), g(mm + 1, 0.0), cs(mm), sn(mm);
         std::vector<double> hh(2 * (mm + 1) + 2);
-        FLX_RET(precond(R(c), V, 0));
+        FLX_TRY(prec_apply64(s, f, R(c), V, 0));
         FLX_TRY(mdot(s, V, n, n, 1, V, mp, hb + 2 * (mm + 1)));
         FLX_TRY(hipMemcpyAsync(hh.data(), hb + 2 * (mm + 1), sizeof(double), hipMemcpyDeviceToHost, s));
         FLX_TRY(hipStreamSynchronize(s));
@@ -301,7 +324,7 @@ struct Solver {
             double *vk = V + (size_t)k * n;
             FLX_TRY(gemv_resid(s, A, lda, n, vk, (const double *)m.zeros.p, t,
                                (double *)m.part.p, slot(SLOT_T)));       // t = -A v_k
-            FLX_RET(precond(t, w, 0));                                    // w = -M^{-1} A v_k
+            FLX_TRY(prec_apply64(s, f, t, w, 0));                         // w = -M^{-1} A v_k
             FLX_TRY(vscale_copy(s, w, -1.0, w, n));
             // classical Gram-Schmidt, twice
             FLX_TRY(mdot(s, V, n, n, k + 1, w, mp, hb));
@@ -355,10 +378,7 @@ struct Solver {
     {
         MixBufs &m = h->mix;
         FLX_TRY(hipMemsetAsync(m.zeros.p, 0, sizeof(double) * n, s));
-        for (int c = 0; c < nrhs; ++c) {
-            int st = precond(Bc(c), Xc(c), 0);
-            if (st) return st;
-        }
+        for (int c = 0; c < nrhs; ++c) FLX_TRY(prec_apply64(s, f, Bc(c), Xc(c), 0));
         bool conv = false;
         *inner_total = 0;
         for (int it = 0; it <= o.itermax; ++it) {
@@ -435,10 +455,24 @@ static int run_mixed(flx_context *h, int n, int nrhs, double *A, int lda, const 
     if (r) return r;
     const auto t1 = clk::now();
     int iters = 0, inner = 0;
-    double eta = NAN;
+    double eta = NAN, kappa = 0.0;
+    int refine = o.refine;
+    if (code == 0 && refine == FLX_REFINE_AUTO) {
+        r = S.estimate(&kappa);
+        if (r) return r;
+        const double u = (o.prec == MIX_FP32) ? 5.960464477539063e-08
+                         : (o.prec == MIX_FP16) ? 4.8828125e-04 : 3.90625e-03;
+        // kappa_inf overestimates kappa_2 (by up to n); the thresholds are
+        // set where the refinement, with the factor already paid for, is
+        // still expected to cost less than an FP64 LU
+        const double ku = kappa * u;
+        if (!(ku < 100.0)) code = -4;                     // FP64 at once
+        else if (ku < 2.0) { refine = FLX_REFINE_CLASSIC; S.o.itermax = std::max(S.o.itermax, 60); }
+        else refine = FLX_REFINE_GMRES;
+    }
     if (code == 0) {
-        r = (o.refine == FLX_REFINE_GMRES) ? S.gmres_ir(&iters, &inner, &eta)
-                                           : S.classic(&iters, &eta);
+        r = (refine == FLX_REFINE_GMRES) ? S.gmres_ir(&iters, &inner, &eta)
+                                         : S.classic(&iters, &eta);
         if (r) return r;
         int terr[4] = {0, 0, 0, 0};
         FLX_TRY(hipMemcpy(terr, h->mix.err.p, sizeof terr, hipMemcpyDeviceToHost));
@@ -447,7 +481,7 @@ static int run_mixed(flx_context *h, int n, int nrhs, double *A, int lda, const 
     }
     const auto t2 = clk::now();
     int info = 0, fell = 0;
-    if (code < 0 && (o.dsgesv || o.fallback)) {
+    if (code < 0 && (o.dsgesv || o.fallback || code == -4)) {   // -4: the ladder chose FP64
         r = S.fallback64(&info, ipiv);
         if (r) return r;
         fell = 1;
@@ -473,6 +507,7 @@ static int run_mixed(flx_context *h, int n, int nrhs, double *A, int lda, const 
         st->info = info;
         st->backward_error = eta;
         st->anorm = S.hslot(SLOT_ANRM);
+        st->kappa_est = kappa;
         st->factor_ms = std::chrono::duration<double, std::milli>(t1 - t0).count();
         st->refine_ms = std::chrono::duration<double, std::milli>(t2 - t1).count();
         st->fallback_ms = std::chrono::duration<double, std::milli>(t3 - t2).count();
@@ -488,8 +523,8 @@ extern "C" int flx_dgesv_mixed(flx_handle h, int n, int nrhs, double *A, int lda
     if (!h || !valid_dims(n, nrhs, lda, ldb, ldx)) return FLX_INVALID_ARG;
     Opts o;
     if (opt) {
-        if (opt->prec < 0 || opt->prec > 2 || opt->refine < 0 || opt->refine > 1 ||
-            opt->itermax < 0 || opt->gmres_m < 0 || opt->gmres_m > 200)
+        if (opt->prec < 0 || opt->prec > 2 || opt->refine < 0 || opt->refine > 2 ||
+            opt->itermax < 0 || opt->gmres_m < 0 || opt->gmres_m > 400)
             return FLX_INVALID_ARG;
         o.prec = opt->prec;
         o.refine = opt->refine;

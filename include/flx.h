@@ -118,7 +118,11 @@ int flx_fill_uniform(flx_handle h, double *A, int m, int n, int lda,
  * lapack.in.h:10611-10623, netlib dsgesv.f:131-337), and extends it with
  * BF16/FP16 factors and GMRES-IR. */
 enum flx_mixed_prec { FLX_MIXED_FP32 = 0, FLX_MIXED_BF16 = 1, FLX_MIXED_FP16 = 2 };
-enum flx_refine { FLX_REFINE_CLASSIC = 0, FLX_REFINE_GMRES = 1 };
+/* FLX_REFINE_AUTO: after the factor, kappa_inf is estimated from two solves
+ * with it (||A||_inf times a lower bound of ||A^{-1}||_inf); with u the
+ * factor's unit roundoff, kappa u < 2 -> classic (at least 60 steps),
+ * < 100 -> GMRES-IR, else the FP64 LU at once (stats.iter = -4). */
+enum flx_refine { FLX_REFINE_CLASSIC = 0, FLX_REFINE_GMRES = 1, FLX_REFINE_AUTO = 2 };
 typedef struct flx_mixed_opts {
     int prec;          /* enum flx_mixed_prec (FP16 adds power-of-two row and
                           column equilibration and per-step operand scaling) */
@@ -140,6 +144,7 @@ typedef struct flx_mixed_stats {
     int info;              /* INFO of the FP64 fallback factorisation */
     double backward_error; /* max over columns, inf-norm, at exit */
     double anorm;          /* ||A||_inf */
+    double kappa_est;      /* condition estimate (FLX_REFINE_AUTO), else 0 */
     double factor_ms, refine_ms, fallback_ms;
 } flx_mixed_stats;
 int flx_dgesv_mixed(flx_handle h, int n, int nrhs, double *A, int lda,

@@ -48,15 +48,20 @@ def test_fp32_classic_converges_on_sizes(gpu, n):
     assert fwd_err(x, xt) <= 4.0 * 1e2 * backward_error(a, x, b) + 100 * EPS
 
 
-@pytest.mark.parametrize("prec,refine,kappa", [
-    ("fp32", "classic", 1e6), ("fp32", "gmres", 1e6), ("fp32", "gmres", 1e10),
-    ("bf16", "gmres", 1e2), ("bf16", "gmres", 1e4), ("fp16", "gmres", 1e2),
-    ("fp16", "gmres", 1e4), ("bf16", "classic", 1e2), ("fp16", "classic", 1e2)])
-def test_ladder_reaches_fp64_backward_error(gpu, prec, refine, kappa):
+# BF16 factors (u = 2^-8) precondition only mildly conditioned systems: at
+# n = 1200, n u_bf16 > 1, so BF16 is exercised at kappa = 1e2 only.  FP32
+# factors applied in FP64 precondition GMRES up to kappa ~ 1e8; beyond that
+# the solver's ladder goes to FP64 (test_auto_ladder).
+@pytest.mark.parametrize("prec,refine,kappa,gm", [
+    ("fp32", "classic", 1e6, 0), ("fp32", "gmres", 1e6, 0), ("fp32", "gmres", 1e8, 100),
+    ("bf16", "gmres", 1e2, 0), ("fp16", "gmres", 1e2, 0), ("fp16", "gmres", 1e4, 0),
+    ("bf16", "classic", 1e2, 0), ("fp16", "classic", 1e2, 0)])
+def test_ladder_reaches_fp64_backward_error(gpu, prec, refine, kappa, gm):
     n = 1200
     a = T.svd_matrix(n, kappa, 77)
     b, xt = T.rhs_from_solution(a, 1, 5)
-    x, st, _ = run_mixed(gpu, a, b, prec=prec, refine=refine, fallback=False, itermax=50)
+    x, st, _ = run_mixed(gpu, a, b, prec=prec, refine=refine, fallback=False, itermax=50,
+                         gmres_m=gm)
     eta = backward_error(a, x, b)
     assert st["iter"] >= 0 and st["fallback"] == 0, st
     assert eta <= n * EPS, (eta, st)
@@ -166,3 +171,19 @@ def test_graph_replays_match_eager(gpu):
             if rep == 1:
                 x2, st2, _ = run_mixed(gpu, a2, b2, prec=prec, refine="classic", graph=True)
                 assert backward_error(a2, x2, b2) <= 400 * EPS
+
+
+def test_auto_ladder_picks_by_condition_estimate(gpu):
+    """refine='auto': after the FP32 factor a condition estimate chooses
+    classic refinement, GMRES-IR or FP64 at once; every result reaches the
+    FP64 backward-error level, and kappa = 1e12 goes straight to FP64."""
+    n = 1000
+    for kappa, want in ((1e1, "classic"), (1e12, "fp64")):
+        a = T.svd_matrix(n, kappa, 31)
+        b, xt = T.rhs_from_solution(a, 1, 32)
+        x, st, _ = run_mixed(gpu, a, b, prec="fp32", refine="auto")
+        assert backward_error(a, x, b) <= n * EPS, (kappa, st)
+        if want == "fp64":
+            assert st["fallback"] == 1 and st["iter"] == -4, st
+        else:
+            assert st["fallback"] == 0 and st["inner_iters"] == 0 and st["iter"] >= 0, st
