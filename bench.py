@@ -287,7 +287,16 @@ def main():
     ap.add_argument("--quick", action="store_true",
                     help="only the FP64 LU workload (no mixed / batched / selector sections)")
     ap.add_argument("--mixed-n", type=int, default=16384)
+    ap.add_argument("--section", choices=["mixed", "batched", "selector"],
+                    help="run only this section and print its JSON (for profiling)")
     args = ap.parse_args()
+    if args.section:
+        import torch
+        assert torch.cuda.is_available(), "bench.py needs a GPU"
+        fn = {"mixed": lambda: mixed_section(args.mixed_n), "batched": batched_section,
+              "selector": selector_section}[args.section]
+        print(json.dumps({args.section: fn()}))
+        return
     if args.gpus != 1:
         print(f"bench.py measures one GPU only: --gpus {args.gpus} is not supported "
               f"(the LU path is single-GPU; see DESIGN.md for how it would shard)",
