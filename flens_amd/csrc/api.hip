@@ -87,6 +87,7 @@ extern "C" int flx_destroy(flx_handle h)
     free_buf(h->W); free_buf(h->A); free_buf(h->B); free_buf(h->C); free_buf(h->X);
     free_buf(h->piv0); free_buf(h->ipiv); free_buf(h->info); free_buf(h->perm);
     free_buf(h->tinv); free_buf(h->sb); free_buf(h->sx); free_buf(h->mlp_part);
+    free_buf(h->trsw);
     MixBufs &x = h->mix;
     for (DevBuf *b : {&x.W32, &x.perm, &x.cand, &x.ccount, &x.counters, &x.u11, &x.win, &x.list,
                       &x.zero, &x.lpmax, &x.la, &x.ub, &x.inv_l, &x.inv_u, &x.y32, &x.y64, &x.flags,
@@ -251,8 +252,11 @@ int dgetrs_dev(flx_context *h, int trans, int n, int nrhs, const double *A, int 
             perm = (const int *)h->perm.p;
         }
         FLX_TRY(gather_rows_cm(h->stream, B, ldb, T1, n, perm, n, nrhs));
-        FLX_TRY(trsm_cm_f64(h->stream, 0, 0, 1, n, nrhs, A, lda, T1, n, T2, n, ti));
-        FLX_TRY(trsm_cm_f64(h->stream, 1, 0, 0, n, nrhs, A, lda, T2, n, B, ldb, ti));
+        FLX_TRY(init_work(h));
+        FLX_TRY(ensure_zeroed(h, h->trsw, sizeof(uint64_t) * trs_persist_words(n)));
+        FLX_TRY(trs_persist_f64(h->stream, n, nrhs, A, lda, T1, n, T2, B, ldb,
+                                (uint64_t *)h->trsw.p, &h->trs_epoch, h->wk.flags));
+        (void)ti;
     } else {
         FLX_TRY(copy_cm(h->stream, B, ldb, T1, n, n, nrhs));
         FLX_TRY(trsm_cm_f64(h->stream, 1, 1, 0, n, nrhs, A, lda, T1, n, T2, n, ti));

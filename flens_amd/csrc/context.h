@@ -29,6 +29,8 @@ struct flx_context {
     DevBuf W, A, B, C, X, piv0, ipiv, info, perm, tinv;
     DevBuf sb, sx;   // staging of the dsgesv_ seam
     DevBuf mlp_part; // per-block gradients of flx_mlp_grad
+    DevBuf trsw;     // tagged solution words of the single-launch getrs
+    unsigned trs_epoch = 0;
     MixBufs mix;
     int last_error = 0;
     flx::Profiler *prof = nullptr;   // non-null while profiling

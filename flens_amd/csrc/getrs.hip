@@ -264,6 +264,227 @@ hipError_t trsm_cm_f64(hipStream_t s, int upper, int trans, int unit, int n, int
 }
 
 // ---------------------------------------------------------------------------
+// Single-launch triangular solve (NoTrans L then U), column-major FP64
+// factor, up to 16 right-hand sides per launch.  One 256-thread workgroup
+// per 64-row block; workgroup g owns block g (L) or nbk-1-g (U), so every
+// block it depends on belongs to a workgroup dispatched before it.
+//   * A finished block publishes its 64 x 16 solution values as pairs of
+//     64-bit words (epoch << 32 | half of the double), so one agent-scope
+//     load tests readiness and delivers the value; waits are bounded.
+//   * Dependency phase: lane = row, wave w = columns w*16..+15 of each
+//     finished block, all 16 right-hand sides; the factor entries of a pair
+//     of blocks are requested before waiting for them.
+//   * Diagonal block: wave w substitutes right-hand sides 4w..4w+3, one row
+//     per lane, the solved entry broadcast by readlane.
+// ---------------------------------------------------------------------------
+constexpr int TP_BS = 64, TP_RHS = 16, TP_GRP = 2;
+constexpr int TP_SPIN_LIMIT = 1 << 22;
+
+FLX_DEV double tp_readlane(double v, int l)
+{
+    const long long b = __double_as_longlong(v);
+    const int lo = __builtin_amdgcn_readlane((int)(b & 0xffffffffll), l);
+    const int hi = __builtin_amdgcn_readlane((int)(b >> 32), l);
+    return __longlong_as_double(((long long)hi << 32) | (unsigned)lo);
+}
+
+template <int UPPER>
+__global__ __launch_bounds__(256) void trs_persist_kernel(const double *__restrict__ LU, long lda,
+                                                          int n, int nrhs,
+                                                          const double *__restrict__ Bin, long ldb,
+                                                          double *__restrict__ Xout, long ldx,
+                                                          uint64_t *__restrict__ xt,
+                                                          unsigned epoch, int *__restrict__ err)
+{
+    __shared__ double s_x[2][TP_GRP][TP_BS][TP_RHS + 1];
+    __shared__ double s_red[4][TP_BS][TP_RHS + 1];
+    __shared__ double s_d[TP_BS][TP_BS + 1];
+    __shared__ int s_ok;
+    const int nbk = (n + TP_BS - 1) / TP_BS;
+    const int g = blockIdx.x;
+    const int bi = UPPER ? nbk - 1 - g : g;
+    const int r0 = bi * TP_BS, rows = min(TP_BS, n - r0);
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int ndep = UPPER ? nbk - 1 - bi : bi;
+    const bool row_ok = lane < rows;
+    const int row = r0 + (row_ok ? lane : 0);
+    // the diagonal block, staged in LDS at the start (identity outside the matrix)
+    for (int q = tid; q < TP_BS * TP_BS; q += 256) {
+        const int c = q / TP_BS, r = q % TP_BS;
+        s_d[r][c] = (r < rows && c < rows) ? LU[(long)(r0 + c) * lda + r0 + r] : (r == c ? 1.0 : 0.0);
+    }
+    double acc[TP_RHS];
+#pragma unroll
+    for (int k = 0; k < TP_RHS; ++k) acc[k] = 0.0;
+    int par = 0, spins = 0;
+    for (int d0 = 0; d0 < ndep; d0 += TP_GRP, par ^= 1) {
+        const int cnt = min(TP_GRP, ndep - d0);
+        double lv[TP_GRP][16];
+#pragma unroll
+        for (int b = 0; b < TP_GRP; ++b) {
+            const int d = d0 + b;
+            const int jb = UPPER ? nbk - 1 - d : d;
+            const int c0 = jb * TP_BS + wave * 16;
+#pragma unroll
+            for (int c = 0; c < 16; ++c)
+                lv[b][c] = (b < cnt && row_ok && c0 + c < n) ? LU[(long)(c0 + c) * lda + row] : 0.0;
+        }
+        // one thread waits for each block's sentinel word (written last by
+        // its producer's thread), then every thread fetches 8 words per
+        // block and all verify their tags (a late word repeats the wait)
+        for (;;) {
+            if (tid == 0) {
+                for (;;) {
+                    bool ready = true;
+                    for (int b = 0; b < cnt; ++b) {
+                        const int d = d0 + b;
+                        const int jb = UPPER ? nbk - 1 - d : d;
+                        const uint64_t *w = xt + ((long)(jb * TP_BS + TP_BS - 1) * TP_RHS + TP_RHS - 1) * 2;
+                        ready = ready && (unsigned)(__hip_atomic_load(w + 1, __ATOMIC_RELAXED,
+                                                                      __HIP_MEMORY_SCOPE_AGENT) >> 32) == epoch;
+                    }
+                    // every poll counts, also those after a failed verification
+                    if (++spins > TP_SPIN_LIMIT || ready) break;
+                    __builtin_amdgcn_s_sleep(2);
+                }
+                s_ok = spins > TP_SPIN_LIMIT;
+            }
+            __syncthreads();
+            const bool gave_up = s_ok;
+            bool ok = true;
+#pragma unroll
+            for (int b = 0; b < TP_GRP; ++b) {
+                if (b < cnt) {
+                    const int d = d0 + b;
+                    const int jb = UPPER ? nbk - 1 - d : d;
+#pragma unroll
+                    for (int e = 0; e < 4; ++e) {
+                        const int idx = tid * 4 + e;          // 0 .. 1023 = row * 16 + rhs
+                        const int rr = idx >> 4, k = idx & 15;
+                        const uint64_t *w = xt + ((long)(jb * TP_BS + rr) * TP_RHS + k) * 2;
+                        const uint64_t hi = __hip_atomic_load(w, __ATOMIC_RELAXED,
+                                                              __HIP_MEMORY_SCOPE_AGENT);
+                        const uint64_t lo = __hip_atomic_load(w + 1, __ATOMIC_RELAXED,
+                                                              __HIP_MEMORY_SCOPE_AGENT);
+                        ok = ok && (unsigned)(hi >> 32) == epoch && (unsigned)(lo >> 32) == epoch;
+                        s_x[par][b][rr][k] = __longlong_as_double(
+                            (long long)(((hi & 0xffffffffull) << 32) | (lo & 0xffffffffull)));
+                    }
+                }
+            }
+            const int all = __syncthreads_and(ok);
+            if (all) break;
+            if (gave_up) {                 // go on with what is there
+                if (tid == 0) atomicOr(err, 2);
+                break;
+            }
+        }
+#pragma unroll
+        for (int b = 0; b < TP_GRP; ++b)
+#pragma unroll
+            for (int c = 0; c < 16; ++c) {
+                const double l = lv[b][c];
+#pragma unroll
+                for (int k = 0; k < TP_RHS; ++k) acc[k] = fma(l, s_x[par][b][wave * 16 + c][k], acc[k]);
+            }
+    }
+#pragma unroll
+    for (int k = 0; k < TP_RHS; ++k) s_red[wave][lane][k] = acc[k];
+    __syncthreads();
+    // right-hand side of this block, wave w: rhs 4w..4w+3
+    double v[4];
+#pragma unroll
+    for (int kk = 0; kk < 4; ++kk) {
+        const int k = wave * 4 + kk;
+        double t = 0.0;
+        if (row_ok && k < nrhs) {
+            t = Bin[(long)k * ldb + r0 + lane];
+#pragma unroll
+            for (int w = 0; w < 4; ++w) t -= s_red[w][lane][k];
+        }
+        v[kk] = t;
+    }
+    double drow[TP_BS];
+#pragma unroll
+    for (int c = 0; c < TP_BS; ++c) drow[c] = s_d[lane][c];
+    if (!UPPER) {
+#pragma unroll
+        for (int c = 0; c < TP_BS; ++c) {
+#pragma unroll
+            for (int kk = 0; kk < 4; ++kk) {
+                const double xc = tp_readlane(v[kk], c);
+                if (lane > c) v[kk] = fma(-drow[c], xc, v[kk]);
+            }
+        }
+    } else {
+#pragma unroll
+        for (int c = TP_BS - 1; c >= 0; --c) {
+#pragma unroll
+            for (int kk = 0; kk < 4; ++kk) {
+                if (lane == c) v[kk] = v[kk] / drow[c];
+                const double xc = tp_readlane(v[kk], c);
+                if (lane < c) v[kk] = fma(-drow[c], xc, v[kk]);
+            }
+        }
+    }
+    uint64_t wl = 0;
+    uint64_t *wsent = nullptr;
+#pragma unroll
+    for (int kk = 0; kk < 4; ++kk) {
+        const int k = wave * 4 + kk;
+        const double x = (row_ok && k < nrhs) ? v[kk] : 0.0;
+        if (row_ok && k < nrhs) Xout[(long)k * ldx + r0 + lane] = x;
+        const uint64_t b = (uint64_t)__double_as_longlong(x);
+        uint64_t *w = xt + ((long)(r0 + lane) * TP_RHS + k) * 2;
+        __hip_atomic_store(w, ((uint64_t)epoch << 32) | (b >> 32), __ATOMIC_RELAXED,
+                           __HIP_MEMORY_SCOPE_AGENT);
+        const uint64_t lo = ((uint64_t)epoch << 32) | (b & 0xffffffffull);
+        if (lane == TP_BS - 1 && k == TP_RHS - 1) { wsent = w + 1; wl = lo; }
+        else __hip_atomic_store(w + 1, lo, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    }
+    // the sentinel goes last, after every other word of the block is out
+    __threadfence();
+    __syncthreads();
+    if (wsent) __hip_atomic_store(wsent, wl, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+size_t trs_persist_words(int n) { return (size_t)((n + TP_BS - 1) / TP_BS) * TP_BS * TP_RHS * 2; }
+
+__global__ void zero_words_kernel(uint64_t *p, size_t n)
+{
+    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = 0ull;
+}
+
+// X = U^{-1} L^{-1} Bp (Bp already row-permuted); T is n x 16 scratch.
+hipError_t trs_persist_f64(hipStream_t s, int n, int nrhs, const double *LU, long lda,
+                           const double *Bp, long ldbp, double *T, double *X, long ldx,
+                           uint64_t *xt, unsigned *epoch, int *err)
+{
+    const int nbk = (n + TP_BS - 1) / TP_BS;
+    if (nbk <= 0 || nrhs <= 0) return hipSuccess;
+    auto next = [&]() -> unsigned {
+        if (++*epoch == 0u) {
+            const size_t nw = trs_persist_words(n);
+            hipLaunchKernelGGL(zero_words_kernel, dim3((unsigned)((nw + 255) / 256)), dim3(256), 0,
+                               s, xt, nw);
+            *epoch = 1u;
+        }
+        return *epoch;
+    };
+    for (int k0 = 0; k0 < nrhs; k0 += TP_RHS) {
+        const int kc = min(TP_RHS, nrhs - k0);
+        ProfScope ps(tl_prof, s, PROF_TRS, 2.0 * (double)n * n * kc,
+                     8.0 * ((double)n * n + 4.0 * n * kc));
+        hipLaunchKernelGGL(trs_persist_kernel<0>, dim3(nbk), dim3(256), 0, s, LU, lda, n, kc,
+                           Bp + (long)k0 * ldbp, ldbp, T, (long)n, xt, next(), err);
+        hipLaunchKernelGGL(trs_persist_kernel<1>, dim3(nbk), dim3(256), 0, s, LU, lda, n, kc,
+                           (const double *)T, (long)n, X + (long)k0 * ldx, ldx, xt, next(), err);
+    }
+    return hipGetLastError();
+}
+
+// ---------------------------------------------------------------------------
 // dlaswp on a column-major matrix: rows k1..k2 (1-based), incx = +1 forward
 // / -1 backward (flens/lapack/la/laswp.tcc:54-93).  One thread per column.
 // ---------------------------------------------------------------------------
