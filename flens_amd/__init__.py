@@ -38,7 +38,7 @@ LIB_PATH = os.path.join(HERE, "libflx.so")
 # every symbol include/flx.h declares
 EXPORTS = [
     "flx_create", "flx_destroy", "flx_set_stream", "flx_sync", "flx_status_string",
-    "flx_set_lu_blocking", "flx_dgemm", "flx_dgetrf", "flx_dgetrs", "flx_dgesv",
+    "flx_set_lu_blocking", "flx_set_lu_lookahead", "flx_dgemm", "flx_dgetrf", "flx_dgetrs", "flx_dgesv",
     "flx_dlaswp", "flx_fill_uniform", "flx_profile_start", "flx_profile_stop",
     "flx_dgesv_mixed", "flx_dsgesv",
     "flx_batch_gesv_create", "flx_batch_gemm_create", "flx_batch_execute", "flx_batch_config",
@@ -101,6 +101,7 @@ def lib():
     L.flx_status_string.argtypes = [_i]
     L.flx_status_string.restype = C.c_char_p
     L.flx_set_lu_blocking.argtypes = [_vp, _i, _i]
+    L.flx_set_lu_lookahead.argtypes = [_vp, _i]
     L.flx_dgemm.argtypes = [_vp, _i, _i, _i, _i, _i, _d, _vp, _i, _vp, _i, _d, _vp, _i]
     L.flx_dgetrf.argtypes = [_vp, _i, _i, _vp, _i, _vp, _vp]
     L.flx_dgetrs.argtypes = [_vp, _i, _i, _i, _vp, _i, _vp, _vp, _i]
@@ -162,6 +163,12 @@ class Handle:
 
     def set_lu_blocking(self, nb: int, ib: int = 32):
         _check(lib().flx_set_lu_blocking(self._h, nb, ib), "flx_set_lu_blocking")
+
+    def set_lu_lookahead(self, depth: int):
+        """0: each trailing update finishes before the next panel; 1: the
+        next block's columns are updated first and the rest of the update
+        runs on a second stream beside the next block's panels."""
+        _check(lib().flx_set_lu_lookahead(self._h, depth), "flx_set_lu_lookahead")
 
     def sync(self):
         _check(lib().flx_sync(self._h), "flx_sync")

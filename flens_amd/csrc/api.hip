@@ -102,6 +102,10 @@ extern "C" int flx_destroy(flx_handle h)
     if (h->wk.list) hipFree(h->wk.list);
     if (h->wk.flags) hipFree(h->wk.flags);
     if (h->wk.moved) hipFree(h->wk.moved);
+    if (h->wk.side) hipStreamDestroy(h->wk.side);
+    if (h->wk.ev_a) hipEventDestroy(h->wk.ev_a);
+    if (h->wk.ev_r) hipEventDestroy(h->wk.ev_r);
+    free_buf(h->lu_inv); free_buf(h->lu_mv);
     prof_delete(h->prof);
     delete h;
     return FLX_SUCCESS;
@@ -119,6 +123,13 @@ extern "C" int flx_set_lu_blocking(flx_handle h, int nb, int ib)
     if (!h || nb < 1 || ib < 1 || ib > 64 || nb > 256) return FLX_INVALID_ARG;
     h->opt.nb = nb;
     h->opt.ib = ib;
+    return FLX_SUCCESS;
+}
+
+extern "C" int flx_set_lu_lookahead(flx_handle h, int depth)
+{
+    if (!h || depth < 0 || depth > 1) return FLX_INVALID_ARG;
+    h->opt.lookahead = depth;
     return FLX_SUCCESS;
 }
 
@@ -207,11 +218,20 @@ int dgetrf_dev(flx_context *h, int m, int n, double *A, int lda, int *ipiv, int 
     const long ldw = round_ld(n);
     FLX_TRY(ensure(h, h->W, sizeof(double) * (size_t)ldw * (size_t)(m > 0 ? m : 1)));
     FLX_TRY(ensure(h, h->piv0, sizeof(int) * (size_t)(mn > 0 ? mn : 1)));
-    if (keep_perm) {
-        FLX_TRY(ensure(h, h->perm, sizeof(int) * (size_t)(m > 0 ? m : 1)));
-        h->wk.perm = (int *)h->perm.p;
-    } else {
-        h->wk.perm = nullptr;
+    (void)keep_perm;   // the composed permutation is always kept (lookahead needs it)
+    FLX_TRY(ensure(h, h->perm, sizeof(int) * (size_t)(m > 0 ? m : 1)));
+    h->wk.perm = (int *)h->perm.p;
+    FLX_TRY(ensure(h, h->lu_inv, sizeof(int) * (size_t)(m > 0 ? m : 1)));
+    FLX_TRY(ensure(h, h->lu_mv, sizeof(int) * (1 + 2 * LU_BLOCK_MOVES)));
+    h->wk.inv = (int *)h->lu_inv.p;
+    h->wk.mv = (int *)h->lu_mv.p;
+    if (!h->wk.side) {
+        FLX_TRY(hipStreamCreateWithFlags(&h->wk.side, hipStreamNonBlocking));
+        FLX_TRY(hipEventCreateWithFlags(&h->wk.ev_a, hipEventDisableTiming));
+        FLX_TRY(hipEventCreateWithFlags(&h->wk.ev_r, hipEventDisableTiming));
+        int cus = 0;
+        FLX_TRY(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, h->device));
+        h->wk.side_groups = cus > 8 ? cus - cus / 4 : cus;
     }
     double *W = (double *)h->W.p;
     FLX_TRY(fill_int(h->stream, info, 1, INT_MAX));

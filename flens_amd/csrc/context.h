@@ -30,6 +30,7 @@ struct flx_context {
     DevBuf sb, sx;   // staging of the dsgesv_ seam
     DevBuf mlp_part; // per-block gradients of flx_mlp_grad
     DevBuf trsw;     // tagged solution words of the single-launch getrs
+    DevBuf lu_inv, lu_mv;   // lookahead: inverse permutation, block moves
     unsigned trs_epoch = 0;
     MixBufs mix;
     int last_error = 0;

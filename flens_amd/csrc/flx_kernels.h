@@ -47,12 +47,15 @@ struct ProfScope {
 // ---- GEMM (gemm.hip) -------------------------------------------------------
 // column-major C <- beta*C + alpha*op(A)*op(B); blas_beta0 selects the BLAS
 // convention that beta == 0 never reads C (else the FLENS generic 0*C).
+// max_groups > 0 caps the workgroups (each then loops over several tiles).
 hipError_t gemm_f64(hipStream_t s, int ta, int tb, int M, int N, int K,
                     double alpha, const double *A, long lda, const double *B,
-                    long ldb, double beta, double *C, long ldc, int blas_beta0);
+                    long ldb, double beta, double *C, long ldc, int blas_beta0,
+                    int max_groups = 0);
 hipError_t gemm_f32(hipStream_t s, int ta, int tb, int M, int N, int K,
                     float alpha, const float *A, long lda, const float *B,
-                    long ldb, float beta, float *C, long ldc, int blas_beta0);
+                    long ldb, float beta, float *C, long ldc, int blas_beta0,
+                    int max_groups = 0);
 
 // ---- LU (getrf.hip) --------------------------------------------------------
 // Device scratch owned by a handle and reused by every factorisation.
@@ -66,11 +69,22 @@ struct LuWork {
     unsigned long long *moved = nullptr; // [0] = row-interchange elements moved
     uint32_t gen = 0;           // all-gather generation counter
     Profiler *prof = nullptr;
+    // lookahead (the rest of each trailing update on a second stream)
+    int *inv = nullptr;         // inverse of perm before the current block (length m)
+    int *mv = nullptr;          // block moves [count, dst, src, ...], 1 + 2 * LU_BLOCK_MOVES
+    hipStream_t side = nullptr; // default priority, ordered by the two events
+    hipEvent_t ev_a = nullptr, ev_r = nullptr;
+    // workgroups the side stream's launches may use at once, so that the
+    // panels and the small launches of the next block always find free
+    // compute units (a trailing update left uncapped fills the device)
+    int side_groups = 0;
 };
+constexpr int LU_BLOCK_MOVES = 2 * 256;   // rows one outer block can move
 
 struct LuOptions {
     int nb = 256;   // outer block (trailing update K)
     int ib = 32;    // panel width of the cooperative panel kernel
+    int lookahead = 1;   // overlap the next block's panels with the rest of the update
 };
 
 size_t lu_exch_words();
@@ -94,7 +108,7 @@ hipError_t getrf_rm_f64(hipStream_t s, LuWork &wk, double *W, long ldw, int m,
 // X <- L^{-1} X with L the k x k unit lower triangle at L (k <= 256).
 template <typename T>
 hipError_t trsm_rm_llu(hipStream_t s, Profiler *prof, const T *L, long ldl, T *X, long ldx,
-                       int k, int ncols);
+                       int k, int ncols, int max_groups = 0);
 // Applies an interchange list (per-workgroup slices of PANEL_LIST_SLICE ints,
 // G slices) to the columns [0, c1) and [c2, n) of the row-major W, and
 // composes it into perm (when not null).

@@ -19,6 +19,7 @@
 // hit distinct banks.  Global->register prefetch of tile t+1 overlaps the
 // MFMAs of tile t; LDS is double-buffered, one barrier per K tile; interior
 // tiles run a predicate-free path with 16-byte loads.
+#include <climits>
 #include "flx_common.h"
 #include "flx_kernels.h"
 
@@ -163,22 +164,19 @@ FLX_DEV void gemm_mainloop(acc_t (&acc)[GMI][GNJ], T (*As)[GBK][GBM + APAD],
     }
 }
 
+// One 128x128 tile of C.  The caller separates two tiles of one workgroup
+// by a barrier (the LDS buffers are reused).
 template <typename T, int TA, int TB>
-__global__ __launch_bounds__(GNT) void gemm_mfma_kernel(
-    int M, int N, int K, T alpha, const T *__restrict__ A, long lda,
-    const T *__restrict__ B, long ldb, T beta, T *__restrict__ C, long ldc,
-    int vecA, int vecB, int blas_beta0)
+FLX_DEV void gemm_tile(T (*As)[GBK][GBM + APAD], T (*Bs)[GBN][GBK + BPAD], int m0, int n0,
+                       int M, int N, int K, T alpha, const T *__restrict__ A, long lda,
+                       const T *__restrict__ B, long ldb, T beta, T *__restrict__ C, long ldc,
+                       int vecA, int vecB, int blas_beta0)
 {
     typedef typename MfmaAcc<T>::type acc_t;
-    __shared__ T As[2][GBK][GBM + APAD];
-    __shared__ T Bs[2][GBN][GBK + BPAD];
-
     const int tid = threadIdx.x;
     const int lane = tid & 63;
     const int wave = tid >> 6;
     const int wm = wave % GWM, wn = wave / GWM;
-    const int m0 = blockIdx.x * GBM;
-    const int n0 = blockIdx.y * GBN;
     const int lrow = lane & 15, lk = lane >> 4;
 
     acc_t acc[GMI][GNJ];
@@ -219,19 +217,40 @@ __global__ __launch_bounds__(GNT) void gemm_mfma_kernel(
         }
 }
 
+// Tiles are numbered m-fastest; workgroup b takes tiles b, b + grid, ...
+// (one tile each unless the launch caps the grid so that concurrent work
+// on another stream keeps some compute units).
+template <typename T, int TA, int TB>
+__global__ __launch_bounds__(GNT) void gemm_mfma_kernel(
+    int M, int N, int K, T alpha, const T *__restrict__ A, long lda,
+    const T *__restrict__ B, long ldb, T beta, T *__restrict__ C, long ldc,
+    int vecA, int vecB, int blas_beta0, int mt, int tiles)
+{
+    __shared__ T As[2][GBK][GBM + APAD];
+    __shared__ T Bs[2][GBN][GBK + BPAD];
+    for (int t = blockIdx.x; t < tiles; t += gridDim.x) {
+        if (t != (int)blockIdx.x) __syncthreads();
+        gemm_tile<T, TA, TB>(As, Bs, (t % mt) * GBM, (t / mt) * GBN, M, N, K, alpha, A, lda, B,
+                             ldb, beta, C, ldc, vecA, vecB, blas_beta0);
+    }
+}
+
 template <typename T>
 static hipError_t launch_gemm(hipStream_t s, int ta, int tb, int M, int N, int K, T alpha,
                               const T *A, long lda, const T *B, long ldb, T beta, T *C,
-                              long ldc, int blas_beta0)
+                              long ldc, int blas_beta0, int max_groups)
 {
     if (M <= 0 || N <= 0) return hipSuccess;
-    dim3 grid((M + GBM - 1) / GBM, (N + GBN - 1) / GBN);
+    const int mt = (M + GBM - 1) / GBM, nt = (N + GBN - 1) / GBN;
+    if ((long)mt * nt > INT_MAX) return hipErrorInvalidValue;
+    const int tiles = mt * nt;
+    dim3 grid(max_groups > 0 && max_groups < tiles ? max_groups : tiles);
     const uintptr_t al = 16;
     int vecA = ((uintptr_t)A % al == 0) && (lda * (long)sizeof(T)) % al == 0;
     int vecB = ((uintptr_t)B % al == 0) && (ldb * (long)sizeof(T)) % al == 0;
 #define FLX_LG(a_, b_)                                                                   \
     hipLaunchKernelGGL((gemm_mfma_kernel<T, a_, b_>), grid, dim3(GNT), 0, s, M, N, K,    \
-                       alpha, A, lda, B, ldb, beta, C, ldc, vecA, vecB, blas_beta0)
+                       alpha, A, lda, B, ldb, beta, C, ldc, vecA, vecB, blas_beta0, mt, tiles)
     if (!ta && !tb) FLX_LG(0, 0);
     else if (!ta && tb) FLX_LG(0, 1);
     else if (ta && !tb) FLX_LG(1, 0);
@@ -242,18 +261,18 @@ static hipError_t launch_gemm(hipStream_t s, int ta, int tb, int M, int N, int K
 
 hipError_t gemm_f64(hipStream_t s, int ta, int tb, int M, int N, int K, double alpha,
                     const double *A, long lda, const double *B, long ldb, double beta,
-                    double *C, long ldc, int blas_beta0)
+                    double *C, long ldc, int blas_beta0, int max_groups)
 {
     return launch_gemm<double>(s, ta, tb, M, N, K, alpha, A, lda, B, ldb, beta, C, ldc,
-                               blas_beta0);
+                               blas_beta0, max_groups);
 }
 
 hipError_t gemm_f32(hipStream_t s, int ta, int tb, int M, int N, int K, float alpha,
                     const float *A, long lda, const float *B, long ldb, float beta, float *C,
-                    long ldc, int blas_beta0)
+                    long ldc, int blas_beta0, int max_groups)
 {
     return launch_gemm<float>(s, ta, tb, M, N, K, alpha, A, lda, B, ldb, beta, C, ldc,
-                              blas_beta0);
+                              blas_beta0, max_groups);
 }
 
 } // namespace flx

@@ -149,54 +149,57 @@ __global__ __launch_bounds__(256) void trsm_rm_llu_kernel(const T *__restrict__ 
 {
     __shared__ T xs[TR_KMAX][TR_CW + 1];
     __shared__ T ls[TR_KMAX][TR_BS + 1];   // L(kb.., kb..kb+15) strip
-    const int c0 = blockIdx.x * TR_CW;
-    const int cw = min(TR_CW, ncols - c0);
     const int tid = threadIdx.x;
     const int tc = tid % TR_CW, tg = tid / TR_CW;   // 32 columns x 8 row groups
-    for (int q = tid; q < k * TR_CW; q += 256) {
-        int rr = q / TR_CW, c = q % TR_CW;
-        xs[rr][c] = (c < cw) ? X[(long)rr * ldx + c0 + c] : T(0);
-    }
-    for (int kb = 0; kb < k; kb += TR_BS) {
-        const int bs = min(TR_BS, k - kb);
-        // stage the strip of L below (and including) the diagonal block
-        for (int q = tid; q < (k - kb) * TR_BS; q += 256) {
-            int rr = q / TR_BS, cc = q % TR_BS;
-            ls[rr][cc] = (cc < bs) ? L[(long)(kb + rr) * ldl + kb + cc] : T(0);
+    // workgroup b solves the column chunks b, b + grid, ...
+    for (int c0 = blockIdx.x * TR_CW; c0 < ncols; c0 += gridDim.x * TR_CW) {
+        const int cw = min(TR_CW, ncols - c0);
+        if (c0 != (int)blockIdx.x * TR_CW) __syncthreads();
+        for (int q = tid; q < k * TR_CW; q += 256) {
+            int rr = q / TR_CW, c = q % TR_CW;
+            xs[rr][c] = (c < cw) ? X[(long)rr * ldx + c0 + c] : T(0);
         }
-        __syncthreads();
-        // diagonal block: column tc held in registers by threads tg == 0
-        if (tg == 0) {
+        for (int kb = 0; kb < k; kb += TR_BS) {
+            const int bs = min(TR_BS, k - kb);
+            // stage the strip of L below (and including) the diagonal block
+            for (int q = tid; q < (k - kb) * TR_BS; q += 256) {
+                int rr = q / TR_BS, cc = q % TR_BS;
+                ls[rr][cc] = (cc < bs) ? L[(long)(kb + rr) * ldl + kb + cc] : T(0);
+            }
+            __syncthreads();
+            // diagonal block: column tc held in registers by threads tg == 0
+            if (tg == 0) {
+                T xb[TR_BS];
+#pragma unroll
+                for (int i = 0; i < TR_BS; ++i) xb[i] = (i < bs) ? xs[kb + i][tc] : T(0);
+#pragma unroll
+                for (int i = 1; i < TR_BS; ++i) {
+                    T v = xb[i];
+#pragma unroll
+                    for (int q = 0; q < i; ++q) v = fma(-ls[i][q], xb[q], v);
+                    xb[i] = v;
+                }
+#pragma unroll
+                for (int i = 0; i < TR_BS; ++i)
+                    if (i < bs) xs[kb + i][tc] = xb[i];
+            }
+            __syncthreads();
+            // fold the solved block into the rows below
             T xb[TR_BS];
 #pragma unroll
-            for (int i = 0; i < TR_BS; ++i) xb[i] = (i < bs) ? xs[kb + i][tc] : T(0);
+            for (int q = 0; q < TR_BS; ++q) xb[q] = xs[kb + q < k ? kb + q : kb][tc];
+            for (int rr = kb + bs + tg; rr < k; rr += 8) {
+                T v = xs[rr][tc];
 #pragma unroll
-            for (int i = 1; i < TR_BS; ++i) {
-                T v = xb[i];
-#pragma unroll
-                for (int q = 0; q < i; ++q) v = fma(-ls[i][q], xb[q], v);
-                xb[i] = v;
+                for (int q = 0; q < TR_BS; ++q) v = fma(-ls[rr - kb][q], xb[q], v);
+                xs[rr][tc] = v;
             }
-#pragma unroll
-            for (int i = 0; i < TR_BS; ++i)
-                if (i < bs) xs[kb + i][tc] = xb[i];
+            __syncthreads();
         }
-        __syncthreads();
-        // fold the solved block into the rows below
-        T xb[TR_BS];
-#pragma unroll
-        for (int q = 0; q < TR_BS; ++q) xb[q] = xs[kb + q < k ? kb + q : kb][tc];
-        for (int rr = kb + bs + tg; rr < k; rr += 8) {
-            T v = xs[rr][tc];
-#pragma unroll
-            for (int q = 0; q < TR_BS; ++q) v = fma(-ls[rr - kb][q], xb[q], v);
-            xs[rr][tc] = v;
+        for (int q = tid; q < k * TR_CW; q += 256) {
+            int rr = q / TR_CW, c = q % TR_CW;
+            if (c < cw) X[(long)rr * ldx + c0 + c] = xs[rr][c];
         }
-        __syncthreads();
-    }
-    for (int q = tid; q < k * TR_CW; q += 256) {
-        int rr = q / TR_CW, c = q % TR_CW;
-        if (c < cw) X[(long)rr * ldx + c0 + c] = xs[rr][c];
     }
 }
 
@@ -273,20 +276,22 @@ hipError_t piv0_to_ipiv(hipStream_t s, const int *piv0, int *ipiv, int n, int *i
 
 template <typename T>
 hipError_t trsm_rm_llu(hipStream_t s, Profiler *prof, const T *L, long ldl, T *X, long ldx,
-                       int k, int ncols)
+                       int k, int ncols, int max_groups)
 {
     if (ncols <= 0 || k <= 0) return hipSuccess;
     if (k > TR_KMAX) return hipErrorInvalidValue;
     ProfScope ps(prof, s, PROF_TRSM_RM, (double)k * k * ncols,
                  sizeof(T) * (2.0 * k * ncols + 0.5 * k * k));
-    hipLaunchKernelGGL(trsm_rm_llu_kernel<T>, dim3((ncols + TR_CW - 1) / TR_CW), dim3(256), 0,
-                       s, L, ldl, X, ldx, k, ncols);
+    int groups = (ncols + TR_CW - 1) / TR_CW;
+    if (max_groups > 0 && groups > max_groups) groups = max_groups;
+    hipLaunchKernelGGL(trsm_rm_llu_kernel<T>, dim3(groups), dim3(256), 0, s, L, ldl, X, ldx, k,
+                       ncols);
     return hipGetLastError();
 }
 template hipError_t trsm_rm_llu<double>(hipStream_t, Profiler *, const double *, long, double *,
-                                        long, int, int);
+                                        long, int, int, int);
 template hipError_t trsm_rm_llu<float>(hipStream_t, Profiler *, const float *, long, float *,
-                                       long, int, int);
+                                       long, int, int, int);
 
 template <typename T>
 hipError_t rowperm_rm(hipStream_t s, Profiler *prof, T *W, long ldw, int m, int c1, int c2,
@@ -317,13 +322,103 @@ hipError_t iota(hipStream_t s, int *p, int n)
 
 // W[r0.., c0..c0+nc) -= W[r0.., k0..k0+kk) * W[k0..k0+kk, c0..c0+nc)
 static hipError_t rm_update(hipStream_t s, Profiler *prof, double *W, long ldw, int r0,
-                            int nr, int c0, int nc, int k0, int kk)
+                            int nr, int c0, int nc, int k0, int kk, int max_groups = 0)
 {
     if (nr <= 0 || nc <= 0 || kk <= 0) return hipSuccess;
     ProfScope ps(prof, s, PROF_GEMM, 2.0 * nr * nc * kk,
                  8.0 * ((double)nc * kk + (double)kk * nr + 2.0 * nc * nr));
     return gemm_f64(s, 0, 0, nc, nr, kk, -1.0, W + (long)k0 * ldw + c0, ldw,
-                    W + (long)r0 * ldw + k0, ldw, 1.0, W + (long)r0 * ldw + c0, ldw, 0);
+                    W + (long)r0 * ldw + k0, ldw, 1.0, W + (long)r0 * ldw + c0, ldw, 0,
+                    max_groups);
+}
+
+// ---------------------------------------------------------------------------
+// Deferred row interchanges of one outer block: the block's sub-panels move
+// rows only inside the block's columns; afterwards the composed permutation
+// moves the same rows in every other column at once.  inv is the inverse of
+// perm before the block, so a row now at position p came from inv[perm[p]].
+// ---------------------------------------------------------------------------
+__global__ void inv_perm_kernel(const int *__restrict__ perm, int *__restrict__ inv, int m)
+{
+    const int p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p < m) {
+        const int o = perm[p];
+        if (o >= 0 && o < m) inv[o] = p;
+    }
+}
+
+__global__ void block_moves_kernel(const int *__restrict__ perm, const int *__restrict__ inv,
+                                   int J, int m, int *__restrict__ mv)
+{
+    const int p = J + blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= m) return;
+    const int o = perm[p];
+    if (o < 0 || o >= m) return;
+    const int src = inv[o];
+    if (src != p && src >= 0 && src < m) {
+        const int e = atomicAdd(&mv[0], 1);
+        if (e < LU_BLOCK_MOVES) {
+            mv[1 + 2 * e] = p;
+            mv[2 + 2 * e] = src;
+        }
+    }
+}
+
+static_assert(2 * TR_KMAX <= LU_BLOCK_MOVES, "an outer block moves at most 2*nb rows");
+constexpr int AM_CW = 16;   // columns per workgroup
+
+// W[dst_e][c] = W_old[src_e][c] for the columns [0, cs) and [ce, n)
+__global__ __launch_bounds__(256) void apply_moves_kernel(double *__restrict__ Wm, long ldw, int m,
+                                                          int cs, int ce, int n,
+                                                          const int *__restrict__ mv,
+                                                          unsigned long long *__restrict__ moved)
+{
+    __shared__ double tile[LU_BLOCK_MOVES][AM_CW];
+    __shared__ int s_dst[LU_BLOCK_MOVES], s_src[LU_BLOCK_MOVES];
+    const int cnt = min(mv[0], LU_BLOCK_MOVES);
+    if (cnt <= 0) return;
+    // chunks cover [0, cs) then [ce, n)
+    const int nleft = (cs + AM_CW - 1) / AM_CW;
+    const int chunk = blockIdx.x;
+    int c0, c1;
+    if (chunk < nleft) { c0 = chunk * AM_CW; c1 = min(cs, c0 + AM_CW); }
+    else { c0 = ce + (chunk - nleft) * AM_CW; c1 = min(n, c0 + AM_CW); }
+    const int cw = c1 - c0;
+    if (cw <= 0) return;
+    for (int e = threadIdx.x; e < cnt; e += 256) {
+        const int d = mv[1 + 2 * e], sr = mv[2 + 2 * e];
+        const bool ok = d >= 0 && d < m && sr >= 0 && sr < m;
+        s_dst[e] = ok ? d : 0;
+        s_src[e] = ok ? sr : 0;
+    }
+    __syncthreads();
+    if (moved && threadIdx.x == 0) atomicAdd(moved, (unsigned long long)cnt * cw);
+    for (int q = threadIdx.x; q < cnt * AM_CW; q += 256) {
+        const int e = q / AM_CW, c = q % AM_CW;
+        if (c < cw) tile[e][c] = Wm[(long)s_src[e] * ldw + c0 + c];
+    }
+    __syncthreads();
+    for (int q = threadIdx.x; q < cnt * AM_CW; q += 256) {
+        const int e = q / AM_CW, c = q % AM_CW;
+        if (c < cw) Wm[(long)s_dst[e] * ldw + c0 + c] = tile[e][c];
+    }
+}
+
+static hipError_t defer_apply(hipStream_t s, LuWork &wk, double *W, long ldw, int m, int n,
+                              int J, int jb)
+{
+    const int rows = m - J;
+    hipLaunchKernelGGL(fill_int_kernel, dim3(1), dim3(1), 0, s, wk.mv, 1, 0);
+    hipLaunchKernelGGL(block_moves_kernel, dim3((rows + 255) / 256), dim3(256), 0, s,
+                       (const int *)wk.perm, (const int *)wk.inv, J, m, wk.mv);
+    const int nleft = (J + AM_CW - 1) / AM_CW;
+    const int nright = (n - (J + jb) + AM_CW - 1) / AM_CW;
+    if (nleft + nright > 0) {
+        ProfScope ps(wk.prof, s, PROF_ROWPERM, 0.0, 0.0);
+        hipLaunchKernelGGL(apply_moves_kernel, dim3(nleft + nright), dim3(256), 0, s, W, ldw, m, J,
+                           J + jb, n, (const int *)wk.mv, wk.moved);
+    }
+    return hipGetLastError();
 }
 
 hipError_t getrf_rm_f64(hipStream_t s, LuWork &wk, double *W, long ldw, int m, int n,
@@ -338,17 +433,28 @@ hipError_t getrf_rm_f64(hipStream_t s, LuWork &wk, double *W, long ldw, int m, i
     if (nb > TR_KMAX) nb = TR_KMAX;
     nb = (nb / ib) * ib;
     if (m > PANEL_ROWS_MAX) return hipErrorInvalidValue;
+    const bool look = opt.lookahead && wk.perm && wk.inv && wk.mv && wk.side && wk.ev_a &&
+                      wk.ev_r;
     // the composed permutation, when requested, starts as the identity
     if (wk.perm) FLX_CHECK(iota(s, wk.perm, m));
+    bool rest_pending = false;
     for (int J = 0; J < mn; J += nb) {
         const int jb = (mn - J < nb) ? mn - J : nb;
+        if (look)
+            hipLaunchKernelGGL(inv_perm_kernel, dim3((m + 255) / 256), dim3(256), 0, s,
+                               (const int *)wk.perm, wk.inv, m);
         for (int j = J; j < J + jb; j += ib) {
             const int w = (J + jb - j < ib) ? J + jb - j : ib;
             FLX_CHECK(launch_panel(s, wk, W + (long)j * ldw + j, ldw, m - j, w, piv0 + j, j,
                                    info, ib));
-            // apply this sub-panel's interchanges to every other column
-            FLX_CHECK(rowperm_rm(s, wk.prof, W, ldw, m, j, j + w, n, wk.list,
-                                 panel_groups(m - j), wk.perm, wk.moved));
+            if (look) {
+                // interchanges inside the block's columns now, the rest deferred
+                FLX_CHECK(rowperm_rm(s, wk.prof, W + J, ldw, m, j - J, j + w - J, jb, wk.list,
+                                     panel_groups(m - j), wk.perm, wk.moved));
+            } else {
+                FLX_CHECK(rowperm_rm(s, wk.prof, W, ldw, m, j, j + w, n, wk.list,
+                                     panel_groups(m - j), wk.perm, wk.moved));
+            }
             // inner update of the rest of the outer block
             const int cr = J + jb - (j + w);
             if (cr > 0) {
@@ -358,12 +464,40 @@ hipError_t getrf_rm_f64(hipStream_t s, LuWork &wk, double *W, long ldw, int m, i
             }
         }
         const int nr = n - (J + jb);
+        if (!look) {
+            if (nr > 0) {
+                FLX_CHECK(trsm_rm_llu(s, wk.prof, (const double *)W + (long)J * ldw + J, ldw,
+                                      W + (long)J * ldw + J + jb, ldw, jb, nr));
+                FLX_CHECK(rm_update(s, wk.prof, W, ldw, J + jb, m - J - jb, J + jb, nr, J, jb));
+            }
+            continue;
+        }
+        // the previous block's rest update must be done before its columns move
+        if (rest_pending) FLX_CHECK(hipStreamWaitEvent(s, wk.ev_r, 0));
+        rest_pending = false;
+        FLX_CHECK(defer_apply(s, wk, W, ldw, m, n, J, jb));
         if (nr > 0) {
+            // the next block's columns first (they feed its panels) ...
+            const int nl = nr < nb ? nr : nb;
             FLX_CHECK(trsm_rm_llu(s, wk.prof, (const double *)W + (long)J * ldw + J, ldw,
-                                  W + (long)J * ldw + J + jb, ldw, jb, nr));
-            FLX_CHECK(rm_update(s, wk.prof, W, ldw, J + jb, m - J - jb, J + jb, nr, J, jb));
+                                  W + (long)J * ldw + J + jb, ldw, jb, nl));
+            FLX_CHECK(rm_update(s, wk.prof, W, ldw, J + jb, m - J - jb, J + jb, nl, J, jb));
+            // ... the rest on the side stream, beside the next block's panels
+            const int c0 = J + jb + nl;
+            if (n > c0) {
+                FLX_CHECK(hipEventRecord(wk.ev_a, s));
+                FLX_CHECK(hipStreamWaitEvent(wk.side, wk.ev_a, 0));
+                FLX_CHECK(trsm_rm_llu(wk.side, wk.prof, (const double *)W + (long)J * ldw + J,
+                                      ldw, W + (long)J * ldw + c0, ldw, jb, n - c0,
+                                      wk.side_groups));
+                FLX_CHECK(rm_update(wk.side, wk.prof, W, ldw, J + jb, m - J - jb, c0, n - c0, J,
+                                    jb, wk.side_groups));
+                FLX_CHECK(hipEventRecord(wk.ev_r, wk.side));
+                rest_pending = true;
+            }
         }
     }
+    if (rest_pending) FLX_CHECK(hipStreamWaitEvent(s, wk.ev_r, 0));
     return hipSuccess;
 }
 

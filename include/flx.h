@@ -57,6 +57,14 @@ const char *flx_status_string(int status);
  * (<= 64).  The reference's equivalent knob is ilaenv(1,"GETRF") = 64,
  * flens/lapack/la/ilaenv.tcc:91-99. */
 int flx_set_lu_blocking(flx_handle h, int nb, int ib);
+/* Lookahead depth of the LU (0 or 1, default 1).  With 1 the row
+ * interchanges of an outer block reach the columns outside it in one pass
+ * after its last panel, the next block's columns are updated first, and the
+ * rest of the trailing update runs on a second stream beside the next
+ * block's panels.  Results are identical for both depths.  No counterpart in
+ * the reference (flens/lapack/ge/trf.tcc:94-139 is the right-looking
+ * blocked LU without lookahead). */
+int flx_set_lu_lookahead(flx_handle h, int depth);
 
 /* Launch profiler (no counterpart in the reference, which has no timers:
  * flens/debug only logs calls).  Between start and stop every library
