@@ -60,6 +60,11 @@ hipError_t init_work(flx_context *h)
     if (e != hipSuccess) return e;
     e = hipMemsetAsync(h->wk.moved, 0, sizeof(unsigned long long), h->stream);
     if (e != hipSuccess) return e;
+    // side-stream launches keep a quarter of the compute units free
+    int cus = 0;
+    e = hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, h->device);
+    if (e != hipSuccess) return e;
+    h->wk.side_groups = cus > 8 ? cus - cus / 4 : cus;
     return hipMemsetAsync(h->wk.flags, 0, sizeof(int) * 4, h->stream);
 }
 
@@ -92,8 +97,11 @@ extern "C" int flx_destroy(flx_handle h)
     for (DevBuf *b : {&x.W32, &x.perm, &x.cand, &x.ccount, &x.counters, &x.u11, &x.win, &x.list,
                       &x.zero, &x.lpmax, &x.la, &x.ub, &x.inv_l, &x.inv_u, &x.y32, &x.y64, &x.flags,
                       &x.err, &x.dr, &x.dc, &x.part, &x.scal, &x.R, &x.V, &x.w, &x.t, &x.zeros,
-                      &x.mpart, &x.hbuf})
+                      &x.mpart, &x.hbuf, &x.inv, &x.mv})
         free_buf(*b);
+    if (x.side) hipStreamDestroy(x.side);
+    if (x.ev_a) hipEventDestroy(x.ev_a);
+    if (x.ev_r) hipEventDestroy(x.ev_r);
     if (x.graph) hipGraphExecDestroy(x.graph);
     if (x.gstream) hipStreamDestroy(x.gstream);
     if (x.ev0) hipEventDestroy(x.ev0);
@@ -229,9 +237,6 @@ int dgetrf_dev(flx_context *h, int m, int n, double *A, int lda, int *ipiv, int 
         FLX_TRY(hipStreamCreateWithFlags(&h->wk.side, hipStreamNonBlocking));
         FLX_TRY(hipEventCreateWithFlags(&h->wk.ev_a, hipEventDisableTiming));
         FLX_TRY(hipEventCreateWithFlags(&h->wk.ev_r, hipEventDisableTiming));
-        int cus = 0;
-        FLX_TRY(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, h->device));
-        h->wk.side_groups = cus > 8 ? cus - cus / 4 : cus;
     }
     double *W = (double *)h->W.p;
     FLX_TRY(fill_int(h->stream, info, 1, INT_MAX));

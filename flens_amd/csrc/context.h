@@ -12,6 +12,9 @@ struct DevBuf {
 // buffers of the mixed-precision solver
 struct MixBufs {
     DevBuf W32, perm, cand, ccount, counters, u11, win, list, zero, lpmax, la, ub;
+    DevBuf inv, mv;                         // lookahead of the factor
+    hipStream_t side = nullptr;             // default priority, ordered by events
+    hipEvent_t ev_a = nullptr, ev_r = nullptr;
     DevBuf inv_l, inv_u, y32, y64, flags, err, dr, dc, part, scal, R, V, w, t, zeros, mpart, hbuf;
     unsigned epoch = 0;
     // captured factorisation: valid while the key matches

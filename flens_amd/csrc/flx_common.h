@@ -21,6 +21,12 @@ enum flx_status_internal {
 
 // Agent-scope relaxed atomics: coherent across the XCDs of one device
 // without fences (each 64-bit word is single-copy atomic).
+// Latency-bound kernels on the factorisation's critical path raise their
+// waves' instruction-issue priority (s_setprio, per wave, within the compute
+// unit), so a concurrent trailing update sharing the compute units does not
+// stretch their dependency chains.
+FLX_DEV void critical_path_priority() { __builtin_amdgcn_s_setprio(3); }
+
 FLX_DEV uint64_t ld_agent(const uint64_t *p)
 {
     return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);

@@ -102,6 +102,15 @@ hipError_t launch_panel(hipStream_t s, LuWork &wk, double *P, long ldw, int mp, 
 // Factor the m x n row-major matrix W (leading dimension ldw) in place.
 // piv0 receives 0-based absolute pivot rows (length min(m,n)); info receives
 // the 1-based first zero pivot column (INT_MAX if none; caller initialises).
+// Deferred interchanges of one outer block [J, J+jb): inv = inverse of perm
+// before the block; afterwards defer_moves moves the rows the block's
+// sub-panels moved in its own columns in every other column too (mv holds
+// 1 + 2 * LU_BLOCK_MOVES ints).
+hipError_t inv_perm(hipStream_t s, const int *perm, int *inv, int m);
+template <typename T>
+hipError_t defer_moves(hipStream_t s, Profiler *prof, T *W, long ldw, int m, int n, int J,
+                       int jb, const int *perm, const int *inv, int *mv,
+                       unsigned long long *moved);
 hipError_t getrf_rm_f64(hipStream_t s, LuWork &wk, double *W, long ldw, int m,
                         int n, int *piv0, int *info, const LuOptions &opt);
 // Row-major building blocks shared by the FP64 and the mixed-precision LU.
@@ -162,6 +171,11 @@ struct MixWork {
     unsigned *lpmax = nullptr;    // 2 (FP16 operand maxima)
     void *la = nullptr, *ub = nullptr;   // lp_pack_elems(n, nb) 16-bit each
     Profiler *prof = nullptr;
+    // lookahead, as in LuWork (all null: no lookahead)
+    int *inv = nullptr, *mv = nullptr;
+    hipStream_t side = nullptr;
+    hipEvent_t ev_a = nullptr, ev_r = nullptr;
+    int side_groups = 0;
 };
 size_t tslu_nodes(int mp);
 size_t lp_pack_elems(int n, int nb);

@@ -90,6 +90,7 @@ __global__ __launch_bounds__(256) void rowperm_kernel(T *__restrict__ Wm, long l
                                                       int *__restrict__ perm,
                                                       unsigned long long *__restrict__ moved)
 {
+    critical_path_priority();
     __shared__ T tile[RP_MAXE][RP_CW];
     __shared__ int s_dst[RP_MAXE], s_src[RP_MAXE], s_off[PANEL_GROUPS_MAX + 1];
     const int cnt = load_list(list, G, m, s_dst, s_src, s_off);
@@ -368,12 +369,14 @@ static_assert(2 * TR_KMAX <= LU_BLOCK_MOVES, "an outer block moves at most 2*nb 
 constexpr int AM_CW = 16;   // columns per workgroup
 
 // W[dst_e][c] = W_old[src_e][c] for the columns [0, cs) and [ce, n)
-__global__ __launch_bounds__(256) void apply_moves_kernel(double *__restrict__ Wm, long ldw, int m,
+template <typename T>
+__global__ __launch_bounds__(256) void apply_moves_kernel(T *__restrict__ Wm, long ldw, int m,
                                                           int cs, int ce, int n,
                                                           const int *__restrict__ mv,
                                                           unsigned long long *__restrict__ moved)
 {
-    __shared__ double tile[LU_BLOCK_MOVES][AM_CW];
+    critical_path_priority();
+    __shared__ T tile[LU_BLOCK_MOVES][AM_CW];
     __shared__ int s_dst[LU_BLOCK_MOVES], s_src[LU_BLOCK_MOVES];
     const int cnt = min(mv[0], LU_BLOCK_MOVES);
     if (cnt <= 0) return;
@@ -404,22 +407,37 @@ __global__ __launch_bounds__(256) void apply_moves_kernel(double *__restrict__ W
     }
 }
 
-static hipError_t defer_apply(hipStream_t s, LuWork &wk, double *W, long ldw, int m, int n,
-                              int J, int jb)
+hipError_t inv_perm(hipStream_t s, const int *perm, int *inv, int m)
+{
+    if (m <= 0) return hipSuccess;
+    hipLaunchKernelGGL(inv_perm_kernel, dim3((m + 255) / 256), dim3(256), 0, s, perm, inv, m);
+    return hipGetLastError();
+}
+
+template <typename T>
+hipError_t defer_moves(hipStream_t s, Profiler *prof, T *W, long ldw, int m, int n, int J,
+                       int jb, const int *perm, const int *inv, int *mv,
+                       unsigned long long *moved)
 {
     const int rows = m - J;
-    hipLaunchKernelGGL(fill_int_kernel, dim3(1), dim3(1), 0, s, wk.mv, 1, 0);
-    hipLaunchKernelGGL(block_moves_kernel, dim3((rows + 255) / 256), dim3(256), 0, s,
-                       (const int *)wk.perm, (const int *)wk.inv, J, m, wk.mv);
+    if (rows <= 0) return hipSuccess;
+    hipLaunchKernelGGL(fill_int_kernel, dim3(1), dim3(1), 0, s, mv, 1, 0);
+    hipLaunchKernelGGL(block_moves_kernel, dim3((rows + 255) / 256), dim3(256), 0, s, perm, inv,
+                       J, m, mv);
     const int nleft = (J + AM_CW - 1) / AM_CW;
     const int nright = (n - (J + jb) + AM_CW - 1) / AM_CW;
     if (nleft + nright > 0) {
-        ProfScope ps(wk.prof, s, PROF_ROWPERM, 0.0, 0.0);
-        hipLaunchKernelGGL(apply_moves_kernel, dim3(nleft + nright), dim3(256), 0, s, W, ldw, m, J,
-                           J + jb, n, (const int *)wk.mv, wk.moved);
+        ProfScope ps(prof, s, PROF_ROWPERM, 0.0, 0.0);
+        hipLaunchKernelGGL(apply_moves_kernel<T>, dim3(nleft + nright), dim3(256), 0, s, W, ldw,
+                           m, J, J + jb, n, (const int *)mv, moved);
     }
     return hipGetLastError();
 }
+template hipError_t defer_moves<double>(hipStream_t, Profiler *, double *, long, int, int, int,
+                                        int, const int *, const int *, int *,
+                                        unsigned long long *);
+template hipError_t defer_moves<float>(hipStream_t, Profiler *, float *, long, int, int, int, int,
+                                       const int *, const int *, int *, unsigned long long *);
 
 hipError_t getrf_rm_f64(hipStream_t s, LuWork &wk, double *W, long ldw, int m, int n,
                         int *piv0, int *info, const LuOptions &opt)
@@ -440,9 +458,7 @@ hipError_t getrf_rm_f64(hipStream_t s, LuWork &wk, double *W, long ldw, int m, i
     bool rest_pending = false;
     for (int J = 0; J < mn; J += nb) {
         const int jb = (mn - J < nb) ? mn - J : nb;
-        if (look)
-            hipLaunchKernelGGL(inv_perm_kernel, dim3((m + 255) / 256), dim3(256), 0, s,
-                               (const int *)wk.perm, wk.inv, m);
+        if (look) FLX_CHECK(inv_perm(s, wk.perm, wk.inv, m));
         for (int j = J; j < J + jb; j += ib) {
             const int w = (J + jb - j < ib) ? J + jb - j : ib;
             FLX_CHECK(launch_panel(s, wk, W + (long)j * ldw + j, ldw, m - j, w, piv0 + j, j,
@@ -475,7 +491,8 @@ hipError_t getrf_rm_f64(hipStream_t s, LuWork &wk, double *W, long ldw, int m, i
         // the previous block's rest update must be done before its columns move
         if (rest_pending) FLX_CHECK(hipStreamWaitEvent(s, wk.ev_r, 0));
         rest_pending = false;
-        FLX_CHECK(defer_apply(s, wk, W, ldw, m, n, J, jb));
+        FLX_CHECK(defer_moves<double>(s, wk.prof, W, ldw, m, n, J, jb, wk.perm, wk.inv, wk.mv,
+                                      wk.moved));
         if (nr > 0) {
             // the next block's columns first (they feed its panels) ...
             const int nl = nr < nb ? nr : nb;

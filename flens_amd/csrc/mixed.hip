@@ -186,6 +186,7 @@ __global__ __launch_bounds__(TS_R) void tslu_kernel(const float *__restrict__ P,
                                                     int *__restrict__ list,
                                                     int *__restrict__ zero)
 {
+    critical_path_priority();
     __shared__ TsShared sh;
     __shared__ int s_last;
     __shared__ int s_D[TS_B], s_V[TS_B];
@@ -277,6 +278,7 @@ __global__ __launch_bounds__(256) void tslu_finish_kernel(float *__restrict__ P,
                                                           int w, const float *__restrict__ u11,
                                                           const int *__restrict__ win, int nsel)
 {
+    critical_path_priority();
     __shared__ float su[TS_B][TS_B + 1];
     __shared__ int sw[TS_B];
     const int t = threadIdx.x;
@@ -385,11 +387,14 @@ template <typename V>
 __global__ __launch_bounds__(256) void gemm_lp_kernel(const V *__restrict__ La,
                                                       const V *__restrict__ Ub, int Kp,
                                                       float *__restrict__ C, long ldc, int nr,
-                                                      int nc, const unsigned *__restrict__ maxes)
+                                                      int nc, const unsigned *__restrict__ maxes,
+                                                      int tx, int tiles)
 {
+  // workgroup b takes the 128x128 tiles b, b + grid, ... (column tile fastest)
+  for (int t = blockIdx.x; t < tiles; t += gridDim.x) {
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    const int i0 = blockIdx.y * 128 + (wave >> 1) * 64;
-    const int j0 = blockIdx.x * 128 + (wave & 1) * 64;
+    const int i0 = (t / tx) * 128 + (wave >> 1) * 64;
+    const int j0 = (t % tx) * 128 + (wave & 1) * 64;
     const int lr = lane & 15, lk = lane >> 4;
     float4_t acc[4][4];
 #pragma unroll
@@ -454,6 +459,7 @@ __global__ __launch_bounds__(256) void gemm_lp_kernel(const V *__restrict__ La,
                 }
             }
         }
+  }
 }
 
 // max |x| over an nr x nc row-major block into *out (float bits, >= 0)
@@ -527,14 +533,15 @@ size_t lp_pack_elems(int n, int nb)
 
 // C[r0.., c0..] -= W[r0.., k0..k0+kk) * W[k0..k0+kk, c0..) in precision prec
 static hipError_t lp_update(hipStream_t s, const MixWork &mw, int prec, float *W, long ldw,
-                            int r0, int nr, int c0, int nc, int k0, int kk)
+                            int r0, int nr, int c0, int nc, int k0, int kk, int max_groups = 0)
 {
     if (nr <= 0 || nc <= 0 || kk <= 0) return hipSuccess;
     ProfScope ps(mw.prof, s, PROF_GEMM, 2.0 * nr * nc * kk,
                  4.0 * (2.0 * nr * nc + (double)nr * kk + (double)kk * nc));
     if (prec == MIX_FP32)
         return gemm_f32(s, 0, 0, nc, nr, kk, -1.0f, W + (long)k0 * ldw + c0, ldw,
-                        W + (long)r0 * ldw + k0, ldw, 1.0f, W + (long)r0 * ldw + c0, ldw, 0);
+                        W + (long)r0 * ldw + k0, ldw, 1.0f, W + (long)r0 * ldw + c0, ldw, 0,
+                        max_groups);
     const int Kp = ((kk + 31) / 32) * 32;
     const int nr_pad = ((nr + 127) / 128) * 128, nc_pad = ((nc + 127) / 128) * 128;
     unsigned *mx = nullptr;
@@ -549,7 +556,8 @@ static hipError_t lp_update(hipStream_t s, const MixWork &mw, int prec, float *W
     const long la_total = (long)nr_pad * Kp;
     const int gl = (int)((la_total + 255) / 256 < 8192 ? (la_total + 255) / 256 : 8192);
     dim3 gu((nc_pad + 31) / 32, (Kp + 31) / 32);
-    dim3 gg(nc_pad / 128, nr_pad / 128);
+    const int gtx = nc_pad / 128, gtiles = gtx * (nr_pad / 128);
+    dim3 gg(max_groups > 0 && max_groups < gtiles ? max_groups : gtiles);
     if (prec == MIX_BF16) {
         __bf16 *La = reinterpret_cast<__bf16 *>(mw.la), *Ub = reinterpret_cast<__bf16 *>(mw.ub);
         hipLaunchKernelGGL(pack_l_kernel<__bf16>, dim3(gl), dim3(256), 0, s, L, ldw, nr, kk, La,
@@ -559,7 +567,8 @@ static hipError_t lp_update(hipStream_t s, const MixWork &mw, int prec, float *W
         hipLaunchKernelGGL(gemm_lp_kernel<bf16x8_t>, gg, dim3(256), 0, s,
                            reinterpret_cast<const bf16x8_t *>(La),
                            reinterpret_cast<const bf16x8_t *>(Ub), Kp,
-                           W + (long)r0 * ldw + c0, ldw, nr, nc, (const unsigned *)nullptr);
+                           W + (long)r0 * ldw + c0, ldw, nr, nc, (const unsigned *)nullptr, gtx,
+                           gtiles);
     } else {
         _Float16 *La = reinterpret_cast<_Float16 *>(mw.la), *Ub = reinterpret_cast<_Float16 *>(mw.ub);
         hipLaunchKernelGGL(pack_l_kernel<_Float16>, dim3(gl), dim3(256), 0, s, L, ldw, nr, kk,
@@ -569,7 +578,8 @@ static hipError_t lp_update(hipStream_t s, const MixWork &mw, int prec, float *W
         hipLaunchKernelGGL(gemm_lp_kernel<f16x8_t>, gg, dim3(256), 0, s,
                            reinterpret_cast<const f16x8_t *>(La),
                            reinterpret_cast<const f16x8_t *>(Ub), Kp,
-                           W + (long)r0 * ldw + c0, ldw, nr, nc, (const unsigned *)mx);
+                           W + (long)r0 * ldw + c0, ldw, nr, nc, (const unsigned *)mx, gtx,
+                           gtiles);
     }
     return hipGetLastError();
 }
@@ -587,9 +597,16 @@ hipError_t getrf_rm_mixed(hipStream_t s, const MixWork &mw, float *W, long ldw, 
     if (nb < ib) nb = ib;
     if (nb > 256) nb = 256;
     nb = (nb / ib) * ib;
+    // lookahead: interchanges inside the outer block first, the rest of the
+    // columns once per block; the next block's columns are updated (in FP32)
+    // before the rest of the trailing update, which runs in prec on the side
+    // stream beside the next block's panels
+    const bool look = mw.inv && mw.mv && mw.side && mw.ev_a && mw.ev_r;
     FLX_CHECK(iota(s, mw.perm, n));
+    bool rest_pending = false;
     for (int J = 0; J < n; J += nb) {
         const int jb = (n - J < nb) ? n - J : nb;
+        if (look) FLX_CHECK(inv_perm(s, mw.perm, mw.inv, n));
         for (int j = J; j < J + jb; j += ib) {
             const int w = (J + jb - j < ib) ? J + jb - j : ib;
             const int mp = n - j;
@@ -605,8 +622,12 @@ hipError_t getrf_rm_mixed(hipStream_t s, const MixWork &mw, float *W, long ldw, 
                                    (const int *)mw.win, w < mp ? w : mp);
                 FLX_CHECK(hipGetLastError());
             }
-            FLX_CHECK(rowperm_rm<float>(s, mw.prof, W, ldw, n, n, n, n, mw.list, 1, mw.perm,
-                                        nullptr));
+            if (look)
+                FLX_CHECK(rowperm_rm<float>(s, mw.prof, W + J, ldw, n, jb, jb, jb, mw.list, 1,
+                                            mw.perm, nullptr));
+            else
+                FLX_CHECK(rowperm_rm<float>(s, mw.prof, W, ldw, n, n, n, n, mw.list, 1, mw.perm,
+                                            nullptr));
             const int cr = J + jb - (j + w);
             if (cr > 0) {
                 FLX_CHECK(trsm_rm_llu<float>(s, mw.prof, W + (long)j * ldw + j, ldw,
@@ -615,12 +636,39 @@ hipError_t getrf_rm_mixed(hipStream_t s, const MixWork &mw, float *W, long ldw, 
             }
         }
         const int nr = n - (J + jb);
+        if (!look) {
+            if (nr > 0) {
+                FLX_CHECK(trsm_rm_llu<float>(s, mw.prof, W + (long)J * ldw + J, ldw,
+                                             W + (long)J * ldw + J + jb, ldw, jb, nr));
+                FLX_CHECK(lp_update(s, mw, prec, W, ldw, J + jb, nr, J + jb, nr, J, jb));
+            }
+            continue;
+        }
+        // the previous block's rest update reads columns this block moves
+        if (rest_pending) FLX_CHECK(hipStreamWaitEvent(s, mw.ev_r, 0));
+        rest_pending = false;
+        FLX_CHECK(defer_moves<float>(s, mw.prof, W, ldw, n, n, J, jb, mw.perm, mw.inv, mw.mv,
+                                     nullptr));
         if (nr > 0) {
+            const int nl = nr < nb ? nr : nb;
             FLX_CHECK(trsm_rm_llu<float>(s, mw.prof, W + (long)J * ldw + J, ldw,
-                                         W + (long)J * ldw + J + jb, ldw, jb, nr));
-            FLX_CHECK(lp_update(s, mw, prec, W, ldw, J + jb, nr, J + jb, nr, J, jb));
+                                         W + (long)J * ldw + J + jb, ldw, jb, nl));
+            FLX_CHECK(lp_update(s, mw, MIX_FP32, W, ldw, J + jb, nr, J + jb, nl, J, jb));
+            const int c0 = J + jb + nl;
+            if (n > c0) {
+                FLX_CHECK(hipEventRecord(mw.ev_a, s));
+                FLX_CHECK(hipStreamWaitEvent(mw.side, mw.ev_a, 0));
+                FLX_CHECK(trsm_rm_llu<float>(mw.side, mw.prof, W + (long)J * ldw + J, ldw,
+                                             W + (long)J * ldw + c0, ldw, jb, n - c0,
+                                             mw.side_groups));
+                FLX_CHECK(lp_update(mw.side, mw, prec, W, ldw, J + jb, nr, c0, n - c0, J, jb,
+                                    mw.side_groups));
+                FLX_CHECK(hipEventRecord(mw.ev_r, mw.side));
+                rest_pending = true;
+            }
         }
     }
+    if (rest_pending) FLX_CHECK(hipStreamWaitEvent(s, mw.ev_r, 0));
     return hipSuccess;
 }
 

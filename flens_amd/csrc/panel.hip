@@ -109,6 +109,7 @@ __global__ __launch_bounds__(NT) void panel_kernel(
     __shared__ int s_cnt;
     __shared__ int s_fail;
 
+    critical_path_priority();
     // Thread tid owns original panel rows g*ROWS + tid + k*NT.  Rows never
     // move during the panel: pos[k] is the row's current position under the
     // interchanges so far (LAPACK's row index), done[k] marks pivot rows.

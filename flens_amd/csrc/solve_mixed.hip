@@ -81,6 +81,7 @@ struct Solver {
     int alloc()
     {
         MixBufs &m = h->mix;
+        FLX_TRY(init_work(h));
         ldw = round_ld(n);
         const size_t nodes = tslu_nodes(n);
         FLX_TRY(ensure(h, m.W32, sizeof(float) * (size_t)ldw * n));
@@ -93,6 +94,13 @@ struct Solver {
         FLX_TRY(ensure(h, m.list, sizeof(int) * PANEL_LIST_SLICE));
         FLX_TRY(ensure(h, m.zero, sizeof(int) * 4));
         FLX_TRY(ensure(h, m.lpmax, sizeof(unsigned) * 4));
+        FLX_TRY(ensure(h, m.inv, sizeof(int) * n));
+        FLX_TRY(ensure(h, m.mv, sizeof(int) * (1 + 2 * LU_BLOCK_MOVES)));
+        if (!m.side) {
+            FLX_TRY(hipStreamCreateWithFlags(&m.side, hipStreamNonBlocking));
+            FLX_TRY(hipEventCreateWithFlags(&m.ev_a, hipEventDisableTiming));
+            FLX_TRY(hipEventCreateWithFlags(&m.ev_r, hipEventDisableTiming));
+        }
         if (o.prec != MIX_FP32) {
             FLX_TRY(ensure(h, m.la, 2 * lp_pack_elems(n, o.nb)));
             FLX_TRY(ensure(h, m.ub, 2 * lp_pack_elems(n, o.nb)));
@@ -131,6 +139,13 @@ struct Solver {
         mw.la = m.la.p;
         mw.ub = m.ub.p;
         mw.prof = h->prof;
+        const bool look = h->opt.lookahead != 0;
+        mw.inv = look ? (int *)m.inv.p : nullptr;
+        mw.mv = (int *)m.mv.p;
+        mw.side = m.side;
+        mw.ev_a = m.ev_a;
+        mw.ev_r = m.ev_r;
+        mw.side_groups = h->wk.side_groups;
         f.n = n;
         f.W = (const float *)m.W32.p;
         f.ldw = ldw;
@@ -159,7 +174,7 @@ struct Solver {
         MixBufs &m = h->mix;
         const size_t key[8] = {(size_t)n, (size_t)ldw, (size_t)o.prec, (size_t)o.nb,
                                (size_t)m.W32.p, (size_t)m.cand.p, (size_t)m.la.p,
-                               (size_t)m.perm.p};
+                               (size_t)m.perm.p ^ ((size_t)mw.inv << 1)};
         if (!m.gstream) {
             FLX_TRY(hipStreamCreateWithFlags(&m.gstream, hipStreamNonBlocking));
             FLX_TRY(hipEventCreateWithFlags(&m.ev0, hipEventDisableTiming));

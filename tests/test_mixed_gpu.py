@@ -65,7 +65,9 @@ def test_ladder_reaches_fp64_backward_error(gpu, prec, refine, kappa, gm):
     eta = backward_error(a, x, b)
     assert st["iter"] >= 0 and st["fallback"] == 0, st
     assert eta <= n * EPS, (eta, st)
-    assert abs(st["backward_error"] - eta) <= 0.5 * eta + 1e-18
+    # the device's residual and this one differ by FP64 rounding, of order
+    # sqrt(n) eps relative to ||A|| ||x|| (they agree only to that level)
+    assert abs(st["backward_error"] - eta) <= 0.5 * eta + np.sqrt(n) * EPS
     # forward error: within 10x of the FP64 oracle's, or within what the
     # achieved backward error allows (kappa * eta)
     ao, bo = a.copy(order="F"), b.copy(order="F")
