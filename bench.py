@@ -246,6 +246,77 @@ def batched_section(count: int = 20000, seed: int = 4):
     return out
 
 
+def aux_section(n: int = 8192, seed: int = 5):
+    """Norms, equilibration, condition estimate, refinement and the expert
+    driver at n = 8192 (one right-hand side), A = U[-1,1) with rows scaled
+    over 1e-6..1e6 (so equilibration has work to do).  Matrix passes are
+    priced against HBM: lan reads A once (8 n^2 bytes), equ twice; con and
+    rfs are dominated by triangular solves and are reported as times."""
+    import torch
+    import flens_amd as F
+    h = F.default_handle()
+    L = F.lib()
+    A0 = F.DeviceMatrix.empty(n, n)
+    assert L.flx_fill_uniform(h.ptr, F._vp(A0.ptr), n, n, n, seed, 0) == 0
+    A0.storage.mul_(torch.logspace(-6, 6, n, dtype=torch.float64, device="cuda")[None, :])
+    B0 = F.DeviceMatrix.empty(n, 1)
+    assert L.flx_fill_uniform(h.ptr, F._vp(B0.ptr), n, 1, n, seed + 1, 0) == 0
+    out = {"n": n, "nrhs": 1}
+    byt = 8.0 * n * n
+
+    def timed(fn, reps=3):
+        fn()
+        torch.cuda.synchronize()
+        best = 1e30
+        for _ in range(reps):
+            t = time.perf_counter()
+            r = fn()
+            torch.cuda.synchronize()
+            best = min(best, time.perf_counter() - t)
+        return best, r
+
+    for nm in "MOIF":
+        t, v = timed(lambda: F.lapack.lan(nm, A0))
+        passes = 2 if nm == "F" else 1
+        out[f"lan_{nm}"] = {"ms": 1e3 * t, "value": v,
+                            "GB_per_s": passes * byt / t / 1e9,
+                            "hbm_frac": passes * byt / t / 1e9 / PEAK_HBM}
+    t, e = timed(lambda: F.lapack.equ(A0))
+    out["equ"] = {"ms": 1e3 * t, "GB_per_s": 2 * byt / t / 1e9,
+                  "hbm_frac": 2 * byt / t / 1e9 / PEAK_HBM, "info": e[5]}
+    LU = F.DeviceMatrix.empty(n, n)
+    LU.storage.copy_(A0.storage)
+    piv, info = F.lapack.trf(LU)
+    torch.cuda.synchronize()
+    anorm = F.lapack.lan("O", A0)
+    t, rc = timed(lambda: F.lapack.con("O", LU, anorm))
+    out["con"] = {"ms": 1e3 * t, "rcond": rc}
+    X = F.DeviceMatrix.empty(n, 1)
+
+    def rfs_once():
+        X.storage.copy_(B0.storage)
+        F.lapack.trs(0, LU, piv, X)
+        return F.lapack.rfs(0, A0, LU, piv, B0, X)
+    t, fb = timed(rfs_once)
+    out["rfs"] = {"ms": 1e3 * t, "ferr": float(fb[0][0]), "berr": float(fb[1][0])}
+    A1, B1, AF = F.DeviceMatrix.empty(n, n), F.DeviceMatrix.empty(n, 1), F.DeviceMatrix.empty(n, n)
+    p1 = torch.zeros(n, dtype=torch.int32, device="cuda")
+    r = torch.zeros(n, dtype=torch.float64, device="cuda")
+    c = torch.zeros(n, dtype=torch.float64, device="cuda")
+
+    def svx_once():
+        A1.storage.copy_(A0.storage)
+        B1.storage.copy_(B0.storage)
+        return F.lapack.svx("E", 0, A1, AF, p1, "N", r, c, B1, X)
+    t, d = timed(svx_once, reps=2)
+    out["svx_E"] = {"ms": 1e3 * t, "equed": d["equed"], "rcond": d["rcond"],
+                    "ferr": float(d["ferr"][0]), "berr": float(d["berr"][0]), "info": d["info"]}
+    progress(f"aux: lan O {out['lan_O']['ms']:.2f} ms, equ {out['equ']['ms']:.2f} ms, "
+             f"con {out['con']['ms']:.1f} ms, rfs {out['rfs']['ms']:.1f} ms, "
+             f"svx(E) {out['svx_E']['ms']:.1f} ms")
+    return out
+
+
 def selector_section(train_count: int = 3000, eval_count: int = 3000):
     """Train the selector on two self-generated batches and compare the
     learned dispatch with FP32 + refinement for every system on a held-out
@@ -287,14 +358,14 @@ def main():
     ap.add_argument("--quick", action="store_true",
                     help="only the FP64 LU workload (no mixed / batched / selector sections)")
     ap.add_argument("--mixed-n", type=int, default=16384)
-    ap.add_argument("--section", choices=["mixed", "batched", "selector"],
+    ap.add_argument("--section", choices=["mixed", "batched", "selector", "aux"],
                     help="run only this section and print its JSON (for profiling)")
     args = ap.parse_args()
     if args.section:
         import torch
         assert torch.cuda.is_available(), "bench.py needs a GPU"
         fn = {"mixed": lambda: mixed_section(args.mixed_n), "batched": batched_section,
-              "selector": selector_section}[args.section]
+              "selector": selector_section, "aux": aux_section}[args.section]
         print(json.dumps({args.section: fn()}))
         return
     if args.gpus != 1:
@@ -397,6 +468,7 @@ def main():
         out["mixed_vs_fp64"] = mixed_section(args.mixed_n)
         out["batched"] = batched_section()
         out["selector"] = selector_section()
+        out["aux"] = aux_section()
     if not args.no_cpu:
         progress("CPU baseline: oracle getrf, first block step, 1 core")
         out["cpu_baseline"] = cpu_baseline(n, nrhs)

@@ -43,7 +43,9 @@ EXPORTS = [
     "flx_dgesv_mixed", "flx_dsgesv",
     "flx_batch_gesv_create", "flx_batch_gemm_create", "flx_batch_execute", "flx_batch_config",
     "flx_batch_destroy", "flx_mlp_forward", "flx_mlp_grad", "flx_mlp_adam",
+    "flx_dlange", "flx_dgeequ", "flx_dlaqge", "flx_dgecon", "flx_dgerfs", "flx_dgesvx",
     "cblas_dgemm", "cblas_dtrsm", "dgetrf_", "dgetrs_", "dgesv_", "dlaswp_", "dsgesv_",
+    "dlange_", "dgeequ_", "dgecon_", "dgerfs_", "dgesvx_",
     "xerbla_",
 ]
 
@@ -101,6 +103,22 @@ def lib():
     L.flx_status_string.argtypes = [_i]
     L.flx_status_string.restype = C.c_char_p
     L.flx_set_lu_blocking.argtypes = [_vp, _i, _i]
+    _c, _cp = C.c_char, C.c_char_p
+    L.flx_dlange.argtypes = [_vp, _c, _i, _i, _vp, _i, _dp]
+    L.flx_dgeequ.argtypes = [_vp, _i, _i, _vp, _i, _vp, _vp, _dp, _dp, _dp, _ip]
+    L.flx_dlaqge.argtypes = [_vp, _i, _i, _vp, _i, _vp, _vp, _d, _d, _d, _cp]
+    L.flx_dgecon.argtypes = [_vp, _c, _i, _vp, _i, _d, _dp]
+    L.flx_dgerfs.argtypes = [_vp, _i, _i, _i, _vp, _i, _vp, _i, _vp, _vp, _i, _vp, _i, _dp, _dp]
+    L.flx_dgesvx.argtypes = [_vp, _c, _i, _i, _i, _vp, _i, _vp, _i, _vp, _cp, _vp, _vp, _vp, _i,
+                             _vp, _i, _dp, _dp, _dp, _dp, _ip]
+    L.dlange_.argtypes = [_cp, _ip, _ip, _dp, _ip, _dp]
+    L.dlange_.restype = _d
+    L.dgeequ_.argtypes = [_ip, _ip, _dp, _ip, _dp, _dp, _dp, _dp, _dp, _ip]
+    L.dgecon_.argtypes = [_cp, _ip, _dp, _ip, _dp, _dp, _dp, _ip, _ip]
+    L.dgerfs_.argtypes = [_cp, _ip, _ip, _dp, _ip, _dp, _ip, _ip, _dp, _ip, _dp, _ip, _dp, _dp,
+                          _dp, _ip, _ip]
+    L.dgesvx_.argtypes = [_cp, _cp, _ip, _ip, _dp, _ip, _dp, _ip, _ip, _cp, _dp, _dp, _dp, _ip,
+                          _dp, _ip, _dp, _dp, _dp, _dp, _ip, _ip]
     L.flx_set_lu_lookahead.argtypes = [_vp, _i]
     L.flx_dgemm.argtypes = [_vp, _i, _i, _i, _i, _i, _d, _vp, _i, _vp, _i, _d, _vp, _i]
     L.flx_dgetrf.argtypes = [_vp, _i, _i, _vp, _i, _vp, _vp]
@@ -269,7 +287,142 @@ class blas:
         return C
 
 
+def _iref(v: int):
+    return C.byref(C.c_int(int(v)))
+
+
+def _vec(t, n):
+    """Device float64 vector of length n (a new one when t is None)."""
+    import torch
+    if t is None:
+        t = torch.zeros(max(1, n), dtype=torch.float64, device="cuda")
+    return t
+
+
 class lapack:
+    # ---- norms, equilibration, condition estimate, refinement -----------
+    # Host numpy arrays go through the Fortran seams (dlange_ ...), device
+    # DeviceMatrix arguments through the C ABI (flx_dlange ...).
+    @staticmethod
+    def lan(norm: str, A, handle: Handle | None = None) -> float:
+        """flens::lapack::lan (GeMatrix): norm 'M', 'O', 'I' or 'F'."""
+        if isinstance(A, np.ndarray):
+            m, n = A.shape
+            return lib().dlange_(norm.encode(), _iref(m), _iref(n), A.ctypes.data_as(_dp),
+                                 _iref(max(1, m)), None)
+        h = handle or default_handle()
+        v = C.c_double(0.0)
+        _check(lib().flx_dlange(h.ptr, norm.encode(), A.m, A.n, _vp(A.ptr), A.ld, C.byref(v)),
+               "flx_dlange")
+        return v.value
+
+    @staticmethod
+    def equ(A, handle: Handle | None = None):
+        """flens::lapack::equ -> (r, c, rowcnd, colcnd, amax, info)."""
+        rc, cc, am = C.c_double(np.nan), C.c_double(np.nan), C.c_double(np.nan)
+        info = C.c_int(0)
+        if isinstance(A, np.ndarray):
+            m, n = A.shape
+            r, c = np.zeros(max(1, m)), np.zeros(max(1, n))
+            lib().dgeequ_(_iref(m), _iref(n), A.ctypes.data_as(_dp), _iref(max(1, m)),
+                          r.ctypes.data_as(_dp), c.ctypes.data_as(_dp), C.byref(rc),
+                          C.byref(cc), C.byref(am), C.byref(info))
+            return r[:m], c[:n], rc.value, cc.value, am.value, info.value
+        h = handle or default_handle()
+        r, c = _vec(None, A.m), _vec(None, A.n)
+        _check(lib().flx_dgeequ(h.ptr, A.m, A.n, _vp(A.ptr), A.ld, _vp(r.data_ptr()),
+                                _vp(c.data_ptr()), C.byref(rc), C.byref(cc), C.byref(am),
+                                C.byref(info)), "flx_dgeequ")
+        return r[: A.m], c[: A.n], rc.value, cc.value, am.value, info.value
+
+    @staticmethod
+    def laq(A, r, c, rowcnd, colcnd, amax, handle: Handle | None = None) -> str:
+        """flens::lapack::laq on device A; returns 'N', 'R', 'C' or 'B'."""
+        h = handle or default_handle()
+        eq = C.create_string_buffer(1)
+        _check(lib().flx_dlaqge(h.ptr, A.m, A.n, _vp(A.ptr), A.ld, _vp(r.data_ptr()),
+                                _vp(c.data_ptr()), float(rowcnd), float(colcnd), float(amax),
+                                eq), "flx_dlaqge")
+        return eq.raw.decode()
+
+    @staticmethod
+    def con(norm: str, LU, anorm: float, handle: Handle | None = None) -> float:
+        """flens::lapack::con on an LU factor: reciprocal condition number."""
+        rc = C.c_double(0.0)
+        if isinstance(LU, np.ndarray):
+            n = LU.shape[0]
+            info = C.c_int(0)
+            lib().dgecon_(norm.encode(), _iref(n), LU.ctypes.data_as(_dp), _iref(max(1, n)),
+                          C.byref(C.c_double(anorm)), C.byref(rc), None, None, C.byref(info))
+            if info.value:
+                raise FlxError(f"dgecon_ info={info.value}")
+            return rc.value
+        h = handle or default_handle()
+        _check(lib().flx_dgecon(h.ptr, norm.encode(), LU.n, _vp(LU.ptr), LU.ld, float(anorm),
+                                C.byref(rc)), "flx_dgecon")
+        return rc.value
+
+    @staticmethod
+    def rfs(trans, A, AF, piv, B, X, handle: Handle | None = None):
+        """flens::lapack::rfs: refines X in place; returns (ferr, berr)."""
+        if isinstance(A, np.ndarray):
+            n, nrhs = B.shape
+            piv = np.ascontiguousarray(piv, dtype=np.int32)
+            ferr, berr = np.zeros(max(1, nrhs)), np.zeros(max(1, nrhs))
+            info = C.c_int(0)
+            ln = _iref(max(1, n))
+            lib().dgerfs_(b"T" if trans else b"N", _iref(n), _iref(nrhs), A.ctypes.data_as(_dp),
+                          ln, AF.ctypes.data_as(_dp), ln, piv.ctypes.data_as(_ip),
+                          B.ctypes.data_as(_dp), ln, X.ctypes.data_as(_dp), ln,
+                          ferr.ctypes.data_as(_dp), berr.ctypes.data_as(_dp), None, None,
+                          C.byref(info))
+            if info.value:
+                raise FlxError(f"dgerfs_ info={info.value}")
+            return ferr[:nrhs], berr[:nrhs]
+        h = handle or default_handle()
+        nrhs = B.n
+        ferr, berr = np.zeros(max(1, nrhs)), np.zeros(max(1, nrhs))
+        _check(lib().flx_dgerfs(h.ptr, int(bool(trans)), A.n, nrhs, _vp(A.ptr), A.ld,
+                                _vp(AF.ptr), AF.ld, _vp(piv.data_ptr()), _vp(B.ptr), B.ld,
+                                _vp(X.ptr), X.ld, ferr.ctypes.data_as(_dp),
+                                berr.ctypes.data_as(_dp)), "flx_dgerfs")
+        return ferr[:nrhs], berr[:nrhs]
+
+    @staticmethod
+    def svx(fact: str, trans, A, AF, piv, equed: str, r, c, B, X,
+            handle: Handle | None = None):
+        """flens::lapack::svx.  Device arguments are updated in place (host
+        numpy ones through dgesvx_).  Returns dict(rcond, ferr, berr, rpvgrw,
+        equed, info)."""
+        eq = C.create_string_buffer(equed.encode(), 1)
+        rc, g = C.c_double(0.0), C.c_double(0.0)
+        info = C.c_int(0)
+        if isinstance(A, np.ndarray):
+            n, nrhs = B.shape
+            ferr, berr = np.zeros(max(1, nrhs)), np.zeros(max(1, nrhs))
+            work = np.zeros(max(1, 4 * n))
+            ln = _iref(max(1, n))
+            lib().dgesvx_(fact.encode(), b"T" if trans else b"N", _iref(n), _iref(nrhs),
+                          A.ctypes.data_as(_dp), ln, AF.ctypes.data_as(_dp), ln,
+                          piv.ctypes.data_as(_ip), eq, r.ctypes.data_as(_dp),
+                          c.ctypes.data_as(_dp), B.ctypes.data_as(_dp), ln,
+                          X.ctypes.data_as(_dp), ln, C.byref(rc), ferr.ctypes.data_as(_dp),
+                          berr.ctypes.data_as(_dp), work.ctypes.data_as(_dp), None,
+                          C.byref(info))
+            return dict(rcond=rc.value, ferr=ferr[:nrhs], berr=berr[:nrhs], rpvgrw=work[0],
+                        equed=eq.raw.decode(), info=info.value)
+        h = handle or default_handle()
+        nrhs = B.n
+        ferr, berr = np.zeros(max(1, nrhs)), np.zeros(max(1, nrhs))
+        _check(lib().flx_dgesvx(h.ptr, fact.encode(), int(bool(trans)), A.n, nrhs, _vp(A.ptr),
+                                A.ld, _vp(AF.ptr), AF.ld, _vp(piv.data_ptr()), eq,
+                                _vp(r.data_ptr()), _vp(c.data_ptr()), _vp(B.ptr), B.ld,
+                                _vp(X.ptr), X.ld, C.byref(rc), ferr.ctypes.data_as(_dp),
+                                berr.ctypes.data_as(_dp), C.byref(g), C.byref(info)),
+               "flx_dgesvx")
+        return dict(rcond=rc.value, ferr=ferr[:nrhs], berr=berr[:nrhs], rpvgrw=g.value,
+                    equed=eq.raw.decode(), info=info.value)
+
     @staticmethod
     def trf(A, piv=None, handle: Handle | None = None):
         """LU with partial pivoting in place.  Returns (piv, info): numpy

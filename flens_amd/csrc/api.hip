@@ -99,6 +99,12 @@ extern "C" int flx_destroy(flx_handle h)
                       &x.err, &x.dr, &x.dc, &x.part, &x.scal, &x.R, &x.V, &x.w, &x.t, &x.zeros,
                       &x.mpart, &x.hbuf, &x.inv, &x.mv})
         free_buf(*b);
+    {
+        AuxBufs &y = h->aux;
+        for (DevBuf *b : {&y.slot, &y.v, &y.x, &y.r, &y.w, &y.lines, &y.isgn, &y.part, &y.ipiv1,
+                          &y.perm0})
+            free_buf(*b);
+    }
     if (x.side) hipStreamDestroy(x.side);
     if (x.ev_a) hipEventDestroy(x.ev_a);
     if (x.ev_r) hipEventDestroy(x.ev_r);
@@ -114,6 +120,7 @@ extern "C" int flx_destroy(flx_handle h)
     if (h->wk.ev_a) hipEventDestroy(h->wk.ev_a);
     if (h->wk.ev_r) hipEventDestroy(h->wk.ev_r);
     free_buf(h->lu_inv); free_buf(h->lu_mv);
+    free_buf(h->sr); free_buf(h->sc);
     prof_delete(h->prof);
     delete h;
     return FLX_SUCCESS;
@@ -730,4 +737,247 @@ extern "C" void dlaswp_(const int *N, double *A, const int *LDA, const int *K1, 
         st = run();
     }
     report("dlaswp_", st);
+}
+
+// ---- norms, equilibration, condition estimate, refinement, expert driver --
+static char up(const char *c) { return c ? (char)(c[0] & ~0x20) : ' '; }
+
+extern "C" double dlange_(const char *NORM, const int *M, const int *N, const double *A,
+                          const int *LDA, double *WORK)
+{
+    (void)WORK;
+    const int m = *M, n = *N;
+    if ((m < n ? m : n) <= 0) return 0.0;
+    char nm = up(NORM);
+    if (nm == 'E') nm = 'F';
+    if (NORM && NORM[0] == '1') nm = 'O';
+    if (nm != 'M' && nm != 'O' && nm != 'I' && nm != 'F') return 0.0;
+    flx_context *h = default_handle();
+    double v = 0.0;
+    int st = h ? FLX_SUCCESS : FLX_HIP_ERROR;
+    if (st == FLX_SUCCESS) {
+        auto run = [&]() -> int {
+            FLX_TRY(ensure(h, h->A, sizeof(double) * (size_t)m * n));
+            FLX_TRY(h2d(h->A.p, A, m, n, *LDA));
+            return flx_dlange(h, nm, m, n, (const double *)h->A.p, m, &v);
+        };
+        st = run();
+    }
+    report("dlange_", st);
+    return v;
+}
+
+extern "C" void dgeequ_(const int *M, const int *N, const double *A, const int *LDA, double *R,
+                        double *C, double *ROWCND, double *COLCND, double *AMAX, int *INFO)
+{
+    *INFO = 0;
+    if (*M < 0) *INFO = -1;
+    else if (*N < 0) *INFO = -2;
+    else if (*LDA < (*M > 1 ? *M : 1)) *INFO = -4;
+    if (*INFO != 0) {
+        int i = -*INFO;
+        xerbla_("DGEEQU", &i, 6);
+        return;
+    }
+    const int m = *M, n = *N;
+    flx_context *h = default_handle();
+    int st = h ? FLX_SUCCESS : FLX_HIP_ERROR;
+    if (st == FLX_SUCCESS) {
+        auto run = [&]() -> int {
+            FLX_TRY(ensure(h, h->A, sizeof(double) * (size_t)(m > 0 ? m : 1) * (n > 0 ? n : 1)));
+            FLX_TRY(ensure(h, h->sr, sizeof(double) * (size_t)(m > 0 ? m : 1)));
+            FLX_TRY(ensure(h, h->sc, sizeof(double) * (size_t)(n > 0 ? n : 1)));
+            FLX_TRY(h2d(h->A.p, A, m, n, *LDA));
+            int info = 0;
+            int s = flx_dgeequ(h, m, n, (const double *)h->A.p, m > 1 ? m : 1, (double *)h->sr.p,
+                               (double *)h->sc.p, ROWCND, COLCND, AMAX, &info);
+            if (s) return s;
+            FLX_TRY(hipStreamSynchronize(h->stream));
+            *INFO = info;
+            if (m > 0 && n > 0) {
+                // R holds the row maxima even when a zero row stops the routine
+                FLX_TRY(hipMemcpy(R, h->sr.p, sizeof(double) * m, hipMemcpyDeviceToHost));
+                if (info == 0 || info > m)
+                    FLX_TRY(hipMemcpy(C, h->sc.p, sizeof(double) * n, hipMemcpyDeviceToHost));
+            }
+            return FLX_SUCCESS;
+        };
+        st = run();
+    }
+    report("dgeequ_", st);
+}
+
+extern "C" void dgecon_(const char *NORM, const int *N, const double *A, const int *LDA,
+                        const double *ANORM, double *RCOND, double *WORK, int *IWORK, int *INFO)
+{
+    (void)WORK;
+    (void)IWORK;
+    const char nm = (NORM && NORM[0] == '1') ? 'O' : up(NORM);
+    *INFO = 0;
+    if (nm != 'O' && nm != 'I') *INFO = -1;
+    else if (*N < 0) *INFO = -2;
+    else if (*LDA < (*N > 1 ? *N : 1)) *INFO = -4;
+    else if (*ANORM < 0.0) *INFO = -5;
+    if (*INFO != 0) {
+        int i = -*INFO;
+        xerbla_("DGECON", &i, 6);
+        return;
+    }
+    const int n = *N;
+    flx_context *h = default_handle();
+    int st = h ? FLX_SUCCESS : FLX_HIP_ERROR;
+    if (h && n > 16384) st = FLX_UNSUPPORTED;
+    if (st == FLX_SUCCESS) {
+        auto run = [&]() -> int {
+            FLX_TRY(ensure(h, h->A, sizeof(double) * (size_t)(n > 0 ? n : 1) * (n > 0 ? n : 1)));
+            FLX_TRY(h2d(h->A.p, A, n, n, *LDA));
+            return flx_dgecon(h, nm, n, (const double *)h->A.p, n > 1 ? n : 1, *ANORM, RCOND);
+        };
+        st = run();
+    }
+    report("dgecon_", st);
+}
+
+extern "C" void dgerfs_(const char *TRANS, const int *N, const int *NRHS, const double *A,
+                        const int *LDA, const double *AF, const int *LDAF, const int *IPIV,
+                        const double *B, const int *LDB, double *X, const int *LDX, double *FERR,
+                        double *BERR, double *WORK, int *IWORK, int *INFO)
+{
+    (void)WORK;
+    (void)IWORK;
+    const char t = up(TRANS);
+    const int l = *N > 1 ? *N : 1;
+    *INFO = 0;
+    if (t != 'N' && t != 'T' && t != 'C') *INFO = -1;
+    else if (*N < 0) *INFO = -2;
+    else if (*NRHS < 0) *INFO = -3;
+    else if (*LDA < l) *INFO = -5;
+    else if (*LDAF < l) *INFO = -7;
+    else if (*LDB < l) *INFO = -10;
+    else if (*LDX < l) *INFO = -12;
+    if (*INFO != 0) {
+        int i = -*INFO;
+        xerbla_("DGERFS", &i, 6);
+        return;
+    }
+    const int n = *N, nrhs = *NRHS;
+    if (n == 0 || nrhs == 0) {
+        for (int j = 0; j < nrhs; ++j) { FERR[j] = 0.0; BERR[j] = 0.0; }
+        return;
+    }
+    flx_context *h = default_handle();
+    int st = h ? FLX_SUCCESS : FLX_HIP_ERROR;
+    if (h && n > 16384) st = FLX_UNSUPPORTED;
+    if (st == FLX_SUCCESS) {
+        auto run = [&]() -> int {
+            const size_t nn = sizeof(double) * (size_t)n * n, nb = sizeof(double) * (size_t)n * nrhs;
+            FLX_TRY(ensure(h, h->A, nn));
+            FLX_TRY(ensure(h, h->sb, nn));
+            FLX_TRY(ensure(h, h->C, nb));
+            FLX_TRY(ensure(h, h->sx, nb));
+            FLX_TRY(ensure(h, h->ipiv, sizeof(int) * (size_t)n));
+            FLX_TRY(h2d(h->A.p, A, n, n, *LDA));
+            FLX_TRY(h2d(h->sb.p, AF, n, n, *LDAF));
+            FLX_TRY(h2d(h->C.p, B, n, nrhs, *LDB));
+            FLX_TRY(h2d(h->sx.p, X, n, nrhs, *LDX));
+            FLX_TRY(hipMemcpy(h->ipiv.p, IPIV, sizeof(int) * n, hipMemcpyHostToDevice));
+            int s = flx_dgerfs(h, t != 'N', n, nrhs, (const double *)h->A.p, n,
+                               (const double *)h->sb.p, n, (const int *)h->ipiv.p,
+                               (const double *)h->C.p, n, (double *)h->sx.p, n, FERR, BERR);
+            if (s) return s;
+            FLX_TRY(hipStreamSynchronize(h->stream));
+            FLX_TRY(d2h(X, h->sx.p, n, nrhs, *LDX));
+            return FLX_SUCCESS;
+        };
+        st = run();
+    }
+    report("dgerfs_", st);
+}
+
+extern "C" void dgesvx_(const char *FACT, const char *TRANS, const int *N, const int *NRHS,
+                        double *A, const int *LDA, double *AF, const int *LDAF, int *IPIV,
+                        char *EQUED, double *R, double *C, double *B, const int *LDB, double *X,
+                        const int *LDX, double *RCOND, double *FERR, double *BERR, double *WORK,
+                        int *IWORK, int *INFO)
+{
+    (void)IWORK;
+    const char f = up(FACT), t = up(TRANS);
+    const int n = *N, nrhs = *NRHS, l = n > 1 ? n : 1;
+    char eq = (f == 'F') ? up(EQUED) : 'N';
+    const bool rowequ = f == 'F' && (eq == 'R' || eq == 'B');
+    const bool colequ = f == 'F' && (eq == 'C' || eq == 'B');
+    *INFO = 0;
+    if (f != 'N' && f != 'E' && f != 'F') *INFO = -1;
+    else if (t != 'N' && t != 'T' && t != 'C') *INFO = -2;
+    else if (n < 0) *INFO = -3;
+    else if (nrhs < 0) *INFO = -4;
+    else if (*LDA < l) *INFO = -6;
+    else if (*LDAF < l) *INFO = -8;
+    else if (f == 'F' && eq != 'N' && eq != 'R' && eq != 'C' && eq != 'B') *INFO = -10;
+    else if (*LDB < l) *INFO = -14;
+    else if (*LDX < l) *INFO = -16;
+    if (*INFO == 0 && rowequ)
+        for (int j = 0; j < n; ++j)
+            if (!(R[j] > 0.0)) { *INFO = -11; break; }
+    if (*INFO == 0 && colequ)
+        for (int j = 0; j < n; ++j)
+            if (!(C[j] > 0.0)) { *INFO = -12; break; }
+    if (*INFO != 0) {
+        int i = -*INFO;
+        xerbla_("DGESVX", &i, 6);
+        return;
+    }
+    flx_context *h = default_handle();
+    int st = h ? FLX_SUCCESS : FLX_HIP_ERROR;
+    if (h && n > 16384) st = FLX_UNSUPPORTED;
+    if (st == FLX_SUCCESS) {
+        auto run = [&]() -> int {
+            const size_t nn = sizeof(double) * (size_t)l * l;
+            const size_t nb = sizeof(double) * (size_t)l * (nrhs > 0 ? nrhs : 1);
+            FLX_TRY(ensure(h, h->A, nn));
+            FLX_TRY(ensure(h, h->sb, nn));
+            FLX_TRY(ensure(h, h->C, nb));
+            FLX_TRY(ensure(h, h->sx, nb));
+            FLX_TRY(ensure(h, h->ipiv, sizeof(int) * (size_t)l));
+            FLX_TRY(ensure(h, h->sr, sizeof(double) * (size_t)l));
+            FLX_TRY(ensure(h, h->sc, sizeof(double) * (size_t)l));
+            FLX_TRY(h2d(h->A.p, A, n, n, *LDA));
+            FLX_TRY(h2d(h->C.p, B, n, nrhs, *LDB));
+            if (f == 'F') {
+                FLX_TRY(h2d(h->sb.p, AF, n, n, *LDAF));
+                if (n > 0)
+                    FLX_TRY(hipMemcpy(h->ipiv.p, IPIV, sizeof(int) * n, hipMemcpyHostToDevice));
+                if (rowequ) FLX_TRY(h2d(h->sr.p, R, n, 1, l));
+                if (colequ) FLX_TRY(h2d(h->sc.p, C, n, 1, l));
+            }
+            double rcond = 0.0, rpvgrw = 0.0;
+            int info = 0;
+            int s = flx_dgesvx(h, f, t != 'N', n, nrhs, (double *)h->A.p, l, (double *)h->sb.p,
+                               l, (int *)h->ipiv.p, &eq, (double *)h->sr.p, (double *)h->sc.p,
+                               (double *)h->C.p, l, (double *)h->sx.p, l, &rcond, FERR, BERR,
+                               &rpvgrw, &info);
+            if (s) return s;
+            FLX_TRY(hipStreamSynchronize(h->stream));
+            *RCOND = rcond;
+            WORK[0] = rpvgrw;
+            *INFO = info;
+            *EQUED = eq;
+            if (f == 'E' && eq != 'N') FLX_TRY(d2h(A, h->A.p, n, n, *LDA));
+            if (f != 'F') {
+                FLX_TRY(d2h(AF, h->sb.p, n, n, *LDAF));
+                if (n > 0)
+                    FLX_TRY(hipMemcpy(IPIV, h->ipiv.p, sizeof(int) * n, hipMemcpyDeviceToHost));
+            }
+            if (f == 'E' && n > 0) {
+                FLX_TRY(d2h(R, h->sr.p, n, 1, l));
+                FLX_TRY(d2h(C, h->sc.p, n, 1, l));
+            }
+            const bool rowq = eq == 'R' || eq == 'B', colq = eq == 'C' || eq == 'B';
+            if ((t == 'N') ? rowq : colq) FLX_TRY(d2h(B, h->C.p, n, nrhs, *LDB));
+            if (info == 0 || info == n + 1) FLX_TRY(d2h(X, h->sx.p, n, nrhs, *LDX));
+            return FLX_SUCCESS;
+        };
+        st = run();
+    }
+    report("dgesvx_", st);
 }

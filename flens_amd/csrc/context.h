@@ -24,18 +24,26 @@ struct MixBufs {
     size_t gkey[8] = {0, 0, 0, 0, 0, 0, 0, 0};
 };
 
+// buffers of the norm / equilibration / condition / refinement routines
+struct AuxBufs {
+    DevBuf slot, v, x, r, w, lines, isgn, part, ipiv1, perm0;
+    int ident_n = 0;   // length of the identity ipiv1 / perm0 held
+};
+
 struct flx_context {
     hipStream_t stream = nullptr;
     int device = 0;
     flx::LuWork wk;
     flx::LuOptions opt;
     DevBuf W, A, B, C, X, piv0, ipiv, info, perm, tinv;
-    DevBuf sb, sx;   // staging of the dsgesv_ seam
+    DevBuf sb, sx;   // staging of the dsgesv_ seam (and AF, X of dgerfs_ / dgesvx_)
+    DevBuf sr, sc;   // staging of the equilibration scales (dgeequ_, dgesvx_)
     DevBuf mlp_part; // per-block gradients of flx_mlp_grad
     DevBuf trsw;     // tagged solution words of the single-launch getrs
     DevBuf lu_inv, lu_mv;   // lookahead: inverse permutation, block moves
     unsigned trs_epoch = 0;
     MixBufs mix;
+    AuxBufs aux;
     int last_error = 0;
     flx::Profiler *prof = nullptr;   // non-null while profiling
 };
@@ -47,6 +55,12 @@ struct ProfBind {
 };
 
 int hip_status(hipError_t e);
+
+#define FLX_RET(x)                                                            \
+    do {                                                                      \
+        int r_ = (x);                                                         \
+        if (r_) return r_;                                                    \
+    } while (0)
 
 #define FLX_TRY(x)                                                            \
     do {                                                                      \

@@ -23,12 +23,6 @@
 
 using namespace flx;
 
-#define FLX_RET(x)                                                            \
-    do {                                                                      \
-        int r_ = (x);                                                         \
-        if (r_) return r_;                                                    \
-    } while (0)
-
 namespace {
 
 constexpr double EPS64 = 1.1102230246251565e-16;   // 2^-53, lamch(Eps)

@@ -169,6 +169,65 @@ int flx_dsgesv(flx_handle h, int n, int nrhs, double *A, int lda, int *ipiv,
                const double *B, int ldb, double *X, int ldx, int *iter,
                int *info);
 
+/* ------------------- norms, equilibration, condition, refinement -----
+ * Device pointers for matrices and vectors; scalar results (value, rowcnd,
+ * rcond, ferr, berr, ...) are written to host memory and are valid when the
+ * call returns (these routines synchronise the handle's stream). */
+
+/* ||A|| for norm 'M' (max |a_ij|), 'O' or '1' (max column sum), 'I' (max
+ * row sum), 'F' (Frobenius).  Replaces flens::lapack::lan for GeMatrix
+ * (flens/lapack/la/lan.tcc:55-130).  'O' and 'I' sum in the reference's
+ * order (bit-identical); 'M' is exact; NaN entries are skipped by 'M', 'O'
+ * and 'I' as the reference's std::max does, and propagate through 'F'. */
+int flx_dlange(flx_handle h, char norm, int m, int n, const double *A, int lda,
+               double *value);
+
+/* Row and column scalings r (m) and c (n, device) that equilibrate A, with
+ * *rowcnd, *colcnd, *amax (host) and *info (host: 0, i <= m for a zero row
+ * i, m + j for a zero column j).  Replaces flens::lapack::equ
+ * (flens/lapack/ge/equ.tcc:62-194); results bit-identical. */
+int flx_dgeequ(flx_handle h, int m, int n, const double *A, int lda, double *r,
+               double *c, double *rowcnd, double *colcnd, double *amax,
+               int *info);
+
+/* Applies the scalings when they help (thresholds 0.1 and the safe range):
+ * *equed (host) = 'N', 'R', 'C' or 'B'.  Replaces flens::lapack::laq
+ * (flens/lapack/la/laq.tcc:57-118). */
+int flx_dlaqge(flx_handle h, int m, int n, double *A, int lda, const double *r,
+               const double *c, double rowcnd, double colcnd, double amax,
+               char *equed);
+
+/* Reciprocal condition number estimate of A from its LU factor (as
+ * flx_dgetrf leaves it) and ||A|| in norm 'O' ('1') or 'I'.  Replaces
+ * flens::lapack::con (flens/lapack/ge/cond.tcc:62-165) with lacn2
+ * (flens/lapack/la/lacn2.tcc:55-185); the triangular solves are the LU's,
+ * i.e. latrs (flens/lapack/la/latrs.tcc) without its rescaling, so a
+ * solution growing toward the overflow threshold yields *rcond = 0. */
+int flx_dgecon(flx_handle h, char norm, int n, const double *LU, int lda,
+               double anorm, double *rcond);
+
+/* Iterative refinement of X (n x nrhs, device) for op(A) X = B with
+ * forward and backward error bounds ferr, berr (nrhs, host).  AF, ipiv:
+ * the factor of A (device).  Replaces flens::lapack::rfs
+ * (flens/lapack/ge/rfs.tcc:63-261): at most 5 steps per right-hand side,
+ * stop when berr <= eps or it stops halving. */
+int flx_dgerfs(flx_handle h, int trans, int n, int nrhs, const double *A,
+               int lda, const double *AF, int ldaf, const int *ipiv,
+               const double *B, int ldb, double *X, int ldx, double *ferr,
+               double *berr);
+
+/* Expert driver: fact 'N' (factor A), 'E' (equilibrate, then factor) or
+ * 'F' (AF, ipiv and, per *equed, r / c given); solves op(A) X = B, with
+ * *rcond, ferr, berr, *rpvgrw (reciprocal pivot growth, LAPACK's WORK(1))
+ * and *info (0; i <= n: U(i,i) = 0; n + 1: rcond < eps) on the host.
+ * Replaces flens::lapack::svx (flens/lapack/ge/svx.tcc:60-262); unlike the
+ * reference, which takes EQUED by value, *equed reports the scaling used. */
+int flx_dgesvx(flx_handle h, char fact, int trans, int n, int nrhs, double *A,
+               int lda, double *AF, int ldaf, int *ipiv, char *equed,
+               double *r, double *c, double *B, int ldb, double *X, int ldx,
+               double *rcond, double *ferr, double *berr, double *rpvgrw,
+               int *info);
+
 /* ------------------------------------------------------ batched plans */
 /* Thousands of independent small problems in one persistent launch each
  * (no counterpart in the reference, whose trf / trs / mm handle one matrix
@@ -283,6 +342,30 @@ void dlaswp_(const int *N, double *A, const int *LDA, const int *K1,
 void dsgesv_(const int *N, const int *NRHS, double *A, const int *LDA,
              int *IPIV, double *B, const int *LDB, double *X, const int *LDX,
              double *WORK, float *SWORK, int *ITER, int *INFO);
+
+/* Fortran seams of the routines above (host pointers; WORK / IWORK are
+ * accepted and unused except dgesvx_'s WORK(1) = reciprocal pivot growth):
+ * cxxlapack/netlib/interface/lapack.in.h:8145-8151 (dlange), 6155-6166
+ * (dgeequ), 6143-6153 (dgecon), 6535-6553 (dgerfs), 6676-6699 (dgesvx). */
+double dlange_(const char *NORM, const int *M, const int *N, const double *A,
+               const int *LDA, double *WORK);
+void dgeequ_(const int *M, const int *N, const double *A, const int *LDA,
+             double *R, double *C, double *ROWCND, double *COLCND,
+             double *AMAX, int *INFO);
+void dgecon_(const char *NORM, const int *N, const double *A, const int *LDA,
+             const double *ANORM, double *RCOND, double *WORK, int *IWORK,
+             int *INFO);
+void dgerfs_(const char *TRANS, const int *N, const int *NRHS, const double *A,
+             const int *LDA, const double *AF, const int *LDAF,
+             const int *IPIV, const double *B, const int *LDB, double *X,
+             const int *LDX, double *FERR, double *BERR, double *WORK,
+             int *IWORK, int *INFO);
+void dgesvx_(const char *FACT, const char *TRANS, const int *N,
+             const int *NRHS, double *A, const int *LDA, double *AF,
+             const int *LDAF, int *IPIV, char *EQUED, double *R, double *C,
+             double *B, const int *LDB, double *X, const int *LDX,
+             double *RCOND, double *FERR, double *BERR, double *WORK,
+             int *IWORK, int *INFO);
 
 /* Error hook called before a negative INFO is returned, as the reference's
  * LAPACK_ERROR (flens/lapack/interface/include/config.h:40-45).  libflx

@@ -361,3 +361,6 @@ int orc_dgetrf_gaps(int m, int n, double *a, int lda, int *ipiv, int nb,
 {
     return orc_dgetrf_gap(m, n, a, lda, ipiv, nb, gap);
 }
+
+/* norms, equilibration, condition estimate, refinement, expert driver */
+#include "aux_body.h"

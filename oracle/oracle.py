@@ -70,6 +70,20 @@ def lib():
         _lib.orc_dsgesv.argtypes = [_i, _i, _dp, _i, _ip, _dp, _i, _dp, _i, _i, _ip]
         _lib.orc_dsgesv.restype = _i
         _lib.orc_dgetrf_steps.argtypes = [_i, _i, _dp, _i, _i, _i]
+        _lib.orc_dlange.argtypes = [C.c_char, _i, _i, _dp, _i]
+        _lib.orc_dlange.restype = _d
+        _lib.orc_dgeequ.argtypes = [_i, _i, _dp, _i, _dp, _dp, _dp, _dp, _dp]
+        _lib.orc_dgeequ.restype = _i
+        _lib.orc_dlaqge.argtypes = [_i, _i, _dp, _i, _dp, _dp, _d, _d, _d]
+        _lib.orc_dlaqge.restype = _i
+        _lib.orc_dgecon.argtypes = [C.c_char, _i, _dp, _i, _d]
+        _lib.orc_dgecon.restype = _d
+        _lib.orc_dgerfs.argtypes = [_i, _i, _i, _dp, _i, _dp, _i, _ip, _dp, _i, _dp, _i, _dp,
+                                    _dp]
+        _lib.orc_dgesvx.argtypes = [C.c_char, _i, _i, _i, _dp, _i, _dp, _i, _ip,
+                                    C.POINTER(C.c_char), _dp, _dp, _dp, _i, _dp, _i, _dp, _dp,
+                                    _dp, _dp, _i]
+        _lib.orc_dgesvx.restype = _i
     return _lib
 
 
@@ -87,6 +101,19 @@ def ref():
         _ref.ref_dgesv.argtypes = [_i, _i, _dp, _i, _ip, _dp, _i, _i]
         _ref.ref_dgesv.restype = _i
         _ref.ref_dlaswp.argtypes = [_i, _i, _dp, _i, _i, _i, _ip, _i]
+        _ref.ref_dlange.argtypes = [C.c_char, _i, _i, _dp, _i]
+        _ref.ref_dlange.restype = _d
+        _ref.ref_dgeequ.argtypes = [_i, _i, _dp, _i, _dp, _dp, _dp, _dp, _dp]
+        _ref.ref_dgeequ.restype = _i
+        _ref.ref_dlaqge.argtypes = [_i, _i, _dp, _i, _dp, _dp, _d, _d, _d]
+        _ref.ref_dlaqge.restype = _i
+        _ref.ref_dgecon.argtypes = [C.c_char, _i, _dp, _i, _d]
+        _ref.ref_dgecon.restype = _d
+        _ref.ref_dgerfs.argtypes = [_i, _i, _i, _dp, _i, _dp, _i, _ip, _dp, _i, _dp, _i, _dp,
+                                    _dp]
+        _ref.ref_dgesvx.argtypes = [C.c_char, _i, _i, _i, _dp, _i, _dp, _i, _ip, C.c_char,
+                                    _dp, _dp, _dp, _i, _dp, _i, _dp, _dp, _dp, _dp, _i]
+        _ref.ref_dgesvx.restype = _i
     return _ref
 
 
@@ -262,3 +289,81 @@ def dsgesv(a, b, itermax=30):
     info = lib().orc_dsgesv(n, nrhs, _p(a), _ld(a), _p(ipiv), _p(b), _ld(b), _p(x), _ld(x),
                             itermax, C.byref(it))
     return x, ipiv[:n], info, it.value
+
+
+# ------------------------------------------ norms, equilibration, rcond ---
+def _impl(use_ref):
+    return (ref(), "ref_") if use_ref else (lib(), "orc_")
+
+
+def dlange(norm, a, use_ref=False):
+    """flens::lapack::lan (GeMatrix): norm 'M', 'O', 'I' or 'F'."""
+    L, p = _impl(use_ref)
+    m, n = a.shape
+    return getattr(L, p + "dlange")(norm.encode(), m, n, _p(a), _ld(a))
+
+
+def dgeequ(a, use_ref=False):
+    """flens::lapack::equ -> (r, c, rowcnd, colcnd, amax, info); rowcnd and
+    colcnd are NaN where the routine leaves them unset."""
+    L, p = _impl(use_ref)
+    m, n = a.shape
+    r = np.zeros(max(1, m))
+    c = np.zeros(max(1, n))
+    rc, cc, am = C.c_double(np.nan), C.c_double(np.nan), C.c_double(np.nan)
+    info = getattr(L, p + "dgeequ")(m, n, _p(a), _ld(a), _p(r), _p(c), C.byref(rc),
+                                    C.byref(cc), C.byref(am))
+    return r[:m], c[:n], rc.value, cc.value, am.value, info
+
+
+def dlaqge(a, r, c, rowcnd, colcnd, amax, use_ref=False):
+    """flens::lapack::laq: scales a in place, returns 'N', 'R', 'C' or 'B'."""
+    L, p = _impl(use_ref)
+    m, n = a.shape
+    r = np.ascontiguousarray(r, dtype=np.float64)
+    c = np.ascontiguousarray(c, dtype=np.float64)
+    return chr(getattr(L, p + "dlaqge")(m, n, _p(a), _ld(a), _p(r), _p(c), float(rowcnd),
+                                        float(colcnd), float(amax)))
+
+
+def dgecon(norm, lu, anorm, use_ref=False):
+    """flens::lapack::con on an LU factor: reciprocal condition number."""
+    L, p = _impl(use_ref)
+    n = lu.shape[0]
+    return getattr(L, p + "dgecon")(norm.encode(), n, _p(lu), _ld(lu), float(anorm))
+
+
+def dgerfs(trans, a, af, ipiv, b, x, use_ref=False):
+    """flens::lapack::rfs: refines x in place; returns (ferr, berr)."""
+    L, p = _impl(use_ref)
+    n, nrhs = b.shape
+    ipiv = np.ascontiguousarray(ipiv, dtype=np.int32)
+    ferr = np.zeros(max(1, nrhs))
+    berr = np.zeros(max(1, nrhs))
+    getattr(L, p + "dgerfs")(int(trans), n, nrhs, _p(a), _ld(a), _p(af), _ld(af), _p(ipiv),
+                             _p(b), _ld(b), _p(x), _ld(x), _p(ferr), _p(berr))
+    return ferr[:nrhs], berr[:nrhs]
+
+
+def dgesvx(fact, trans, a, af, ipiv, equed, r, c, b, nb=64, use_ref=False):
+    """flens::lapack::svx.  a, af, ipiv, r, c, b are updated in place.
+    Returns dict(x, rcond, ferr, berr, rpvgrw, equed, info); the compiled
+    reference cannot report equed (it takes it by value)."""
+    n, nrhs = b.shape
+    x = np.zeros((n, nrhs), order="F")
+    ferr = np.zeros(max(1, nrhs))
+    berr = np.zeros(max(1, nrhs))
+    rc, g = C.c_double(0.0), C.c_double(0.0)
+    if use_ref:
+        info = ref().ref_dgesvx(fact.encode(), int(trans), n, nrhs, _p(a), _ld(a), _p(af),
+                                _ld(af), _p(ipiv), equed.encode(), _p(r), _p(c), _p(b), _ld(b),
+                                _p(x), _ld(x), C.byref(rc), _p(ferr), _p(berr), C.byref(g), nb)
+        eq = None
+    else:
+        e = C.c_char(equed.encode())
+        info = lib().orc_dgesvx(fact.encode(), int(trans), n, nrhs, _p(a), _ld(a), _p(af),
+                                _ld(af), _p(ipiv), C.byref(e), _p(r), _p(c), _p(b), _ld(b),
+                                _p(x), _ld(x), C.byref(rc), _p(ferr), _p(berr), C.byref(g), nb)
+        eq = e.value.decode()
+    return dict(x=x, rcond=rc.value, ferr=ferr[:nrhs], berr=berr[:nrhs], rpvgrw=g.value,
+                equed=eq, info=info)

@@ -98,4 +98,105 @@ void ref_dlaswp(int m, int n, double *a, int lda, int k1, int k2,
     }
 }
 
+// ---- FLENS-LAPACK auxiliaries: norms, equilibration, condition estimate,
+// refinement, expert driver (generic implementations) --------------------
+
+static lapack::Norm ref_norm(char c)
+{
+    switch (c) {
+    case 'M': case 'm': return lapack::MaximumNorm;
+    case 'O': case 'o': case '1': return lapack::OneNorm;
+    case 'I': case 'i': return lapack::InfinityNorm;
+    default: return lapack::FrobeniusNorm;
+    }
+}
+
+typedef DenseVector<ArrayView<double> > DVecView;
+typedef DenseVector<Array<double> >     DVec;
+typedef DenseVector<Array<int> >        IVec;
+
+// flens::lapack::lan, GeMatrix variant (flens/lapack/la/lan.tcc:55-130)
+double ref_dlange(char norm, int m, int n, const double *a, int lda)
+{
+    DGeConstView A = DConstView(m, n, a, std::max(lda, 1));
+    DVec work(std::max(m, 1));
+    return lapack::lan(ref_norm(norm), A, work);
+}
+
+// flens::lapack::equ (flens/lapack/ge/equ.tcc:62-194)
+int ref_dgeequ(int m, int n, const double *a, int lda, double *r, double *c,
+               double *rowcnd, double *colcnd, double *amax)
+{
+    DGeConstView A = DConstView(m, n, a, std::max(lda, 1));
+    DVecView R = ArrayView<double>(m, r, 1);
+    DVecView Cv = ArrayView<double>(n, c, 1);
+    return lapack::equ(A, R, Cv, *rowcnd, *colcnd, *amax);
+}
+
+// flens::lapack::laq (flens/lapack/la/laq.tcc:57-118); returns 'N','R','C','B'
+int ref_dlaqge(int m, int n, double *a, int lda, const double *r, const double *c,
+               double rowcnd, double colcnd, double amax)
+{
+    DGeView A = DView(m, n, a, std::max(lda, 1));
+    DVecView R = ArrayView<double>(m, const_cast<double *>(r), 1);
+    DVecView Cv = ArrayView<double>(n, const_cast<double *>(c), 1);
+    return (int)lapack::laq(A, R, Cv, rowcnd, colcnd, amax);
+}
+
+// flens::lapack::con, real variant (flens/lapack/ge/cond.tcc:62-165) on an
+// LU factor as trf leaves it
+double ref_dgecon(char norm, int n, const double *lu, int lda, double anorm)
+{
+    DGeConstView A = DConstView(n, n, lu, std::max(lda, 1));
+    DVec work(4 * std::max(n, 1));
+    IVec iwork(std::max(n, 1));
+    double rcond = 0;
+    lapack::con(ref_norm(norm), A, anorm, rcond, work, iwork);
+    return rcond;
+}
+
+// flens::lapack::rfs (flens/lapack/ge/rfs.tcc:63-261)
+void ref_dgerfs(int trans, int n, int nrhs, const double *a, int lda, const double *af,
+                int ldaf, const int *ipiv, const double *b, int ldb, double *x, int ldx,
+                double *ferr, double *berr)
+{
+    DGeConstView A = DConstView(n, n, a, std::max(lda, 1));
+    DGeConstView AF = DConstView(n, n, af, std::max(ldaf, 1));
+    IView piv = ArrayView<int>(n, const_cast<int *>(ipiv), 1);
+    DGeConstView B = DConstView(n, nrhs, b, std::max(ldb, 1));
+    DGeView X = DView(n, nrhs, x, std::max(ldx, 1));
+    DVecView F = ArrayView<double>(nrhs, ferr, 1);
+    DVecView Bk = ArrayView<double>(nrhs, berr, 1);
+    DVec work(3 * std::max(n, 1));
+    IVec iwork(std::max(n, 1));
+    lapack::rfs(trans ? Trans : NoTrans, A, AF, piv, B, X, F, Bk, work, iwork);
+}
+
+// flens::lapack::svx (flens/lapack/ge/svx.tcc:60-262).  fact: 'N','E','F';
+// equed is an input only (the reference takes it by value); work0 receives
+// the reciprocal pivot growth (work(1)).
+int ref_dgesvx(char fact, int trans, int n, int nrhs, double *a, int lda, double *af,
+               int ldaf, int *ipiv, char equed, double *r, double *c, double *b, int ldb,
+               double *x, int ldx, double *rcond, double *ferr, double *berr, double *work0,
+               int nb)
+{
+    g_getrf_nb = nb;
+    DGeView A = DView(n, n, a, std::max(lda, 1));
+    DGeView AF = DView(n, n, af, std::max(ldaf, 1));
+    IView piv = ArrayView<int>(n, ipiv, 1);
+    DVecView R = ArrayView<double>(n, r, 1);
+    DVecView Cv = ArrayView<double>(n, c, 1);
+    DGeView B = DView(n, nrhs, b, std::max(ldb, 1));
+    DGeView X = DView(n, nrhs, x, std::max(ldx, 1));
+    DVecView F = ArrayView<double>(nrhs, ferr, 1);
+    DVecView Bk = ArrayView<double>(nrhs, berr, 1);
+    DVec work(4 * std::max(n, 1));
+    IVec iwork(std::max(n, 1));
+    const int info = lapack::svx(lapack::SVX::Fact(fact), trans ? Trans : NoTrans, A, AF, piv,
+                                 lapack::SVX::Equilibration(equed), R, Cv, B, X, *rcond, F, Bk, work,
+                                 iwork);
+    *work0 = work(1);
+    return info;
+}
+
 } // extern "C"
