@@ -311,9 +311,18 @@ def aux_section(n: int = 8192, seed: int = 5):
     t, d = timed(svx_once, reps=2)
     out["svx_E"] = {"ms": 1e3 * t, "equed": d["equed"], "rcond": d["rcond"],
                     "ferr": float(d["ferr"][0]), "berr": float(d["berr"][0]), "info": d["info"]}
+    # getri: inv(A) from the factor (LAPACK's count 4/3 n^3 flops)
+    Ainv = F.DeviceMatrix.empty(n, n)
+
+    def tri_once():
+        Ainv.storage.copy_(LU.storage)
+        return F.lapack.tri(Ainv, piv)
+    t, info_t = timed(tri_once, reps=2)
+    out["tri"] = {"ms": 1e3 * t, "info": info_t,
+                  "tflops": 4.0 / 3.0 * n ** 3 / t / 1e12}
     progress(f"aux: lan O {out['lan_O']['ms']:.2f} ms, equ {out['equ']['ms']:.2f} ms, "
              f"con {out['con']['ms']:.1f} ms, rfs {out['rfs']['ms']:.1f} ms, "
-             f"svx(E) {out['svx_E']['ms']:.1f} ms")
+             f"svx(E) {out['svx_E']['ms']:.1f} ms, tri {out['tri']['ms']:.1f} ms")
     return out
 
 

@@ -44,8 +44,9 @@ EXPORTS = [
     "flx_batch_gesv_create", "flx_batch_gemm_create", "flx_batch_execute", "flx_batch_config",
     "flx_batch_destroy", "flx_mlp_forward", "flx_mlp_grad", "flx_mlp_adam",
     "flx_dlange", "flx_dgeequ", "flx_dlaqge", "flx_dgecon", "flx_dgerfs", "flx_dgesvx",
+    "flx_dgetri",
     "cblas_dgemm", "cblas_dtrsm", "dgetrf_", "dgetrs_", "dgesv_", "dlaswp_", "dsgesv_",
-    "dlange_", "dgeequ_", "dgecon_", "dgerfs_", "dgesvx_",
+    "dlange_", "dgeequ_", "dgecon_", "dgerfs_", "dgesvx_", "dgetri_",
     "xerbla_",
 ]
 
@@ -111,6 +112,8 @@ def lib():
     L.flx_dgerfs.argtypes = [_vp, _i, _i, _i, _vp, _i, _vp, _i, _vp, _vp, _i, _vp, _i, _dp, _dp]
     L.flx_dgesvx.argtypes = [_vp, _c, _i, _i, _i, _vp, _i, _vp, _i, _vp, _cp, _vp, _vp, _vp, _i,
                              _vp, _i, _dp, _dp, _dp, _dp, _ip]
+    L.flx_dgetri.argtypes = [_vp, _i, _vp, _i, _vp, _ip]
+    L.dgetri_.argtypes = [_ip, _dp, _ip, _ip, _dp, _ip, _ip]
     L.dlange_.argtypes = [_cp, _ip, _ip, _dp, _ip, _dp]
     L.dlange_.restype = _d
     L.dgeequ_.argtypes = [_ip, _ip, _dp, _ip, _dp, _dp, _dp, _dp, _dp, _ip]
@@ -422,6 +425,24 @@ class lapack:
                "flx_dgesvx")
         return dict(rcond=rc.value, ferr=ferr[:nrhs], berr=berr[:nrhs], rpvgrw=g.value,
                     equed=eq.raw.decode(), info=info.value)
+
+    @staticmethod
+    def tri(A, piv, handle: Handle | None = None) -> int:
+        """flens::lapack::tri (GeMatrix): inv(A) in place from its LU factor
+        and pivots; returns info (i > 0: U(i,i) = 0, A untouched)."""
+        info = C.c_int(0)
+        if isinstance(A, np.ndarray):
+            n = A.shape[0]
+            piv = np.ascontiguousarray(piv, dtype=np.int32)
+            work = np.zeros(max(1, n))
+            lib().dgetri_(_iref(n), A.ctypes.data_as(_dp), _iref(max(1, n)),
+                          piv.ctypes.data_as(_ip), work.ctypes.data_as(_dp), _iref(max(1, n)),
+                          C.byref(info))
+            return info.value
+        h = handle or default_handle()
+        _check(lib().flx_dgetri(h.ptr, A.n, _vp(A.ptr), A.ld, _vp(piv.data_ptr()),
+                                C.byref(info)), "flx_dgetri")
+        return info.value
 
     @staticmethod
     def trf(A, piv=None, handle: Handle | None = None):

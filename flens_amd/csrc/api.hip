@@ -981,3 +981,41 @@ extern "C" void dgesvx_(const char *FACT, const char *TRANS, const int *N, const
     }
     report("dgesvx_", st);
 }
+
+extern "C" void dgetri_(const int *N, double *A, const int *LDA, const int *IPIV, double *WORK,
+                        const int *LWORK, int *INFO)
+{
+    const int n = *N;
+    // workspace query: the reference reports n * nb with nb = 64 (ilaenv GETRI)
+    const int lwopt = (n > 1 ? n : 1) * 64;
+    *INFO = 0;
+    if (n < 0) *INFO = -1;
+    else if (*LDA < (n > 1 ? n : 1)) *INFO = -3;
+    else if (*LWORK < (n > 1 ? n : 1) && *LWORK != -1) *INFO = -6;
+    if (*INFO != 0) {
+        int i = -*INFO;
+        xerbla_("DGETRI", &i, 6);
+        return;
+    }
+    if (WORK) WORK[0] = (double)lwopt;
+    if (*LWORK == -1 || n == 0) return;
+    flx_context *h = default_handle();
+    int st = h ? FLX_SUCCESS : FLX_HIP_ERROR;
+    if (h && n > 16384) st = FLX_UNSUPPORTED;
+    if (st == FLX_SUCCESS) {
+        auto run = [&]() -> int {
+            FLX_TRY(ensure(h, h->A, sizeof(double) * (size_t)n * n));
+            FLX_TRY(ensure(h, h->ipiv, sizeof(int) * (size_t)n));
+            FLX_TRY(h2d(h->A.p, A, n, n, *LDA));
+            FLX_TRY(hipMemcpy(h->ipiv.p, IPIV, sizeof(int) * n, hipMemcpyHostToDevice));
+            int info = 0;
+            int s = flx_dgetri(h, n, (double *)h->A.p, n, (const int *)h->ipiv.p, &info);
+            if (s) return s;
+            *INFO = info;
+            if (info == 0) FLX_TRY(d2h(A, h->A.p, n, n, *LDA));
+            return FLX_SUCCESS;
+        };
+        st = run();
+    }
+    report("dgetri_", st);
+}

@@ -228,6 +228,13 @@ int flx_dgesvx(flx_handle h, char fact, int trans, int n, int nrhs, double *A,
                double *rcond, double *ferr, double *berr, double *rpvgrw,
                int *info);
 
+/* inv(A) in place from the LU factor (A, ipiv as flx_dgetrf leaves them);
+ * *info (host) = i > 0 when U(i,i) = 0, A then untouched.  Replaces
+ * flens::lapack::tri for GeMatrix (flens/lapack/ge/tri.tcc:64-187, with
+ * trtri flens/lapack/tr/tri.tcc); triangular inverses by recursive
+ * doubling on DGEMM, inv(A) = inv(U) inv(L) P^T. */
+int flx_dgetri(flx_handle h, int n, double *A, int lda, const int *ipiv, int *info);
+
 /* ------------------------------------------------------ batched plans */
 /* Thousands of independent small problems in one persistent launch each
  * (no counterpart in the reference, whose trf / trs / mm handle one matrix
@@ -366,6 +373,11 @@ void dgesvx_(const char *FACT, const char *TRANS, const int *N,
              double *B, const int *LDB, double *X, const int *LDX,
              double *RCOND, double *FERR, double *BERR, double *WORK,
              int *IWORK, int *INFO);
+
+/* lapack.in.h:6758-6766.  LWORK = -1 is a workspace query (WORK(1) =
+ * 64 n, the reference's n * ilaenv(GETRI)); WORK is otherwise unused. */
+void dgetri_(const int *N, double *A, const int *LDA, const int *IPIV,
+             double *WORK, const int *LWORK, int *INFO);
 
 /* Error hook called before a negative INFO is returned, as the reference's
  * LAPACK_ERROR (flens/lapack/interface/include/config.h:40-45).  libflx

@@ -733,4 +733,50 @@ int orc_dgesvx(char fact, int trans, int n, int nrhs, double *a, int lda, double
     return info;
 }
 
+/* flens/lapack/ge/tri.tcc:64-187 with trtri (flens/lapack/tr/tri.tcc) on
+ * their unblocked paths -- ti2 (flens/lapack/tr/ti2.tcc:58-90, upper,
+ * non-unit, trmv per cxxblas/level2/trmv.tcc) and the column-by-column solve
+ * of inv(A) L = inv(U) (gemv per cxxblas/level2/gemv.tcc) -- which the
+ * reference runs for n <= 64 (ilaenv GETRI = TRTRI = 64).  Returns info
+ * (first zero U(i,i), A untouched). */
+int orc_dgetri(int n, double *a, int lda, const int *ipiv)
+{
+    if (n == 0) return 0;
+    for (int i = 0; i < n; ++i)
+        if (A_(i, i) == 0.0) return i + 1;
+    /* ti2: inv(U) in place */
+    for (int j = 0; j < n; ++j) {
+        A_(j, j) = 1.0 / A_(j, j);
+        const double ajj = -A_(j, j);
+        for (int i = 0; i < j; ++i) {   /* u12 <- U11 u12 (U11 already inverted) */
+            double t = 0.0;
+            for (int k = i; k < j; ++k) t += A_(i, k) * A_(k, j);
+            A_(i, j) = t;
+        }
+        for (int i = 0; i < j; ++i) A_(i, j) *= ajj;
+    }
+    /* inv(A) L = inv(U), columns n..1 */
+    double *work = (double *)calloc((size_t)(n > 0 ? n : 1), sizeof(double));
+    for (int j = n - 1; j >= 0; --j) {
+        for (int i = j + 1; i < n; ++i) { work[i] = A_(i, j); A_(i, j) = 0.0; }
+        if (j < n - 1) {
+            for (int i = 0; i < n; ++i) {
+                const double t = aux_dot(n - j - 1, &A_(i, j + 1), lda, work + j + 1, 1);
+                A_(i, j) += -1.0 * t;
+            }
+        }
+    }
+    free(work);
+    for (int j = n - 2; j >= 0; --j) {
+        const int jp = ipiv[j] - 1;
+        if (jp != j)
+            for (int i = 0; i < n; ++i) {
+                const double t = A_(i, j);
+                A_(i, j) = A_(i, jp);
+                A_(i, jp) = t;
+            }
+    }
+    return 0;
+}
+
 #undef A_

@@ -84,6 +84,8 @@ def lib():
                                     C.POINTER(C.c_char), _dp, _dp, _dp, _i, _dp, _i, _dp, _dp,
                                     _dp, _dp, _i]
         _lib.orc_dgesvx.restype = _i
+        _lib.orc_dgetri.argtypes = [_i, _dp, _i, _ip]
+        _lib.orc_dgetri.restype = _i
     return _lib
 
 
@@ -114,6 +116,8 @@ def ref():
         _ref.ref_dgesvx.argtypes = [C.c_char, _i, _i, _i, _dp, _i, _dp, _i, _ip, C.c_char,
                                     _dp, _dp, _dp, _i, _dp, _i, _dp, _dp, _dp, _dp, _i]
         _ref.ref_dgesvx.restype = _i
+        _ref.ref_dgetri.argtypes = [_i, _dp, _i, _ip]
+        _ref.ref_dgetri.restype = _i
     return _ref
 
 
@@ -367,3 +371,12 @@ def dgesvx(fact, trans, a, af, ipiv, equed, r, c, b, nb=64, use_ref=False):
         eq = e.value.decode()
     return dict(x=x, rcond=rc.value, ferr=ferr[:nrhs], berr=berr[:nrhs], rpvgrw=g.value,
                 equed=eq, info=info)
+
+
+def dgetri(a, ipiv, use_ref=False):
+    """flens::lapack::tri (GeMatrix): inv(A) in place from the LU factor;
+    returns info."""
+    L, p = _impl(use_ref)
+    n = a.shape[0]
+    ipiv = np.ascontiguousarray(ipiv, dtype=np.int32)
+    return getattr(L, p + "dgetri")(n, _p(a), _ld(a), _p(ipiv))

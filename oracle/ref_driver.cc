@@ -199,4 +199,13 @@ int ref_dgesvx(char fact, int trans, int n, int nrhs, double *a, int lda, double
     return info;
 }
 
+// flens::lapack::tri for GeMatrix (flens/lapack/ge/tri.tcc:64-187)
+int ref_dgetri(int n, double *a, int lda, const int *ipiv)
+{
+    DGeView A = DView(n, n, a, std::max(lda, 1));
+    IView piv = ArrayView<int>(n, const_cast<int *>(ipiv), 1);
+    DVec work;
+    return lapack::tri(A, piv, work);
+}
+
 } // extern "C"

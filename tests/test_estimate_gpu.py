@@ -215,3 +215,47 @@ def test_svx_factored_and_singular(gpu):
     cs = torch.zeros(8, dtype=torch.float64, device="cuda")
     ds = gpu.lapack.svx("N", 0, As, AFs, ps, "N", rs, cs, Bs, Xs)
     assert ds["info"] == 6 and ds["rcond"] == 0.0
+
+
+# ----------------------------------------------------------------- getri ---
+@pytest.mark.parametrize("n", [1, 7, 64, 65, 200, 1000, 2500])
+def test_getri_matches_oracle(gpu, n):
+    import torch
+    a = rnd(n, n, 120 + n)
+    lu = a.copy(order="F")
+    piv, _ = O.dgetrf(lu)
+    xo = lu.copy(order="F")
+    assert O.dgetri(xo, piv) == 0
+    A = dev(gpu, lu)
+    pd = torch.from_numpy(piv.astype(np.int32)).cuda()
+    assert gpu.lapack.tri(A, pd) == 0
+    x = A.numpy()
+    # dget03 ratio of the device inverse, and closeness to the oracle's
+    r = np.linalg.norm(np.eye(n) - a @ x, 1)
+    ratio = r / (n * np.linalg.norm(a, 1) * np.linalg.norm(x, 1) * EPS)
+    assert ratio < 30, (n, ratio)
+    kappa = np.linalg.norm(a, 1) * np.linalg.norm(xo, 1)
+    assert np.abs(x - xo).max() <= 8 * n * EPS * kappa * np.abs(xo).max(), n
+    # the dgetri_ seam
+    xh = lu.copy(order="F")
+    assert gpu.lapack.tri(xh, piv) == 0
+    assert np.array_equal(xh, x)
+
+
+def test_getri_closed_forms_and_singular(gpu):
+    import torch
+    rng = np.random.default_rng(110)
+    n = 300
+    p = np.asfortranarray(np.eye(n)[rng.permutation(n)])
+    lu = p.copy(order="F")
+    piv, _ = O.dgetrf(lu)
+    A = dev(gpu, lu)
+    assert gpu.lapack.tri(A, torch.from_numpy(piv.astype(np.int32)).cuda()) == 0
+    assert np.array_equal(A.numpy(), p.T)                  # inv(P) = P^T exactly
+    s = rnd(70, 70, 111)
+    s[:, 40] = 0.0
+    lu = s.copy(order="F")
+    piv, info = O.dgetrf(lu)
+    A = dev(gpu, lu)
+    assert gpu.lapack.tri(A, torch.from_numpy(piv.astype(np.int32)).cuda()) == info == 41
+    assert np.array_equal(A.numpy(), lu)                   # untouched

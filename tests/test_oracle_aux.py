@@ -294,3 +294,50 @@ def test_svx_vs_reference(n):
     f = _svx("F", 0, a, b, True, equed=d["equed"], r=r, c=c, af=af, piv=piv)
     for k in ("x", "ferr", "berr", "rcond", "rpvgrw", "info"):
         assert same(o[6][k], f[6][k]), (n, "F", k)
+
+
+# ----------------------------------------------------------------- getri ---
+def test_getri_closed_forms():
+    rng = np.random.default_rng(110)
+    n = 9
+    p = np.asfortranarray(np.eye(n)[rng.permutation(n)])
+    lu = p.copy(order="F")
+    piv, _ = O.dgetrf(lu)
+    assert O.dgetri(lu, piv) == 0 and np.array_equal(lu, p.T)   # inv(P) = P^T exactly
+    d = np.ldexp(1.0, rng.integers(-5, 6, size=n)) * np.where(rng.random(n) < 0.5, -1, 1)
+    lu = np.asfortranarray(np.diag(d))
+    piv, _ = O.dgetrf(lu)
+    assert O.dgetri(lu, piv) == 0 and np.array_equal(lu, np.diag(1.0 / d))
+    s = rnd(6, 6, 111)
+    s[:, 2] = 0.0
+    lu = s.copy(order="F")
+    piv, info = O.dgetrf(lu)
+    before = lu.copy(order="F")
+    assert O.dgetri(lu, piv) == info == 3 and np.array_equal(lu, before)
+
+
+def test_getri_inverse_residual():
+    for n in (1, 7, 64, 130):
+        a = rnd(n, n, 120 + n)
+        lu = a.copy(order="F")
+        piv, _ = O.dgetrf(lu)
+        assert O.dgetri(lu, piv) == 0
+        # LAPACK dget03: ||I - A inv(A)||_1 / (n ||A||_1 ||inv(A)||_1 eps) < 30
+        r = np.linalg.norm(np.eye(n) - a @ lu, 1)
+        ratio = r / (n * np.linalg.norm(a, 1) * np.linalg.norm(lu, 1) * EPS)
+        assert ratio < 30, (n, ratio)
+
+
+@needs_ref
+def test_getri_vs_reference():
+    # bit-exact on the reference's unblocked path (n <= 64), close beyond
+    for n in (1, 2, 5, 40, 64, 65, 150):
+        a = rnd(n, n, 130 + n)
+        lu = a.copy(order="F")
+        piv, _ = O.dgetrf(lu)
+        x1, x2 = lu.copy(order="F"), lu.copy(order="F")
+        assert O.dgetri(x1, piv) == O.dgetri(x2, piv, use_ref=True) == 0
+        if n <= 64:
+            assert same(x1, x2), n
+        else:
+            assert np.abs(x1 - x2).max() <= 64 * n * EPS * np.abs(x2).max(), n
