@@ -45,7 +45,7 @@ EXPORTS = [
     "flx_batch_destroy", "flx_mlp_forward", "flx_mlp_grad", "flx_mlp_adam",
     "flx_dlange", "flx_dgeequ", "flx_dlaqge", "flx_dgecon", "flx_dgerfs", "flx_dgesvx",
     "flx_dgetri",
-    "cblas_dgemm", "cblas_dtrsm", "dgetrf_", "dgetrs_", "dgesv_", "dlaswp_", "dsgesv_",
+    "cblas_dgemm", "cblas_dtrsm", "dgemm_", "dgetrf_", "dgetrs_", "dgesv_", "dlaswp_", "dsgesv_",
     "dlange_", "dgeequ_", "dgecon_", "dgerfs_", "dgesvx_", "dgetri_",
     "xerbla_",
 ]

@@ -425,8 +425,11 @@ static hipError_t d2h(double *dst, const void *src, int rows, int cols, int ld)
                        sizeof(double) * rows, cols, hipMemcpyDeviceToHost);
 }
 
+// blas_beta0 = 1: BLAS convention (beta = 0 never reads C); 0: the FLENS
+// generic order, C scaled by beta first (0 * C propagates NaN / Inf)
 static int host_gemm(int ta, int tb, int M, int N, int K, double alpha, const double *A, int lda,
-                     const double *B, int ldb, double beta, double *C, int ldc)
+                     const double *B, int ldb, double beta, double *C, int ldc,
+                     int blas_beta0 = 1)
 {
     flx_context *h = default_handle();
     if (!h) return FLX_HIP_ERROR;
@@ -440,10 +443,10 @@ static int host_gemm(int ta, int tb, int M, int N, int K, double alpha, const do
         FLX_TRY(h2d(h->A.p, A, ar, ac, lda));
         FLX_TRY(h2d(h->B.p, B, br, bc, ldb));
     }
-    if (beta != 0.0) FLX_TRY(h2d(h->C.p, C, M, N, ldc));
+    if (beta != 0.0 || !blas_beta0) FLX_TRY(h2d(h->C.p, C, M, N, ldc));
     FLX_TRY(gemm_f64(h->stream, ta, tb, M, N, need_ab ? K : 0, alpha, (const double *)h->A.p,
                      ar > 1 ? ar : 1, (const double *)h->B.p, br > 1 ? br : 1, beta,
-                     (double *)h->C.p, M > 1 ? M : 1, 1));
+                     (double *)h->C.p, M > 1 ? M : 1, blas_beta0));
     FLX_TRY(hipStreamSynchronize(h->stream));
     FLX_TRY(d2h(C, h->C.p, M, N, ldc));
     return FLX_SUCCESS;
@@ -1018,4 +1021,34 @@ extern "C" void dgetri_(const int *N, double *A, const int *LDA, const int *IPIV
         st = run();
     }
     report("dgetri_", st);
+}
+
+// Fortran BLAS seam, the convention of FLENS's own export
+// (flens/blas/interface/blas/gemm.cc:121-229): argument checks with info
+// 1, 2, 3, 4, 5, 8, 10, 13 through xerbla_, then blas::mm's generic order
+// (C scaled by beta first, so beta = 0 computes 0 * C).
+extern "C" void dgemm_(const char *TRANSA, const char *TRANSB, const int *M, const int *N,
+                       const int *K, const double *ALPHA, const double *A, const int *LDA,
+                       const double *B, const int *LDB, const double *BETA, double *C,
+                       const int *LDC)
+{
+    const char ta = up(TRANSA), tb = up(TRANSB);
+    const int m = *M, n = *N, k = *K;
+    const int nrowa = (ta == 'N') ? m : k, nrowb = (tb == 'N') ? k : n;
+    int info = 0;
+    if (ta != 'N' && ta != 'C' && ta != 'T') info = 1;
+    else if (tb != 'N' && tb != 'C' && tb != 'T') info = 2;
+    else if (m < 0) info = 3;
+    else if (n < 0) info = 4;
+    else if (k < 0) info = 5;
+    else if (*LDA < (nrowa > 1 ? nrowa : 1)) info = 8;
+    else if (*LDB < (nrowb > 1 ? nrowb : 1)) info = 10;
+    else if (*LDC < (m > 1 ? m : 1)) info = 13;
+    if (info) {
+        xerbla_("DGEMM ", &info, 6);
+        return;
+    }
+    if (m == 0 || n == 0) return;
+    report("dgemm_", host_gemm(ta != 'N', tb != 'N', m, n, k, *ALPHA, A, *LDA, B, *LDB, *BETA, C,
+                               *LDC, 0));
 }

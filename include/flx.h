@@ -350,6 +350,15 @@ void dsgesv_(const int *N, const int *NRHS, double *A, const int *LDA,
              int *IPIV, double *B, const int *LDB, double *X, const int *LDX,
              double *WORK, float *SWORK, int *ITER, int *INFO);
 
+/* Fortran BLAS GEMM with the conventions of FLENS's own export
+ * (flens/blas/interface/blas/gemm.cc:121-229): info 1, 2, 3, 4, 5, 8, 10, 13
+ * through xerbla_, C scaled by beta first (beta = 0 computes 0 * C, as the
+ * generic cxxblas::gemm, cxxblas/level3/gemm.tcc:175). */
+void dgemm_(const char *TRANSA, const char *TRANSB, const int *M, const int *N,
+            const int *K, const double *ALPHA, const double *A, const int *LDA,
+            const double *B, const int *LDB, const double *BETA, double *C,
+            const int *LDC);
+
 /* Fortran seams of the routines above (host pointers; WORK / IWORK are
  * accepted and unused except dgesvx_'s WORK(1) = reciprocal pivot growth):
  * cxxlapack/netlib/interface/lapack.in.h:8145-8151 (dlange), 6155-6166
