@@ -40,11 +40,13 @@ EXPORTS = [
     "flx_create", "flx_destroy", "flx_set_stream", "flx_sync", "flx_status_string",
     "flx_set_lu_blocking", "flx_set_lu_lookahead", "flx_dgemm", "flx_dgetrf", "flx_dgetrs", "flx_dgesv",
     "flx_dlaswp", "flx_fill_uniform", "flx_profile_start", "flx_profile_stop",
+    "flx_profile_classes",
     "flx_dgesv_mixed", "flx_dsgesv",
     "flx_batch_gesv_create", "flx_batch_gemm_create", "flx_batch_execute", "flx_batch_config",
     "flx_batch_destroy", "flx_mlp_forward", "flx_mlp_grad", "flx_mlp_adam",
     "flx_dlange", "flx_dgeequ", "flx_dlaqge", "flx_dgecon", "flx_dgerfs", "flx_dgesvx",
-    "flx_dgetri",
+    "flx_dgetri", "flx_save_gematrix", "flx_load_gematrix_header", "flx_load_gematrix",
+    "flx_save_densevector", "flx_load_densevector",
     "cblas_dgemm", "cblas_dtrsm", "dgemm_", "dgetrf_", "dgetrs_", "dgesv_", "dlaswp_", "dsgesv_",
     "dlange_", "dgeequ_", "dgecon_", "dgerfs_", "dgesvx_", "dgetri_",
     "xerbla_",
@@ -113,6 +115,11 @@ def lib():
     L.flx_dgesvx.argtypes = [_vp, _c, _i, _i, _i, _vp, _i, _vp, _i, _vp, _cp, _vp, _vp, _vp, _i,
                              _vp, _i, _dp, _dp, _dp, _dp, _ip]
     L.flx_dgetri.argtypes = [_vp, _i, _vp, _i, _vp, _ip]
+    L.flx_save_gematrix.argtypes = [_vp, _cp, _i, _i, _vp, _i, _i, _i]
+    L.flx_load_gematrix_header.argtypes = [_cp, _ip, _ip, _ip, _ip]
+    L.flx_load_gematrix.argtypes = [_vp, _cp, _vp, _i]
+    L.flx_save_densevector.argtypes = [_vp, _cp, _i, _vp, _i, _i]
+    L.flx_load_densevector.argtypes = [_vp, _cp, _vp, _i, _i, _ip, _ip]
     L.dgetri_.argtypes = [_ip, _dp, _ip, _ip, _dp, _ip, _ip]
     L.dlange_.argtypes = [_cp, _ip, _ip, _dp, _ip, _dp]
     L.dlange_.restype = _d
@@ -196,8 +203,16 @@ class Handle:
 
     PROF_CLASSES = ["gemm", "panel", "rowperm", "trsm_rm", "transpose", "trs", "other"]
 
-    def profile_start(self):
+    def profile_start(self, classes=None):
+        """Start the launch profiler; classes limits it to some kernel
+        classes (names of PROF_CLASSES)."""
         _check(lib().flx_profile_start(self._h), "flx_profile_start")
+        if classes is not None:
+            mask = 0
+            for c in classes:
+                mask |= 1 << self.PROF_CLASSES.index(c)
+            lib().flx_profile_classes.argtypes = [_vp, C.c_uint]
+            _check(lib().flx_profile_classes(self._h, mask), "flx_profile_classes")
 
     def profile_stop(self) -> dict:
         out = (C.c_double * (4 * len(self.PROF_CLASSES)))()
@@ -300,6 +315,61 @@ def _vec(t, n):
     if t is None:
         t = torch.zeros(max(1, n), dtype=torch.float64, device="cuda")
     return t
+
+
+class io:
+    """flens::save / flens::load binary files (IndexType int, first index 1
+    by default).  Device DeviceMatrix / torch vectors go through libflx
+    (row order produced on the device); numpy arrays are read and written
+    on the host in the same format."""
+
+    @staticmethod
+    def save(path: str, x, first: int = 1, handle: Handle | None = None) -> None:
+        if isinstance(x, np.ndarray):
+            with open(path, "wb") as f:
+                if x.ndim == 2:
+                    np.array([x.shape[0], x.shape[1], first, first], dtype=np.int32).tofile(f)
+                    np.ascontiguousarray(x, dtype=np.float64).tofile(f)   # row by row
+                else:
+                    np.array([x.shape[0], first], dtype=np.int32).tofile(f)
+                    x.tofile(f)
+            return
+        h = handle or default_handle()
+        if isinstance(x, DeviceMatrix):
+            _check(lib().flx_save_gematrix(h.ptr, path.encode(), x.m, x.n, _vp(x.ptr), x.ld, first,
+                                           first), "flx_save_gematrix")
+            return
+        import torch
+        elem = 4 if x.dtype == torch.int32 else 8
+        _check(lib().flx_save_densevector(h.ptr, path.encode(), x.numel(), _vp(x.data_ptr()), elem,
+                                          first), "flx_save_densevector")
+
+    @staticmethod
+    def load_numpy(path: str) -> np.ndarray:
+        """Host read of a matrix file: (m, n) float64, Fortran order."""
+        with open(path, "rb") as f:
+            hdr = np.fromfile(f, dtype=np.int32, count=4)
+            m, n = int(hdr[0]), int(hdr[1])
+            data = np.fromfile(f, dtype=np.float64, count=m * n)
+        return np.asfortranarray(data.reshape(m, n))
+
+    @staticmethod
+    def load_vector_numpy(path: str, dtype=np.float64) -> np.ndarray:
+        """Host read of a vector file (float64, or int32 for pivots)."""
+        with open(path, "rb") as f:
+            hdr = np.fromfile(f, dtype=np.int32, count=2)
+            return np.fromfile(f, dtype=dtype, count=int(hdr[0]))
+
+    @staticmethod
+    def load(path: str, handle: Handle | None = None) -> "DeviceMatrix":
+        """Device read of a matrix file into a new DeviceMatrix."""
+        h = handle or default_handle()
+        m, n = C.c_int(0), C.c_int(0)
+        _check(lib().flx_load_gematrix_header(path.encode(), C.byref(m), C.byref(n), None, None),
+               "flx_load_gematrix_header")
+        A = DeviceMatrix.empty(m.value, n.value)
+        _check(lib().flx_load_gematrix(h.ptr, path.encode(), _vp(A.ptr), A.ld), "flx_load_gematrix")
+        return A
 
 
 class lapack:

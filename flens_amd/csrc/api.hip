@@ -175,6 +175,13 @@ extern "C" int flx_profile_start(flx_handle h)
     return FLX_SUCCESS;
 }
 
+extern "C" int flx_profile_classes(flx_handle h, unsigned mask)
+{
+    if (!h || !h->prof) return FLX_INVALID_ARG;
+    prof_set_mask(h->prof, mask);
+    return FLX_SUCCESS;
+}
+
 extern "C" int flx_profile_stop(flx_handle h, double *out)
 {
     if (!h || !h->prof || !out) return FLX_INVALID_ARG;

@@ -32,6 +32,7 @@ void prof_delete(Profiler *p);
 void prof_reset(Profiler *p);
 void prof_collect(Profiler *p, double *out);
 // opens a record (returns nullptr when profiling is off)
+void prof_set_mask(Profiler *p, unsigned mask);
 void *prof_begin(Profiler *p, hipStream_t s, int cls, double flops, double bytes);
 void prof_end(Profiler *p, hipStream_t s, void *rec);
 

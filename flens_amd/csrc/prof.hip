@@ -13,6 +13,7 @@ struct ProfRec {
 };
 
 struct Profiler {
+    unsigned mask = ~0u;   // classes recorded (bit cls)
     std::vector<ProfRec> recs;
     std::vector<hipEvent_t> pool;
     size_t next = 0;
@@ -44,9 +45,11 @@ void prof_reset(Profiler *p)
     p->next = 0;
 }
 
+void prof_set_mask(Profiler *p, unsigned mask) { if (p) p->mask = mask; }
+
 void *prof_begin(Profiler *p, hipStream_t s, int cls, double flops, double bytes)
 {
-    if (!p) return nullptr;
+    if (!p || !((p->mask >> cls) & 1u)) return nullptr;
     ProfRec r;
     r.a = p->get();
     r.b = p->get();

@@ -79,6 +79,10 @@ enum flx_prof_class {
 };
 int flx_profile_start(flx_handle h);
 int flx_profile_stop(flx_handle h, double *out);
+/* After flx_profile_start: record only the classes whose bit (1 << class)
+ * is set in mask, so a timed region can carry events on one kernel class
+ * without the cost of events on every launch. */
+int flx_profile_classes(flx_handle h, unsigned mask);
 
 /* ------------------------------------------------- device-pointer entries */
 /* C <- beta*C + alpha*op(A)*op(B), trans = 0 NoTrans / 1 Trans.
@@ -234,6 +238,25 @@ int flx_dgesvx(flx_handle h, char fact, int trans, int n, int nrhs, double *A,
  * trtri flens/lapack/tr/tri.tcc); triangular inverses by recursive
  * doubling on DGEMM, inv(A) = inv(U) inv(L) P^T. */
 int flx_dgetri(flx_handle h, int n, double *A, int lda, const int *ipiv, int *info);
+
+/* ------------------------------------------------------------------- io */
+/* FLENS's binary files (flens::save / flens::load, IndexType int):
+ * GeMatrix flens/io/fullstorage/save.tcc:49-82 + load.tcc:52-80 -- int32
+ * numRows, numCols, firstRow, firstCol, then the elements row by row;
+ * DenseVector flens/io/array/save.tcc:43-66 + load.tcc:43-68 -- int32
+ * length, firstIndex, then the elements.  Matrices and vectors are on the
+ * device (column-major matrices); the row order is produced on the device.
+ * FLENS's default first index is 1. */
+int flx_save_gematrix(flx_handle h, const char *path, int m, int n, const double *A,
+                      int lda, int first_row, int first_col);
+int flx_load_gematrix_header(const char *path, int *m, int *n, int *first_row,
+                             int *first_col);
+int flx_load_gematrix(flx_handle h, const char *path, double *A, int lda);
+/* elem = 8 for double, 4 for int (pivot vectors) */
+int flx_save_densevector(flx_handle h, const char *path, int n, const void *x, int elem,
+                         int first_index);
+int flx_load_densevector(flx_handle h, const char *path, void *x, int capacity, int elem,
+                         int *n, int *first_index);
 
 /* ------------------------------------------------------ batched plans */
 /* Thousands of independent small problems in one persistent launch each

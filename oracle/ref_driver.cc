@@ -208,4 +208,30 @@ int ref_dgetri(int n, double *a, int lda, const int *ipiv)
     return lapack::tri(A, piv, work);
 }
 
+// flens::save / flens::load (flens/io/fullstorage/save.tcc:49-82,
+// load.tcc:52-80; flens/io/array/save.tcc:43-66)
+int ref_save_gematrix(const char *path, int m, int n, const double *a, int lda)
+{
+    DGeConstView A = DConstView(m, n, a, std::max(lda, 1));
+    return flens::save(std::string(path), A) ? 0 : 1;
+}
+
+int ref_load_gematrix(const char *path, int *m, int *n, double *out, int cap)
+{
+    GeMatrix<FullStorage<double, ColMajor> > A;
+    if (!flens::load(std::string(path), A)) return 1;
+    *m = A.numRows();
+    *n = A.numCols();
+    if ((long)A.numRows() * A.numCols() > cap) return 2;
+    for (int j = 1; j <= A.numCols(); ++j)
+        for (int i = 1; i <= A.numRows(); ++i) out[(i - 1) + (long)(j - 1) * A.numRows()] = A(i, j);
+    return 0;
+}
+
+int ref_save_ivector(const char *path, int n, const int *x)
+{
+    IView v = ArrayView<int>(n, const_cast<int *>(x), 1);
+    return flens::save(std::string(path), v) ? 0 : 1;
+}
+
 } // extern "C"
