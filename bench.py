@@ -249,6 +249,41 @@ def batched_section(count: int = 20000, seed: int = 4):
     return out
 
 
+def gemm_section(reps: int = 20, seed: int = 7):
+    """The DGEMM workload C = 1.5 A B - 0.5 C at M = N = K = 2048 (A, B, C
+    U[-1,1), column-major, ld = 2048) and an LU-shaped trailing update
+    C(7936 x 7936) -= A(7936 x 256) B(256 x 7936).  A 256 MiB device copy
+    runs first: it is the calibration dispatch for HBM-traffic counters."""
+    import torch
+    import flens_amd as F
+    h = F.default_handle()
+    L = F.lib()
+    cal = torch.empty(32 << 20, dtype=torch.float64, device="cuda")
+    cal2 = torch.empty_like(cal)
+    cal2.copy_(cal)
+    out = {}
+    for name, (m, n, k, alpha, beta) in {"dgemm2048": (2048, 2048, 2048, 1.5, -0.5),
+                                         "trailing7936x256": (7936, 7936, 256, -1.0, 1.0)}.items():
+        A, B, Cm = F.DeviceMatrix.empty(m, k), F.DeviceMatrix.empty(k, n), F.DeviceMatrix.empty(m, n)
+        for i, X in enumerate((A, B, Cm)):
+            assert L.flx_fill_uniform(h.ptr, F._vp(X.ptr), X.m, X.n, X.ld, seed + i, 0) == 0
+        F.blas.mm(0, 0, alpha, A, B, beta, Cm)
+        torch.cuda.synchronize()
+        t = time.perf_counter()
+        for _ in range(reps):
+            F.blas.mm(0, 0, alpha, A, B, beta, Cm)
+        torch.cuda.synchronize()
+        dt = (time.perf_counter() - t) / reps
+        fl = 2.0 * m * n * k
+        out[name] = {"m": m, "n": n, "k": k, "ms": 1e3 * dt, "tflops": fl / dt / 1e12,
+                     "frac_fp64_mfma": fl / dt / 1e12 / PEAK_FP64_MFMA,
+                     "algorithmic_GB": 8.0 * (m * k + k * n + 2.0 * m * n) / 1e9}
+    progress(f"gemm: 2048^3 {out['dgemm2048']['ms']:.3f} ms "
+             f"({out['dgemm2048']['tflops']:.1f} TF/s), trailing 7936^2x256 "
+             f"{out['trailing7936x256']['ms']:.3f} ms ({out['trailing7936x256']['tflops']:.1f} TF/s)")
+    return out
+
+
 def aux_section(n: int = 8192, seed: int = 5):
     """Norms, equilibration, condition estimate, refinement and the expert
     driver at n = 8192 (one right-hand side), A = U[-1,1) with rows scaled
@@ -370,14 +405,15 @@ def main():
     ap.add_argument("--quick", action="store_true",
                     help="only the FP64 LU workload (no mixed / batched / selector sections)")
     ap.add_argument("--mixed-n", type=int, default=16384)
-    ap.add_argument("--section", choices=["mixed", "batched", "selector", "aux"],
+    ap.add_argument("--section", choices=["mixed", "batched", "selector", "aux", "gemm"],
                     help="run only this section and print its JSON (for profiling)")
     args = ap.parse_args()
     if args.section:
         import torch
         assert torch.cuda.is_available(), "bench.py needs a GPU"
         fn = {"mixed": lambda: mixed_section(args.mixed_n), "batched": batched_section,
-              "selector": selector_section, "aux": aux_section}[args.section]
+              "selector": selector_section, "aux": aux_section,
+              "gemm": gemm_section}[args.section]
         print(json.dumps({args.section: fn()}))
         return
     if args.gpus != 1:
@@ -485,6 +521,7 @@ def main():
         out["batched"] = batched_section()
         out["selector"] = selector_section()
         out["aux"] = aux_section()
+        out["gemm"] = gemm_section()
     if not args.no_cpu:
         progress("CPU baseline: oracle getrf, first block step, 1 core")
         out["cpu_baseline"] = cpu_baseline(n, nrhs)

@@ -102,7 +102,7 @@ extern "C" int flx_destroy(flx_handle h)
     {
         AuxBufs &y = h->aux;
         for (DevBuf *b : {&y.slot, &y.v, &y.x, &y.r, &y.w, &y.lines, &y.isgn, &y.part, &y.ipiv1,
-                          &y.perm0})
+                          &y.perm0, &y.pf, &y.pb})
             free_buf(*b);
     }
     if (x.side) hipStreamDestroy(x.side);
@@ -275,7 +275,8 @@ extern "C" int flx_dgetrf(flx_handle h, int m, int n, double *A, int lda, int *i
 
 // NoTrans: X = U^{-1} L^{-1} P B ; Trans: X = P^T L^{-T} U^{-T} B
 int dgetrs_dev(flx_context *h, int trans, int n, int nrhs, const double *A, int lda,
-                      const int *ipiv, const int *perm_fwd, double *B, int ldb)
+                      const int *ipiv, const int *perm_fwd, double *B, int ldb,
+                      const int *perm_bwd)
 {
     const size_t bytes = sizeof(double) * (size_t)n * (size_t)nrhs;
     FLX_TRY(ensure(h, h->B, bytes));
@@ -301,9 +302,13 @@ int dgetrs_dev(flx_context *h, int trans, int n, int nrhs, const double *A, int 
         FLX_TRY(trsm_cm_f64(h->stream, 1, 1, 0, n, nrhs, A, lda, T1, n, T2, n, ti));
         FLX_TRY(trsm_cm_f64(h->stream, 0, 1, 1, n, nrhs, A, lda, T2, n, T1, n, ti));
         // apply the interchanges last to first: X = P^T Y
-        FLX_TRY(ensure(h, h->perm, sizeof(int) * (size_t)n));
-        FLX_TRY(ipiv_to_perm(h->stream, ipiv, (int *)h->perm.p, n, 1));
-        FLX_TRY(gather_rows_cm(h->stream, T1, n, B, ldb, (const int *)h->perm.p, n, nrhs));
+        const int *pb = perm_bwd;
+        if (!pb) {
+            FLX_TRY(ensure(h, h->perm, sizeof(int) * (size_t)n));
+            FLX_TRY(ipiv_to_perm(h->stream, ipiv, (int *)h->perm.p, n, 1));
+            pb = (const int *)h->perm.p;
+        }
+        FLX_TRY(gather_rows_cm(h->stream, T1, n, B, ldb, pb, n, nrhs));
     }
     return FLX_SUCCESS;
 }

@@ -26,7 +26,7 @@ struct MixBufs {
 
 // buffers of the norm / equilibration / condition / refinement routines
 struct AuxBufs {
-    DevBuf slot, v, x, r, w, lines, isgn, part, ipiv1, perm0;
+    DevBuf slot, v, x, r, w, lines, isgn, part, ipiv1, perm0, pf, pb;
     int ident_n = 0;   // length of the identity ipiv1 / perm0 held
 };
 
@@ -79,5 +79,8 @@ inline long round_ld(long n) { return ((n + 31) / 32) * 32; }
 // FP64 LU / solve on device column-major data (api.hip)
 int dgetrf_dev(flx_context *h, int m, int n, double *A, int lda, int *ipiv, int *info,
                bool keep_perm);
+// perm_fwd / perm_bwd: the row orders of ipiv applied forward / backward
+// (ipiv_to_perm), when the caller already holds them; else derived from ipiv
 int dgetrs_dev(flx_context *h, int trans, int n, int nrhs, const double *A, int lda,
-               const int *ipiv, const int *perm_fwd, double *B, int ldb);
+               const int *ipiv, const int *perm_fwd, double *B, int ldb,
+               const int *perm_bwd = nullptr);

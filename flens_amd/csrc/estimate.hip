@@ -813,7 +813,7 @@ struct Aux {
             if (st.kase == 0) break;
             const int tr = (st.kase == kase1) ? 0 : 1;
             FLX_RET(dgetrs_dev(h, tr, n, 1, LU, lda, (const int *)b.ipiv1.p,
-                               (const int *)b.perm0.p, x, n));
+                               (const int *)b.perm0.p, x, n, (const int *)b.perm0.p));
             int i;
             FLX_RET(iamax(x, n, &i));
             double xi;
@@ -862,6 +862,15 @@ struct Aux {
         double *r = (double *)b.r.p, *w = (double *)b.w.p, *v = (double *)b.v.p;
         int *sgn = (int *)b.isgn.p;
         const int g = grid1(n);
+        // the row orders of ipiv, once for all the solves below
+        FLX_TRY(ensure(h, b.pf, sizeof(int) * (size_t)n));
+        FLX_TRY(ensure(h, b.pb, sizeof(int) * (size_t)n));
+        const int *pf = (const int *)b.pf.p, *pb = (const int *)b.pb.p;
+        FLX_TRY(ipiv_to_perm(s, ipiv, (int *)b.pf.p, n, 0));
+        FLX_TRY(ipiv_to_perm(s, ipiv, (int *)b.pb.p, n, 1));
+        auto solve = [&](int t, double *y) -> int {
+            return dgetrs_dev(h, t, n, 1, AF, ldaf, ipiv, pf, y, n, pb);
+        };
         for (int j = 0; j < nrhs; ++j) {
             const double *bj = B + (size_t)j * ldb;
             double *xj = X + (size_t)j * ldx;
@@ -870,7 +879,7 @@ struct Aux {
             for (;;) {
                 FLX_RET(residual(trans, n, A, lda, xj, bj, r, w, safe1, safe2, &berr[j]));
                 if (berr[j] > eps && 2.0 * berr[j] <= lstres && count <= itmax) {
-                    FLX_RET(dgetrs_dev(h, trans, n, 1, AF, ldaf, ipiv, nullptr, r, n));
+                    FLX_RET(solve(trans, r));
                     hipLaunchKernelGGL(vadd_kernel, dim3(g), dim3(256), 0, s, xj,
                                        (const double *)r, n);
                     lstres = berr[j];
@@ -888,13 +897,13 @@ struct Aux {
                 FLX_RET(lacn2(st, n, v, r, sgn));
                 if (st.kase == 0) break;
                 if (st.kase == 1) {
-                    FLX_RET(dgetrs_dev(h, !trans, n, 1, AF, ldaf, ipiv, nullptr, r, n));
+                    FLX_RET(solve(!trans, r));
                     hipLaunchKernelGGL(vmul_kernel, dim3(g), dim3(256), 0, s, r,
                                        (const double *)w, n);
                 } else {
                     hipLaunchKernelGGL(vmul_kernel, dim3(g), dim3(256), 0, s, r,
                                        (const double *)w, n);
-                    FLX_RET(dgetrs_dev(h, trans, n, 1, AF, ldaf, ipiv, nullptr, r, n));
+                    FLX_RET(solve(trans, r));
                 }
             }
             ferr[j] = st.est;

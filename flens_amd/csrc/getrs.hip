@@ -27,19 +27,24 @@ __global__ void ipiv_to_perm_kernel(const int *__restrict__ ipiv, int *__restric
                                     int n, int reverse)
 {
     __shared__ int s[PERM_LDS];
+    __shared__ int sp[PERM_LDS];   // the pivots, staged so the serial loop reads LDS only
     const bool lds = n <= PERM_LDS;
     int *p = lds ? s : perm;
-    for (int i = threadIdx.x; i < n; i += blockDim.x) p[i] = i;
+    for (int i = threadIdx.x; i < n; i += blockDim.x) {
+        p[i] = i;
+        if (lds) sp[i] = ipiv[i];
+    }
     __syncthreads();
+    const int *pv = lds ? sp : ipiv;
     if (threadIdx.x == 0) {
         if (!reverse) {
             for (int i = 0; i < n; ++i) {
-                int q = ipiv[i] - 1;
+                int q = pv[i] - 1;
                 if (q != i && q >= 0 && q < n) { int t = p[i]; p[i] = p[q]; p[q] = t; }
             }
         } else {
             for (int i = n - 1; i >= 0; --i) {
-                int q = ipiv[i] - 1;
+                int q = pv[i] - 1;
                 if (q != i && q >= 0 && q < n) { int t = p[i]; p[i] = p[q]; p[q] = t; }
             }
         }
