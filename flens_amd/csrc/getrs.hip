@@ -293,7 +293,8 @@ FLX_DEV double tp_readlane(double v, int l)
     return __longlong_as_double(((long long)hi << 32) | (unsigned)lo);
 }
 
-template <int UPPER>
+// R right-hand sides per launch (1, 4 or 16), GRP dependency blocks per wait
+template <int UPPER, int R, int GRP>
 __global__ __launch_bounds__(256) void trs_persist_kernel(const double *__restrict__ LU, long lda,
                                                           int n, int nrhs,
                                                           const double *__restrict__ Bin, long ldb,
@@ -301,8 +302,8 @@ __global__ __launch_bounds__(256) void trs_persist_kernel(const double *__restri
                                                           uint64_t *__restrict__ xt,
                                                           unsigned epoch, int *__restrict__ err)
 {
-    __shared__ double s_x[2][TP_GRP][TP_BS][TP_RHS + 1];
-    __shared__ double s_red[4][TP_BS][TP_RHS + 1];
+    __shared__ double s_x[2][GRP][TP_BS][R + 1];
+    __shared__ double s_red[4][TP_BS][R + 1];
     __shared__ double s_d[TP_BS][TP_BS + 1];
     __shared__ int s_ok;
     const int nbk = (n + TP_BS - 1) / TP_BS;
@@ -318,15 +319,15 @@ __global__ __launch_bounds__(256) void trs_persist_kernel(const double *__restri
         const int c = q / TP_BS, r = q % TP_BS;
         s_d[r][c] = (r < rows && c < rows) ? LU[(long)(r0 + c) * lda + r0 + r] : (r == c ? 1.0 : 0.0);
     }
-    double acc[TP_RHS];
+    double acc[R];
 #pragma unroll
-    for (int k = 0; k < TP_RHS; ++k) acc[k] = 0.0;
+    for (int k = 0; k < R; ++k) acc[k] = 0.0;
     int par = 0, spins = 0;
-    for (int d0 = 0; d0 < ndep; d0 += TP_GRP, par ^= 1) {
-        const int cnt = min(TP_GRP, ndep - d0);
-        double lv[TP_GRP][16];
+    for (int d0 = 0; d0 < ndep; d0 += GRP, par ^= 1) {
+        const int cnt = min(GRP, ndep - d0);
+        double lv[GRP][16];
 #pragma unroll
-        for (int b = 0; b < TP_GRP; ++b) {
+        for (int b = 0; b < GRP; ++b) {
             const int d = d0 + b;
             const int jb = UPPER ? nbk - 1 - d : d;
             const int c0 = jb * TP_BS + wave * 16;
@@ -344,7 +345,7 @@ __global__ __launch_bounds__(256) void trs_persist_kernel(const double *__restri
                     for (int b = 0; b < cnt; ++b) {
                         const int d = d0 + b;
                         const int jb = UPPER ? nbk - 1 - d : d;
-                        const uint64_t *w = xt + ((long)(jb * TP_BS + TP_BS - 1) * TP_RHS + TP_RHS - 1) * 2;
+                        const uint64_t *w = xt + ((long)(jb * TP_BS + TP_BS - 1) * R + R - 1) * 2;
                         ready = ready && (unsigned)(__hip_atomic_load(w + 1, __ATOMIC_RELAXED,
                                                                       __HIP_MEMORY_SCOPE_AGENT) >> 32) == epoch;
                     }
@@ -358,15 +359,14 @@ __global__ __launch_bounds__(256) void trs_persist_kernel(const double *__restri
             const bool gave_up = s_ok;
             bool ok = true;
 #pragma unroll
-            for (int b = 0; b < TP_GRP; ++b) {
+            for (int b = 0; b < GRP; ++b) {
                 if (b < cnt) {
                     const int d = d0 + b;
                     const int jb = UPPER ? nbk - 1 - d : d;
 #pragma unroll
-                    for (int e = 0; e < 4; ++e) {
-                        const int idx = tid * 4 + e;          // 0 .. 1023 = row * 16 + rhs
-                        const int rr = idx >> 4, k = idx & 15;
-                        const uint64_t *w = xt + ((long)(jb * TP_BS + rr) * TP_RHS + k) * 2;
+                    for (int idx = tid; idx < TP_BS * R; idx += 256) {   // row * R + rhs
+                        const int rr = idx / R, k = idx % R;
+                        const uint64_t *w = xt + ((long)(jb * TP_BS + rr) * R + k) * 2;
                         const uint64_t hi = __hip_atomic_load(w, __ATOMIC_RELAXED,
                                                               __HIP_MEMORY_SCOPE_AGENT);
                         const uint64_t lo = __hip_atomic_load(w + 1, __ATOMIC_RELAXED,
@@ -385,24 +385,25 @@ __global__ __launch_bounds__(256) void trs_persist_kernel(const double *__restri
             }
         }
 #pragma unroll
-        for (int b = 0; b < TP_GRP; ++b)
+        for (int b = 0; b < GRP; ++b)
 #pragma unroll
             for (int c = 0; c < 16; ++c) {
                 const double l = lv[b][c];
 #pragma unroll
-                for (int k = 0; k < TP_RHS; ++k) acc[k] = fma(l, s_x[par][b][wave * 16 + c][k], acc[k]);
+                for (int k = 0; k < R; ++k) acc[k] = fma(l, s_x[par][b][wave * 16 + c][k], acc[k]);
             }
     }
 #pragma unroll
-    for (int k = 0; k < TP_RHS; ++k) s_red[wave][lane][k] = acc[k];
+    for (int k = 0; k < R; ++k) s_red[wave][lane][k] = acc[k];
     __syncthreads();
-    // right-hand side of this block, wave w: rhs 4w..4w+3
-    double v[4];
+    // right-hand sides of this block, wave w: KK of them from w * KK
+    constexpr int KK = (R + 3) / 4;
+    double v[KK];
 #pragma unroll
-    for (int kk = 0; kk < 4; ++kk) {
-        const int k = wave * 4 + kk;
+    for (int kk = 0; kk < KK; ++kk) {
+        const int k = wave * KK + kk;
         double t = 0.0;
-        if (row_ok && k < nrhs) {
+        if (row_ok && k < nrhs && k < R) {
             t = Bin[(long)k * ldb + r0 + lane];
 #pragma unroll
             for (int w = 0; w < 4; ++w) t -= s_red[w][lane][k];
@@ -416,7 +417,7 @@ __global__ __launch_bounds__(256) void trs_persist_kernel(const double *__restri
 #pragma unroll
         for (int c = 0; c < TP_BS; ++c) {
 #pragma unroll
-            for (int kk = 0; kk < 4; ++kk) {
+            for (int kk = 0; kk < KK; ++kk) {
                 const double xc = tp_readlane(v[kk], c);
                 if (lane > c) v[kk] = fma(-drow[c], xc, v[kk]);
             }
@@ -425,7 +426,7 @@ __global__ __launch_bounds__(256) void trs_persist_kernel(const double *__restri
 #pragma unroll
         for (int c = TP_BS - 1; c >= 0; --c) {
 #pragma unroll
-            for (int kk = 0; kk < 4; ++kk) {
+            for (int kk = 0; kk < KK; ++kk) {
                 if (lane == c) v[kk] = v[kk] / drow[c];
                 const double xc = tp_readlane(v[kk], c);
                 if (lane < c) v[kk] = fma(-drow[c], xc, v[kk]);
@@ -435,16 +436,17 @@ __global__ __launch_bounds__(256) void trs_persist_kernel(const double *__restri
     uint64_t wl = 0;
     uint64_t *wsent = nullptr;
 #pragma unroll
-    for (int kk = 0; kk < 4; ++kk) {
-        const int k = wave * 4 + kk;
+    for (int kk = 0; kk < KK; ++kk) {
+        const int k = wave * KK + kk;
+        if (k >= R) continue;                 // waves beyond the last right-hand side
         const double x = (row_ok && k < nrhs) ? v[kk] : 0.0;
         if (row_ok && k < nrhs) Xout[(long)k * ldx + r0 + lane] = x;
         const uint64_t b = (uint64_t)__double_as_longlong(x);
-        uint64_t *w = xt + ((long)(r0 + lane) * TP_RHS + k) * 2;
+        uint64_t *w = xt + ((long)(r0 + lane) * R + k) * 2;
         __hip_atomic_store(w, ((uint64_t)epoch << 32) | (b >> 32), __ATOMIC_RELAXED,
                            __HIP_MEMORY_SCOPE_AGENT);
         const uint64_t lo = ((uint64_t)epoch << 32) | (b & 0xffffffffull);
-        if (lane == TP_BS - 1 && k == TP_RHS - 1) { wsent = w + 1; wl = lo; }
+        if (lane == TP_BS - 1 && k == R - 1) { wsent = w + 1; wl = lo; }
         else __hip_atomic_store(w + 1, lo, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     }
     // the sentinel goes last, after every other word of the block is out
@@ -477,14 +479,25 @@ hipError_t trs_persist_f64(hipStream_t s, int n, int nrhs, const double *LU, lon
         }
         return *epoch;
     };
+    // 16 right-hand sides per pass; one or a few get a lighter variant (fewer
+    // words per finished block, more blocks per wait)
     for (int k0 = 0; k0 < nrhs; k0 += TP_RHS) {
         const int kc = min(TP_RHS, nrhs - k0);
         ProfScope ps(tl_prof, s, PROF_TRS, 2.0 * (double)n * n * kc,
                      8.0 * ((double)n * n + 4.0 * n * kc));
-        hipLaunchKernelGGL(trs_persist_kernel<0>, dim3(nbk), dim3(256), 0, s, LU, lda, n, kc,
-                           Bp + (long)k0 * ldbp, ldbp, T, (long)n, xt, next(), err);
-        hipLaunchKernelGGL(trs_persist_kernel<1>, dim3(nbk), dim3(256), 0, s, LU, lda, n, kc,
-                           (const double *)T, (long)n, X + (long)k0 * ldx, ldx, xt, next(), err);
+#define FLX_TP(R_, G_)                                                                           \
+        do {                                                                                     \
+            hipLaunchKernelGGL((trs_persist_kernel<0, R_, G_>), dim3(nbk), dim3(256), 0, s, LU,  \
+                               lda, n, kc, Bp + (long)k0 * ldbp, ldbp, T, (long)n, xt, next(),   \
+                               err);                                                             \
+            hipLaunchKernelGGL((trs_persist_kernel<1, R_, G_>), dim3(nbk), dim3(256), 0, s, LU,  \
+                               lda, n, kc, (const double *)T, (long)n, X + (long)k0 * ldx, ldx,  \
+                               xt, next(), err);                                                 \
+        } while (0)
+        if (kc == 1) FLX_TP(1, 4);
+        else if (kc <= 4) FLX_TP(4, 4);
+        else FLX_TP(16, TP_GRP);
+#undef FLX_TP
     }
     return hipGetLastError();
 }
