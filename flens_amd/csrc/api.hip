@@ -298,9 +298,12 @@ int dgetrs_dev(flx_context *h, int trans, int n, int nrhs, const double *A, int 
                                 (uint64_t *)h->trsw.p, &h->trs_epoch, h->wk.flags));
         (void)ti;
     } else {
-        FLX_TRY(copy_cm(h->stream, B, ldb, T1, n, n, nrhs));
-        FLX_TRY(trsm_cm_f64(h->stream, 1, 1, 0, n, nrhs, A, lda, T1, n, T2, n, ti));
-        FLX_TRY(trsm_cm_f64(h->stream, 0, 1, 1, n, nrhs, A, lda, T2, n, T1, n, ti));
+        // Y = L^{-T} U^{-T} B into T1, one launch per triangle
+        FLX_TRY(init_work(h));
+        FLX_TRY(ensure_zeroed(h, h->trsw, sizeof(uint64_t) * trs_persist_words(n)));
+        FLX_TRY(trs_persist_f64(h->stream, n, nrhs, A, lda, B, ldb, T2, T1, n,
+                                (uint64_t *)h->trsw.p, &h->trs_epoch, h->wk.flags, 1));
+        (void)ti;
         // apply the interchanges last to first: X = P^T Y
         const int *pb = perm_bwd;
         if (!pb) {

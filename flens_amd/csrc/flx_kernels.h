@@ -147,13 +147,14 @@ size_t trs_inv_doubles(int n);
 hipError_t trsm_cm_f64(hipStream_t s, int uplo_upper, int trans, int unit,
                        int n, int nrhs, const double *LU, long lda, double *B,
                        long ldb, double *X, long ldx, double *inv);
-// Single-launch-per-triangle NoTrans solve (see getrs.hip): X = U^{-1}
-// L^{-1} Bp with Bp already row-permuted; T is n x 16 scratch; xt holds
+// Single-launch-per-triangle solve (see getrs.hip): X = U^{-1} L^{-1} Bp with
+// Bp already row-permuted (trans = 0), or X = L^{-T} U^{-T} Bp (trans = 1,
+// the row order applied afterwards); T is n x 16 scratch; xt holds
 // trs_persist_words(n) tagged words (zeroed once), epoch is their host tag.
 size_t trs_persist_words(int n);
 hipError_t trs_persist_f64(hipStream_t s, int n, int nrhs, const double *LU, long lda,
                            const double *Bp, long ldbp, double *T, double *X, long ldx,
-                           uint64_t *xt, unsigned *epoch, int *err);
+                           uint64_t *xt, unsigned *epoch, int *err, int trans = 0);
 hipError_t laswp_cm_f64(hipStream_t s, double *A, long lda, int ncols, int k1,
                         int k2, const int *ipiv, int incx);
 

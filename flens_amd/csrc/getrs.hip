@@ -293,8 +293,11 @@ FLX_DEV double tp_readlane(double v, int l)
     return __longlong_as_double(((long long)hi << 32) | (unsigned)lo);
 }
 
-// R right-hand sides per launch (1, 4 or 16), GRP dependency blocks per wait
-template <int UPPER, int R, int GRP>
+// FWD: blocks in ascending order (a lower triangle); UNIT: unit diagonal;
+// TRANS: the triangle is read transposed (op = T^T: row i of the operand is
+// column i of the stored factor).  R right-hand sides per launch (1, 4 or
+// 16), GRP dependency blocks per wait.
+template <int FWD, int UNIT, int TRANS, int R, int GRP>
 __global__ __launch_bounds__(256) void trs_persist_kernel(const double *__restrict__ LU, long lda,
                                                           int n, int nrhs,
                                                           const double *__restrict__ Bin, long ldb,
@@ -308,16 +311,18 @@ __global__ __launch_bounds__(256) void trs_persist_kernel(const double *__restri
     __shared__ int s_ok;
     const int nbk = (n + TP_BS - 1) / TP_BS;
     const int g = blockIdx.x;
-    const int bi = UPPER ? nbk - 1 - g : g;
+    const int bi = FWD ? g : nbk - 1 - g;
     const int r0 = bi * TP_BS, rows = min(TP_BS, n - r0);
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-    const int ndep = UPPER ? nbk - 1 - bi : bi;
+    const int ndep = FWD ? bi : nbk - 1 - bi;
     const bool row_ok = lane < rows;
     const int row = r0 + (row_ok ? lane : 0);
     // the diagonal block, staged in LDS at the start (identity outside the matrix)
     for (int q = tid; q < TP_BS * TP_BS; q += 256) {
         const int c = q / TP_BS, r = q % TP_BS;
-        s_d[r][c] = (r < rows && c < rows) ? LU[(long)(r0 + c) * lda + r0 + r] : (r == c ? 1.0 : 0.0);
+        s_d[r][c] = (r < rows && c < rows)
+                        ? (TRANS ? LU[(long)(r0 + r) * lda + r0 + c] : LU[(long)(r0 + c) * lda + r0 + r])
+                        : (r == c ? 1.0 : 0.0);
     }
     double acc[R];
 #pragma unroll
@@ -329,11 +334,13 @@ __global__ __launch_bounds__(256) void trs_persist_kernel(const double *__restri
 #pragma unroll
         for (int b = 0; b < GRP; ++b) {
             const int d = d0 + b;
-            const int jb = UPPER ? nbk - 1 - d : d;
+            const int jb = FWD ? d : nbk - 1 - d;
             const int c0 = jb * TP_BS + wave * 16;
 #pragma unroll
             for (int c = 0; c < 16; ++c)
-                lv[b][c] = (b < cnt && row_ok && c0 + c < n) ? LU[(long)(c0 + c) * lda + row] : 0.0;
+                lv[b][c] = (b < cnt && row_ok && c0 + c < n)
+                               ? (TRANS ? LU[(long)row * lda + c0 + c] : LU[(long)(c0 + c) * lda + row])
+                               : 0.0;
         }
         // one thread waits for each block's sentinel word (written last by
         // its producer's thread), then every thread fetches 8 words per
@@ -344,7 +351,7 @@ __global__ __launch_bounds__(256) void trs_persist_kernel(const double *__restri
                     bool ready = true;
                     for (int b = 0; b < cnt; ++b) {
                         const int d = d0 + b;
-                        const int jb = UPPER ? nbk - 1 - d : d;
+                        const int jb = FWD ? d : nbk - 1 - d;
                         const uint64_t *w = xt + ((long)(jb * TP_BS + TP_BS - 1) * R + R - 1) * 2;
                         ready = ready && (unsigned)(__hip_atomic_load(w + 1, __ATOMIC_RELAXED,
                                                                       __HIP_MEMORY_SCOPE_AGENT) >> 32) == epoch;
@@ -362,7 +369,7 @@ __global__ __launch_bounds__(256) void trs_persist_kernel(const double *__restri
             for (int b = 0; b < GRP; ++b) {
                 if (b < cnt) {
                     const int d = d0 + b;
-                    const int jb = UPPER ? nbk - 1 - d : d;
+                    const int jb = FWD ? d : nbk - 1 - d;
 #pragma unroll
                     for (int idx = tid; idx < TP_BS * R; idx += 256) {   // row * R + rhs
                         const int rr = idx / R, k = idx % R;
@@ -413,11 +420,12 @@ __global__ __launch_bounds__(256) void trs_persist_kernel(const double *__restri
     double drow[TP_BS];
 #pragma unroll
     for (int c = 0; c < TP_BS; ++c) drow[c] = s_d[lane][c];
-    if (!UPPER) {
+    if (FWD) {
 #pragma unroll
         for (int c = 0; c < TP_BS; ++c) {
 #pragma unroll
             for (int kk = 0; kk < KK; ++kk) {
+                if (!UNIT && lane == c) v[kk] = v[kk] / drow[c];
                 const double xc = tp_readlane(v[kk], c);
                 if (lane > c) v[kk] = fma(-drow[c], xc, v[kk]);
             }
@@ -427,7 +435,7 @@ __global__ __launch_bounds__(256) void trs_persist_kernel(const double *__restri
         for (int c = TP_BS - 1; c >= 0; --c) {
 #pragma unroll
             for (int kk = 0; kk < KK; ++kk) {
-                if (lane == c) v[kk] = v[kk] / drow[c];
+                if (!UNIT && lane == c) v[kk] = v[kk] / drow[c];
                 const double xc = tp_readlane(v[kk], c);
                 if (lane < c) v[kk] = fma(-drow[c], xc, v[kk]);
             }
@@ -463,10 +471,12 @@ __global__ void zero_words_kernel(uint64_t *p, size_t n)
     if (i < n) p[i] = 0ull;
 }
 
-// X = U^{-1} L^{-1} Bp (Bp already row-permuted); T is n x 16 scratch.
+// trans = 0: X = U^{-1} L^{-1} Bp (Bp already row-permuted);
+// trans = 1: X = L^{-T} U^{-T} Bp (the caller applies P^T afterwards).
+// T is n x 16 scratch.
 hipError_t trs_persist_f64(hipStream_t s, int n, int nrhs, const double *LU, long lda,
                            const double *Bp, long ldbp, double *T, double *X, long ldx,
-                           uint64_t *xt, unsigned *epoch, int *err)
+                           uint64_t *xt, unsigned *epoch, int *err, int trans)
 {
     const int nbk = (n + TP_BS - 1) / TP_BS;
     if (nbk <= 0 || nrhs <= 0) return hipSuccess;
@@ -480,23 +490,31 @@ hipError_t trs_persist_f64(hipStream_t s, int n, int nrhs, const double *LU, lon
         return *epoch;
     };
     // 16 right-hand sides per pass; one or a few get a lighter variant (fewer
-    // words per finished block, more blocks per wait)
+    // words per finished block, more blocks per wait).  First triangle: L
+    // (forward, unit) or U^T (forward); second: U (backward) or L^T
+    // (backward, unit).
     for (int k0 = 0; k0 < nrhs; k0 += TP_RHS) {
         const int kc = min(TP_RHS, nrhs - k0);
         ProfScope ps(tl_prof, s, PROF_TRS, 2.0 * (double)n * n * kc,
                      8.0 * ((double)n * n + 4.0 * n * kc));
-#define FLX_TP(R_, G_)                                                                           \
+#define FLX_TP(U1_, U2_, TR_, R_, G_)                                                            \
         do {                                                                                     \
-            hipLaunchKernelGGL((trs_persist_kernel<0, R_, G_>), dim3(nbk), dim3(256), 0, s, LU,  \
-                               lda, n, kc, Bp + (long)k0 * ldbp, ldbp, T, (long)n, xt, next(),   \
-                               err);                                                             \
-            hipLaunchKernelGGL((trs_persist_kernel<1, R_, G_>), dim3(nbk), dim3(256), 0, s, LU,  \
-                               lda, n, kc, (const double *)T, (long)n, X + (long)k0 * ldx, ldx,  \
-                               xt, next(), err);                                                 \
+            hipLaunchKernelGGL((trs_persist_kernel<1, U1_, TR_, R_, G_>), dim3(nbk), dim3(256),  \
+                               0, s, LU, lda, n, kc, Bp + (long)k0 * ldbp, ldbp, T, (long)n, xt, \
+                               next(), err);                                                     \
+            hipLaunchKernelGGL((trs_persist_kernel<0, U2_, TR_, R_, G_>), dim3(nbk), dim3(256),  \
+                               0, s, LU, lda, n, kc, (const double *)T, (long)n,                 \
+                               X + (long)k0 * ldx, ldx, xt, next(), err);                        \
         } while (0)
-        if (kc == 1) FLX_TP(1, 4);
-        else if (kc <= 4) FLX_TP(4, 4);
-        else FLX_TP(16, TP_GRP);
+        if (!trans) {
+            if (kc == 1) FLX_TP(1, 0, 0, 1, 4);
+            else if (kc <= 4) FLX_TP(1, 0, 0, 4, 4);
+            else FLX_TP(1, 0, 0, 16, TP_GRP);
+        } else {
+            if (kc == 1) FLX_TP(0, 1, 1, 1, 4);
+            else if (kc <= 4) FLX_TP(0, 1, 1, 4, 4);
+            else FLX_TP(0, 1, 1, 16, TP_GRP);
+        }
 #undef FLX_TP
     }
     return hipGetLastError();
