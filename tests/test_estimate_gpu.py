@@ -259,3 +259,24 @@ def test_getri_closed_forms_and_singular(gpu):
     A = dev(gpu, lu)
     assert gpu.lapack.tri(A, torch.from_numpy(piv.astype(np.int32)).cuda()) == info == 41
     assert np.array_equal(A.numpy(), lu)                   # untouched
+
+
+def test_empty_and_zero_rhs_cases(gpu):
+    """Quick returns as in the reference: n = 0 and nrhs = 0."""
+    import torch
+    e = gpu.DeviceMatrix.empty(0, 0)
+    assert gpu.lapack.lan("O", e) == 0.0
+    r, c, rc, cc, am, info = gpu.lapack.equ(e)
+    assert (rc, cc, am, info) == (1.0, 1.0, 0.0, 0)
+    assert gpu.lapack.con("O", e, 1.0) == 1.0
+    assert gpu.lapack.tri(e, torch.zeros(1, dtype=torch.int32, device="cuda")) == 0
+    n = 5
+    a = rnd(n, n, 3)
+    lu = a.copy(order="F")
+    piv, _ = O.dgetrf(lu)
+    pd = torch.from_numpy(piv.astype(np.int32)).cuda()
+    B0 = gpu.DeviceMatrix.empty(n, 0)
+    X0 = gpu.DeviceMatrix.empty(n, 0)
+    f, b = gpu.lapack.rfs(0, dev(gpu, a), dev(gpu, lu), pd, B0, X0)
+    assert len(f) == 0 and len(b) == 0
+    assert gpu.lapack.con("O", dev(gpu, lu), 0.0) == 0.0      # anorm = 0 -> 0

@@ -341,3 +341,17 @@ def test_getri_vs_reference():
             assert same(x1, x2), n
         else:
             assert np.abs(x1 - x2).max() <= 64 * n * EPS * np.abs(x2).max(), n
+
+
+def test_empty_quick_returns():
+    e = np.zeros((0, 0), order="F")
+    assert O.dlange("F", e) == 0.0
+    r, c, rc, cc, am, info = O.dgeequ(e)
+    assert (rc, cc, am, info) == (1.0, 1.0, 0.0, 0)
+    assert O.dgecon("I", e, 1.0) == 1.0
+    assert O.dgetri(e, np.zeros(1, dtype=np.int32)) == 0
+    a = rnd(4, 4, 5)
+    lu = a.copy(order="F")
+    piv, _ = O.dgetrf(lu)
+    f, b = O.dgerfs(0, a, lu, piv, np.zeros((4, 0), order="F"), np.zeros((4, 0), order="F"))
+    assert len(f) == 0 and len(b) == 0
