@@ -13,6 +13,7 @@ struct DevBuf {
 struct MixBufs {
     DevBuf W32, perm, cand, ccount, counters, u11, win, list, zero, lpmax, la, ub;
     DevBuf inv, mv;                         // lookahead of the factor
+    DevBuf stage, stage_perm;               // rows that move in a panel step
     hipStream_t side = nullptr;             // default priority, ordered by events
     hipEvent_t ev_a = nullptr, ev_r = nullptr;
     DevBuf inv_l, inv_u, y32, y64, flags, err, dr, dc, part, scal, R, V, w, t, zeros, mpart, hbuf;

@@ -3,8 +3,25 @@
 
 #include <hip/hip_runtime.h>
 #include <stdint.h>
+#include <type_traits>
 
 #define FLX_DEV static __device__ __forceinline__
+
+// Compile-time loop: f(std::integral_constant<int, I>) for I = B .. E-1.
+// Bodies that index register arrays by the loop variable need it constant;
+// a loop the compiler leaves rolled indexes registers through
+// s_set_gpr_idx, one serialised move per access.
+template <int B, int E>
+struct StaticFor {
+    template <typename F>
+    __device__ __forceinline__ static void run(F &&f)
+    {
+        if constexpr (B < E) {
+            f(std::integral_constant<int, B>{});
+            StaticFor<B + 1, E>::run(f);
+        }
+    }
+};
 
 typedef double double4_t __attribute__((ext_vector_type(4)));
 typedef float float4_t __attribute__((ext_vector_type(4)));

@@ -166,7 +166,7 @@ struct MixWork {
     int *cand = nullptr;          // tslu_nodes(n) x 32 candidate positions
     int *ccount = nullptr;        // tslu_nodes(n)
     unsigned *counters = nullptr; // tslu_nodes(n), zero between launches
-    float *u11 = nullptr;         // 32 x 32
+    float *u11 = nullptr;         // MIX_U11_FLOATS
     int *win = nullptr;           // 32
     int *list = nullptr;          // PANEL_LIST_SLICE
     int *zero = nullptr;          // first zero pivot column (INT_MAX: none)
@@ -178,7 +178,16 @@ struct MixWork {
     hipStream_t side = nullptr;
     hipEvent_t ev_a = nullptr, ev_r = nullptr;
     int side_groups = 0;
+    // with lookahead: the rows a panel step moves (MIX_STAGE_ROWS x
+    // MIX_STAGE_LD floats) and their perm entries (MIX_STAGE_ROWS ints);
+    // null: the panel step runs as separate finish / interchange / update
+    float *stage = nullptr;
+    int *stage_perm = nullptr;
 };
+constexpr int MIX_STAGE_ROWS = 64, MIX_STAGE_LD = 256;
+// u11: L11\U11 (32 x 32), the strict triangles of U11 and L11 shifted
+// (32 x 32 each), 1 / diag(U11)
+constexpr int MIX_U11_FLOATS = 3 * 32 * 32 + 32;
 size_t tslu_nodes(int mp);
 size_t lp_pack_elems(int n, int nb);
 // Factor the n x n row-major FP32 W in place (tournament-pivoted panels,

@@ -39,12 +39,18 @@ constexpr int TS_B = 32;     // panel width (max)
 constexpr int TS_R = 256;    // rows per leaf = threads per workgroup
 constexpr int TS_F = 8;      // tree fan-in: 8 children x 32 candidates = 256 rows
 
-// Selection key: larger |v| first, then the lower panel position; 0 = none.
-FLX_DEV uint64_t ts_key(float v, int pos)
-{
-    const uint32_t a = __float_as_uint(fabsf(v)) + 1u;
-    return ((uint64_t)a << 32) | (uint64_t)(~(uint32_t)pos);
-}
+constexpr int TS_SLD = 256;        // staged row length (floats): the widest block
+constexpr int TS_SLOTS = 2 * TS_B; // staged rows: winners, then displaced top rows
+static_assert(TS_SLD == MIX_STAGE_LD && TS_SLOTS == MIX_STAGE_ROWS, "stage layout");
+// u11 buffer: L11\U11 (32 x 32); U11 right of the diagonal, row c shifted
+// to start at column c + 1: uus[c][i] = U(c, c + 1 + i); L11 below the
+// diagonal by columns, likewise: lts[k][i] = L(k + 1 + i, k) (both zero
+// outside the triangle); 1 / diag(U11) (32)
+constexpr int TS_UU = TS_B * TS_B, TS_LT = TS_UU + TS_B * TS_B, TS_DINV = TS_LT + TS_B * TS_B;
+constexpr int TS_U11 = TS_DINV + TS_B;
+static_assert(TS_U11 == MIX_U11_FLOATS, "u11 layout");
+static_assert(TS_B % 32 == 0 && MIX_STAGE_LD % 32 == 0, "float4 chunks of the stage");
+static_assert(TS_R == 256, "the selection key holds the thread index in 8 bits");
 
 FLX_DEV uint64_t wave_max_u64(uint64_t v)
 {
@@ -65,6 +71,7 @@ struct TsShared {
     float4 row[2][4][TS_B / 4];   // per-wave candidate rows, double-buffered
     uint32_t key[2][4];
     int win[TS_B];                // selected positions in order
+    float u[TS_B][TS_B + 1];      // root: the selected rows' L\U entries
 };
 
 FLX_DEV uint32_t ts_key32(float v, int t)
@@ -75,19 +82,18 @@ FLX_DEV uint32_t ts_key32(float v, int t)
 }
 
 // Partial-pivoting elimination over the workgroup's rows (one per thread,
-// fully unrolled column loop so every register index is static, one
-// barrier per column).  Selects up to w rows into sh.win (positions, in
-// order) and returns how many.  When u11 is given (the root) every
-// selected row's eliminated values -- L multipliers left of the diagonal,
-// U entries from it -- are written to u11 in selection order, and the
-// first exactly-zero pivot column (zbase + j) goes to zero_col.
+// the column loop unrolled at compile time so every register index is a
+// constant, one barrier per column).  Selects up to w rows into sh.win
+// (positions, in order) and returns how many.  When u11 is given (the root)
+// every selected row's eliminated values -- L multipliers left of the
+// diagonal, U entries from it -- are written to u11 in selection order.
 FLX_DEV int ts_select(float (&r)[TS_B], bool act, int pos, int w, TsShared &sh,
-                      float *__restrict__ u11, int *zero_col, int zbase)
+                      float *__restrict__ u11)
 {
     const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
     int nsel = 0, sel = -1;
-#pragma unroll
-    for (int j = 0; j < TS_B; ++j) {
+    StaticFor<0, TS_B>::run([&](auto jc) {
+        constexpr int j = decltype(jc)::value;
         if (j < w) {
             const int par = j & 1;
             const uint32_t key = act ? ts_key32(r[j], t) : 0u;
@@ -118,7 +124,6 @@ FLX_DEV int ts_select(float (&r)[TS_B], bool act, int pos, int w, TsShared &sh,
                     act = false;
                     sel = j;
                     sh.win[j] = pos;
-                    if (zero_col && pr[j] == 0.0f) atomicMin(zero_col, zbase + j);
                 }
                 nsel = j + 1;
                 if (act) {
@@ -130,12 +135,14 @@ FLX_DEV int ts_select(float (&r)[TS_B], bool act, int pos, int w, TsShared &sh,
                 }
             }
         }
-    }
+    });
     if (u11 && sel >= 0) {
         float4 *dst = reinterpret_cast<float4 *>(u11 + sel * TS_B);
 #pragma unroll
         for (int c = 0; c < TS_B / 4; ++c)
             dst[c] = make_float4(r[4 * c], r[4 * c + 1], r[4 * c + 2], r[4 * c + 3]);
+#pragma unroll
+        for (int c = 0; c < TS_B; ++c) sh.u[sel][c] = r[c];
     }
     __syncthreads();
     return nsel;
@@ -173,9 +180,20 @@ FLX_DEV void ts_load_row(float (&r)[TS_B], const float *__restrict__ src, bool o
 }
 
 // P: panel top-left in W (row-major, ld ldw), mp rows x w (<= 32) columns.
-// Outputs: u11 (32 x 32, selection order), win (selected positions,
-// relative to the panel top), list (one interchange slice, absolute rows
-// r0 + position), zero (first zero-pivot column, INT_MAX if none).
+// Outputs: u11 (TS_U11 floats: the 32 x 32 L11\U11 in selection order,
+// then U11's rows and L11's columns each padded with zeros to 64 entries,
+// then the 32 reciprocals of U11's diagonal, 0 for a zero pivot), win
+// (selected positions, relative to the panel top), list (one interchange
+// slice, absolute rows r0 + position), zero (first zero-pivot column,
+// INT_MAX if none).
+// With stage given, the root also copies the rows that move -- winner k to
+// slot k, the k-th displaced top row to slot 32 + k -- over the jb columns of
+// the outer block (Wb: the block's first column in the panel's top row,
+// row stride ldw) into stage (TS_SLD floats per slot), and their entries of
+// perm (absolute positions) into stage_perm, so the interchange can be
+// applied in place by a later kernel whose workgroups run in any order.
+// One launch per panel: the last workgroup to finish a tree node's
+// children (an atomic arrival count, no spinning) climbs to that node.
 __global__ __launch_bounds__(TS_R) void tslu_kernel(const float *__restrict__ P, long ldw, int mp,
                                                     int w, int r0, TsTree tree,
                                                     int *__restrict__ cand,
@@ -184,12 +202,16 @@ __global__ __launch_bounds__(TS_R) void tslu_kernel(const float *__restrict__ P,
                                                     float *__restrict__ u11,
                                                     int *__restrict__ win,
                                                     int *__restrict__ list,
-                                                    int *__restrict__ zero)
+                                                    int *__restrict__ zero,
+                                                    const float *__restrict__ Wb, int jb,
+                                                    float *__restrict__ stage,
+                                                    const int *__restrict__ perm,
+                                                    int *__restrict__ stage_perm)
 {
     critical_path_priority();
     __shared__ TsShared sh;
     __shared__ int s_last;
-    __shared__ int s_D[TS_B], s_V[TS_B];
+    __shared__ int s_D[TS_B], s_V[TS_B], s_n[2];
     const int t = threadIdx.x;
     int node = blockIdx.x;     // index within the level
     int lev = 0;
@@ -200,7 +222,7 @@ __global__ __launch_bounds__(TS_R) void tslu_kernel(const float *__restrict__ P,
     int nsel;
     for (;;) {
         const bool root = (lev == tree.nlev - 1);
-        nsel = ts_select(r, act, pos, w, sh, root ? u11 : nullptr, root ? zero : nullptr, r0);
+        nsel = ts_select(r, act, pos, w, sh, root ? u11 : nullptr);
         if (root) break;
         const int id = tree.off[lev] + node;
         if (t < nsel) st_agent_i(&cand[(long)id * TS_B + t], sh.win[t]);
@@ -232,9 +254,27 @@ __global__ __launch_bounds__(TS_R) void tslu_kernel(const float *__restrict__ P,
         node = parent;
         ++lev;
     }
-    // root: publish the winners and the interchange list (wave 0).  Rows
-    // k < nsel receive winner k; the displaced top rows (not winners) fill
-    // the winners' vacated slots (>= nsel), both in ascending order.
+    // root.  u11's rows were written by their threads in ts_select, and
+    // copied to sh.u (its closing barrier orders them before the reads
+    // below); the rows >= nsel and the shifted triangles follow.
+#pragma unroll
+    for (int q = 0; q < TS_B * TS_B / TS_R; ++q) {
+        const int e = t + q * TS_R;
+        const int k = e / TS_B, i = e % TS_B;
+        if (k >= nsel) u11[e] = 0.0f;
+        const int c = k + 1 + i;
+        u11[TS_UU + e] = (k < nsel && c < nsel) ? sh.u[k][c] : 0.0f;
+        u11[TS_LT + e] = (c < nsel) ? sh.u[c][k] : 0.0f;
+    }
+    if (t < TS_B) {
+        // pivot reciprocals; the first exactly-zero pivot column
+        const float d = (t < nsel) ? sh.u[t][t] : 1.0f;
+        u11[TS_DINV + t] = (t < nsel && d != 0.0f) ? 1.0f / d : 0.0f;
+        if (d == 0.0f) atomicMin(zero, r0 + t);
+    }
+    // the interchange list (wave 0).  Rows k < nsel receive winner k; the
+    // displaced top rows (not winners) fill the winners' vacated slots
+    // (>= nsel), both in ascending order.
     if (t < 64) {
         const int k = t;
         const int wkp = (k < nsel) ? sh.win[k] : -1;
@@ -258,17 +298,267 @@ __global__ __launch_bounds__(TS_R) void tslu_kernel(const float *__restrict__ P,
         }
         if (inD) s_D[rankD] = k;
         if (inV) s_V[rankV] = wkp;
-        if (k == 0) { sh.key[0][0] = __popcll(b1); sh.key[0][1] = __popcll(bD); }
+        if (k == 0) { s_n[0] = __popcll(b1); s_n[1] = __popcll(bD); }
+    }
+    __syncthreads();
+    const int n1 = s_n[0], nD = s_n[1];
+    if (t < nD) {
+        list[1 + 2 * (n1 + t)] = r0 + s_V[t];
+        list[2 + 2 * (n1 + t)] = r0 + s_D[t];
+    }
+    if (t == 0) list[0] = n1 + nD;
+    if (!stage) return;
+    // staging: 4 threads per slot, each a quarter of the row
+    const int slot = t >> 2, part = t & 3;
+    int src = -1;
+    if (slot < nsel) src = sh.win[slot];
+    else if (slot >= TS_B && slot - TS_B < nD) src = s_D[slot - TS_B];
+    if (src < 0) return;
+    const float *from = Wb + (long)src * ldw;
+    float *to = stage + (long)slot * TS_SLD;
+    // all loads first (unconditional, clamped in bounds), then the stores:
+    // one memory latency, not one per chunk
+    if ((jb & 3) == 0) {
+        float4 v[TS_SLD / 16];
+#pragma unroll
+        for (int q = 0; q < TS_SLD / 16; ++q) {
+            const int c = 4 * part + 16 * q;
+            v[q] = *reinterpret_cast<const float4 *>(from + (c < jb ? c : 0));
+        }
+#pragma unroll
+        for (int q = 0; q < TS_SLD / 16; ++q) {
+            const int c = 4 * part + 16 * q;
+            if (c < jb) *reinterpret_cast<float4 *>(to + c) = v[q];
+        }
+    } else {
+        float v[TS_SLD / 4];
+#pragma unroll
+        for (int q = 0; q < TS_SLD / 4; ++q) {
+            const int c = part + 4 * q;
+            v[q] = from[c < jb ? c : 0];
+        }
+#pragma unroll
+        for (int q = 0; q < TS_SLD / 4; ++q)
+            if (part + 4 * q < jb) to[part + 4 * q] = v[q];
+    }
+    if (part == 0) stage_perm[slot] = perm[r0 + src];
+}
+
+// The rest of a panel step in one launch, once tslu_kernel has chosen the
+// pivots and staged the moving rows (block columns [J, J + jb), panel at
+// columns jo .. jo + w of the block, rows r0 .. r0 + mp):
+//   U12 = L11^-1 (the winners' columns right of the panel)   (each workgroup)
+//   row p < nsel:  L11\U11 row p | U12 row p, left part from the stage
+//   other rows:    x = a U11^-1 (the L21 row), a22 -= x U12; a row that moves
+//                  in (a winner's vacated slot) takes its data from the stage
+// and perm follows the rows.  Workgroups write only their own rows and read
+// other rows only from the stage, so they need no ordering among themselves.
+// AP_ROWS rows per workgroup, 256 / AP_ROWS threads per row.
+constexpr int AP_ROWS = 32;
+constexpr int AP_TPR = 256 / AP_ROWS;
+constexpr int AP_CR = TS_SLD - TS_B;   // widest right part
+
+__global__ __launch_bounds__(256) void panel_apply_kernel(float *__restrict__ Wb, long ldw, int mp,
+                                                          int jo, int w, int jb,
+                                                          const float *__restrict__ u11,
+                                                          const int *__restrict__ list, int r0,
+                                                          const float *__restrict__ stage,
+                                                          const int *__restrict__ stage_perm,
+                                                          int *__restrict__ perm)
+{
+    critical_path_priority();
+    __shared__ float4 s_tri[2][TS_B][TS_B / 4];  // uus, lts (shifted triangles)
+    __shared__ float s_di[TS_B];                 // 1 / diag(U11)
+    __shared__ float4 s12[TS_B][AP_CR / 4];      // U12
+    __shared__ float sx[AP_ROWS][TS_B + 4];      // L21 rows of the tile
+    __shared__ int s_slot[AP_ROWS];
+    const int t = threadIdx.x;
+    const int nsel = min(w, mp);
+    const int cr = jb - jo - w;                  // columns right of the panel
+    const int p0 = blockIdx.x * AP_ROWS;
+    // the triangles' coefficients into LDS (every thread one or two loads)
+    {
+        const float4 *g = reinterpret_cast<const float4 *>(u11 + TS_UU);
+        float4 *l = &s_tri[0][0][0];
+#pragma unroll
+        for (int q = 0; q < 2 * TS_B * TS_B / 4 / 256; ++q) l[t + 256 * q] = g[t + 256 * q];
+        if (t < TS_B) s_di[t] = u11[TS_DINV + t];
     }
     __syncthreads();
     if (t < 64) {
-        const int k = t;
-        const int n1 = (int)sh.key[0][0], nD = (int)sh.key[0][1];
-        if (k < nD) {
-            list[1 + 2 * (n1 + k)] = r0 + s_V[k];
-            list[2 + 2 * (n1 + k)] = r0 + s_D[k];
+        // wave 0: which rows of the tile move in, then the panel columns
+        if (t < AP_ROWS) s_slot[t] = (p0 + t < nsel) ? p0 + t : -1;
+        __builtin_amdgcn_wave_barrier();
+        // rows moving in below the winners: the list's entries past the
+        // winners' own, in order, are the displaced top rows (stage 32 + d)
+        const int cnt = list[0];
+        const int dl = (t < cnt) ? list[1 + 2 * t] - r0 : -1;
+        const bool in = dl >= nsel;
+        const uint64_t m = __ballot(in);
+        const int d = __popcll(m & ((1ull << t) - 1ull));
+        if (in && dl >= p0 && dl < p0 + AP_ROWS) s_slot[dl - p0] = TS_B + d;
+        __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        // one thread per row
+        const int p = p0 + t;
+        if (t < AP_ROWS && p < mp) {
+            float *dst = Wb + (long)p * ldw + jo;
+            const int slot = s_slot[t];
+            if (p < nsel) {
+                float v[TS_B];
+#pragma unroll
+                for (int c = 0; c < TS_B; ++c) v[c] = u11[p * TS_B + c];
+#pragma unroll
+                for (int c = 0; c < TS_B; ++c)
+                    if (c < w) dst[c] = v[c];
+            } else {
+                const float *src = (slot >= 0) ? stage + (long)slot * TS_SLD + jo : dst;
+                float a[TS_B];
+#pragma unroll
+                for (int c = 0; c < TS_B; ++c) a[c] = (c < w) ? src[c] : 0.0f;
+                // x U11 = a, right-looking and rotating: at step c register 0
+                // holds a_c, finished as x_c, and x_c U(c, c + 1 + i) leaves
+                // the entries still open, which shift down by one, so all
+                // register indices are constants in a rolled loop (short
+                // code: this kernel runs once per panel on the critical path)
+#pragma unroll 2
+                for (int c = 0; c < TS_B; ++c) {
+                    float co[TS_B];
+#pragma unroll
+                    for (int q = 0; q < TS_B / 4; ++q) {
+                        const float4 v = s_tri[0][c][q];
+                        co[4 * q] = v.x; co[4 * q + 1] = v.y; co[4 * q + 2] = v.z; co[4 * q + 3] = v.w;
+                    }
+                    const float xc = a[0] * s_di[c];
+#pragma unroll
+                    for (int i = 0; i < TS_B - 1; ++i) a[i] = fmaf(-xc, co[i], a[i + 1]);
+                    a[TS_B - 1] = xc;
+                }
+#pragma unroll
+                for (int c = 0; c < TS_B; ++c) {
+                    sx[t][c] = (c < nsel) ? a[c] : 0.0f;
+                    if (c < w) dst[c] = a[c];
+                }
+            }
+            if (slot >= 0) perm[r0 + p] = stage_perm[slot];
         }
-        if (k == 0) list[0] = n1 + nD;
+    } else if (cr > 0) {
+        // waves 1-3, beside wave 0: U12 by forward substitution with the
+        // unit-lower L11, one column per thread (two for the first 32
+        // threads when more than 192 columns remain), rotating like x above
+        const int c1 = t - 64, c2 = c1 + 192;
+        const bool two = cr > 192 && t < 128;      // uniform in each wave
+        float x[TS_B], y[TS_B];
+#pragma unroll
+        for (int k = 0; k < TS_B; ++k)
+            x[k] = (k < nsel && c1 < cr) ? stage[k * TS_SLD + jo + w + c1] : 0.0f;
+        if (two) {
+#pragma unroll
+            for (int k = 0; k < TS_B; ++k)
+                y[k] = (k < nsel && c2 < cr) ? stage[k * TS_SLD + jo + w + c2] : 0.0f;
+#pragma unroll 2
+            for (int k = 0; k < TS_B; ++k) {
+                float co[TS_B];
+#pragma unroll
+                for (int c = 0; c < TS_B / 4; ++c) {
+                    const float4 v = s_tri[1][k][c];
+                    co[4 * c] = v.x; co[4 * c + 1] = v.y; co[4 * c + 2] = v.z; co[4 * c + 3] = v.w;
+                }
+                const float x0 = x[0], y0 = y[0];
+#pragma unroll
+                for (int i = 0; i < TS_B - 1; ++i) {
+                    x[i] = fmaf(-co[i], x0, x[i + 1]);
+                    y[i] = fmaf(-co[i], y0, y[i + 1]);
+                }
+                x[TS_B - 1] = x0;
+                y[TS_B - 1] = y0;
+            }
+        } else {
+#pragma unroll 2
+            for (int k = 0; k < TS_B; ++k) {
+                float co[TS_B];
+#pragma unroll
+                for (int c = 0; c < TS_B / 4; ++c) {
+                    const float4 v = s_tri[1][k][c];
+                    co[4 * c] = v.x; co[4 * c + 1] = v.y; co[4 * c + 2] = v.z; co[4 * c + 3] = v.w;
+                }
+                const float x0 = x[0];
+#pragma unroll
+                for (int i = 0; i < TS_B - 1; ++i) x[i] = fmaf(-co[i], x0, x[i + 1]);
+                x[TS_B - 1] = x0;
+            }
+        }
+        float *col = reinterpret_cast<float *>(&s12[0][0]);
+        if (c1 < cr)
+#pragma unroll
+            for (int k = 0; k < TS_B; ++k) col[k * AP_CR + c1] = x[k];
+        if (two && c2 < cr)
+#pragma unroll
+            for (int k = 0; k < TS_B; ++k) col[k * AP_CR + c2] = y[k];
+    }
+    __syncthreads();
+    // left part (rows that move) and the right part, AP_TPR threads per row;
+    // loads are issued together ahead of their use
+    const int i = t / AP_TPR, part = t % AP_TPR;
+    const int p = p0 + i;
+    if (p >= mp) return;
+    const int slot = s_slot[i];
+    float *row = Wb + (long)p * ldw;
+    if (slot >= 0 && jo > 0) {
+        // jo is a multiple of 32: float4 chunks
+        const float4 *from = reinterpret_cast<const float4 *>(stage + (long)slot * TS_SLD);
+        float4 *to = reinterpret_cast<float4 *>(row);
+        constexpr int NL = (TS_SLD - TS_B) / 4 / AP_TPR;
+        float4 v[NL];
+#pragma unroll
+        for (int q = 0; q < NL; ++q) {
+            const int c4 = part + AP_TPR * q;
+            v[q] = from[c4 < jo / 4 ? c4 : 0];
+        }
+#pragma unroll
+        for (int q = 0; q < NL; ++q)
+            if (part + AP_TPR * q < jo / 4) to[part + AP_TPR * q] = v[q];
+    }
+    if (cr <= 0) return;
+    float *rr = row + jo + w;
+    if (p < nsel) {
+        for (int c = part; c < cr; c += AP_TPR)
+            rr[c] = reinterpret_cast<const float *>(&s12[0][0])[p * AP_CR + c];
+        return;
+    }
+    const float *src = (slot >= 0) ? stage + (long)slot * TS_SLD + jo + w : rr;
+    float xr[TS_B];
+#pragma unroll
+    for (int k = 0; k < TS_B; ++k) xr[k] = sx[i][k];
+    if ((cr & 3) == 0 && (w & 3) == 0 && (ldw & 3) == 0) {
+        // rolled, with the next chunk's load issued before this chunk's FMAs
+        const int n4 = cr / 4;
+        int c4 = part;
+        float4 v = *reinterpret_cast<const float4 *>(src + 4 * (c4 < n4 ? c4 : 0));
+#pragma unroll 1
+        for (; c4 < n4; c4 += AP_TPR) {
+            const int nx = c4 + AP_TPR;
+            const float4 vn = *reinterpret_cast<const float4 *>(src + 4 * (nx < n4 ? nx : 0));
+#pragma unroll
+            for (int k = 0; k < TS_B; ++k) {
+                const float4 u = s12[k][c4];
+                v.x = fmaf(-xr[k], u.x, v.x);
+                v.y = fmaf(-xr[k], u.y, v.y);
+                v.z = fmaf(-xr[k], u.z, v.z);
+                v.w = fmaf(-xr[k], u.w, v.w);
+            }
+            *reinterpret_cast<float4 *>(rr + 4 * c4) = v;
+            v = vn;
+        }
+    } else {
+        const float *u12 = reinterpret_cast<const float *>(&s12[0][0]);
+#pragma unroll 1
+        for (int c = part; c < cr; c += AP_TPR) {
+            float v = src[c];
+#pragma unroll
+            for (int k = 0; k < TS_B; ++k) v = fmaf(-xr[k], u12[k * AP_CR + c], v);
+            rr[c] = v;
+        }
     }
 }
 
@@ -602,6 +892,8 @@ hipError_t getrf_rm_mixed(hipStream_t s, const MixWork &mw, float *W, long ldw, 
     // before the rest of the trailing update, which runs in prec on the side
     // stream beside the next block's panels
     const bool look = mw.inv && mw.mv && mw.side && mw.ev_a && mw.ev_r;
+    // the fused panel step moves only the block's columns, as the lookahead does
+    const bool fused = look && mw.stage && mw.stage_perm;
     FLX_CHECK(iota(s, mw.perm, n));
     bool rest_pending = false;
     for (int J = 0; J < n; J += nb) {
@@ -611,12 +903,33 @@ hipError_t getrf_rm_mixed(hipStream_t s, const MixWork &mw, float *W, long ldw, 
             const int w = (J + jb - j < ib) ? J + jb - j : ib;
             const int mp = n - j;
             const TsTree tree = tslu_tree(mp);
+            if (fused) {
+                // pivots, then the whole in-block step in one launch
+                float *Wb = W + (long)j * ldw + J;
+                {
+                    ProfScope ps(mw.prof, s, PROF_PANEL, 0.0, 4.0 * mp * w);
+                    hipLaunchKernelGGL(tslu_kernel, dim3(tree.count[0]), dim3(TS_R), 0, s,
+                                       (const float *)(W + (long)j * ldw + j), ldw, mp, w, j,
+                                       tree, mw.cand, mw.ccount, mw.counters, mw.u11, mw.win,
+                                       mw.list, mw.zero, (const float *)Wb, jb, mw.stage,
+                                       (const int *)mw.perm, mw.stage_perm);
+                }
+                ProfScope ps(mw.prof, s, PROF_TRSM_RM, 2.0 * (mp - w) * (double)w * (J + jb - j - w),
+                             8.0 * mp * (double)(J + jb - j));
+                hipLaunchKernelGGL(panel_apply_kernel, dim3((mp + AP_ROWS - 1) / AP_ROWS),
+                                   dim3(256), 0, s, Wb, ldw, mp, j - J, w, jb,
+                                   (const float *)mw.u11, (const int *)mw.list, j,
+                                   (const float *)mw.stage, (const int *)mw.stage_perm, mw.perm);
+                FLX_CHECK(hipGetLastError());
+                continue;
+            }
             {
                 ProfScope ps(mw.prof, s, PROF_PANEL, 0.0, 8.0 * mp * w);
                 hipLaunchKernelGGL(tslu_kernel, dim3(tree.count[0]), dim3(TS_R), 0, s,
                                    (const float *)(W + (long)j * ldw + j), ldw, mp, w, j, tree,
                                    mw.cand, mw.ccount, mw.counters, mw.u11, mw.win, mw.list,
-                                   mw.zero);
+                                   mw.zero, (const float *)nullptr, 0, (float *)nullptr,
+                                   (const int *)nullptr, (int *)nullptr);
                 hipLaunchKernelGGL(tslu_finish_kernel, dim3((mp + 255) / 256), dim3(256), 0, s,
                                    W + (long)j * ldw + j, ldw, mp, w, (const float *)mw.u11,
                                    (const int *)mw.win, w < mp ? w : mp);

@@ -83,13 +83,15 @@ struct Solver {
         FLX_TRY(ensure(h, m.cand, sizeof(int) * nodes * 32));
         FLX_TRY(ensure(h, m.ccount, sizeof(int) * nodes));
         FLX_TRY(ensure_zeroed(h, m.counters, sizeof(unsigned) * nodes));
-        FLX_TRY(ensure(h, m.u11, sizeof(float) * 32 * 32));
+        FLX_TRY(ensure(h, m.u11, sizeof(float) * MIX_U11_FLOATS));
         FLX_TRY(ensure(h, m.win, sizeof(int) * 32));
         FLX_TRY(ensure(h, m.list, sizeof(int) * PANEL_LIST_SLICE));
         FLX_TRY(ensure(h, m.zero, sizeof(int) * 4));
         FLX_TRY(ensure(h, m.lpmax, sizeof(unsigned) * 4));
         FLX_TRY(ensure(h, m.inv, sizeof(int) * n));
         FLX_TRY(ensure(h, m.mv, sizeof(int) * (1 + 2 * LU_BLOCK_MOVES)));
+        FLX_TRY(ensure(h, m.stage, sizeof(float) * MIX_STAGE_ROWS * MIX_STAGE_LD));
+        FLX_TRY(ensure(h, m.stage_perm, sizeof(int) * MIX_STAGE_ROWS));
         if (!m.side) {
             FLX_TRY(hipStreamCreateWithFlags(&m.side, hipStreamNonBlocking));
             FLX_TRY(hipEventCreateWithFlags(&m.ev_a, hipEventDisableTiming));
@@ -140,6 +142,8 @@ struct Solver {
         mw.ev_a = m.ev_a;
         mw.ev_r = m.ev_r;
         mw.side_groups = h->wk.side_groups;
+        mw.stage = look ? (float *)m.stage.p : nullptr;
+        mw.stage_perm = look ? (int *)m.stage_perm.p : nullptr;
         f.n = n;
         f.W = (const float *)m.W32.p;
         f.ldw = ldw;

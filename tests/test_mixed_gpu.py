@@ -36,12 +36,22 @@ def run_mixed(gpu, a, b, **kw):
     return X.numpy(), st, A.numpy()
 
 
-@pytest.mark.parametrize("n", [1, 7, 31, 32, 33, 100, 257, 520, 1000, 2100])
-def test_fp32_classic_converges_on_sizes(gpu, n):
-    # tree shapes: one leaf (<= 256 rows), several leaves, two tree levels (2100)
+@pytest.mark.parametrize("look", [1, 0])
+@pytest.mark.parametrize("n", [1, 7, 31, 32, 33, 100, 257, 300, 513, 520, 1000, 2100])
+def test_fp32_classic_converges_on_sizes(gpu, n, look):
+    # tree shapes: one leaf (<= 256 rows), several leaves, two tree levels
+    # (2100); outer blocks of 256 with a ragged last block (300: 44 columns,
+    # 513: one column).  look = 1: each panel step is one fused launch after
+    # the pivot tournament; look = 0: separate finish, interchange, solve
+    # and update launches.
     a = T.svd_matrix(n, 1e2, 10 + n)
     b, xt = T.rhs_from_solution(a, 1, n)
-    x, st, a_after = run_mixed(gpu, a, b, prec="fp32", refine="classic")
+    h = gpu.default_handle()
+    h.set_lu_lookahead(look)
+    try:
+        x, st, a_after = run_mixed(gpu, a, b, prec="fp32", refine="classic")
+    finally:
+        h.set_lu_lookahead(1)
     assert st["fallback"] == 0 and 0 <= st["iter"] <= 5, st
     assert backward_error(a, x, b) <= n * EPS, st
     assert np.array_equal(a_after, a)      # A untouched without fallback
