@@ -38,7 +38,7 @@ LIB_PATH = os.path.join(HERE, "libflx.so")
 # every symbol include/flx.h declares
 EXPORTS = [
     "flx_create", "flx_destroy", "flx_set_stream", "flx_sync", "flx_status_string",
-    "flx_set_lu_blocking", "flx_set_lu_lookahead", "flx_dgemm", "flx_dgetrf", "flx_dgetrs", "flx_dgesv",
+    "flx_set_lu_blocking", "flx_set_lu_lookahead", "flx_set_lu_side_groups", "flx_dgemm", "flx_dgetrf", "flx_dgetrs", "flx_dgesv",
     "flx_dlaswp", "flx_fill_uniform", "flx_profile_start", "flx_profile_stop",
     "flx_profile_classes",
     "flx_dgesv_mixed", "flx_dsgesv",
@@ -130,6 +130,7 @@ def lib():
     L.dgesvx_.argtypes = [_cp, _cp, _ip, _ip, _dp, _ip, _dp, _ip, _ip, _cp, _dp, _dp, _dp, _ip,
                           _dp, _ip, _dp, _dp, _dp, _dp, _ip, _ip]
     L.flx_set_lu_lookahead.argtypes = [_vp, _i]
+    L.flx_set_lu_side_groups.argtypes = [_vp, _i, _i]
     L.flx_dgemm.argtypes = [_vp, _i, _i, _i, _i, _i, _d, _vp, _i, _vp, _i, _d, _vp, _i]
     L.flx_dgetrf.argtypes = [_vp, _i, _i, _vp, _i, _vp, _vp]
     L.flx_dgetrs.argtypes = [_vp, _i, _i, _i, _vp, _i, _vp, _vp, _i]
@@ -197,6 +198,12 @@ class Handle:
         next block's columns are updated first and the rest of the update
         runs on a second stream beside the next block's panels."""
         _check(lib().flx_set_lu_lookahead(self._h, depth), "flx_set_lu_lookahead")
+
+    def set_lu_side_groups(self, groups: int, mixed: bool = False):
+        """Workgroups the lookahead's second stream may use in the FP64 LU
+        (or, mixed=True, in the mixed-precision factor); 0: the default."""
+        _check(lib().flx_set_lu_side_groups(self._h, int(bool(mixed)), groups),
+               "flx_set_lu_side_groups")
 
     def sync(self):
         _check(lib().flx_sync(self._h), "flx_sync")

@@ -148,6 +148,20 @@ extern "C" int flx_set_lu_lookahead(flx_handle h, int depth)
     return FLX_SUCCESS;
 }
 
+extern "C" int flx_set_lu_side_groups(flx_handle h, int which, int groups)
+{
+    if (!h || groups < 0 || (which != 0 && which != 1)) return FLX_INVALID_ARG;
+    FLX_TRY(init_work(h));
+    if (which == 1) {
+        h->mix_side_groups = groups;
+        return FLX_SUCCESS;
+    }
+    int cus = 0;
+    FLX_TRY(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, h->device));
+    h->wk.side_groups = (groups == 0) ? (cus > 8 ? cus - cus / 4 : cus) : groups;
+    return FLX_SUCCESS;
+}
+
 extern "C" int flx_sync(flx_handle h)
 {
     if (!h) return FLX_INVALID_ARG;

@@ -43,6 +43,7 @@ struct flx_context {
     DevBuf trsw;     // tagged solution words of the single-launch getrs
     DevBuf lu_inv, lu_mv;   // lookahead: inverse permutation, block moves
     unsigned trs_epoch = 0;
+    int mix_side_groups = 0;   // mixed factor's second-stream cap (0: none)
     MixBufs mix;
     AuxBufs aux;
     int last_error = 0;

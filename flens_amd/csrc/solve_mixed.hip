@@ -141,7 +141,7 @@ struct Solver {
         mw.side = m.side;
         mw.ev_a = m.ev_a;
         mw.ev_r = m.ev_r;
-        mw.side_groups = h->wk.side_groups;
+        mw.side_groups = h->mix_side_groups;
         mw.stage = look ? (float *)m.stage.p : nullptr;
         mw.stage_perm = look ? (int *)m.stage_perm.p : nullptr;
         f.n = n;

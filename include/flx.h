@@ -66,6 +66,14 @@ int flx_set_lu_blocking(flx_handle h, int nb, int ib);
  * blocked LU without lookahead). */
 int flx_set_lu_lookahead(flx_handle h, int depth);
 
+/* Workgroups the lookahead's second stream may occupy with the rest of a
+ * trailing update, so that the next block's panels keep compute units:
+ * which = 0 the FP64 LU (0: the default, three quarters of the compute
+ * units), which = 1 the mixed-precision factor (0: the default, no cap).
+ * Tuning only; results do not depend on it.  No counterpart in the
+ * reference. */
+int flx_set_lu_side_groups(flx_handle h, int which, int groups);
+
 /* Launch profiler (no counterpart in the reference, which has no timers:
  * flens/debug only logs calls).  Between start and stop every library
  * launch of a profiled kernel class is bracketed by HIP events on the
