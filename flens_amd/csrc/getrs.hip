@@ -420,12 +420,15 @@ __global__ __launch_bounds__(256) void trs_persist_kernel(const double *__restri
     double drow[TP_BS];
 #pragma unroll
     for (int c = 0; c < TP_BS; ++c) drow[c] = s_d[lane][c];
+    // the chain multiplies by the reciprocal of the row's diagonal entry
+    // (a division per step would sit on the critical path)
+    const double dinv = UNIT ? 1.0 : 1.0 / s_d[lane][lane];
     if (FWD) {
 #pragma unroll
         for (int c = 0; c < TP_BS; ++c) {
 #pragma unroll
             for (int kk = 0; kk < KK; ++kk) {
-                if (!UNIT && lane == c) v[kk] = v[kk] / drow[c];
+                if (!UNIT && lane == c) v[kk] = v[kk] * dinv;
                 const double xc = tp_readlane(v[kk], c);
                 if (lane > c) v[kk] = fma(-drow[c], xc, v[kk]);
             }
@@ -435,7 +438,7 @@ __global__ __launch_bounds__(256) void trs_persist_kernel(const double *__restri
         for (int c = TP_BS - 1; c >= 0; --c) {
 #pragma unroll
             for (int kk = 0; kk < KK; ++kk) {
-                if (!UNIT && lane == c) v[kk] = v[kk] / drow[c];
+                if (!UNIT && lane == c) v[kk] = v[kk] * dinv;
                 const double xc = tp_readlane(v[kk], c);
                 if (lane < c) v[kk] = fma(-drow[c], xc, v[kk]);
             }

@@ -276,14 +276,24 @@ __global__ __launch_bounds__(256) void trsv_rm_kernel(const float *__restrict__ 
     const bool row_ok = rr < rows;
     const float *wrow = W + (long)(r0 + (row_ok ? rr : 0)) * ldw;
     const TY rhs0 = row_ok ? y[r0 + rr] : TY(0);
-    // wave 0: row `lane` of the diagonal block (identity outside the matrix)
+    // wave 0: row `lane` of the diagonal block (identity outside the
+    // matrix) and, for U, the reciprocal of the row's diagonal entry, so the
+    // substitution chain multiplies instead of dividing (both ahead of the
+    // dependency wait, off the critical path)
     float drow[TV];
+    TY dinv = TY(1);
     if (wave == 0) {
         const bool lok = lane < rows;
         const float *dr = W + (long)(r0 + (lok ? lane : 0)) * ldw + r0;
 #pragma unroll
         for (int c = 0; c < TV; ++c)
             drow[c] = (lok && c < rows) ? dr[c] : (lane == c ? 1.0f : 0.0f);
+        if (UPPER) {
+            float dd = 1.0f;
+#pragma unroll
+            for (int c = 0; c < TV; ++c) dd = (lane == c) ? drow[c] : dd;
+            dinv = TY(1) / (TY)dd;
+        }
     }
     // dependency blocks in groups of TV_FETCH: the group's factor entries are
     // requested first, then wave 0 waits until every block of the group is
@@ -346,7 +356,7 @@ __global__ __launch_bounds__(256) void trsv_rm_kernel(const float *__restrict__ 
         } else {
 #pragma unroll
             for (int c = TV - 1; c >= 0; --c) {
-                if (lane == c) bv = bv / (TY)drow[c];
+                if (lane == c) bv = bv * dinv;
                 const TY xc = TW::readlane(bv, c);
                 if (lane < c) bv = bv - (TY)drow[c] * xc;
             }
