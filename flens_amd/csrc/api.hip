@@ -120,6 +120,7 @@ extern "C" int flx_destroy(flx_handle h)
     if (h->wk.ev_a) hipEventDestroy(h->wk.ev_a);
     if (h->wk.ev_r) hipEventDestroy(h->wk.ev_r);
     free_buf(h->lu_inv); free_buf(h->lu_mv);
+    free_buf(h->lu_stage); free_buf(h->lu_sdst); free_buf(h->lu_sperm);
     free_buf(h->sr); free_buf(h->sc);
     prof_delete(h->prof);
     delete h;
@@ -261,6 +262,12 @@ int dgetrf_dev(flx_context *h, int m, int n, double *A, int lda, int *ipiv, int 
     FLX_TRY(ensure(h, h->lu_mv, sizeof(int) * (1 + 2 * LU_BLOCK_MOVES)));
     h->wk.inv = (int *)h->lu_inv.p;
     h->wk.mv = (int *)h->lu_mv.p;
+    FLX_TRY(ensure(h, h->lu_stage, sizeof(double) * LU_STAGE_ROWS * LU_STAGE_LD));
+    FLX_TRY(ensure(h, h->lu_sdst, sizeof(int) * (LU_STAGE_TOP + 1)));
+    FLX_TRY(ensure(h, h->lu_sperm, sizeof(int) * LU_STAGE_TOP));
+    h->wk.stage = (double *)h->lu_stage.p;
+    h->wk.sdst = (int *)h->lu_sdst.p;
+    h->wk.sperm = (int *)h->lu_sperm.p;
     if (!h->wk.side) {
         FLX_TRY(hipStreamCreateWithFlags(&h->wk.side, hipStreamNonBlocking));
         FLX_TRY(hipEventCreateWithFlags(&h->wk.ev_a, hipEventDisableTiming));

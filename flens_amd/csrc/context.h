@@ -42,6 +42,7 @@ struct flx_context {
     DevBuf mlp_part; // per-block gradients of flx_mlp_grad
     DevBuf trsw;     // tagged solution words of the single-launch getrs
     DevBuf lu_inv, lu_mv;   // lookahead: inverse permutation, block moves
+    DevBuf lu_stage, lu_sdst, lu_sperm;   // fused panel step
     unsigned trs_epoch = 0;
     int mix_side_groups = 0;   // mixed factor's second-stream cap (0: none)
     MixBufs mix;

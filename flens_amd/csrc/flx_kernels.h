@@ -79,8 +79,15 @@ struct LuWork {
     // panels and the small launches of the next block always find free
     // compute units (a trailing update left uncapped fills the device)
     int side_groups = 0;
+    // fused panel step (lookahead): staged rows (LU_STAGE_ROWS x LU_STAGE_LD
+    // doubles), their destinations (+ the count at [LU_STAGE_TOP]) and perm
+    // entries; null: separate interchange / solve / update launches
+    double *stage = nullptr;
+    int *sdst = nullptr, *sperm = nullptr;
 };
 constexpr int LU_BLOCK_MOVES = 2 * 256;   // rows one outer block can move
+constexpr int LU_STAGE_TOP = 128;         // stage rows: list sources, then the top rows
+constexpr int LU_STAGE_ROWS = LU_STAGE_TOP + 32, LU_STAGE_LD = 256;
 
 struct LuOptions {
     int nb = 256;   // outer block (trailing update K)

@@ -134,6 +134,242 @@ __global__ __launch_bounds__(256) void perm_compose_kernel(const int *__restrict
 }
 
 // ---------------------------------------------------------------------------
+// Fused panel step of the blocked LU (lookahead path), after the
+// cooperative panel kernel has factored and interchanged the panel's own
+// columns j .. j+w of the outer block [J, J+jb):
+//   lu_stage_kernel (one workgroup): the interchange list's source rows
+//     (block columns) into stage rows 0.., the panel's top w rows as they
+//     will be after the interchange into stage rows LU_STAGE_TOP.., the
+//     moved rows' perm entries and destinations;
+//   lu_apply_kernel (32 rows per workgroup): the interchange of the block's
+//     other columns, U12 = L11^-1 (top rows' right part) -- laswp, the
+//     blas::sm of trf.tcc:136-138 -- and A22 -= L21 U12 (trf.tcc:140-145)
+//     for the block's columns right of the panel, in one launch.
+// A workgroup writes only its own rows and reads other rows only from the
+// stage (or the panel's columns, which it does not write), so workgroups
+// need no ordering among themselves.
+// ---------------------------------------------------------------------------
+constexpr int LA_ROWS = 32;                 // rows per apply workgroup
+constexpr int LA_TPR = 256 / LA_ROWS;       // threads per row
+constexpr int LA_CR = LU_STAGE_LD - 32;     // widest right part (block 256, panel >= 32)
+constexpr int LA_NQ = LA_ROWS * LA_CR / 256;   // right-part elements per thread
+static_assert(LU_STAGE_TOP == RP_MAXE, "stage layout");
+
+// The interchange list's slices gathered into LDS with every load issued at
+// once (load_list walks each slice serially): 256 threads x 8 slots cover
+// PANEL_GROUPS_MAX slices of up to 64 entries.  Returns the entry count.
+FLX_DEV int load_list_par(const int *__restrict__ list, int G, int m, int *s_dst, int *s_src,
+                          int *s_cnt)
+{
+    const int t = threadIdx.x;
+    static_assert(PANEL_GROUPS_MAX * 64 == 256 * 8, "slot map");
+    int d[8], sr[8];
+#pragma unroll
+    for (int q = 0; q < 8; ++q) {
+        const int slot = t + 256 * q, g = slot >> 6, e = slot & 63;
+        const int *sl = list + (size_t)(g < G ? g : 0) * PANEL_LIST_SLICE;
+        d[q] = sl[1 + 2 * e];
+        sr[q] = sl[2 + 2 * e];
+    }
+    if (t < PANEL_GROUPS_MAX) s_cnt[t] = (t < G) ? min(list[(size_t)t * PANEL_LIST_SLICE], 64) : 0;
+    __syncthreads();
+    if (t == 0) {
+        int acc = 0;
+        for (int g = 0; g < PANEL_GROUPS_MAX; ++g) {
+            const int c = s_cnt[g];
+            s_cnt[PANEL_GROUPS_MAX + 1 + g] = acc;   // offsets
+            acc += c;
+        }
+        s_cnt[PANEL_GROUPS_MAX] = min(acc, RP_MAXE);
+    }
+    __syncthreads();
+#pragma unroll
+    for (int q = 0; q < 8; ++q) {
+        const int slot = t + 256 * q, g = slot >> 6, e = slot & 63;
+        if (e < s_cnt[g]) {
+            const int o = s_cnt[PANEL_GROUPS_MAX + 1 + g] + e;
+            const bool ok = d[q] >= 0 && d[q] < m && sr[q] >= 0 && sr[q] < m;
+            if (o < RP_MAXE) {
+                s_dst[o] = ok ? d[q] : 0;   // an invalid entry degenerates to row 0 -> row 0
+                s_src[o] = ok ? sr[q] : 0;
+            }
+        }
+    }
+    __syncthreads();
+    return s_cnt[PANEL_GROUPS_MAX];
+}
+
+// One workgroup per stage row: blockIdx.x < LU_STAGE_TOP = list entry,
+// above = top row blockIdx.x - LU_STAGE_TOP.  Workgroup 0 also writes the
+// destinations (count at [LU_STAGE_TOP]) and perm entries.
+__global__ __launch_bounds__(256) void lu_stage_kernel(const double *__restrict__ W, long ldw,
+                                                       int m, int J, int jb, int j, int w,
+                                                       const int *__restrict__ list, int G,
+                                                       const int *__restrict__ perm,
+                                                       double *__restrict__ stage,
+                                                       int *__restrict__ sdst,
+                                                       int *__restrict__ sperm,
+                                                       unsigned long long *__restrict__ moved)
+{
+    critical_path_priority();
+    __shared__ int s_dst[RP_MAXE], s_src[RP_MAXE], s_cnt[2 * PANEL_GROUPS_MAX + 1];
+    __shared__ int s_row;
+    const int t = threadIdx.x, b = blockIdx.x;
+    const int cnt = load_list_par(list, G, m, s_dst, s_src, s_cnt);
+    if (b == 0) {
+        if (t < cnt) {
+            sdst[t] = s_dst[t];
+            sperm[t] = perm ? perm[s_src[t]] : 0;
+        }
+        if (t == 0) sdst[LU_STAGE_TOP] = cnt;
+        if (t == 0 && moved) atomicAdd(moved, (unsigned long long)cnt * (jb - w));
+    }
+    int src = -1;
+    if (b < LU_STAGE_TOP) {
+        if (b < cnt) src = s_src[b];
+    } else {
+        const int k = b - LU_STAGE_TOP;
+        if (k < w) {
+            // top row j + k: the source of the entry moving into it, else itself
+            if (t == 0) s_row = j + k;
+            __syncthreads();
+            if (t < cnt && s_dst[t] == j + k) s_row = s_src[t];
+            __syncthreads();
+            src = s_row;
+        }
+    }
+    if (src < 0 || t >= jb) return;
+    stage[(long)b * LU_STAGE_LD + t] = W[(long)src * ldw + J + t];
+}
+
+__global__ __launch_bounds__(256) void lu_apply_kernel(double *__restrict__ W, long ldw, int m,
+                                                       int J, int jb, int j, int w,
+                                                       const double *__restrict__ stage,
+                                                       const int *__restrict__ sdst,
+                                                       const int *__restrict__ sperm,
+                                                       int *__restrict__ perm)
+{
+    critical_path_priority();
+    __shared__ double s_l[32][33];               // L11 shifted: s_l[k][i] = L(k+1+i, k)
+    __shared__ double s12[32][LA_CR + 1];        // U12
+    __shared__ double s_a[LA_ROWS][LA_CR + 1];   // the tile's right parts
+    __shared__ double sx[LA_ROWS][33];           // the tile's L21 rows
+    __shared__ int s_slot[LA_ROWS];
+    const int t = threadIdx.x;
+    const int jo = j - J;
+    const int cr = jb - jo - w;                  // columns right of the panel
+    const int i0 = j + blockIdx.x * LA_ROWS;     // first row of the tile
+    const int cnt = sdst[LU_STAGE_TOP];
+    {
+        double lv[4];
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            const int e = t + 256 * q, k = e >> 5, i = e & 31, r = k + 1 + i;
+            lv[q] = (r < w) ? W[(long)(j + (r < 32 ? r : 0)) * ldw + j + k] : 0.0;
+        }
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            const int e = t + 256 * q;
+            s_l[e >> 5][e & 31] = lv[q];
+        }
+    }
+    if (t < LA_ROWS) s_slot[t] = -1;
+    __syncthreads();
+    if (t < cnt) {
+        const int d = sdst[t] - i0;
+        if (d >= 0 && d < LA_ROWS) s_slot[d] = t;
+    }
+    __syncthreads();
+    // the tile's L21 rows (panel columns, final) and right parts (moved rows
+    // from the stage), every load issued before any is used
+    {
+        double lx[4];
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            const int e = t + 256 * q, r = e >> 5, k = e & 31, i = i0 + r;
+            lx[q] = (i < m && k < w && i - j >= w) ? W[(long)i * ldw + j + k] : 0.0;
+        }
+        double av[LA_NQ];
+#pragma unroll
+        for (int q = 0; q < LA_NQ; ++q) {
+            const int e = t + 256 * q, r = e / LA_CR, c = e % LA_CR, i = i0 + r;
+            const int sl = s_slot[r];
+            const double *src = (sl >= 0) ? stage + (long)sl * LU_STAGE_LD + jo + w
+                                          : W + (long)(i < m ? i : j) * ldw + j + w;
+            av[q] = (c < cr && i < m) ? src[c] : 0.0;
+        }
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            const int e = t + 256 * q;
+            sx[e >> 5][e & 31] = lx[q];
+        }
+#pragma unroll
+        for (int q = 0; q < LA_NQ; ++q) {
+            const int e = t + 256 * q;
+            s_a[e / LA_CR][e % LA_CR] = av[q];
+        }
+    }
+    // U12 by forward substitution with the unit-lower L11, one column per
+    // thread; x rotates (register 0 is the finished entry of the step), so
+    // the rolled loop indexes registers by constants only
+    if (cr > 0 && t < cr) {
+        double x[32];
+#pragma unroll
+        for (int k = 0; k < 32; ++k)
+            x[k] = (k < w) ? stage[(long)(LU_STAGE_TOP + k) * LU_STAGE_LD + jo + w + t] : 0.0;
+#pragma unroll 1
+        for (int k = 0; k < 32; ++k) {
+            const double x0 = x[0];
+#pragma unroll
+            for (int i = 0; i < 31; ++i) x[i] = fma(-s_l[k][i], x0, x[i + 1]);
+            x[31] = x0;
+        }
+#pragma unroll
+        for (int k = 0; k < 32; ++k) s12[k][t] = x[k];
+    }
+    __syncthreads();
+    const int r = t / LA_TPR, part = t % LA_TPR;
+    const int i = i0 + r;
+    if (i >= m) return;
+    const int slot = s_slot[r];
+    double *row = W + (long)i * ldw + J;
+    if (slot >= 0) {
+        // the left part (a multiple of 32 columns) of a row that moves in
+        const double *src = stage + (long)slot * LU_STAGE_LD;
+        constexpr int NL = LA_CR / LA_TPR;
+        double lv[NL];
+#pragma unroll
+        for (int q = 0; q < NL; ++q) {
+            const int c = part + LA_TPR * q;
+            lv[q] = src[c < jo ? c : 0];
+        }
+#pragma unroll
+        for (int q = 0; q < NL; ++q) {
+            const int c = part + LA_TPR * q;
+            if (c < jo) row[c] = lv[q];
+        }
+        if (part == 0 && perm) perm[i] = sperm[slot];
+    }
+    if (cr <= 0) return;
+    double *rr = row + jo + w;
+    const int p = i - j;
+    if (p < w) {
+        for (int c = part; c < cr; c += LA_TPR) rr[c] = s12[p][c];
+        return;
+    }
+    double xr[32];
+#pragma unroll
+    for (int k = 0; k < 32; ++k) xr[k] = sx[r][k];
+#pragma unroll 1
+    for (int c = part; c < cr; c += LA_TPR) {
+        double v = s_a[r][c];
+#pragma unroll
+        for (int k = 0; k < 32; ++k) v = fma(-xr[k], s12[k][c], v);
+        rr[c] = v;
+    }
+}
+
+// ---------------------------------------------------------------------------
 // Row-major unit-lower triangular solve X <- L^{-1} X (L k x k, X k x ncols),
 // blas::sm(Left, NoTrans, 1, A11.lowerUnit(), A12) of trf.tcc:136-138.
 // One workgroup per 32-column chunk; X stays in LDS; 16-row diagonal blocks
@@ -463,6 +699,27 @@ hipError_t getrf_rm_f64(hipStream_t s, LuWork &wk, double *W, long ldw, int m, i
             const int w = (J + jb - j < ib) ? J + jb - j : ib;
             FLX_CHECK(launch_panel(s, wk, W + (long)j * ldw + j, ldw, m - j, w, piv0 + j, j,
                                    info, ib));
+            if (look && wk.stage && jb <= LU_STAGE_LD && w == PANEL_W) {
+                // the rest of the panel step in two launches: the stage, then
+                // interchange + U12 + in-block update together
+                {
+                    ProfScope ps(wk.prof, s, PROF_ROWPERM, 0.0, 16.0 * 64 * jb);
+                    hipLaunchKernelGGL(lu_stage_kernel, dim3(LU_STAGE_ROWS), dim3(256), 0, s,
+                                       (const double *)W, ldw, m, J, jb, j, w,
+                                       (const int *)wk.list, panel_groups(m - j),
+                                       (const int *)wk.perm, wk.stage, wk.sdst, wk.sperm,
+                                       wk.moved);
+                }
+                const int cr = J + jb - (j + w);
+                ProfScope ps(wk.prof, s, PROF_TRSM_RM, 2.0 * (m - j - w) * (double)w * cr,
+                             16.0 * (m - j) * (double)(cr > 0 ? cr : 0));
+                hipLaunchKernelGGL(lu_apply_kernel, dim3((m - j + LA_ROWS - 1) / LA_ROWS),
+                                   dim3(256), 0, s, W, ldw, m, J, jb, j, w,
+                                   (const double *)wk.stage, (const int *)wk.sdst,
+                                   (const int *)wk.sperm, wk.perm);
+                FLX_CHECK(hipGetLastError());
+                continue;
+            }
             if (look) {
                 // interchanges inside the block's columns now, the rest deferred
                 FLX_CHECK(rowperm_rm(s, wk.prof, W + J, ldw, m, j - J, j + w - J, jb, wk.list,
