@@ -271,18 +271,23 @@ struct Solver {
     }
 
     // kappa_inf lower-bound estimate from the factor: ||A||_inf times
-    // max(||A^{-1} e||_inf, ||A^{-1} sign(A^{-1} e)||_inf), e = ones
+    // max(||A^{-1} e||_inf, ||A^{-1} sign(A^{-1} e)||_inf), e = ones.  The
+    // factor is applied in FP32 (an order-of-magnitude estimate; the factor's
+    // own error dominates the arithmetic's), about half the time of the FP64
+    // application.
     int estimate(double *kappa)
     {
         MixBufs &m = h->mix;
         double *v = R(0);   // R holds nothing yet
         double *w = (double *)m.t.p;
         FLX_TRY(fill_const(s, v, n, 1.0));
-        FLX_TRY(prec_apply64(s, f, v, w, 0));
+        FLX_TRY(zero_slots(s, slot(SLOT_VN), 1));
+        FLX_TRY(prec_apply(s, f, v, w, 0, slot(SLOT_VN), nullptr));
         FLX_TRY(sign_of(s, w, v, n));
         FLX_TRY(zero_slots(s, slot(SLOT_T), 2));
         FLX_TRY(vmax(s, w, n, slot(SLOT_T)));
-        FLX_TRY(prec_apply64(s, f, v, w, 0));
+        FLX_TRY(zero_slots(s, slot(SLOT_VN), 1));
+        FLX_TRY(prec_apply(s, f, v, w, 0, slot(SLOT_VN), nullptr));
         FLX_TRY(vmax(s, w, n, slot(SLOT_T + 1)));
         FLX_TRY(read_slots());
         const double ainv = std::max(hslot(SLOT_T), hslot(SLOT_T + 1));
