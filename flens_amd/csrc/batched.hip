@@ -259,27 +259,54 @@ FLX_DEV int wg_getrf(T *__restrict__ A, int n, int *__restrict__ ipiv, BShared &
             }
         }
         __syncthreads();
-        // ---- A22 -= L21 U12 on 16x16 MFMA tiles (K = w)
+        // ---- A22 -= L21 U12 on 16x16 MFMA tiles (K = w).  Each wave owns
+        // 16-row blocks: their L21 fragments are loaded once, then the row
+        // block's tiles are visited with the next tile's U12 fragments and C
+        // entries requested before this tile's MFMAs (one memory latency per
+        // row block instead of one per MFMA).
         const int m2 = n - k - w;
         if (m2 > 0) {
             const int nt = (m2 + 15) / 16;
             const int lr = lane & 15, lk = lane >> 4;
-            for (int tile = wave; tile < nt * nt; tile += 4) {
-                const int ti = tile % nt, tj = tile / nt;
-                const int i0 = k + w + ti * 16, j0 = k + w + tj * 16;
-                typename Acc4<T>::type acc = {T(0), T(0), T(0), T(0)};
-                for (int kk = 0; kk < w; kk += 4) {
-                    const int kq = kk + lk;
-                    const int ar = i0 + lr, bc = j0 + lr;
-                    const T a = (ar < n && kq < w) ? A[(long)(k + kq) * n + ar] : T(0);
-                    const T b = (bc < n && kq < w) ? A[(long)bc * n + k + kq] : T(0);
-                    acc = mfma_t(a, b, acc);
-                }
-                const int col = j0 + lr;
+            constexpr int KS = BPW / 4;
+            for (int ti = wave; ti < nt; ti += 4) {
+                const int i0 = k + w + ti * 16, ar = i0 + lr;
+                T a[KS];
 #pragma unroll
-                for (int q = 0; q < 4; ++q) {
-                    const int row = i0 + Acc4<T>::row(lk, q);
-                    if (row < n && col < n) A[(long)col * n + row] -= acc[q];
+                for (int q = 0; q < KS; ++q) {
+                    const int kq = 4 * q + lk;
+                    a[q] = (ar < n && kq < w) ? A[(long)(k + kq) * n + ar] : T(0);
+                }
+                T b[KS], cv[4];
+                auto fetch = [&](int tj) {
+                    const int j0 = k + w + tj * 16, bc = j0 + lr;
+                    const bool cok = tj < nt && bc < n;
+#pragma unroll
+                    for (int q = 0; q < KS; ++q) {
+                        const int kq = 4 * q + lk;
+                        b[q] = (cok && kq < w) ? A[(long)bc * n + k + kq] : T(0);
+                    }
+#pragma unroll
+                    for (int q = 0; q < 4; ++q) {
+                        const int row = i0 + Acc4<T>::row(lk, q);
+                        cv[q] = (cok && row < n) ? A[(long)bc * n + row] : T(0);
+                    }
+                };
+                fetch(0);
+                for (int tj = 0; tj < nt; ++tj) {
+                    typename Acc4<T>::type acc = {T(0), T(0), T(0), T(0)};
+#pragma unroll
+                    for (int q = 0; q < KS; ++q) acc = mfma_t(a[q], b[q], acc);
+                    T c[4];
+#pragma unroll
+                    for (int q = 0; q < 4; ++q) c[q] = cv[q];
+                    fetch(tj + 1);
+                    const int col = k + w + tj * 16 + lr;
+#pragma unroll
+                    for (int q = 0; q < 4; ++q) {
+                        const int row = i0 + Acc4<T>::row(lk, q);
+                        if (row < n && col < n) A[(long)col * n + row] = c[q] - acc[q];
+                    }
                 }
             }
         }
