@@ -97,6 +97,148 @@ struct BShared {
     int idx;
 };
 
+// ---- one panel of wg_getrf: rows k..n-1 of columns k..k+w-1, SL rows per
+// thread (1 while the panel has at most BT rows, else 2), factored with
+// partial pivoting and written back at the final positions; L11 to LDS and
+// the row moves for the other columns to sh.mv_*.
+template <typename T, int SL>
+FLX_DEV void wg_panel(T *__restrict__ A, int n, int k, int w, int *__restrict__ ipiv,
+                      BShared &sh)
+{
+    const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
+    T r[SL][BPW];
+    int pos[SL], start[SL];
+#pragma unroll
+    for (int s = 0; s < SL; ++s) {
+        start[s] = k + t + BT * s;
+        pos[s] = start[s];
+        const bool ok = start[s] < n;
+#pragma unroll
+        for (int c = 0; c < BPW; ++c)
+            r[s][c] = (ok && c < w) ? A[(long)(k + c) * n + start[s]] : T(0);
+    }
+    // The column loop stays rolled: it runs once per panel of every
+    // system, and unrolled (32 columns x SL rows) it overflows the
+    // instruction cache.  Column j of a row is read by a select over the
+    // row's registers and written back inside the update, which runs
+    // over all BPW columns with the pivot row's entries at and left of j
+    // posted as zeros -- every register index stays static (a dynamic
+    // one would move the rows to scratch memory).
+#pragma unroll 1
+    for (int j = 0; j < w; ++j) {
+        const int par = j & 1;
+        const int kj = k + j;
+        T vj[SL];
+#pragma unroll
+        for (int s = 0; s < SL; ++s) {
+            T v = r[s][0];
+#pragma unroll
+            for (int c = 1; c < BPW; ++c) v = (c == j) ? r[s][c] : v;
+            vj[s] = v;
+        }
+        // best candidate of this thread: largest |v|, then lowest position
+        uint32_t bhi = 0, blo = 0;
+        int bpos = INT_MAX, bs = -1;
+#pragma unroll
+        for (int s = 0; s < SL; ++s) {
+            if (pos[s] >= kj && pos[s] < n) {
+                uint32_t hi, lo;
+                abs_key(vj[s], hi, lo);
+                const bool better = bs < 0 || hi > bhi || (hi == bhi && lo > blo) ||
+                                    (hi == bhi && lo == blo && pos[s] < bpos);
+                if (better) { bhi = hi; blo = lo; bpos = pos[s]; bs = s; }
+            }
+        }
+        // wave: max hi, then max lo among those, then min position
+        const uint32_t valid = bs >= 0 ? 1u : 0u;
+        const uint32_t mhi = wave_umax(valid ? bhi + 1u : 0u);
+        const bool c1 = valid && bhi + 1u == mhi;
+        const uint32_t mlo = wave_umax(c1 ? blo : 0u);
+        const bool c2 = c1 && blo == mlo;
+        const uint32_t mpos = wave_umin(c2 ? (uint32_t)bpos : 0xffffffffu);
+        if (c2 && (uint32_t)bpos == mpos) {
+            // the slot's row by blending with exact 0/1 weights; entries at
+            // and left of column j posted as zero (the update's mask), the
+            // pivot itself separately
+            T wgt[SL];
+#pragma unroll
+            for (int s = 0; s < SL; ++s) wgt[s] = (bs == s) ? T(1) : T(0);
+            T pv = vj[0] * wgt[0];
+#pragma unroll
+            for (int s = 1; s < SL; ++s) pv = pv + vj[s] * wgt[s];
+            sh.cpiv[par][wave] = (double)pv;
+#pragma unroll
+            for (int c = 0; c < BPW; ++c) {
+                T v = r[0][c] * wgt[0];
+#pragma unroll
+                for (int s = 1; s < SL; ++s) v = v + r[s][c] * wgt[s];
+                sh.cand[par][wave][c] = (c > j) ? (double)v : 0.0;
+            }
+        }
+        if (lane == 0) {
+            sh.khi[par][wave] = mhi;
+            sh.klo[par][wave] = mlo;
+            sh.kpos[par][wave] = (int)mpos;
+        }
+        __syncthreads();
+        int bw = 0;
+#pragma unroll
+        for (int q = 1; q < 4; ++q) {
+            const uint32_t h = sh.khi[par][q], l = sh.klo[par][q];
+            const int p = sh.kpos[par][q];
+            const uint32_t hb = sh.khi[par][bw], lb = sh.klo[par][bw];
+            const int pb = sh.kpos[par][bw];
+            if (h > hb || (h == hb && (l > lb || (l == lb && p < pb)))) bw = q;
+        }
+        const int pp = sh.kpos[par][bw];   // pivot row position
+        const T piv = (T)sh.cpiv[par][bw];
+        if (t == 0) {
+            ipiv[kj] = pp + 1;
+            const int a = sh.perm[kj];
+            sh.perm[kj] = sh.perm[pp];
+            sh.perm[pp] = a;
+            if (piv == T(0) && sh.info == 0) sh.info = kj + 1;
+        }
+        T pm[BPW];
+#pragma unroll
+        for (int c = 0; c < BPW; ++c) pm[c] = (T)sh.cand[par][bw][c];
+        // swap positions kj <-> pp, eliminate below kj
+#pragma unroll
+        for (int s = 0; s < SL; ++s) {
+            if (pos[s] == pp) pos[s] = kj;
+            else if (pos[s] == kj) pos[s] = pp;
+            if (pos[s] > kj && pos[s] < n) {
+                const T l = (piv != T(0)) ? vj[s] / piv : T(0);
+#pragma unroll
+                for (int c = 0; c < BPW; ++c) r[s][c] = (c == j) ? l : r[s][c] - l * pm[c];
+            }
+        }
+    }
+    __syncthreads();
+    // ---- write the panel back at the final positions; L11 to LDS
+#pragma unroll
+    for (int s = 0; s < SL; ++s) {
+        if (start[s] < n) {
+#pragma unroll
+            for (int c = 0; c < BPW; ++c)
+                if (c < w) A[(long)(k + c) * n + pos[s]] = r[s][c];
+            if (pos[s] < k + w)
+#pragma unroll
+                for (int c = 0; c < BPW; ++c) sh.l11[pos[s] - k][c] = (double)r[s][c];
+        }
+    }
+    // ---- the panel's row moves, for the columns outside it
+    if (t == 0) sh.nmv = 0;
+    __syncthreads();
+#pragma unroll
+    for (int s = 0; s < SL; ++s) {
+        if (start[s] < n && pos[s] != start[s]) {
+            const int e = atomicAdd(&sh.nmv, 1);
+            if (e < 2 * BPW) { sh.mv_dst[e] = pos[s]; sh.mv_src[e] = start[s]; }
+        }
+    }
+}
+
 // ---- LU of one system: A (n x n, ld n, column-major) in place -----------------
 // ipiv (n ints, 1-based, LAPACK convention); sh.perm receives the row order;
 // returns INFO (first exact zero pivot, 1-based, 0 if none).
@@ -109,138 +251,9 @@ FLX_DEV int wg_getrf(T *__restrict__ A, int n, int *__restrict__ ipiv, BShared &
     __syncthreads();
     for (int k = 0; k < n; k += BPW) {
         const int w = min(BPW, n - k);
-        // ---- panel: rows k..n-1 of columns k..k+w-1, BSLOT rows per thread
-        T r[BSLOT][BPW];
-        int pos[BSLOT], start[BSLOT];
-#pragma unroll
-        for (int s = 0; s < BSLOT; ++s) {
-            start[s] = k + t + BT * s;
-            pos[s] = start[s];
-            const bool ok = start[s] < n;
-#pragma unroll
-            for (int c = 0; c < BPW; ++c)
-                r[s][c] = (ok && c < w) ? A[(long)(k + c) * n + start[s]] : T(0);
-        }
-        // The column loop stays rolled: it runs once per panel of every
-        // system, and unrolled (32 columns x BSLOT rows) it overflows the
-        // instruction cache.  Column j of a row is read by a select over the
-        // row's registers and written back inside the update, which runs
-        // over all BPW columns with the pivot row's entries at and left of j
-        // posted as zeros -- every register index stays static (a dynamic
-        // one would move the rows to scratch memory).
-#pragma unroll 1
-        for (int j = 0; j < w; ++j) {
-            const int par = j & 1;
-            const int kj = k + j;
-            T vj[BSLOT];
-#pragma unroll
-            for (int s = 0; s < BSLOT; ++s) {
-                T v = r[s][0];
-#pragma unroll
-                for (int c = 1; c < BPW; ++c) v = (c == j) ? r[s][c] : v;
-                vj[s] = v;
-            }
-            // best candidate of this thread: largest |v|, then lowest position
-            uint32_t bhi = 0, blo = 0;
-            int bpos = INT_MAX, bs = -1;
-#pragma unroll
-            for (int s = 0; s < BSLOT; ++s) {
-                if (pos[s] >= kj && pos[s] < n) {
-                    uint32_t hi, lo;
-                    abs_key(vj[s], hi, lo);
-                    const bool better = bs < 0 || hi > bhi || (hi == bhi && lo > blo) ||
-                                        (hi == bhi && lo == blo && pos[s] < bpos);
-                    if (better) { bhi = hi; blo = lo; bpos = pos[s]; bs = s; }
-                }
-            }
-            // wave: max hi, then max lo among those, then min position
-            const uint32_t valid = bs >= 0 ? 1u : 0u;
-            const uint32_t mhi = wave_umax(valid ? bhi + 1u : 0u);
-            const bool c1 = valid && bhi + 1u == mhi;
-            const uint32_t mlo = wave_umax(c1 ? blo : 0u);
-            const bool c2 = c1 && blo == mlo;
-            const uint32_t mpos = wave_umin(c2 ? (uint32_t)bpos : 0xffffffffu);
-            if (c2 && (uint32_t)bpos == mpos) {
-                // the slot's row by blending with exact 0/1 weights; entries at
-                // and left of column j posted as zero (the update's mask), the
-                // pivot itself separately
-                T wgt[BSLOT];
-#pragma unroll
-                for (int s = 0; s < BSLOT; ++s) wgt[s] = (bs == s) ? T(1) : T(0);
-                T pv = vj[0] * wgt[0];
-#pragma unroll
-                for (int s = 1; s < BSLOT; ++s) pv = pv + vj[s] * wgt[s];
-                sh.cpiv[par][wave] = (double)pv;
-#pragma unroll
-                for (int c = 0; c < BPW; ++c) {
-                    T v = r[0][c] * wgt[0];
-#pragma unroll
-                    for (int s = 1; s < BSLOT; ++s) v = v + r[s][c] * wgt[s];
-                    sh.cand[par][wave][c] = (c > j) ? (double)v : 0.0;
-                }
-            }
-            if (lane == 0) {
-                sh.khi[par][wave] = mhi;
-                sh.klo[par][wave] = mlo;
-                sh.kpos[par][wave] = (int)mpos;
-            }
-            __syncthreads();
-            int bw = 0;
-#pragma unroll
-            for (int q = 1; q < 4; ++q) {
-                const uint32_t h = sh.khi[par][q], l = sh.klo[par][q];
-                const int p = sh.kpos[par][q];
-                const uint32_t hb = sh.khi[par][bw], lb = sh.klo[par][bw];
-                const int pb = sh.kpos[par][bw];
-                if (h > hb || (h == hb && (l > lb || (l == lb && p < pb)))) bw = q;
-            }
-            const int pp = sh.kpos[par][bw];   // pivot row position
-            const T piv = (T)sh.cpiv[par][bw];
-            if (t == 0) {
-                ipiv[kj] = pp + 1;
-                const int a = sh.perm[kj];
-                sh.perm[kj] = sh.perm[pp];
-                sh.perm[pp] = a;
-                if (piv == T(0) && sh.info == 0) sh.info = kj + 1;
-            }
-            T pm[BPW];
-#pragma unroll
-            for (int c = 0; c < BPW; ++c) pm[c] = (T)sh.cand[par][bw][c];
-            // swap positions kj <-> pp, eliminate below kj
-#pragma unroll
-            for (int s = 0; s < BSLOT; ++s) {
-                if (pos[s] == pp) pos[s] = kj;
-                else if (pos[s] == kj) pos[s] = pp;
-                if (pos[s] > kj && pos[s] < n) {
-                    const T l = (piv != T(0)) ? vj[s] / piv : T(0);
-#pragma unroll
-                    for (int c = 0; c < BPW; ++c) r[s][c] = (c == j) ? l : r[s][c] - l * pm[c];
-                }
-            }
-        }
-        __syncthreads();
-        // ---- write the panel back at the final positions; L11 to LDS
-#pragma unroll
-        for (int s = 0; s < BSLOT; ++s) {
-            if (start[s] < n) {
-#pragma unroll
-                for (int c = 0; c < BPW; ++c)
-                    if (c < w) A[(long)(k + c) * n + pos[s]] = r[s][c];
-                if (pos[s] < k + w)
-#pragma unroll
-                    for (int c = 0; c < BPW; ++c) sh.l11[pos[s] - k][c] = (double)r[s][c];
-            }
-        }
-        // ---- the panel's row moves, for the columns outside it
-        if (t == 0) sh.nmv = 0;
-        __syncthreads();
-#pragma unroll
-        for (int s = 0; s < BSLOT; ++s) {
-            if (start[s] < n && pos[s] != start[s]) {
-                const int e = atomicAdd(&sh.nmv, 1);
-                if (e < 2 * BPW) { sh.mv_dst[e] = pos[s]; sh.mv_src[e] = start[s]; }
-            }
-        }
+        // ---- panel (one row per thread while it has at most BT rows)
+        if (n - k <= BT) wg_panel<T, 1>(A, n, k, w, ipiv, sh);
+        else wg_panel<T, BSLOT>(A, n, k, w, ipiv, sh);
         __syncthreads();
         const int nmv = min(sh.nmv, 2 * BPW);
         for (int c0 = 0; c0 < n; c0 += BPW) {
