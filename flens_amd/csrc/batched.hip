@@ -88,10 +88,7 @@ struct BShared {
     int nmv;
     int info;
     double y[BMAX * BRHS];         // solve right-hand sides (T-typed view)
-    union {
-        double xs[BMAX * BRHS];        // refinement iterate (FP64), between factorisations
-        double mv_buf[2 * BPW][BPW];   // interchange staging, inside a factorisation
-    };
+    double xs[BMAX * BRHS];        // refinement iterate (FP64), between factorisations
     unsigned long long red[4 * BRHS];
     double rs[BRHS], bn[BRHS];
     int idx;
@@ -256,21 +253,31 @@ FLX_DEV int wg_getrf(T *__restrict__ A, int n, int *__restrict__ ipiv, BShared &
         else wg_panel<T, BSLOT>(A, n, k, w, ipiv, sh);
         __syncthreads();
         const int nmv = min(sh.nmv, 2 * BPW);
-        for (int c0 = 0; c0 < n; c0 += BPW) {
-            if (c0 >= k && c0 < k + w) continue;   // panel columns are already placed
-            for (int q = t; q < nmv * BPW; q += BT) {
-                const int e = q / BPW, cc = q % BPW, c = c0 + cc;
-                if (c < n && !(c >= k && c < k + w))
-                    sh.mv_buf[e][cc] = (double)A[(long)c * n + sh.mv_src[e]];
+        // the moves on the other columns, MVC columns per round: every
+        // (column, move) pair's source is loaded before the barrier (a
+        // destination row may be another move's source), all loads of a
+        // round issued together
+        if (nmv > 0) {
+            constexpr int MVQ = 32;                 // pairs per thread per round
+            const int mvc = (MVQ * BT) / nmv;       // columns per round
+            for (int c0 = 0; c0 < n; c0 += mvc) {
+                T v[MVQ];
+#pragma unroll
+                for (int i = 0; i < MVQ; ++i) {
+                    const int q = t + BT * i, cc = q / nmv, e = q - cc * nmv, c = c0 + cc;
+                    const bool ok = cc < mvc && c < n && !(c >= k && c < k + w);
+                    v[i] = ok ? A[(long)c * n + sh.mv_src[e]] : T(0);
+                }
+                __syncthreads();
+#pragma unroll
+                for (int i = 0; i < MVQ; ++i) {
+                    const int q = t + BT * i, cc = q / nmv, e = q - cc * nmv, c = c0 + cc;
+                    const bool ok = cc < mvc && c < n && !(c >= k && c < k + w);
+                    if (ok) A[(long)c * n + sh.mv_dst[e]] = v[i];
+                }
             }
-            __syncthreads();
-            for (int q = t; q < nmv * BPW; q += BT) {
-                const int e = q / BPW, cc = q % BPW, c = c0 + cc;
-                if (c < n && !(c >= k && c < k + w))
-                    A[(long)c * n + sh.mv_dst[e]] = (T)sh.mv_buf[e][cc];
-            }
-            __syncthreads();
         }
+        __syncthreads();
         // ---- U12 = L11^{-1} A12, one column per thread
         for (int jc = k + w + t; jc < n; jc += BT) {
             // rows >= w are computed from padding and never stored
