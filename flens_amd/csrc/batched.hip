@@ -199,13 +199,18 @@ FLX_DEV void wg_panel(T *__restrict__ A, int n, int k, int w, int *__restrict__ 
         T pm[BPW];
 #pragma unroll
         for (int c = 0; c < BPW; ++c) pm[c] = (T)sh.cand[par][bw][c];
+        // scaling as tf2 (tf2.tcc:116-124): by the reciprocal of the pivot,
+        // dividing when |pivot| is below the smallest normal number
+        constexpr T TINY = sizeof(T) == 8 ? (T)DBL_MIN : (T)FLT_MIN;
+        const bool tiny = fabs(piv) < TINY;
+        const T inv = (piv != T(0)) ? T(1) / piv : T(0);
         // swap positions kj <-> pp, eliminate below kj
 #pragma unroll
         for (int s = 0; s < SL; ++s) {
             if (pos[s] == pp) pos[s] = kj;
             else if (pos[s] == kj) pos[s] = pp;
             if (pos[s] > kj && pos[s] < n) {
-                const T l = (piv != T(0)) ? vj[s] / piv : T(0);
+                const T l = (piv == T(0)) ? T(0) : tiny ? vj[s] / piv : vj[s] * inv;
 #pragma unroll
                 for (int c = 0; c < BPW; ++c) r[s][c] = (c == j) ? l : r[s][c] - l * pm[c];
             }
