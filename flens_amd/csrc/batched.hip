@@ -258,34 +258,40 @@ FLX_DEV int wg_getrf(T *__restrict__ A, int n, int *__restrict__ ipiv, BShared &
         else wg_panel<T, BSLOT>(A, n, k, w, ipiv, sh);
         __syncthreads();
         const int nmv = min(sh.nmv, 2 * BPW);
-        // the moves on the other columns, MVC columns per round: every
+        // the moves on the other columns, cstep * MVQ columns per round: every
         // (column, move) pair's source is loaded before the barrier (a
         // destination row may be another move's source), all loads of a
         // round issued together
         if (nmv > 0) {
-            constexpr int MVQ = 32;                 // pairs per thread per round
-            const int mvc = (MVQ * BT) / nmv;       // columns per round
-            for (int c0 = 0; c0 < n; c0 += mvc) {
+            constexpr int MVQ = 32;                 // loads per thread per round
+            // thread t carries move t % nmv on columns t / nmv + cstep * i
+            const int cstep = BT / nmv, e = t % nmv, cc0 = t / nmv;
+            const bool act = cc0 < cstep;
+            const int src = sh.mv_src[e], dst = sh.mv_dst[e];
+            for (int c0 = 0; c0 < n; c0 += cstep * MVQ) {
                 T v[MVQ];
 #pragma unroll
                 for (int i = 0; i < MVQ; ++i) {
-                    const int q = t + BT * i, cc = q / nmv, e = q - cc * nmv, c = c0 + cc;
-                    const bool ok = cc < mvc && c < n && !(c >= k && c < k + w);
-                    v[i] = ok ? A[(long)c * n + sh.mv_src[e]] : T(0);
+                    const int c = c0 + cc0 + cstep * i;
+                    const bool ok = act && c < n && !(c >= k && c < k + w);
+                    v[i] = ok ? A[(long)c * n + src] : T(0);
                 }
                 __syncthreads();
 #pragma unroll
                 for (int i = 0; i < MVQ; ++i) {
-                    const int q = t + BT * i, cc = q / nmv, e = q - cc * nmv, c = c0 + cc;
-                    const bool ok = cc < mvc && c < n && !(c >= k && c < k + w);
-                    if (ok) A[(long)c * n + sh.mv_dst[e]] = v[i];
+                    const int c = c0 + cc0 + cstep * i;
+                    const bool ok = act && c < n && !(c >= k && c < k + w);
+                    if (ok) A[(long)c * n + dst] = v[i];
                 }
             }
         }
         __syncthreads();
         // ---- U12 = L11^{-1} A12, one column per thread
         for (int jc = k + w + t; jc < n; jc += BT) {
-            // rows >= w are computed from padding and never stored
+            // rows >= w are computed from padding and never stored; the
+            // fence keeps the L11 reads inside the loop (hoisted out of it
+            // they take 496 registers)
+            __asm__ volatile("" ::: "memory");
             T x[BPW];
             T *col = A + (long)jc * n + k;
 #pragma unroll
@@ -489,7 +495,7 @@ FLX_DEV int wg_select(const float *__restrict__ P, const float (&f)[MLP_IN])
 }
 
 template <int MODE>
-__global__ __launch_bounds__(BT) void batched_gesv_kernel(const BatchProb *__restrict__ probs,
+__global__ __launch_bounds__(BT, 2) void batched_gesv_kernel(const BatchProb *__restrict__ probs,
                                                            int nprob, int *__restrict__ counter,
                                                            double *__restrict__ A,
                                                            double *__restrict__ B,
