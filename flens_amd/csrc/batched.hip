@@ -258,106 +258,148 @@ FLX_DEV int wg_getrf(T *__restrict__ A, int n, int *__restrict__ ipiv, BShared &
         else wg_panel<T, BSLOT>(A, n, k, w, ipiv, sh);
         __syncthreads();
         const int nmv = min(sh.nmv, 2 * BPW);
-        // the moves on the other columns, cstep * MVQ columns per round: every
-        // (column, move) pair's source is loaded before the barrier (a
-        // destination row may be another move's source), all loads of a
-        // round issued together
-        if (nmv > 0) {
-            constexpr int MVQ = 32;                 // loads per thread per round
-            // thread t carries move t % nmv on columns t / nmv + cstep * i
-            const int cstep = BT / nmv, e = t % nmv, cc0 = t / nmv;
-            const bool act = cc0 < cstep;
+        const int c1 = k + w, m2 = n - c1;      // first trailing column, trailing size
+        // ---- phase A: wave 0 replaces L11 (unit lower) by its inverse X,
+        // row by row (X[i][j] = -(L[i][j] + sum_{j<q<i} L[i][q] X[q][j]));
+        // waves 1-3 apply the row moves to the factored columns 0..k-1,
+        // 16 columns per pass, each column's sources loaded before its
+        // destinations are written (a column belongs to one wave, so no
+        // barrier is needed between the two)
+        if (wave == 0) {
+            if (m2 > 0) {
+                const int j = lane & (BPW - 1);
+                for (int i = 1; i < BPW; ++i) {
+                    const double li = sh.l11[i][j];
+                    double acc = li;
+                    for (int q = 1; q < i; ++q) {
+                        const double xq = sh.l11[q][j];
+                        const double lq = readlane_d(li, q);
+                        if (j < q) acc = fma(lq, xq, acc);
+                    }
+                    if (lane < i) sh.l11[i][lane] = -acc;
+                }
+            }
+        } else if (nmv > 0) {
+            const bool act = lane < nmv;
+            const int e = act ? lane : 0;
             const int src = sh.mv_src[e], dst = sh.mv_dst[e];
-            for (int c0 = 0; c0 < n; c0 += cstep * MVQ) {
-                T v[MVQ];
+            for (int c0 = (wave - 1) * 16; c0 < k; c0 += 3 * 16) {
+                T v[16];
 #pragma unroll
-                for (int i = 0; i < MVQ; ++i) {
-                    const int c = c0 + cc0 + cstep * i;
-                    const bool ok = act && c < n && !(c >= k && c < k + w);
-                    v[i] = ok ? A[(long)c * n + src] : T(0);
-                }
-                __syncthreads();
+                for (int c = 0; c < 16; ++c)
+                    v[c] = (act && c0 + c < k) ? A[(long)(c0 + c) * n + src] : T(0);
 #pragma unroll
-                for (int i = 0; i < MVQ; ++i) {
-                    const int c = c0 + cc0 + cstep * i;
-                    const bool ok = act && c < n && !(c >= k && c < k + w);
-                    if (ok) A[(long)c * n + dst] = v[i];
-                }
+                for (int c = 0; c < 16; ++c)
+                    if (act && c0 + c < k) A[(long)(c0 + c) * n + dst] = v[c];
             }
         }
         __syncthreads();
-        // ---- U12 = L11^{-1} A12, one column per thread
-        for (int jc = k + w + t; jc < n; jc += BT) {
-            // rows >= w are computed from padding and never stored; the
-            // fence keeps the L11 reads inside the loop (hoisted out of it
-            // they take 496 registers)
-            __asm__ volatile("" ::: "memory");
-            T x[BPW];
-            T *col = A + (long)jc * n + k;
-#pragma unroll
-            for (int i = 0; i < BPW; ++i) x[i] = (i < w) ? col[i] : T(0);
-#pragma unroll
-            for (int q = 0; q < BPW - 1; ++q) {
-                const T xq = x[q];
-#pragma unroll
-                for (int i = q + 1; i < BPW; ++i) x[i] = x[i] - (T)sh.l11[i][q] * xq;
-            }
-            if (w == BPW) {
-#pragma unroll
-                for (int i = 0; i < BPW; ++i) col[i] = x[i];
-            } else {
-                for (int i = 0; i < w; ++i) col[i] = x[i];
-            }
-        }
-        __syncthreads();
-        // ---- A22 -= L21 U12 on 16x16 MFMA tiles (K = w).  Each wave owns
-        // 16-row blocks: their L21 fragments are loaded once, then the row
-        // block's tiles are visited with the next tile's U12 fragments and C
-        // entries requested before this tile's MFMAs (one memory latency per
-        // row block instead of one per MFMA).
-        const int m2 = n - k - w;
+        // ---- phase B: the trailing columns in strips of two 16-column
+        // tiles, one strip per wave at a time: row moves, U12 = X A12 and
+        // A22 -= L21 U12 on 16x16 MFMA tiles (each L21 fragment serves both
+        // tiles).  The K index of MFMA step q on lane group lk is
+        // kx(q) = 16 (q / 4) + (row of accumulator entry q % 4), so the two
+        // U12 accumulator tiles of a column tile are the update's B
+        // fragments as they stand.
         if (m2 > 0) {
-            const int nt = (m2 + 15) / 16;
             const int lr = lane & 15, lk = lane >> 4;
-            constexpr int KS = BPW / 4;
-            for (int ti = wave; ti < nt; ti += 4) {
-                const int i0 = k + w + ti * 16, ar = i0 + lr;
-                T a[KS];
+            constexpr int KS = BPW / 4, ST = 2;
+            auto kx = [&](int q) { return 16 * (q / 4) + Acc4<T>::row(lk, q % 4); };
+            // X fragments: tile 0 (rows 0-15) needs K < 16 only
+            T x0[4], x1[KS];
 #pragma unroll
-                for (int q = 0; q < KS; ++q) {
-                    const int kq = 4 * q + lk;
-                    a[q] = (ar < n && kq < w) ? A[(long)(k + kq) * n + ar] : T(0);
+            for (int q = 0; q < KS; ++q) {
+                const int kk = 4 * q + lk;
+                if (q < 4) x0[q] = kk < lr ? (T)sh.l11[lr][kk] : (kk == lr ? T(1) : T(0));
+                x1[q] = kk < 16 + lr ? (T)sh.l11[16 + lr][kk] : (kk == 16 + lr ? T(1) : T(0));
+            }
+            const bool act = lane < nmv;
+            const int e = act ? lane : 0;
+            const int src = sh.mv_src[e], dst = sh.mv_dst[e];
+            const int nt = (m2 + 15) / 16, nst = (nt + ST - 1) / ST;
+            for (int st = wave; st < nst; st += 4) {
+                const int c0 = c1 + 16 * ST * st;
+                // row moves of the strip's columns, 16 at a time
+                if (nmv > 0) {
+#pragma unroll
+                    for (int h = 0; h < ST; ++h) {
+                        T v[16];
+#pragma unroll
+                        for (int c = 0; c < 16; ++c) {
+                            const int cc = c0 + 16 * h + c;
+                            v[c] = (act && cc < n) ? A[(long)cc * n + src] : T(0);
+                        }
+#pragma unroll
+                        for (int c = 0; c < 16; ++c) {
+                            const int cc = c0 + 16 * h + c;
+                            if (act && cc < n) A[(long)cc * n + dst] = v[c];
+                        }
+                    }
+                    // the strip's other lanes read these rows next
+                    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+                    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
                 }
-                T b[KS], cv[4];
-                auto fetch = [&](int tj) {
-                    const int j0 = k + w + tj * 16, bc = j0 + lr;
-                    const bool cok = tj < nt && bc < n;
+                bool cok[ST];
+                T *cp[ST];
+                T bu[ST][KS];
 #pragma unroll
-                    for (int q = 0; q < KS; ++q) {
-                        const int kq = 4 * q + lk;
-                        b[q] = (cok && kq < w) ? A[(long)bc * n + k + kq] : T(0);
-                    }
+                for (int h = 0; h < ST; ++h) {
+                    const int col = c0 + 16 * h + lr;
+                    cok[h] = col < n;
+                    cp[h] = A + (long)(cok[h] ? col : c0) * n;
+                    // U12 (32 x 16) = X A12
+                    T g[KS];
 #pragma unroll
-                    for (int q = 0; q < 4; ++q) {
-                        const int row = i0 + Acc4<T>::row(lk, q);
-                        cv[q] = (cok && row < n) ? A[(long)bc * n + row] : T(0);
+                    for (int q = 0; q < KS; ++q) g[q] = cok[h] ? cp[h][k + 4 * q + lk] : T(0);
+                    typename Acc4<T>::type u0 = {T(0), T(0), T(0), T(0)}, u1 = u0;
+#pragma unroll
+                    for (int q = 0; q < 4; ++q) u0 = mfma_t(x0[q], g[q], u0);
+#pragma unroll
+                    for (int q = 0; q < KS; ++q) u1 = mfma_t(x1[q], g[q], u1);
+#pragma unroll
+                    for (int q = 0; q < 4; ++q) { bu[h][q] = u0[q]; bu[h][4 + q] = u1[q]; }
+                    if (cok[h]) {
+#pragma unroll
+                        for (int q = 0; q < KS; ++q) cp[h][k + kx(q)] = bu[h][q];
                     }
+                }
+                // A22 -= L21 U12, 16-row blocks, the next block's L21
+                // fragments and C entries requested before this block's MFMAs
+                T a[KS], cv[ST][4];
+                auto fetch = [&](int ti) {
+                    const int ar = c1 + 16 * ti + lr;
+                    const bool rok = ti < nt && ar < n;
+#pragma unroll
+                    for (int q = 0; q < KS; ++q)
+                        a[q] = rok ? A[(long)(k + kx(q)) * n + ar] : T(0);
+#pragma unroll
+                    for (int h = 0; h < ST; ++h)
+#pragma unroll
+                        for (int q = 0; q < 4; ++q) {
+                            const int row = c1 + 16 * ti + Acc4<T>::row(lk, q);
+                            cv[h][q] = (ti < nt && cok[h] && row < n) ? cp[h][row] : T(0);
+                        }
                 };
                 fetch(0);
-                for (int tj = 0; tj < nt; ++tj) {
-                    typename Acc4<T>::type acc = {T(0), T(0), T(0), T(0)};
+                for (int ti = 0; ti < nt; ++ti) {
+                    typename Acc4<T>::type acc[ST];
+                    T c[ST][4];
 #pragma unroll
-                    for (int q = 0; q < KS; ++q) acc = mfma_t(a[q], b[q], acc);
-                    T c[4];
+                    for (int h = 0; h < ST; ++h) {
+                        acc[h] = {T(0), T(0), T(0), T(0)};
 #pragma unroll
-                    for (int q = 0; q < 4; ++q) c[q] = cv[q];
-                    fetch(tj + 1);
-                    const int col = k + w + tj * 16 + lr;
+                        for (int q = 0; q < KS; ++q) acc[h] = mfma_t(a[q], bu[h][q], acc[h]);
 #pragma unroll
-                    for (int q = 0; q < 4; ++q) {
-                        const int row = i0 + Acc4<T>::row(lk, q);
-                        if (row < n && col < n) A[(long)col * n + row] = c[q] - acc[q];
+                        for (int q = 0; q < 4; ++q) c[h][q] = cv[h][q];
                     }
+                    fetch(ti + 1);
+#pragma unroll
+                    for (int h = 0; h < ST; ++h)
+#pragma unroll
+                        for (int q = 0; q < 4; ++q) {
+                            const int row = c1 + 16 * ti + Acc4<T>::row(lk, q);
+                            if (cok[h] && row < n) cp[h][row] = c[h][q] - acc[h][q];
+                        }
                 }
             }
         }
@@ -376,6 +418,7 @@ FLX_DEV void wg_trs(const T *__restrict__ A, int n, int nrhs, BShared &sh)
     const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
     T *y = reinterpret_cast<T *>(sh.y);   // y[i * BRHS + c]
     const int nbk = (n + 63) / 64;
+    constexpr int QC = 8;                 // block columns per batch of loads
     for (int pass = 0; pass < 2; ++pass) {
         const bool lower = pass == 0;
         for (int bb = 0; bb < nbk; ++bb) {
@@ -383,49 +426,76 @@ FLX_DEV void wg_trs(const T *__restrict__ A, int n, int nrhs, BShared &sh)
             const int r0 = blk * 64, rows = min(64, n - r0);
             if (wave == 0) {
                 const int i = r0 + lane;
+                const bool rok = lane < rows;
                 T v[BRHS];
 #pragma unroll
-                for (int c = 0; c < BRHS; ++c) v[c] = (lane < rows && c < nrhs) ? y[i * BRHS + c] : T(0);
-                if (lower) {
-                    for (int q = 0; q < rows; ++q) {
-                        const T lq = (lane < rows && lane > q) ? A[(long)(r0 + q) * n + i] : T(0);
-#pragma unroll
-                        for (int c = 0; c < BRHS; ++c) {
-                            const T xq = readlane_t(v[c], q);
-                            v[c] = v[c] - lq * xq;
-                        }
-                    }
-                } else {
-                    for (int q = rows - 1; q >= 0; --q) {
-                        const T d = A[(long)(r0 + q) * n + r0 + q];
+                for (int c = 0; c < BRHS; ++c) v[c] = (rok && c < nrhs) ? y[i * BRHS + c] : T(0);
+                // step s handles block column q = s (L, forwards) or
+                // q = 63 - s (U, backwards); this lane's entry of column q:
+                // below the diagonal for L, on and above it for U
+                auto col = [&](int s) { return lower ? s : 63 - s; };
+                auto ld = [&](int q) -> T {
+                    const bool ok = rok && q < rows && (lower ? lane > q : lane <= q);
+                    return ok ? A[(long)(r0 + q) * n + i] : T(0);
+                };
+                auto step = [&](int q, T a) {
+                    if (q >= rows) return;          // past the block (uniform)
+                    if (!lower) {
+                        const T d = readlane_t(a, q);
                         if (lane == q) {
 #pragma unroll
                             for (int c = 0; c < BRHS; ++c) v[c] = v[c] / d;
                         }
-                        const T uq = (lane < q) ? A[(long)(r0 + q) * n + i] : T(0);
-#pragma unroll
-                        for (int c = 0; c < BRHS; ++c) {
-                            const T xq = readlane_t(v[c], q);
-                            v[c] = v[c] - uq * xq;
-                        }
                     }
+                    const T m = (lower || lane < q) ? a : T(0);
+#pragma unroll
+                    for (int c = 0; c < BRHS; ++c) {
+                        const T xq = readlane_t(v[c], q);
+                        v[c] = v[c] - m * xq;
+                    }
+                };
+                // two batches in flight: the next batch's loads are issued
+                // before the current batch's steps
+                T b0[QC], b1[QC];
+#pragma unroll
+                for (int u = 0; u < QC; ++u) b0[u] = ld(col(u));
+                for (int s0 = 0; s0 < 64; s0 += 2 * QC) {
+#pragma unroll
+                    for (int u = 0; u < QC; ++u) b1[u] = ld(col(s0 + QC + u));
+#pragma unroll
+                    for (int u = 0; u < QC; ++u) step(col(s0 + u), b0[u]);
+#pragma unroll
+                    for (int u = 0; u < QC; ++u)
+                        b0[u] = (s0 + 2 * QC < 64) ? ld(col(s0 + 2 * QC + u)) : T(0);
+#pragma unroll
+                    for (int u = 0; u < QC; ++u) step(col(s0 + QC + u), b1[u]);
                 }
-                if (lane < rows)
+                if (rok)
 #pragma unroll
                     for (int c = 0; c < BRHS; ++c)
                         if (c < nrhs) y[i * BRHS + c] = v[c];
             }
             __syncthreads();
-            // fold into the remaining rows: below (L) or above (U)
+            // fold into the remaining rows: below (L) or above (U), the
+            // block's 64 column entries of a row loaded 16 at a time
             const int f0 = lower ? r0 + rows : 0, f1 = lower ? n : r0;
             for (int i = f0 + t; i < f1; i += BT) {
                 T acc[BRHS];
 #pragma unroll
                 for (int c = 0; c < BRHS; ++c) acc[c] = (c < nrhs) ? y[i * BRHS + c] : T(0);
-                for (int q = 0; q < rows; ++q) {
-                    const T a = A[(long)(r0 + q) * n + i];
+                for (int q0 = 0; q0 < rows; q0 += 16) {
+                    T av[16];
 #pragma unroll
-                    for (int c = 0; c < BRHS; ++c) acc[c] = acc[c] - a * y[(r0 + q) * BRHS + c];
+                    for (int u = 0; u < 16; ++u)
+                        av[u] = (q0 + u < rows) ? A[(long)(r0 + q0 + u) * n + i] : T(0);
+#pragma unroll
+                    for (int u = 0; u < 16; ++u) {
+                        if (q0 + u < rows) {
+#pragma unroll
+                            for (int c = 0; c < BRHS; ++c)
+                                acc[c] = acc[c] - av[u] * y[(r0 + q0 + u) * BRHS + c];
+                        }
+                    }
                 }
 #pragma unroll
                 for (int c = 0; c < BRHS; ++c)
@@ -442,9 +512,18 @@ FLX_DEV void wg_load_rhs(const double *__restrict__ src, int n, int nrhs, const 
                          BShared &sh)
 {
     T *y = reinterpret_cast<T *>(sh.y);
-    for (int q = threadIdx.x; q < n * BRHS; q += BT) {
-        const int i = q / BRHS, c = q % BRHS;
-        y[q] = (c < nrhs) ? (T)(src[(long)c * n + sh.perm[i]] * (scale ? scale[c] : 1.0)) : T(0);
+    for (int q0 = threadIdx.x; q0 < n * BRHS; q0 += 8 * BT) {
+        double v[8];
+#pragma unroll
+        for (int u = 0; u < 8; ++u) {
+            const int q = q0 + BT * u, i = q / BRHS, c = q % BRHS;
+            v[u] = (q < n * BRHS && c < nrhs) ? src[(long)c * n + sh.perm[i]] : 0.0;
+        }
+#pragma unroll
+        for (int u = 0; u < 8; ++u) {
+            const int q = q0 + BT * u, c = q % BRHS;
+            if (q < n * BRHS) y[q] = (c < nrhs) ? (T)(v[u] * (scale ? scale[c] : 1.0)) : T(0);
+        }
     }
     __syncthreads();
 }
@@ -532,10 +611,22 @@ __global__ __launch_bounds__(BT, 2) void batched_gesv_kernel(const BatchProb *__
             double *r = R + p.b;
             // FP32 copy (overflow -> fall back, as dsgesv's ITER = -2)
             int ovf = 0;
-            for (long q = t; q < (long)n * n; q += BT) {
-                const double v = a[q];
-                ovf |= fabs(v) > (double)FLT_MAX;
-                w[q] = (float)v;
+            const long nn = (long)n * n;
+            for (long q0 = t; q0 < nn; q0 += 8L * BT) {
+                double v[8];
+#pragma unroll
+                for (int u = 0; u < 8; ++u) {
+                    const long q = q0 + (long)BT * u;
+                    v[u] = q < nn ? a[q] : 0.0;
+                }
+#pragma unroll
+                for (int u = 0; u < 8; ++u) {
+                    const long q = q0 + (long)BT * u;
+                    if (q < nn) {
+                        ovf |= fabs(v[u]) > (double)FLT_MAX;
+                        w[q] = (float)v[u];
+                    }
+                }
             }
             ovf = __syncthreads_or(ovf);
             int inf32 = ovf ? 1 : wg_getrf<float>(w, n, pv, sh);
@@ -546,7 +637,13 @@ __global__ __launch_bounds__(BT, 2) void batched_gesv_kernel(const BatchProb *__
                 double an = 0.0, amin = DBL_MAX, dmin = DBL_MAX;
                 for (int i = t; i < n; i += BT) {
                     double s = 0.0;
-                    for (int j = 0; j < n; ++j) s += fabs(a[(long)j * n + i]);
+                    for (int j0 = 0; j0 < n; j0 += 16) {
+                        double v[16];
+#pragma unroll
+                        for (int u = 0; u < 16; ++u) v[u] = (j0 + u < n) ? a[(long)(j0 + u) * n + i] : 0.0;
+#pragma unroll
+                        for (int u = 0; u < 16; ++u) s += fabs(v[u]);
+                    }
                     an = fmax(an, s);
                     amin = fmin(amin, s);
                     const double d = fabs(a[(long)i * n + i]);
@@ -627,10 +724,18 @@ __global__ __launch_bounds__(BT, 2) void batched_gesv_kernel(const BatchProb *__
                         double acc[BRHS];
 #pragma unroll
                         for (int c = 0; c < BRHS; ++c) acc[c] = (c < nrhs) ? bx[(long)c * n + i] : 0.0;
-                        for (int j = 0; j < n; ++j) {
-                            const double aij = a[(long)j * n + i];
+                        for (int j0 = 0; j0 < n; j0 += 8) {
+                            double av[8];
 #pragma unroll
-                            for (int c = 0; c < BRHS; ++c) acc[c] = fma(-aij, sh.xs[j * BRHS + c], acc[c]);
+                            for (int u = 0; u < 8; ++u) av[u] = (j0 + u < n) ? a[(long)(j0 + u) * n + i] : 0.0;
+#pragma unroll
+                            for (int u = 0; u < 8; ++u) {
+                                if (j0 + u < n) {
+#pragma unroll
+                                    for (int c = 0; c < BRHS; ++c)
+                                        acc[c] = fma(-av[u], sh.xs[(j0 + u) * BRHS + c], acc[c]);
+                                }
+                            }
                         }
 #pragma unroll
                         for (int c = 0; c < BRHS; ++c) {
