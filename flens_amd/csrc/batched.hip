@@ -259,38 +259,26 @@ FLX_DEV int wg_getrf(T *__restrict__ A, int n, int *__restrict__ ipiv, BShared &
         __syncthreads();
         const int nmv = min(sh.nmv, 2 * BPW);
         const int c1 = k + w, m2 = n - c1;      // first trailing column, trailing size
+        // row position after this panel, by original row: the later
+        // panels' interchanges on this panel's columns are applied once, at
+        // the end (sh.y is free until the solve: n ints per panel)
+        {
+            int *const after = reinterpret_cast<int *>(sh.y) + (k / BPW) * n;
+            for (int i = t; i < n; i += BT) after[sh.perm[i]] = i;
+        }
         // ---- phase A: wave 0 replaces L11 (unit lower) by its inverse X,
-        // row by row (X[i][j] = -(L[i][j] + sum_{j<q<i} L[i][q] X[q][j]));
-        // waves 1-3 apply the row moves to the factored columns 0..k-1,
-        // 16 columns per pass, each column's sources loaded before its
-        // destinations are written (a column belongs to one wave, so no
-        // barrier is needed between the two)
-        if (wave == 0) {
-            if (m2 > 0) {
-                const int j = lane & (BPW - 1);
-                for (int i = 1; i < BPW; ++i) {
-                    const double li = sh.l11[i][j];
-                    double acc = li;
-                    for (int q = 1; q < i; ++q) {
-                        const double xq = sh.l11[q][j];
-                        const double lq = readlane_d(li, q);
-                        if (j < q) acc = fma(lq, xq, acc);
-                    }
-                    if (lane < i) sh.l11[i][lane] = -acc;
+        // row by row (X[i][j] = -(L[i][j] + sum_{j<q<i} L[i][q] X[q][j]))
+        if (wave == 0 && m2 > 0) {
+            const int j = lane & (BPW - 1);
+            for (int i = 1; i < BPW; ++i) {
+                const double li = sh.l11[i][j];
+                double acc = li;
+                for (int q = 1; q < i; ++q) {
+                    const double xq = sh.l11[q][j];
+                    const double lq = readlane_d(li, q);
+                    if (j < q) acc = fma(lq, xq, acc);
                 }
-            }
-        } else if (nmv > 0) {
-            const bool act = lane < nmv;
-            const int e = act ? lane : 0;
-            const int src = sh.mv_src[e], dst = sh.mv_dst[e];
-            for (int c0 = (wave - 1) * 16; c0 < k; c0 += 3 * 16) {
-                T v[16];
-#pragma unroll
-                for (int c = 0; c < 16; ++c)
-                    v[c] = (act && c0 + c < k) ? A[(long)(c0 + c) * n + src] : T(0);
-#pragma unroll
-                for (int c = 0; c < 16; ++c)
-                    if (act && c0 + c < k) A[(long)(c0 + c) * n + dst] = v[c];
+                if (lane < i) sh.l11[i][lane] = -acc;
             }
         }
         __syncthreads();
@@ -400,6 +388,40 @@ FLX_DEV int wg_getrf(T *__restrict__ A, int n, int *__restrict__ ipiv, BShared &
                             const int row = c1 + 16 * ti + Acc4<T>::row(lk, q);
                             if (cok[h] && row < n) cp[h][row] = c[h][q] - acc[h][q];
                         }
+                }
+            }
+        }
+        __syncthreads();
+    }
+    // ---- the later panels' interchanges on the factored columns (the
+    // reference applies them panel by panel, flens/lapack/ge/trf.tcc:126):
+    // row r >= k + BPW of a column in panel k takes the entry of the row now
+    // at r as it stood after that panel.  Four columns per wave per pass,
+    // every load of a pass before its stores.
+    if (n > BPW) {
+        const int *after = reinterpret_cast<const int *>(sh.y);
+        constexpr int GC = 4, RL = BMAX / 64;
+        const int jend = ((n - 1) / BPW) * BPW;   // first column of the last panel
+        for (int j0 = GC * wave; j0 < jend; j0 += 4 * GC) {
+            T v[GC][RL];
+#pragma unroll
+            for (int g = 0; g < GC; ++g) {
+                const int j = j0 + g, pj = j / BPW;
+#pragma unroll
+                for (int u = 0; u < RL; ++u) {
+                    const int r = (pj + 1) * BPW + lane + 64 * u;
+                    v[g][u] = (j < n && r < n) ? A[(long)j * n + after[pj * n + sh.perm[r]]] : T(0);
+                }
+            }
+            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
+#pragma unroll
+            for (int g = 0; g < GC; ++g) {
+                const int j = j0 + g, pj = j / BPW;
+#pragma unroll
+                for (int u = 0; u < RL; ++u) {
+                    const int r = (pj + 1) * BPW + lane + 64 * u;
+                    if (j < n && r < n) A[(long)j * n + r] = v[g][u];
                 }
             }
         }
