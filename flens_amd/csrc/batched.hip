@@ -631,52 +631,58 @@ __global__ __launch_bounds__(BT, 2) void batched_gesv_kernel(const BatchProb *__
         if (MODE >= 1) {
             float *w = W32 + p.a;
             double *r = R + p.b;
-            // FP32 copy (overflow -> fall back, as dsgesv's ITER = -2)
+            // FP32 copy (overflow -> fall back, as dsgesv's ITER = -2) in the
+            // same pass as the row sums of |A| (||A||_inf and the row
+            // statistics of the features): rows t and t + BT (n <= 2 BT),
+            // 16 columns per batch of loads
+            wg_colmax_reset(sh);
+            if (t < 2) sh.red[3 * BRHS + 1 + t] = (unsigned long long)__double_as_longlong(DBL_MAX);
+            __syncthreads();
             int ovf = 0;
-            const long nn = (long)n * n;
-            for (long q0 = t; q0 < nn; q0 += 8L * BT) {
-                double v[8];
+            {
+                double rs[2] = {0.0, 0.0}, dg[2] = {0.0, 0.0};
+                for (int j0 = 0; j0 < n; j0 += 16) {
+                    double v[2][16];
 #pragma unroll
-                for (int u = 0; u < 8; ++u) {
-                    const long q = q0 + (long)BT * u;
-                    v[u] = q < nn ? a[q] : 0.0;
+                    for (int h = 0; h < 2; ++h)
+#pragma unroll
+                        for (int u = 0; u < 16; ++u) {
+                            const int i = t + BT * h;
+                            v[h][u] = (i < n && j0 + u < n) ? a[(long)(j0 + u) * n + i] : 0.0;
+                        }
+#pragma unroll
+                    for (int h = 0; h < 2; ++h)
+#pragma unroll
+                        for (int u = 0; u < 16; ++u) {
+                            const int i = t + BT * h, j = j0 + u;
+                            if (i < n && j < n) {
+                                const double av = fabs(v[h][u]);
+                                ovf |= av > (double)FLT_MAX;
+                                w[(long)j * n + i] = (float)v[h][u];
+                                rs[h] += av;
+                                if (j == i) dg[h] = av;
+                            }
+                        }
                 }
+                double an = 0.0, amin = DBL_MAX, dmin = DBL_MAX;
 #pragma unroll
-                for (int u = 0; u < 8; ++u) {
-                    const long q = q0 + (long)BT * u;
-                    if (q < nn) {
-                        ovf |= fabs(v[u]) > (double)FLT_MAX;
-                        w[q] = (float)v[u];
+                for (int h = 0; h < 2; ++h) {
+                    if (t + BT * h < n) {
+                        an = fmax(an, rs[h]);
+                        amin = fmin(amin, rs[h]);
+                        dmin = fmin(dmin, dg[h] / fmax(rs[h] - dg[h], 1e-300));
                     }
                 }
+                atomicMax(&sh.red[0], (unsigned long long)__double_as_longlong(an));
+                atomicMin(&sh.red[3 * BRHS + 1], (unsigned long long)__double_as_longlong(amin));
+                atomicMin(&sh.red[3 * BRHS + 2], (unsigned long long)__double_as_longlong(dmin));
             }
             ovf = __syncthreads_or(ovf);
             int inf32 = ovf ? 1 : wg_getrf<float>(w, n, pv, sh);
             if (ovf) it_code = -2;
             else if (inf32) it_code = -3;
             if (!ovf && !inf32) {
-                // ||A||_inf, ||b_c||_inf, and the row statistics of the features
-                double an = 0.0, amin = DBL_MAX, dmin = DBL_MAX;
-                for (int i = t; i < n; i += BT) {
-                    double s = 0.0;
-                    for (int j0 = 0; j0 < n; j0 += 16) {
-                        double v[16];
-#pragma unroll
-                        for (int u = 0; u < 16; ++u) v[u] = (j0 + u < n) ? a[(long)(j0 + u) * n + i] : 0.0;
-#pragma unroll
-                        for (int u = 0; u < 16; ++u) s += fabs(v[u]);
-                    }
-                    an = fmax(an, s);
-                    amin = fmin(amin, s);
-                    const double d = fabs(a[(long)i * n + i]);
-                    dmin = fmin(dmin, d / fmax(s - d, 1e-300));
-                }
-                wg_colmax_reset(sh);
-                if (t < 2) sh.red[3 * BRHS + 1 + t] = (unsigned long long)__double_as_longlong(DBL_MAX);
-                __syncthreads();
-                atomicMax(&sh.red[0], (unsigned long long)__double_as_longlong(an));
-                atomicMin(&sh.red[3 * BRHS + 1], (unsigned long long)__double_as_longlong(amin));
-                atomicMin(&sh.red[3 * BRHS + 2], (unsigned long long)__double_as_longlong(dmin));
+                // ||b_c||_inf
                 for (int q = t; q < n * nrhs; q += BT)
                     atomicMax(&sh.red[BRHS + q / n],
                               (unsigned long long)__double_as_longlong(fabs(bx[q])));

@@ -94,6 +94,45 @@ def test_batched_fp64_solves_vs_oracle(gpu):
             assert np.array_equal(x, b)      # the solve is skipped on INFO > 0
 
 
+EDGE_N = (1, 2, 15, 16, 17, 31, 32, 33, 47, 48, 63, 64, 65, 95, 96, 97, 127, 160, 255, 256,
+          257, 300, 383, 449, 480, 481, 500, 511, 512)
+
+
+def test_batched_fp64_edge_sizes(gpu):
+    """Panel, strip and tile boundaries (multiples of 16 and 32, one row
+    per thread up to 256 rows, two above) up to the largest n: the whole
+    factor (dget01, so the final row order of every L column), pivots and
+    solutions."""
+    rng = np.random.default_rng(21)
+    n = np.array(EDGE_N, dtype=np.int32)
+    nrhs = rng.integers(1, 9, len(n)).astype(np.int32)
+    A = [np.asfortranarray(rng.standard_normal((k, k))) for k in n]
+    B = [np.asfortranarray(rng.standard_normal((k, r))) for k, r in zip(n, nrhs)]
+    (outs, cfg) = run_solves(gpu, n, nrhs, A, B, mode=0)
+    lu_all, x_all, piv_all, info, _ = outs[0]
+    lus = unpack(lu_all, [(k, k) for k in n])
+    xs = unpack(x_all, [(k, r) for k, r in zip(n, nrhs)])
+    po = np.concatenate([[0], np.cumsum(n)])
+    for i in range(len(n)):
+        piv = piv_all[po[i]:po[i + 1]]
+        rpiv, rinfo, gap = O.dgetrf_gaps(A[i].copy(order="F"))
+        assert info[i] == rinfo == 0, (n[i], info[i], rinfo)
+        d = np.nonzero(piv != rpiv)[0]
+        assert d.size == 0 or gap[d[0]] < TAU, (n[i], d[:3])
+        assert O.dget01(A[i], lus[i], piv) < 30, n[i]
+        assert O.dget02(A[i], xs[i], B[i]) < 30, n[i]
+    # the FP32 factor + FP64 refinement on the same sizes
+    (outs, cfg) = run_solves(gpu, n, nrhs, A, B, mode=1)
+    xs = unpack(outs[0][1], [(k, r) for k, r in zip(n, nrhs)])
+    for i in range(len(n)):
+        a, b, x = A[i], B[i], xs[i]
+        r = b - a @ x
+        an = np.abs(a).sum(1).max()
+        eta = max(np.abs(r[:, j]).max() / (an * np.abs(x[:, j]).max() + np.abs(b[:, j]).max())
+                  for j in range(b.shape[1]))
+        assert eta <= n[i] * EPS, (n[i], eta, outs[0][4][i])
+
+
 def test_batched_mixed_solves_and_fallback(gpu):
     count = 200
     ill = {3, 40, 41, 90}
