@@ -431,9 +431,10 @@ FLX_DEV int wg_getrf(T *__restrict__ A, int n, int *__restrict__ ipiv, BShared &
 }
 
 // ---- solve with the factors: Y (n x nrhs, rows already permuted, in sh.y
-// as T) <- U^{-1} L^{-1} Y.  64-row blocks: wave 0 substitutes within the
-// block (one row per lane, the solved row broadcast by readlane), then all
-// threads fold the block into the rows below (above, for U).
+// as T) <- U^{-1} L^{-1} Y.  64-row blocks: each wave substitutes two
+// right-hand sides within the block (one row per lane, the solved row
+// broadcast by readlane), then all threads fold the block into the rows
+// below (above, for U).
 template <typename T>
 FLX_DEV void wg_trs(const T *__restrict__ A, int n, int nrhs, BShared &sh)
 {
@@ -446,12 +447,14 @@ FLX_DEV void wg_trs(const T *__restrict__ A, int n, int nrhs, BShared &sh)
         for (int bb = 0; bb < nbk; ++bb) {
             const int blk = lower ? bb : nbk - 1 - bb;
             const int r0 = blk * 64, rows = min(64, n - r0);
-            if (wave == 0) {
-                const int i = r0 + lane;
+            // wave w substitutes right-hand sides RW w .. RW w + RW - 1
+            constexpr int RW = BRHS / 4;
+            if (RW * wave < nrhs) {
+                const int i = r0 + lane, cw = RW * wave;
                 const bool rok = lane < rows;
-                T v[BRHS];
+                T v[RW];
 #pragma unroll
-                for (int c = 0; c < BRHS; ++c) v[c] = (rok && c < nrhs) ? y[i * BRHS + c] : T(0);
+                for (int c = 0; c < RW; ++c) v[c] = (rok && cw + c < nrhs) ? y[i * BRHS + cw + c] : T(0);
                 // step s handles block column q = s (L, forwards) or
                 // q = 63 - s (U, backwards); this lane's entry of column q:
                 // below the diagonal for L, on and above it for U
@@ -466,12 +469,12 @@ FLX_DEV void wg_trs(const T *__restrict__ A, int n, int nrhs, BShared &sh)
                         const T d = readlane_t(a, q);
                         if (lane == q) {
 #pragma unroll
-                            for (int c = 0; c < BRHS; ++c) v[c] = v[c] / d;
+                            for (int c = 0; c < RW; ++c) v[c] = v[c] / d;
                         }
                     }
                     const T m = (lower || lane < q) ? a : T(0);
 #pragma unroll
-                    for (int c = 0; c < BRHS; ++c) {
+                    for (int c = 0; c < RW; ++c) {
                         const T xq = readlane_t(v[c], q);
                         v[c] = v[c] - m * xq;
                     }
@@ -494,8 +497,8 @@ FLX_DEV void wg_trs(const T *__restrict__ A, int n, int nrhs, BShared &sh)
                 }
                 if (rok)
 #pragma unroll
-                    for (int c = 0; c < BRHS; ++c)
-                        if (c < nrhs) y[i * BRHS + c] = v[c];
+                    for (int c = 0; c < RW; ++c)
+                        if (cw + c < nrhs) y[i * BRHS + cw + c] = v[c];
             }
             __syncthreads();
             // fold into the remaining rows: below (L) or above (U), the
