@@ -845,7 +845,7 @@ struct GemmProb {
 };
 
 // C tile (TM x TN) per 256-thread workgroup pass; 4 waves of (TM/2 x TN/2),
-// 16x16x4 FP64 MFMAs with operands read from L2.
+// 16x16x4 FP64 MFMAs with operands read from L2 (no LDS).
 template <int TM, int TN>
 __global__ __launch_bounds__(256) void batched_gemm_kernel(const GemmProb *__restrict__ probs,
                                                            int nprob, int ntiles,
@@ -882,24 +882,47 @@ __global__ __launch_bounds__(256) void batched_gemm_kernel(const GemmProb *__res
         for (int x = 0; x < FI; ++x)
 #pragma unroll
             for (int y = 0; y < FJ; ++y) acc[x][y] = double4_t{0.0, 0.0, 0.0, 0.0};
-        for (int kk = 0; kk < p.k; kk += 4) {
-            const int kq = kk + lk;
-            double fa[FI], fb[FJ];
+        // K in batches of KU MFMA steps (4 each), two batches in flight: the
+        // next batch's fragments are requested before this batch's MFMAs
+        constexpr int KU = 4;
+        double fa[2][KU][FI], fb[2][KU][FJ];
+        auto fetch = [&](int buf, int kb) {
 #pragma unroll
-            for (int x = 0; x < FI; ++x) {
-                const int r = i0 + x * 16 + lr;
-                fa[x] = (r < p.m && kq < p.k) ? a[(long)kq * p.m + r] : 0.0;
+            for (int u = 0; u < KU; ++u) {
+                const int kq = kb + 4 * u + lk;
+#pragma unroll
+                for (int x = 0; x < FI; ++x) {
+                    const int r = i0 + x * 16 + lr;
+                    fa[buf][u][x] = (r < p.m && kq < p.k) ? a[(long)kq * p.m + r] : 0.0;
+                }
+#pragma unroll
+                for (int y = 0; y < FJ; ++y) {
+                    const int cc = j0 + y * 16 + lr;
+                    fb[buf][u][y] = (cc < p.n && kq < p.k) ? b[(long)cc * p.k + kq] : 0.0;
+                }
             }
+        };
+        auto steps = [&](int buf, int kb) {
 #pragma unroll
-            for (int y = 0; y < FJ; ++y) {
-                const int cc = j0 + y * 16 + lr;
-                fb[y] = (cc < p.n && kq < p.k) ? b[(long)cc * p.k + kq] : 0.0;
+            for (int u = 0; u < KU; ++u) {
+                if (kb + 4 * u < p.k) {            // uniform
+#pragma unroll
+                    for (int x = 0; x < FI; ++x)
+#pragma unroll
+                        for (int y = 0; y < FJ; ++y)
+                            acc[x][y] = mfma_t(fa[buf][u][x], fb[buf][u][y], acc[x][y]);
+                }
             }
-#pragma unroll
-            for (int x = 0; x < FI; ++x)
-#pragma unroll
-                for (int y = 0; y < FJ; ++y) acc[x][y] = mfma_t(fa[x], fb[y], acc[x][y]);
+        };
+        fetch(0, 0);
+        for (int kk = 0; kk < p.k; kk += 8 * KU) {
+            fetch(1, kk + 4 * KU);
+            steps(0, kk);
+            fetch(0, kk + 8 * KU);
+            steps(1, kk + 4 * KU);
         }
+        // C: every entry of the tile loaded before any is written
+        double cv[FI][FJ][4];
 #pragma unroll
         for (int x = 0; x < FI; ++x)
 #pragma unroll
@@ -908,10 +931,19 @@ __global__ __launch_bounds__(256) void batched_gemm_kernel(const GemmProb *__res
 #pragma unroll
                 for (int q = 0; q < 4; ++q) {
                     const int row = i0 + x * 16 + lk + 4 * q;
-                    if (row < p.m && col < p.n) {
-                        double *cp = c + (long)col * p.m + row;
-                        *cp = p.beta * *cp + p.alpha * acc[x][y][q];
-                    }
+                    cv[x][y][q] = (row < p.m && col < p.n) ? c[(long)col * p.m + row] : 0.0;
+                }
+            }
+#pragma unroll
+        for (int x = 0; x < FI; ++x)
+#pragma unroll
+            for (int y = 0; y < FJ; ++y) {
+                const int col = j0 + y * 16 + lr;
+#pragma unroll
+                for (int q = 0; q < 4; ++q) {
+                    const int row = i0 + x * 16 + lk + 4 * q;
+                    if (row < p.m && col < p.n)
+                        c[(long)col * p.m + row] = p.beta * cv[x][y][q] + p.alpha * acc[x][y][q];
                 }
             }
     }
