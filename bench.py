@@ -418,7 +418,7 @@ def main():
         return
     if args.gpus != 1:
         print(f"bench.py measures one GPU only: --gpus {args.gpus} is not supported "
-              f"(the LU path is single-GPU; see DESIGN.md for how it would shard)",
+              f"(the LU path runs on one GPU; no multi-GPU path is built)",
               file=sys.stderr)
         sys.exit(2)
 

@@ -1,7 +1,7 @@
 """GPU parity for the FP64 MFMA GEMM (flx_dgemm / cblas_dgemm) against the
 oracle's restatement of cxxblas gemm_generic (cxxblas/level3/gemm.tcc:149-299).
 
-Bar (BASELINE north star): |C_gpu - C_ref| <= 4*K*eps64*(|alpha||A||B| +
+Bar: |C_gpu - C_ref| <= 4*K*eps64*(|alpha||A||B| +
 |beta||C|) elementwise, eps64 = 2^-53; netlib DMMCH ratio < 16 over the
 dblat3 grid; bit-exact where the arithmetic is exact (integer data,
 alpha = 0); the reference's beta = 0 convention on the flx path.

@@ -2,7 +2,7 @@
 the dgetrf_ / dgetrs_ / dgesv_ / dlaswp_ seams) against the oracle and the
 reference's own fixtures.
 
-Bar (BASELINE north star): pivot vectors identical to the reference's on
+Bar: pivot vectors identical to the reference's on
 columns without near-ties (the first mismatch must sit on a column whose
 relative top-2 gap is below TAU), LAPACK dget01 ratio
 ||PA - LU||_1 / (n ||A||_1 eps) < 30, dget02 solve ratio < 30, INFO and
