@@ -755,12 +755,12 @@ __global__ __launch_bounds__(BT, 2) void batched_gesv_kernel(const BatchProb *__
                         double acc[BRHS];
 #pragma unroll
                         for (int c = 0; c < BRHS; ++c) acc[c] = (c < nrhs) ? bx[(long)c * n + i] : 0.0;
-                        for (int j0 = 0; j0 < n; j0 += 8) {
-                            double av[8];
+                        for (int j0 = 0; j0 < n; j0 += 16) {
+                            double av[16];
 #pragma unroll
-                            for (int u = 0; u < 8; ++u) av[u] = (j0 + u < n) ? a[(long)(j0 + u) * n + i] : 0.0;
+                            for (int u = 0; u < 16; ++u) av[u] = (j0 + u < n) ? a[(long)(j0 + u) * n + i] : 0.0;
 #pragma unroll
-                            for (int u = 0; u < 8; ++u) {
+                            for (int u = 0; u < 16; ++u) {
                                 if (j0 + u < n) {
 #pragma unroll
                                     for (int c = 0; c < BRHS; ++c)
