@@ -392,6 +392,14 @@ def selector_section(train_count: int = 3000, eval_count: int = 3000):
     return res
 
 
+# HBM bytes per launch of the trailing-update GEMM in the default workload
+# (n = 8192, nrhs = 16, nb = 128, ib = 32): rocprofv3 --pmc FETCH_SIZE and
+# WRITE_SIZE over 875 launches, averaged, scaled by what the same counters
+# read for a transpose of known size in the same run (fetched = 2.0 x
+# FETCH_SIZE, written = 1.0 x WRITE_SIZE): (2 x 88665.4 + 87862.1) KiB.
+LU8192_GEMM_TRAFFIC_BYTES = (2.0 * 88665.4 + 87862.1) * 1024.0
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -520,7 +528,9 @@ def main():
         "backward_error": eta,
         "backward_error_bound": n * eps64,
         "roofline": {"bound": "mfma", "achieved": achieved, "peak": PEAK_FP64_MFMA,
-                     "unit": "TFLOP/s", "frac": achieved / PEAK_FP64_MFMA, "traffic": None,
+                     "unit": "TFLOP/s", "frac": achieved / PEAK_FP64_MFMA,
+                     "traffic": (LU8192_GEMM_TRAFFIC_BYTES
+                                 if (n, nrhs, args.nb, args.ib) == (8192, 16, 128, 32) else None),
                      "kernel": "gemm_mfma_kernel<double,0,0> (trailing update)",
                      "launches": g["launches"], "avg_launch_us": 1e3 * g["ms"] / max(1, g["launches"])},
         "breakdown": breakdown,
