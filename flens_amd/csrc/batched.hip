@@ -68,11 +68,11 @@ FLX_DEV float4_t mfma_t(float a, float b, float4_t c)
 template <typename T> struct Acc4;
 template <> struct Acc4<double> {
     typedef double4_t type;
-    static FLX_DEV int row(int lk, int q) { return lk + 4 * q; }
+    FLX_DEV int row(int lk, int q) { return lk + 4 * q; }
 };
 template <> struct Acc4<float> {
     typedef float4_t type;
-    static FLX_DEV int row(int lk, int q) { return 4 * lk + q; }
+    FLX_DEV int row(int lk, int q) { return 4 * lk + q; }
 };
 
 // shared memory of one workgroup (the solve buffers double as scratch)
