@@ -24,7 +24,7 @@ hipError_t ensure(flx_context *h, DevBuf &b, size_t bytes)
     if (b.p) {
         hipError_t e = hipStreamSynchronize(h->stream);
         if (e != hipSuccess) return e;
-        hipFree(b.p);
+        (void)hipFree(b.p);
         b.p = nullptr;
         b.cap = 0;
     }
@@ -80,7 +80,7 @@ extern "C" int flx_create(flx_handle *out)
 
 void free_buf(DevBuf &b)
 {
-    if (b.p) hipFree(b.p);
+    if (b.p) (void)hipFree(b.p);
     b.p = nullptr;
     b.cap = 0;
 }
@@ -88,7 +88,7 @@ void free_buf(DevBuf &b)
 extern "C" int flx_destroy(flx_handle h)
 {
     if (!h) return FLX_INVALID_ARG;
-    hipStreamSynchronize(h->stream);
+    (void)hipStreamSynchronize(h->stream);
     free_buf(h->W); free_buf(h->A); free_buf(h->B); free_buf(h->C); free_buf(h->X);
     free_buf(h->piv0); free_buf(h->ipiv); free_buf(h->info); free_buf(h->perm);
     free_buf(h->tinv); free_buf(h->sb); free_buf(h->sx); free_buf(h->mlp_part);
@@ -105,20 +105,20 @@ extern "C" int flx_destroy(flx_handle h)
                           &y.perm0, &y.pf, &y.pb})
             free_buf(*b);
     }
-    if (x.side) hipStreamDestroy(x.side);
-    if (x.ev_a) hipEventDestroy(x.ev_a);
-    if (x.ev_r) hipEventDestroy(x.ev_r);
-    if (x.graph) hipGraphExecDestroy(x.graph);
-    if (x.gstream) hipStreamDestroy(x.gstream);
-    if (x.ev0) hipEventDestroy(x.ev0);
-    if (x.ev1) hipEventDestroy(x.ev1);
-    if (h->wk.exch) hipFree(h->wk.exch);
-    if (h->wk.list) hipFree(h->wk.list);
-    if (h->wk.flags) hipFree(h->wk.flags);
-    if (h->wk.moved) hipFree(h->wk.moved);
-    if (h->wk.side) hipStreamDestroy(h->wk.side);
-    if (h->wk.ev_a) hipEventDestroy(h->wk.ev_a);
-    if (h->wk.ev_r) hipEventDestroy(h->wk.ev_r);
+    if (x.side) (void)hipStreamDestroy(x.side);
+    if (x.ev_a) (void)hipEventDestroy(x.ev_a);
+    if (x.ev_r) (void)hipEventDestroy(x.ev_r);
+    if (x.graph) (void)hipGraphExecDestroy(x.graph);
+    if (x.gstream) (void)hipStreamDestroy(x.gstream);
+    if (x.ev0) (void)hipEventDestroy(x.ev0);
+    if (x.ev1) (void)hipEventDestroy(x.ev1);
+    if (h->wk.exch) (void)hipFree(h->wk.exch);
+    if (h->wk.list) (void)hipFree(h->wk.list);
+    if (h->wk.flags) (void)hipFree(h->wk.flags);
+    if (h->wk.moved) (void)hipFree(h->wk.moved);
+    if (h->wk.side) (void)hipStreamDestroy(h->wk.side);
+    if (h->wk.ev_a) (void)hipEventDestroy(h->wk.ev_a);
+    if (h->wk.ev_r) (void)hipEventDestroy(h->wk.ev_r);
     free_buf(h->lu_inv); free_buf(h->lu_mv);
     free_buf(h->lu_stage); free_buf(h->lu_sdst); free_buf(h->lu_sperm);
     free_buf(h->sr); free_buf(h->sc);
@@ -171,8 +171,8 @@ extern "C" int flx_sync(flx_handle h)
         int f = 0;
         FLX_TRY(hipMemcpy(&f, h->wk.flags, sizeof(int), hipMemcpyDeviceToHost));
         if (f) {
-            int z = 0;
-            hipMemcpy(h->wk.flags, &z, sizeof(int), hipMemcpyHostToDevice);
+            int z = 0;   // re-armed for the next call; the timeout is what is reported
+            (void)hipMemcpy(h->wk.flags, &z, sizeof(int), hipMemcpyHostToDevice);
             return FLX_SYNC_TIMEOUT;
         }
     }

@@ -83,17 +83,17 @@ static int capture(flx_batch *p)
         FLX_TRY(hipEventCreateWithFlags(&p->ev1, hipEventDisableTiming));
     }
     if (p->exec) {
-        hipGraphExecDestroy(p->exec);
+        (void)hipGraphExecDestroy(p->exec);
         p->exec = nullptr;
     }
     hipGraph_t g = nullptr;
     FLX_TRY(hipStreamBeginCapture(p->gs, hipStreamCaptureModeThreadLocal));
     hipError_t e = launch(p, p->gs, p->tile_cfg, p->grid, p->A, p->B, p->GC);
     hipError_t e2 = hipStreamEndCapture(p->gs, &g);
-    if (e != hipSuccess) { if (g) hipGraphDestroy(g); return hip_status(e); }
+    if (e != hipSuccess) { if (g) (void)hipGraphDestroy(g); return hip_status(e); }
     if (e2 != hipSuccess) return hip_status(e2);
     e = hipGraphInstantiate(&p->exec, g, nullptr, nullptr, 0);
-    hipGraphDestroy(g);
+    (void)hipGraphDestroy(g);
     if (e != hipSuccess) { p->exec = nullptr; return hip_status(e); }
     return FLX_SUCCESS;
 }
@@ -149,11 +149,11 @@ static int autotune(flx_batch *p, bool tune)
             }
             if (ms_best < best) { best = ms_best; bcfg = cfg; bgrid = grid; }
         }
-    hipEventDestroy(e0);
-    hipEventDestroy(e1);
-    if (tA) hipFree(tA);
-    if (tB) hipFree(tB);
-    if (tC) hipFree(tC);
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    if (tA) (void)hipFree(tA);
+    if (tB) (void)hipFree(tB);
+    if (tC) (void)hipFree(tC);
     p->tile_cfg = bcfg;
     p->grid = bgrid;
     p->tuned_ms = best;
@@ -307,17 +307,17 @@ extern "C" int flx_batch_config(flx_batch_t p, int *cfg, double *tuned_ms)
 extern "C" int flx_batch_destroy(flx_batch_t p)
 {
     if (!p) return FLX_INVALID_ARG;
-    if (p->gs) hipStreamSynchronize(p->gs);
-    if (p->exec) hipGraphExecDestroy(p->exec);
-    if (p->gs) hipStreamDestroy(p->gs);
-    if (p->ev0) hipEventDestroy(p->ev0);
-    if (p->ev1) hipEventDestroy(p->ev1);
-    if (p->probs) hipFree(p->probs);
+    if (p->gs) (void)hipStreamSynchronize(p->gs);
+    if (p->exec) (void)hipGraphExecDestroy(p->exec);
+    if (p->gs) (void)hipStreamDestroy(p->gs);
+    if (p->ev0) (void)hipEventDestroy(p->ev0);
+    if (p->ev1) (void)hipEventDestroy(p->ev1);
+    if (p->probs) (void)hipFree(p->probs);
     for (void *g : p->gprobs)
-        if (g) hipFree(g);
-    if (p->counter) hipFree(p->counter);
-    if (p->W32) hipFree(p->W32);
-    if (p->R) hipFree(p->R);
+        if (g) (void)hipFree(g);
+    if (p->counter) (void)hipFree(p->counter);
+    if (p->W32) (void)hipFree(p->W32);
+    if (p->R) (void)hipFree(p->R);
     delete p;
     return FLX_SUCCESS;
 }

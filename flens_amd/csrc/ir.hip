@@ -206,32 +206,27 @@ constexpr int TV = 64;
 constexpr int TV_FETCH = 8;
 constexpr int TV_SPIN_LIMIT = 1 << 22;
 
-FLX_DEV uint64_t tv_word(unsigned epoch, float x)
-{
-    return ((uint64_t)epoch << 32) | (uint64_t)__float_as_uint(x);
-}
-
 // Published solution words: a float travels as one tagged word, a double
 // as two (high and low halves), each (epoch << 32 | 32 payload bits).
 template <typename TY> struct TvWords;
 template <> struct TvWords<float> {
     static constexpr int N = 1;
-    static FLX_DEV void put(uint64_t *p, unsigned e, float v)
+    FLX_DEV void put(uint64_t *p, unsigned e, float v)
     {
         __hip_atomic_store(p, ((uint64_t)e << 32) | __float_as_uint(v), __ATOMIC_RELAXED,
                            __HIP_MEMORY_SCOPE_AGENT);
     }
-    static FLX_DEV bool get(const uint64_t *p, unsigned e, float &v)
+    FLX_DEV bool get(const uint64_t *p, unsigned e, float &v)
     {
         const uint64_t w = __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         v = __uint_as_float((unsigned)(w & 0xffffffffull));
         return (unsigned)(w >> 32) == e;
     }
-    static FLX_DEV float readlane(float v, int l) { return readlane_f(v, l); }
+    FLX_DEV float readlane(float v, int l) { return readlane_f(v, l); }
 };
 template <> struct TvWords<double> {
     static constexpr int N = 2;
-    static FLX_DEV void put(uint64_t *p, unsigned e, double v)
+    FLX_DEV void put(uint64_t *p, unsigned e, double v)
     {
         const uint64_t b = (uint64_t)__double_as_longlong(v);
         __hip_atomic_store(p, ((uint64_t)e << 32) | (b >> 32), __ATOMIC_RELAXED,
@@ -239,14 +234,14 @@ template <> struct TvWords<double> {
         __hip_atomic_store(p + 1, ((uint64_t)e << 32) | (b & 0xffffffffull), __ATOMIC_RELAXED,
                            __HIP_MEMORY_SCOPE_AGENT);
     }
-    static FLX_DEV bool get(const uint64_t *p, unsigned e, double &v)
+    FLX_DEV bool get(const uint64_t *p, unsigned e, double &v)
     {
         const uint64_t hi = __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         const uint64_t lo = __hip_atomic_load(p + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         v = __longlong_as_double((long long)(((hi & 0xffffffffull) << 32) | (lo & 0xffffffffull)));
         return (unsigned)(hi >> 32) == e && (unsigned)(lo >> 32) == e;
     }
-    static FLX_DEV double readlane(double v, int l)
+    FLX_DEV double readlane(double v, int l)
     {
         const long long b = __double_as_longlong(v);
         const int lo = __builtin_amdgcn_readlane((int)(b & 0xffffffffll), l);

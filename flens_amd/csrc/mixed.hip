@@ -52,16 +52,6 @@ static_assert(TS_U11 == MIX_U11_FLOATS, "u11 layout");
 static_assert(TS_B % 32 == 0 && MIX_STAGE_LD % 32 == 0, "float4 chunks of the stage");
 static_assert(TS_R == 256, "the selection key holds the thread index in 8 bits");
 
-FLX_DEV uint64_t wave_max_u64(uint64_t v)
-{
-#pragma unroll
-    for (int o = 32; o > 0; o >>= 1) {
-        uint64_t u = __shfl_xor(v, o, 64);
-        v = u > v ? u : v;
-    }
-    return v;
-}
-
 // 256 threads, one row each.  Selection key (32 bits): |v| with its low 8
 // mantissa bits replaced by 255 - thread, so the wave maximum is one DPP
 // reduction and ties go to the lower thread.  Dropping those bits lets a
@@ -651,12 +641,12 @@ static TsTree tslu_tree(int mp)
 template <typename V>
 FLX_DEV float4_t mfma_lp(V a, V b, float4_t c);
 template <>
-FLX_DEV float4_t mfma_lp<bf16x8_t>(bf16x8_t a, bf16x8_t b, float4_t c)
+__device__ __forceinline__ float4_t mfma_lp<bf16x8_t>(bf16x8_t a, bf16x8_t b, float4_t c)
 {
     return __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, c, 0, 0, 0);
 }
 template <>
-FLX_DEV float4_t mfma_lp<f16x8_t>(f16x8_t a, f16x8_t b, float4_t c)
+__device__ __forceinline__ float4_t mfma_lp<f16x8_t>(f16x8_t a, f16x8_t b, float4_t c)
 {
     return __builtin_amdgcn_mfma_f32_16x16x32_f16(a, b, c, 0, 0, 0);
 }
@@ -768,8 +758,8 @@ __global__ __launch_bounds__(256) void absmax_rm_kernel(const float *__restrict_
 }
 
 template <typename E> FLX_DEV E to_lp(float v);
-template <> FLX_DEV __bf16 to_lp<__bf16>(float v) { return (__bf16)v; }
-template <> FLX_DEV _Float16 to_lp<_Float16>(float v) { return (_Float16)v; }
+template <> __device__ __forceinline__ __bf16 to_lp<__bf16>(float v) { return (__bf16)v; }
+template <> __device__ __forceinline__ _Float16 to_lp<_Float16>(float v) { return (_Float16)v; }
 
 // La[i][k] = -L[i][k] * s  (rows i < nr_pad, k < Kp; zero outside nr x kk)
 template <typename E>

@@ -35,7 +35,7 @@ Profiler *prof_new() { return new Profiler(); }
 void prof_delete(Profiler *p)
 {
     if (!p) return;
-    for (hipEvent_t e : p->pool) hipEventDestroy(e);
+    for (hipEvent_t e : p->pool) (void)hipEventDestroy(e);
     delete p;
 }
 
@@ -57,7 +57,7 @@ void *prof_begin(Profiler *p, hipStream_t s, int cls, double flops, double bytes
     r.cls = cls;
     r.flops = flops;
     r.bytes = bytes;
-    hipEventRecord(r.a, s);
+    (void)hipEventRecord(r.a, s);
     p->recs.push_back(r);
     return (void *)(uintptr_t)p->recs.size();   // 1-based index
 }
@@ -65,7 +65,7 @@ void *prof_begin(Profiler *p, hipStream_t s, int cls, double flops, double bytes
 void prof_end(Profiler *p, hipStream_t s, void *rec)
 {
     size_t i = (size_t)(uintptr_t)rec - 1;
-    hipEventRecord(p->recs[i].b, s);
+    (void)hipEventRecord(p->recs[i].b, s);
 }
 
 // out[cls*4 + {0,1,2,3}] = {launches, milliseconds, flops, bytes}
@@ -74,7 +74,7 @@ void prof_collect(Profiler *p, double *out)
     for (int i = 0; i < PROF_NCLS * 4; ++i) out[i] = 0;
     for (const ProfRec &r : p->recs) {
         float ms = 0;
-        hipEventElapsedTime(&ms, r.a, r.b);
+        (void)hipEventElapsedTime(&ms, r.a, r.b);
         out[r.cls * 4 + 0] += 1;
         out[r.cls * 4 + 1] += ms;
         out[r.cls * 4 + 2] += r.flops;

@@ -181,7 +181,7 @@ struct Solver {
         if (!m.graph || memcmp(key, m.gkey, sizeof key) != 0) {
             if (m.graph) {
                 FLX_TRY(hipStreamSynchronize(m.gstream));
-                hipGraphExecDestroy(m.graph);
+                (void)hipGraphExecDestroy(m.graph);
                 m.graph = nullptr;
             }
             hipGraph_t g = nullptr;
@@ -189,11 +189,11 @@ struct Solver {
             hipError_t e = getrf_rm_mixed(m.gstream, mw, (float *)m.W32.p, ldw, n, o.prec, o.nb);
             hipGraph_t gg = nullptr;
             hipError_t e2 = hipStreamEndCapture(m.gstream, &gg);
-            if (e != hipSuccess) { if (gg) hipGraphDestroy(gg); return hip_status(e); }
+            if (e != hipSuccess) { if (gg) (void)hipGraphDestroy(gg); return hip_status(e); }
             if (e2 != hipSuccess) return hip_status(e2);
             g = gg;
             e = hipGraphInstantiate(&m.graph, g, nullptr, nullptr, 0);
-            hipGraphDestroy(g);
+            (void)hipGraphDestroy(g);
             if (e != hipSuccess) { m.graph = nullptr; return hip_status(e); }
             memcpy(m.gkey, key, sizeof key);
         }
