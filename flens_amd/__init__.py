@@ -40,7 +40,7 @@ EXPORTS = [
     "flx_create", "flx_destroy", "flx_set_stream", "flx_sync", "flx_status_string",
     "flx_set_lu_blocking", "flx_set_lu_lookahead", "flx_set_lu_side_groups", "flx_dgemm", "flx_dgetrf", "flx_dgetrs", "flx_dgesv",
     "flx_dlaswp", "flx_fill_uniform", "flx_profile_start", "flx_profile_stop",
-    "flx_profile_classes",
+    "flx_profile_classes", "flx_test_raise_sync_flag",
     "flx_dgesv_mixed", "flx_dsgesv",
     "flx_batch_gesv_create", "flx_batch_gemm_create", "flx_batch_execute", "flx_batch_config",
     "flx_batch_destroy", "flx_mlp_forward", "flx_mlp_grad", "flx_mlp_adam",
@@ -130,6 +130,7 @@ def lib():
     L.dgesvx_.argtypes = [_cp, _cp, _ip, _ip, _dp, _ip, _dp, _ip, _ip, _cp, _dp, _dp, _dp, _ip,
                           _dp, _ip, _dp, _dp, _dp, _dp, _ip, _ip]
     L.flx_set_lu_lookahead.argtypes = [_vp, _i]
+    L.flx_test_raise_sync_flag.argtypes = [_vp]
     L.flx_set_lu_side_groups.argtypes = [_vp, _i, _i]
     L.flx_dgemm.argtypes = [_vp, _i, _i, _i, _i, _i, _d, _vp, _i, _vp, _i, _d, _vp, _i]
     L.flx_dgetrf.argtypes = [_vp, _i, _i, _vp, _i, _vp, _vp]

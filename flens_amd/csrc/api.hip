@@ -97,7 +97,7 @@ extern "C" int flx_destroy(flx_handle h)
     for (DevBuf *b : {&x.W32, &x.perm, &x.cand, &x.ccount, &x.counters, &x.u11, &x.win, &x.list,
                       &x.zero, &x.lpmax, &x.la, &x.ub, &x.inv_l, &x.inv_u, &x.y32, &x.y64, &x.flags,
                       &x.err, &x.dr, &x.dc, &x.part, &x.scal, &x.R, &x.V, &x.w, &x.t, &x.zeros,
-                      &x.mpart, &x.hbuf, &x.inv, &x.mv, &x.stage, &x.stage_perm})
+                      &x.mpart, &x.hbuf, &x.inv, &x.mv, &x.stage, &x.stage_perm, &x.F})
         free_buf(*b);
     {
         AuxBufs &y = h->aux;
@@ -179,6 +179,14 @@ extern "C" int flx_sync(flx_handle h)
     return FLX_SUCCESS;
 }
 
+extern "C" int flx_test_raise_sync_flag(flx_handle h)
+{
+    if (!h) return FLX_INVALID_ARG;
+    FLX_TRY(init_work(h));
+    FLX_TRY(fill_int(h->stream, h->wk.flags, 1, 1));
+    return FLX_SUCCESS;
+}
+
 extern "C" int flx_profile_start(flx_handle h)
 {
     if (!h) return FLX_INVALID_ARG;
@@ -248,7 +256,7 @@ extern "C" int flx_dgemm(flx_handle h, int transa, int transb, int m, int n, int
 
 // LU of device column-major A via the row-major working copy
 int dgetrf_dev(flx_context *h, int m, int n, double *A, int lda, int *ipiv, int *info,
-                      bool keep_perm)
+               bool keep_perm, double *out, int ldo)
 {
     const int mn = m < n ? m : n;
     FLX_TRY(init_work(h));
@@ -277,7 +285,8 @@ int dgetrf_dev(flx_context *h, int m, int n, double *A, int lda, int *ipiv, int 
     FLX_TRY(fill_int(h->stream, info, 1, INT_MAX));
     FLX_TRY(transpose_f64(h->stream, A, lda, W, ldw, m, n));
     FLX_TRY(getrf_rm_f64(h->stream, h->wk, W, ldw, m, n, (int *)h->piv0.p, info, h->opt));
-    FLX_TRY(transpose_f64(h->stream, W, ldw, A, lda, n, m));
+    if (out) FLX_TRY(transpose_f64(h->stream, W, ldw, out, ldo, n, m));
+    else FLX_TRY(transpose_f64(h->stream, W, ldw, A, lda, n, m));
     FLX_TRY(piv0_to_ipiv(h->stream, (int *)h->piv0.p, ipiv, mn, info));
     return FLX_SUCCESS;
 }
@@ -361,7 +370,7 @@ extern "C" int flx_dgesv(flx_handle h, int n, int nrhs, double *A, int lda, int 
     if (st) return st;
     int info = 0;
     FLX_TRY(hipMemcpyAsync(&info, dinfo, sizeof(int), hipMemcpyDeviceToHost, h->stream));
-    FLX_TRY(hipStreamSynchronize(h->stream));
+    FLX_RET(flx_sync(h));   // INFO is only meaningful when every panel wait succeeded
     if (info_host) *info_host = info;
     if (info == 0 && nrhs > 0)
         return dgetrs_dev(h, 0, n, nrhs, A, lda, ipiv, (const int *)h->perm.p, B, ldb);
@@ -483,7 +492,7 @@ static int host_gemm(int ta, int tb, int M, int N, int K, double alpha, const do
     FLX_TRY(gemm_f64(h->stream, ta, tb, M, N, need_ab ? K : 0, alpha, (const double *)h->A.p,
                      ar > 1 ? ar : 1, (const double *)h->B.p, br > 1 ? br : 1, beta,
                      (double *)h->C.p, M > 1 ? M : 1, blas_beta0));
-    FLX_TRY(hipStreamSynchronize(h->stream));
+    FLX_RET(flx_sync(h));
     FLX_TRY(d2h(C, h->C.p, M, N, ldc));
     return FLX_SUCCESS;
 }
@@ -533,8 +542,6 @@ static int host_trsm(int right, int upper, int trans, int unit, int M, int N, do
     FLX_TRY(h2d(h->A.p, A, na, na, lda));
     FLX_TRY(h2d(h->C.p, B, M, N, ldb));
     double *dB = (double *)h->C.p, *dT = (double *)h->W.p;
-    if (alpha != 1.0)
-        FLX_TRY(gemm_f64(h->stream, 0, 0, M, N, 0, 0.0, nullptr, 1, nullptr, 1, alpha, dB, M, 0));
     if (!right) {
         // X = op(A)^{-1} B : solve consumes dT copy, writes dB
         FLX_TRY(copy_cm(h->stream, dB, M, dT, M, M, N));
@@ -548,7 +555,11 @@ static int host_trsm(int right, int upper, int trans, int unit, int M, int N, do
         FLX_TRY(transpose_f64(h->stream, dB, N, dT, M, N, M));       // dT = X (M x N)
         FLX_TRY(copy_cm(h->stream, dT, M, dB, M, M, N));
     }
-    FLX_TRY(hipStreamSynchronize(h->stream));
+    // alpha after the solve, as trsm_generic's closing gescal
+    // (cxxblas/level3/trsm.tcc:67): B <- alpha * X, also for alpha = 0
+    if (alpha != 1.0)
+        FLX_TRY(gemm_f64(h->stream, 0, 0, M, N, 0, 0.0, nullptr, 1, nullptr, 1, alpha, dB, M, 0));
+    FLX_RET(flx_sync(h));
     FLX_TRY(d2h(B, h->C.p, M, N, ldb));
     return FLX_SUCCESS;
 }
@@ -649,7 +660,7 @@ extern "C" void dgetrs_(const char *TRANS, const int *N, const int *NRHS, const 
             int s = dgetrs_dev(h, t != 'N', n, nrhs, (const double *)h->A.p, n,
                                (const int *)h->ipiv.p, nullptr, (double *)h->C.p, n);
             if (s) return s;
-            FLX_TRY(hipStreamSynchronize(h->stream));
+            FLX_RET(flx_sync(h));
             FLX_TRY(d2h(B, h->C.p, n, nrhs, *LDB));
             return FLX_SUCCESS;
         };
@@ -769,7 +780,7 @@ extern "C" void dlaswp_(const int *N, double *A, const int *LDA, const int *K1, 
             FLX_TRY(hipMemcpy(h->ipiv.p, IPIV, sizeof(int) * k2, hipMemcpyHostToDevice));
             FLX_TRY(laswp_cm_f64(h->stream, (double *)h->A.p, rows, n, k1, k2,
                                  (const int *)h->ipiv.p, incx > 0 ? 1 : -1));
-            FLX_TRY(hipStreamSynchronize(h->stream));
+            FLX_RET(flx_sync(h));
             FLX_TRY(d2h(A, h->A.p, rows, n, *LDA));
             return FLX_SUCCESS;
         };
@@ -831,7 +842,7 @@ extern "C" void dgeequ_(const int *M, const int *N, const double *A, const int *
             int s = flx_dgeequ(h, m, n, (const double *)h->A.p, m > 1 ? m : 1, (double *)h->sr.p,
                                (double *)h->sc.p, ROWCND, COLCND, AMAX, &info);
             if (s) return s;
-            FLX_TRY(hipStreamSynchronize(h->stream));
+            FLX_RET(flx_sync(h));
             *INFO = info;
             if (m > 0 && n > 0) {
                 // R holds the row maxima even when a zero row stops the routine
@@ -924,7 +935,7 @@ extern "C" void dgerfs_(const char *TRANS, const int *N, const int *NRHS, const 
                                (const double *)h->sb.p, n, (const int *)h->ipiv.p,
                                (const double *)h->C.p, n, (double *)h->sx.p, n, FERR, BERR);
             if (s) return s;
-            FLX_TRY(hipStreamSynchronize(h->stream));
+            FLX_RET(flx_sync(h));
             FLX_TRY(d2h(X, h->sx.p, n, nrhs, *LDX));
             return FLX_SUCCESS;
         };
@@ -996,7 +1007,7 @@ extern "C" void dgesvx_(const char *FACT, const char *TRANS, const int *N, const
                                (double *)h->C.p, l, (double *)h->sx.p, l, &rcond, FERR, BERR,
                                &rpvgrw, &info);
             if (s) return s;
-            FLX_TRY(hipStreamSynchronize(h->stream));
+            FLX_RET(flx_sync(h));
             *RCOND = rcond;
             WORK[0] = rpvgrw;
             *INFO = info;

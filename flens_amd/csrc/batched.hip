@@ -141,6 +141,14 @@ FLX_DEV void wg_panel(T *__restrict__ A, int n, int k, int w, int *__restrict__ 
             if (pos[s] >= kj && pos[s] < n) {
                 uint32_t hi, lo;
                 abs_key(vj[s], hi, lo);
+                if (vj[s] != vj[s]) {
+                    // iamax's NaN rule (cxxblas/level1/iamax.tcc:42-58): a NaN
+                    // wins only at the first live position, elsewhere it ranks
+                    // as |0| (the first position, lowest, wins every tie)
+                    const bool first = pos[s] == kj;
+                    hi = first ? 0xfffffffeu : 0u;
+                    lo = first ? 0xffffffffu : 0u;
+                }
                 const bool better = bs < 0 || hi > bhi || (hi == bhi && lo > blo) ||
                                     (hi == bhi && lo == blo && pos[s] < bpos);
                 if (better) { bhi = hi; blo = lo; bpos = pos[s]; bs = s; }

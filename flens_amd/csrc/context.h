@@ -9,6 +9,8 @@ struct DevBuf {
     size_t cap = 0;
 };
 
+constexpr int MIX_GKEY = 22;   // values keying the captured factorisation
+
 // buffers of the mixed-precision solver
 struct MixBufs {
     DevBuf W32, perm, cand, ccount, counters, u11, win, list, zero, lpmax, la, ub;
@@ -17,12 +19,13 @@ struct MixBufs {
     hipStream_t side = nullptr;             // default priority, ordered by events
     hipEvent_t ev_a = nullptr, ev_r = nullptr;
     DevBuf inv_l, inv_u, y32, y64, flags, err, dr, dc, part, scal, R, V, w, t, zeros, mpart, hbuf;
+    DevBuf F;                               // FP64 fallback factors (A kept for the residual)
     unsigned epoch = 0;
     // captured factorisation: valid while the key matches
     hipGraphExec_t graph = nullptr;
     hipStream_t gstream = nullptr;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
-    size_t gkey[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+    size_t gkey[MIX_GKEY] = {};
 };
 
 // buffers of the norm / equilibration / condition / refinement routines
@@ -80,8 +83,9 @@ void free_buf(DevBuf &b);
 inline long round_ld(long n) { return ((n + 31) / 32) * 32; }
 
 // FP64 LU / solve on device column-major data (api.hip)
+// out (ldo): where the factors go (default: over A; else A is left as is)
 int dgetrf_dev(flx_context *h, int m, int n, double *A, int lda, int *ipiv, int *info,
-               bool keep_perm);
+               bool keep_perm, double *out = nullptr, int ldo = 0);
 // perm_fwd / perm_bwd: the row orders of ipiv applied forward / backward
 // (ipiv_to_perm), when the caller already holds them; else derived from ipiv
 int dgetrs_dev(flx_context *h, int trans, int n, int nrhs, const double *A, int lda,

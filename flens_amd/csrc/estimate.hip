@@ -975,7 +975,8 @@ extern "C" int flx_dgecon(flx_handle h, char norm, int n, const double *LU, int 
         anorm < 0.0)
         return FLX_INVALID_ARG;
     Aux a(h);
-    return a.gecon((char)nc, n, LU, lda, anorm, rcond);
+    FLX_RET(a.gecon((char)nc, n, LU, lda, anorm, rcond));
+    return flx_sync(h);   // a bounded wait of the solves that gave up is reported here
 }
 
 extern "C" int flx_dgerfs(flx_handle h, int trans, int n, int nrhs, const double *A, int lda,
@@ -988,7 +989,8 @@ extern "C" int flx_dgerfs(flx_handle h, int trans, int n, int nrhs, const double
         ldb < l || ldx < l)
         return FLX_INVALID_ARG;
     Aux a(h);
-    return a.gerfs(trans ? 1 : 0, n, nrhs, A, lda, AF, ldaf, ipiv, B, ldb, X, ldx, ferr, berr);
+    FLX_RET(a.gerfs(trans ? 1 : 0, n, nrhs, A, lda, AF, ldaf, ipiv, B, ldb, X, ldx, ferr, berr));
+    return flx_sync(h);
 }
 
 extern "C" int flx_dgesvx(flx_handle h, char fact, int trans, int n, int nrhs, double *A,
@@ -1100,7 +1102,7 @@ extern "C" int flx_dgesvx(flx_handle h, char fact, int trans, int n, int nrhs, d
         for (int j = 0; j < nrhs; ++j) ferr[j] /= cnd;
     }
     FLX_TRY(hipGetLastError());
-    FLX_TRY(hipStreamSynchronize(s));
+    FLX_RET(flx_sync(h));
     if (*rcond < EPS64) *info = n + 1;
     return FLX_SUCCESS;
 }

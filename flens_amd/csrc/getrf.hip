@@ -320,8 +320,15 @@ __global__ __launch_bounds__(256) void lu_apply_kernel(double *__restrict__ W, l
 #pragma unroll 1
         for (int k = 0; k < 32; ++k) {
             const double x0 = x[0];
+            // only the real rows below k take the update: the finished
+            // (rotated) entries and the rows beyond the panel just move, so
+            // an Inf / NaN in x0 reaches exactly the rows the reference's
+            // substitution reaches (cxxblas trsv: x_i -= sum_{q<i} L(i,q) x_q)
 #pragma unroll
-            for (int i = 0; i < 31; ++i) x[i] = fma(-s_l[k][i], x0, x[i + 1]);
+            for (int i = 0; i < 31; ++i) {
+                const double t = fma(-s_l[k][i], x0, x[i + 1]);
+                x[i] = (k + 1 + i < w) ? t : x[i + 1];
+            }
             x[31] = x0;
         }
 #pragma unroll

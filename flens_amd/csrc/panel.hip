@@ -56,6 +56,17 @@ struct Cand {
     uint64_t key;
     int pos, rid;
 };
+// The key of a live row at position pos in column j.  The reference's iamax
+// (cxxblas/level1/iamax.tcc:42-58) starts from the first entry's |x| and
+// replaces it only on a strictly larger one, so a NaN is chosen only at the
+// first live position (where it then stays: every key below ~0) and is
+// skipped anywhere else (key 0 loses to every live number, |v| = 0 included).
+FLX_DEV uint64_t pivot_key(double v, int pos, int j)
+{
+    const double a = fabs(v);
+    if (a != a) return pos == j ? ~0ull : 0ull;
+    return (uint64_t)__double_as_longlong(a) + 1;
+}
 FLX_DEV bool cand_gt(uint64_t ka, int pa, uint64_t kb, int pb)
 {
     return ka > kb || (ka == kb && pa < pb);
@@ -105,6 +116,7 @@ __global__ __launch_bounds__(NT) void panel_kernel(
     __shared__ int s_wpos[NWAVE], s_wrid[NWAVE];
     __shared__ int s_win[3];                   // pos, rid, source workgroup
     __shared__ double s_pv;
+    __shared__ int s_ufin;                     // 1: every entry of s_u is finite
     __shared__ int s_piv[W];
     __shared__ int s_cnt;
     __shared__ int s_fail;
@@ -142,8 +154,7 @@ __global__ __launch_bounds__(NT) void panel_kernel(
         int kp = INT_MAX, kr = -1;
 #pragma unroll
         for (int k = 0; k < RPT; ++k) {
-            const uint64_t kk =
-                done[k] ? 0 : (uint64_t)__double_as_longlong(fabs(row[k][0])) + 1;
+            const uint64_t kk = done[k] ? 0 : pivot_key(row[k][0], pos[k], j);
             if (cand_gt(kk, done[k] ? INT_MAX : pos[k], key, kp)) { key = kk; kp = pos[k]; kr = rid[k]; }
         }
         const Cand wb = wave_best(key, kp, kr);
@@ -162,9 +173,15 @@ __global__ __launch_bounds__(NT) void panel_kernel(
             for (int k = 0; k < RPT; ++k)
                 if (rid[k] == br) {
                     s_pv = row[k][0];
+                    bool fin = true;
 #pragma unroll
-                    for (int c = 0; c < W - 1; ++c) s_u[c] = (c + 1 < live) ? row[k][c + 1] : 0.0;
+                    for (int c = 0; c < W - 1; ++c) {
+                        const double uc = (c + 1 < live) ? row[k][c + 1] : 0.0;
+                        fin = fin && __builtin_isfinite(uc);
+                        s_u[c] = uc;
+                    }
                     s_u[W - 1] = 0.0;
+                    s_ufin = fin ? 1 : 0;
                 }
             __syncthreads();
         } else {
@@ -247,8 +264,8 @@ __global__ __launch_bounds__(NT) void panel_kernel(
                 }
                 // the winner's row, shifted by one, finished columns zeroed
                 const int src = nhdr + (bad ? 0 : wg) * rw;
+                double uc = 0.0;
                 if (lane < W) {
-                    double uc = 0.0;
                     if (lane < live)
                         uc = __longlong_as_double((long long)(((uint64_t)s_words[src + 2 * lane] << 32) |
                                                               (uint64_t)s_words[src + 2 * lane + 1]));
@@ -256,12 +273,17 @@ __global__ __launch_bounds__(NT) void panel_kernel(
                     else s_u[lane - 1] = uc;
                     if (lane == W - 1) s_u[W - 1] = 0.0;
                 }
+                // the pivot itself (lane 0) is not part of u
+                const bool nonfin = lane > 0 && lane < W && !__builtin_isfinite(uc);
+                const unsigned long long bad_u = __ballot(nonfin);
+                if (lane == 0) s_ufin = bad_u ? 0 : 1;
             }
             __syncthreads();
             wpos = s_win[0];
             wrid = s_win[1];
         }
         const double pv = s_pv;
+        const bool ufin = s_ufin != 0;
 
         // ---- interchange bookkeeping, scale, rank-1 update + rotate --------
 #pragma unroll
@@ -279,15 +301,21 @@ __global__ __launch_bounds__(NT) void panel_kernel(
             if (!zero) l = tiny ? l / pv : l * inv;
             const double lm = done[k] ? 0.0 : l;
             const double last = done[k] ? row[k][0] : l;
-            if (__builtin_isfinite(lm)) {
+            // the reference updates only the rows below the pivot
+            // (tf2.tcc:130-132): a finished row must not see 0 * Inf / NaN
+            // from a pivot row that holds them
+            const bool plain = done[k] ? ufin : __builtin_isfinite(lm);
+            if (plain) {
                 // new[c] = row[c+1] - l*u[c+1]; finished columns see u = 0
 #pragma unroll
                 for (int c = 0; c < W - 1; ++c) row[k][c] = fma(-lm, s_u[c], row[k][c + 1]);
             } else {
-                // a non-finite multiplier must not touch the finished columns
+                // a non-finite multiplier must not touch the finished
+                // columns; a finished row only rotates
+                const int upd = done[k] ? 0 : live - 1;
 #pragma unroll
                 for (int c = 0; c < W - 1; ++c)
-                    row[k][c] = (c < live - 1) ? fma(-lm, s_u[c], row[k][c + 1]) : row[k][c + 1];
+                    row[k][c] = (c < upd) ? fma(-lm, s_u[c], row[k][c + 1]) : row[k][c + 1];
             }
             row[k][W - 1] = last;
         }

@@ -170,9 +170,14 @@ struct Solver {
     int factor_graph()
     {
         MixBufs &m = h->mix;
-        const size_t key[8] = {(size_t)n, (size_t)ldw, (size_t)o.prec, (size_t)o.nb,
-                               (size_t)m.W32.p, (size_t)m.cand.p, (size_t)m.la.p,
-                               (size_t)m.perm.p ^ ((size_t)mw.inv << 1)};
+        // every value a captured launch depends on: shape, precision, blocking,
+        // the side stream's workgroup cap and every workspace address
+        const size_t key[MIX_GKEY] = {
+            (size_t)n, (size_t)ldw, (size_t)o.prec, (size_t)o.nb, (size_t)mw.side_groups,
+            (size_t)m.W32.p, (size_t)mw.perm, (size_t)mw.cand, (size_t)mw.ccount,
+            (size_t)mw.counters, (size_t)mw.u11, (size_t)mw.win, (size_t)mw.list,
+            (size_t)mw.zero, (size_t)mw.lpmax, (size_t)mw.la, (size_t)mw.ub, (size_t)mw.inv,
+            (size_t)mw.mv, (size_t)mw.stage, (size_t)mw.stage_perm, (size_t)mw.side};
         if (!m.gstream) {
             FLX_TRY(hipStreamCreateWithFlags(&m.gstream, hipStreamNonBlocking));
             FLX_TRY(hipEventCreateWithFlags(&m.ev0, hipEventDisableTiming));
@@ -415,18 +420,36 @@ struct Solver {
         return FLX_SUCCESS;
     }
 
-    // FP64 LU + solve (A overwritten, X = solution); returns LAPACK info
-    int fallback64(int *info, int *ipiv)
+    // FP64 LU + solve: the factorisation runs on a copy F of A, so that the
+    // backward error of the fallback's X is measured against A itself; the
+    // factors then replace A, as dsgesv's fallback leaves them.  *info =
+    // LAPACK info of the FP64 factorisation, *eta its solution's backward
+    // error (NaN when info > 0: no solution).
+    int fallback64(int *info, int *ipiv, double *eta)
     {
+        MixBufs &m = h->mix;
         FLX_TRY(copy_cm(s, B, ldb, X, ldx, n, nrhs));
         FLX_TRY(ensure(h, h->info, sizeof(int)));
         FLX_TRY(ensure(h, h->ipiv, sizeof(int) * n));
+        FLX_TRY(ensure(h, m.F, sizeof(double) * (size_t)n * n));
+        double *F = (double *)m.F.p;
         int *piv = ipiv ? ipiv : (int *)h->ipiv.p;
-        int st = dgetrf_dev(h, n, n, const_cast<double *>(A), lda, piv, (int *)h->info.p, true);
-        if (st) return st;
+        FLX_RET(dgetrf_dev(h, n, n, const_cast<double *>(A), lda, piv, (int *)h->info.p, true,
+                           F, n));
         FLX_TRY(hipMemcpyAsync(info, h->info.p, sizeof(int), hipMemcpyDeviceToHost, s));
-        FLX_TRY(hipStreamSynchronize(s));
-        if (*info == 0) return dgetrs_dev(h, 0, n, nrhs, A, lda, piv, (const int *)h->perm.p, X, ldx);
+        FLX_RET(flx_sync(h));   // a panel wait that gave up is this call's error
+        *eta = NAN;
+        if (*info == 0) {
+            FLX_RET(dgetrs_dev(h, 0, n, nrhs, F, n, piv, (const int *)h->perm.p, X, ldx));
+            FLX_RET(flx_sync(h));
+            bool conv = false;
+            const bool keep = o.dsgesv;
+            o.dsgesv = false;
+            const int r = check(&conv, eta);
+            o.dsgesv = keep;
+            if (r) return r;
+        }
+        FLX_TRY(copy_cm(s, F, n, const_cast<double *>(A), lda, n, n));
         return FLX_SUCCESS;
     }
 
@@ -500,17 +523,9 @@ static int run_mixed(flx_context *h, int n, int nrhs, double *A, int lda, const 
     const auto t2 = clk::now();
     int info = 0, fell = 0;
     if (code < 0 && (o.dsgesv || o.fallback || code == -4)) {   // -4: the ladder chose FP64
-        r = S.fallback64(&info, ipiv);
+        r = S.fallback64(&info, ipiv, &eta);
         if (r) return r;
         fell = 1;
-        if (info == 0) {
-            bool conv = false;
-            Opts keep = S.o;
-            S.o.dsgesv = false;
-            r = S.check(&conv, &eta);
-            S.o = keep;
-            if (r) return r;
-        }
     } else if (code >= 0 && ipiv) {
         r = S.perm_to_ipiv(ipiv);
         if (r) return r;

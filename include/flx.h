@@ -51,7 +51,12 @@ typedef struct flx_context *flx_handle;
 int flx_create(flx_handle *h);
 int flx_destroy(flx_handle h);
 int flx_set_stream(flx_handle h, void *hip_stream);
-int flx_sync(flx_handle h);                /* waits, then reports device-side errors */
+int flx_sync(flx_handle h);                /* waits, then reports device-side errors:
+                                              FLX_SYNC_TIMEOUT when a bounded wait of
+                                              a queued launch gave up since the last
+                                              report (asynchronous entry points such
+                                              as flx_dgetrs report it here or in the
+                                              next synchronous entry point) */
 const char *flx_status_string(int status);
 /* LU blocking: nb = outer block (trailing-update K), ib = panel width
  * (<= 64).  The reference's equivalent knob is ilaenv(1,"GETRF") = 64,
@@ -73,6 +78,14 @@ int flx_set_lu_lookahead(flx_handle h, int depth);
  * Tuning only; results do not depend on it.  No counterpart in the
  * reference. */
 int flx_set_lu_side_groups(flx_handle h, int which, int groups);
+
+/* Test hook for the error path: raises the handle's bounded-wait failure
+ * flag on its stream, exactly as a device-side wait that gave up would, so
+ * tests can check which call reports FLX_SYNC_TIMEOUT without making a
+ * kernel time out.  Every synchronous entry point (and flx_sync) reports a
+ * raised flag as FLX_SYNC_TIMEOUT and re-arms it; the FLENS seams abort
+ * with that status.  No counterpart in the reference. */
+int flx_test_raise_sync_flag(flx_handle h);
 
 /* Launch profiler (no counterpart in the reference, which has no timers:
  * flens/debug only logs calls).  Between start and stop every library

@@ -105,6 +105,81 @@ void orc_dgemm(int transa, int transb, int m, int n, int k, double alpha,
     }
 }
 
+/* --------------------------------------------------------------- TRSM --- */
+/*
+ * cxxblas/level2/trsv.tcc:42-... (trsv_generic) for real data in RowMajor
+ * storage (the order trsm_generic always reaches it in): each x_i first
+ * subtracts the dot product of its already solved neighbours, formed by
+ * dotu_generic (cxxblas/level1/dot.tcc:42-52: sum from 0, ascending), then
+ * divides by the diagonal unless diag is Unit.  Conj / ConjTrans are NoTrans
+ * / Trans for real data (dot_generic conjugates nothing there).
+ */
+static void orc_trsv_rm(int upper, int trans, int unit, int n, const double *a,
+                        int lda, double *x, int incx)
+{
+#define AR_(i, j) a[(size_t)(i) * (size_t)lda + (size_t)(j)]
+#define X_(i) x[(size_t)(i) * (size_t)incx]
+    if (!trans && upper) {            /* backward, row i right of the diagonal */
+        for (int i = n - 1; i >= 0; --i) {
+            double s = 0.0;
+            for (int k = 0; k < n - i - 1; ++k) s += AR_(i, i + 1 + k) * X_(i + 1 + k);
+            X_(i) -= s;
+            if (!unit) X_(i) /= AR_(i, i);
+        }
+    } else if (!trans) {              /* Lower: forward, row i left of the diagonal */
+        for (int i = 0; i < n; ++i) {
+            double s = 0.0;
+            for (int k = 0; k < i; ++k) s += AR_(i, k) * X_(k);
+            X_(i) -= s;
+            if (!unit) X_(i) /= AR_(i, i);
+        }
+    } else if (upper) {               /* Trans, Upper: forward, column i above */
+        for (int i = 0; i < n; ++i) {
+            double s = 0.0;
+            for (int k = 0; k < i; ++k) s += AR_(k, i) * X_(k);
+            X_(i) -= s;
+            if (!unit) X_(i) /= AR_(i, i);
+        }
+    } else {                          /* Trans, Lower: backward, column i below */
+        for (int i = n - 1; i >= 0; --i) {
+            double s = 0.0;
+            for (int k = 0; k < n - i - 1; ++k) s += AR_(i + 1 + k, i) * X_(i + 1 + k);
+            X_(i) -= s;
+            if (!unit) X_(i) /= AR_(i, i);
+        }
+    }
+#undef AR_
+#undef X_
+}
+
+/*
+ * cxxblas/level3/trsm.tcc:40-68 (trsm_generic), the routine blas::sm
+ * reaches (flens/blas/level3/sm.tcc:63).  order 0 RowMajor / 1 ColMajor,
+ * right 0 Left / 1 Right, upper, trans, unit as 0/1.  A ColMajor call is
+ * the RowMajor one with side and uplo flipped and m, n swapped; Right solves
+ * every row of B with trsv(trans ^ Trans), Left every column (stride ldb)
+ * with trsv(trans); only then is B scaled by alpha (gescal,
+ * cxxblas/level1extensions/gescal.tcc:40-62, i.e. scal's b *= alpha).
+ */
+void orc_dtrsm(int order, int right, int upper, int trans, int unit, int m, int n,
+               double alpha, const double *a, int lda, double *b, int ldb)
+{
+    if (order == 1) {
+        right = !right;
+        upper = !upper;
+        int t = m; m = n; n = t;
+    }
+    if (right) {
+        for (int i = 0; i < m; ++i)
+            orc_trsv_rm(upper, !trans, unit, n, a, lda, b + (size_t)i * ldb, 1);
+    } else {
+        for (int j = 0; j < n; ++j)
+            orc_trsv_rm(upper, trans, unit, m, a, lda, b + j, ldb);
+    }
+    for (int i = 0; i < m; ++i)
+        for (int j = 0; j < n; ++j) b[(size_t)i * ldb + j] *= alpha;
+}
+
 /* Single-precision GEMM with the same order (used by the sgetrf restatement
  * only through orc_sgetrf; exported for completeness of the mixed oracle). */
 

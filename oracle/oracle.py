@@ -86,7 +86,14 @@ def lib():
         _lib.orc_dgesvx.restype = _i
         _lib.orc_dgetri.argtypes = [_i, _dp, _i, _ip]
         _lib.orc_dgetri.restype = _i
+        _lib.orc_dtrsm.argtypes = [_i, _i, _i, _i, _i, _i, _i, _d, _dp, _i, _dp, _i]
     return _lib
+
+
+def have_ref() -> bool:
+    """Whether the compiled reference exists here (checked without loading
+    it, so collecting a test module never maps it into the process)."""
+    return os.path.exists(_REF)
 
 
 def ref():
@@ -118,6 +125,7 @@ def ref():
         _ref.ref_dgesvx.restype = _i
         _ref.ref_dgetri.argtypes = [_i, _dp, _i, _ip]
         _ref.ref_dgetri.restype = _i
+        _ref.ref_dtrsm.argtypes = [_i, _i, _i, _i, _i, _i, _i, _d, _dp, _i, _dp, _i]
     return _ref
 
 
@@ -281,6 +289,25 @@ def dmmch(transa, transb, alpha, a, b, beta, c0, c):
     k = a.shape[0] if transa else a.shape[1]
     return lib().orc_dmmch(int(transa), int(transb), m, n, k, float(alpha), _p(a), _ld(a),
                            _p(b), _ld(b), float(beta), _p(c0), _p(c), _ld(c))
+
+
+def dtrsm(order, right, upper, trans, unit, alpha, a, b, use_ref=False):
+    """B <- alpha * op(A)^{-1} B (left) or alpha * B op(A)^{-1} (right), in
+    place, trsm_generic's order.  order 'F': a, b column-major arrays;
+    order 'C': row-major (C-ordered) arrays, as a CBLAS RowMajor caller
+    holds them."""
+    if order == "F":
+        assert a.flags.f_contiguous and b.flags.f_contiguous
+        m, n = b.shape
+        lda, ldb, o = max(1, a.shape[0]), max(1, m), 1
+    else:
+        assert a.flags.c_contiguous and b.flags.c_contiguous
+        m, n = b.shape
+        lda, ldb, o = max(1, a.shape[1]), max(1, n), 0
+    fn = ref().ref_dtrsm if use_ref else lib().orc_dtrsm
+    fn(o, int(right), int(upper), int(trans), int(unit), m, n, float(alpha), _p(a), lda,
+       _p(b), ldb)
+    return b
 
 
 def dsgesv(a, b, itermax=30):

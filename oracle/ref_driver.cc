@@ -98,6 +98,20 @@ void ref_dlaswp(int m, int n, double *a, int lda, int k1, int k2,
     }
 }
 
+// cxxblas::trsm (cxxblas/level3/trsm.tcc:40-82, generic: no CBLAS here), the
+// routine blas::sm calls (flens/blas/level3/sm.tcc:63).  order 0 RowMajor /
+// 1 ColMajor; right, upper, trans, unit as 0/1.
+void ref_dtrsm(int order, int right, int upper, int trans, int unit, int m, int n,
+               double alpha, const double *a, int lda, double *b, int ldb)
+{
+    cxxblas::trsm(order ? cxxblas::ColMajor : cxxblas::RowMajor,
+                  right ? cxxblas::Right : cxxblas::Left,
+                  upper ? cxxblas::Upper : cxxblas::Lower,
+                  trans ? cxxblas::Trans : cxxblas::NoTrans,
+                  unit ? cxxblas::Unit : cxxblas::NonUnit,
+                  m, n, alpha, a, lda, b, ldb);
+}
+
 // ---- FLENS-LAPACK auxiliaries: norms, equilibration, condition estimate,
 // refinement, expert driver (generic implementations) --------------------
 

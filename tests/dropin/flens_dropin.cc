@@ -1,9 +1,9 @@
 // A FLENS program that uses only FLENS's public API (GeMatrix, blas::mm,
-// lapack::trf / trs / sv).  Compiled with the reference's own switches for
+// blas::sm, lapack::trf / trs / sv).  Compiled with the reference's own switches for
 // an external BLAS/LAPACK (-DWITH_REFBLAS -DUSE_CXXLAPACK
 // -DALWAYS_USE_CXXLAPACK) and linked against libflx instead of a CPU
-// BLAS/LAPACK, its GEMM and LU calls land in libflx's cblas_dgemm, dgetrf_,
-// dgetrs_ and dgesv_ (flens/lapack/lapack.h:48-55,
+// BLAS/LAPACK, its GEMM, triangular-solve and LU calls land in libflx's
+// cblas_dgemm, cblas_dtrsm, dgetrf_, dgetrs_ and dgesv_ (flens/lapack/lapack.h:48-55,
 // flens/lapack/ge/trf.tcc:164-173, cxxblas/level3/gemm.tcc:327-349).
 //
 // Prints one line per check, "ok <name> <value>" or "FAIL <name> <value>".
@@ -64,6 +64,15 @@ int main()
         int info = lapack::trf(Ab, piv);
         bool ok = info == 0 && piv(1) == 2 && piv(2) == 3 && piv(3) == 4 && piv(4) == 4;
         bad += report("kat_trf_pivots", ok, info);
+        // the example's second half: blas::sm(Left, NoTrans, 1, A.upper(), B)
+        // (flens/examples/lapack-getrf.cc:56-61) -> cxxblas::trsm -> cblas_dtrsm
+        const Underscore<int> _;
+        const auto A = Ab(_, _(1, 4));
+        auto B = Ab(_, _(5, 5));
+        blas::sm(Left, NoTrans, 1.0, A.upper(), B);
+        double dx = std::fabs(B(1, 1) - 1) + std::fabs(B(2, 1) - 9) + std::fabs(B(3, 1) - 9) +
+                    std::fabs(B(4, 1) - 9);
+        bad += report("kat_sm_solution", dx < 1e-12, dx);
     }
     // blas::mm: C = 1.5 A B - 0.5 C against a scalar loop
     {

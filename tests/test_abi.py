@@ -57,13 +57,14 @@ DROPIN = os.path.join(ROOT, "oracle", "_ref", "flens_dropin")
 def test_flens_program_binds_to_libflx():
     """A FLENS program built with FLENS's external BLAS/LAPACK switches
     (tests/dropin/flens_dropin.cc, built by oracle/Makefile where the
-    reference headers exist) resolves its GEMM and LU calls to libflx."""
+    reference headers exist) resolves its GEMM, triangular-solve and LU calls
+    to libflx."""
     import pytest
     if not os.path.exists(DROPIN):
         pytest.skip("drop-in program not built (needs the reference headers)")
     und = subprocess.run(["nm", "-u", DROPIN], capture_output=True, text=True, check=True).stdout
     needed = {line.split()[-1] for line in und.splitlines() if line.strip()}
-    assert {"cblas_dgemm", "dgetrf_", "dgetrs_", "dgesv_"} <= needed
+    assert {"cblas_dgemm", "cblas_dtrsm", "dgetrf_", "dgetrs_", "dgesv_"} <= needed
     ldd = subprocess.run(["ldd", DROPIN], capture_output=True, text=True, check=True).stdout
     line = next(l for l in ldd.splitlines() if "libflx.so" in l)
     assert os.path.realpath(line.split("=>")[1].split("(")[0].strip()) == os.path.realpath(F.LIB_PATH)

@@ -170,10 +170,15 @@ def test_batched_gemm(gpu):
     import torch
     rng = np.random.default_rng(5)
     count = 250
-    m, n, k = (np.clip(np.exp(rng.uniform(0, np.log(300), count)).round(), 1, 300).astype(np.int32)
+    # sizes over the whole range of the batched GEMM workload (up to 512),
+    # with full 512^3 problems and tile-ragged ones
+    m, n, k = (np.clip(np.exp(rng.uniform(0, np.log(512), count)).round(), 1, 512).astype(np.int32)
                for _ in range(3))
     m[7] = 0
     k[9] = 0
+    m[40:43] = n[40:43] = k[40:43] = 512
+    m[43], n[43], k[43] = 511, 509, 512
+    m[44], n[44], k[44] = 512, 17, 500
     alpha = rng.uniform(-2, 2, count)
     beta = rng.uniform(-2, 2, count)
     beta[11] = 0.0

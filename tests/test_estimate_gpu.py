@@ -170,6 +170,15 @@ def test_svx_matches_oracle(gpu, n):
             r = torch.zeros(max(1, n), dtype=torch.float64, device="cuda")
             c = torch.zeros(max(1, n), dtype=torch.float64, device="cuda")
             d = gpu.lapack.svx(fact, tr, A, AF, piv, "N", r, c, B, X)
+            if d["info"] != do["info"] or not np.isfinite(X.numpy()).all():
+                # say which stage went wrong: the factor or the solves
+                A3 = dev(gpu, a0)
+                gpu.lapack.trf(A3)
+                Bt = dev(gpu, b0)
+                gpu.lapack.trs(tr, AF, piv, Bt)
+                print("svx mismatch:", n, fact, tr, d, "AF == trf(A):",
+                      np.array_equal(AF.numpy(), A3.numpy()), "AF NaN:", int(np.isnan(AF.numpy()).sum()),
+                      "trs(AF) NaN:", np.isnan(Bt.numpy()).sum(0).tolist())
             assert d["info"] == do["info"] and d["equed"] == do["equed"], (fact, tr, d, do)
             assert same(r.cpu().numpy()[:n], ro) and same(c.cpu().numpy()[:n], co)
             assert same(A.numpy(), ao) and same(B.numpy(), bo)   # equilibration bit-exact

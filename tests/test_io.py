@@ -12,7 +12,7 @@ import pytest
 import flens_amd as F
 from oracle import oracle as O
 
-needs_ref = pytest.mark.skipif(O.ref() is None, reason="compiled reference (oracle/_ref) absent")
+needs_ref = pytest.mark.skipif(not O.have_ref(), reason="compiled reference (oracle/_ref) absent")
 
 
 def _ref():

@@ -4,7 +4,8 @@ reference's own fixtures.
 
 Bar: pivot vectors identical to the reference's on
 columns without near-ties (the first mismatch must sit on a column whose
-relative top-2 gap is below TAU), LAPACK dget01 ratio
+relative top-2 gap is below tau(n) = 64 n eps64: two summation orders of the
+j-th column differ by about j eps64 relative), LAPACK dget01 ratio
 ||PA - LU||_1 / (n ||A||_1 eps) < 30, dget02 solve ratio < 30, INFO and
 pivot conventions of flens/lapack/ge/tf2.tcc:101-126 on exact zeros.
 """
@@ -18,15 +19,26 @@ from oracle import oracle as O
 
 pytestmark = pytest.mark.gpu
 GOLD = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
-TAU = 1e-9   # relative top-2 pivot gap below which a summation-order flip is legitimate
+EPS = 2.0 ** -53
+
+
+def tau(n):
+    """Relative top-2 pivot gap below which a summation-order flip is
+    legitimate: the GPU and the reference sum a column's j updates in
+    different orders, a relative difference of order j eps64."""
+    return 64.0 * max(n, 1) * EPS
 
 
 def first_mismatch_ok(piv_gpu, piv_ref, gap):
-    d = np.nonzero(np.asarray(piv_gpu) != np.asarray(piv_ref))[0]
+    """(ok, first mismatching column or -1).  ok: identical, or the first
+    difference sits on a near-tie column (later columns then legitimately
+    follow a different elimination)."""
+    piv_gpu, piv_ref = np.asarray(piv_gpu), np.asarray(piv_ref)
+    d = np.nonzero(piv_gpu != piv_ref)[0]
     if d.size == 0:
         return True, -1
     j = int(d[0])
-    return bool(gap[j] < TAU), j
+    return bool(gap[j] < tau(len(piv_ref))), j
 
 
 def dev_getrf(F, a, nb=None, ib=None):
@@ -59,17 +71,48 @@ def test_lu_grid_vs_reference(gpu):
 
 
 @pytest.mark.parametrize("nb,ib", [(64, 32), (128, 32), (256, 32), (64, 64), (256, 64), (32, 32)])
-def test_lu_blockings_multi_workgroup_panel(gpu, nb, ib):
-    # heights above 256 rows use several cooperating workgroups per panel
-    for (m, n, seed) in [(700, 700, 1), (1030, 1030, 2), (600, 300, 3), (300, 650, 4)]:
-        a0 = O.uniform(m, n, 40 + seed)
-        lu, piv, info = dev_getrf(gpu, a0, nb, ib)
-        piv = piv[: min(m, n)]
+@pytest.mark.parametrize("look", [1, 0])
+def test_lu_blockings_multi_workgroup_panel(gpu, nb, ib, look):
+    # heights above 512 rows use several cooperating workgroups per panel;
+    # look = 0 runs the blocked LU without lookahead (no second stream)
+    h = gpu.default_handle()
+    h.set_lu_lookahead(look)
+    try:
+        for (m, n, seed) in [(700, 700, 1), (1030, 1030, 2), (600, 300, 3), (300, 650, 4)]:
+            a0 = O.uniform(m, n, 40 + seed)
+            lu, piv, info = dev_getrf(gpu, a0, nb, ib)
+            piv = piv[: min(m, n)]
+            ref = a0.copy(order="F")
+            rpiv, rinfo, gap = O.dgetrf_gaps(ref, nb=64)
+            ok, j = first_mismatch_ok(piv, rpiv, gap)
+            assert ok and info == rinfo == 0, (m, n, nb, ib, look, j)
+            assert O.dget01(a0, lu, piv) < 30
+    finally:
+        h.set_lu_lookahead(1)
+
+
+def test_lu_nan_pivot_rule(gpu):
+    """iamax's NaN rule (cxxblas/level1/iamax.tcc:42-58): a NaN is the pivot
+    only at the first live position of its column and is skipped elsewhere.
+    Pivots, INFO and the NaN pattern of L\\U must equal the oracle's, which
+    follows the reference's loop; the finite entries agree to rounding."""
+    n = 1100
+    cases = []
+    a = O.uniform(n, n, 77); a[700, 0] = np.nan; cases.append(a)      # skipped, spreads along a row
+    a = O.uniform(n, n, 78); a[0, 0] = np.nan; cases.append(a)        # first position: chosen
+    a = O.uniform(n, n, 79); a[5, 300] = np.nan; a[900, 300] = np.nan; cases.append(a)
+    a = O.uniform(300, 300, 80); a[:, 10] = np.nan; cases.append(a)  # a whole NaN column
+    for k, a0 in enumerate(cases):
+        lu, piv, info = dev_getrf(gpu, a0)
         ref = a0.copy(order="F")
-        rpiv, rinfo, gap = O.dgetrf_gaps(ref, nb=64)
-        ok, j = first_mismatch_ok(piv, rpiv, gap)
-        assert ok and info == rinfo == 0, (m, n, nb, ib, j)
-        assert O.dget01(a0, lu, piv) < 30
+        rpiv, rinfo = O.dgetrf(ref)
+        m = a0.shape[0]
+        assert info == rinfo, (k, info, rinfo)
+        d = np.nonzero(piv[:m] != rpiv)[0]
+        assert d.size == 0, (k, int(d[0]), int(piv[d[0]]), int(rpiv[d[0]]))
+        assert np.array_equal(np.isnan(lu), np.isnan(ref)), k
+        fin = np.isfinite(ref)
+        assert np.allclose(lu[fin], ref[fin], rtol=1e-9, atol=1e-9), k
 
 
 def test_lu_exact_zero_info_and_ties(gpu):
@@ -187,12 +230,30 @@ def test_lu8192_vs_reference_fixture(gpu, nb):
     assert L.flx_fill_uniform(h.ptr, gpu._vp(A.ptr), n, n, n, 2, 0) == 0
     a0 = A.storage.T.clone()
     h.set_lu_blocking(nb, 32)
-    piv, info = gpu.lapack.trf(A)
+    pivd, info = gpu.lapack.trf(A)
     h.set_lu_blocking(256, 32)
-    piv = piv.cpu().numpy()
+    piv = pivd.cpu().numpy()
     assert int(info.cpu()[0]) == 0
     ok, j = first_mismatch_ok(piv, z["piv"], z["gap"])
-    assert ok, j
+    what = "identical" if j < 0 else f"first differ at column {j} (gap {float(z['gap'][j]):.3g})"
+    print(f"nb={nb}: pivots {what}")
+    assert ok, (j, float(z["gap"][j]))
+    # the solve of the same workload (B ~ U[-1,1) seed 3, nrhs = 16) against
+    # the reference's first rows of X, LAPACK dget04's criterion:
+    # ||x - x_ref|| / (||x_ref|| kappa eps) < 30 for each of the two
+    # solutions (kappa from flx_dgecon on the factor)
+    nrhs = 16
+    B = gpu.DeviceMatrix.empty(n, nrhs)
+    assert L.flx_fill_uniform(h.ptr, gpu._vp(B.ptr), n, nrhs, n, 3, 0) == 0
+    gpu.lapack.trs(0, A, pivd, B)
+    x = B.numpy()
+    anorm = float(a0.abs().sum(1).max().item())
+    rcond = gpu.lapack.con("I", A, anorm)
+    xr = z["x_rows"]
+    for c in range(nrhs):
+        num = np.abs(x[: xr.shape[0], c] - xr[:, c]).max()
+        den = np.abs(x[:, c]).max() * (1.0 / rcond) * 2.0 ** -53
+        assert num / den < 60, (c, num / den, 1.0 / rcond)
     lu = A.storage.T
     l = torch.tril(lu, -1) + torch.eye(n, dtype=lu.dtype, device=lu.device)
     u = torch.triu(lu)

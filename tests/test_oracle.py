@@ -211,7 +211,7 @@ def test_dsgesv_semantics():
     assert it == -2
 
 
-@pytest.mark.skipif(O.ref() is None, reason="reference sources not built here")
+@pytest.mark.skipif(not O.have_ref(), reason="reference sources not built here")
 def test_oracle_vs_compiled_reference_random():
     rng = np.random.default_rng(11)
     for _ in range(6):
@@ -273,3 +273,151 @@ def test_selector_network_oracle_against_finite_differences():
     bad_gb2 = bad_d2.sum(0)
     o = M.IN * M.H + M.H + M.H * M.H             # offset of b2
     assert np.abs(bad_gb2 - g[o:o + M.H]).max() > 1e-3 * np.abs(g[o:o + M.H]).max()
+
+
+# ---------------------------------------------------------------- dsgesv ---
+def _dsgesv_ratios(a, b, itmax=30):
+    """The refinement's exit ratios max_j ||r_j||_inf / (||x_j||_inf cte),
+    cte = ||A||_inf 2^-53 sqrt(n), for the iterates x_0, x_1, ... built from
+    the oracle's FP32 LU (pinned below on exact cases) with FP64 residuals
+    formed by numpy (another summation order than the oracle's DGEMM)."""
+    n = a.shape[0]
+    sa = np.asfortranarray(a.astype(np.float32))
+    piv, info = O.sgetrf(sa)
+    assert info == 0
+    sx = np.asfortranarray(b.astype(np.float32))
+    O.sgetrs(0, sa, piv, sx)
+    x = sx.astype(np.float64)
+    cte = np.abs(a).sum(1).max() * 2.0 ** -53 * np.sqrt(n)
+    out = []
+    for _ in range(itmax + 1):
+        r = b - a @ x
+        out.append(max(np.abs(r[:, j]).max() / (np.abs(x[:, j]).max() * cte)
+                       for j in range(b.shape[1])))
+        if out[-1] <= 1.0:
+            break
+        sr = np.asfortranarray(r.astype(np.float32))
+        O.sgetrs(0, sa, piv, sr)
+        x = x + sr.astype(np.float64)
+    return out
+
+
+def test_dsgesv_exit_rule_exact():
+    """orc_dsgesv stops at the first iterate with ||r||_inf <= ||x||_inf
+    ||A||_inf 2^-53 sqrt(n) (netlib dsgesv.f) and not one iterate earlier or
+    later.  The two cases are chosen so that a threshold off by a factor of
+    1.6 either way moves ITER: in the first the last rejected iterate misses
+    by 1.56x, in the second the accepted one passes with 0.71x."""
+    from flens_amd import testmat as T
+    for seed, kappa, want in ((0, 1e6, 4), (5, 1e7, 5)):
+        a = T.svd_matrix(48, kappa, seed)
+        b, _ = T.rhs_from_solution(a, 1, seed)
+        ratios = _dsgesv_ratios(a, b)
+        assert len(ratios) == want + 1 and ratios[-1] <= 1.0
+        assert ratios[-2] > 1.4 and (seed != 0 or ratios[-2] < 1.7)
+        assert seed != 5 or 0.6 < ratios[-1] < 0.8
+        x, piv, info, it = O.dsgesv(a.copy(order="F"), b)
+        assert info == 0 and it == want, (seed, it, ratios)
+
+
+def test_dsgesv_fallback_codes_on_the_oracle():
+    """ITER -3 (an exactly singular FP32 factor; the FP64 fallback's INFO is
+    the zero column) and ITER -31 (no convergence within 30 steps at
+    kappa = 1e12; the FP64 fallback's X solves the system and A holds its
+    factors), checked on the oracle alone."""
+    from flens_amd import testmat as T
+    n = 50
+    a = np.asfortranarray(O.uniform(n, n, 5))
+    a[:, 17] = 0.0
+    b = O.uniform(n, 1, 2)
+    x, piv, info, it = O.dsgesv(a.copy(order="F"), b)
+    assert it == -3 and info == 18
+    a = T.svd_matrix(n, 1e12, 2)
+    b, _ = T.rhs_from_solution(a, 1, 3)
+    ac = a.copy(order="F")
+    x, piv, info, it = O.dsgesv(ac, b)
+    assert it == -31 and info == 0
+    assert O.dget02(a, x, b) < 30
+    assert O.dget01(a, ac, piv) < 30
+    lu = a.copy(order="F")
+    p64, _ = O.dgetrf(lu)
+    assert np.array_equal(piv, p64) and np.array_equal(ac, lu)
+
+
+def test_sgetrf_exact_cases():
+    """orc_sgetrf on matrices whose partial-pivot LU is exact in FP32:
+    A = P^T L U with |l_ij| <= 1/2 dyadic, U integer with a dominant
+    diagonal -- every product and sum is representable, so the factors are
+    L and U exactly and the pivots are those of P (a closed form).  The FP64
+    oracle gives the same bits."""
+    rng = np.random.default_rng(3)
+    for n in (1, 5, 33, 70, 130):
+        l = np.tril(rng.integers(-2, 3, (n, n)) / 4.0, -1) + np.eye(n)
+        u = np.triu(rng.integers(-3, 4, (n, n)).astype(float), 1) + np.diag(rng.integers(8, 16, n) * 1.0)
+        a = l @ u
+        # a row permutation applied as a sequence of interchanges with
+        # ipiv[i] >= i that partial pivoting must reproduce: the rows of
+        # L U are moved so each column's pivot sits below the diagonal
+        perm_piv = np.array([rng.integers(i, n) for i in range(n)], np.int32) + 1
+        pa = a.copy()
+        for i in range(n - 1, -1, -1):
+            q = perm_piv[i] - 1
+            pa[[i, q]] = pa[[q, i]]
+        s = np.asfortranarray(pa.astype(np.float32))
+        piv, info = O.sgetrf(s)
+        assert info == 0
+        assert np.array_equal(piv, perm_piv), n
+        assert np.array_equal(np.tril(s, -1) + np.eye(n), l.astype(np.float32)), n
+        assert np.array_equal(np.triu(s), u.astype(np.float32)), n
+        d = np.asfortranarray(pa.copy())
+        piv64, _ = O.dgetrf(d)
+        assert np.array_equal(piv64, perm_piv) and np.array_equal(d.astype(np.float32), s)
+
+
+# ------------------------------------------------------------------ trsm ---
+@pytest.mark.skipif(not O.have_ref(), reason="reference sources not built here")
+def test_trsm_bit_exact_vs_compiled_reference():
+    """orc_dtrsm (trsm_generic's order: per-row/column trsv with sequential
+    dot products, alpha applied after the solve) equals the reference's
+    cxxblas::trsm bit for bit, every side/uplo/trans/diag, both orders."""
+    import itertools
+    rng = np.random.default_rng(1)
+    for order in "FC":
+        for right, upper, trans, unit in itertools.product((0, 1), repeat=4):
+            for (m, n) in ((5, 7), (13, 1), (1, 9), (33, 17)):
+                na = n if right else m
+                a = rng.uniform(-1, 1, (na, na)) + np.diag(2 + rng.uniform(0, 1, na))
+                b = rng.uniform(-1, 1, (m, n))
+                a, b = np.array(a, order=order), np.array(b, order=order)
+                for alpha in (1.0, 0.7, -1.3):
+                    b1, b2 = b.copy(order=order), b.copy(order=order)
+                    O.dtrsm(order, right, upper, trans, unit, alpha, a, b1)
+                    O.dtrsm(order, right, upper, trans, unit, alpha, a, b2, use_ref=True)
+                    assert np.array_equal(b1, b2), (order, right, upper, trans, unit, m, n, alpha)
+
+
+def test_trsm_closed_forms():
+    """Closed forms that fix side, triangle, transpose and diag: against a
+    solve in long-double-free exact arithmetic (integer triangles with unit
+    or power-of-two diagonals solve exactly), the strict other triangle is
+    never read (filled with NaN), and alpha scales the solution."""
+    import itertools
+    rng = np.random.default_rng(2)
+    for order in "FC":
+        for right, upper, trans, unit in itertools.product((0, 1), repeat=4):
+            m, n = 9, 6
+            na = n if right else m
+            t = rng.integers(-2, 3, (na, na)).astype(float)
+            t = np.triu(t, 1) if upper else np.tril(t, -1)
+            dg = np.ones(na) if unit else 2.0 ** rng.integers(-2, 3, na)
+            full = t + np.diag(dg)
+            x = rng.integers(-4, 5, (m, n)).astype(float)
+            op = full.T if trans else full
+            b = (op @ x) if not right else (x @ op)          # exact: small integers/dyadics
+            a = full.copy()
+            a[np.tril_indices(na, -1) if upper else np.triu_indices(na, 1)] = np.nan
+            if unit:
+                a[np.diag_indices(na)] = np.nan               # never read either
+            a, bb = np.array(a, order=order), np.array(b, order=order)
+            O.dtrsm(order, right, upper, trans, unit, -2.0, a, bb)
+            assert np.array_equal(bb, -2.0 * x), (order, right, upper, trans, unit)

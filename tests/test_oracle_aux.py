@@ -18,7 +18,7 @@ import pytest
 
 from oracle import oracle as O
 
-HAVE_REF = O.ref() is not None
+HAVE_REF = O.have_ref()
 needs_ref = pytest.mark.skipif(not HAVE_REF, reason="compiled reference (oracle/_ref) absent")
 EPS = 2.0 ** -53
 
