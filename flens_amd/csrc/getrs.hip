@@ -391,13 +391,17 @@ __global__ __launch_bounds__(256) void trs_persist_kernel(const double *__restri
                 break;
             }
         }
+        // only the blocks of this group: the slots of a short last group hold
+        // another launch's LDS contents, and 0 * Inf / NaN is not 0
 #pragma unroll
         for (int b = 0; b < GRP; ++b)
+            if (b < cnt) {
 #pragma unroll
-            for (int c = 0; c < 16; ++c) {
-                const double l = lv[b][c];
+                for (int c = 0; c < 16; ++c) {
+                    const double l = lv[b][c];
 #pragma unroll
-                for (int k = 0; k < R; ++k) acc[k] = fma(l, s_x[par][b][wave * 16 + c][k], acc[k]);
+                    for (int k = 0; k < R; ++k) acc[k] = fma(l, s_x[par][b][wave * 16 + c][k], acc[k]);
+                }
             }
     }
 #pragma unroll
