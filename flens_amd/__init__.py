@@ -40,7 +40,7 @@ EXPORTS = [
     "flx_create", "flx_destroy", "flx_set_stream", "flx_sync", "flx_status_string",
     "flx_set_lu_blocking", "flx_set_lu_lookahead", "flx_set_lu_side_groups", "flx_dgemm", "flx_dgetrf", "flx_dgetrs", "flx_dgesv",
     "flx_dlaswp", "flx_fill_uniform", "flx_profile_start", "flx_profile_stop",
-    "flx_profile_classes", "flx_test_raise_sync_flag",
+    "flx_profile_classes", "flx_test_raise_sync_flag", "flx_dgetrf_noalloc", "flx_noalloc_bytes",
     "flx_dgesv_mixed", "flx_dsgesv",
     "flx_batch_gesv_create", "flx_batch_gemm_create", "flx_batch_execute", "flx_batch_config",
     "flx_batch_destroy", "flx_mlp_forward", "flx_mlp_grad", "flx_mlp_adam",
@@ -55,6 +55,8 @@ EXPORTS = [
 _dp = C.POINTER(C.c_double)
 _ip = C.POINTER(C.c_int)
 _i, _d, _vp, _u64 = C.c_int, C.c_double, C.c_void_p, C.c_uint64
+
+NOALLOC_SCRATCH_MIB = 36   # FLX_NOALLOC_SCRATCH_MIB of include/flx.h
 
 PREC = {"fp32": 0, "bf16": 1, "fp16": 2}
 REFINE = {"classic": 0, "gmres": 1, "auto": 2}
@@ -131,6 +133,9 @@ def lib():
                           _dp, _ip, _dp, _dp, _dp, _dp, _ip, _ip]
     L.flx_set_lu_lookahead.argtypes = [_vp, _i]
     L.flx_test_raise_sync_flag.argtypes = [_vp]
+    L.flx_dgetrf_noalloc.argtypes = [_vp, _i, _i, _vp, _i, _vp, _vp]
+    L.flx_noalloc_bytes.argtypes = [_i, _i]
+    L.flx_noalloc_bytes.restype = C.c_size_t
     L.flx_set_lu_side_groups.argtypes = [_vp, _i, _i]
     L.flx_dgemm.argtypes = [_vp, _i, _i, _i, _i, _i, _d, _vp, _i, _vp, _i, _d, _vp, _i]
     L.flx_dgetrf.argtypes = [_vp, _i, _i, _vp, _i, _vp, _vp]

@@ -229,6 +229,7 @@ extern "C" const char *flx_status_string(int s)
     case FLX_ALLOC_FAILED: return "device allocation failed";
     case FLX_SYNC_TIMEOUT: return "device-side wait timed out";
     case FLX_UNSUPPORTED: return "unsupported shape or option";
+    case FLX_SCRATCH_LIMIT: return "problem larger than the no-allocation scratch";
     default: return "unknown status";
     }
 }

@@ -28,6 +28,7 @@
 #ifndef FLX_H
 #define FLX_H
 
+#include <stddef.h>
 #include <stdint.h>
 
 #ifdef __cplusplus
@@ -41,7 +42,8 @@ enum flx_status {
     FLX_HIP_ERROR = 2,
     FLX_ALLOC_FAILED = 3,
     FLX_SYNC_TIMEOUT = 4,     /* a bounded device-side wait gave up */
-    FLX_UNSUPPORTED = 5
+    FLX_UNSUPPORTED = 5,
+    FLX_SCRATCH_LIMIT = 6     /* larger than flx_dgetrf_noalloc's fixed scratch */
 };
 
 typedef struct flx_context *flx_handle;
@@ -130,6 +132,23 @@ int flx_dgetrs(flx_handle h, int trans, int n, int nrhs, const double *A,
  * This entry synchronises the stream to read INFO. */
 int flx_dgesv(flx_handle h, int n, int nrhs, double *A, int lda, int *ipiv,
               double *B, int ldb, int *info_host);
+
+/* flx_dgetrf without any allocation, for callers that cannot allocate
+ * device memory (code under HIP graph capture, tight serving loops,
+ * harnesses that forbid hipMalloc).  No counterpart in the reference.  Same
+ * arguments, kernels and results as flx_dgetrf (run without lookahead; the
+ * factors and pivots do not depend on it); asynchronous on the handle's
+ * stream.  Its scratch is one static device array of
+ * FLX_NOALLOC_SCRATCH_MIB MiB per device (held until the process ends):
+ * flx_noalloc_bytes(m, n) bytes are needed -- about 8 m ceil32(n) bytes, so
+ * square matrices up to n = 2160 -- and larger shapes return
+ * FLX_SCRATCH_LIMIT without launching anything.  All calls on one device
+ * share that array: calls that overlap on a device race.  *info (device) =
+ * INT_MIN when a bounded device-side wait of the panel gave up. */
+enum { FLX_NOALLOC_SCRATCH_MIB = 36 };
+int flx_dgetrf_noalloc(flx_handle h, int m, int n, double *A, int lda, int *ipiv,
+                       int *info);
+size_t flx_noalloc_bytes(int m, int n);
 
 /* Row interchanges k1..k2 (1-based), incx = +1 / -1.  Replaces
  * flens::lapack::laswp (flens/lapack/la/laswp.tcc:126-186). */
