@@ -17,11 +17,17 @@ Also measured in the same run (unless --quick), each in its own JSON key:
                  solved without it.
 
 Prints ONE JSON line.  `roofline` is for the dominant kernel of the primary
-workload (the MFMA trailing-update GEMM), from HIP events recorded around
-each of its launches on the stream it runs on, over the timed region.
-`cpu_baseline` times the oracle's restatement of the reference's generic
-getrf on one core over a bounded sample (the first block step of the same
-n = 8192 factorisation) and projects the full trf+trs flop count.
+workload (the cooperative panel factorisation, HBM-bound: algorithmic bytes
+16 * rows * width per launch over its average launch time), from HIP events
+recorded around each of its launches on the stream it runs on, over a
+second pass of the timed steps; `roofline_gemm` gives the trailing-update
+GEMM the same way.  `peaks_measured` holds the on-device probes (MFMA
+loops, streaming copy) that `frac_of_probe` divides by.  `nb_sweep` times
+the same workload at nb in {64, 128, 256}.  `vs_baseline` carries the
+metric's ratio: mixed+IR over plain FP64 solves/s on the n = 16384 job.
+`cpu_baseline` is the oracle's restatement of the reference's generic
+getrf + getrs on one core, projected from block steps sampled at several
+depths with their full trailing width (cpu_trf_projection).
 """
 from __future__ import annotations
 
@@ -38,6 +44,7 @@ sys.path.insert(0, ROOT)
 METRIC = "solves/s & TFLOP/s at backward err <= n*eps64: mixed+IR vs plain FP64 LU"
 PEAK_FP64_MFMA = 78.6     # TFLOP/s, MI355X dense FP64 matrix
 PEAK_HBM = 8000.0         # GB/s
+PEAK_MFMA = {"fp64": 78.6, "fp32": 157.3, "bf16": 2500.0, "fp16": 2500.0}   # dense TFLOP/s
 
 
 def lu_flops(n: int, nrhs: int) -> float:
@@ -59,42 +66,121 @@ def progress(msg: str):
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
 
-def cpu_baseline(n: int, nrhs: int, nb: int = 64, target_s: float = 15.0):
-    """Oracle (restatement of the reference's generic trf, 1 core) on a
-    bounded sample of the same factorisation: the first nb-wide block step
-    (panel, interchanges, trsm, trailing GEMM) restricted to the first c
-    columns of the n x n matrix, leading dimension n.  In a right-looking
-    step each trailing column depends only on the panel, so this is that
-    step's work for c columns at the workload's memory layout.  c doubles
-    until the sample takes about target_s; the rate is projected to the
-    full trf+trs flop count."""
+def cpu_trf_projection(n: int, nb: int, depths: int = 4, rows=(128, 512)):
+    """Projected one-core time of the oracle's blocked getrf (a restatement
+    of the reference's generic trf, flens/lapack/ge/trf.tcc:59-152) on an
+    n x n matrix with block size nb.
+
+    The block step at depth j (m = n - j rows, r = n - j - jb trailing
+    columns) is the panel tf2 on m x jb plus the interchanges, the
+    unit-lower trsm of the jb x r block row and the trailing GEMM.  The
+    generic GEMM (gemm.tcc: column-major mapped to row-major, one gemv per
+    row of C) re-reads the whole jb x r block row for every row of C, so
+    its per-row cost depends on r and the cache that block fits in, and a
+    sample must keep the full width r.  At a few depths j:
+      P(m)   the panel alone, on the full m x jb (a[j:, j:j+jb]);
+      U(ms)  a step on a[j:j+ms, j:] (all r trailing columns, ms rows) less
+             the panel on its ms rows, for two ms: the trsm and GEMM of the
+             step, linear in the rows, extrapolated to m.
+    The sum over all n/nb steps interpolates P and U linearly in m.  The
+    interchanges of the already-factored columns (laswp on cols0) are not
+    sampled.  Returns (seconds, flops sampled, seconds sampled, text)."""
+    import numpy as np
     from oracle import oracle as O
-    a_full = O.uniform(n, n, 2)
-    c, fl, dt = nb * 2, 0.0, 0.0
-    while True:
-        a = a_full[:, :c]          # column slice of a Fortran array: same lda = n
-        fl = O.getrf_steps_flops(n, c, nb, 1)
-        t = time.perf_counter()
-        O.dgetrf_steps(a, nb, 1)
-        dt = time.perf_counter() - t
-        progress(f"CPU sample: {c} columns, {fl / 1e9:.2f} GFLOP in {dt:.1f} s")
-        if dt * 2.2 > target_s or c * 2 > n:
-            break
-        c *= 2
-        a_full = O.uniform(n, n, 2)
-    rate = fl / dt
-    total = lu_flops(n, nrhs)
+    a = O.uniform(n, n, 2)
+    starts = list(range(0, n, nb))
+    pick = sorted({starts[int(round(q * (len(starts) - 1)))] for q in np.linspace(0, 0.9, depths)})
+    fl_s = dt_s = 0.0
+
+    def run(i0, j0, r, c):
+        nonlocal fl_s, dt_s
+        t0 = time.perf_counter()
+        O.dgetrf_step_at(a, i0, j0, r, c, nb)
+        dt = time.perf_counter() - t0
+        fl_s += O.getrf_steps_flops(r, c, nb, 1)
+        dt_s += dt
+        return dt
+
+    ms, P, U = [], [], []
+    for j in pick:
+        progress(f"CPU sample nb={nb}: depth j={j}")
+        m = n - j
+        jb = min(nb, m)
+        p_full = run(j, j, m, jb)
+        u = []
+        for extra in rows:
+            r = min(jb + extra, m)
+            t_step = run(j, j, r, n - j)
+            t_pan = run(j, j, r, jb)
+            u.append((r, max(0.0, t_step - t_pan)))
+        (r1, u1), (r2, u2) = u
+        h = max(0.0, (u2 - u1) / (r2 - r1)) if r2 > r1 else 0.0
+        ms.append(m)
+        P.append(p_full)
+        U.append(max(0.0, u1 + (m - r1) * h))
+    ms_a = np.array(ms[::-1], dtype=float)
+    P_a, U_a = np.array(P[::-1]), np.array(U[::-1])
+    total = 0.0
+    for j in starts:
+        m = n - j
+        pm = float(np.interp(m, ms_a, P_a))
+        # the update shrinks with r * m below the last sampled depth
+        um = float(np.interp(m, ms_a, U_a)) if m >= ms_a[0] else U_a[0] * (m * max(0, m - nb)) / (ms_a[0] * max(1.0, ms_a[0] - nb))
+        if m > ms_a[0] or j == starts[-1]:
+            pass
+        total += pm + (um if m > nb else 0.0)
+    desc = (f"nb={nb}: at depths j in {pick}, the panel on the full (n-j) x nb and the "
+            f"step's trsm + GEMM on {rows[0]} and {rows[1]} rows below the panel over all "
+            f"n-j columns (lda={n}), interpolated over the {len(starts)} block steps")
+    return total, fl_s, dt_s, desc
+
+
+def cpu_trs_time(n: int, nrhs: int):
+    """One-core time of the oracle's getrs (trs.tcc:84-92: laswp, unit-lower
+    and upper triangular solves) for nrhs right-hand sides, from a solve with
+    one right-hand side at the full n (each right-hand side column is solved
+    independently by the same loops)."""
+    from oracle import oracle as O
+    import numpy as np
+    a = O.uniform(n, n, 2)
+    a[np.arange(n), np.arange(n)] += 4.0       # any nonsingular triangle: the time does not depend on values
+    ipiv = np.arange(1, n + 1, dtype=np.int32)
+    b = O.uniform(n, 1, 3)
+    t0 = time.perf_counter()
+    O.dgetrs(0, a, ipiv, b)
+    return nrhs * (time.perf_counter() - t0)
+
+
+def cpu_baseline(n: int, nrhs: int, nbs=(64, 128, 256), headline_nb: int = 128):
+    """The oracle (restatement of the reference's generic trf + trs, one
+    core) on a bounded sample of the C2 workload: block steps at several
+    depths per nb (cpu_trf_projection) and one right-hand side of the solve,
+    projected to the full n = 8192 trf + trs.  Returns the cpu_baseline
+    object (for headline_nb) and the per-nb projections."""
+    t_trs = cpu_trs_time(n, nrhs)
+    per_nb = {}
+    fl_tot = dt_tot = 0.0
+    for nb in nbs:
+        t_trf, fl_s, dt_s, desc = cpu_trf_projection(n, nb)
+        fl_tot += fl_s
+        dt_tot += dt_s
+        per_nb[nb] = {"trf_s": t_trf, "trs_s": t_trs, "solves_per_s": 1.0 / (t_trf + t_trs),
+                      "sample": desc}
+        progress(f"CPU projection nb={nb}: trf {t_trf:.0f} s + trs {t_trs:.1f} s "
+                 f"(sampled {fl_s / 1e9:.2f} GFLOP in {dt_s:.1f} s)")
+    h = per_nb[headline_nb]
     return {
-        "value": rate / total,
+        "value": h["solves_per_s"],
         "unit": "solves/s",
         "cores": 1,
         "kind": "port",
-        "sample": (f"first nb={nb} block step of the n={n} getrf (panel + laswp + trsm + "
-                   f"trailing GEMM) on the first {c} columns, lda={n}: {fl / 1e9:.2f} GFLOP in "
-                   f"{dt:.1f} s = {rate / 1e9:.3f} GFLOP/s on 1 core of {cpu_model()} "
-                   f"({os.cpu_count()} visible); projected to the {total / 1e9:.0f} GFLOP "
-                   f"trf+trs workload"),
-    }
+        "projected_trf_s": h["trf_s"],
+        "projected_trs_s": t_trs,
+        "sample": (f"{h['sample']}; trs from one right-hand side x {nrhs}; "
+                   f"all nb together sampled {fl_tot / 1e9:.2f} GFLOP in {dt_tot:.1f} s on 1 core "
+                   f"of {cpu_model()} ({os.cpu_count()} visible), in a thread beside the "
+                   f"GPU sections"),
+    }, per_nb
 
 
 def mixed_section(n: int, kappas=(1e2, 1e6, 1e10), seed: int = 1234):
@@ -116,6 +202,7 @@ def mixed_section(n: int, kappas=(1e2, 1e6, 1e10), seed: int = 1234):
     torch.cuda.synchronize()
     progress(f"mixed: generated n={n} test matrices in {time.perf_counter() - t:.1f} s")
     fl = 2.0 / 3.0 * n ** 3 + 2.0 * n * n
+    ffl = 2.0 / 3.0 * n ** 3          # the factorisation's flops
     eps = 2.0 ** -53
     out = {"n": n, "nrhs": 1, "flops_per_solve": fl, "per_kappa": []}
     A2 = F.DeviceMatrix.empty(n, n)
@@ -145,9 +232,10 @@ def mixed_section(n: int, kappas=(1e2, 1e6, 1e10), seed: int = 1234):
                        "backward_error": eta, "forward_error": fwd}
         for name, kw in (("fp32_auto", dict(prec="fp32", refine="auto", itermax=30, gmres_m=40)),
                          ("fp32_classic", dict(prec="fp32", refine="classic", itermax=30)),
+                         ("bf16_gmres", dict(prec="bf16", refine="gmres", itermax=3, gmres_m=40)),
                          ("fp16_gmres", dict(prec="fp16", refine="gmres", itermax=3, gmres_m=40))):
             bt, st = 1e30, None
-            for rep in range(2 if name == "fp32_auto" else 1):
+            for rep in range(2):
                 X.storage.zero_()
                 A2.storage.copy_(A.storage)
                 torch.cuda.synchronize()
@@ -161,6 +249,9 @@ def mixed_section(n: int, kappas=(1e2, 1e6, 1e10), seed: int = 1234):
                          "inner_iters": st["inner_iters"], "fallback": st["fallback"],
                          "factor_ms": st["factor_ms"], "refine_ms": st["refine_ms"],
                          "kappa_est": st["kappa_est"],
+                         "factor_tflops": ffl / (st["factor_ms"] * 1e-3) / 1e12 if st["factor_ms"] > 0 else 0.0,
+                         "factor_frac_of_mfma_peak": (ffl / (st["factor_ms"] * 1e-3) / 1e12 /
+                                                      PEAK_MFMA[kw["prec"]]) if st["factor_ms"] > 0 else 0.0,
                          "meets_n_eps": eta <= n * eps,
                          "speedup_vs_fp64": rec["fp64"]["ms"] / (1e3 * bt)}
         tot64 += rec["fp64"]["ms"]
@@ -170,6 +261,7 @@ def mixed_section(n: int, kappas=(1e2, 1e6, 1e10), seed: int = 1234):
                  f"fp32 auto {rec['fp32_auto']['ms']:.1f} ms (iter {rec['fp32_auto']['iter']}, "
                  f"inner {rec['fp32_auto']['inner_iters']}), "
                  f"fp32+IR {rec['fp32_classic']['ms']:.1f} ms (iter {rec['fp32_classic']['iter']}), "
+                 f"bf16+GMRES {rec['bf16_gmres']['ms']:.1f} ms (iter {rec['bf16_gmres']['iter']}), "
                  f"fp16+GMRES {rec['fp16_gmres']['ms']:.1f} ms (iter {rec['fp16_gmres']['iter']})")
     out["job"] = {"fp64_solves_per_s": len(kappas) / (tot64 * 1e-3),
                   "mixed_solves_per_s": len(kappas) / (totmx * 1e-3),
@@ -178,8 +270,10 @@ def mixed_section(n: int, kappas=(1e2, 1e6, 1e10), seed: int = 1234):
 
 
 def batched_section(count: int = 20000, seed: int = 4):
-    """20,000 systems n ~ logU[8,512], A ~ N(0,1), nrhs ~ U{1..8} (FP64 and
-    FP32 + FP64 refinement), and 20,000 GEMMs M,N,K ~ logU[16,512], each
+    """20,000 systems n ~ logU[8,512], A ~ N(0,1), nrhs ~ U{1..8} under each
+    precision ladder (FP64; FP32 + classic refinement; BF16 and FP16-scaled
+    LU + GMRES-IR; the rule-based ladder), with the LU tile autotuned per
+    size bin, and 20,000 GEMMs M,N,K ~ logU[16,512], each
     batch one persistent graph-captured launch with an autotuned shape.
     Roofline per problem: max(flops / 78.6 TF, bytes / 8 TB/s), summed."""
     import numpy as np
@@ -203,7 +297,9 @@ def batched_section(count: int = 20000, seed: int = 4):
     by_i = 8.0 * (2.0 * nd ** 2 + 2.0 * nd * nrhs)
     roof = float(np.maximum(fl_i / (PEAK_FP64_MFMA * 1e12), by_i / (PEAK_HBM * 1e9)).sum())
     out = {"count": count, "flops": float(fl_i.sum()), "roofline_ms": 1e3 * roof}
-    for mode, name in ((0, "fp64"), (1, "fp32_ir")):
+    for mode, name in ((F.BATCH_FP64, "fp64"), (F.BATCH_FP32_IR, "fp32_ir"),
+                       (F.BATCH_BF16_GMRES, "bf16_gmres"), (F.BATCH_FP16_GMRES, "fp16_gmres"),
+                       (F.BATCH_RULE, "rule")):
         plan = F.batched.gesv_plan(n, nrhs, A, B, ipiv, info, iters, mode=mode, itmax=10, tune=True)
         ts = []
         for rep in range(3):
@@ -218,10 +314,11 @@ def batched_section(count: int = 20000, seed: int = 4):
         dt = min(ts)
         out[name] = {"ms": 1e3 * dt, "solves_per_s": count / dt, "tflops": float(fl_i.sum()) / dt / 1e12,
                      "roofline_frac": roof / dt, "config": plan.config(),
+                     "bin_tiles": plan.bin_tiles(),
                      "info_nonzero": int((info.cpu().numpy() != 0).sum()),
                      **({"refinement_failures": int((it < 0).sum()),
                          "mean_iters": float(it[it >= 0].mean()) if (it >= 0).any() else 0.0}
-                        if mode == 1 else {})}
+                        if mode != F.BATCH_FP64 else {})}
         plan.close()
         progress(f"batched {name}: {1e3 * dt:.1f} ms, {count / dt:.0f} solves/s")
     del A, B, A0, B0
@@ -364,31 +461,47 @@ def aux_section(n: int = 8192, seed: int = 5):
     return out
 
 
-def selector_section(train_count: int = 3000, eval_count: int = 3000):
-    """Train the selector on two self-generated batches and compare the
-    learned dispatch with FP32 + refinement for every system on a held-out
-    batch (summed in-kernel time)."""
+def selector_section(train_batches: int = 10, train_count: int = 20000, eval_count: int = 20000):
+    """Train the selector on train_batches x train_count self-generated
+    systems, each solved under all 8 classes (4 precision ladders x 2
+    tiles), and compare the learned dispatch on a held-out batch with every
+    fixed policy (summed in-kernel time and batch wall time)."""
     import numpy as np
     import torch
     from flens_amd import selector as S
     t = time.perf_counter()
     Xs, ys, ws = [], [], []
-    for seed in (101, 102):
-        n, nrhs, kind, A, B = S.generate(train_count, seed, nmax=256)
-        f, lab, wt, t0, t1, it1 = S.collect(n, nrhs, A, B)
-        Xs.append(f); ys.append(lab); ws.append(wt)
-    params, hist = S.train(torch.cat(Xs), np.concatenate(ys), np.concatenate(ws), epochs=120)
-    n, nrhs, kind, A, B = S.generate(eval_count, 303, nmax=256)
-    _, lab, _, t0, t1, it1 = S.collect(n, nrhs, A, B)
-    r2 = S.run(n, nrhs, A, B, 2, params=params)
-    t2 = r2["ticks"].astype(np.float64)
-    res = {"train_samples": 2 * train_count, "eval_systems": eval_count,
-           "loss_first": hist[0], "loss_last": hist[-1],
-           "accuracy": float((np.minimum(r2["choice"], 1) == lab).mean()),
-           "ticks_fp32_ir": float(t1.sum()), "ticks_learned": float(t2.sum()),
-           "ticks_fp64": float(t0.sum()), "gain_vs_fp32_ir": float(t1.sum() / t2.sum()),
-           "seconds": time.perf_counter() - t}
-    progress(f"selector: accuracy {res['accuracy']:.3f}, gain x{res['gain_vs_fp32_ir']:.2f}")
+    for b in range(train_batches):
+        n, nrhs, kind, A, B = S.generate(train_count, 101 + b)
+        col = S.collect(n, nrhs, A, B)
+        X, y, w = S.training_set(col)
+        Xs.append(X); ys.append(y); ws.append(w)
+        del A, B, col
+        progress(f"selector: collected batch {b + 1}/{train_batches} ({train_count} systems x 8 classes)")
+    X = torch.cat(Xs)
+    params, hist = S.train(X, np.concatenate(ys), np.concatenate(ws), epochs=30)
+    t_train = time.perf_counter() - t
+    n, nrhs, kind, A, B = S.generate(eval_count, 303)
+    pol = S.policies(n, nrhs, A, B, params)
+    ln = pol["learned"]
+    eta = S.backward_errors(n, nrhs, A, B, ln["X"])
+    ticks = {k: float(v["ticks"].astype(np.float64).sum()) for k, v in pol.items()}
+    wall = {k: v["ms"] for k, v in pol.items()}
+    fixed = {k: v for k, v in ticks.items() if k != "learned"}
+    best = min(fixed, key=fixed.get)
+    res = {"systems_solved_for_training": train_batches * train_count * 8,
+           "training_systems": train_batches * train_count, "samples": int(X.shape[0]),
+           "eval_systems": eval_count, "loss_first": hist[0], "loss_last": hist[-1],
+           "ticks": ticks, "batch_ms": wall, "best_fixed_policy": best,
+           "learned_vs_best_fixed_ticks": ticks["learned"] / fixed[best],
+           "learned_vs_best_fixed_ms": wall["learned"] / wall[best],
+           "learned_ladders": np.bincount(ln["used"] & 3, minlength=4).tolist(),
+           "learned_tiles": np.bincount(ln["used"] >> 2, minlength=2).tolist(),
+           "fp64_tuned_bin_tiles": pol["fp64_tuned"]["bins"],
+           "max_backward_error_over_n_eps": float(eta.max()),
+           "train_seconds": t_train, "seconds": time.perf_counter() - t}
+    progress(f"selector: learned / best fixed ({best}) = {res['learned_vs_best_fixed_ticks']:.3f} "
+             f"(ticks), {res['learned_vs_best_fixed_ms']:.3f} (wall)")
     return res
 
 
@@ -398,6 +511,12 @@ def selector_section(train_count: int = 3000, eval_count: int = 3000):
 # read for a transpose of known size in the same run (fetched = 2.0 x
 # FETCH_SIZE, written = 1.0 x WRITE_SIZE): (2 x 88665.4 + 87862.1) KiB.
 LU8192_GEMM_TRAFFIC_BYTES = (2.0 * 88665.4 + 87862.1) * 1024.0
+# HBM bytes per panel launch in the default workload: rocprofv3 --pmc
+# FETCH_SIZE and WRITE_SIZE over 1792 launches, averaged (962.50 and 1109.02
+# KiB), scaled by what the same counters read for the 1 GiB streaming copy of
+# the peak probe in the same run (fetched = 1.93635 x FETCH_SIZE, written =
+# 0.976493 x WRITE_SIZE).
+LU8192_PANEL_TRAFFIC_BYTES = (1.93635 * 962.50 + 0.976493 * 1109.02) * 1024.0
 
 
 def main():
@@ -466,21 +585,20 @@ def main():
     progress(f"timed region done: {1e3 * (t1 - t0) / args.steps:.2f} ms per step")
     elapsed = t1 - t0
     ms = 1e3 * elapsed / args.steps
-    # The roofline kernel's launch times: the same K steps again with HIP
-    # events around the trailing-update GEMM launches only, on the streams
-    # they run on.  The events add about 2 ms per step, so they stay out of
-    # the pass that gives `value`.
-    h.profile_start(classes=["gemm"])
+    # The roofline kernels' launch times: the same K steps again with HIP
+    # events around the panel and trailing-update GEMM launches only, on the
+    # streams they run on.  The events cost time of their own, so they stay
+    # out of the pass that gives `value`.
+    h.profile_start(classes=["panel", "gemm"])
     t2 = time.perf_counter()
     for _ in range(args.steps):
         step()
     torch.cuda.synchronize()
     t3 = time.perf_counter()
-    gprof = h.profile_stop()["gemm"]
+    rprof = h.profile_stop()
     h.sync()
-    progress(f"roofline pass (GEMM events): {1e3 * (t3 - t2) / args.steps:.2f} ms per step")
-    # per-kernel breakdown from separate steps: the launch profiler's events
-    # cost time of their own, so they stay out of the timed region
+    progress(f"roofline pass (panel + GEMM events): {1e3 * (t3 - t2) / args.steps:.2f} ms per step")
+    # per-kernel breakdown from separate steps
     prof_steps = max(1, min(args.steps, 5))
     progress(f"profiling {prof_steps} more steps (HIP events per launch); checking the solution")
     h.profile_start()
@@ -501,9 +619,50 @@ def main():
     if not eta <= n * eps64:
         raise RuntimeError(f"backward error {eta:.3e} above n*eps64 = {n * eps64:.3e}")
 
+    # The CPU baseline (one core, the oracle's C code: ctypes releases the
+    # interpreter lock during each call) runs in a thread beside the GPU
+    # sections that follow, which leave the host cores idle.
+    cpu_res, cpu_thread = {}, None
+    if not args.no_cpu:
+        import threading
+
+        def cpu_job():
+            try:
+                cpu_res["result"] = cpu_baseline(
+                    n, nrhs, headline_nb=args.nb if args.nb in (64, 128, 256) else 128)
+            except Exception as e:      # re-raised by the main thread
+                cpu_res["error"] = e
+        progress("CPU baseline: oracle getrf + getrs, sampled block steps, 1 core (background)")
+        cpu_thread = threading.Thread(target=cpu_job, daemon=True)
+        cpu_thread.start()
+    # the nb sweep of the same workload (the reference's block-size knob,
+    # ilaenv's GETRF block size)
+    sweep = []
+    if not args.quick:
+        for nb in (64, 128, 256):
+            h.set_lu_blocking(nb, args.ib)
+            for _ in range(2):
+                step()
+            k = max(3, args.steps // 2)
+            torch.cuda.synchronize()
+            ts = time.perf_counter()
+            for _ in range(k):
+                step()
+            torch.cuda.synchronize()
+            dt = (time.perf_counter() - ts) / k
+            sweep.append({"nb": nb, "ib": args.ib, "steps": k, "ms_per_step": 1e3 * dt,
+                          "solves_per_s": 1.0 / dt, "tflops": lu_flops(n, nrhs) / dt / 1e12})
+            progress(f"nb sweep: nb={nb} {1e3 * dt:.2f} ms per step")
+        h.set_lu_blocking(args.nb, args.ib)
+
+    peaks = F.probe_peaks(h)
+    progress("on-device peaks: " + ", ".join(f"{k} {v:.1f}" for k, v in peaks.items()))
+
     fl = lu_flops(n, nrhs)
-    g = gprof
-    achieved = g["flops"] / (g["ms"] * 1e-3) / 1e12 if g["ms"] > 0 else 0.0
+    g, pn = rprof["gemm"], rprof["panel"]
+    g_ach = g["flops"] / (g["ms"] * 1e-3) / 1e12 if g["ms"] > 0 else 0.0
+    p_ach = pn["bytes"] / (pn["ms"] * 1e-3) / 1e9 if pn["ms"] > 0 else 0.0
+    headline = (n, nrhs, args.nb, args.ib) == (8192, 16, 128, 32)
     breakdown = {k: {"launches_per_step": v["launches"] / prof_steps,
                      "ms_per_step": v["ms"] / prof_steps,
                      **({"GB_per_s": v["bytes"] / (v["ms"] * 1e-3) / 1e9} if v["ms"] > 0 and v["bytes"] > 0 else {})}
@@ -519,31 +678,56 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
+        "vs_baseline_is": ("mixed+IR solves/s / plain FP64 LU solves/s over the n=16384 "
+                           "kappa in {1e2,1e6,1e10} job (mixed_vs_fp64.job.speedup); "
+                           "BASELINE.md publishes no number for this metric"),
         "dtype": "f64",
         "data": "synthetic (seeded U[-1,1) generated on the device)",
         "config": {"workload": f"plain FP64 LU + solve (getrf+getrs), n={n}, nrhs={nrhs}, "
                                f"A,B ~ U[-1,1), nb={args.nb}, ib={args.ib}",
                    "n": n, "nrhs": nrhs, "nb": args.nb, "ib": args.ib},
         "tflops": fl / elapsed * args.steps / 1e12,
+        "frac_fp64_peak": fl / elapsed * args.steps / 1e12 / PEAK_FP64_MFMA,
         "backward_error": eta,
         "backward_error_bound": n * eps64,
-        "roofline": {"bound": "mfma", "achieved": achieved, "peak": PEAK_FP64_MFMA,
-                     "unit": "TFLOP/s", "frac": achieved / PEAK_FP64_MFMA,
-                     "traffic": (LU8192_GEMM_TRAFFIC_BYTES
-                                 if (n, nrhs, args.nb, args.ib) == (8192, 16, 128, 32) else None),
-                     "kernel": "gemm_mfma_kernel<double,0,0> (trailing update)",
-                     "launches": g["launches"], "avg_launch_us": 1e3 * g["ms"] / max(1, g["launches"])},
+        "roofline": {"bound": "hbm", "achieved": p_ach, "peak": PEAK_HBM, "unit": "GB/s",
+                     "frac": p_ach / PEAK_HBM,
+                     "traffic": LU8192_PANEL_TRAFFIC_BYTES if headline else None,
+                     "kernel": "panel_kernel<256,2> (cooperative panel factorisation)",
+                     "algorithmic_bytes_per_launch": pn["bytes"] / max(1, pn["launches"]),
+                     "launches": pn["launches"], "avg_launch_us": 1e3 * pn["ms"] / max(1, pn["launches"]),
+                     "share_of_step": pn["ms"] / args.steps / ms,
+                     "frac_of_probe": p_ach / peaks["hbm_copy_gbps"]},
+        "roofline_gemm": {"bound": "mfma", "achieved": g_ach, "peak": PEAK_FP64_MFMA,
+                          "unit": "TFLOP/s", "frac": g_ach / PEAK_FP64_MFMA,
+                          "traffic": LU8192_GEMM_TRAFFIC_BYTES if headline else None,
+                          "kernel": "gemm_mfma_kernel<double,0,0> (trailing update)",
+                          "launches": g["launches"], "avg_launch_us": 1e3 * g["ms"] / max(1, g["launches"]),
+                          "frac_of_probe": g_ach / peaks["fp64_mfma_tflops"]},
+        "peaks_measured": peaks,
         "breakdown": breakdown,
     }
+    if sweep:
+        out["nb_sweep"] = sweep
     if not args.quick:
         out["mixed_vs_fp64"] = mixed_section(args.mixed_n)
+        out["vs_baseline"] = out["mixed_vs_fp64"]["job"]["speedup"]
         out["batched"] = batched_section()
         out["selector"] = selector_section()
         out["aux"] = aux_section()
         out["gemm"] = gemm_section()
-    if not args.no_cpu:
-        progress("CPU baseline: oracle getrf, first block step, 1 core")
-        out["cpu_baseline"] = cpu_baseline(n, nrhs)
+    if cpu_thread is not None:
+        progress("waiting for the CPU baseline")
+        cpu_thread.join()
+        if "error" in cpu_res:
+            raise cpu_res["error"]
+        cb, per_nb = cpu_res["result"]
+        out["cpu_baseline"] = cb
+        for rec in out.get("nb_sweep", []):
+            c = per_nb.get(rec["nb"])
+            if c:
+                rec["cpu_projected_s"] = c["trf_s"] + c["trs_s"]
+                rec["gpu_vs_cpu"] = rec["solves_per_s"] * (c["trf_s"] + c["trs_s"])
     print(json.dumps(out))
 
 

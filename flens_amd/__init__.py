@@ -41,6 +41,7 @@ EXPORTS = [
     "flx_set_lu_blocking", "flx_set_lu_lookahead", "flx_set_lu_side_groups", "flx_dgemm", "flx_dgetrf", "flx_dgetrs", "flx_dgesv",
     "flx_dlaswp", "flx_fill_uniform", "flx_profile_start", "flx_profile_stop",
     "flx_profile_classes", "flx_test_raise_sync_flag", "flx_dgetrf_noalloc", "flx_noalloc_bytes",
+    "flx_probe_mfma", "flx_probe_copy", "flx_batch_bin_tiles",
     "flx_dgesv_mixed", "flx_dsgesv",
     "flx_batch_gesv_create", "flx_batch_gemm_create", "flx_batch_execute", "flx_batch_config",
     "flx_batch_destroy", "flx_mlp_forward", "flx_mlp_grad", "flx_mlp_adam",
@@ -69,10 +70,15 @@ class MixedOpts(C.Structure):
 
 
 class BatchExtra(C.Structure):
-    _fields_ = [("selector", _vp), ("features", _vp), ("ticks", _vp), ("choice", _vp)]
+    _fields_ = [("selector", _vp), ("features", _vp), ("ticks", _vp), ("choice", _vp),
+                ("cfg", _vp), ("inner", _vp), ("used", _vp)]
 
 
-MLP_IN, MLP_HIDDEN, MLP_OUT, MLP_PARAMS = 5, 64, 4, 4804
+MLP_IN, MLP_HIDDEN, MLP_OUT, MLP_PARAMS = 5, 64, 8, 5064
+# solve plan modes (FLX_BATCH_*) and the ladders of a class (ladder + 4 x tile)
+BATCH_FP64, BATCH_FP32_IR, BATCH_LEARNED, BATCH_BF16_GMRES, BATCH_FP16_GMRES, BATCH_RULE, \
+    BATCH_TABLE = range(7)
+LADDERS = ("fp64", "fp32_ir", "bf16_gmres", "fp16_gmres")
 
 
 class MixedStats(C.Structure):
@@ -136,6 +142,9 @@ def lib():
     L.flx_dgetrf_noalloc.argtypes = [_vp, _i, _i, _vp, _i, _vp, _vp]
     L.flx_noalloc_bytes.argtypes = [_i, _i]
     L.flx_noalloc_bytes.restype = C.c_size_t
+    L.flx_probe_mfma.argtypes = [_vp, _i, _i, _dp]
+    L.flx_probe_copy.argtypes = [_vp, _vp, _vp, C.c_size_t, _dp]
+    L.flx_batch_bin_tiles.argtypes = [_vp, _ip, _i]
     L.flx_set_lu_side_groups.argtypes = [_vp, _i, _i]
     L.flx_dgemm.argtypes = [_vp, _i, _i, _i, _i, _i, _d, _vp, _i, _vp, _i, _d, _vp, _i]
     L.flx_dgetrf.argtypes = [_vp, _i, _i, _vp, _i, _vp, _vp]
@@ -649,6 +658,15 @@ class BatchPlan:
         _check(lib().flx_batch_config(self._p, cfg, C.byref(ms)), "flx_batch_config")
         return {"tile_cfg": cfg[0], "grid": cfg[1], "tuned_ms": ms.value}
 
+    def bin_tiles(self):
+        """Solve plans: the autotuned tile of each size bin (n in
+        [8 2^b, 16 2^b), b = 0..5)."""
+        t = (C.c_int * 6)()
+        nb = lib().flx_batch_bin_tiles(self._p, t, 6)
+        if nb < 0:
+            _check(nb, "flx_batch_bin_tiles")
+        return list(t)[:nb]
+
     def close(self):
         if self._p:
             lib().flx_batch_destroy(self._p)
@@ -699,21 +717,24 @@ class batched:
 
     @staticmethod
     def gesv_plan(n, nrhs, A, B, ipiv, info, iters=None, mode=0, itmax=5, tune=True,
-                  selector=None, features=None, ticks=None, choice=None,
-                  handle: Handle | None = None) -> BatchPlan:
-        """A, B: device float64 tensors (packed); ipiv, info, iters, choice:
-        int32; selector, features: float32; ticks: int64 (all on the device)."""
+                  selector=None, features=None, ticks=None, choice=None, cfg=None,
+                  inner=None, used=None, handle: Handle | None = None) -> BatchPlan:
+        """A, B: device float64 tensors (packed); ipiv, info, iters, choice,
+        cfg, inner, used: int32; selector, features: float32; ticks: int64
+        (all on the device).  mode: BATCH_*."""
         h = handle or default_handle()
         n = np.ascontiguousarray(n, dtype=np.int32)
         nrhs = np.ascontiguousarray(nrhs, dtype=np.int32)
         ptr = (lambda t: _vp(t.data_ptr()) if t is not None else None)
-        ex = BatchExtra(ptr(selector), ptr(features), ptr(ticks), ptr(choice))
+        ex = BatchExtra(ptr(selector), ptr(features), ptr(ticks), ptr(choice), ptr(cfg),
+                        ptr(inner), ptr(used))
         p = _vp()
         _check(lib().flx_batch_gesv_create(
             h.ptr, C.byref(p), len(n), n.ctypes.data_as(_ip), nrhs.ctypes.data_as(_ip),
             _vp(A.data_ptr()), _vp(B.data_ptr()), _vp(ipiv.data_ptr()), _vp(info.data_ptr()),
             ptr(iters), mode, itmax, int(bool(tune)), C.byref(ex)), "flx_batch_gesv_create")
-        return BatchPlan(p, (A, B, ipiv, info, iters, selector, features, ticks, choice), h)
+        return BatchPlan(p, (A, B, ipiv, info, iters, selector, features, ticks, choice, cfg,
+                             inner, used), h)
 
     @staticmethod
     def gemm_plan(m, n, k, alpha, beta, A, B, Cm, tune=True,
@@ -745,3 +766,27 @@ def default_handle() -> Handle:
             raise FlxError("no GPU: flens_amd has no CPU path")
         _default = Handle(torch.cuda.current_stream())
     return _default
+
+
+def probe_peaks(handle: Handle | None = None, copy_bytes: int = 1 << 30) -> dict:
+    """Achievable peaks on this device: dense MFMA TFLOP/s for FP64, FP32,
+    BF16 and FP16 operands (flx_probe_mfma) and streaming-copy GB/s
+    (flx_probe_copy over two copy_bytes buffers)."""
+    import torch
+    h = handle or default_handle()
+    L = lib()
+    out = {}
+    for name, kind, iters in (("fp64", 0, 20000), ("fp32", 1, 20000),
+                              ("bf16", 2, 80000), ("fp16", 3, 80000)):
+        v = C.c_double(0.0)
+        _check(L.flx_probe_mfma(h.ptr, kind, iters, C.byref(v)), "flx_probe_mfma")
+        out[f"{name}_mfma_tflops"] = v.value
+    src = torch.empty(copy_bytes // 8, dtype=torch.float64, device="cuda").fill_(1.0)
+    dst = torch.empty_like(src)
+    torch.cuda.synchronize()
+    v = C.c_double(0.0)
+    _check(L.flx_probe_copy(h.ptr, _vp(src.data_ptr()), _vp(dst.data_ptr()), copy_bytes,
+                            C.byref(v)), "flx_probe_copy")
+    out["hbm_copy_gbps"] = v.value
+    del src, dst
+    return out

@@ -18,8 +18,18 @@ namespace {
 
 struct BatchProbH {   // layout of flx::BatchProb (batched.hip)
     long a, b;
-    int n, nrhs, piv, id;
+    int n, nrhs, piv, id, cfg, pad;
 };
+
+// size bins of the solve autotuner: n in [8 * 2^b, 16 * 2^b), b = 0..5
+// (n < 8 in bin 0, n = 512 in bin 5)
+constexpr int SOLVE_BINS = 6;
+inline int solve_bin(int n)
+{
+    int b = 0;
+    while (b < SOLVE_BINS - 1 && n >= (16 << b)) ++b;
+    return b;
+}
 struct GemmProbH {    // layout of flx::GemmProb (batched.hip)
     long a, b, c;
     int m, n, k, tile0;
@@ -32,6 +42,14 @@ struct flx_batch {
     flx_context *h = nullptr;
     int kind = 0;               // 0 solves, 1 GEMMs
     int mode = 0, itmax = 5;
+    int policy = 0;             // batched_gesv policy of the mode
+    std::vector<BatchProbH> tab;   // solves: the problem table (host copy)
+    int bin_tile[SOLVE_BINS] = {0, 0, 0, 0, 0, 0};
+    double *G = nullptr;        // scratch of the low-precision ladders
+    int Ggrid = 0;
+    double *stats = nullptr;    // row statistics per system (pre-pass)
+    float *selT = nullptr;      // the selector transposed for the kernel
+    int *cls = nullptr;         // the selector's class per system
     int count = 0;
     void *probs = nullptr;      // device problem table
     int *counter = nullptr;
@@ -69,8 +87,9 @@ static hipError_t launch(flx_batch *p, hipStream_t s, int tile_cfg, int grid, do
                          double *B, double *C)
 {
     if (p->kind == 0)
-        return batched_gesv(s, p->mode, p->probs, p->count, p->counter, A, B, p->W32, p->R,
-                            p->ipiv, p->info, p->iters, p->itmax, grid, p->out);
+        return batched_gesv(s, p->policy, p->probs, p->count, p->counter, A, B, p->W32, p->R,
+                            grid <= p->Ggrid ? p->G : nullptr, p->ipiv, p->info, p->iters,
+                            p->itmax, grid, p->out);
     return batched_gemm(s, tile_cfg, p->gprobs[tile_cfg], p->count, p->ntiles[tile_cfg],
                         p->counter, p->GA, p->GB, C, grid);
 }
@@ -98,62 +117,134 @@ static int capture(flx_batch *p)
     return FLX_SUCCESS;
 }
 
-// Times each candidate launch shape on copies of the plan's inputs (the
-// solves work in place) and keeps the fastest; cfgs = tile shapes (GEMMs).
+// device buffers and events of one tuning session, released on every exit
+struct TuneScratch {
+    std::vector<void *> bufs;
+    std::vector<hipEvent_t> evs;
+    ~TuneScratch()
+    {
+        for (void *b : bufs) (void)hipFree(b);
+        for (hipEvent_t e : evs) (void)hipEventDestroy(e);
+    }
+    template <typename T> hipError_t alloc(T **p, size_t n)
+    {
+        hipError_t e = hipMalloc(reinterpret_cast<void **>(p), sizeof(T) * std::max<size_t>(1, n));
+        if (e == hipSuccess) bufs.push_back(*p);
+        return e;
+    }
+    hipError_t event(hipEvent_t *e)
+    {
+        hipError_t r = hipEventCreate(e);
+        if (r == hipSuccess) evs.push_back(*e);
+        return r;
+    }
+};
+
+// one timed run of a candidate on copies of the inputs (the solves work in
+// place); returns milliseconds in *ms
+static int timed_run(flx_batch *p, TuneScratch &ts, int cfg, int grid, double *tA, double *tB,
+                     double *tC, float *ms)
+{
+    hipStream_t s = p->h->stream;
+    hipEvent_t e0, e1;
+    FLX_TRY(ts.event(&e0));
+    FLX_TRY(ts.event(&e1));
+    if (p->kind == 0) {
+        FLX_TRY(hipMemcpyAsync(tA, p->A, sizeof(double) * p->a_total, hipMemcpyDeviceToDevice, s));
+        FLX_TRY(hipMemcpyAsync(tB, p->B, sizeof(double) * p->b_total, hipMemcpyDeviceToDevice, s));
+    } else {
+        FLX_TRY(hipMemcpyAsync(tC, p->GC, sizeof(double) * p->c_total, hipMemcpyDeviceToDevice, s));
+    }
+    FLX_TRY(hipEventRecord(e0, s));
+    FLX_TRY(launch(p, s, cfg, grid, tA, tB, tC));
+    FLX_TRY(hipEventRecord(e1, s));
+    FLX_TRY(hipEventSynchronize(e1));
+    FLX_TRY(hipEventElapsedTime(ms, e0, e1));
+    return FLX_SUCCESS;
+}
+
+static int upload_table(flx_batch *p)
+{
+    if (p->count > 0)
+        FLX_TRY(hipMemcpy(p->probs, p->tab.data(), sizeof(BatchProbH) * p->count,
+                          hipMemcpyHostToDevice));
+    return FLX_SUCCESS;
+}
+
+// The autotuner.  Solves: every system runs once with each tile
+// configuration; the in-kernel times, summed per size bin (solve_bin),
+// give each bin its tile (the table's tile bits), then the persistent grid
+// is timed.  GEMMs: tile shape x grid, timed as whole batches.
 static int autotune(flx_batch *p, bool tune)
 {
     const int cus = cu_count(p->h);
-    std::vector<int> grids = {cus, 2 * cus, 4 * cus};
-    std::vector<int> cfgs = {0};
-    if (p->kind == 1) cfgs = {0, 1, 2};
     if (!tune) {
         p->grid = 2 * cus;
         p->tile_cfg = (p->kind == 1) ? 1 : 0;
         return FLX_SUCCESS;
     }
-    hipStream_t s = p->h->stream;
+    TuneScratch ts;
     double *tA = nullptr, *tB = nullptr, *tC = nullptr;
-    int *saved_piv = nullptr;
     if (p->kind == 0) {
-        FLX_TRY(hipMalloc(&tA, sizeof(double) * std::max(1L, p->a_total)));
-        FLX_TRY(hipMalloc(&tB, sizeof(double) * std::max(1L, p->b_total)));
+        FLX_TRY(ts.alloc(&tA, p->a_total));
+        FLX_TRY(ts.alloc(&tB, p->b_total));
     } else {
-        FLX_TRY(hipMalloc(&tC, sizeof(double) * std::max(1L, p->c_total)));
+        FLX_TRY(ts.alloc(&tC, p->c_total));
     }
-    (void)saved_piv;
-    hipEvent_t e0, e1;
-    FLX_TRY(hipEventCreate(&e0));
-    FLX_TRY(hipEventCreate(&e1));
     double best = 1e30;
-    int bcfg = cfgs[0], bgrid = grids[0];
-    for (int cfg : cfgs)
-        for (int grid : grids) {
+    int bcfg = 0, bgrid = 2 * cus;
+    if (p->kind == 0) {
+        long long *ticks = nullptr;
+        FLX_TRY(ts.alloc(&ticks, p->count));
+        const BatchOut saved = p->out;
+        BatchOut tout;
+        tout.tsys = ticks;
+        tout.sel = saved.sel;
+        tout.stats = saved.stats;
+        tout.selT = saved.selT;
+        tout.cls = saved.cls;
+        p->out = tout;
+        double sum[2][SOLVE_BINS] = {};
+        std::vector<long long> h(p->count);
+        int st = FLX_SUCCESS;
+        for (int tile = 0; tile < 2 && st == FLX_SUCCESS; ++tile) {
+            for (auto &e : p->tab) e.cfg = (e.cfg & 3) | (tile << 2);
+            st = upload_table(p);
+            float ms = 0.f;
+            if (st == FLX_SUCCESS) st = timed_run(p, ts, 0, 2 * cus, tA, tB, tC, &ms);
+            if (st == FLX_SUCCESS &&
+                hipMemcpy(h.data(), ticks, sizeof(long long) * p->count, hipMemcpyDeviceToHost) !=
+                    hipSuccess)
+                st = FLX_HIP_ERROR;
+            if (st == FLX_SUCCESS)
+                for (const auto &e : p->tab) sum[tile][solve_bin(e.n)] += (double)h[e.id];
+        }
+        p->out = saved;
+        FLX_RET(st);
+        for (int b = 0; b < SOLVE_BINS; ++b) p->bin_tile[b] = sum[1][b] < sum[0][b] ? 1 : 0;
+        for (auto &e : p->tab) e.cfg = (e.cfg & 3) | (p->bin_tile[solve_bin(e.n)] << 2);
+        FLX_RET(upload_table(p));
+        for (int grid : {cus, 2 * cus}) {
             float ms_best = 1e30f;
             for (int rep = 0; rep < 2; ++rep) {
-                if (p->kind == 0) {
-                    FLX_TRY(hipMemcpyAsync(tA, p->A, sizeof(double) * p->a_total,
-                                           hipMemcpyDeviceToDevice, s));
-                    FLX_TRY(hipMemcpyAsync(tB, p->B, sizeof(double) * p->b_total,
-                                           hipMemcpyDeviceToDevice, s));
-                } else {
-                    FLX_TRY(hipMemcpyAsync(tC, p->GC, sizeof(double) * p->c_total,
-                                           hipMemcpyDeviceToDevice, s));
-                }
-                FLX_TRY(hipEventRecord(e0, s));
-                FLX_TRY(launch(p, s, cfg, grid, tA, tB, tC));
-                FLX_TRY(hipEventRecord(e1, s));
-                FLX_TRY(hipEventSynchronize(e1));
                 float ms = 0.f;
-                FLX_TRY(hipEventElapsedTime(&ms, e0, e1));
+                FLX_RET(timed_run(p, ts, 0, grid, tA, tB, tC, &ms));
                 ms_best = std::min(ms_best, ms);
             }
-            if (ms_best < best) { best = ms_best; bcfg = cfg; bgrid = grid; }
+            if (ms_best < best) { best = ms_best; bgrid = grid; }
         }
-    (void)hipEventDestroy(e0);
-    (void)hipEventDestroy(e1);
-    if (tA) (void)hipFree(tA);
-    if (tB) (void)hipFree(tB);
-    if (tC) (void)hipFree(tC);
+    } else {
+        for (int cfg : {0, 1, 2})
+            for (int grid : {cus, 2 * cus, 4 * cus}) {
+                float ms_best = 1e30f;
+                for (int rep = 0; rep < 2; ++rep) {
+                    float ms = 0.f;
+                    FLX_RET(timed_run(p, ts, cfg, grid, tA, tB, tC, &ms));
+                    ms_best = std::min(ms_best, ms);
+                }
+                if (ms_best < best) { best = ms_best; bcfg = cfg; bgrid = grid; }
+            }
+    }
     p->tile_cfg = bcfg;
     p->grid = bgrid;
     p->tuned_ms = best;
@@ -166,16 +257,24 @@ extern "C" int flx_batch_gesv_create(flx_handle h, flx_batch_t *out, int count, 
                                      const flx_batch_extra *extra)
 {
     if (!h || !out || count < 0 || (count > 0 && (!n || !nrhs || !A || !B || !ipiv || !info)) ||
-        mode < 0 || mode > 2 || itmax < 0 || itmax > 100)
+        mode < 0 || mode > FLX_BATCH_TABLE || itmax < 0 || itmax > 100)
         return FLX_INVALID_ARG;
-    if (mode == 2 && (!extra || !extra->selector)) return FLX_INVALID_ARG;
+    if (mode == FLX_BATCH_LEARNED && (!extra || !extra->selector)) return FLX_INVALID_ARG;
+    if (mode == FLX_BATCH_TABLE && (!extra || !extra->cfg)) return FLX_INVALID_ARG;
     if (batched_prob_bytes() != sizeof(BatchProbH)) return FLX_HIP_ERROR;
+    // the ladder of every system: fixed by the mode, or the caller's table
+    static const int LADDER_OF_MODE[] = {0, 1, 0, 2, 3, 1, 0};
     std::vector<BatchProbH> tab(count);
     long ao = 0, bo = 0;
     int po = 0;
     for (int i = 0; i < count; ++i) {
         if (n[i] < 1 || n[i] > 512 || nrhs[i] < 0 || nrhs[i] > 8) return FLX_UNSUPPORTED;
-        tab[i] = BatchProbH{ao, bo, n[i], nrhs[i], po, i};
+        int cfg = LADDER_OF_MODE[mode];
+        if (mode == FLX_BATCH_TABLE) {
+            cfg = extra->cfg[i];
+            if (cfg < 0 || cfg > 7) return FLX_INVALID_ARG;
+        }
+        tab[i] = BatchProbH{ao, bo, n[i], nrhs[i], po, i, cfg, 0};
         ao += (long)n[i] * n[i];
         bo += (long)n[i] * nrhs[i];
         po += n[i];
@@ -190,8 +289,10 @@ extern "C" int flx_batch_gesv_create(flx_handle h, flx_batch_t *out, int count, 
     p->h = h;
     p->kind = 0;
     p->mode = mode;
+    p->policy = mode == FLX_BATCH_LEARNED ? 1 : mode == FLX_BATCH_RULE ? 2 : 0;
     p->itmax = itmax;
     p->count = count;
+    p->tab = std::move(tab);
     p->A = A; p->B = B; p->ipiv = ipiv; p->info = info; p->iters = iters;
     p->a_total = ao;
     p->b_total = bo;
@@ -200,19 +301,35 @@ extern "C" int flx_batch_gesv_create(flx_handle h, flx_batch_t *out, int count, 
         p->out.feat = extra->features;
         p->out.tsys = extra->ticks;
         p->out.choice = extra->choice;
+        p->out.inner = extra->inner;
+        p->out.used = extra->used;
     }
     auto fail = [&](int st) { flx_batch_destroy(p); return st; };
     if (hipMalloc(&p->probs, sizeof(BatchProbH) * std::max(1, count)) != hipSuccess ||
         hipMalloc(&p->counter, sizeof(int)) != hipSuccess)
         return fail(FLX_ALLOC_FAILED);
-    if (mode >= 1 && (hipMalloc(&p->W32, sizeof(float) * std::max(1L, ao)) != hipSuccess ||
-                      hipMalloc(&p->R, sizeof(double) * std::max(1L, bo)) != hipSuccess))
-        return fail(FLX_ALLOC_FAILED);
-    if (count > 0 &&
-        hipMemcpy(p->probs, tab.data(), sizeof(BatchProbH) * count, hipMemcpyHostToDevice) !=
-            hipSuccess)
-        return fail(FLX_HIP_ERROR);
-    int st = autotune(p, tune != 0);
+    if (mode != FLX_BATCH_FP64) {
+        p->Ggrid = 2 * cu_count(h);
+        if (hipMalloc(&p->W32, sizeof(float) * std::max(1L, ao)) != hipSuccess ||
+            hipMalloc(&p->R, sizeof(double) * std::max(1L, bo)) != hipSuccess ||
+            hipMalloc(&p->G, sizeof(double) * batched_scratch_doubles(p->Ggrid)) != hipSuccess)
+            return fail(FLX_ALLOC_FAILED);
+    }
+    if (mode != FLX_BATCH_FP64 || p->out.feat) {
+        if (hipMalloc(&p->stats, sizeof(double) * 3 * std::max(1, count)) != hipSuccess)
+            return fail(FLX_ALLOC_FAILED);
+        p->out.stats = p->stats;
+    }
+    if (mode == FLX_BATCH_LEARNED) {
+        if (hipMalloc(&p->selT, sizeof(float) * FLX_MLP_PARAMS) != hipSuccess ||
+            hipMalloc(&p->cls, sizeof(int) * std::max(1, count)) != hipSuccess)
+            return fail(FLX_ALLOC_FAILED);
+        p->out.selT = p->selT;
+        p->out.cls = p->cls;
+    }
+    int st = upload_table(p);
+    if (st) return fail(st);
+    st = autotune(p, tune != 0);
     if (st) return fail(st);
     st = capture(p);
     if (st) return fail(st);
@@ -304,6 +421,13 @@ extern "C" int flx_batch_config(flx_batch_t p, int *cfg, double *tuned_ms)
     return FLX_SUCCESS;
 }
 
+extern "C" int flx_batch_bin_tiles(flx_batch_t p, int *tiles, int nbins)
+{
+    if (!p || !tiles || p->kind != 0 || nbins < SOLVE_BINS) return FLX_INVALID_ARG;
+    for (int b = 0; b < SOLVE_BINS; ++b) tiles[b] = p->bin_tile[b];
+    return SOLVE_BINS;
+}
+
 extern "C" int flx_batch_destroy(flx_batch_t p)
 {
     if (!p) return FLX_INVALID_ARG;
@@ -318,6 +442,10 @@ extern "C" int flx_batch_destroy(flx_batch_t p)
     if (p->counter) (void)hipFree(p->counter);
     if (p->W32) (void)hipFree(p->W32);
     if (p->R) (void)hipFree(p->R);
+    if (p->G) (void)hipFree(p->G);
+    if (p->stats) (void)hipFree(p->stats);
+    if (p->selT) (void)hipFree(p->selT);
+    if (p->cls) (void)hipFree(p->cls);
     delete p;
     return FLX_SUCCESS;
 }

@@ -244,7 +244,7 @@ hipError_t fill_const(hipStream_t s, double *v, int n, double c);
 hipError_t sign_of(hipStream_t s, const double *x, double *y, int n);
 
 // ---- precision selector network (mlp.hip) ----------------------------------
-constexpr int MLP_IN = 5, MLP_HIDDEN = 64, MLP_OUT = 4;
+constexpr int MLP_IN = 5, MLP_HIDDEN = 64, MLP_OUT = 8;
 constexpr int MLP_PARAMS = MLP_HIDDEN * MLP_IN + MLP_HIDDEN + MLP_HIDDEN * MLP_HIDDEN +
                            MLP_HIDDEN + MLP_OUT * MLP_HIDDEN + MLP_OUT;
 size_t mlp_part_floats(int nb);
@@ -258,18 +258,28 @@ hipError_t mlp_adam(hipStream_t s, float *P, const float *grad, float *m, float 
 // Problem tables are built by the host in the layouts of batched.hip.
 size_t batched_prob_bytes();
 size_t batched_gemm_prob_bytes();
-// Optional outputs: feat (count x MLP_IN selector features, modes 1-2),
-// tsys (count, in-kernel time per system in 100 MHz ticks), choice (count,
-// the selector's class, mode 2); sel = selector parameters (mode 2).
+// Optional outputs: feat (count x MLP_IN selector features), tsys (count,
+// in-kernel time per system in 100 MHz ticks), choice (count, the
+// selector's class), inner (GMRES steps), used (the ladder | tile << 2 the
+// system ran); sel = selector parameters (policy 1).
 struct BatchOut {
     float *feat = nullptr;
     long long *tsys = nullptr;
     int *choice = nullptr;
+    int *inner = nullptr;
+    int *used = nullptr;
     const float *sel = nullptr;
+    double *stats = nullptr;   // 3 per system: row statistics from a pre-pass kernel
+    float *selT = nullptr;     // MLP_PARAMS floats: the selector transposed for the kernel
+    int *cls = nullptr;        // per system: the selector's pre-factor class
 };
-hipError_t batched_gesv(hipStream_t s, int mode, const void *probs, int nprob, int *counter,
-                        double *A, double *B, float *W32, double *R, int *ipiv, int *info,
-                        int *iters, int itmax, int grid, const BatchOut &out);
+// policy 0: each system's table ladder and tile; 1: the selector network;
+// 2: the rule-based ladder.  G: per-workgroup scratch of the low-precision
+// ladders (batched_scratch_doubles(grid) doubles; null: FP64 only).
+hipError_t batched_gesv(hipStream_t s, int policy, const void *probs, int nprob, int *counter,
+                        double *A, double *B, float *W32, double *R, double *G, int *ipiv,
+                        int *info, int *iters, int itmax, int grid, const BatchOut &out);
+size_t batched_scratch_doubles(int grid);
 hipError_t batched_gemm(hipStream_t s, int tile_cfg, const void *probs, int nprob, int ntiles,
                         int *counter, const double *A, const double *B, double *C, int grid);
 int batched_gemm_tile(int cfg, int *tm, int *tn);

@@ -107,6 +107,19 @@ int flx_profile_stop(flx_handle h, double *out);
  * without the cost of events on every launch. */
 int flx_profile_classes(flx_handle h, unsigned mask);
 
+/* On-device peak probes (no counterpart in the reference: measurement
+ * utilities for roofline denominators).  flx_probe_mfma times a grid that
+ * keeps every SIMD issuing independent MFMA chains of one shape -- kind 0
+ * v_mfma_f64_16x16x4f64, 1 v_mfma_f32_16x16x4f32, 2
+ * v_mfma_f32_16x16x32_bf16, 3 v_mfma_f32_16x16x32_f16 -- for iters
+ * iterations (8 instructions per wave each) and reports dense TFLOP/s, best
+ * of 3 after one untimed run.  flx_probe_copy streams bytes (a multiple of
+ * 32, both pointers 32-byte aligned device memory) from src to dst and
+ * reports read + write GB/s, best of 5.  Both synchronise the handle's
+ * stream. */
+int flx_probe_mfma(flx_handle h, int kind, int iters, double *tflops);
+int flx_probe_copy(flx_handle h, const void *src, void *dst, size_t bytes, double *gbps);
+
 /* ------------------------------------------------- device-pointer entries */
 /* C <- beta*C + alpha*op(A)*op(B), trans = 0 NoTrans / 1 Trans.
  * Replaces cxxblas::gemm (cxxblas/level3/gemm.tcc:149-299, reached from
@@ -307,10 +320,13 @@ int flx_load_densevector(flx_handle h, const char *path, void *x, int capacity, 
  *   solves: A_i (n_i x n_i) at offset sum_{j<i} n_j^2, B_i (n_i x nrhs_i)
  *           at sum_{j<i} n_j nrhs_j (overwritten by X_i), ipiv_i at
  *           sum_{j<i} n_j; 1 <= n_i <= 512, 0 <= nrhs_i <= 8; info[i] as
- *           LAPACK's INFO; mode 0 = FP64 LU, mode 1 = FP32 LU + FP64
- *           refinement (itmax steps, stop at ||r|| <= sqrt(n) 2^-53 (||A||
- *           ||x|| + ||b||)) with FP64 LU when it fails; iters[i] (optional)
- *           = refinement steps or a dsgesv-style negative code.
+ *           LAPACK's INFO; the mode (FLX_BATCH_*) sets each system's
+ *           precision ladder; iters[i] (optional) = refinement steps or a
+ *           dsgesv-style negative code (-2 overflow in the low format, -3
+ *           singular low-precision factor, -4 sent to FP64 by the selector
+ *           or the rule, -itmax-1 no convergence), the system then solved
+ *           by the FP64 LU.  Refinement stops at ||r_c||_inf <= sqrt(n)
+ *           2^-53 (||A||_inf ||x_c||_inf + ||b_c||_inf) for every column.
  *   GEMMs:  C_i <- beta_i C_i + alpha_i A_i B_i (beta applied to C first,
  *           as the reference's generic gemm), A_i m_i x k_i at
  *           sum m_j k_j, B_i k_i x n_i at sum k_j n_j, C_i at sum m_j n_j.
@@ -319,23 +335,48 @@ int flx_load_densevector(flx_handle h, const char *path, void *x, int capacity, 
  * of the inputs when tune = 1, and captures the batch into a HIP graph;
  * flx_batch_execute replays it on the handle's stream. */
 typedef struct flx_batch *flx_batch_t;
-/* Optional per-system outputs / inputs of a solve plan (device pointers,
- * any may be NULL): the selector's parameters (mode 2), its input features
- * (count x FLX_MLP_IN, modes 1-2), the in-kernel time of every system in
- * 100 MHz ticks, and the selector's class (mode 2). */
+/* Solve plan modes.  The ladders: FP64 LU; FP32 LU + classic refinement
+ * (FP64 residuals); BF16-operand LU + GMRES-IR; FP16-operand LU of the
+ * row-scaled matrix (rows to 0.1 x the largest FP16 value) + GMRES-IR
+ * (FP64 GMRES vectors, the low-precision factor as preconditioner).  In
+ * the BF16 / FP16 LU the trailing updates multiply operands rounded to the
+ * format with FP32 accumulation; factors are kept in FP32.
+ *   FLX_BATCH_FP64, _FP32_IR, _BF16_GMRES, _FP16_GMRES: one ladder for all;
+ *   FLX_BATCH_LEARNED: the selector network (extra->selector) reads each
+ *     system's features before any factor -- log2 n / 9, nrhs / 8, the
+ *     condition feature (-0.25: not yet estimated), log10 of the smallest
+ *     diagonal-dominance ratio / 4, log10 of the max/min row-norm ratio /
+ *     12 -- and picks a class = ladder + 4 x tile; after a low-precision
+ *     factor it reads them again with log10 kappa_inf / 16 (kappa_inf =
+ *     ||A||_inf times a lower bound of ||A^{-1}||_inf from two solves with
+ *     the factor) and sends the system to the FP64 LU when it then picks
+ *     the FP64 ladder;
+ *   FLX_BATCH_RULE: the rule-based ladder: FP32 LU, classic refinement
+ *     while kappa_inf <= 1e6, else FP64;
+ *   FLX_BATCH_TABLE: extra->cfg[i] = ladder + 4 x tile per system.
+ * Every failed or skipped refinement ends in the FP64 LU of A.  tile 0 /
+ * 1: the trailing update works in strips of two / one 16-column tiles per
+ * wave; with tune = 1 the autotuner picks it per size bin (n in [8 2^b,
+ * 16 2^b), b = 0..5) from the in-kernel time of every system under both,
+ * except where the selector or the table sets it. */
+enum { FLX_BATCH_FP64 = 0, FLX_BATCH_FP32_IR = 1, FLX_BATCH_LEARNED = 2,
+       FLX_BATCH_BF16_GMRES = 3, FLX_BATCH_FP16_GMRES = 4, FLX_BATCH_RULE = 5,
+       FLX_BATCH_TABLE = 6 };
+/* Optional per-system inputs / outputs of a solve plan (device pointers,
+ * any may be NULL): the selector's parameters (FLX_BATCH_LEARNED), the
+ * features as last evaluated (count x FLX_MLP_IN), the in-kernel time of
+ * every system in 100 MHz ticks, the selector's pre-factor class, the
+ * configuration table (FLX_BATCH_TABLE), the GMRES steps taken and the
+ * ladder + 4 x tile each system ran. */
 typedef struct flx_batch_extra {
     const float *selector;
     float *features;
     long long *ticks;
     int *choice;
+    const int *cfg;
+    int *inner;
+    int *used;
 } flx_batch_extra;
-/* mode 2 = learned: after the FP32 factor the selector network (FP32 MLP,
- * flx_mlp_*) reads the system's features -- log2 n / 9, nrhs / 8,
- * log10 kappa_inf / 16 with kappa_inf = ||A||_inf times a lower bound of
- * ||A^{-1}||_inf from two solves, log10 of the smallest diagonal-dominance
- * ratio / 4, log10 of the max/min row-norm ratio / 12 -- and either goes on
- * with the refinement (classes 1-3) or factors in FP64 at once (class 0);
- * iters[i] = -4 records that choice. */
 int flx_batch_gesv_create(flx_handle h, flx_batch_t *plan, int count,
                           const int *n, const int *nrhs, double *A,
                           double *B, int *ipiv, int *info, int *iters,
@@ -350,15 +391,19 @@ int flx_batch_execute(flx_batch_t plan);
 /* cfg[0] = GEMM tile shape (0: 32x32, 1: 64x64, 2: 128x64), cfg[1] = grid;
  * *tuned_ms = the winning candidate's time (0 when not tuned). */
 int flx_batch_config(flx_batch_t plan, int *cfg, double *tuned_ms);
+/* Solve plans: the tile chosen for each of the nbins >= 6 size bins;
+ * returns the number of bins (6). */
+int flx_batch_bin_tiles(flx_batch_t plan, int *tiles, int nbins);
 int flx_batch_destroy(flx_batch_t plan);
 
 /* ------------------------------------------- precision selector network */
 /* FP32 MLP  in (FLX_MLP_IN) -> 64 ReLU -> 64 ReLU -> FLX_MLP_OUT logits
- * ({FP64, FP32+IR, BF16+GMRES-IR, FP16-scaled+GMRES-IR}), parameters in one
+ * (class = ladder + 4 x tile, ladders {FP64, FP32+IR, BF16+GMRES-IR,
+ * FP16-scaled+GMRES-IR}, tiles as in FLX_BATCH_*), parameters in one
  * flat array: W1[64][IN], b1[64], W2[64][64], b2[64], W3[OUT][64], b3[OUT].
  * Forward, mean softmax cross-entropy gradient (deterministic) and an Adam
  * step, on device pointers.  No counterpart in the reference. */
-enum { FLX_MLP_IN = 5, FLX_MLP_HIDDEN = 64, FLX_MLP_OUT = 4, FLX_MLP_PARAMS = 4804 };
+enum { FLX_MLP_IN = 5, FLX_MLP_HIDDEN = 64, FLX_MLP_OUT = 8, FLX_MLP_PARAMS = 5064 };
 int flx_mlp_forward(flx_handle h, const float *params, const float *X, int nb,
                     float *logits);
 int flx_mlp_grad(flx_handle h, const float *params, const float *X,

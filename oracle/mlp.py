@@ -9,7 +9,7 @@ from __future__ import annotations
 
 import numpy as np
 
-IN, H, OUT = 5, 64, 4
+IN, H, OUT = 5, 64, 8
 SIZES = [("W1", (H, IN)), ("b1", (H,)), ("W2", (H, H)), ("b2", (H,)), ("W3", (OUT, H)),
          ("b3", (OUT,))]
 PARAMS = sum(int(np.prod(s)) for _, s in SIZES)

@@ -197,6 +197,18 @@ def dgetrf_steps(a, nb, nsteps):
     return a
 
 
+def dgetrf_step_at(a, i0: int, j0: int, rows: int, cols: int, nb: int):
+    """One block step of the blocked getrf on the sub-block
+    a[i0:i0+rows, j0:j0+cols] of the Fortran-ordered square array a, with
+    a's own leading dimension (timing samples of the step at depth j0)."""
+    assert a.flags.f_contiguous and a.dtype == np.float64
+    n = a.shape[0]
+    assert 0 <= i0 and 0 <= j0 and rows > 0 and cols > 0
+    assert i0 + rows <= n and j0 + cols <= a.shape[1]
+    p = C.cast(C.c_void_p(a.ctypes.data + 8 * (i0 + j0 * n)), _dp)
+    lib().orc_dgetrf_steps(rows, cols, p, n, nb, 1)
+
+
 def getrf_steps_flops(m: int, n: int, nb: int, nsteps: int) -> float:
     """Exact flop count of the first nsteps block steps (panel getf2 with
     one reciprocal-multiply per subdiagonal entry, unit-lower trsm, GEMM)."""

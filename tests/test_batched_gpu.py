@@ -46,16 +46,20 @@ def unpack(vec, shapes):
     return out
 
 
-def run_solves(gpu, n, nrhs, A, B, mode, tune=False, reps=1):
+def run_solves(gpu, n, nrhs, A, B, mode, tune=False, reps=1, cfg=None, itmax=5):
     import torch
     dev = "cuda"
+    cfgt = None
+    if cfg is not None:
+        cfgt = torch.from_numpy(np.broadcast_to(np.asarray(cfg, np.int32), (len(n),)).copy()).to(dev)
     At = torch.from_numpy(pack(A)).to(dev)
     Bt = torch.from_numpy(pack(B)).to(dev)
     A0, B0 = At.clone(), Bt.clone()
     ipiv = torch.zeros(int(n.sum()), dtype=torch.int32, device=dev)
     info = torch.zeros(len(n), dtype=torch.int32, device=dev)
     iters = torch.zeros(len(n), dtype=torch.int32, device=dev)
-    plan = gpu.batched.gesv_plan(n, nrhs, At, Bt, ipiv, info, iters, mode=mode, tune=tune)
+    plan = gpu.batched.gesv_plan(n, nrhs, At, Bt, ipiv, info, iters, mode=mode, tune=tune,
+                                 cfg=cfgt, itmax=itmax)
     outs = []
     for _ in range(reps):
         At.copy_(A0)
@@ -64,9 +68,10 @@ def run_solves(gpu, n, nrhs, A, B, mode, tune=False, reps=1):
         torch.cuda.synchronize()
         outs.append((At.cpu().numpy(), Bt.cpu().numpy(), ipiv.cpu().numpy(), info.cpu().numpy(),
                      iters.cpu().numpy()))
-    cfg = plan.config()
+    conf = plan.config()
+    conf["bin_tiles"] = plan.bin_tiles()
     plan.close()
-    return outs, cfg
+    return outs, conf
 
 
 def test_batched_fp64_solves_vs_oracle(gpu):
@@ -133,6 +138,73 @@ def test_batched_fp64_edge_sizes(gpu):
         assert eta <= n[i] * EPS, (n[i], eta, outs[0][4][i])
 
 
+def test_batched_fp64_edge_sizes_one_tile_strips(gpu):
+    """The FP64 LU with the other tile configuration (one 16-column tile per
+    wave strip of the trailing update): the same factors, pivots and
+    solutions as the oracle at every boundary size."""
+    rng = np.random.default_rng(22)
+    n = np.array(EDGE_N, dtype=np.int32)
+    nrhs = rng.integers(1, 9, len(n)).astype(np.int32)
+    A = [np.asfortranarray(rng.standard_normal((k, k))) for k in n]
+    B = [np.asfortranarray(rng.standard_normal((k, r))) for k, r in zip(n, nrhs)]
+    (outs, _) = run_solves(gpu, n, nrhs, A, B, mode=gpu.BATCH_TABLE, cfg=4)
+    lu_all, x_all, piv_all, info, _ = outs[0]
+    lus = unpack(lu_all, [(k, k) for k in n])
+    xs = unpack(x_all, [(k, r) for k, r in zip(n, nrhs)])
+    po = np.concatenate([[0], np.cumsum(n)])
+    for i in range(len(n)):
+        piv = piv_all[po[i]:po[i + 1]]
+        rpiv, rinfo, gap = O.dgetrf_gaps(A[i].copy(order="F"))
+        assert info[i] == rinfo == 0, (n[i], info[i], rinfo)
+        d = np.nonzero(piv != rpiv)[0]
+        assert d.size == 0 or gap[d[0]] < TAU, (n[i], d[:3])
+        assert O.dget01(A[i], lus[i], piv) < 30, n[i]
+        assert O.dget02(A[i], xs[i], B[i]) < 30, n[i]
+
+
+def backward_error(a, b, x):
+    r = b - a @ x
+    an = np.abs(a).sum(1).max()
+    return max(np.abs(r[:, j]).max() / (an * np.abs(x[:, j]).max() + np.abs(b[:, j]).max())
+               for j in range(b.shape[1]))
+
+
+@pytest.mark.parametrize("mode", ["BATCH_BF16_GMRES", "BATCH_FP16_GMRES", "BATCH_RULE"])
+def test_batched_low_precision_ladders(gpu, mode):
+    """BF16-operand and FP16-scaled LU + GMRES-IR, and the rule-based ladder,
+    both tiles: every system reaches the FP64 backward-error level (through
+    the FP64 LU where refinement fails), most by refinement; the
+    kappa = 1e13 systems end in the FP64 LU."""
+    count = 160
+    ill = {3, 40, 41, 90}
+    n, nrhs, A, B = make_solves(count, 14, illcond=ill)
+    m = getattr(gpu, mode)
+    for tile_cfg in (None, 1):
+        if tile_cfg is None:
+            outs, _ = run_solves(gpu, n, nrhs, A, B, mode=m, itmax=10)
+        else:
+            ladder = {"BATCH_BF16_GMRES": 2, "BATCH_FP16_GMRES": 3, "BATCH_RULE": 1}[mode]
+            if mode == "BATCH_RULE":
+                outs, _ = run_solves(gpu, n, nrhs, A, B, mode=m, itmax=10, tune=True)
+            else:
+                outs, _ = run_solves(gpu, n, nrhs, A, B, mode=gpu.BATCH_TABLE, itmax=10,
+                                     cfg=ladder + 4)
+        lu_all, x_all, piv_all, info, iters = outs[0]
+        xs = unpack(x_all, [(k, r) for k, r in zip(n, nrhs)])
+        conv = 0
+        for i in range(count):
+            assert info[i] == 0, (i, info[i])
+            eta = backward_error(A[i], B[i], xs[i])
+            assert eta <= n[i] * EPS, (mode, tile_cfg, i, n[i], eta, iters[i])
+            if i in ill and n[i] > 4:
+                assert iters[i] < 0, (i, iters[i])
+            conv += iters[i] >= 0
+        # FP16 (unit roundoff 2^-11) leaves Gaussian systems with kappa u
+        # near 1 to the FP64 LU more often
+        need = 0.8 * count if mode == "BATCH_FP16_GMRES" else count - len(ill) - 10
+        assert conv >= need, (mode, tile_cfg, conv)
+
+
 def test_batched_mixed_solves_and_fallback(gpu):
     count = 200
     ill = {3, 40, 41, 90}
@@ -163,6 +235,7 @@ def test_batched_graph_replays_and_tuning(gpu):
             assert np.array_equal(u, v)
     outs_t, cfg_t = run_solves(gpu, n, nrhs, A, B, mode=1, tune=True, reps=2)
     assert cfg_t["grid"] > 0 and cfg_t["tuned_ms"] > 0
+    assert len(cfg_t["bin_tiles"]) == 6 and set(cfg_t["bin_tiles"]) <= {0, 1}
     assert np.array_equal(outs_t[0][1], outs_t[1][1])
 
 

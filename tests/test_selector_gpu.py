@@ -41,45 +41,44 @@ def test_network_kernels_match_cpu_oracle(gpu):
     assert np.abs(pt.cpu().numpy() - pr).max() <= 1e-5
 
 
-def test_trained_selector_beats_dispatch_without_it(gpu):
+def test_selector_trained_on_own_solves_dispatches_within_best_fixed_policy(gpu):
+    """Train on batches solved under every class (4 ladders x 2 tiles), then
+    dispatch a held-out batch with the network fused into the persistent
+    kernel.  Every system must reach the FP64 backward-error level; the
+    learned dispatch's summed in-kernel time must stay within 1 % (timing
+    noise) of the best fixed policy (every class for all systems, FP64 and
+    the rule-based ladder with autotuned tiles) and at most 3/4 of every
+    fixed policy that uses a low-precision ladder: on this kernel the
+    low-precision ladders are slower than FP64 for every system, so the
+    network has to learn to keep the systems away from them and to pick
+    the tile."""
     import torch
     from flens_amd import selector as S
     Xs, ys, ws = [], [], []
-    for seed in (101, 102):
-        n, nrhs, kind, A, B = S.generate(3000, seed, nmax=256)
-        f, lab, wt, t0, t1, it1 = S.collect(n, nrhs, A, B)
-        Xs.append(f)
-        ys.append(lab)
-        ws.append(wt)
+    for seed in (101, 102, 103):
+        n, nrhs, kind, A, B = S.generate(6000, seed)
+        col = S.collect(n, nrhs, A, B)
+        X, y, w = S.training_set(col)
+        Xs.append(X)
+        ys.append(y)
+        ws.append(w)
     X = torch.cat(Xs)
     y = np.concatenate(ys)
     w = np.concatenate(ws)
-    params, hist = S.train(X, y, w, epochs=120, lr=3e-3, batch=2048)
+    params, hist = S.train(X, y, w, epochs=40)
     assert hist[-1] < hist[0]
-    # held-out batch: learned dispatch (mode 2) against FP32+IR for all (mode 1)
-    n, nrhs, kind, A, B = S.generate(3000, 303, nmax=256)
-    _, lab, _, t0, t1, it1 = S.collect(n, nrhs, A, B)
-    r2 = S.run(n, nrhs, A, B, 2, params=params)
-    t2 = r2["ticks"].astype(np.float64)
-    assert (r2["info"] == 0).all()
-    # every system still reaches the FP64 backward-error level
-    Xh = r2["X"].cpu().numpy()
-    Ah = A.cpu().numpy()
-    Bh = B.cpu().numpy()
-    ao = bo = 0
-    for i in range(len(n)):
-        k, r = int(n[i]), int(nrhs[i])
-        a = Ah[ao:ao + k * k].reshape(k, k).T
-        b = Bh[bo:bo + k * r].reshape(r, k).T
-        x = Xh[bo:bo + k * r].reshape(r, k).T
-        res = b - a @ x
-        eta = (np.abs(res).max(0) / (np.abs(a).sum(1).max() * np.abs(x).max(0) + np.abs(b).max(0))).max()
-        assert eta <= k * 2.0 ** -53, (i, k, eta)
-        ao += k * k
-        bo += k * r
-    acc = float((np.minimum(r2["choice"], 1) == lab).mean())
-    print(f"selector accuracy {acc:.3f}; summed ticks: fp64 {t0.sum():.3e} fp32+ir {t1.sum():.3e} "
-          f"learned {t2.sum():.3e}")
-    assert acc >= 0.9
-    assert 0.0 < lab.mean() < 1.0          # both outcomes occur in the held-out batch
-    assert t2.sum() < t1.sum()
+    n, nrhs, kind, A, B = S.generate(4000, 303)
+    pol = S.policies(n, nrhs, A, B, params)
+    ln = pol["learned"]
+    assert (ln["info"] == 0).all()
+    eta = S.backward_errors(n, nrhs, A, B, ln["X"])
+    assert eta.max() <= 1.0, (int(eta.argmax()), eta.max())
+    ticks = {k: float(v["ticks"].astype(np.float64).sum()) for k, v in pol.items()}
+    fixed = {k: v for k, v in ticks.items() if k != "learned"}
+    lowp = {k: v for k, v in fixed.items()
+            if k in ("rule_tuned",) or (k.startswith("class") and int(k[5:]) % 4 != 0)}
+    best = min(fixed, key=fixed.get)
+    print("summed ticks:", {k: f"{v:.3e}" for k, v in ticks.items()}, "best fixed:", best,
+          "chosen ladders:", np.bincount(ln["used"] & 3, minlength=4))
+    assert ticks["learned"] <= 1.01 * fixed[best]
+    assert ticks["learned"] <= 0.75 * min(lowp.values())
