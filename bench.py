@@ -693,7 +693,7 @@ def main():
         "roofline": {"bound": "hbm", "achieved": p_ach, "peak": PEAK_HBM, "unit": "GB/s",
                      "frac": p_ach / PEAK_HBM,
                      "traffic": LU8192_PANEL_TRAFFIC_BYTES if headline else None,
-                     "kernel": "panel_kernel<256,2> (cooperative panel factorisation)",
+                     "kernel": "panel_kernel<256,2,32> (cooperative panel factorisation)",
                      "algorithmic_bytes_per_launch": pn["bytes"] / max(1, pn["launches"]),
                      "launches": pn["launches"], "avg_launch_us": 1e3 * pn["ms"] / max(1, pn["launches"]),
                      "share_of_step": pn["ms"] / args.steps / ms,

@@ -300,7 +300,7 @@ extern "C" int flx_dgetrf(flx_handle h, int m, int n, double *A, int lda, int *i
         FLX_TRY(fill_int(h->stream, info, 1, 0));
         return FLX_SUCCESS;
     }
-    if (m > 16384) return FLX_UNSUPPORTED;
+    if (m > LU_N_MAX) return FLX_UNSUPPORTED;
     return dgetrf_dev(h, m, n, A, lda, ipiv, info, false);
 }
 
@@ -364,7 +364,7 @@ extern "C" int flx_dgesv(flx_handle h, int n, int nrhs, double *A, int lda, int 
     if (!h || n < 0 || nrhs < 0 || lda < (n > 1 ? n : 1) || ldb < (n > 1 ? n : 1))
         return FLX_INVALID_ARG;
     if (n == 0) { if (info_host) *info_host = 0; return FLX_SUCCESS; }
-    if (n > 16384) return FLX_UNSUPPORTED;
+    if (n > LU_N_MAX) return FLX_UNSUPPORTED;
     FLX_TRY(ensure(h, h->info, sizeof(int)));
     int *dinfo = (int *)h->info.p;
     int st = dgetrf_dev(h, n, n, A, lda, ipiv, dinfo, true);
@@ -609,7 +609,7 @@ extern "C" void dgetrf_(const int *M, const int *N, double *A, const int *LDA, i
     if (m == 0 || n == 0) return;
     flx_context *h = default_handle();
     int st = h ? FLX_SUCCESS : FLX_HIP_ERROR;
-    if (h && m > 16384) st = FLX_UNSUPPORTED;
+    if (h && m > LU_N_MAX) st = FLX_UNSUPPORTED;
     if (st == FLX_SUCCESS) {
         auto run = [&]() -> int {
             FLX_TRY(ensure(h, h->A, sizeof(double) * (size_t)m * n));
@@ -687,7 +687,7 @@ extern "C" void dgesv_(const int *N, const int *NRHS, double *A, const int *LDA,
     if (n == 0) return;
     flx_context *h = default_handle();
     int st = h ? FLX_SUCCESS : FLX_HIP_ERROR;
-    if (h && n > 16384) st = FLX_UNSUPPORTED;
+    if (h && n > LU_N_MAX) st = FLX_UNSUPPORTED;
     if (st == FLX_SUCCESS) {
         auto run = [&]() -> int {
             FLX_TRY(ensure(h, h->A, sizeof(double) * (size_t)n * n));
@@ -734,7 +734,7 @@ extern "C" void dsgesv_(const int *N, const int *NRHS, double *A, const int *LDA
     if (n == 0) return;
     flx_context *h = default_handle();
     int st = h ? FLX_SUCCESS : FLX_HIP_ERROR;
-    if (h && n > 16384) st = FLX_UNSUPPORTED;
+    if (h && n > LU_N_MAX) st = FLX_UNSUPPORTED;
     if (st == FLX_SUCCESS) {
         auto run = [&]() -> int {
             const size_t rb = sizeof(double) * (size_t)n * (nrhs > 0 ? nrhs : 1);
@@ -877,7 +877,7 @@ extern "C" void dgecon_(const char *NORM, const int *N, const double *A, const i
     const int n = *N;
     flx_context *h = default_handle();
     int st = h ? FLX_SUCCESS : FLX_HIP_ERROR;
-    if (h && n > 16384) st = FLX_UNSUPPORTED;
+    if (h && n > LU_N_MAX) st = FLX_UNSUPPORTED;
     if (st == FLX_SUCCESS) {
         auto run = [&]() -> int {
             FLX_TRY(ensure(h, h->A, sizeof(double) * (size_t)(n > 0 ? n : 1) * (n > 0 ? n : 1)));
@@ -918,7 +918,7 @@ extern "C" void dgerfs_(const char *TRANS, const int *N, const int *NRHS, const 
     }
     flx_context *h = default_handle();
     int st = h ? FLX_SUCCESS : FLX_HIP_ERROR;
-    if (h && n > 16384) st = FLX_UNSUPPORTED;
+    if (h && n > LU_N_MAX) st = FLX_UNSUPPORTED;
     if (st == FLX_SUCCESS) {
         auto run = [&]() -> int {
             const size_t nn = sizeof(double) * (size_t)n * n, nb = sizeof(double) * (size_t)n * nrhs;
@@ -980,7 +980,7 @@ extern "C" void dgesvx_(const char *FACT, const char *TRANS, const int *N, const
     }
     flx_context *h = default_handle();
     int st = h ? FLX_SUCCESS : FLX_HIP_ERROR;
-    if (h && n > 16384) st = FLX_UNSUPPORTED;
+    if (h && n > LU_N_MAX) st = FLX_UNSUPPORTED;
     if (st == FLX_SUCCESS) {
         auto run = [&]() -> int {
             const size_t nn = sizeof(double) * (size_t)l * l;
@@ -1052,7 +1052,7 @@ extern "C" void dgetri_(const int *N, double *A, const int *LDA, const int *IPIV
     if (*LWORK == -1 || n == 0) return;
     flx_context *h = default_handle();
     int st = h ? FLX_SUCCESS : FLX_HIP_ERROR;
-    if (h && n > 16384) st = FLX_UNSUPPORTED;
+    if (h && n > LU_N_MAX) st = FLX_UNSUPPORTED;
     if (st == FLX_SUCCESS) {
         auto run = [&]() -> int {
             FLX_TRY(ensure(h, h->A, sizeof(double) * (size_t)n * n));

@@ -1004,7 +1004,7 @@ extern "C" int flx_dgesvx(flx_handle h, char fact, int trans, int n, int nrhs, d
         nrhs < 0 || lda < l || ldaf < l || ldb < l || ldx < l)
         return FLX_INVALID_ARG;
     if (fact != 'N' && fact != 'E' && fact != 'F') return FLX_INVALID_ARG;
-    if (n > 16384) return FLX_UNSUPPORTED;
+    if (n > LU_N_MAX) return FLX_UNSUPPORTED;
     Aux a(h);
     hipStream_t s = h->stream;
     const double small = SAFEMIN, big = 1.0 / small;

@@ -101,7 +101,10 @@ size_t lu_exch_words();
 // [count, dst_0, src_0, dst_1, src_1, ...] in absolute rows.
 constexpr int PANEL_ROWS_PER_WG = 512;
 constexpr int PANEL_LIST_SLICE = 1 + 2 * 64;
-constexpr int PANEL_GROUPS_MAX = 32;
+constexpr int PANEL_GROUPS_MAX = 64;
+// largest matrix order of the FP64 LU and the paths built on it (panel
+// heights up to PANEL_GROUPS_MAX x PANEL_ROWS_PER_WG rows)
+constexpr int LU_N_MAX = 32768;
 inline int panel_groups(int mp) { return (mp + PANEL_ROWS_PER_WG - 1) / PANEL_ROWS_PER_WG; }
 // Panel factorisation of the mp x w row-major panel P (w <= 32); piv gets
 // 0-based absolute pivot rows, the composed interchanges go to wk.list.

@@ -40,7 +40,7 @@ namespace flx {
 // each does only after it finished reading the previous one.
 // ---------------------------------------------------------------------------
 constexpr int PANEL_W = 32;             // panel width of the cooperative kernel
-constexpr int PANEL_ROWS_MAX = 16384;   // panel height limit
+constexpr int PANEL_ROWS_MAX = LU_N_MAX;   // panel height limit
 constexpr int PANEL_WMAX = 64;
 
 // ---------------------------------------------------------------------------
@@ -156,16 +156,17 @@ constexpr int LA_NQ = LA_ROWS * LA_CR / 256;   // right-part elements per thread
 static_assert(LU_STAGE_TOP == RP_MAXE, "stage layout");
 
 // The interchange list's slices gathered into LDS with every load issued at
-// once (load_list walks each slice serially): 256 threads x 8 slots cover
+// once (load_list walks each slice serially): 256 threads x SL slots cover
 // PANEL_GROUPS_MAX slices of up to 64 entries.  Returns the entry count.
 FLX_DEV int load_list_par(const int *__restrict__ list, int G, int m, int *s_dst, int *s_src,
                           int *s_cnt)
 {
     const int t = threadIdx.x;
-    static_assert(PANEL_GROUPS_MAX * 64 == 256 * 8, "slot map");
-    int d[8], sr[8];
+    constexpr int SL = PANEL_GROUPS_MAX * 64 / 256;   // slots per thread
+    static_assert(PANEL_GROUPS_MAX * 64 == 256 * SL, "slot map");
+    int d[SL], sr[SL];
 #pragma unroll
-    for (int q = 0; q < 8; ++q) {
+    for (int q = 0; q < SL; ++q) {
         const int slot = t + 256 * q, g = slot >> 6, e = slot & 63;
         const int *sl = list + (size_t)(g < G ? g : 0) * PANEL_LIST_SLICE;
         d[q] = sl[1 + 2 * e];
@@ -184,7 +185,7 @@ FLX_DEV int load_list_par(const int *__restrict__ list, int G, int m, int *s_dst
     }
     __syncthreads();
 #pragma unroll
-    for (int q = 0; q < 8; ++q) {
+    for (int q = 0; q < SL; ++q) {
         const int slot = t + 256 * q, g = slot >> 6, e = slot & 63;
         if (e < s_cnt[g]) {
             const int o = s_cnt[PANEL_GROUPS_MAX + 1 + g] + e;

@@ -25,7 +25,7 @@
 
 namespace flx {
 
-constexpr int PANEL_ROWS_MAX = 16384;   // panel height limit
+constexpr int PANEL_ROWS_MAX = LU_N_MAX;   // panel height limit
 constexpr int PANEL_WMAX = 64;
 constexpr int PANEL_W = 32;             // panel width of the cooperative kernel
 // moved rows one workgroup can report (pivot rows + rows they displace)
@@ -87,7 +87,7 @@ FLX_DEV Cand wave_best(uint64_t key, int pos, int rid)
     return c;
 }
 
-template <int NT, int RPT>
+template <int NT, int RPT, int GMAX>
 __global__ __launch_bounds__(NT) void panel_kernel(
     double *__restrict__ P, long ldw, int mp, int w, uint64_t *__restrict__ ex,
     uint32_t gen, int *__restrict__ piv, int piv_base, int *__restrict__ list,
@@ -96,7 +96,7 @@ __global__ __launch_bounds__(NT) void panel_kernel(
     constexpr int W = PANEL_W;
     constexpr int NWAVE = NT / 64;
     constexpr int ROWS = NT * RPT;                 // panel rows per workgroup
-    constexpr int GMAX = PANEL_ROWS_MAX / ROWS;
+    static_assert(GMAX <= PANEL_ROWS_MAX / ROWS, "groups");
     // words one workgroup reads per column at most: G headers (4 words) and
     // the live part of G candidate rows (2 words per element)
     constexpr int WORDS_MAX = 4 * GMAX + 2 * W * GMAX;
@@ -388,13 +388,21 @@ hipError_t launch_panel(hipStream_t s, LuWork &wk, double *P, long ldw,
     const double pf = (double)w * w * mp - (double)w * w * w / 3.0;
     ProfScope ps(wk.prof, s, PROF_PANEL, pf, 16.0 * mp * w);
     // 256 threads (one wave per SIMD) x 2 rows per thread: 512 rows per
-    // workgroup, at most 32 cooperating workgroups
+    // workgroup, at most PANEL_GROUPS_MAX cooperating workgroups
     (void)ib;
     constexpr int NT = 256, RPT = 2;
     static_assert(NT * RPT == PANEL_ROWS_PER_WG, "panel geometry");
+    // the words a workgroup gathers per column scale with the group count:
+    // panels of at most half the largest height use the smaller variant
     const int G = panel_groups(mp);
-    hipLaunchKernelGGL((panel_kernel<NT, RPT>), dim3(G), dim3(NT), 0, s, P, ldw, mp,
-                       w, wk.exch, wk.gen, piv, piv_base, wk.list, info, piv_base, wk.flags);
+    if (G <= PANEL_GROUPS_MAX / 2)
+        hipLaunchKernelGGL((panel_kernel<NT, RPT, PANEL_GROUPS_MAX / 2>), dim3(G), dim3(NT), 0, s,
+                           P, ldw, mp, w, wk.exch, wk.gen, piv, piv_base, wk.list, info, piv_base,
+                           wk.flags);
+    else
+        hipLaunchKernelGGL((panel_kernel<NT, RPT, PANEL_GROUPS_MAX>), dim3(G), dim3(NT), 0, s, P,
+                           ldw, mp, w, wk.exch, wk.gen, piv, piv_base, wk.list, info, piv_base,
+                           wk.flags);
     return hipGetLastError();
 }
 

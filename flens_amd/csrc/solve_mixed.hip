@@ -572,7 +572,7 @@ extern "C" int flx_dgesv_mixed(flx_handle h, int n, int nrhs, double *A, int lda
     if (st) memset(st, 0, sizeof *st);
     if (n == 0 || nrhs == 0) return FLX_SUCCESS;
     if (n > 65536) return FLX_UNSUPPORTED;
-    if (o.fallback && n > 16384) return FLX_UNSUPPORTED;
+    if (o.fallback && n > LU_N_MAX) return FLX_UNSUPPORTED;
     return run_mixed(h, n, nrhs, A, lda, B, ldb, X, ldx, o, st, nullptr, nullptr, nullptr);
 }
 
@@ -584,7 +584,7 @@ extern "C" int flx_dsgesv(flx_handle h, int n, int nrhs, double *A, int lda, int
     if (iter) *iter = 0;
     if (info) *info = 0;
     if (n == 0) return FLX_SUCCESS;
-    if (n > 16384) return FLX_UNSUPPORTED;
+    if (n > LU_N_MAX) return FLX_UNSUPPORTED;
     Opts o;
     o.dsgesv = true;
     o.itermax = 30;

@@ -131,7 +131,9 @@ int flx_dgemm(flx_handle h, int transa, int transb, int m, int n, int k,
 
 /* LU with partial pivoting in place; ipiv (min(m,n) ints) and info (one
  * int) are device pointers.  Replaces flens::lapack::trf
- * (flens/lapack/ge/trf.tcc:183-261). */
+ * (flens/lapack/ge/trf.tcc:183-261).  m <= 32768 (FLX_UNSUPPORTED above:
+ * the panel's cooperating workgroups hold 512 rows each, at most 64); the
+ * same limit holds for the solvers and estimators built on it. */
 int flx_dgetrf(flx_handle h, int m, int n, double *A, int lda, int *ipiv,
                int *info);
 

@@ -280,3 +280,48 @@ def test_flens_program_runs_on_libflx(gpu):
     r = subprocess.run([exe], capture_output=True, text=True, timeout=120)
     print(r.stdout)
     assert r.returncode == 0 and "DROPIN PASSED" in r.stdout, r.stdout + r.stderr
+
+
+def test_lu_n20000_device_and_host_seam(gpu):
+    """An order above 16384 (40 cooperating panel workgroups): the device
+    LU's dget01 ratio ||P A - L U||_1 / (n ||A||_1 eps) is computed on the
+    device in FP64 (the n^3 product is out of the oracle's reach), and the
+    FLENS dgetrf_ seam on host memory computes the same factors and pivots
+    instead of refusing the size."""
+    import ctypes as C
+    import torch
+    n = 20000
+    L = gpu.lib()
+    h = gpu.default_handle()
+    A0 = gpu.DeviceMatrix.empty(n, n)
+    assert L.flx_fill_uniform(h.ptr, gpu._vp(A0.ptr), n, n, n, 2, 0) == 0
+    A = gpu.DeviceMatrix.empty(n, n)
+    A.storage.copy_(A0.storage)
+    piv, info = gpu.lapack.trf(A)
+    torch.cuda.synchronize()
+    assert info == 0
+    ipiv = piv.cpu().numpy() if hasattr(piv, "cpu") else np.asarray(piv)
+    assert ipiv.min() >= 1 and ipiv.max() <= n and (ipiv[:n] >= np.arange(1, n + 1)).all()
+    perm = np.arange(n)
+    for i in range(n):
+        k = ipiv[i] - 1
+        perm[i], perm[k] = perm[k], perm[i]
+    lu = A.storage.T
+    Lm = torch.tril(lu, -1)
+    Lm.diagonal().fill_(1.0)
+    R = A0.storage.T[torch.from_numpy(perm).cuda()]
+    R -= Lm @ torch.triu(lu)
+    del Lm
+    ratio = (R.abs().sum(0).max() / (n * A0.storage.T.abs().sum(0).max() * 2.0 ** -53)).item()
+    del R
+    print(f"n = {n}: dget01 = {ratio:.3f}")
+    assert ratio < 30
+    # the host seam: same factors and pivots, bit for bit
+    a_host = np.asfortranarray(A0.storage.cpu().numpy().T)
+    ip = np.zeros(n, dtype=np.int32)
+    inf = C.c_int(-1)
+    L.dgetrf_(C.byref(C.c_int(n)), C.byref(C.c_int(n)), a_host.ctypes.data_as(gpu._dp),
+              C.byref(C.c_int(n)), ip.ctypes.data_as(gpu._ip), C.byref(inf))
+    assert inf.value == 0
+    assert np.array_equal(ip, ipiv[:n])
+    assert np.array_equal(a_host, A.storage.cpu().numpy().T)
