@@ -65,6 +65,7 @@ hipError_t init_work(flx_context *h)
     e = hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, h->device);
     if (e != hipSuccess) return e;
     h->wk.side_groups = cus > 8 ? cus - cus / 4 : cus;
+    h->wk.cus = cus;
     return hipMemsetAsync(h->wk.flags, 0, sizeof(int) * 4, h->stream);
 }
 
@@ -324,14 +325,13 @@ int dgetrs_dev(flx_context *h, int trans, int n, int nrhs, const double *A, int 
         }
         FLX_TRY(gather_rows_cm(h->stream, B, ldb, T1, n, perm, n, nrhs));
         FLX_TRY(init_work(h));
-        FLX_TRY(ensure_zeroed(h, h->trsw, sizeof(uint64_t) * trs_persist_words(n)));
-        FLX_TRY(trs_persist_f64(h->stream, n, nrhs, A, lda, T1, n, T2, B, ldb,
-                                (uint64_t *)h->trsw.p, &h->trs_epoch, h->wk.flags));
-        (void)ti;
+        FLX_TRY(ensure_zeroed(h, h->trsw, sizeof(uint64_t) * trs_chain_words(n)));
+        FLX_TRY(trs_chain_f64(h->stream, n, nrhs, A, lda, T1, n, T2, B, ldb, ti,
+                              (uint64_t *)h->trsw.p, &h->trs_epoch, h->wk.flags, h->wk.cus));
     } else {
         // Y = L^{-T} U^{-T} B into T1, one launch per triangle
         FLX_TRY(init_work(h));
-        FLX_TRY(ensure_zeroed(h, h->trsw, sizeof(uint64_t) * trs_persist_words(n)));
+        FLX_TRY(ensure_zeroed(h, h->trsw, sizeof(uint64_t) * trs_chain_words(n)));
         FLX_TRY(trs_persist_f64(h->stream, n, nrhs, A, lda, B, ldb, T2, T1, n,
                                 (uint64_t *)h->trsw.p, &h->trs_epoch, h->wk.flags, 1));
         (void)ti;

@@ -79,6 +79,7 @@ struct LuWork {
     // panels and the small launches of the next block always find free
     // compute units (a trailing update left uncapped fills the device)
     int side_groups = 0;
+    int cus = 0;                // compute units of the device
     // fused panel step (lookahead): staged rows (LU_STAGE_ROWS x LU_STAGE_LD
     // doubles), their destinations (+ the count at [LU_STAGE_TOP]) and perm
     // entries; null: separate interchange / solve / update launches
@@ -162,6 +163,13 @@ hipError_t trsm_cm_f64(hipStream_t s, int uplo_upper, int trans, int unit,
 // the row order applied afterwards); T is n x 16 scratch; xt holds
 // trs_persist_words(n) tagged words (zeroed once), epoch is their host tag.
 size_t trs_persist_words(int n);
+// NoTrans solve U^{-1} L^{-1} Bp by the chain-and-helpers kernel: inv holds
+// trs_inv_doubles(n) doubles, w trs_chain_words(n) tagged words (zeroed
+// once), at most cus workgroups.
+size_t trs_chain_words(int n);
+hipError_t trs_chain_f64(hipStream_t s, int n, int nrhs, const double *LU, long lda,
+                         const double *Bp, long ldbp, double *T, double *X, long ldx,
+                         double *inv, uint64_t *w, unsigned *epoch, int *err, int cus);
 hipError_t trs_persist_f64(hipStream_t s, int n, int nrhs, const double *LU, long lda,
                            const double *Bp, long ldbp, double *T, double *X, long ldx,
                            uint64_t *xt, unsigned *epoch, int *err, int trans = 0);

@@ -10,6 +10,7 @@
 // are blocked by 64 rows: one launch per diagonal block solves the block in
 // LDS and folds it into the remaining rows (GEMM-shaped, coalesced reads of
 // the factor columns).
+#include <algorithm>
 #include <climits>
 #include "flx_common.h"
 #include "flx_kernels.h"
@@ -523,6 +524,356 @@ hipError_t trs_persist_f64(hipStream_t s, int n, int nrhs, const double *LU, lon
             else FLX_TP(0, 1, 1, 16, TP_GRP);
         }
 #undef FLX_TP
+    }
+    return hipGetLastError();
+}
+
+// ---------------------------------------------------------------------------
+// Chain-and-helpers triangular solve (NoTrans: L unit forward, then U
+// backward), column-major FP64 factor, up to 16 right-hand sides per launch,
+// 64-row blocks, one launch per triangle after tri_inv_kernel has inverted
+// every diagonal block.
+//   * Workgroup 0 (the chain) walks the blocks in solve order g = 0, 1, ...:
+//       v  = B_g - P_g - T(g, g-1) x_{g-1}      (x_{g-1} kept in LDS)
+//       x_g = inv(T_gg) v                       (a 64 x 64 product, no
+//                                                sequential substitution)
+//     and publishes x_g as tagged words.  The factor block T(g+1, g), the
+//     next inverse and the next right-hand side rows are requested one step
+//     ahead, so the chain's step is two block products and two barriers.
+//   * Workgroups 1 .. H (helpers) own blocks g = 2, 3, ... round robin and
+//     fold every finished x_j, j <= g - 2, into P_g = sum_j T(g, j) x_j as
+//     soon as it is published, then publish P_g -- the bulk of the work,
+//     off the chain's critical path.
+//   * Every published value is a pair of 64-bit words (epoch << 32 | half
+//     of the double): a reader checks every word's tag, so producers need
+//     no fence, and all waits are bounded (FLX flag bit 2 on a give-up).
+// The grid is bounded by the CU count, so every workgroup is resident at
+// once for any n.
+// ---------------------------------------------------------------------------
+constexpr int TC_BS = 64;
+constexpr int TC_GRP = 2;        // helper: finished blocks per wait
+
+FLX_DEV uint64_t tc_word_hi(unsigned epoch, double x)
+{
+    return ((uint64_t)epoch << 32) | ((uint64_t)__double_as_longlong(x) >> 32);
+}
+FLX_DEV uint64_t tc_word_lo(unsigned epoch, double x)
+{
+    return ((uint64_t)epoch << 32) | ((uint64_t)__double_as_longlong(x) & 0xffffffffull);
+}
+FLX_DEV double tc_join(uint64_t hi, uint64_t lo)
+{
+    return __longlong_as_double((long long)(((hi & 0xffffffffull) << 32) | (lo & 0xffffffffull)));
+}
+FLX_DEV uint64_t tc_ld(const uint64_t *p)
+{
+    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+FLX_DEV void tc_st(uint64_t *p, uint64_t v)
+{
+    __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// the pair of words of a tagged value (row r of block bi, right-hand side k)
+template <int R>
+FLX_DEV const uint64_t *tc_addr(const uint64_t *w, int bi, int r, int k)
+{
+    return w + ((long)(bi * TC_BS + r) * R + k) * 2;
+}
+// a value whose words were loaded ahead (hi, lo): re-polled only while a
+// word still carries an older epoch; false on a give-up
+FLX_DEV bool tc_settle(const uint64_t *p, uint64_t &hi, uint64_t &lo, unsigned epoch, int &spins)
+{
+    while ((unsigned)(hi >> 32) != epoch || (unsigned)(lo >> 32) != epoch) {
+        if (++spins > TP_SPIN_LIMIT) return false;
+        __builtin_amdgcn_s_sleep(1);
+        hi = tc_ld(p);
+        lo = tc_ld(p + 1);
+    }
+    return true;
+}
+
+// the chain's operands of step g into registers: the inverse of its
+// diagonal block and the factor block left of it (columns of block g - 1),
+// IPT entries of each per thread, coalesced; its right-hand side rows
+template <int FWD, int R>
+FLX_DEV void tc_fetch(int g, int nbk, int n, const double *__restrict__ LU, long lda,
+                      const double *__restrict__ inv, const double *__restrict__ Bin, long ldb,
+                      int nrhs, double (&inext)[TC_BS * TC_BS / 256],
+                      double (&tnext)[TC_BS * TC_BS / 256], double (&bnext)[4])
+{
+    constexpr int IPT = TC_BS * TC_BS / 256;
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int lr = lane & 15, lk = lane >> 4;
+    const int bi = FWD ? g : nbk - 1 - g, r0 = bi * TC_BS, rows = min(TC_BS, n - r0);
+    const double *ib = inv + (size_t)bi * TC_BS * TC_BS;
+#pragma unroll
+    for (int u = 0; u < IPT; ++u) inext[u] = ib[tid + 256 * u];
+    const int c0 = g > 0 ? (FWD ? g - 1 : nbk - g) * TC_BS : 0;
+#pragma unroll
+    for (int u = 0; u < IPT; ++u) {
+        const int q = tid + 256 * u, c = q / TC_BS, r = q % TC_BS;
+        tnext[u] = (g > 0 && r < rows && c0 + c < n) ? LU[(long)(c0 + c) * lda + r0 + r] : 0.0;
+    }
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+        const int rr = 16 * wave + lk + 4 * q;
+        bnext[q] = (rr < rows && lr < R && lr < nrhs) ? Bin[(long)lr * ldb + r0 + rr] : 0.0;
+    }
+}
+
+// a barrier for LDS traffic only: outstanding global loads (prefetches) and
+// stores (published words) stay in flight across it
+FLX_DEV void lds_barrier()
+{
+    asm volatile("s_waitcnt lgkmcnt(0)\n\ts_barrier" ::: "memory");
+}
+
+template <int FWD, int R>
+__global__ __launch_bounds__(256) void trs_chain_kernel(const double *__restrict__ LU, long lda,
+                                                        int n, int nrhs,
+                                                        const double *__restrict__ inv,
+                                                        const double *__restrict__ Bin, long ldb,
+                                                        double *__restrict__ Xout, long ldx,
+                                                        uint64_t *__restrict__ xw,
+                                                        uint64_t *__restrict__ pw,
+                                                        unsigned epoch, int *__restrict__ err)
+{
+    constexpr int IPT = TC_BS * TC_BS / 256; // inverse entries per thread
+    constexpr int XPT = (TC_BS * R + 255) / 256;   // block values per thread (helpers)
+    __shared__ double s_inv[TC_BS][TC_BS + 1];
+    __shared__ double s_t[TC_BS][TC_BS + 1];      // T(g, g-1), row-major
+    __shared__ double s_v[TC_BS][17];              // chain: 16 right-hand side slots
+    __shared__ double s_xc[TC_BS][17];
+    __shared__ double s_x[2][TC_GRP][TC_BS][R + 1];
+    __shared__ double s_red[4][TC_BS][R + 1];
+    __shared__ int s_bad;
+    const int nbk = (n + TC_BS - 1) / TC_BS;
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    auto blk = [&](int g) { return FWD ? g : nbk - 1 - g; };
+    if (tid == 0) s_bad = 0;
+    __syncthreads();
+    if (blockIdx.x == 0) {
+        // ------------------------------------------------------------ chain
+        // Wave w owns rows 16w .. 16w + 15 of the block and every right-hand
+        // side: both block products are 16 x 16 x 64 MFMA tiles (sixteen
+        // v_mfma_f64_16x16x4 steps per wave).  Lane (lr, lk) holds rows
+        // 16w + lk + 4q (q = 0..3) of right-hand side lr.  Step g's operands
+        // -- the inverse of its diagonal block, the factor block left of it
+        // (columns of block g-1), its right-hand side rows -- are requested
+        // during step g-1; the chain's barriers wait for LDS traffic only,
+        // so those loads and the published stores stay in flight.
+        const int lr = lane & 15, lk = lane >> 4;
+        int spins = 0;
+        bool bad = false;
+        double inext[IPT], tnext[IPT], bnext[4];
+        tc_fetch<FWD, R>(0, nbk, n, LU, lda, inv, Bin, ldb, nrhs, inext, tnext, bnext);
+        for (int g = 0; g < nbk; ++g) {
+            const int bi = blk(g), r0 = bi * TC_BS, rows = min(TC_BS, n - r0);
+#pragma unroll
+            for (int u = 0; u < IPT; ++u) {
+                const int q = tid + 256 * u;
+                s_inv[q / TC_BS][q % TC_BS] = inext[u];
+                s_t[q % TC_BS][q / TC_BS] = tnext[u];
+            }
+            double v[4];
+#pragma unroll
+            for (int q = 0; q < 4; ++q) v[q] = bnext[q];
+            lds_barrier();
+            // P_g, published by a helper (g >= 2): every word requested first
+            if (g >= 2) {
+                uint64_t ph[4], pl[4];
+#pragma unroll
+                for (int q = 0; q < 4; ++q) {
+                    ph[q] = pl[q] = (uint64_t)epoch << 32;
+                    if (lr < R) {
+                        const uint64_t *p = tc_addr<R>(pw, bi, 16 * wave + lk + 4 * q, lr);
+                        ph[q] = tc_ld(p);
+                        pl[q] = tc_ld(p + 1);
+                    }
+                }
+#pragma unroll
+                for (int q = 0; q < 4; ++q) {
+                    if (lr < R) {
+                        if (!bad)
+                            bad = !tc_settle(tc_addr<R>(pw, bi, 16 * wave + lk + 4 * q, lr), ph[q],
+                                             pl[q], epoch, spins);
+                        v[q] -= tc_join(ph[q], pl[q]);
+                    }
+                }
+            }
+            // v -= T(g, g-1) x_{g-1}
+            if (g > 0) {
+                // four accumulators: four independent MFMA chains of four steps
+                double4_t acc[4];
+#pragma unroll
+                for (int a = 0; a < 4; ++a) acc[a] = double4_t{0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+                for (int q = 0; q < TC_BS / 4; ++q)
+                    acc[q & 3] = __builtin_amdgcn_mfma_f64_16x16x4f64(
+                        s_t[16 * wave + lr][4 * q + lk], s_xc[4 * q + lk][lr], acc[q & 3], 0, 0, 0);
+#pragma unroll
+                for (int q = 0; q < 4; ++q) v[q] -= (acc[0][q] + acc[1][q]) + (acc[2][q] + acc[3][q]);
+            }
+            // the next step's operands are requested now and arrive during
+            // this step's second product and stores
+            if (g + 1 < nbk)
+                tc_fetch<FWD, R>(g + 1, nbk, n, LU, lda, inv, Bin, ldb, nrhs, inext, tnext, bnext);
+#pragma unroll
+            for (int q = 0; q < 4; ++q) s_v[16 * wave + lk + 4 * q][lr] = v[q];
+            lds_barrier();
+            // x_g = inv(T_gg) v
+            double4_t xa[4];
+#pragma unroll
+            for (int a = 0; a < 4; ++a) xa[a] = double4_t{0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+            for (int q = 0; q < TC_BS / 4; ++q)
+                xa[q & 3] = __builtin_amdgcn_mfma_f64_16x16x4f64(
+                    s_inv[16 * wave + lr][4 * q + lk], s_v[4 * q + lk][lr], xa[q & 3], 0, 0, 0);
+            double x[4];
+#pragma unroll
+            for (int q = 0; q < 4; ++q) x[q] = (xa[0][q] + xa[1][q]) + (xa[2][q] + xa[3][q]);
+            lds_barrier();            // every read of s_xc and s_v is done
+#pragma unroll
+            for (int q = 0; q < 4; ++q) {
+                const int rr = 16 * wave + lk + 4 * q;
+                const double xv = rr < rows ? x[q] : 0.0;
+                s_xc[rr][lr] = xv;
+                if (lr < R) {
+                    if (rr < rows && lr < nrhs) Xout[(long)lr * ldx + r0 + rr] = xv;
+                    uint64_t *p = xw + ((long)(bi * TC_BS + rr) * R + lr) * 2;
+                    tc_st(p, tc_word_hi(epoch, xv));
+                    tc_st(p + 1, tc_word_lo(epoch, xv));
+                }
+            }
+        }
+        if (bad) atomicOr(err, 2);
+        return;
+    }
+    // ------------------------------------------------------------- helpers
+    const int H = gridDim.x - 1;
+    int spins = 0;
+    bool gave_up = false;
+    for (int g = 2 + (blockIdx.x - 1); g < nbk; g += H) {
+        const int bi = blk(g), r0 = bi * TC_BS, rows = min(TC_BS, n - r0);
+        const bool row_ok = lane < rows;
+        const int row = r0 + (row_ok ? lane : 0);
+        const int ndep = g - 1;                  // x_0 .. x_{g-2}
+        double acc[R];
+#pragma unroll
+        for (int k = 0; k < R; ++k) acc[k] = 0.0;
+        int par = 0;
+        for (int d0 = 0; d0 < ndep; d0 += TC_GRP, par ^= 1) {
+            const int cnt = min(TC_GRP, ndep - d0);
+            double lv[TC_GRP][16];
+#pragma unroll
+            for (int b = 0; b < TC_GRP; ++b) {
+                const int c0 = blk(d0 + b) * TC_BS + wave * 16;
+#pragma unroll
+                for (int c = 0; c < 16; ++c)
+                    lv[b][c] = (b < cnt && row_ok && c0 + c < n) ? LU[(long)(c0 + c) * lda + row] : 0.0;
+            }
+            // the group's x values into LDS: every word requested, then
+            // each checked against the epoch (a stale one is re-polled)
+            uint64_t wh[TC_GRP][XPT], wl[TC_GRP][XPT];
+#pragma unroll
+            for (int b = 0; b < TC_GRP; ++b)
+#pragma unroll
+                for (int u = 0; u < XPT; ++u) {
+                    const int idx = tid + 256 * u;
+                    wh[b][u] = wl[b][u] = (uint64_t)epoch << 32;
+                    if (b < cnt && idx < TC_BS * R) {
+                        const uint64_t *p = tc_addr<R>(xw, blk(d0 + b), idx / R, idx % R);
+                        wh[b][u] = tc_ld(p);
+                        wl[b][u] = tc_ld(p + 1);
+                    }
+                }
+#pragma unroll
+            for (int b = 0; b < TC_GRP; ++b)
+#pragma unroll
+                for (int u = 0; u < XPT; ++u) {
+                    const int idx = tid + 256 * u;
+                    if (b < cnt && idx < TC_BS * R) {
+                        if (!gave_up)
+                            gave_up = !tc_settle(tc_addr<R>(xw, blk(d0 + b), idx / R, idx % R),
+                                                 wh[b][u], wl[b][u], epoch, spins);
+                        s_x[par][b][idx / R][idx % R] = tc_join(wh[b][u], wl[b][u]);
+                    }
+                }
+            __syncthreads();
+#pragma unroll
+            for (int b = 0; b < TC_GRP; ++b)
+                if (b < cnt) {
+#pragma unroll
+                    for (int c = 0; c < 16; ++c) {
+                        const double l = lv[b][c];
+#pragma unroll
+                        for (int k = 0; k < R; ++k) acc[k] = fma(l, s_x[par][b][wave * 16 + c][k], acc[k]);
+                    }
+                }
+        }
+#pragma unroll
+        for (int k = 0; k < R; ++k) s_red[wave][lane][k] = acc[k];
+        __syncthreads();
+        for (int idx = tid; idx < TC_BS * R; idx += 256) {
+            const int rr = idx / R, k = idx % R;
+            const double pv = s_red[0][rr][k] + s_red[1][rr][k] + s_red[2][rr][k] + s_red[3][rr][k];
+            uint64_t *p = pw + ((long)(bi * TC_BS + rr) * R + k) * 2;
+            tc_st(p, tc_word_hi(epoch, pv));
+            tc_st(p + 1, tc_word_lo(epoch, pv));
+        }
+        __syncthreads();           // s_red and s_x are reused by the next block
+    }
+    if (gave_up) atomicOr(err, 2);
+}
+
+// words of the chain solve's two tagged arrays (x and partial sums)
+size_t trs_chain_words(int n) { return 2 * trs_persist_words(n); }
+
+// NoTrans: X = U^{-1} L^{-1} Bp (Bp already row-permuted), T n x nrhs scratch,
+// inv trs_inv_doubles(n) scratch, w trs_chain_words(n) tagged words.
+hipError_t trs_chain_f64(hipStream_t s, int n, int nrhs, const double *LU, long lda,
+                         const double *Bp, long ldbp, double *T, double *X, long ldx,
+                         double *inv, uint64_t *w, unsigned *epoch, int *err, int cus)
+{
+    const int nbk = (n + TC_BS - 1) / TC_BS;
+    if (nbk <= 0 || nrhs <= 0) return hipSuccess;
+    const size_t half = trs_persist_words(n);
+    uint64_t *xw = w, *pw = w + half;
+    auto next = [&]() -> unsigned {
+        if (++*epoch == 0u) {
+            hipLaunchKernelGGL(zero_words_kernel, dim3((unsigned)((2 * half + 255) / 256)), dim3(256),
+                               0, s, w, 2 * half);
+            *epoch = 1u;
+        }
+        return *epoch;
+    };
+    // chain + helpers, all resident: at most one workgroup per CU
+    const int grid = 1 + std::max(1, std::min(nbk - 2, cus - 1));
+    hipLaunchKernelGGL((tri_inv_kernel<0, 1, 1>), dim3(nbk), dim3(64), 0, s, LU, lda, n, inv);
+    for (int k0 = 0; k0 < nrhs; k0 += TP_RHS) {
+        const int kc = min(TP_RHS, nrhs - k0);
+        ProfScope ps(tl_prof, s, PROF_TRS, 2.0 * (double)n * n * kc,
+                     8.0 * ((double)n * n + 4.0 * n * kc));
+        const double *bsrc = Bp + (long)k0 * ldbp;
+        double *tdst = T + (long)k0 * n;
+#define FLX_TC(FW_, R_, IN_, LDI_, OUT_, LDO_)                                                     \
+        hipLaunchKernelGGL((trs_chain_kernel<FW_, R_>), dim3(grid), dim3(256), 0, s, LU, lda, n, \
+                           kc, inv, IN_, LDI_, OUT_, LDO_, xw, pw, next(), err)
+        if (kc == 1) FLX_TC(1, 1, bsrc, ldbp, tdst, (long)n);
+        else if (kc <= 4) FLX_TC(1, 4, bsrc, ldbp, tdst, (long)n);
+        else FLX_TC(1, 16, bsrc, ldbp, tdst, (long)n);
+    }
+    hipLaunchKernelGGL((tri_inv_kernel<0, 0, 0>), dim3(nbk), dim3(64), 0, s, LU, lda, n, inv);
+    for (int k0 = 0; k0 < nrhs; k0 += TP_RHS) {
+        const int kc = min(TP_RHS, nrhs - k0);
+        ProfScope ps(tl_prof, s, PROF_TRS, 0.0, 0.0);
+        const double *tsrc = T + (long)k0 * n;
+        double *xdst = X + (long)k0 * ldx;
+        if (kc == 1) FLX_TC(0, 1, tsrc, (long)n, xdst, ldx);
+        else if (kc <= 4) FLX_TC(0, 4, tsrc, (long)n, xdst, ldx);
+        else FLX_TC(0, 16, tsrc, (long)n, xdst, ldx);
+#undef FLX_TC
     }
     return hipGetLastError();
 }
