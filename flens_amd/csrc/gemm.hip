@@ -217,9 +217,29 @@ FLX_DEV void gemm_tile(T (*As)[GBK][GBM + APAD], T (*Bs)[GBN][GBK + BPAD], int m
         }
 }
 
-// Tiles are numbered m-fastest; workgroup b takes tiles b, b + grid, ...
-// (one tile each unless the launch caps the grid so that concurrent work
-// on another stream keeps some compute units).
+// Tile order.  The hardware hands workgroup b to XCD b mod 8, and each XCD
+// has its own L2, so the dispatch sequence t (workgroup b takes t = b,
+// b + grid, ...) is mapped to tiles such that the tiles of one XCD form a
+// contiguous stretch of a grouped order: bands of GBAND tile-rows, each band
+// walked column by column.  The tiles an XCD runs at once then share a few
+// A row-panels and B column-panels in its L2 instead of touching eight
+// times as many.  Both maps are bijections on [0, tiles).
+constexpr int GXCD = 8, GBAND = 8;
+FLX_DEV void gemm_tile_of(int t, int mt, int nt, int tiles, int &mi, int &ni)
+{
+    const int per = tiles / GXCD, rem = tiles % GXCD;
+    const int x = t % GXCD, s = t / GXCD;
+    const int L = x * per + min(x, rem) + s;          // contiguous stretch per XCD
+    const int band = L / (GBAND * nt);
+    const int h = min(GBAND, mt - band * GBAND);      // rows in this band
+    const int loc = L - band * GBAND * nt;
+    mi = band * GBAND + loc % h;
+    ni = loc / h;
+}
+
+// Workgroup b takes dispatch slots b, b + grid, ... (one tile each unless
+// the launch caps the grid so that concurrent work on another stream keeps
+// some compute units).
 template <typename T, int TA, int TB>
 __global__ __launch_bounds__(GNT) void gemm_mfma_kernel(
     int M, int N, int K, T alpha, const T *__restrict__ A, long lda,
@@ -228,9 +248,12 @@ __global__ __launch_bounds__(GNT) void gemm_mfma_kernel(
 {
     __shared__ T As[2][GBK][GBM + APAD];
     __shared__ T Bs[2][GBN][GBK + BPAD];
+    const int nt = tiles / mt;
     for (int t = blockIdx.x; t < tiles; t += gridDim.x) {
         if (t != (int)blockIdx.x) __syncthreads();
-        gemm_tile<T, TA, TB>(As, Bs, (t % mt) * GBM, (t / mt) * GBN, M, N, K, alpha, A, lda, B,
+        int mi, ni;
+        gemm_tile_of(t, mt, nt, tiles, mi, ni);
+        gemm_tile<T, TA, TB>(As, Bs, mi * GBM, ni * GBN, M, N, K, alpha, A, lda, B,
                              ldb, beta, C, ldc, vecA, vecB, blas_beta0);
     }
 }
