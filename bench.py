@@ -340,7 +340,8 @@ def batched_section(count: int = 20000, seed: int = 4):
         ts.append(time.perf_counter() - t)
     dt = min(ts)
     out["gemm"] = {"count": count, "ms": 1e3 * dt, "tflops": float(gfl_i.sum()) / dt / 1e12,
-                   "roofline_ms": 1e3 * groof, "roofline_frac": groof / dt, "config": plan.config()}
+                   "roofline_ms": 1e3 * groof, "roofline_frac": groof / dt, "config": plan.config(),
+                   "bin_tiles": plan.bin_tiles()}
     plan.close()
     progress(f"batched gemm: {1e3 * dt:.1f} ms")
     return out

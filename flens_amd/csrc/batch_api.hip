@@ -32,9 +32,19 @@ inline int solve_bin(int n)
 }
 struct GemmProbH {    // layout of flx::GemmProb (batched.hip)
     long a, b, c;
-    int m, n, k, tile0;
+    int m, n, k, tile0, cfg, id;
     double alpha, beta;
 };
+
+// size bins of the GEMM autotuner: max(m, n) in [16 * 2^b, 32 * 2^b),
+// b = 0..5 (below 16 in bin 0, 512 and above in bin 5)
+inline int gemm_bin(int m, int n)
+{
+    const int d = m > n ? m : n;
+    int b = 0;
+    while (b < SOLVE_BINS - 1 && d >= (32 << b)) ++b;
+    return b;
+}
 
 } // namespace
 
@@ -49,6 +59,7 @@ struct flx_batch {
     int Ggrid = 0;
     double *stats = nullptr;    // row statistics per system (pre-pass)
     float *selT = nullptr;      // the selector transposed for the kernel
+    unsigned long long *gticks = nullptr;   // GEMM tuning: per-problem time
     int *cls = nullptr;         // the selector's class per system
     int count = 0;
     void *probs = nullptr;      // device problem table
@@ -62,8 +73,9 @@ struct flx_batch {
     const double *GA = nullptr, *GB = nullptr;
     double *GC = nullptr;
     long c_total = 0;
-    int ntiles[3] = {0, 0, 0};
-    void *gprobs[3] = {nullptr, nullptr, nullptr};   // tables per tile shape
+    int ntiles = 0;
+    std::vector<GemmProbH> gtab;   // problem table (host copy), tile0 per the cfgs
+    void *gprobs = nullptr;
     // launch shape
     int tile_cfg = 1, grid = 0;
     double tuned_ms = 0.0;
@@ -90,8 +102,9 @@ static hipError_t launch(flx_batch *p, hipStream_t s, int tile_cfg, int grid, do
         return batched_gesv(s, p->policy, p->probs, p->count, p->counter, A, B, p->W32, p->R,
                             grid <= p->Ggrid ? p->G : nullptr, p->ipiv, p->info, p->iters,
                             p->itmax, grid, p->out);
-    return batched_gemm(s, tile_cfg, p->gprobs[tile_cfg], p->count, p->ntiles[tile_cfg],
-                        p->counter, p->GA, p->GB, C, grid);
+    (void)tile_cfg;
+    return batched_gemm(s, p->gprobs, p->count, p->ntiles, p->counter, p->GA, p->GB, C, grid,
+                        p->gticks);
 }
 
 static int capture(flx_batch *p)
@@ -160,6 +173,23 @@ static int timed_run(flx_batch *p, TuneScratch &ts, int cfg, int grid, double *t
     FLX_TRY(hipEventRecord(e1, s));
     FLX_TRY(hipEventSynchronize(e1));
     FLX_TRY(hipEventElapsedTime(ms, e0, e1));
+    return FLX_SUCCESS;
+}
+
+// tile0 of every GEMM problem from its cfg; uploads the table
+static int upload_gemm_table(flx_batch *p)
+{
+    int tiles = 0;
+    for (auto &g : p->gtab) {
+        int tm = 0, tn = 0;
+        batched_gemm_tile(g.cfg, &tm, &tn);
+        g.tile0 = tiles;
+        tiles += ((g.m + tm - 1) / tm) * ((g.n + tn - 1) / tn);
+    }
+    p->ntiles = tiles;
+    if (!p->gtab.empty())
+        FLX_TRY(hipMemcpy(p->gprobs, p->gtab.data(), sizeof(GemmProbH) * p->gtab.size(),
+                          hipMemcpyHostToDevice));
     return FLX_SUCCESS;
 }
 
@@ -234,16 +264,45 @@ static int autotune(flx_batch *p, bool tune)
             if (ms_best < best) { best = ms_best; bgrid = grid; }
         }
     } else {
-        for (int cfg : {0, 1, 2})
-            for (int grid : {cus, 2 * cus, 4 * cus}) {
-                float ms_best = 1e30f;
-                for (int rep = 0; rep < 2; ++rep) {
-                    float ms = 0.f;
-                    FLX_RET(timed_run(p, ts, cfg, grid, tA, tB, tC, &ms));
-                    ms_best = std::min(ms_best, ms);
-                }
-                if (ms_best < best) { best = ms_best; bcfg = cfg; bgrid = grid; }
+        // every problem under each tile shape, its tiles' in-kernel time
+        // summed per size bin (gemm_bin), then the grid
+        unsigned long long *gt = nullptr;
+        FLX_TRY(ts.alloc(&gt, p->gtab.size()));
+        double sum[3][SOLVE_BINS] = {};
+        std::vector<unsigned long long> hv(p->gtab.size());
+        for (int cfg = 0; cfg < 3; ++cfg) {
+            for (auto &g : p->gtab) g.cfg = cfg;
+            FLX_RET(upload_gemm_table(p));
+            FLX_TRY(hipMemset(gt, 0, sizeof(unsigned long long) * std::max<size_t>(1, hv.size())));
+            p->gticks = gt;
+            float ms = 0.f;
+            const int st = timed_run(p, ts, 0, 2 * cus, tA, tB, tC, &ms);
+            p->gticks = nullptr;
+            FLX_RET(st);
+            if (!hv.empty())
+                FLX_TRY(hipMemcpy(hv.data(), gt, sizeof(unsigned long long) * hv.size(),
+                                  hipMemcpyDeviceToHost));
+            for (size_t i = 0; i < p->gtab.size(); ++i)
+                sum[cfg][gemm_bin(p->gtab[i].m, p->gtab[i].n)] += (double)hv[i];
+        }
+        for (int b = 0; b < SOLVE_BINS; ++b) {
+            int bc = 0;
+            for (int cfg = 1; cfg < 3; ++cfg)
+                if (sum[cfg][b] < sum[bc][b]) bc = cfg;
+            p->bin_tile[b] = bc;
+        }
+        for (auto &g : p->gtab) g.cfg = p->bin_tile[gemm_bin(g.m, g.n)];
+        FLX_RET(upload_gemm_table(p));
+        for (int grid : {cus, 2 * cus, 4 * cus}) {
+            float ms_best = 1e30f;
+            for (int rep = 0; rep < 2; ++rep) {
+                float ms = 0.f;
+                FLX_RET(timed_run(p, ts, 0, grid, tA, tB, tC, &ms));
+                ms_best = std::min(ms_best, ms);
             }
+            if (ms_best < best) { best = ms_best; bgrid = grid; }
+        }
+        bcfg = -1;   // per size bin
     }
     p->tile_cfg = bcfg;
     p->grid = bgrid;
@@ -371,26 +430,21 @@ extern "C" int flx_batch_gemm_create(flx_handle h, flx_batch_t *out, int count, 
         return (double)m[x] * n[x] * k[x] > (double)m[y] * n[y] * k[y];
     });
     if (hipMalloc(&p->counter, sizeof(int)) != hipSuccess) return fail(FLX_ALLOC_FAILED);
-    for (int cfg = 0; cfg < 3; ++cfg) {
-        int tm = 0, tn = 0;
-        batched_gemm_tile(cfg, &tm, &tn);
-        std::vector<GemmProbH> tab;
-        tab.reserve(count);
-        int tiles = 0;
-        for (int i : order) {
-            if (m[i] == 0 || n[i] == 0) continue;
-            GemmProbH g{ao[i], bo[i], co[i], m[i], n[i], k[i], tiles, alpha[i], beta[i]};
-            tab.push_back(g);
-            tiles += ((m[i] + tm - 1) / tm) * ((n[i] + tn - 1) / tn);
-        }
-        p->ntiles[cfg] = tiles;
-        if (hipMalloc(&p->gprobs[cfg], sizeof(GemmProbH) * std::max<size_t>(1, tab.size())) !=
-            hipSuccess)
-            return fail(FLX_ALLOC_FAILED);
-        if (!tab.empty() && hipMemcpy(p->gprobs[cfg], tab.data(), sizeof(GemmProbH) * tab.size(),
-                                      hipMemcpyHostToDevice) != hipSuccess)
-            return fail(FLX_HIP_ERROR);
-        if (cfg == 0) p->count = (int)tab.size();
+    p->gtab.reserve(count);
+    for (int i : order) {
+        if (m[i] == 0 || n[i] == 0) continue;
+        // id: the position in the table (the tuner's per-problem time slot)
+        const int pos = (int)p->gtab.size();
+        p->gtab.push_back(GemmProbH{ao[i], bo[i], co[i], m[i], n[i], k[i], 0, 1, pos, alpha[i],
+                                    beta[i]});
+    }
+    p->count = (int)p->gtab.size();
+    for (int b = 0; b < SOLVE_BINS; ++b) p->bin_tile[b] = 1;
+    if (hipMalloc(&p->gprobs, sizeof(GemmProbH) * std::max<size_t>(1, p->gtab.size())) != hipSuccess)
+        return fail(FLX_ALLOC_FAILED);
+    {
+        const int st0 = upload_gemm_table(p);
+        if (st0) return fail(st0);
     }
     int st = autotune(p, tune != 0);
     if (st) return fail(st);
@@ -423,7 +477,7 @@ extern "C" int flx_batch_config(flx_batch_t p, int *cfg, double *tuned_ms)
 
 extern "C" int flx_batch_bin_tiles(flx_batch_t p, int *tiles, int nbins)
 {
-    if (!p || !tiles || p->kind != 0 || nbins < SOLVE_BINS) return FLX_INVALID_ARG;
+    if (!p || !tiles || nbins < SOLVE_BINS) return FLX_INVALID_ARG;
     for (int b = 0; b < SOLVE_BINS; ++b) tiles[b] = p->bin_tile[b];
     return SOLVE_BINS;
 }
@@ -437,8 +491,7 @@ extern "C" int flx_batch_destroy(flx_batch_t p)
     if (p->ev0) (void)hipEventDestroy(p->ev0);
     if (p->ev1) (void)hipEventDestroy(p->ev1);
     if (p->probs) (void)hipFree(p->probs);
-    for (void *g : p->gprobs)
-        if (g) (void)hipFree(g);
+    if (p->gprobs) (void)hipFree(p->gprobs);
     if (p->counter) (void)hipFree(p->counter);
     if (p->W32) (void)hipFree(p->W32);
     if (p->R) (void)hipFree(p->R);

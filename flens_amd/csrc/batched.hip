@@ -1387,38 +1387,23 @@ struct GemmProb {
     long a, b, c;        // offsets (elements); column-major, ld = rows
     int m, n, k;
     int tile0;           // first tile index of this problem
+    int cfg;             // tile shape: 0 32x32, 1 64x64, 2 128x64
+    int id;              // the caller's index (per-problem timing)
     double alpha, beta;
 };
 
-// C tile (TM x TN) per 256-thread workgroup pass; 4 waves of (TM/2 x TN/2),
-// 16x16x4 FP64 MFMAs with operands read from L2 (no LDS).
+// C tile (TM x TN) of problem p, tile `local` of its own tiles (m-fastest):
+// 4 waves of (TM/2 x TN/2), 16x16x4 FP64 MFMAs with operands read from L2
+// (no LDS).
 template <int TM, int TN>
-__global__ __launch_bounds__(256) void batched_gemm_kernel(const GemmProb *__restrict__ probs,
-                                                           int nprob, int ntiles,
-                                                           int *__restrict__ counter,
-                                                           const double *__restrict__ A,
-                                                           const double *__restrict__ Bm,
-                                                           double *__restrict__ Cm)
+FLX_DEV void bg_tile(const GemmProb &p, int local, const double *__restrict__ A,
+                     const double *__restrict__ Bm, double *__restrict__ Cm)
 {
     constexpr int WM = TM / 2, WN = TN / 2, FI = WM / 16, FJ = WN / 16;
-    __shared__ int s_tile;
     const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
     const int lr = lane & 15, lk = lane >> 4;
-    for (;;) {
-        if (t == 0) s_tile = atomicAdd(counter, 1);
-        __syncthreads();
-        const int tile = s_tile;
-        __syncthreads();
-        if (tile >= ntiles) break;
-        // the problem owning this tile: last p with tile0 <= tile
-        int lo = 0, hi = nprob - 1;
-        while (lo < hi) {
-            const int mid = (lo + hi + 1) >> 1;
-            if (probs[mid].tile0 <= tile) lo = mid; else hi = mid - 1;
-        }
-        const GemmProb p = probs[lo];
+    {
         const int tm = (p.m + TM - 1) / TM;
-        const int local = tile - p.tile0;
         const int i0 = (local % tm) * TM + (wave & 1) * WM;
         const int j0 = (local / tm) * TN + (wave >> 1) * WN;
         const double *a = A + p.a, *b = Bm + p.b;
@@ -1495,6 +1480,41 @@ __global__ __launch_bounds__(256) void batched_gemm_kernel(const GemmProb *__res
     }
 }
 
+// Persistent workgroups take tiles from a flat list over all problems (an
+// atomic counter); each tile runs with its problem's shape.  tticks (when
+// given) accumulates the in-kernel time of every problem's tiles (100 MHz
+// ticks), which the autotuner compares per size bin.
+__global__ __launch_bounds__(256) void batched_gemm_kernel(const GemmProb *__restrict__ probs,
+                                                           int nprob, int ntiles,
+                                                           int *__restrict__ counter,
+                                                           const double *__restrict__ A,
+                                                           const double *__restrict__ Bm,
+                                                           double *__restrict__ Cm,
+                                                           unsigned long long *__restrict__ tticks)
+{
+    __shared__ int s_tile;
+    const int t = threadIdx.x;
+    for (;;) {
+        if (t == 0) s_tile = atomicAdd(counter, 1);
+        __syncthreads();
+        const int tile = s_tile;
+        __syncthreads();
+        if (tile >= ntiles) break;
+        // the problem owning this tile: last p with tile0 <= tile
+        int lo = 0, hi = nprob - 1;
+        while (lo < hi) {
+            const int mid = (lo + hi + 1) >> 1;
+            if (probs[mid].tile0 <= tile) lo = mid; else hi = mid - 1;
+        }
+        const GemmProb p = probs[lo];
+        const long long t0 = wall_clock64();
+        if (p.cfg == 0) bg_tile<32, 32>(p, tile - p.tile0, A, Bm, Cm);
+        else if (p.cfg == 1) bg_tile<64, 64>(p, tile - p.tile0, A, Bm, Cm);
+        else bg_tile<128, 64>(p, tile - p.tile0, A, Bm, Cm);
+        if (tticks && t == 0) atomicAdd(&tticks[p.id], (unsigned long long)(wall_clock64() - t0));
+    }
+}
+
 // ---- launchers ------------------------------------------------------------------
 __global__ void zero_int_kernel(int *p) { *p = 0; }
 
@@ -1537,25 +1557,14 @@ hipError_t batched_gesv(hipStream_t s, int policy, const void *probs, int nprob,
 
 size_t batched_scratch_doubles(int grid) { return (size_t)grid * GSTRIDE; }
 
-hipError_t batched_gemm(hipStream_t s, int tile_cfg, const void *probs, int nprob, int ntiles,
-                        int *counter, const double *A, const double *B, double *C, int grid)
+hipError_t batched_gemm(hipStream_t s, const void *probs, int nprob, int ntiles, int *counter,
+                        const double *A, const double *B, double *C, int grid,
+                        unsigned long long *tticks)
 {
     if (nprob <= 0 || ntiles <= 0) return hipSuccess;
     hipLaunchKernelGGL(zero_int_kernel, dim3(1), dim3(1), 0, s, counter);
-    switch (tile_cfg) {
-    case 0:
-        hipLaunchKernelGGL((batched_gemm_kernel<32, 32>), dim3(grid), dim3(256), 0, s,
-                           (const GemmProb *)probs, nprob, ntiles, counter, A, B, C);
-        break;
-    case 1:
-        hipLaunchKernelGGL((batched_gemm_kernel<64, 64>), dim3(grid), dim3(256), 0, s,
-                           (const GemmProb *)probs, nprob, ntiles, counter, A, B, C);
-        break;
-    default:
-        hipLaunchKernelGGL((batched_gemm_kernel<128, 64>), dim3(grid), dim3(256), 0, s,
-                           (const GemmProb *)probs, nprob, ntiles, counter, A, B, C);
-        break;
-    }
+    hipLaunchKernelGGL(batched_gemm_kernel, dim3(grid), dim3(256), 0, s, (const GemmProb *)probs,
+                       nprob, ntiles, counter, A, B, C, tticks);
     return hipGetLastError();
 }
 

@@ -291,8 +291,11 @@ hipError_t batched_gesv(hipStream_t s, int policy, const void *probs, int nprob,
                         double *A, double *B, float *W32, double *R, double *G, int *ipiv,
                         int *info, int *iters, int itmax, int grid, const BatchOut &out);
 size_t batched_scratch_doubles(int grid);
-hipError_t batched_gemm(hipStream_t s, int tile_cfg, const void *probs, int nprob, int ntiles,
-                        int *counter, const double *A, const double *B, double *C, int grid);
+// GEMMs: each problem's tile shape in its table entry (cfg); tticks (may be
+// null) accumulates every problem's in-kernel time by its id.
+hipError_t batched_gemm(hipStream_t s, const void *probs, int nprob, int ntiles, int *counter,
+                        const double *A, const double *B, double *C, int grid,
+                        unsigned long long *tticks);
 int batched_gemm_tile(int cfg, int *tm, int *tn);
 
 // ---- utilities (util.hip) --------------------------------------------------

@@ -390,10 +390,14 @@ int flx_batch_gemm_create(flx_handle h, flx_batch_t *plan, int count,
                           const double *A, const double *B, double *C,
                           int tune);
 int flx_batch_execute(flx_batch_t plan);
-/* cfg[0] = GEMM tile shape (0: 32x32, 1: 64x64, 2: 128x64), cfg[1] = grid;
- * *tuned_ms = the winning candidate's time (0 when not tuned). */
+/* cfg[0] = GEMM tile shape (0: 32x32, 1: 64x64, 2: 128x64; -1: chosen
+ * per size bin, see flx_batch_bin_tiles), cfg[1] = grid; *tuned_ms = the
+ * winning grid's time (0 when not tuned).  With tune = 1 a GEMM plan times
+ * every problem under each shape (per-tile in-kernel time) and gives each
+ * size bin of max(m, n) -- [16 2^b, 32 2^b), b = 0..5 -- its fastest shape. */
 int flx_batch_config(flx_batch_t plan, int *cfg, double *tuned_ms);
-/* Solve plans: the tile chosen for each of the nbins >= 6 size bins;
+/* The tile chosen for each of the nbins >= 6 size bins (solve plans: the
+ * LU strip width per n bin; GEMM plans: the shape per max(m, n) bin);
  * returns the number of bins (6). */
 int flx_batch_bin_tiles(flx_batch_t plan, int *tiles, int nbins);
 int flx_batch_destroy(flx_batch_t plan);
