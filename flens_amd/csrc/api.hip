@@ -54,6 +54,8 @@ hipError_t init_work(flx_context *h)
     if (e != hipSuccess) return e;
     e = hipMalloc(&h->wk.list, sizeof(int) * PANEL_LIST_SLICE * PANEL_GROUPS_MAX);
     if (e != hipSuccess) return e;
+    e = hipMalloc(&h->wk.lsc, sizeof(double) * lu_lsc_doubles(LU_N_MAX));
+    if (e != hipSuccess) return e;
     e = hipMalloc(&h->wk.flags, sizeof(int) * 4);
     if (e != hipSuccess) return e;
     e = hipMalloc(&h->wk.moved, sizeof(unsigned long long));
@@ -115,6 +117,7 @@ extern "C" int flx_destroy(flx_handle h)
     if (x.ev1) (void)hipEventDestroy(x.ev1);
     if (h->wk.exch) (void)hipFree(h->wk.exch);
     if (h->wk.list) (void)hipFree(h->wk.list);
+    if (h->wk.lsc) (void)hipFree(h->wk.lsc);
     if (h->wk.flags) (void)hipFree(h->wk.flags);
     if (h->wk.moved) (void)hipFree(h->wk.moved);
     if (h->wk.side) (void)hipStreamDestroy(h->wk.side);

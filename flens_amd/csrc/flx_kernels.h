@@ -63,6 +63,7 @@ hipError_t gemm_f32(hipStream_t s, int ta, int tb, int M, int N, int K,
 struct LuWork {
     uint64_t *exch = nullptr;   // panel all-gather slots
     size_t exch_words = 0;
+    double *lsc = nullptr;      // panel multipliers (lu_lsc_doubles of the tallest panel)
     int *list = nullptr;        // row-permutation list of one sub-panel
     int *perm = nullptr;        // composed row permutation (length m)
     int perm_len = 0;
@@ -97,7 +98,8 @@ struct LuOptions {
 };
 
 size_t lu_exch_words();
-// Panel kernel geometry: 512 rows per workgroup, at most 32 workgroups; the
+
+// Panel kernel geometry: 512 rows per workgroup, at most 64 workgroups; the
 // interchange list has one slice of 1 + 2*64 ints per workgroup:
 // [count, dst_0, src_0, dst_1, src_1, ...] in absolute rows.
 constexpr int PANEL_ROWS_PER_WG = 512;
@@ -106,6 +108,9 @@ constexpr int PANEL_GROUPS_MAX = 64;
 // largest matrix order of the FP64 LU and the paths built on it (panel
 // heights up to PANEL_GROUPS_MAX x PANEL_ROWS_PER_WG rows)
 constexpr int LU_N_MAX = 32768;
+// doubles of the panel kernel's multiplier scratch (LuWork::lsc) for panels of
+// up to m rows
+inline size_t lu_lsc_doubles(int m) { return (size_t)(m > 0 ? m : 1) * 32; }
 inline int panel_groups(int mp) { return (mp + PANEL_ROWS_PER_WG - 1) / PANEL_ROWS_PER_WG; }
 // Panel factorisation of the mp x w row-major panel P (w <= 32); piv gets
 // 0-based absolute pivot rows, the composed interchanges go to wk.list.

@@ -37,7 +37,7 @@ inline size_t align256(size_t x) { return (x + 255) & ~(size_t)255; }
 
 // byte offsets of the working buffers inside the scratch array
 struct Layout {
-    size_t exch, flags, list, perm, piv0, W, total;
+    size_t exch, flags, list, lsc, perm, piv0, W, total;
 };
 
 Layout layout(int m, int n)
@@ -48,6 +48,7 @@ Layout layout(int m, int n)
     L.exch = o;  o = align256(o + sizeof(uint64_t) * lu_exch_words());
     L.flags = o; o = align256(o + sizeof(int) * 4);
     L.list = o;  o = align256(o + sizeof(int) * PANEL_LIST_SLICE * PANEL_GROUPS_MAX);
+    L.lsc = o;   o = align256(o + sizeof(double) * lu_lsc_doubles(m));
     L.perm = o;  o = align256(o + sizeof(int) * (size_t)m);
     L.piv0 = o;  o = align256(o + sizeof(int) * (size_t)mn);
     L.W = o;     o = align256(o + sizeof(double) * (size_t)round_ld(n) * (size_t)m);
@@ -119,6 +120,7 @@ extern "C" int flx_dgetrf_noalloc(flx_handle h, int m, int n, double *A, int lda
     wk.exch_words = lu_exch_words();
     wk.flags = (int *)(base + L.flags);
     wk.list = (int *)(base + L.list);
+    wk.lsc = (double *)(base + L.lsc);
     wk.perm = (int *)(base + L.perm);
     wk.perm_len = m;
     wk.gen = d.gen;

@@ -4,19 +4,28 @@
 // 56-136): per column iamax (first maximum), row swap, scaling by the
 // reciprocal of the pivot (division below the safe minimum), rank-1 update.
 //
-// One workgroup holds 512 consecutive panel rows in registers (256 threads x
-// 2 rows, fully unrolled column loop so every register index is static);
-// taller panels use up to 32 cooperating workgroups.  Per column:
-//   1. workgroup-local pivot search (shuffles + LDS), first max on ties;
-//   2. every workgroup posts its best candidate (|v|, row, the row's data)
-//      and the owner of row j posts row j, as sequence-tagged 64-bit words;
-//   3. every workgroup reads all posts (one agent-scope round trip, no
-//      fence, no counter), picks the same winner, swaps, scales and applies
-//      the rank-1 update to its own rows.
-// The posts are double-buffered by column parity: a workgroup can only
-// overwrite a slot after every workgroup has posted the next column, which
-// each does only after it finished reading the previous one.  Every wait is
-// bounded; a failed wait sets the error flag and the kernel still exits.
+// One workgroup holds PANEL_ROWS_PER_WG consecutive panel rows in registers
+// (512 threads x 2 rows, fully unrolled column loop so every register index
+// is static); taller panels use up to PANEL_GROUPS_MAX cooperating
+// workgroups.  Per column:
+//   1. workgroup-local pivot search (DPP wave reductions, one barrier),
+//      first max on ties;
+//   2. the wave that owns the workgroup's candidate posts it -- a header
+//      (position, row) and the row's 32 elements -- as 65 sequence-tagged
+//      64-bit words, one store per lane;
+//   3. every workgroup reads all posts (each thread at most a few words,
+//      stale words re-polled together), stages their payloads in LDS (one
+//      barrier), and every wave picks the same winner and takes the pivot
+//      row into scalar registers (readlane), then scales and applies the
+//      rank-1 update to its own rows.
+// An agent-scope store-to-load round trip between two workgroups costs about
+// 0.9 us (same XCD) to 1.2 us (different XCDs) on MI355X, so a column costs
+// at least that much: the design keeps one round trip and two barriers per
+// column.  The posts are double-buffered by column parity: a workgroup can
+// only overwrite a slot after every workgroup has posted the next column,
+// which each does only after it finished reading the previous one.  Every
+// wait is bounded; a failed wait sets the error flag and the kernel still
+// exits.
 #include <climits>
 #include <cfloat>
 #include <type_traits>
@@ -33,14 +42,21 @@ constexpr int PANEL_LIST_PER_WG = 2 * PANEL_W;
 static_assert(1 + 2 * PANEL_LIST_PER_WG <= PANEL_LIST_SLICE, "list slice too small");
 constexpr uint32_t SPIN_LIMIT = 1u << 20;  // ~1 s of polling per word
 
-// Exchange slots, in 64-bit words, per column parity, sized for the smallest
-// workgroup (256 threads -> at most 64 workgroups):
-//   headers [64][4] | candidate rows [64][2*WMAX] | diagonal row [2*WMAX]
-constexpr int EX_GMAX = PANEL_ROWS_MAX / 256;
-constexpr size_t EX_HDR = 0;
-constexpr size_t EX_ROWS = EX_GMAX * 4;
-constexpr size_t EX_DIAG = EX_ROWS + (size_t)EX_GMAX * 2 * PANEL_WMAX;
-constexpr size_t EX_PAR = EX_DIAG + 2 * PANEL_WMAX;
+// Exchange slots, in 64-bit words, per column parity: one slot of EX_SLOT
+// words per workgroup -- words [0, 64) the candidate row (two words per
+// element, low half first), [64, 66) the reciprocal of its leading element
+// (low, high), word 66 the header (position << 16 | row, or EX_NONE when
+// the workgroup has no live row).  Positions and rows are below
+// LU_N_MAX = 2^15.
+constexpr int EX_GMAX = PANEL_ROWS_MAX / 512;   // room for the smallest workgroup
+constexpr int EX_ROWW = 2 * PANEL_W;            // row words of one post
+constexpr int EX_INV = EX_ROWW;                 // reciprocal of element 0
+constexpr int EX_HDR = EX_ROWW + 2;             // header
+constexpr int EX_POST = EX_ROWW + 3;            // words one workgroup posts per column
+constexpr size_t EX_SLOT = 128;
+constexpr size_t EX_PAR = (size_t)EX_GMAX * EX_SLOT;
+constexpr uint32_t EX_NONE = 0xffffffffu;
+static_assert(PANEL_ROWS_MAX <= 32768, "positions and rows are packed in 16 bits");
 
 size_t lu_exch_words() { return 2 * EX_PAR; }
 
@@ -87,20 +103,89 @@ FLX_DEV Cand wave_best(uint64_t key, int pos, int rid)
     return c;
 }
 
+// Largest 32-bit value over the first SPAN lanes (4, 16 or 64), uniform:
+// xor-1 and xor-2 quad permutations, then row rotations by 4 and 8 give
+// every lane the maximum of its 16-lane row; four readlanes join the rows.
+template <int SPAN>
+FLX_DEV uint32_t span_umax(uint32_t v)
+{
+    v = max(v, (uint32_t)__builtin_amdgcn_mov_dpp((int)v, 0xb1, 0xf, 0xf, false));   // quad_perm 1,0,3,2
+    v = max(v, (uint32_t)__builtin_amdgcn_mov_dpp((int)v, 0x4e, 0xf, 0xf, false));   // quad_perm 2,3,0,1
+    if constexpr (SPAN > 4) {
+        v = max(v, (uint32_t)__builtin_amdgcn_mov_dpp((int)v, 0x124, 0xf, 0xf, false)); // row_ror 4
+        v = max(v, (uint32_t)__builtin_amdgcn_mov_dpp((int)v, 0x128, 0xf, 0xf, false)); // row_ror 8
+    }
+    uint32_t m = (uint32_t)__builtin_amdgcn_readlane((int)v, 0);
+    if constexpr (SPAN > 16) {
+        m = max(m, (uint32_t)__builtin_amdgcn_readlane((int)v, 16));
+        m = max(m, (uint32_t)__builtin_amdgcn_readlane((int)v, 32));
+        m = max(m, (uint32_t)__builtin_amdgcn_readlane((int)v, 48));
+    }
+    return m;
+}
+// wave_best for candidates held by the first SPAN lanes (keys 0 beyond):
+// when one lane alone holds the largest high key word -- all but exact or
+// near-exact ties -- it is the winner; otherwise the exact wave_best.
+template <int SPAN>
+FLX_DEV Cand span_best(uint64_t key, int pos, int rid)
+{
+    const uint32_t hi = (uint32_t)(key >> 32);
+    const uint32_t m = span_umax<SPAN>(hi);
+    const unsigned long long top = __ballot(hi == m);
+    if (__popcll(top) == 1) {
+        const int L = (int)__ffsll(top) - 1;
+        Cand c;
+        c.key = ((uint64_t)m << 32) | (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)key, L);
+        c.pos = __builtin_amdgcn_readlane(pos, L);
+        c.rid = __builtin_amdgcn_readlane(rid, L);
+        return c;
+    }
+    return wave_best(key, pos, rid);
+}
+
+// v as a wave-uniform (scalar) value; v must be the same in every lane
+FLX_DEV double uniform_f64(double v)
+{
+    const uint64_t b = (uint64_t)__double_as_longlong(v);
+    const uint32_t lo = (uint32_t)__builtin_amdgcn_readfirstlane((int)(uint32_t)b);
+    const uint32_t hi = (uint32_t)__builtin_amdgcn_readfirstlane((int)(uint32_t)(b >> 32));
+    return __longlong_as_double((long long)(((uint64_t)hi << 32) | lo));
+}
+
+// the double whose low and high halves are the two 32-bit LDS words at p
+// (8-byte aligned)
+FLX_DEV double lds_f64(const uint32_t *p)
+{
+    double d;
+    __builtin_memcpy(&d, p, sizeof d);
+    return d;
+}
+
+// the two 32-bit halves of a double, as a uniform value from lane src
+FLX_DEV double read_lane_f64(double v, int src)
+{
+    const uint64_t b = (uint64_t)__double_as_longlong(v);
+    const uint32_t lo = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)b, src);
+    const uint32_t hi = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)(b >> 32), src);
+    return __longlong_as_double((long long)(((uint64_t)hi << 32) | lo));
+}
+
 template <int NT, int RPT, int GMAX>
 __global__ __launch_bounds__(NT) void panel_kernel(
     double *__restrict__ P, long ldw, int mp, int w, uint64_t *__restrict__ ex,
-    uint32_t gen, int *__restrict__ piv, int piv_base, int *__restrict__ list,
-    int *__restrict__ info, int info_base, int *__restrict__ err)
+    uint32_t gen, double *__restrict__ lsc, int *__restrict__ piv, int piv_base,
+    int *__restrict__ list, int *__restrict__ info, int info_base, int *__restrict__ err)
 {
     constexpr int W = PANEL_W;
     constexpr int NWAVE = NT / 64;
     constexpr int ROWS = NT * RPT;                 // panel rows per workgroup
-    static_assert(GMAX <= PANEL_ROWS_MAX / ROWS, "groups");
-    // words one workgroup reads per column at most: G headers (4 words) and
-    // the live part of G candidate rows (2 words per element)
-    constexpr int WORDS_MAX = 4 * GMAX + 2 * W * GMAX;
+    static_assert(GMAX * ROWS <= PANEL_ROWS_MAX && GMAX <= EX_GMAX && GMAX <= 64, "groups");
+    static_assert(W == 32 && EX_ROWW == 64 && NWAVE <= 4, "one posted word per lane");
+    // words one workgroup reads per column at most, and per thread
+    constexpr int WORDS_MAX = EX_POST * GMAX;
     constexpr int PER = (WORDS_MAX + NT - 1) / NT;
+    constexpr int LS = EX_POST + 1;                // LDS stride of a post (8-byte aligned)
+    constexpr int GSPAN = GMAX <= 4 ? 4 : GMAX <= 16 ? 16 : 64;
 
     const int G = gridDim.x;
     const int g = blockIdx.x;
@@ -108,15 +193,14 @@ __global__ __launch_bounds__(NT) void panel_kernel(
     const int lane = tid & 63, wave = tid >> 6;
     const int mn = mp < w ? mp : w;
 
-    __shared__ uint64_t s_raw[WORDS_MAX];     // posted words as first read
-    __shared__ uint32_t s_words[WORDS_MAX];   // low halves of the posted words
-    __shared__ double s_own[W];                // this workgroup's candidate row
-    __shared__ double s_u[W];                  // pivot row, shifted, finished cols = 0
+    // posts of one column, by column parity: element c of a post's row at
+    // [2c] (low half) and [2c + 1] (high half), its reciprocal at EX_INV,
+    // the header at EX_HDR
+    __shared__ __align__(16) uint32_t s_words[2][LS * GMAX];
+    __shared__ __align__(16) double s_u[NWAVE][2][W];   // per wave: shifted pivot rows
+    __shared__ __align__(16) double s_own[W + 1];        // the candidate, posted form + 1/pivot
     __shared__ uint64_t s_wkey[NWAVE];
     __shared__ int s_wpos[NWAVE], s_wrid[NWAVE];
-    __shared__ int s_win[3];                   // pos, rid, source workgroup
-    __shared__ double s_pv;
-    __shared__ int s_ufin;                     // 1: every entry of s_u is finite
     __shared__ int s_piv[W];
     __shared__ int s_cnt;
     __shared__ int s_fail;
@@ -125,227 +209,290 @@ __global__ __launch_bounds__(NT) void panel_kernel(
     // Thread tid owns original panel rows g*ROWS + tid + k*NT.  Rows never
     // move during the panel: pos[k] is the row's current position under the
     // interchanges so far (LAPACK's row index), done[k] marks pivot rows.
-    // Each row is kept rotated: element 0 is always the current column j,
-    // elements W-j.. hold the finished columns 0..j-1.
+    // A live row is kept rotated: element e is column j + e at column j;
+    // its multipliers (the finished columns, L) go to the scratch lsc
+    // (column-major, leading dimension mp) as they are made.  A pivot row
+    // is never touched again after its column: element e is column pos + e.
+    //
+    // The rank-1 update of column j is split.  Right after the pivot is
+    // known only element 0 of the live rows is updated (column j + 1, the
+    // next pivot search's input); elements 1..W-2 follow one step later,
+    // while the next column's posts are in flight.  In between a live row
+    // is "pending": elements 2..W-1 still hold the old values, lm[k] its
+    // multiplier.  A candidate row is posted in that form (element 0, old
+    // elements 2..W-1, multiplier), and readers finish it with the previous
+    // pivot row, which every workgroup holds: the same operation on the
+    // same operands as the owner's own update, so every copy is
+    // bit-identical.  The live part of a row takes fma(-l, u, a) whatever
+    // l is (as the reference's rank-1 update of the rows below the pivot,
+    // tf2.tcc:130-132, does for a non-finite multiplier too).
     int rid[RPT], pos[RPT];
-    bool done[RPT];
+    bool done[RPT], any[RPT];
     dvecW row[RPT];
+    double lm[RPT];
 #pragma unroll
     for (int k = 0; k < RPT; ++k) {
         rid[k] = g * ROWS + tid + k * NT;
         pos[k] = rid[k];
         done[k] = rid[k] >= mp;
-        const double *src = P + (long)(rid[k] < mp ? rid[k] : 0) * ldw;
+        any[k] = __ballot(rid[k] < mp) != 0;   // the wave has a row in slot k
+        lm[k] = 0.0;
+        row[k] = dvecW(0.0);
+        if (rid[k] < mp) {
+            const double *src = P + (long)rid[k] * ldw;
+            if (w == W) {
+                row[k] = *(const dvecW *)src;
+            } else {
 #pragma unroll
-        for (int c = 0; c < W; ++c) {
-            double v = 0.0;
-            if (rid[k] < mp && c < w) v = src[c];
-            row[k][c] = v;
+                for (int c = 0; c < W; ++c)
+                    if (c < w) row[k][c] = src[c];
+            }
         }
     }
     if (tid == 0) { s_fail = 0; s_cnt = 0; }
     int first_zero = 0;
     __syncthreads();
 
+    // The pivot row of column jj (the winner of group wg, posted pending
+    // when jj > 0): element c in lane c, finished with the pivot row of
+    // column jj - 1 (u_prev); writes this wave's shifted copy u (finished
+    // columns 0).
+    auto pivot_row = [&](int jj, int wg, const double *u_prev, double *u) {
+        const uint32_t *post = &s_words[jj & 1][wg * LS];
+        double e = 0.0;
+        if (lane < W) {
+            const double p = lds_f64(post + 2 * lane);
+            e = p;
+            if (jj > 0 && lane > 0) e = fma(-lds_f64(post + 2 * (W - 1)), u_prev[lane], p);
+        }
+        const int lv = W - jj;
+        if (lane > 0 && lane < W) u[lane - 1] = lane < lv ? e : 0.0;
+        if (lane == 0) u[W - 1] = 0.0;
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    };
+    // the deferred part of a column's update on the live rows, with its
+    // pivot row u
+    auto finish_update = [&](const double *u) {
+#pragma unroll
+        for (int k = 0; k < RPT; ++k) {
+            if (!any[k]) continue;
+            if (!done[k]) {
+                const double m = -lm[k];
+#pragma unroll
+                for (int c0 = 1; c0 < W - 1; c0 += 8) {
+                    double uu[8];
+#pragma unroll
+                    for (int q = 0; q < 8; ++q) uu[q] = c0 + q < W - 1 ? u[c0 + q] : 0.0;
+#pragma unroll
+                    for (int q = 0; q < 8; ++q)
+                        if (c0 + q < W - 1) row[k][c0 + q] = fma(m, uu[q], row[k][c0 + q + 1]);
+                }
+            }
+        }
+    };
+
+    int wg_prev = 0;
     for (int j = 0; j < mn; ++j) {
         const int live = W - j;                  // live elements of a rotated row
-        // ---- 1. local pivot candidate ------------------------------------
+        const int par = j & 1;
+        // ---- 1. workgroup-local pivot candidate ------------------------------
         uint64_t key = 0;
         int kp = INT_MAX, kr = -1;
+        double kv = 1.0;
 #pragma unroll
         for (int k = 0; k < RPT; ++k) {
             const uint64_t kk = done[k] ? 0 : pivot_key(row[k][0], pos[k], j);
-            if (cand_gt(kk, done[k] ? INT_MAX : pos[k], key, kp)) { key = kk; kp = pos[k]; kr = rid[k]; }
+            if (cand_gt(kk, done[k] ? INT_MAX : pos[k], key, kp)) {
+                key = kk; kp = pos[k]; kr = rid[k]; kv = row[k][0];
+            }
         }
-        const Cand wb = wave_best(key, kp, kr);
+        // every lane's candidate reciprocal, ready when its row is posted
+        const double kinv = 1.0 / kv;
+        const Cand wb = span_best<64>(key, kp, kr);
         if (lane == 0) { s_wkey[wave] = wb.key; s_wpos[wave] = wb.pos; s_wrid[wave] = wb.rid; }
         __syncthreads();
-        uint64_t bk = s_wkey[0];
-        int bp = s_wpos[0], br = s_wrid[0];
+        {
+            const bool q = lane < NWAVE;
+            const Cand c = span_best<4>(q ? s_wkey[lane] : 0ull, q ? s_wpos[lane] : INT_MAX,
+                                        q ? s_wrid[lane] : -1);
+            const int br = c.key == 0 ? -1 : c.rid;   // -1: no live row in this workgroup
+            // ---- 2. the candidate row in posted form, one word per lane -----
+            const int otid = br >= 0 ? (br - g * ROWS) % NT : 0;
+            if (wave == (otid >> 6)) {
+                if (lane == (otid & 63) && br >= 0) {
 #pragma unroll
-        for (int q = 1; q < NWAVE; ++q)
-            if (cand_gt(s_wkey[q], s_wpos[q], bk, bp)) { bk = s_wkey[q]; bp = s_wpos[q]; br = s_wrid[q]; }
-
-        int wpos = bp, wrid = br;
-        if (G == 1) {
-            // single workgroup: the winner's owner fills the shifted pivot row
+                    for (int k = 0; k < RPT; ++k)
+                        if (rid[k] == br) {
+                            dvecW t = row[k];
+                            if (j > 0) {
 #pragma unroll
-            for (int k = 0; k < RPT; ++k)
-                if (rid[k] == br) {
-                    s_pv = row[k][0];
-                    bool fin = true;
-#pragma unroll
-                    for (int c = 0; c < W - 1; ++c) {
-                        const double uc = (c + 1 < live) ? row[k][c + 1] : 0.0;
-                        fin = fin && __builtin_isfinite(uc);
-                        s_u[c] = uc;
-                    }
-                    s_u[W - 1] = 0.0;
-                    s_ufin = fin ? 1 : 0;
+                                for (int e = 1; e < W - 1; ++e) t[e] = row[k][e + 1];
+                                t[W - 1] = lm[k];
+                            }
+                            *(dvecW *)s_own = t;
+                        }
+                    s_own[W] = kinv;
                 }
-            __syncthreads();
-        } else {
-#pragma unroll
-            for (int k = 0; k < RPT; ++k)
-                if (rid[k] == br) {
-#pragma unroll
-                    for (int c = 0; c < W; ++c) s_own[c] = row[k][c];
+                __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+                __builtin_amdgcn_wave_barrier();
+                __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+                // lane l: half l & 1 of element l >> 1; lanes 0, 1: the
+                // reciprocal; lane 2: the header
+                const uint64_t bits = (uint64_t)__double_as_longlong(s_own[lane >> 1]);
+                const uint32_t half = (lane & 1) ? (uint32_t)(bits >> 32) : (uint32_t)bits;
+                const uint64_t ibits = (uint64_t)__double_as_longlong(s_own[W]);
+                const uint32_t hdr = br >= 0 ? (((uint32_t)c.pos << 16) | (uint32_t)br) : EX_NONE;
+                const uint32_t extra = lane == 0 ? (uint32_t)ibits
+                                     : lane == 1 ? (uint32_t)(ibits >> 32) : hdr;
+                if (G == 1) {
+                    s_words[par][lane] = half;
+                    if (lane < 3) s_words[par][EX_INV + lane] = extra;
+                } else {
+                    const uint64_t tag = (uint64_t)((gen << 8) | (uint32_t)(j + 1)) << 32;
+                    uint64_t *slot = ex + (size_t)par * EX_PAR + (size_t)g * EX_SLOT;
+                    st_agent(slot + lane, tag | half);
+                    if (lane < 3) st_agent(slot + EX_INV + lane, tag | extra);
                 }
-            __syncthreads();
-            // ---- 2./3. all-gather: header + live part of the candidate row -
-            const uint32_t seq = (gen << 8) | (uint32_t)(j + 1);
-            uint64_t *base = ex + (size_t)(j & 1) * EX_PAR;
-            const uint64_t tag = (uint64_t)seq << 32;
-            if (tid < 2 * live) {
-                const int c = tid >> 1;
-                uint64_t b = (uint64_t)__double_as_longlong(s_own[c]);
-                st_agent(base + EX_ROWS + (size_t)g * 2 * PANEL_WMAX + tid,
-                         tag | ((tid & 1) ? (b & 0xffffffffull) : (b >> 32)));
-            } else if (tid >= NT - 4) {
-                const int q = tid - (NT - 4);
-                uint64_t pay = q == 0 ? (bk >> 32)
-                             : q == 1 ? (bk & 0xffffffffull)
-                             : q == 2 ? (uint64_t)(uint32_t)bp : (uint64_t)(uint32_t)br;
-                st_agent(base + EX_HDR + g * 4 + q, tag | pay);
             }
-            // words: [0, 4G) headers, then G rows of 2*live words each
-            const int nhdr = 4 * G, rw = 2 * live, total = nhdr + G * rw;
-            auto word_addr = [&](int idx) -> size_t {
-                if (idx < nhdr) return EX_HDR + idx;
-                const int q = idx - nhdr;
-                return EX_ROWS + (size_t)(q / rw) * 2 * PANEL_WMAX + q % rw;
-            };
-            // every word this thread reads is requested at once (one round
-            // trip) and staged in LDS; one runtime loop re-polls stale tags
+        }
+        // ---- 3. the previous column's update, while the posts travel -------
+        if (j > 0) {
+            pivot_row(j - 1, wg_prev, s_u[wave][par], s_u[wave][par ^ 1]);
+            finish_update(s_u[wave][par ^ 1]);
+        }
+        // ---- 4. gather every workgroup's post ------------------------------
+        if (G > 1) {
+            const uint32_t seq = (gen << 8) | (uint32_t)(j + 1);
+            const uint64_t *base = ex + (size_t)par * EX_PAR;
+            const int total = EX_POST * G;
+            // every word this thread reads is requested at once; stale ones
+            // are requested again together until every tag is the column's
             uint64_t v[PER];
 #pragma unroll
             for (int k = 0; k < PER; ++k) {
                 const int idx = tid + k * NT;
-                v[k] = idx < total ? ld_agent(base + word_addr(idx)) : tag;
+                const int q = idx < total ? idx : 0;
+                v[k] = idx < total ? ld_agent(base + (size_t)(q / EX_POST) * EX_SLOT + q % EX_POST)
+                                   : ((uint64_t)seq << 32);
+            }
+            uint32_t spins = 0;
+            for (;;) {
+                bool stale = false;
+#pragma unroll
+                for (int k = 0; k < PER; ++k) stale = stale || seq_of(v[k]) != seq;
+                if (!stale) break;
+                // give up for good once any wait of this workgroup failed
+                if (*(volatile int *)&s_fail || ++spins > SPIN_LIMIT) {
+                    s_fail = 1;
+                    break;
+                }
+#pragma unroll
+                for (int k = 0; k < PER; ++k) {
+                    const int idx = tid + k * NT;
+                    const int q = idx < total ? idx : 0;
+                    if (seq_of(v[k]) != seq)
+                        v[k] = ld_agent(base + (size_t)(q / EX_POST) * EX_SLOT + q % EX_POST);
+                }
             }
 #pragma unroll
             for (int k = 0; k < PER; ++k) {
                 const int idx = tid + k * NT;
-                if (idx < total) s_raw[idx] = v[k];
+                if (idx < total) s_words[par][(idx / EX_POST) * LS + idx % EX_POST] = (uint32_t)v[k];
             }
-            for (int idx = tid; idx < total; idx += NT) {
-                uint64_t x = s_raw[idx];
-                uint32_t spins = 0;
-                while (seq_of(x) != seq) {
-                    // give up for good once any wait of this workgroup failed
-                    if (*(volatile int *)&s_fail || ++spins > SPIN_LIMIT) {
-                        s_fail = 1;
-                        break;
-                    }
-                    __builtin_amdgcn_s_sleep(1);
-                    x = ld_agent(base + word_addr(idx));
-                }
-                s_words[idx] = (uint32_t)(x & 0xffffffffull);
-            }
-            __syncthreads();
-            if (wave == 0) {
-                uint64_t qk = 0;
-                int qp = INT_MAX;
-                if (lane < G) {
-                    qk = ((uint64_t)s_words[4 * lane] << 32) | s_words[4 * lane + 1];
-                    qp = (int)s_words[4 * lane + 2];
-                }
-                const Cand wc = wave_best(qk, qp, lane);   // rid slot carries the group
-                const int wg = wc.rid;
-                int wr = (wg >= 0 && wg < G) ? (int)s_words[4 * wg + 3] : -1;
-                int wp = wc.pos;
-                // a winner outside the panel can only come from a failed wait:
-                // keep every later index in bounds and report it
-                const bool bad = wp < j || wp >= mp || wr < 0 || wr >= mp || wg < 0 || wg >= G;
-                if (lane == 0) {
-                    if (bad) s_fail = 1;
-                    s_win[0] = bad ? j : wp;
-                    s_win[1] = bad ? -1 : wr;
-                }
-                // the winner's row, shifted by one, finished columns zeroed
-                const int src = nhdr + (bad ? 0 : wg) * rw;
-                double uc = 0.0;
-                if (lane < W) {
-                    if (lane < live)
-                        uc = __longlong_as_double((long long)(((uint64_t)s_words[src + 2 * lane] << 32) |
-                                                              (uint64_t)s_words[src + 2 * lane + 1]));
-                    if (lane == 0) s_pv = uc;
-                    else s_u[lane - 1] = uc;
-                    if (lane == W - 1) s_u[W - 1] = 0.0;
-                }
-                // the pivot itself (lane 0) is not part of u
-                const bool nonfin = lane > 0 && lane < W && !__builtin_isfinite(uc);
-                const unsigned long long bad_u = __ballot(nonfin);
-                if (lane == 0) s_ufin = bad_u ? 0 : 1;
-            }
-            __syncthreads();
-            wpos = s_win[0];
-            wrid = s_win[1];
         }
-        const double pv = s_pv;
-        const bool ufin = s_ufin != 0;
+        __syncthreads();
+        // ---- 5. every wave picks the winner (lane q evaluates workgroup q) -
+        uint64_t qk = 0;
+        int qp = INT_MAX;
+        uint32_t qh = EX_NONE;
+        if (lane < G) {
+            qh = s_words[par][lane * LS + EX_HDR];
+            if (qh != EX_NONE) {
+                qp = (int)(qh >> 16);
+                qk = pivot_key(lds_f64(&s_words[par][lane * LS]), qp, j);
+            }
+        }
+        const Cand wc = span_best<GSPAN>(qk, qp, lane);   // rid slot carries the group
+        int wg = wc.rid;
+        const uint32_t wh = (uint32_t)__builtin_amdgcn_readlane((int)qh, wg >= 0 && wg < 64 ? wg : 0);
+        int wpos = wc.pos;
+        int wrid = (int)(wh & 0xffffu);
+        // a winner outside the panel can only come from a failed wait: keep
+        // every later index in bounds and report it
+        const bool bad = wc.key == 0 || wg < 0 || wg >= G || wh == EX_NONE || wpos < j ||
+                         wpos >= mp || wrid >= mp;
+        if (bad) {
+            if (lane == 0) s_fail = 1;
+            wg = 0;
+            wpos = j;
+            wrid = -1;
+        }
+        wg_prev = wg;
+        // the pivot, its reciprocal and the winner's element 1 (finished
+        // with the previous pivot row as the owner would), as scalars
+        const uint32_t *post = &s_words[par][wg * LS];
+        const double pv = uniform_f64(lds_f64(post));
+        const double inv = uniform_f64(lds_f64(post + EX_INV));
+        const double p1 = uniform_f64(lds_f64(post + 2));
+        const double lw = uniform_f64(lds_f64(post + 2 * (W - 1)));
+        const double up1 = uniform_f64(s_u[wave][par ^ 1][1]);
+        const double u0 = 1 < live ? (j > 0 ? fma(-lw, up1, p1) : p1) : 0.0;
 
-        // ---- interchange bookkeeping, scale, rank-1 update + rotate --------
+        // ---- 6. interchange bookkeeping, multipliers, element 0 ------------
 #pragma unroll
         for (int k = 0; k < RPT; ++k) {
             if (rid[k] == wrid) { pos[k] = j; done[k] = true; }
             else if (pos[k] == j) pos[k] = wpos;
         }
         const bool zero = pv == 0.0;
-        const double inv = 1.0 / pv;
         const bool tiny = fabs(pv) < DBL_MIN;
 #pragma unroll
         for (int k = 0; k < RPT; ++k) {
-            // multiplier of a live row; finished rows just rotate (l = 0)
-            double l = row[k][0];
-            if (!zero) l = tiny ? l / pv : l * inv;
-            const double lm = done[k] ? 0.0 : l;
-            const double last = done[k] ? row[k][0] : l;
-            // the reference updates only the rows below the pivot
-            // (tf2.tcc:130-132): a finished row must not see 0 * Inf / NaN
-            // from a pivot row that holds them
-            const bool plain = done[k] ? ufin : __builtin_isfinite(lm);
-            if (plain) {
-                // new[c] = row[c+1] - l*u[c+1]; finished columns see u = 0
-#pragma unroll
-                for (int c = 0; c < W - 1; ++c) row[k][c] = fma(-lm, s_u[c], row[k][c + 1]);
-            } else {
-                // a non-finite multiplier must not touch the finished
-                // columns; a finished row only rotates
-                const int upd = done[k] ? 0 : live - 1;
-#pragma unroll
-                for (int c = 0; c < W - 1; ++c)
-                    row[k][c] = (c < upd) ? fma(-lm, s_u[c], row[k][c + 1]) : row[k][c + 1];
+            if (!any[k]) continue;
+            if (!done[k]) {
+                // the multiplier (tf2.tcc:112-121: times the reciprocal,
+                // divided below the safe minimum; a zero pivot scales nothing)
+                double l = row[k][0];
+                if (!zero) {
+                    if (tiny) l = l / pv;
+                    else l = l * inv;
+                }
+                lm[k] = l;
+                lsc[(long)j * mp + rid[k]] = l;
+                row[k][0] = fma(-l, u0, row[k][1]);
             }
-            row[k][W - 1] = last;
         }
         if (tid == 0) {
             s_piv[j] = wpos;
             if (zero && first_zero == 0) first_zero = j + 1;
         }
-        // s_u / s_own are rewritten next column only after its first barrier,
-        // which every thread reaches after finishing the update above
+        // s_wkey / s_own / s_words[par] are rewritten only after later
+        // barriers, which every thread reaches after its reads above
     }
-    // undo the remaining rotation: element c holds column c again
-    for (int j = mn; j < W; ++j) {
-#pragma unroll
-        for (int k = 0; k < RPT; ++k) {
-            const double first = row[k][0];
-#pragma unroll
-            for (int c = 0; c < W - 1; ++c) row[k][c] = row[k][c + 1];
-            row[k][W - 1] = first;
-        }
+    if (mn > 0) {
+        const int jl = mn - 1, pl = jl & 1;
+        pivot_row(jl, wg_prev, s_u[wave][pl ^ 1], s_u[wave][pl]);
+        finish_update(s_u[wave][pl]);
     }
 
-    // write every row to its final position; record the moved ones
+    // write every row to its final position: columns below `start` are
+    // its multipliers (from lsc), element e of the registers is column
+    // start + e (a pivot row: start = its column; a row left below the
+    // panel's pivots: start = mn); record the moved rows
     int *mylist = list + (size_t)g * PANEL_LIST_SLICE;
 #pragma unroll
     for (int k = 0; k < RPT; ++k) {
         if (rid[k] < mp) {
             double *dst = P + (long)pos[k] * ldw;
+            const int start = done[k] ? pos[k] : mn;
 #pragma unroll
             for (int c = 0; c < W; ++c)
-                if (c < w) dst[c] = row[k][c];
+                if (c < start) dst[c] = lsc[(long)c * mp + rid[k]];
+#pragma unroll
+            for (int e = 0; e < W; ++e)
+                if (start + e < w) dst[start + e] = row[k][e];
             if (pos[k] != rid[k]) {
                 const int e = atomicAdd(&s_cnt, 1);
                 if (e < PANEL_LIST_PER_WG) {
@@ -376,7 +523,7 @@ hipError_t launch_panel(hipStream_t s, LuWork &wk, double *P, long ldw,
                                int mp, int w, int *piv, int piv_base, int *info,
                                int ib)
 {
-    if (mp > PANEL_ROWS_MAX) return hipErrorInvalidValue;
+    if (mp > PANEL_ROWS_MAX || !wk.lsc) return hipErrorInvalidValue;
     wk.gen = (wk.gen + 1) & 0xffffffu;
     if (wk.gen == 0) {
         // sequence space wrapped: clear the slots so no stale tag can match
@@ -388,21 +535,25 @@ hipError_t launch_panel(hipStream_t s, LuWork &wk, double *P, long ldw,
     const double pf = (double)w * w * mp - (double)w * w * w / 3.0;
     ProfScope ps(wk.prof, s, PROF_PANEL, pf, 16.0 * mp * w);
     // 256 threads (one wave per SIMD) x 2 rows per thread: 512 rows per
-    // workgroup, at most PANEL_GROUPS_MAX cooperating workgroups
+    // workgroup, at most LU_N_MAX / 512 cooperating workgroups
     (void)ib;
     constexpr int NT = 256, RPT = 2;
     static_assert(NT * RPT == PANEL_ROWS_PER_WG, "panel geometry");
-    // the words a workgroup gathers per column scale with the group count:
-    // panels of at most half the largest height use the smaller variant
+    // the words a workgroup gathers per column scale with the group count
     const int G = panel_groups(mp);
-    if (G <= PANEL_GROUPS_MAX / 2)
-        hipLaunchKernelGGL((panel_kernel<NT, RPT, PANEL_GROUPS_MAX / 2>), dim3(G), dim3(NT), 0, s,
-                           P, ldw, mp, w, wk.exch, wk.gen, piv, piv_base, wk.list, info, piv_base,
-                           wk.flags);
+    if (G <= 8)
+        hipLaunchKernelGGL((panel_kernel<NT, RPT, 8>), dim3(G), dim3(NT), 0, s, P, ldw, mp, w,
+                           wk.exch, wk.gen, wk.lsc, piv, piv_base, wk.list, info, piv_base, wk.flags);
+    else if (G <= 16)
+        hipLaunchKernelGGL((panel_kernel<NT, RPT, 16>), dim3(G), dim3(NT), 0, s, P, ldw, mp, w,
+                           wk.exch, wk.gen, wk.lsc, piv, piv_base, wk.list, info, piv_base, wk.flags);
+    else if (G <= 32)
+        hipLaunchKernelGGL((panel_kernel<NT, RPT, 32>), dim3(G), dim3(NT), 0, s, P, ldw, mp, w,
+                           wk.exch, wk.gen, wk.lsc, piv, piv_base, wk.list, info, piv_base, wk.flags);
     else
-        hipLaunchKernelGGL((panel_kernel<NT, RPT, PANEL_GROUPS_MAX>), dim3(G), dim3(NT), 0, s, P,
-                           ldw, mp, w, wk.exch, wk.gen, piv, piv_base, wk.list, info, piv_base,
-                           wk.flags);
+        hipLaunchKernelGGL((panel_kernel<NT, RPT, LU_N_MAX / PANEL_ROWS_PER_WG>), dim3(G),
+                           dim3(NT), 0, s, P, ldw, mp, w, wk.exch, wk.gen, wk.lsc, piv,
+                           piv_base, wk.list, info, piv_base, wk.flags);
     return hipGetLastError();
 }
 
