@@ -61,12 +61,13 @@ NOALLOC_SCRATCH_MIB = 36   # FLX_NOALLOC_SCRATCH_MIB of include/flx.h
 
 PREC = {"fp32": 0, "bf16": 1, "fp16": 2}
 REFINE = {"classic": 0, "gmres": 1, "auto": 2}
+PIVOT = {"tournament": 0, "rbt": 1}
 
 
 class MixedOpts(C.Structure):
     _fields_ = [("prec", C.c_int), ("refine", C.c_int), ("itermax", C.c_int),
                 ("gmres_m", C.c_int), ("tol", C.c_double), ("gmres_tol", C.c_double),
-                ("fallback", C.c_int), ("nb", C.c_int), ("graph", C.c_int)]
+                ("fallback", C.c_int), ("nb", C.c_int), ("graph", C.c_int), ("pivot", C.c_int)]
 
 
 class BatchExtra(C.Structure):
@@ -619,13 +620,15 @@ class lapack:
 
     @staticmethod
     def sv_mixed(A, B, X, prec="fp32", refine="classic", itermax=30, gmres_m=40, tol=0.0,
-                 gmres_tol=0.0, fallback=True, nb=256, graph=True,
+                 gmres_tol=0.0, fallback=True, nb=256, graph=True, pivot="tournament",
                  handle: Handle | None = None) -> dict:
         """Mixed-precision solve of A X = B on device matrices; returns the
-        solver's statistics (iterations, fallback, backward error, times)."""
+        solver's statistics (iterations, fallback, backward error, times).
+        pivot: "tournament" (CALU panels) or "rbt" (random butterflies, no
+        pivoting)."""
         h = handle or default_handle()
         o = MixedOpts(PREC[prec], REFINE[refine], itermax, gmres_m, tol, gmres_tol,
-                      int(bool(fallback)), nb, int(bool(graph)))
+                      int(bool(fallback)), nb, int(bool(graph)), PIVOT[pivot])
         st = MixedStats()
         _check(lib().flx_dgesv_mixed(h.ptr, A.n, B.n, _vp(A.ptr), A.ld, _vp(B.ptr), B.ld,
                                      _vp(X.ptr), X.ld, C.byref(o), C.byref(st)),

@@ -9,7 +9,7 @@ struct DevBuf {
     size_t cap = 0;
 };
 
-constexpr int MIX_GKEY = 22;   // values keying the captured factorisation
+constexpr int MIX_GKEY = 23;   // values keying the captured factorisation
 
 // buffers of the mixed-precision solver
 struct MixBufs {
@@ -20,6 +20,8 @@ struct MixBufs {
     hipEvent_t ev_a = nullptr, ev_r = nullptr;
     DevBuf inv_l, inv_u, y32, y64, flags, err, dr, dc, part, scal, R, V, w, t, zeros, mpart, hbuf;
     DevBuf F;                               // FP64 fallback factors (A kept for the residual)
+    DevBuf rbt;                             // random butterflies of order rbt_n (4 x n doubles)
+    int rbt_n = 0;
     unsigned epoch = 0;
     // captured factorisation: valid while the key matches
     hipGraphExec_t graph = nullptr;

@@ -3,6 +3,7 @@
 
 #include <hip/hip_runtime.h>
 #include <stdint.h>
+#include <cfloat>
 #include <type_traits>
 
 #define FLX_DEV static __device__ __forceinline__
@@ -113,4 +114,19 @@ FLX_DEV uint32_t wave_umin(uint32_t v)
     v = min(v, (uint32_t)__builtin_amdgcn_update_dpp(-1, (int)v, 0x142, 0xa, 0xf, false));
     v = min(v, (uint32_t)__builtin_amdgcn_update_dpp(-1, (int)v, 0x143, 0xc, 0xf, false));
     return (uint32_t)__builtin_amdgcn_readlane((int)v, 63);
+}
+
+// max of non-negative doubles through their bit patterns
+FLX_DEV void atomic_max_nonneg(unsigned long long *slot, double v)
+{
+    if (v > 0.0 || v != v) atomicMax(slot, (unsigned long long)__double_as_longlong(v));
+}
+
+// power of two 2^-e with v = f 2^e, f in [0.5, 1): scales v into [0.5, 1)
+FLX_DEV double pow2_recip(double v)
+{
+    if (!(v > 0.0) || !(v <= DBL_MAX)) return 1.0;
+    int e;
+    frexp(v, &e);
+    return ldexp(1.0, -e);
 }

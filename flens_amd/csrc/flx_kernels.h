@@ -220,15 +220,44 @@ hipError_t getrf_rm_mixed(hipStream_t s, const MixWork &mw, float *W, long ldw, 
 // W[i][j] = float(A(i,j) dr[i] dc[j]) (dr, dc optional); *flag |= 1 on overflow
 hipError_t conv_d2s_rm(hipStream_t s, const double *A, long lda, int n, const double *dr,
                        const double *dc, float *W, long ldw, int *flag);
+// Factor the n x n row-major FP32 W in place without pivoting (for a matrix
+// made safe for it by random butterflies, rbt.hip); *mw.zero gets the first
+// exact zero pivot column.
+hipError_t getrf_rm_genp(hipStream_t s, const MixWork &mw, float *W, long ldw, int n,
+                         int prec, int nb);
+
+// ---- random butterfly transforms (rbt.hip) ----------------------------------
+// Two depth-2 recursive butterflies U, V of order n (n % 4 == 0), each as its
+// two level arrays of n doubles (1/sqrt(2) folded in).
+struct Rbt {
+    const double *u1 = nullptr, *u2 = nullptr, *v1 = nullptr, *v2 = nullptr;
+};
+bool rbt_supported(int n);
+// n random butterfly values exp(r / 10) / sqrt(2), r uniform in [-1/2, 1/2)
+void rbt_values(int n, unsigned long long seed, double *out);
+// W = float(U^T D_r A D_c V) row-major (dr, dc optional); *flag |= 1 on overflow
+hipError_t rbt_conv(hipStream_t s, const double *A, long lda, int n, const double *dr,
+                    const double *dc, const Rbt &rb, float *W, long ldw, int *flag);
+// y = U^T D_r v: to y32 scaled by 2^-e of *nrm (when nrm), else to y64
+hipError_t rbt_in(hipStream_t s, const double *v, const double *dr,
+                  const unsigned long long *nrm, const Rbt &rb, float *y32, double *y64, int n);
+// out = (add ? out : 0) + D_c V y (y32 scaled back by 2^e of *nrm, or y64);
+// ||out||_inf -> onrm when given
+hipError_t rbt_out(hipStream_t s, const float *y32, const double *y64, const double *dc,
+                   const unsigned long long *nrm, const Rbt &rb, double *out, int add, int n,
+                   unsigned long long *onrm);
 
 // ---- refinement (ir.hip) -------------------------------------------------------
-// The FP32 factor as a preconditioner: M^{-1} v = D_c U^{-1} L^{-1} P D_r v.
+// The FP32 factor as a preconditioner: M^{-1} v = D_c U^{-1} L^{-1} P D_r v,
+// or with random butterflies M^{-1} v = D_c V U^{-1} L^{-1} U^T D_r v.
 struct PrecFactor {
     int n = 0;
     const float *W = nullptr;  // row-major L\U
     long ldw = 0;
     const int *perm = nullptr;
     const double *dr = nullptr, *dc = nullptr;   // optional power-of-two scales
+    bool rbt = false;          // factor of U^T D_r A D_c V (rb): no perm
+    Rbt rb;
     float *y32 = nullptr;      // n
     double *y64 = nullptr;     // n (FP64 application)
     uint64_t *xt = nullptr;    // trsv_words(n) tagged words, zero-initialised once

@@ -22,11 +22,6 @@
 
 namespace flx {
 
-// max of non-negative doubles through their bit patterns
-FLX_DEV void atomic_max_nonneg(unsigned long long *slot, double v)
-{
-    if (v > 0.0 || v != v) atomicMax(slot, (unsigned long long)__double_as_longlong(v));
-}
 
 // ---------------------------------------------------------------------------
 // Row reductions of a column-major FP64 matrix, phase 1: block (bx, by)
@@ -104,14 +99,6 @@ __global__ void rowsum_fin_kernel(const double *__restrict__ part, int S, int m,
     if ((threadIdx.x & 63) == 0) atomic_max_nonneg(nrm, acc);
 }
 
-// power of two 2^-e with v = f 2^e, f in [0.5, 1): scales v into [0.5, 1)
-FLX_DEV double pow2_recip(double v)
-{
-    if (!(v > 0.0) || !(v <= DBL_MAX)) return 1.0;
-    int e;
-    frexp(v, &e);
-    return ldexp(1.0, -e);
-}
 
 // phase 2 for mode 2: dr_i = 2^-e of the row maximum
 __global__ void rowmax_fin_kernel(const double *__restrict__ part, int S, int m,
@@ -474,10 +461,15 @@ hipError_t prec_apply(hipStream_t s, const PrecFactor &f, const double *v, doubl
 {
     const int n = f.n;
     FLX_CHECK(vmax(s, v, n, vnrm));
-    hipLaunchKernelGGL(prec_in_kernel, dim3((n + 255) / 256), dim3(256), 0, s, v,
-                       (const int *)f.perm, f.dr, (const unsigned long long *)vnrm, f.y32, n);
+    if (f.rbt)
+        FLX_CHECK(rbt_in(s, v, f.dr, vnrm, f.rb, f.y32, nullptr, n));
+    else
+        hipLaunchKernelGGL(prec_in_kernel, dim3((n + 255) / 256), dim3(256), 0, s, v,
+                           (const int *)f.perm, f.dr, (const unsigned long long *)vnrm, f.y32, n);
     FLX_CHECK(trsv_rm<float>(s, 0, f.W, f.ldw, n, f.y32, f.xt, next_epoch(s, f), f.err));
     FLX_CHECK(trsv_rm<float>(s, 1, f.W, f.ldw, n, f.y32, f.xt, next_epoch(s, f), f.err));
+    if (f.rbt)
+        return rbt_out(s, f.y32, nullptr, f.dc, vnrm, f.rb, z, add, n, znrm);
     hipLaunchKernelGGL(prec_out_kernel, dim3((n + 255) / 256), dim3(256), 0, s,
                        (const float *)f.y32, f.dc, (const unsigned long long *)vnrm, z, add, n,
                        znrm);
@@ -507,10 +499,15 @@ __global__ void prec_out64_kernel(const double *__restrict__ y, const double *__
 hipError_t prec_apply64(hipStream_t s, const PrecFactor &f, const double *v, double *z, int add)
 {
     const int n = f.n;
-    hipLaunchKernelGGL(prec_in64_kernel, dim3((n + 255) / 256), dim3(256), 0, s, v,
-                       (const int *)f.perm, f.dr, f.y64, n);
+    if (f.rbt)
+        FLX_CHECK(rbt_in(s, v, f.dr, nullptr, f.rb, nullptr, f.y64, n));
+    else
+        hipLaunchKernelGGL(prec_in64_kernel, dim3((n + 255) / 256), dim3(256), 0, s, v,
+                           (const int *)f.perm, f.dr, f.y64, n);
     FLX_CHECK(trsv_rm<double>(s, 0, f.W, f.ldw, n, f.y64, f.xt, next_epoch(s, f), f.err));
     FLX_CHECK(trsv_rm<double>(s, 1, f.W, f.ldw, n, f.y64, f.xt, next_epoch(s, f), f.err));
+    if (f.rbt)
+        return rbt_out(s, nullptr, f.y64, f.dc, nullptr, f.rb, z, add, n, nullptr);
     hipLaunchKernelGGL(prec_out64_kernel, dim3((n + 255) / 256), dim3(256), 0, s,
                        (const double *)f.y64, f.dc, z, add, n);
     return hipGetLastError();

@@ -976,6 +976,153 @@ hipError_t getrf_rm_mixed(hipStream_t s, const MixWork &mw, float *W, long ldw, 
 }
 
 // ---------------------------------------------------------------------------
+// LU without pivoting (for U^T A V with random butterflies, rbt.hip).
+// Panel of w <= 32 columns over the rows [j, n): every workgroup factors the
+// w x w diagonal block in LDS (one wave, a row per lane), then solves its
+// rows below, x U11 = a, one row per thread with the row in registers;
+// workgroup 0 writes the block back.  No exchange between workgroups.
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void genp_panel_kernel(float *__restrict__ P, long ldw, int mp,
+                                                         int w, int *__restrict__ zero, int base)
+{
+    __shared__ float s_d[32][33];
+    __shared__ float s_r[32];
+    const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
+    const int wd = w < mp ? w : mp;              // order of the diagonal block
+    for (int e = t; e < 32 * 32; e += 256) {
+        const int r = e >> 5, c = e & 31;
+        s_d[r][c] = (r < wd && c < w) ? P[(long)r * ldw + c] : (r == c ? 1.0f : 0.0f);
+    }
+    __syncthreads();
+    if (wave == 0) {
+        // right-looking elimination; lane r keeps row r of the block
+        for (int k = 0; k < wd; ++k) {
+            const float piv = s_d[k][k];
+            if (lane > k && lane < wd) {
+                const float l = s_d[lane][k] / piv;
+                s_d[lane][k] = l;
+                for (int c = k + 1; c < w; ++c) s_d[lane][c] = fmaf(-l, s_d[k][c], s_d[lane][c]);
+            }
+            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+            __builtin_amdgcn_wave_barrier();
+            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+        }
+        if (lane < 32) s_r[lane] = 1.0f / s_d[lane][lane];
+        if (blockIdx.x == 0) {
+            const unsigned long long zb = __ballot(lane < wd && s_d[lane][lane] == 0.0f);
+            if (lane == 0 && zb) atomicMin(zero, base + (int)__ffsll(zb) - 1);
+        }
+    }
+    __syncthreads();
+    const int i = wd + blockIdx.x * 256 + t;     // a row below the block
+    if (i < mp) {
+        float x[32];
+        float *row = P + (long)i * ldw;
+        if (w == 32) {
+#pragma unroll
+            for (int c = 0; c < 32; c += 4) {
+                const float4_t v = *(const float4_t *)(row + c);
+                x[c] = v[0]; x[c + 1] = v[1]; x[c + 2] = v[2]; x[c + 3] = v[3];
+            }
+        } else {
+#pragma unroll
+            for (int c = 0; c < 32; ++c) x[c] = c < w ? row[c] : 0.0f;
+        }
+        // x U11 = a: x_k = (a_k - sum_{c<k} x_c U_ck) / U_kk, right-looking
+#pragma unroll
+        for (int k = 0; k < 32; ++k) {
+            const float xk = x[k] * s_r[k];
+            x[k] = xk;
+#pragma unroll
+            for (int c = k + 1; c < 32; ++c) x[c] = fmaf(-xk, s_d[k][c], x[c]);
+        }
+        if (w == 32) {
+#pragma unroll
+            for (int c = 0; c < 32; c += 4) {
+                float4_t v;
+                v[0] = x[c]; v[1] = x[c + 1]; v[2] = x[c + 2]; v[3] = x[c + 3];
+                *(float4_t *)(row + c) = v;
+            }
+        } else {
+#pragma unroll
+            for (int c = 0; c < 32; ++c)
+                if (c < w) row[c] = x[c];
+        }
+    }
+    if (blockIdx.x == 0)
+        for (int e = t; e < 32 * 32; e += 256) {
+            const int r = e >> 5, c = e & 31;
+            if (r < wd && c < w) P[(long)r * ldw + c] = s_d[r][c];
+        }
+}
+
+hipError_t getrf_rm_genp(hipStream_t s, const MixWork &mw, float *W, long ldw, int n,
+                         int prec, int nb)
+{
+    if (n <= 0) return hipSuccess;
+    constexpr int ib = 32;
+    if (nb < ib) nb = ib;
+    if (nb > 256) nb = 256;
+    nb = (nb / ib) * ib;
+    // lookahead: the next block's columns are updated (in FP32) first, the
+    // rest of the trailing update runs in prec on the side stream beside the
+    // next block's panels
+    const bool look = mw.side && mw.ev_a && mw.ev_r;
+    bool rest_pending = false;
+    for (int J = 0; J < n; J += nb) {
+        const int jb = (n - J < nb) ? n - J : nb;
+        for (int j = J; j < J + jb; j += ib) {
+            const int w = (J + jb - j < ib) ? J + jb - j : ib;
+            const int mp = n - j;
+            const int wd = w < mp ? w : mp;
+            {
+                ProfScope ps(mw.prof, s, PROF_PANEL, 0.0, 8.0 * mp * w);
+                const int rows = mp - wd;
+                hipLaunchKernelGGL(genp_panel_kernel, dim3(rows > 0 ? (rows + 255) / 256 : 1),
+                                   dim3(256), 0, s, W + (long)j * ldw + j, ldw, mp, w, mw.zero,
+                                   j);
+                FLX_CHECK(hipGetLastError());
+            }
+            const int cr = J + jb - (j + w);
+            if (cr > 0) {
+                FLX_CHECK(trsm_rm_llu<float>(s, mw.prof, W + (long)j * ldw + j, ldw,
+                                             W + (long)j * ldw + j + w, ldw, w, cr));
+                FLX_CHECK(lp_update(s, mw, MIX_FP32, W, ldw, j + w, n - j - w, j + w, cr, j, w));
+            }
+        }
+        const int nr = n - (J + jb);
+        if (nr <= 0) continue;
+        if (!look) {
+            FLX_CHECK(trsm_rm_llu<float>(s, mw.prof, W + (long)J * ldw + J, ldw,
+                                         W + (long)J * ldw + J + jb, ldw, jb, nr));
+            FLX_CHECK(lp_update(s, mw, prec, W, ldw, J + jb, nr, J + jb, nr, J, jb));
+            continue;
+        }
+        // the previous block's rest update reaches the next block's columns
+        if (rest_pending) FLX_CHECK(hipStreamWaitEvent(s, mw.ev_r, 0));
+        rest_pending = false;
+        const int nl = nr < nb ? nr : nb;
+        FLX_CHECK(trsm_rm_llu<float>(s, mw.prof, W + (long)J * ldw + J, ldw,
+                                     W + (long)J * ldw + J + jb, ldw, jb, nl));
+        FLX_CHECK(lp_update(s, mw, MIX_FP32, W, ldw, J + jb, nr, J + jb, nl, J, jb));
+        const int c0 = J + jb + nl;
+        if (n > c0) {
+            FLX_CHECK(hipEventRecord(mw.ev_a, s));
+            FLX_CHECK(hipStreamWaitEvent(mw.side, mw.ev_a, 0));
+            FLX_CHECK(trsm_rm_llu<float>(mw.side, mw.prof, W + (long)J * ldw + J, ldw,
+                                         W + (long)J * ldw + c0, ldw, jb, n - c0,
+                                         mw.side_groups));
+            FLX_CHECK(lp_update(mw.side, mw, prec, W, ldw, J + jb, nr, c0, n - c0, J, jb,
+                                mw.side_groups));
+            FLX_CHECK(hipEventRecord(mw.ev_r, mw.side));
+            rest_pending = true;
+        }
+    }
+    if (rest_pending) FLX_CHECK(hipStreamWaitEvent(s, mw.ev_r, 0));
+    return hipSuccess;
+}
+
+// ---------------------------------------------------------------------------
 // FP64 column-major A -> FP32 row-major W, W[i][j] = A(i,j) * dr[i] * dc[j]
 // (scales optional).  flag[0] is set when a finite value exceeds FP32's
 // range (the overflow test of LAPACK's dlag2s).

@@ -80,8 +80,9 @@ struct Cand {
 FLX_DEV uint64_t pivot_key(double v, int pos, int j)
 {
     const double a = fabs(v);
-    if (a != a) return pos == j ? ~0ull : 0ull;
-    return (uint64_t)__double_as_longlong(a) + 1;
+    const uint64_t k = (uint64_t)__double_as_longlong(a) + 1;
+    const uint64_t nan_key = pos == j ? ~0ull : 0ull;
+    return __builtin_isnan(a) ? nan_key : k;
 }
 FLX_DEV bool cand_gt(uint64_t ka, int pa, uint64_t kb, int pb)
 {
@@ -198,7 +199,7 @@ __global__ __launch_bounds__(NT) void panel_kernel(
     // the header at EX_HDR
     __shared__ __align__(16) uint32_t s_words[2][LS * GMAX];
     __shared__ __align__(16) double s_u[NWAVE][2][W];   // per wave: shifted pivot rows
-    __shared__ __align__(16) double s_own[W + 1];        // the candidate, posted form + 1/pivot
+    __shared__ __align__(16) double s_own[W + 2];        // the candidate row, multiplier, 1/pivot
     __shared__ uint64_t s_wkey[NWAVE];
     __shared__ int s_wpos[NWAVE], s_wrid[NWAVE];
     __shared__ int s_piv[W];
@@ -251,6 +252,10 @@ __global__ __launch_bounds__(NT) void panel_kernel(
     }
     if (tid == 0) { s_fail = 0; s_cnt = 0; }
     int first_zero = 0;
+    // the rows' loads complete here, so the column loop's head waits for
+    // nothing (a vector-memory wait there would also wait for the stores
+    // of the previous column)
+    __builtin_amdgcn_s_waitcnt(0xf70);   // vmcnt(0)
     __syncthreads();
 
     // The pivot row of column jj (the winner of group wg, posted pending
@@ -325,24 +330,22 @@ __global__ __launch_bounds__(NT) void panel_kernel(
 #pragma unroll
                     for (int k = 0; k < RPT; ++k)
                         if (rid[k] == br) {
-                            dvecW t = row[k];
-                            if (j > 0) {
-#pragma unroll
-                                for (int e = 1; e < W - 1; ++e) t[e] = row[k][e + 1];
-                                t[W - 1] = lm[k];
-                            }
-                            *(dvecW *)s_own = t;
+                            *(dvecW *)s_own = row[k];
+                            s_own[W] = lm[k];
                         }
-                    s_own[W] = kinv;
+                    s_own[W + 1] = kinv;
                 }
                 __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
                 __builtin_amdgcn_wave_barrier();
                 __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-                // lane l: half l & 1 of element l >> 1; lanes 0, 1: the
-                // reciprocal; lane 2: the header
-                const uint64_t bits = (uint64_t)__double_as_longlong(s_own[lane >> 1]);
+                // lane l: half l & 1 of posted element e = l >> 1 (pending
+                // form: element 0, old elements 2..W-1, the multiplier);
+                // lanes 0, 1: the reciprocal; lane 2: the header
+                const int e = lane >> 1;
+                const int src = j == 0 ? e : (e == 0 ? 0 : (e < W - 1 ? e + 1 : W));
+                const uint64_t bits = (uint64_t)__double_as_longlong(s_own[src]);
                 const uint32_t half = (lane & 1) ? (uint32_t)(bits >> 32) : (uint32_t)bits;
-                const uint64_t ibits = (uint64_t)__double_as_longlong(s_own[W]);
+                const uint64_t ibits = (uint64_t)__double_as_longlong(s_own[W + 1]);
                 const uint32_t hdr = br >= 0 ? (((uint32_t)c.pos << 16) | (uint32_t)br) : EX_NONE;
                 const uint32_t extra = lane == 0 ? (uint32_t)ibits
                                      : lane == 1 ? (uint32_t)(ibits >> 32) : hdr;
@@ -404,19 +407,21 @@ __global__ __launch_bounds__(NT) void panel_kernel(
         }
         __syncthreads();
         // ---- 5. every wave picks the winner (lane q evaluates workgroup q) -
-        uint64_t qk = 0;
-        int qp = INT_MAX;
-        uint32_t qh = EX_NONE;
-        if (lane < G) {
-            qh = s_words[par][lane * LS + EX_HDR];
-            if (qh != EX_NONE) {
-                qp = (int)(qh >> 16);
-                qk = pivot_key(lds_f64(&s_words[par][lane * LS]), qp, j);
-            }
-        }
+        // (each lane also reads its group's reciprocal, element 1 and
+        // multiplier, so the winner's are a readlane away)
+        const int qg = lane < G ? lane : 0;
+        const uint32_t *qpost = &s_words[par][qg * LS];
+        const uint32_t qh = lane < G ? qpost[EX_HDR] : EX_NONE;
+        const double qv = lds_f64(qpost);
+        const double qinv = lds_f64(qpost + EX_INV);
+        const double qp1 = lds_f64(qpost + 2);
+        const double qlw = lds_f64(qpost + 2 * (W - 1));
+        const int qp = qh != EX_NONE ? (int)(qh >> 16) : INT_MAX;
+        const uint64_t qk = qh != EX_NONE ? pivot_key(qv, qp, j) : 0ull;
         const Cand wc = span_best<GSPAN>(qk, qp, lane);   // rid slot carries the group
         int wg = wc.rid;
-        const uint32_t wh = (uint32_t)__builtin_amdgcn_readlane((int)qh, wg >= 0 && wg < 64 ? wg : 0);
+        const int wl = wg >= 0 && wg < 64 ? wg : 0;
+        const uint32_t wh = (uint32_t)__builtin_amdgcn_readlane((int)qh, wl);
         int wpos = wc.pos;
         int wrid = (int)(wh & 0xffffu);
         // a winner outside the panel can only come from a failed wait: keep
@@ -432,11 +437,10 @@ __global__ __launch_bounds__(NT) void panel_kernel(
         wg_prev = wg;
         // the pivot, its reciprocal and the winner's element 1 (finished
         // with the previous pivot row as the owner would), as scalars
-        const uint32_t *post = &s_words[par][wg * LS];
-        const double pv = uniform_f64(lds_f64(post));
-        const double inv = uniform_f64(lds_f64(post + EX_INV));
-        const double p1 = uniform_f64(lds_f64(post + 2));
-        const double lw = uniform_f64(lds_f64(post + 2 * (W - 1)));
+        const double pv = read_lane_f64(qv, wg);
+        const double inv = read_lane_f64(qinv, wg);
+        const double p1 = read_lane_f64(qp1, wg);
+        const double lw = read_lane_f64(qlw, wg);
         const double up1 = uniform_f64(s_u[wave][par ^ 1][1]);
         const double u0 = 1 < live ? (j > 0 ? fma(-lw, up1, p1) : p1) : 0.0;
 

@@ -30,6 +30,7 @@ constexpr int SLOT_ANRM = 0, SLOT_VN = 1, SLOT_T = 2, SLOT_COL = 5;   // scalar 
 
 struct Opts {
     int prec = MIX_FP32, refine = FLX_REFINE_CLASSIC, itermax = 30, gmres_m = 40;
+    int pivot = FLX_PIVOT_TOURNAMENT;
     double tol = 0.0, gmres_tol = 1e-10;
     int fallback = 1, nb = 256, graph = 1;
     bool dsgesv = false;
@@ -108,6 +109,18 @@ struct Solver {
         FLX_TRY(ensure(h, m.err, sizeof(int) * 4));
         FLX_TRY(ensure(h, m.dr, sizeof(double) * n));
         FLX_TRY(ensure(h, m.dc, sizeof(double) * n));
+        const bool rbt = o.pivot == FLX_PIVOT_RBT && rbt_supported(n);
+        if (rbt && m.rbt_n != n) {
+            // the butterflies of order n, fixed seeds (the same for every call)
+            FLX_TRY(ensure(h, m.rbt, sizeof(double) * 4 * (size_t)n));
+            std::vector<double> r(4 * (size_t)n);
+            for (int b = 0; b < 4; ++b) rbt_values(n, 0x5eed0000ull + (unsigned long long)b,
+                                                   r.data() + (size_t)b * n);
+            FLX_TRY(hipMemcpyAsync(m.rbt.p, r.data(), sizeof(double) * r.size(),
+                                   hipMemcpyHostToDevice, s));
+            FLX_TRY(hipStreamSynchronize(s));
+            m.rbt_n = n;
+        }
         FLX_TRY(ensure(h, m.part, sizeof(double) * ir_part_doubles(n)));
         const int nslots = SLOT_COL + 3 * nrhs;
         FLX_TRY(ensure(h, m.scal, sizeof(unsigned long long) * nslots));
@@ -150,6 +163,14 @@ struct Solver {
         f.perm = (const int *)m.perm.p;
         f.dr = (o.prec == MIX_FP16) ? (const double *)m.dr.p : nullptr;
         f.dc = (o.prec == MIX_FP16) ? (const double *)m.dc.p : nullptr;
+        f.rbt = rbt;
+        if (rbt) {
+            const double *r = (const double *)m.rbt.p;
+            f.rb.u1 = r;
+            f.rb.u2 = r + n;
+            f.rb.v1 = r + 2 * (size_t)n;
+            f.rb.v2 = r + 3 * (size_t)n;
+        }
         f.y32 = (float *)m.y32.p;
         f.y64 = (double *)m.y64.p;
         f.xt = (uint64_t *)m.flags.p;
@@ -161,6 +182,15 @@ struct Solver {
     double *R(int c) { return (double *)h->mix.R.p + (size_t)c * n; }
     const double *Bc(int c) { return B + (size_t)c * ldb; }
     double *Xc(int c) { return X + (size_t)c * ldx; }
+
+    // the factorisation's launches: tournament-pivoted, or without pivoting
+    // for the butterfly-transformed matrix
+    hipError_t run_factor(hipStream_t st)
+    {
+        MixBufs &m = h->mix;
+        if (f.rbt) return getrf_rm_genp(st, mw, (float *)m.W32.p, ldw, n, o.prec, o.nb);
+        return getrf_rm_mixed(st, mw, (float *)m.W32.p, ldw, n, o.prec, o.nb);
+    }
 
     // The factorisation's launches depend only on (n, prec, nb) and the
     // workspace, so they are captured once into a graph and replayed, on a
@@ -174,6 +204,7 @@ struct Solver {
         // the side stream's workgroup cap and every workspace address
         const size_t key[MIX_GKEY] = {
             (size_t)n, (size_t)ldw, (size_t)o.prec, (size_t)o.nb, (size_t)mw.side_groups,
+            (size_t)f.rbt,
             (size_t)m.W32.p, (size_t)mw.perm, (size_t)mw.cand, (size_t)mw.ccount,
             (size_t)mw.counters, (size_t)mw.u11, (size_t)mw.win, (size_t)mw.list,
             (size_t)mw.zero, (size_t)mw.lpmax, (size_t)mw.la, (size_t)mw.ub, (size_t)mw.inv,
@@ -191,7 +222,7 @@ struct Solver {
             }
             hipGraph_t g = nullptr;
             FLX_TRY(hipStreamBeginCapture(m.gstream, hipStreamCaptureModeThreadLocal));
-            hipError_t e = getrf_rm_mixed(m.gstream, mw, (float *)m.W32.p, ldw, n, o.prec, o.nb);
+            hipError_t e = run_factor(m.gstream);
             hipGraph_t gg = nullptr;
             hipError_t e2 = hipStreamEndCapture(m.gstream, &gg);
             if (e != hipSuccess) { if (gg) (void)hipGraphDestroy(gg); return hip_status(e); }
@@ -222,11 +253,16 @@ struct Solver {
         if (o.prec == MIX_FP16)
             FLX_TRY(equilibrate(s, A, lda, n, (double *)m.part.p, (double *)m.dr.p,
                                 (double *)m.dc.p));
-        FLX_TRY(conv_d2s_rm(s, A, lda, n, f.dr, f.dc, (float *)m.W32.p, ldw, (int *)m.err.p + 1));
+        if (f.rbt)
+            FLX_TRY(rbt_conv(s, A, lda, n, f.dr, f.dc, f.rb, (float *)m.W32.p, ldw,
+                             (int *)m.err.p + 1));
+        else
+            FLX_TRY(conv_d2s_rm(s, A, lda, n, f.dr, f.dc, (float *)m.W32.p, ldw,
+                                (int *)m.err.p + 1));
         if (o.graph && !h->prof) {
             FLX_RET(factor_graph());
         } else {
-            FLX_TRY(getrf_rm_mixed(s, mw, (float *)m.W32.p, ldw, n, o.prec, o.nb));
+            FLX_TRY(run_factor(s));
         }
         int flags[4] = {0, 0, 0, 0}, zero = INT_MAX;
         FLX_TRY(hipMemcpyAsync(flags, m.err.p, sizeof flags, hipMemcpyDeviceToHost, s));
@@ -258,9 +294,13 @@ struct Solver {
         for (int c = 0; c < nrhs; ++c) {
             const double rn = hcol(c, 0), xn = hcol(c, 1), bn = hcol(c, 2);
             const double denom = anrm * xn + bn;
-            const double eta = denom > 0.0 ? rn / denom : (rn > 0.0 ? INFINITY : 0.0);
+            // a NaN anywhere (residual, iterate, right-hand side) is never
+            // convergence: NaN / NaN would otherwise read as 0 below
+            const bool nan = std::isnan(rn) || std::isnan(xn) || std::isnan(denom);
+            const double eta = nan ? NAN
+                             : denom > 0.0 ? rn / denom : (rn > 0.0 ? INFINITY : 0.0);
             em = std::isnan(eta) || std::isnan(em) ? NAN : std::max(em, eta);
-            const bool ok = o.dsgesv ? !(rn > xn * cte) && !std::isnan(rn) : eta <= o.tol;
+            const bool ok = o.dsgesv ? !(rn > xn * cte) && !nan : eta <= o.tol;
             all = all && ok;
         }
         *conv = all;
@@ -568,6 +608,7 @@ extern "C" int flx_dgesv_mixed(flx_handle h, int n, int nrhs, double *A, int lda
         o.fallback = opt->fallback;
         o.nb = opt->nb > 0 ? opt->nb : 256;
         o.graph = opt->graph;
+        o.pivot = opt->pivot;
     }
     if (st) memset(st, 0, sizeof *st);
     if (n == 0 || nrhs == 0) return FLX_SUCCESS;

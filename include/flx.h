@@ -190,6 +190,13 @@ enum flx_mixed_prec { FLX_MIXED_FP32 = 0, FLX_MIXED_BF16 = 1, FLX_MIXED_FP16 = 2
  * factor's unit roundoff, kappa u < 2 -> classic (at least 60 steps),
  * < 100 -> GMRES-IR, else the FP64 LU at once (stats.iter = -4). */
 enum flx_refine { FLX_REFINE_CLASSIC = 0, FLX_REFINE_GMRES = 1, FLX_REFINE_AUTO = 2 };
+/* Pivoting of the low-precision factor.  TOURNAMENT: communication-avoiding
+ * partial pivoting (CALU) of each 32-column panel.  RBT: the factor of
+ * U^T A V for two random recursive butterflies U, V of depth 2 (fixed seeds),
+ * without pivoting -- panels with no pivot search, the preconditioner
+ * applied as V (LU)^{-1} U^T; needs n % 4 == 0, else TOURNAMENT is used.
+ * Either way refinement works on A itself and the FP64 LU is the fallback. */
+enum flx_pivot { FLX_PIVOT_TOURNAMENT = 0, FLX_PIVOT_RBT = 1 };
 typedef struct flx_mixed_opts {
     int prec;          /* enum flx_mixed_prec (FP16 adds power-of-two row and
                           column equilibration and per-step operand scaling) */
@@ -203,6 +210,7 @@ typedef struct flx_mixed_opts {
     int nb;            /* outer block of the factorisation (0: 256) */
     int graph;         /* 1: replay the factorisation's launches from a
                           captured HIP graph (per shape), 0: launch eagerly */
+    int pivot;         /* enum flx_pivot (0: tournament) */
 } flx_mixed_opts;
 typedef struct flx_mixed_stats {
     int iter;              /* refinement steps; < 0: failed (dsgesv ITER codes) */

@@ -234,3 +234,37 @@ def test_mixed_n16384_backward_error(gpu):
     assert abs(st["backward_error"] - eta) <= 0.5 * eta + np.sqrt(n) * EPS, (eta, st)
     fwd = ((X.storage - Xt.storage).abs().max() / Xt.storage.abs().max()).item()
     assert fwd <= 4.0 * kappa * n * eta + 100 * EPS, (fwd, st)
+
+
+@pytest.mark.parametrize("look", [1, 0])
+@pytest.mark.parametrize("n", [4, 30, 100, 256, 300, 1200, 2048])
+def test_butterfly_factor_refines_to_fp64_level(gpu, n, look):
+    # pivot="rbt": the FP32 factor of U^T A V (random depth-2 butterflies),
+    # without pivoting; the refinement still works on A.  n = 30 is not a
+    # multiple of 4, so the tournament factor is used instead.
+    a = T.svd_matrix(n, 1e2, 50 + n)
+    b, xt = T.rhs_from_solution(a, 2, n)
+    h = gpu.default_handle()
+    h.set_lu_lookahead(look)
+    try:
+        for refine in ("classic", "gmres"):
+            x, st, a_after = run_mixed(gpu, a, b, prec="fp32", refine=refine, pivot="rbt")
+            assert st["fallback"] == 0 and st["iter"] >= 0, (refine, st)
+            assert backward_error(a, x, b) <= n * EPS, (refine, st)
+            assert np.array_equal(a_after, a)
+            assert fwd_err(x, xt) <= 4.0 * 1e2 * backward_error(a, x, b) + 100 * EPS
+    finally:
+        h.set_lu_lookahead(1)
+
+
+def test_nan_right_hand_side_is_not_convergence(gpu):
+    # a NaN in B makes every iterate and residual NaN: no refinement may
+    # report convergence (NaN / NaN must not read as a zero backward error)
+    n = 200
+    a = T.svd_matrix(n, 1e2, 77)
+    b, _ = T.rhs_from_solution(a, 1, 77)
+    b[5, 0] = np.nan
+    for refine in ("classic", "gmres"):
+        x, st, _ = run_mixed(gpu, a, b, prec="fp32", refine=refine, itermax=3)
+        assert st["iter"] < 0 and st["fallback"] == 1, (refine, st)
+        assert np.isnan(st["backward_error"]), (refine, st)
