@@ -325,3 +325,36 @@ def test_lu_n20000_device_and_host_seam(gpu):
     assert inf.value == 0
     assert np.array_equal(ip, ipiv[:n])
     assert np.array_equal(a_host, A.storage.cpu().numpy().T)
+
+
+def test_two_handles_factor_concurrently(gpu):
+    """Two factorisations in flight at once on two handles and streams.  A
+    panel's cooperating workgroups wait on one another (bounded waits), so
+    two launches sharing the device must both complete without a failed
+    wait (flx_sync reports one as an error) and give the pivots and
+    backward error of a lone run."""
+    import torch
+    n = 3000
+    a = [O.uniform(n, n, 201), O.uniform(n, n, 202)]
+    streams = [torch.cuda.Stream(), torch.cuda.Stream()]
+    handles = [gpu.Handle(s) for s in streams]
+    pristine = [gpu.DeviceMatrix.from_numpy(x) for x in a]
+    work = [gpu.DeviceMatrix.from_numpy(x) for x in a]
+    torch.cuda.synchronize()
+    for rep in range(3):
+        out = []
+        for k in range(2):
+            with torch.cuda.stream(streams[k]):
+                work[k].storage.copy_(pristine[k].storage)
+            out.append(gpu.lapack.trf(work[k], handle=handles[k]))
+        for h in handles:
+            h.sync()
+        torch.cuda.synchronize()
+        for k in range(2):
+            piv = out[k][0].cpu().numpy()[:n]
+            assert int(out[k][1].cpu().numpy()[0]) == 0
+            ref = a[k].copy(order="F")
+            rpiv, rinfo, gap = O.dgetrf_gaps(ref)
+            ok, j = first_mismatch_ok(piv, rpiv, gap)
+            assert ok, (rep, k, j)
+            assert O.dget01(a[k], work[k].numpy(), piv) < 30, (rep, k)
