@@ -275,7 +275,7 @@ int dgetrf_dev(flx_context *h, int m, int n, double *A, int lda, int *ipiv, int 
     FLX_TRY(ensure(h, h->lu_mv, sizeof(int) * (1 + 2 * LU_BLOCK_MOVES)));
     h->wk.inv = (int *)h->lu_inv.p;
     h->wk.mv = (int *)h->lu_mv.p;
-    FLX_TRY(ensure(h, h->lu_stage, sizeof(double) * LU_STAGE_ROWS * LU_STAGE_LD));
+    FLX_TRY(ensure(h, h->lu_stage, sizeof(double) * LU_STAGE_DOUBLES));
     FLX_TRY(ensure(h, h->lu_sdst, sizeof(int) * (LU_STAGE_TOP + 1)));
     FLX_TRY(ensure(h, h->lu_sperm, sizeof(int) * LU_STAGE_TOP));
     h->wk.stage = (double *)h->lu_stage.p;

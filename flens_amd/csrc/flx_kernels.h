@@ -90,6 +90,9 @@ struct LuWork {
 constexpr int LU_BLOCK_MOVES = 2 * 256;   // rows one outer block can move
 constexpr int LU_STAGE_TOP = 128;         // stage rows: list sources, then the top rows
 constexpr int LU_STAGE_ROWS = LU_STAGE_TOP + 32, LU_STAGE_LD = 256;
+// after the stage rows: inv(L11) of the panel step (32 x 32) and its flag
+constexpr int LU_STAGE_INV = LU_STAGE_ROWS * LU_STAGE_LD;
+constexpr int LU_STAGE_DOUBLES = LU_STAGE_INV + 32 * 32 + 1;
 
 struct LuOptions {
     int nb = 256;   // outer block (trailing update K)

@@ -137,14 +137,19 @@ __global__ __launch_bounds__(256) void perm_compose_kernel(const int *__restrict
 // Fused panel step of the blocked LU (lookahead path), after the
 // cooperative panel kernel has factored and interchanged the panel's own
 // columns j .. j+w of the outer block [J, J+jb):
-//   lu_stage_kernel (one workgroup): the interchange list's source rows
-//     (block columns) into stage rows 0.., the panel's top w rows as they
-//     will be after the interchange into stage rows LU_STAGE_TOP.., the
-//     moved rows' perm entries and destinations;
+//   lu_stage_kernel (one workgroup per stage row, one more): the
+//     interchange list's source rows (block columns) into stage rows 0..,
+//     the panel's top w rows as they will be after the interchange into
+//     stage rows LU_STAGE_TOP.., the moved rows' perm entries and
+//     destinations; the extra workgroup inverts the unit-lower L11 (when its
+//     entries are finite) into the stage's LU_STAGE_INV part;
 //   lu_apply_kernel (32 rows per workgroup): the interchange of the block's
 //     other columns, U12 = L11^-1 (top rows' right part) -- laswp, the
 //     blas::sm of trf.tcc:136-138 -- and A22 -= L21 U12 (trf.tcc:140-145)
-//     for the block's columns right of the panel, in one launch.
+//     for the block's columns right of the panel, in one launch.  U12 is the
+//     product of the inverse's lower triangle with the top rows (every
+//     workgroup, no dependent chain), or, when L11 holds a non-finite
+//     entry, the substitution itself (the reference's NaN / Inf pattern).
 // A workgroup writes only its own rows and reads other rows only from the
 // stage (or the panel's columns, which it does not write), so workgroups
 // need no ordering among themselves.
@@ -216,6 +221,37 @@ __global__ __launch_bounds__(256) void lu_stage_kernel(const double *__restrict_
     __shared__ int s_dst[RP_MAXE], s_src[RP_MAXE], s_cnt[2 * PANEL_GROUPS_MAX + 1];
     __shared__ int s_row;
     const int t = threadIdx.x, b = blockIdx.x;
+    if (b == LU_STAGE_ROWS) {
+        // inv(L11) of the w x w unit-lower block at W[j.., j..] (rows and
+        // columns beyond w: identity), row-major at stage + LU_STAGE_INV,
+        // then a flag: 1 when every strict-lower entry is finite
+        __shared__ double s_l[32][33];
+        __shared__ int s_bad;
+        if (t == 0) s_bad = 0;
+        __syncthreads();
+        for (int e = t; e < 32 * 32; e += 256) {
+            const int r = e >> 5, c = e & 31;
+            const double v = (r < w && c < r) ? W[(long)(j + r) * ldw + j + c] : 0.0;
+            if (!__builtin_isfinite(v)) s_bad = 1;
+            s_l[r][c] = v;
+        }
+        __syncthreads();
+        double *inv = stage + LU_STAGE_INV;
+        if (t < 32) {
+            // column t of the inverse: x = e_t, x_i -= L(i, k) x_k (k < i)
+            double x[32];
+#pragma unroll
+            for (int i = 0; i < 32; ++i) x[i] = i == t ? 1.0 : 0.0;
+#pragma unroll
+            for (int k = 0; k < 31; ++k)
+#pragma unroll
+                for (int i = k + 1; i < 32; ++i) x[i] = fma(-s_l[i][k], x[k], x[i]);
+#pragma unroll
+            for (int i = 0; i < 32; ++i) inv[i * 32 + t] = x[i];
+        }
+        if (t == 0) inv[32 * 32] = s_bad ? 0.0 : 1.0;
+        return;
+    }
     const int cnt = load_list_par(list, G, m, s_dst, s_src, s_cnt);
     if (b == 0) {
         if (t < cnt) {
@@ -310,10 +346,33 @@ __global__ __launch_bounds__(256) void lu_apply_kernel(double *__restrict__ W, l
             s_a[e / LA_CR][e % LA_CR] = av[q];
         }
     }
-    // U12 by forward substitution with the unit-lower L11, one column per
-    // thread; x rotates (register 0 is the finished entry of the step), so
-    // the rolled loop indexes registers by constants only
-    if (cr > 0 && t < cr) {
+    const bool useinv = stage[LU_STAGE_INV + 32 * 32] != 0.0;
+    if (useinv && cr > 0) {
+        // U12 = inv(L11) A12 over the inverse's lower triangle only (row r
+        // takes the top rows k <= r, as the substitution does), every entry
+        // independent; the inverse replaces L11 in s_l (row-major)
+        __syncthreads();
+        for (int e = t; e < 32 * 32; e += 256) s_l[e >> 5][e & 31] = stage[LU_STAGE_INV + e];
+        __syncthreads();
+        if (t < cr) {
+            // one column per thread, the top rows' entries in registers
+            double x[32];
+#pragma unroll
+            for (int k = 0; k < 32; ++k)
+                x[k] = (k < w) ? stage[(long)(LU_STAGE_TOP + k) * LU_STAGE_LD + jo + w + t] : 0.0;
+#pragma unroll
+            for (int r = 0; r < 32; ++r) {
+                double v = x[r];
+#pragma unroll
+                for (int k = 0; k < r; ++k) v = fma(s_l[r][k], x[k], v);
+                s12[r][t] = v;
+            }
+        }
+    }
+    // else U12 by forward substitution with the unit-lower L11, one column
+    // per thread; x rotates (register 0 is the finished entry of the step),
+    // so the rolled loop indexes registers by constants only
+    if (!useinv && cr > 0 && t < cr) {
         double x[32];
 #pragma unroll
         for (int k = 0; k < 32; ++k)
@@ -712,7 +771,7 @@ hipError_t getrf_rm_f64(hipStream_t s, LuWork &wk, double *W, long ldw, int m, i
                 // interchange + U12 + in-block update together
                 {
                     ProfScope ps(wk.prof, s, PROF_ROWPERM, 0.0, 16.0 * 64 * jb);
-                    hipLaunchKernelGGL(lu_stage_kernel, dim3(LU_STAGE_ROWS), dim3(256), 0, s,
+                    hipLaunchKernelGGL(lu_stage_kernel, dim3(LU_STAGE_ROWS + 1), dim3(256), 0, s,
                                        (const double *)W, ldw, m, J, jb, j, w,
                                        (const int *)wk.list, panel_groups(m - j),
                                        (const int *)wk.perm, wk.stage, wk.sdst, wk.sperm,
