@@ -535,11 +535,11 @@ def selector_section(train_batches: int = 2, train_count: int = 12500, eval_coun
 # FETCH_SIZE, written = 1.0 x WRITE_SIZE): (2 x 88665.4 + 87862.1) KiB.
 LU8192_GEMM_TRAFFIC_BYTES = (2.0 * 88665.4 + 87862.1) * 1024.0
 # HBM bytes per panel launch in the default workload: rocprofv3 --pmc
-# FETCH_SIZE and WRITE_SIZE over 1792 launches, averaged (962.50 and 1109.02
+# FETCH_SIZE and WRITE_SIZE over 1792 launches, averaged (1732.55 and 2200.30
 # KiB), scaled by what the same counters read for the 1 GiB streaming copy of
-# the peak probe in the same run (fetched = 1.93635 x FETCH_SIZE, written =
-# 0.976493 x WRITE_SIZE).
-LU8192_PANEL_TRAFFIC_BYTES = (1.93635 * 962.50 + 0.976493 * 1109.02) * 1024.0
+# the peak probe in the same run (fetched = 1.93596 x FETCH_SIZE, written =
+# 0.976495 x WRITE_SIZE).
+LU8192_PANEL_TRAFFIC_BYTES = (1.93596 * 1732.55 + 0.976495 * 2200.30) * 1024.0
 
 
 def main():
