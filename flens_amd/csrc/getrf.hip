@@ -507,6 +507,104 @@ __global__ __launch_bounds__(256) void trsm_rm_llu_kernel(const T *__restrict__ 
     }
 }
 
+// For k <= TR_FULLK the lower part of L is staged in LDS once, beside the
+// chunk's X, so the dependent steps of the substitution wait on LDS only.
+// Layout: 16-row blocks R, block R holding its rows' first 16 (R + 1)
+// columns (through its diagonal block) from offset 256 R (R + 1) / 2, with
+// zeros on and above the diagonal and outside k x k -- every read below is
+// in range and branch-free, and a zero stands where trsm_rm_llu_kernel's
+// strip holds one, so the arithmetic is that kernel's, operation for
+// operation.
+constexpr int TR_FULLK = 128;
+constexpr int TR_FULLB = TR_FULLK / TR_BS;
+constexpr int TR_LPAD = TR_BS * TR_BS * TR_FULLB * (TR_FULLB + 1) / 2;
+
+template <typename T>
+__global__ __launch_bounds__(256) void trsm_rm_llu_full_kernel(const T *__restrict__ L,
+                                                               long ldl, T *__restrict__ X,
+                                                               long ldx, int k, int ncols)
+{
+    __shared__ T xs[TR_FULLK][TR_CW + 1];
+    __shared__ T lp[TR_LPAD];
+    const int tid = threadIdx.x;
+    const int tc = tid % TR_CW, tg = tid / TR_CW;   // 32 columns x 8 row groups
+    // block R: 256 (R + 1) entries, R + 1 per thread, all loads issued first
+    StaticFor<0, TR_FULLB>::run([&](auto RC) {
+        constexpr int R = decltype(RC)::value;
+        constexpr int W = TR_BS * (R + 1), BASE = TR_BS * TR_BS * R * (R + 1) / 2;
+        if (R * TR_BS < k) {
+            T v[R + 1];
+#pragma unroll
+            for (int u = 0; u <= R; ++u) {
+                const int e = tid + 256 * u, r = R * TR_BS + e / W, c = e % W;
+                v[u] = (r < k && c < r) ? L[(long)r * ldl + c] : T(0);
+            }
+#pragma unroll
+            for (int u = 0; u <= R; ++u) lp[BASE + tid + 256 * u] = v[u];
+        }
+    });
+    // workgroup b solves the column chunks b, b + grid, ...
+    for (int c0 = blockIdx.x * TR_CW; c0 < ncols; c0 += gridDim.x * TR_CW) {
+        const int cw = min(TR_CW, ncols - c0);
+        if (c0 != (int)blockIdx.x * TR_CW) __syncthreads();
+        // rows tg, tg + 8, ... of the chunk, all loads issued first
+        {
+            T v[TR_FULLK / 8];
+#pragma unroll
+            for (int u = 0; u < TR_FULLK / 8; ++u) {
+                const int rr = tg + 8 * u;
+                v[u] = (rr < k && tc < cw) ? X[(long)rr * ldx + c0 + tc] : T(0);
+            }
+#pragma unroll
+            for (int u = 0; u < TR_FULLK / 8; ++u) xs[tg + 8 * u][tc] = v[u];
+        }
+        __syncthreads();
+        for (int kb = 0; kb < k; kb += TR_BS) {
+            const int bs = min(TR_BS, k - kb);
+            const int R = kb / TR_BS;
+            const T *lb = lp + TR_BS * TR_BS * R * (R + 1) / 2 + kb;   // block R, column kb
+            const int lw = TR_BS * (R + 1);
+            // diagonal block: column tc held in registers by threads tg == 0
+            if (tg == 0) {
+                T xb[TR_BS];
+#pragma unroll
+                for (int i = 0; i < TR_BS; ++i) {
+                    const T t = xs[kb + i][tc];
+                    xb[i] = (i < bs) ? t : T(0);
+                }
+#pragma unroll
+                for (int i = 1; i < TR_BS; ++i) {
+                    T v = xb[i];
+#pragma unroll
+                    for (int q = 0; q < i; ++q) v = fma(-lb[i * lw + q], xb[q], v);
+                    xb[i] = v;
+                }
+#pragma unroll
+                for (int i = 0; i < TR_BS; ++i) xs[kb + i][tc] = xb[i];
+            }
+            __syncthreads();
+            // fold the solved block into the rows below
+            T xb[TR_BS];
+#pragma unroll
+            for (int q = 0; q < TR_BS; ++q) xb[q] = xs[kb + q < k ? kb + q : kb][tc];
+            for (int rr = kb + bs + tg; rr < k; rr += 8) {
+                const int Rr = rr / TR_BS;
+                const T *lr = lp + TR_BS * TR_BS * Rr * (Rr + 1) / 2 +
+                              (rr % TR_BS) * TR_BS * (Rr + 1) + kb;
+                T v = xs[rr][tc];
+#pragma unroll
+                for (int q = 0; q < TR_BS; ++q) v = fma(-lr[q], xb[q], v);
+                xs[rr][tc] = v;
+            }
+            __syncthreads();
+        }
+        for (int q = tid; q < k * TR_CW; q += 256) {
+            int rr = q / TR_CW, c = q % TR_CW;
+            if (c < cw) X[(long)rr * ldx + c0 + c] = xs[rr][c];
+        }
+    }
+}
+
 // ---------------------------------------------------------------------------
 // out(cols_in x rows_in) = in(rows_in x cols_in)^T; both column-major views
 // (equivalently: column-major <-> row-major conversion).  32x32 LDS tiles.
@@ -588,8 +686,12 @@ hipError_t trsm_rm_llu(hipStream_t s, Profiler *prof, const T *L, long ldl, T *X
                  sizeof(T) * (2.0 * k * ncols + 0.5 * k * k));
     int groups = (ncols + TR_CW - 1) / TR_CW;
     if (max_groups > 0 && groups > max_groups) groups = max_groups;
-    hipLaunchKernelGGL(trsm_rm_llu_kernel<T>, dim3(groups), dim3(256), 0, s, L, ldl, X, ldx, k,
-                       ncols);
+    if (k <= TR_FULLK)
+        hipLaunchKernelGGL(trsm_rm_llu_full_kernel<T>, dim3(groups), dim3(256), 0, s, L, ldl, X,
+                           ldx, k, ncols);
+    else
+        hipLaunchKernelGGL(trsm_rm_llu_kernel<T>, dim3(groups), dim3(256), 0, s, L, ldl, X, ldx,
+                           k, ncols);
     return hipGetLastError();
 }
 template hipError_t trsm_rm_llu<double>(hipStream_t, Profiler *, const double *, long, double *,

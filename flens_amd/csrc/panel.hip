@@ -5,9 +5,10 @@
 // reciprocal of the pivot (division below the safe minimum), rank-1 update.
 //
 // One workgroup holds PANEL_ROWS_PER_WG consecutive panel rows in registers
-// (512 threads x 2 rows, fully unrolled column loop so every register index
+// (256 threads x 2 rows, fully unrolled column loop so every register index
 // is static); taller panels use up to PANEL_GROUPS_MAX cooperating
-// workgroups.  Per column:
+// workgroups.  Rows come in and go out through LDS, so every global access
+// of a wave covers whole rows.  Per column:
 //   1. workgroup-local pivot search (DPP wave reductions, one barrier),
 //      first max on ties;
 //   2. the wave that owns the workgroup's candidate posts it -- a header
@@ -187,6 +188,14 @@ __global__ __launch_bounds__(NT) void panel_kernel(
     constexpr int PER = (WORDS_MAX + NT - 1) / NT;
     constexpr int LS = EX_POST + 1;                // LDS stride of a post (8-byte aligned)
     constexpr int GSPAN = GMAX <= 4 ? 4 : GMAX <= 16 ? 16 : 64;
+    // row staging through LDS, IO_R rows at a time: in, rows of W doubles
+    // at stride IO_LD (16-byte aligned pairs), IO_PER pairs per thread per
+    // chunk; out, rows of 2 W doubles (the multipliers, then the registers
+    // from the row's start column) at the odd stride OUT_LD
+    constexpr int IO_R = 128, IO_LD = W + 2, OUT_LD = 2 * W + 1;
+    constexpr int IO_CHUNKS = RPT * NT / IO_R;
+    constexpr int IO_PER = IO_R * (W / 2) / NT;
+    static_assert(NT % IO_R == 0 && IO_R * (W / 2) % NT == 0, "row staging");
 
     const int G = gridDim.x;
     const int g = blockIdx.x;
@@ -205,6 +214,8 @@ __global__ __launch_bounds__(NT) void panel_kernel(
     __shared__ int s_piv[W];
     __shared__ int s_cnt;
     __shared__ int s_fail;
+    __shared__ __align__(16) double s_io[IO_R * OUT_LD];   // rows in / out, IO_R at a time
+    __shared__ int s_ipos[IO_R];
 
     critical_path_priority();
     // Thread tid owns original panel rows g*ROWS + tid + k*NT.  Rows never
@@ -238,25 +249,48 @@ __global__ __launch_bounds__(NT) void panel_kernel(
         done[k] = rid[k] >= mp;
         any[k] = __ballot(rid[k] < mp) != 0;   // the wave has a row in slot k
         lm[k] = 0.0;
-        row[k] = dvecW(0.0);
-        if (rid[k] < mp) {
-            const double *src = P + (long)rid[k] * ldw;
-            if (w == W) {
-                row[k] = *(const dvecW *)src;
-            } else {
-#pragma unroll
-                for (int c = 0; c < W; ++c)
-                    if (c < w) row[k][c] = src[c];
-            }
-        }
     }
     if (tid == 0) { s_fail = 0; s_cnt = 0; }
     int first_zero = 0;
-    // the rows' loads complete here, so the column loop's head waits for
-    // nothing (a vector-memory wait there would also wait for the stores
-    // of the previous column)
-    __builtin_amdgcn_s_waitcnt(0xf70);   // vmcnt(0)
-    __syncthreads();
+    // The rows arrive through LDS: sixteen lanes load one row as 16-byte
+    // pairs (a wave reads four whole rows per load, not 64 scattered ones),
+    // every chunk's loads issued at once; then each thread takes its row
+    // from LDS.
+    double2 v[IO_CHUNKS][IO_PER];
+#pragma unroll
+    for (int q = 0; q < IO_CHUNKS; ++q) {
+        const int k = q / (NT / IO_R), h = q % (NT / IO_R);
+        const int rbase = g * ROWS + k * NT + h * IO_R;
+#pragma unroll
+        for (int i = 0; i < IO_PER; ++i) {
+            const int p = tid + NT * i, r = p >> 4, c2 = (p & 15) * 2, gr = rbase + r;
+            v[q][i] = double2(0.0, 0.0);
+            if (gr < mp && c2 < w) {
+                const double *src = P + (long)gr * ldw + c2;
+                if (c2 + 1 < w) v[q][i] = *(const double2 *)src;
+                else v[q][i].x = src[0];
+            }
+        }
+    }
+#pragma unroll
+    for (int q = 0; q < IO_CHUNKS; ++q) {
+        const int k = q / (NT / IO_R), h = q % (NT / IO_R);
+#pragma unroll
+        for (int i = 0; i < IO_PER; ++i) {
+            const int p = tid + NT * i;
+            *(double2 *)&s_io[(p >> 4) * IO_LD + (p & 15) * 2] = v[q][i];
+        }
+        __syncthreads();
+        if (tid / IO_R == h) {
+#pragma unroll
+            for (int c = 0; c < W; c += 2) {
+                const double2 e = *(const double2 *)&s_io[(tid % IO_R) * IO_LD + c];
+                row[k][c] = e.x;
+                row[k][c + 1] = e.y;
+            }
+        }
+        __syncthreads();
+    }
 
     // The pivot row of column jj (the winner of group wg, posted pending
     // when jj > 0): element c in lane c, finished with the pivot row of
@@ -486,17 +520,51 @@ __global__ __launch_bounds__(NT) void panel_kernel(
     // start + e (a pivot row: start = its column; a row left below the
     // panel's pivots: start = mn); record the moved rows
     int *mylist = list + (size_t)g * PANEL_LIST_SLICE;
+    // the multipliers are loaded first, all at once (unconditional, in
+    // bounds: lsc holds W columns of mp rows), so the epilogue waits for
+    // one memory latency instead of one per column
+    double mv[RPT][W];
+#pragma unroll
+    for (int k = 0; k < RPT; ++k) {
+        const long r = rid[k] < mp ? rid[k] : 0;
+#pragma unroll
+        for (int c = 0; c < W; ++c) mv[k][c] = lsc[(long)c * mp + r];
+    }
+    // the final rows leave through LDS as they came in: each thread stages
+    // its row, sixteen lanes store one row as 16-byte pairs
+#pragma unroll
+    for (int q = 0; q < IO_CHUNKS; ++q) {
+        const int k = q / (NT / IO_R), h = q % (NT / IO_R);
+        if (tid / IO_R == h) {
+            // multipliers at [0, W), then the registers at [start, start + W)
+            // over them: unconditional stores, the later ones win
+            // (start <= W: a pivot row's column, mn, or 0 for a slot
+            // beyond the panel, whose row is never stored)
+            double *st = s_io + (tid % IO_R) * OUT_LD;
+            const int start = rid[k] >= mp ? 0 : done[k] ? pos[k] : mn;
+#pragma unroll
+            for (int c = 0; c < W; ++c) st[c] = mv[k][c];
+#pragma unroll
+            for (int e = 0; e < W; ++e) st[start + e] = row[k][e];
+            s_ipos[tid % IO_R] = rid[k] < mp ? pos[k] : -1;
+        }
+        __syncthreads();
+#pragma unroll
+        for (int i = 0; i < IO_PER; ++i) {
+            const int p = tid + NT * i, r = p >> 4, c2 = (p & 15) * 2;
+            const int dp = s_ipos[r];
+            if (dp >= 0 && c2 < w) {
+                const double *sr = s_io + r * OUT_LD + c2;
+                double *dst = P + (long)dp * ldw + c2;
+                if (c2 + 1 < w) *(double2 *)dst = double2(sr[0], sr[1]);
+                else dst[0] = sr[0];
+            }
+        }
+        __syncthreads();
+    }
 #pragma unroll
     for (int k = 0; k < RPT; ++k) {
         if (rid[k] < mp) {
-            double *dst = P + (long)pos[k] * ldw;
-            const int start = done[k] ? pos[k] : mn;
-#pragma unroll
-            for (int c = 0; c < W; ++c)
-                if (c < start) dst[c] = lsc[(long)c * mp + rid[k]];
-#pragma unroll
-            for (int e = 0; e < W; ++e)
-                if (start + e < w) dst[start + e] = row[k][e];
             if (pos[k] != rid[k]) {
                 const int e = atomicAdd(&s_cnt, 1);
                 if (e < PANEL_LIST_PER_WG) {
