@@ -229,11 +229,17 @@ __global__ __launch_bounds__(256) void lu_stage_kernel(const double *__restrict_
         __shared__ int s_bad;
         if (t == 0) s_bad = 0;
         __syncthreads();
-        for (int e = t; e < 32 * 32; e += 256) {
-            const int r = e >> 5, c = e & 31;
-            const double v = (r < w && c < r) ? W[(long)(j + r) * ldw + j + c] : 0.0;
-            if (!__builtin_isfinite(v)) s_bad = 1;
-            s_l[r][c] = v;
+        double lv[4];
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            const int e = t + 256 * q, r = e >> 5, c = e & 31;
+            lv[q] = (r < w && c < r) ? W[(long)(j + r) * ldw + j + c] : 0.0;
+        }
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            const int e = t + 256 * q;
+            if (!__builtin_isfinite(lv[q])) s_bad = 1;
+            s_l[e >> 5][e & 31] = lv[q];
         }
         __syncthreads();
         double *inv = stage + LU_STAGE_INV;
@@ -287,7 +293,7 @@ __global__ __launch_bounds__(256) void lu_apply_kernel(double *__restrict__ W, l
                                                        int *__restrict__ perm)
 {
     critical_path_priority();
-    __shared__ double s_l[32][33];               // L11 shifted: s_l[k][i] = L(k+1+i, k)
+    __shared__ double s_l[32][33];               // inv(L11) or L11 shifted (s_l[k][i] = L(k+1+i, k))
     __shared__ double s12[32][LA_CR + 1];        // U12
     __shared__ double s_a[LA_ROWS][LA_CR + 1];   // the tile's right parts
     __shared__ double sx[LA_ROWS][33];           // the tile's L21 rows
@@ -296,36 +302,43 @@ __global__ __launch_bounds__(256) void lu_apply_kernel(double *__restrict__ W, l
     const int jo = j - J;
     const int cr = jb - jo - w;                  // columns right of the panel
     const int i0 = j + blockIdx.x * LA_ROWS;     // first row of the tile
+    // Every load that does not depend on the interchange list is issued at
+    // once: the list's count and entries, the inverse's flag, L11 (shifted:
+    // s_l[k][i] = L(k+1+i, k)) and its inverse (row-major), the tile's L21
+    // rows (panel columns, final) and this thread's column of the top rows.
     const int cnt = sdst[LU_STAGE_TOP];
-    {
-        double lv[4];
+    const int sd = sdst[t < LU_STAGE_TOP ? t : 0];
+    const bool useinv = stage[LU_STAGE_INV + 32 * 32] != 0.0;
+    double lv[4], iv[4], lx[4];
 #pragma unroll
-        for (int q = 0; q < 4; ++q) {
-            const int e = t + 256 * q, k = e >> 5, i = e & 31, r = k + 1 + i;
-            lv[q] = (r < w) ? W[(long)(j + (r < 32 ? r : 0)) * ldw + j + k] : 0.0;
-        }
+    for (int q = 0; q < 4; ++q) {
+        const int e = t + 256 * q, k = e >> 5, i = e & 31, r = k + 1 + i;
+        lv[q] = (r < w) ? W[(long)(j + (r < 32 ? r : 0)) * ldw + j + k] : 0.0;
+        iv[q] = stage[LU_STAGE_INV + e];
+        const int rr = e >> 5, kk = e & 31, ii = i0 + rr;
+        lx[q] = (ii < m && kk < w && ii - j >= w) ? W[(long)ii * ldw + j + kk] : 0.0;
+    }
+    double x[32];
 #pragma unroll
-        for (int q = 0; q < 4; ++q) {
-            const int e = t + 256 * q;
-            s_l[e >> 5][e & 31] = lv[q];
-        }
+    for (int k = 0; k < 32; ++k)
+        x[k] = (t < cr && k < w) ? stage[(long)(LU_STAGE_TOP + k) * LU_STAGE_LD + jo + w + t]
+                                 : 0.0;
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+        const int e = t + 256 * q;
+        s_l[e >> 5][e & 31] = useinv ? iv[q] : lv[q];
+        sx[e >> 5][e & 31] = lx[q];
     }
     if (t < LA_ROWS) s_slot[t] = -1;
     __syncthreads();
     if (t < cnt) {
-        const int d = sdst[t] - i0;
+        const int d = sd - i0;
         if (d >= 0 && d < LA_ROWS) s_slot[d] = t;
     }
     __syncthreads();
-    // the tile's L21 rows (panel columns, final) and right parts (moved rows
-    // from the stage), every load issued before any is used
+    // the tile's right parts (moved rows from the stage), every load issued
+    // before any is used
     {
-        double lx[4];
-#pragma unroll
-        for (int q = 0; q < 4; ++q) {
-            const int e = t + 256 * q, r = e >> 5, k = e & 31, i = i0 + r;
-            lx[q] = (i < m && k < w && i - j >= w) ? W[(long)i * ldw + j + k] : 0.0;
-        }
         double av[LA_NQ];
 #pragma unroll
         for (int q = 0; q < LA_NQ; ++q) {
@@ -336,47 +349,27 @@ __global__ __launch_bounds__(256) void lu_apply_kernel(double *__restrict__ W, l
             av[q] = (c < cr && i < m) ? src[c] : 0.0;
         }
 #pragma unroll
-        for (int q = 0; q < 4; ++q) {
-            const int e = t + 256 * q;
-            sx[e >> 5][e & 31] = lx[q];
-        }
-#pragma unroll
         for (int q = 0; q < LA_NQ; ++q) {
             const int e = t + 256 * q;
             s_a[e / LA_CR][e % LA_CR] = av[q];
         }
     }
-    const bool useinv = stage[LU_STAGE_INV + 32 * 32] != 0.0;
-    if (useinv && cr > 0) {
+    if (useinv && cr > 0 && t < cr) {
         // U12 = inv(L11) A12 over the inverse's lower triangle only (row r
         // takes the top rows k <= r, as the substitution does), every entry
-        // independent; the inverse replaces L11 in s_l (row-major)
-        __syncthreads();
-        for (int e = t; e < 32 * 32; e += 256) s_l[e >> 5][e & 31] = stage[LU_STAGE_INV + e];
-        __syncthreads();
-        if (t < cr) {
-            // one column per thread, the top rows' entries in registers
-            double x[32];
+        // independent, one column per thread in registers
 #pragma unroll
-            for (int k = 0; k < 32; ++k)
-                x[k] = (k < w) ? stage[(long)(LU_STAGE_TOP + k) * LU_STAGE_LD + jo + w + t] : 0.0;
+        for (int r = 0; r < 32; ++r) {
+            double v = x[r];
 #pragma unroll
-            for (int r = 0; r < 32; ++r) {
-                double v = x[r];
-#pragma unroll
-                for (int k = 0; k < r; ++k) v = fma(s_l[r][k], x[k], v);
-                s12[r][t] = v;
-            }
+            for (int k = 0; k < r; ++k) v = fma(s_l[r][k], x[k], v);
+            s12[r][t] = v;
         }
     }
     // else U12 by forward substitution with the unit-lower L11, one column
     // per thread; x rotates (register 0 is the finished entry of the step),
     // so the rolled loop indexes registers by constants only
     if (!useinv && cr > 0 && t < cr) {
-        double x[32];
-#pragma unroll
-        for (int k = 0; k < 32; ++k)
-            x[k] = (k < w) ? stage[(long)(LU_STAGE_TOP + k) * LU_STAGE_LD + jo + w + t] : 0.0;
 #pragma unroll 1
         for (int k = 0; k < 32; ++k) {
             const double x0 = x[0];
@@ -781,7 +774,6 @@ __global__ __launch_bounds__(256) void apply_moves_kernel(T *__restrict__ Wm, lo
                                                           unsigned long long *__restrict__ moved)
 {
     critical_path_priority();
-    __shared__ T tile[LU_BLOCK_MOVES][AM_CW];
     __shared__ int s_dst[LU_BLOCK_MOVES], s_src[LU_BLOCK_MOVES];
     const int cnt = min(mv[0], LU_BLOCK_MOVES);
     if (cnt <= 0) return;
@@ -793,22 +785,42 @@ __global__ __launch_bounds__(256) void apply_moves_kernel(T *__restrict__ Wm, lo
     else { c0 = ce + (chunk - nleft) * AM_CW; c1 = min(n, c0 + AM_CW); }
     const int cw = c1 - c0;
     if (cw <= 0) return;
-    for (int e = threadIdx.x; e < cnt; e += 256) {
-        const int d = mv[1 + 2 * e], sr = mv[2 + 2 * e];
-        const bool ok = d >= 0 && d < m && sr >= 0 && sr < m;
-        s_dst[e] = ok ? d : 0;
-        s_src[e] = ok ? sr : 0;
+    {
+        int d[LU_BLOCK_MOVES / 256], sr[LU_BLOCK_MOVES / 256];
+#pragma unroll
+        for (int i = 0; i < LU_BLOCK_MOVES / 256; ++i) {
+            const int e = threadIdx.x + 256 * i;
+            d[i] = e < cnt ? mv[1 + 2 * e] : 0;
+            sr[i] = e < cnt ? mv[2 + 2 * e] : 0;
+        }
+#pragma unroll
+        for (int i = 0; i < LU_BLOCK_MOVES / 256; ++i) {
+            const int e = threadIdx.x + 256 * i;
+            const bool ok = d[i] >= 0 && d[i] < m && sr[i] >= 0 && sr[i] < m;
+            if (e < cnt) {
+                s_dst[e] = ok ? d[i] : 0;
+                s_src[e] = ok ? sr[i] : 0;
+            }
+        }
     }
     __syncthreads();
     if (moved && threadIdx.x == 0) atomicAdd(moved, (unsigned long long)cnt * cw);
-    for (int q = threadIdx.x; q < cnt * AM_CW; q += 256) {
-        const int e = q / AM_CW, c = q % AM_CW;
-        if (c < cw) tile[e][c] = Wm[(long)s_src[e] * ldw + c0 + c];
+    // every source element of the chunk is read before any destination is
+    // written (a destination row can be another move's source): all loads
+    // issued at once into registers, completed, a barrier, then the stores
+    constexpr int PER = LU_BLOCK_MOVES * AM_CW / 256;
+    T v[PER];
+#pragma unroll
+    for (int i = 0; i < PER; ++i) {
+        const int q = threadIdx.x + 256 * i, e = q / AM_CW, c = q % AM_CW;
+        v[i] = (e < cnt && c < cw) ? Wm[(long)s_src[e] * ldw + c0 + c] : T(0);
     }
+    __builtin_amdgcn_s_waitcnt(0xf70);   // vmcnt(0)
     __syncthreads();
-    for (int q = threadIdx.x; q < cnt * AM_CW; q += 256) {
-        const int e = q / AM_CW, c = q % AM_CW;
-        if (c < cw) Wm[(long)s_dst[e] * ldw + c0 + c] = tile[e][c];
+#pragma unroll
+    for (int i = 0; i < PER; ++i) {
+        const int q = threadIdx.x + 256 * i, e = q / AM_CW, c = q % AM_CW;
+        if (e < cnt && c < cw) Wm[(long)s_dst[e] * ldw + c0 + c] = v[i];
     }
 }
 
