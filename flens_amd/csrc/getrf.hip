@@ -157,7 +157,6 @@ __global__ __launch_bounds__(256) void perm_compose_kernel(const int *__restrict
 constexpr int LA_ROWS = 32;                 // rows per apply workgroup
 constexpr int LA_TPR = 256 / LA_ROWS;       // threads per row
 constexpr int LA_CR = LU_STAGE_LD - 32;     // widest right part (block 256, panel >= 32)
-constexpr int LA_NQ = LA_ROWS * LA_CR / 256;   // right-part elements per thread
 static_assert(LU_STAGE_TOP == RP_MAXE, "stage layout");
 
 // The interchange list's slices gathered into LDS with every load issued at
@@ -285,6 +284,10 @@ __global__ __launch_bounds__(256) void lu_stage_kernel(const double *__restrict_
     stage[(long)b * LU_STAGE_LD + t] = W[(long)src * ldw + J + t];
 }
 
+// CR: the widest right part the instance handles (the LDS arrays scale with
+// it, so at nb <= 128 a workgroup fits beside a trailing-update GEMM's on
+// one compute unit)
+template <int CR>
 __global__ __launch_bounds__(256) void lu_apply_kernel(double *__restrict__ W, long ldw, int m,
                                                        int J, int jb, int j, int w,
                                                        const double *__restrict__ stage,
@@ -292,10 +295,12 @@ __global__ __launch_bounds__(256) void lu_apply_kernel(double *__restrict__ W, l
                                                        const int *__restrict__ sperm,
                                                        int *__restrict__ perm)
 {
+    static_assert(LA_ROWS * CR % 256 == 0 && CR % LA_TPR == 0 && CR <= CR, "apply tile");
+    constexpr int NQ = LA_ROWS * CR / 256;   // right-part elements per thread
     critical_path_priority();
     __shared__ double s_l[32][33];               // inv(L11) or L11 shifted (s_l[k][i] = L(k+1+i, k))
-    __shared__ double s12[32][LA_CR + 1];        // U12
-    __shared__ double s_a[LA_ROWS][LA_CR + 1];   // the tile's right parts
+    __shared__ double s12[32][CR + 1];        // U12
+    __shared__ double s_a[LA_ROWS][CR + 1];   // the tile's right parts
     __shared__ double sx[LA_ROWS][33];           // the tile's L21 rows
     __shared__ int s_slot[LA_ROWS];
     const int t = threadIdx.x;
@@ -339,19 +344,19 @@ __global__ __launch_bounds__(256) void lu_apply_kernel(double *__restrict__ W, l
     // the tile's right parts (moved rows from the stage), every load issued
     // before any is used
     {
-        double av[LA_NQ];
+        double av[NQ];
 #pragma unroll
-        for (int q = 0; q < LA_NQ; ++q) {
-            const int e = t + 256 * q, r = e / LA_CR, c = e % LA_CR, i = i0 + r;
+        for (int q = 0; q < NQ; ++q) {
+            const int e = t + 256 * q, r = e / CR, c = e % CR, i = i0 + r;
             const int sl = s_slot[r];
             const double *src = (sl >= 0) ? stage + (long)sl * LU_STAGE_LD + jo + w
                                           : W + (long)(i < m ? i : j) * ldw + j + w;
             av[q] = (c < cr && i < m) ? src[c] : 0.0;
         }
 #pragma unroll
-        for (int q = 0; q < LA_NQ; ++q) {
+        for (int q = 0; q < NQ; ++q) {
             const int e = t + 256 * q;
-            s_a[e / LA_CR][e % LA_CR] = av[q];
+            s_a[e / CR][e % CR] = av[q];
         }
     }
     if (useinv && cr > 0 && t < cr) {
@@ -396,7 +401,7 @@ __global__ __launch_bounds__(256) void lu_apply_kernel(double *__restrict__ W, l
     if (slot >= 0) {
         // the left part (a multiple of 32 columns) of a row that moves in
         const double *src = stage + (long)slot * LU_STAGE_LD;
-        constexpr int NL = LA_CR / LA_TPR;
+        constexpr int NL = CR / LA_TPR;
         double lv[NL];
 #pragma unroll
         for (int q = 0; q < NL; ++q) {
@@ -894,10 +899,19 @@ hipError_t getrf_rm_f64(hipStream_t s, LuWork &wk, double *W, long ldw, int m, i
                 const int cr = J + jb - (j + w);
                 ProfScope ps(wk.prof, s, PROF_TRSM_RM, 2.0 * (m - j - w) * (double)w * cr,
                              16.0 * (m - j) * (double)(cr > 0 ? cr : 0));
-                hipLaunchKernelGGL(lu_apply_kernel, dim3((m - j + LA_ROWS - 1) / LA_ROWS),
-                                   dim3(256), 0, s, W, ldw, m, J, jb, j, w,
-                                   (const double *)wk.stage, (const int *)wk.sdst,
-                                   (const int *)wk.sperm, wk.perm);
+                const dim3 ag((m - j + LA_ROWS - 1) / LA_ROWS);
+                if (jb - PANEL_W <= 32)
+                    hipLaunchKernelGGL(lu_apply_kernel<32>, ag, dim3(256), 0, s, W, ldw, m, J, jb,
+                                       j, w, (const double *)wk.stage, (const int *)wk.sdst,
+                                       (const int *)wk.sperm, wk.perm);
+                else if (jb - PANEL_W <= 96)
+                    hipLaunchKernelGGL(lu_apply_kernel<96>, ag, dim3(256), 0, s, W, ldw, m, J, jb,
+                                       j, w, (const double *)wk.stage, (const int *)wk.sdst,
+                                       (const int *)wk.sperm, wk.perm);
+                else
+                    hipLaunchKernelGGL(lu_apply_kernel<LA_CR>, ag, dim3(256), 0, s, W, ldw, m, J,
+                                       jb, j, w, (const double *)wk.stage, (const int *)wk.sdst,
+                                       (const int *)wk.sperm, wk.perm);
                 FLX_CHECK(hipGetLastError());
                 continue;
             }
