@@ -123,10 +123,11 @@ hipError_t launch_panel(hipStream_t s, LuWork &wk, double *P, long ldw, int mp, 
 // piv0 receives 0-based absolute pivot rows (length min(m,n)); info receives
 // the 1-based first zero pivot column (INT_MAX if none; caller initialises).
 // Deferred interchanges of one outer block [J, J+jb): inv = inverse of perm
-// before the block; afterwards defer_moves moves the rows the block's
-// sub-panels moved in its own columns in every other column too (mv holds
-// 1 + 2 * LU_BLOCK_MOVES ints).
-hipError_t inv_perm(hipStream_t s, const int *perm, int *inv, int m);
+// before the block (inv_perm, which also clears the move count mv[0]);
+// afterwards defer_moves moves the rows the block's sub-panels moved in its
+// own columns in every other column too (mv holds 1 + 2 * LU_BLOCK_MOVES
+// ints, mv[0] zero on entry).
+hipError_t inv_perm(hipStream_t s, const int *perm, int *inv, int m, int *reset = nullptr);
 template <typename T>
 hipError_t defer_moves(hipStream_t s, Profiler *prof, T *W, long ldw, int m, int n, int J,
                        int jb, const int *perm, const int *inv, int *mv,

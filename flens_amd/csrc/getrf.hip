@@ -443,6 +443,7 @@ __global__ __launch_bounds__(256) void lu_apply_kernel(double *__restrict__ W, l
 constexpr int TR_CW = 32;
 constexpr int TR_KMAX = 256;
 constexpr int TR_BS = 16;
+constexpr int TR_FOLD = 4;   // rows folded together per thread
 
 template <typename T>
 __global__ __launch_bounds__(256) void trsm_rm_llu_kernel(const T *__restrict__ L,
@@ -486,15 +487,26 @@ __global__ __launch_bounds__(256) void trsm_rm_llu_kernel(const T *__restrict__ 
                     if (i < bs) xs[kb + i][tc] = xb[i];
             }
             __syncthreads();
-            // fold the solved block into the rows below
+            // fold the solved block into the rows below, four rows (four
+            // independent chains, each in the order above) per pass
             T xb[TR_BS];
 #pragma unroll
             for (int q = 0; q < TR_BS; ++q) xb[q] = xs[kb + q < k ? kb + q : kb][tc];
-            for (int rr = kb + bs + tg; rr < k; rr += 8) {
-                T v = xs[rr][tc];
+            for (int r0 = kb + bs + tg; r0 < k; r0 += 8 * TR_FOLD) {
+                T v[TR_FOLD];
+                int rc[TR_FOLD];
 #pragma unroll
-                for (int q = 0; q < TR_BS; ++q) v = fma(-ls[rr - kb][q], xb[q], v);
-                xs[rr][tc] = v;
+                for (int u = 0; u < TR_FOLD; ++u) {
+                    rc[u] = min(r0 + 8 * u, k - 1);
+                    v[u] = xs[rc[u]][tc];
+                }
+#pragma unroll
+                for (int q = 0; q < TR_BS; ++q)
+#pragma unroll
+                    for (int u = 0; u < TR_FOLD; ++u) v[u] = fma(-ls[rc[u] - kb][q], xb[q], v[u]);
+#pragma unroll
+                for (int u = 0; u < TR_FOLD; ++u)
+                    if (r0 + 8 * u < k) xs[r0 + 8 * u][tc] = v[u];
             }
             __syncthreads();
         }
@@ -517,15 +529,18 @@ constexpr int TR_FULLK = 128;
 constexpr int TR_FULLB = TR_FULLK / TR_BS;
 constexpr int TR_LPAD = TR_BS * TR_BS * TR_FULLB * (TR_FULLB + 1) / 2;
 
-template <typename T>
+// CW columns per workgroup, 256 / CW row groups folding the rows below
+template <typename T, int CW>
 __global__ __launch_bounds__(256) void trsm_rm_llu_full_kernel(const T *__restrict__ L,
                                                                long ldl, T *__restrict__ X,
                                                                long ldx, int k, int ncols)
 {
-    __shared__ T xs[TR_FULLK][TR_CW + 1];
+    constexpr int NG = 256 / CW;
+    static_assert(256 % CW == 0 && TR_FULLK % NG == 0, "column chunk");
+    __shared__ T xs[TR_FULLK][CW + 1];
     __shared__ T lp[TR_LPAD];
     const int tid = threadIdx.x;
-    const int tc = tid % TR_CW, tg = tid / TR_CW;   // 32 columns x 8 row groups
+    const int tc = tid % CW, tg = tid / CW;   // CW columns x NG row groups
     // block R: 256 (R + 1) entries, R + 1 per thread, all loads issued first
     StaticFor<0, TR_FULLB>::run([&](auto RC) {
         constexpr int R = decltype(RC)::value;
@@ -542,19 +557,19 @@ __global__ __launch_bounds__(256) void trsm_rm_llu_full_kernel(const T *__restri
         }
     });
     // workgroup b solves the column chunks b, b + grid, ...
-    for (int c0 = blockIdx.x * TR_CW; c0 < ncols; c0 += gridDim.x * TR_CW) {
-        const int cw = min(TR_CW, ncols - c0);
-        if (c0 != (int)blockIdx.x * TR_CW) __syncthreads();
-        // rows tg, tg + 8, ... of the chunk, all loads issued first
+    for (int c0 = blockIdx.x * CW; c0 < ncols; c0 += gridDim.x * CW) {
+        const int cw = min(CW, ncols - c0);
+        if (c0 != (int)blockIdx.x * CW) __syncthreads();
+        // rows tg, tg + NG, ... of the chunk, all loads issued first
         {
-            T v[TR_FULLK / 8];
+            T v[TR_FULLK / NG];
 #pragma unroll
-            for (int u = 0; u < TR_FULLK / 8; ++u) {
-                const int rr = tg + 8 * u;
+            for (int u = 0; u < TR_FULLK / NG; ++u) {
+                const int rr = tg + NG * u;
                 v[u] = (rr < k && tc < cw) ? X[(long)rr * ldx + c0 + tc] : T(0);
             }
 #pragma unroll
-            for (int u = 0; u < TR_FULLK / 8; ++u) xs[tg + 8 * u][tc] = v[u];
+            for (int u = 0; u < TR_FULLK / NG; ++u) xs[tg + NG * u][tc] = v[u];
         }
         __syncthreads();
         for (int kb = 0; kb < k; kb += TR_BS) {
@@ -572,32 +587,46 @@ __global__ __launch_bounds__(256) void trsm_rm_llu_full_kernel(const T *__restri
                 }
 #pragma unroll
                 for (int i = 1; i < TR_BS; ++i) {
+                    // row i's coefficients, read as whole 16-entry rows
+                    T co[TR_BS];
+#pragma unroll
+                    for (int q = 0; q < TR_BS; ++q) co[q] = lb[i * lw + q];
                     T v = xb[i];
 #pragma unroll
-                    for (int q = 0; q < i; ++q) v = fma(-lb[i * lw + q], xb[q], v);
+                    for (int q = 0; q < i; ++q) v = fma(-co[q], xb[q], v);
                     xb[i] = v;
                 }
 #pragma unroll
                 for (int i = 0; i < TR_BS; ++i) xs[kb + i][tc] = xb[i];
             }
             __syncthreads();
-            // fold the solved block into the rows below
+            // fold the solved block into the rows below, four rows (four
+            // independent chains, each in the order above) per pass
             T xb[TR_BS];
 #pragma unroll
             for (int q = 0; q < TR_BS; ++q) xb[q] = xs[kb + q < k ? kb + q : kb][tc];
-            for (int rr = kb + bs + tg; rr < k; rr += 8) {
-                const int Rr = rr / TR_BS;
-                const T *lr = lp + TR_BS * TR_BS * Rr * (Rr + 1) / 2 +
-                              (rr % TR_BS) * TR_BS * (Rr + 1) + kb;
-                T v = xs[rr][tc];
+            for (int r0 = kb + bs + tg; r0 < k; r0 += NG * TR_FOLD) {
+                T v[TR_FOLD];
+                const T *lr[TR_FOLD];
 #pragma unroll
-                for (int q = 0; q < TR_BS; ++q) v = fma(-lr[q], xb[q], v);
-                xs[rr][tc] = v;
+                for (int u = 0; u < TR_FOLD; ++u) {
+                    const int rr = min(r0 + NG * u, k - 1), Rr = rr / TR_BS;
+                    lr[u] = lp + TR_BS * TR_BS * Rr * (Rr + 1) / 2 + (rr % TR_BS) * TR_BS * (Rr + 1) +
+                            kb;
+                    v[u] = xs[rr][tc];
+                }
+#pragma unroll
+                for (int q = 0; q < TR_BS; ++q)
+#pragma unroll
+                    for (int u = 0; u < TR_FOLD; ++u) v[u] = fma(-lr[u][q], xb[q], v[u]);
+#pragma unroll
+                for (int u = 0; u < TR_FOLD; ++u)
+                    if (r0 + NG * u < k) xs[r0 + NG * u][tc] = v[u];
             }
             __syncthreads();
         }
-        for (int q = tid; q < k * TR_CW; q += 256) {
-            int rr = q / TR_CW, c = q % TR_CW;
+        for (int q = tid; q < k * CW; q += 256) {
+            int rr = q / CW, c = q % CW;
             if (c < cw) X[(long)rr * ldx + c0 + c] = xs[rr][c];
         }
     }
@@ -684,9 +713,15 @@ hipError_t trsm_rm_llu(hipStream_t s, Profiler *prof, const T *L, long ldl, T *X
                  sizeof(T) * (2.0 * k * ncols + 0.5 * k * k));
     int groups = (ncols + TR_CW - 1) / TR_CW;
     if (max_groups > 0 && groups > max_groups) groups = max_groups;
-    if (k <= TR_FULLK)
-        hipLaunchKernelGGL(trsm_rm_llu_full_kernel<T>, dim3(groups), dim3(256), 0, s, L, ldl, X,
+    if (k <= TR_FULLK && ncols <= 4 * TR_CW) {
+        // a narrow solve (the next block's columns): 8-column chunks, so
+        // more workgroups share the rows below each diagonal block
+        const int g8 = (ncols + 7) / 8;
+        hipLaunchKernelGGL((trsm_rm_llu_full_kernel<T, 8>), dim3(g8), dim3(256), 0, s, L, ldl, X,
                            ldx, k, ncols);
+    } else if (k <= TR_FULLK)
+        hipLaunchKernelGGL((trsm_rm_llu_full_kernel<T, TR_CW>), dim3(groups), dim3(256), 0, s, L,
+                           ldl, X, ldx, k, ncols);
     else
         hipLaunchKernelGGL(trsm_rm_llu_kernel<T>, dim3(groups), dim3(256), 0, s, L, ldl, X, ldx,
                            k, ncols);
@@ -742,9 +777,11 @@ static hipError_t rm_update(hipStream_t s, Profiler *prof, double *W, long ldw, 
 // moves the same rows in every other column at once.  inv is the inverse of
 // perm before the block, so a row now at position p came from inv[perm[p]].
 // ---------------------------------------------------------------------------
-__global__ void inv_perm_kernel(const int *__restrict__ perm, int *__restrict__ inv, int m)
+__global__ void inv_perm_kernel(const int *__restrict__ perm, int *__restrict__ inv, int m,
+                                int *__restrict__ reset)
 {
     const int p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p == 0 && reset) *reset = 0;   // the block's move count, before its moves are listed
     if (p < m) {
         const int o = perm[p];
         if (o >= 0 && o < m) inv[o] = p;
@@ -829,10 +866,11 @@ __global__ __launch_bounds__(256) void apply_moves_kernel(T *__restrict__ Wm, lo
     }
 }
 
-hipError_t inv_perm(hipStream_t s, const int *perm, int *inv, int m)
+hipError_t inv_perm(hipStream_t s, const int *perm, int *inv, int m, int *reset)
 {
     if (m <= 0) return hipSuccess;
-    hipLaunchKernelGGL(inv_perm_kernel, dim3((m + 255) / 256), dim3(256), 0, s, perm, inv, m);
+    hipLaunchKernelGGL(inv_perm_kernel, dim3((m + 255) / 256), dim3(256), 0, s, perm, inv, m,
+                       reset);
     return hipGetLastError();
 }
 
@@ -843,7 +881,7 @@ hipError_t defer_moves(hipStream_t s, Profiler *prof, T *W, long ldw, int m, int
 {
     const int rows = m - J;
     if (rows <= 0) return hipSuccess;
-    hipLaunchKernelGGL(fill_int_kernel, dim3(1), dim3(1), 0, s, mv, 1, 0);
+    // mv[0] was cleared by the block's inv_perm(..., mv)
     hipLaunchKernelGGL(block_moves_kernel, dim3((rows + 255) / 256), dim3(256), 0, s, perm, inv,
                        J, m, mv);
     const int nleft = (J + AM_CW - 1) / AM_CW;
@@ -880,7 +918,7 @@ hipError_t getrf_rm_f64(hipStream_t s, LuWork &wk, double *W, long ldw, int m, i
     bool rest_pending = false;
     for (int J = 0; J < mn; J += nb) {
         const int jb = (mn - J < nb) ? mn - J : nb;
-        if (look) FLX_CHECK(inv_perm(s, wk.perm, wk.inv, m));
+        if (look) FLX_CHECK(inv_perm(s, wk.perm, wk.inv, m, wk.mv));
         for (int j = J; j < J + jb; j += ib) {
             const int w = (J + jb - j < ib) ? J + jb - j : ib;
             FLX_CHECK(launch_panel(s, wk, W + (long)j * ldw + j, ldw, m - j, w, piv0 + j, j,

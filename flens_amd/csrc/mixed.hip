@@ -888,7 +888,7 @@ hipError_t getrf_rm_mixed(hipStream_t s, const MixWork &mw, float *W, long ldw, 
     bool rest_pending = false;
     for (int J = 0; J < n; J += nb) {
         const int jb = (n - J < nb) ? n - J : nb;
-        if (look) FLX_CHECK(inv_perm(s, mw.perm, mw.inv, n));
+        if (look) FLX_CHECK(inv_perm(s, mw.perm, mw.inv, n, mw.mv));
         for (int j = J; j < J + jb; j += ib) {
             const int w = (J + jb - j < ib) ? J + jb - j : ib;
             const int mp = n - j;
