@@ -47,40 +47,59 @@ constexpr int PANEL_WMAX = 64;
 // Row permutation of a row-major matrix outside one panel:
 //   W[dst_e][c] = W_old[src_e][c]   for every list entry e,
 // for columns c in [0, c1) U [c2, n).  Each workgroup owns a 64-column
-// chunk: it reads every source segment into LDS, then writes (no hazard
-// between chunks).  This is laswp (flens/lapack/la/laswp.tcc:54-93) for
+// chunk: it reads every source segment into registers, then writes (no
+// hazard between chunks).  This is laswp (flens/lapack/la/laswp.tcc:54-93) for
 // both column ranges of trf.tcc:126 and :132 at once.
 // ---------------------------------------------------------------------------
 constexpr int RP_CW = 64;
 constexpr int RP_MAXE = 2 * PANEL_WMAX;
 
-// gathers the per-workgroup slices of a panel's interchange list into LDS
-FLX_DEV int load_list(const int *__restrict__ list, int G, int m, int *s_dst, int *s_src,
-                      int *s_off)
+// The interchange list's slices gathered into LDS with every load issued at
+// once: 256 threads x SL slots cover PANEL_GROUPS_MAX slices of up to 64
+// entries, slice offsets by a wave prefix sum.  Returns the entry count.
+FLX_DEV int load_list_par(const int *__restrict__ list, int G, int m, int *s_dst, int *s_src,
+                          int *s_cnt)
 {
     const int t = threadIdx.x;
-    if (t < G) s_off[t] = min(list[(size_t)t * PANEL_LIST_SLICE], 64);
-    __syncthreads();
-    if (t == 0) {
-        int acc = 0;
-        for (int q = 0; q < G; ++q) { int c = s_off[q]; s_off[q] = acc; acc += c; }
-        s_off[PANEL_GROUPS_MAX] = min(acc, RP_MAXE);
+    constexpr int SL = PANEL_GROUPS_MAX * 64 / 256;   // slots per thread
+    static_assert(PANEL_GROUPS_MAX * 64 == 256 * SL, "slot map");
+    int d[SL], sr[SL];
+#pragma unroll
+    for (int q = 0; q < SL; ++q) {
+        const int slot = t + 256 * q, g = slot >> 6, e = slot & 63;
+        const int *sl = list + (size_t)(g < G ? g : 0) * PANEL_LIST_SLICE;
+        d[q] = sl[1 + 2 * e];
+        sr[q] = sl[2 + 2 * e];
+    }
+    static_assert(PANEL_GROUPS_MAX == 64, "one wave scans the slice counts");
+    if (t < 64) {
+        // counts, their exclusive prefix sums (slice offsets), the total
+        const int c = (t < G) ? min(list[(size_t)t * PANEL_LIST_SLICE], 64) : 0;
+        int inc = c;
+#pragma unroll
+        for (int o = 1; o < 64; o <<= 1) {
+            const int y = __shfl_up(inc, o, 64);
+            if (t >= o) inc += y;
+        }
+        s_cnt[t] = c;
+        s_cnt[PANEL_GROUPS_MAX + 1 + t] = inc - c;
+        if (t == 63) s_cnt[PANEL_GROUPS_MAX] = min(inc, RP_MAXE);
     }
     __syncthreads();
-    if (t < G) {
-        const int *sl = list + (size_t)t * PANEL_LIST_SLICE;
-        const int c = min(sl[0], 64);
-        for (int e = 0; e < c; ++e) {
-            const int o = s_off[t] + e;
-            if (o >= RP_MAXE) break;
-            int d = sl[1 + 2 * e], sr = sl[2 + 2 * e];
-            bool ok = d >= 0 && d < m && sr >= 0 && sr < m;
-            s_dst[o] = ok ? d : 0;   // an invalid entry degenerates to row 0 -> row 0
-            s_src[o] = ok ? sr : 0;
+#pragma unroll
+    for (int q = 0; q < SL; ++q) {
+        const int slot = t + 256 * q, g = slot >> 6, e = slot & 63;
+        if (e < s_cnt[g]) {
+            const int o = s_cnt[PANEL_GROUPS_MAX + 1 + g] + e;
+            const bool ok = d[q] >= 0 && d[q] < m && sr[q] >= 0 && sr[q] < m;
+            if (o < RP_MAXE) {
+                s_dst[o] = ok ? d[q] : 0;   // an invalid entry degenerates to row 0 -> row 0
+                s_src[o] = ok ? sr[q] : 0;
+            }
         }
     }
     __syncthreads();
-    return s_off[PANEL_GROUPS_MAX];
+    return s_cnt[PANEL_GROUPS_MAX];
 }
 
 template <typename T>
@@ -91,9 +110,8 @@ __global__ __launch_bounds__(256) void rowperm_kernel(T *__restrict__ Wm, long l
                                                       unsigned long long *__restrict__ moved)
 {
     critical_path_priority();
-    __shared__ T tile[RP_MAXE][RP_CW];
-    __shared__ int s_dst[RP_MAXE], s_src[RP_MAXE], s_off[PANEL_GROUPS_MAX + 1];
-    const int cnt = load_list(list, G, m, s_dst, s_src, s_off);
+    __shared__ int s_dst[RP_MAXE], s_src[RP_MAXE], s_cnt[2 * PANEL_GROUPS_MAX + 1];
+    const int cnt = load_list_par(list, G, m, s_dst, s_src, s_cnt);
     if (cnt <= 0) return;
     const int nleft = (c1 + RP_CW - 1) / RP_CW;
     int chunk = blockIdx.x;
@@ -102,14 +120,22 @@ __global__ __launch_bounds__(256) void rowperm_kernel(T *__restrict__ Wm, long l
     else { cbeg = c2 + (chunk - nleft) * RP_CW; cend = min(n, cbeg + RP_CW); }
     const int cw = cend - cbeg;
     if (moved && threadIdx.x == 0) atomicAdd(moved, (unsigned long long)cnt * cw);
-    for (int q = threadIdx.x; q < cnt * RP_CW; q += 256) {
-        int e = q / RP_CW, c = q % RP_CW;
-        if (c < cw) tile[e][c] = Wm[(long)s_src[e] * ldw + cbeg + c];
+    // every source segment is read before any destination is written:
+    // all loads issued at once into registers, completed, a barrier, then
+    // the stores
+    constexpr int PER = RP_MAXE * RP_CW / 256;
+    T v[PER];
+#pragma unroll
+    for (int i = 0; i < PER; ++i) {
+        const int q = threadIdx.x + 256 * i, e = q / RP_CW, c = q % RP_CW;
+        v[i] = (e < cnt && c < cw) ? Wm[(long)s_src[e] * ldw + cbeg + c] : T(0);
     }
+    __builtin_amdgcn_s_waitcnt(0xf70);   // vmcnt(0)
     __syncthreads();
-    for (int q = threadIdx.x; q < cnt * RP_CW; q += 256) {
-        int e = q / RP_CW, c = q % RP_CW;
-        if (c < cw) Wm[(long)s_dst[e] * ldw + cbeg + c] = tile[e][c];
+#pragma unroll
+    for (int i = 0; i < PER; ++i) {
+        const int q = threadIdx.x + 256 * i, e = q / RP_CW, c = q % RP_CW;
+        if (e < cnt && c < cw) Wm[(long)s_dst[e] * ldw + cbeg + c] = v[i];
     }
     // block 0 also composes the global row permutation
     if (blockIdx.x == 0 && perm) {
@@ -125,8 +151,8 @@ __global__ __launch_bounds__(256) void rowperm_kernel(T *__restrict__ Wm, long l
 __global__ __launch_bounds__(256) void perm_compose_kernel(const int *__restrict__ list, int G,
                                                            int *__restrict__ perm, int m)
 {
-    __shared__ int s_dst[RP_MAXE], s_src[RP_MAXE], s_off[PANEL_GROUPS_MAX + 1];
-    const int cnt = load_list(list, G, m, s_dst, s_src, s_off);
+    __shared__ int s_dst[RP_MAXE], s_src[RP_MAXE], s_cnt[2 * PANEL_GROUPS_MAX + 1];
+    const int cnt = load_list_par(list, G, m, s_dst, s_src, s_cnt);
     int v = 0;
     if (threadIdx.x < cnt) v = perm[s_src[threadIdx.x]];
     __syncthreads();
@@ -158,51 +184,6 @@ constexpr int LA_ROWS = 32;                 // rows per apply workgroup
 constexpr int LA_TPR = 256 / LA_ROWS;       // threads per row
 constexpr int LA_CR = LU_STAGE_LD - 32;     // widest right part (block 256, panel >= 32)
 static_assert(LU_STAGE_TOP == RP_MAXE, "stage layout");
-
-// The interchange list's slices gathered into LDS with every load issued at
-// once (load_list walks each slice serially): 256 threads x SL slots cover
-// PANEL_GROUPS_MAX slices of up to 64 entries.  Returns the entry count.
-FLX_DEV int load_list_par(const int *__restrict__ list, int G, int m, int *s_dst, int *s_src,
-                          int *s_cnt)
-{
-    const int t = threadIdx.x;
-    constexpr int SL = PANEL_GROUPS_MAX * 64 / 256;   // slots per thread
-    static_assert(PANEL_GROUPS_MAX * 64 == 256 * SL, "slot map");
-    int d[SL], sr[SL];
-#pragma unroll
-    for (int q = 0; q < SL; ++q) {
-        const int slot = t + 256 * q, g = slot >> 6, e = slot & 63;
-        const int *sl = list + (size_t)(g < G ? g : 0) * PANEL_LIST_SLICE;
-        d[q] = sl[1 + 2 * e];
-        sr[q] = sl[2 + 2 * e];
-    }
-    if (t < PANEL_GROUPS_MAX) s_cnt[t] = (t < G) ? min(list[(size_t)t * PANEL_LIST_SLICE], 64) : 0;
-    __syncthreads();
-    if (t == 0) {
-        int acc = 0;
-        for (int g = 0; g < PANEL_GROUPS_MAX; ++g) {
-            const int c = s_cnt[g];
-            s_cnt[PANEL_GROUPS_MAX + 1 + g] = acc;   // offsets
-            acc += c;
-        }
-        s_cnt[PANEL_GROUPS_MAX] = min(acc, RP_MAXE);
-    }
-    __syncthreads();
-#pragma unroll
-    for (int q = 0; q < SL; ++q) {
-        const int slot = t + 256 * q, g = slot >> 6, e = slot & 63;
-        if (e < s_cnt[g]) {
-            const int o = s_cnt[PANEL_GROUPS_MAX + 1 + g] + e;
-            const bool ok = d[q] >= 0 && d[q] < m && sr[q] >= 0 && sr[q] < m;
-            if (o < RP_MAXE) {
-                s_dst[o] = ok ? d[q] : 0;   // an invalid entry degenerates to row 0 -> row 0
-                s_src[o] = ok ? sr[q] : 0;
-            }
-        }
-    }
-    __syncthreads();
-    return s_cnt[PANEL_GROUPS_MAX];
-}
 
 // One workgroup per stage row: blockIdx.x < LU_STAGE_TOP = list entry,
 // above = top row blockIdx.x - LU_STAGE_TOP.  Workgroup 0 also writes the
