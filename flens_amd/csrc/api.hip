@@ -123,6 +123,7 @@ extern "C" int flx_destroy(flx_handle h)
     if (h->wk.side) (void)hipStreamDestroy(h->wk.side);
     if (h->wk.ev_a) (void)hipEventDestroy(h->wk.ev_a);
     if (h->wk.ev_r) (void)hipEventDestroy(h->wk.ev_r);
+    if (h->wk.ev_n) (void)hipEventDestroy(h->wk.ev_n);
     free_buf(h->lu_inv); free_buf(h->lu_mv);
     free_buf(h->lu_stage); free_buf(h->lu_sdst); free_buf(h->lu_sperm);
     free_buf(h->sr); free_buf(h->sc);
@@ -285,6 +286,7 @@ int dgetrf_dev(flx_context *h, int m, int n, double *A, int lda, int *ipiv, int 
         FLX_TRY(hipStreamCreateWithFlags(&h->wk.side, hipStreamNonBlocking));
         FLX_TRY(hipEventCreateWithFlags(&h->wk.ev_a, hipEventDisableTiming));
         FLX_TRY(hipEventCreateWithFlags(&h->wk.ev_r, hipEventDisableTiming));
+        FLX_TRY(hipEventCreateWithFlags(&h->wk.ev_n, hipEventDisableTiming));
     }
     double *W = (double *)h->W.p;
     FLX_TRY(fill_int(h->stream, info, 1, INT_MAX));

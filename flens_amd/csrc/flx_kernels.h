@@ -74,8 +74,9 @@ struct LuWork {
     // lookahead (the rest of each trailing update on a second stream)
     int *inv = nullptr;         // inverse of perm before the current block (length m)
     int *mv = nullptr;          // block moves [count, dst, src, ...], 1 + 2 * LU_BLOCK_MOVES
-    hipStream_t side = nullptr; // default priority, ordered by the two events
+    hipStream_t side = nullptr; // default priority, ordered by the events
     hipEvent_t ev_a = nullptr, ev_r = nullptr;
+    hipEvent_t ev_n = nullptr;  // optional: the next block's columns done on the side stream
     // workgroups the side stream's launches may use at once, so that the
     // panels and the small launches of the next block always find free
     // compute units (a trailing update left uncapped fills the device)

@@ -139,34 +139,65 @@ FLX_DEV void mma_tile(acc_t (&acc)[GMI][GNJ], const T (*As)[GBM + APAD],
     }
 }
 
-// K loop: register prefetch of tile t+1 overlaps the MFMAs of tile t; LDS
-// double buffer; one barrier per tile.  FAST = interior block with 16-byte
-// aligned operands and K a multiple of the tile (no predicates at all).
-template <typename T, int TA, int TB, bool FAST, typename acc_t>
+// K loop: register prefetch GPF tiles ahead (GPF + 1 register sets used in
+// rotation, the loop unrolled by GPF + 1 so every set index is static), so a
+// tile's loads have GPF iterations of MFMAs to arrive; LDS double buffer;
+// one barrier per tile.  FAST = interior block with 16-byte aligned operands
+// and K a multiple of the tile (no predicates at all).  GPF = 3 for short K
+// loops on small grids (the LU's critical-path updates: few tiles in flight,
+// exposed load latency); GPF = 1, one set, otherwise (measured faster for
+// long K and large grids).
+template <typename T, int TA, int TB, bool FAST, int GPF, typename acc_t>
 FLX_DEV void gemm_mainloop(acc_t (&acc)[GMI][GNJ], T (*As)[GBK][GBM + APAD],
                            T (*Bs)[GBN][GBK + BPAD], const T *__restrict__ A, long lda,
                            const T *__restrict__ B, long ldb, int M, int N, int K, int m0,
                            int n0, int nkt, int tid, int wm, int wn, int lrow, int lk)
 {
-    TileIO<T, TA, TB, FAST> io;
-    io.load(A, lda, B, ldb, M, N, K, m0, n0, 0, tid);
-    io.store(As[0], Bs[0], tid);
+    if constexpr (GPF == 1) {
+        TileIO<T, TA, TB, FAST> io;
+        io.load(A, lda, B, ldb, M, N, K, m0, n0, 0, tid);
+        io.store(As[0], Bs[0], tid);
+        __syncthreads();
+        for (int kt = 0; kt < nkt; ++kt) {
+            const int buf = kt & 1;
+            const bool more = kt + 1 < nkt;
+            if (more) io.load(A, lda, B, ldb, M, N, K, m0, n0, (kt + 1) * GBK, tid);
+            mma_tile<T>(acc, As[buf], Bs[buf], wm, wn, lrow, lk);
+            if (more) {
+                io.store(As[buf ^ 1], Bs[buf ^ 1], tid);
+                __syncthreads();
+            }
+        }
+        return;
+    }
+    constexpr int NS = GPF + 1;
+    TileIO<T, TA, TB, FAST> io[NS];
+#pragma unroll
+    for (int p = 0; p < GPF; ++p)
+        if (p < nkt) io[p].load(A, lda, B, ldb, M, N, K, m0, n0, p * GBK, tid);
+    io[0].store(As[0], Bs[0], tid);
     __syncthreads();
-    for (int kt = 0; kt < nkt; ++kt) {
-        const int buf = kt & 1;
-        const bool more = kt + 1 < nkt;
-        if (more) io.load(A, lda, B, ldb, M, N, K, m0, n0, (kt + 1) * GBK, tid);
-        mma_tile<T>(acc, As[buf], Bs[buf], wm, wn, lrow, lk);
-        if (more) {
-            io.store(As[buf ^ 1], Bs[buf ^ 1], tid);
-            __syncthreads();
+    for (int kt0 = 0; kt0 < nkt; kt0 += NS) {
+#pragma unroll
+        for (int u = 0; u < NS; ++u) {
+            const int kt = kt0 + u;
+            if (kt >= nkt) break;
+            // tile t lives in set t % NS: set (u + GPF) % NS held tile kt - 1
+            // (in LDS since the previous iteration) and takes tile kt + GPF
+            if (kt + GPF < nkt)
+                io[(u + GPF) % NS].load(A, lda, B, ldb, M, N, K, m0, n0, (kt + GPF) * GBK, tid);
+            mma_tile<T>(acc, As[kt & 1], Bs[kt & 1], wm, wn, lrow, lk);
+            if (kt + 1 < nkt) {
+                io[(u + 1) % NS].store(As[(kt + 1) & 1], Bs[(kt + 1) & 1], tid);
+                __syncthreads();
+            }
         }
     }
 }
 
 // One 128x128 tile of C.  The caller separates two tiles of one workgroup
 // by a barrier (the LDS buffers are reused).
-template <typename T, int TA, int TB>
+template <typename T, int TA, int TB, int GPF>
 FLX_DEV void gemm_tile(T (*As)[GBK][GBM + APAD], T (*Bs)[GBN][GBK + BPAD], int m0, int n0,
                        int M, int N, int K, T alpha, const T *__restrict__ A, long lda,
                        const T *__restrict__ B, long ldb, T beta, T *__restrict__ C, long ldc,
@@ -189,11 +220,11 @@ FLX_DEV void gemm_tile(T (*As)[GBK][GBM + APAD], T (*Bs)[GBN][GBK + BPAD], int m
     if (nkt > 0) {
         const bool fast = (m0 + GBM <= M) && (n0 + GBN <= N) && (K % GBK == 0) && vecA && vecB;
         if (fast)
-            gemm_mainloop<T, TA, TB, true>(acc, As, Bs, A, lda, B, ldb, M, N, K, m0, n0, nkt,
-                                           tid, wm, wn, lrow, lk);
+            gemm_mainloop<T, TA, TB, true, GPF>(acc, As, Bs, A, lda, B, ldb, M, N, K, m0, n0,
+                                                nkt, tid, wm, wn, lrow, lk);
         else
-            gemm_mainloop<T, TA, TB, false>(acc, As, Bs, A, lda, B, ldb, M, N, K, m0, n0, nkt,
-                                            tid, wm, wn, lrow, lk);
+            gemm_mainloop<T, TA, TB, false, GPF>(acc, As, Bs, A, lda, B, ldb, M, N, K, m0, n0,
+                                                 nkt, tid, wm, wn, lrow, lk);
     }
 
     // Epilogue: see MfmaAcc for which row each accumulator item holds.
@@ -240,7 +271,7 @@ FLX_DEV void gemm_tile_of(int t, int mt, int nt, int tiles, int &mi, int &ni)
 // Workgroup b takes dispatch slots b, b + grid, ... (one tile each unless
 // the launch caps the grid so that concurrent work on another stream keeps
 // some compute units).
-template <typename T, int TA, int TB>
+template <typename T, int TA, int TB, int GPF>
 __global__ __launch_bounds__(GNT) void gemm_mfma_kernel(
     int M, int N, int K, T alpha, const T *__restrict__ A, long lda,
     const T *__restrict__ B, long ldb, T beta, T *__restrict__ C, long ldc,
@@ -253,8 +284,8 @@ __global__ __launch_bounds__(GNT) void gemm_mfma_kernel(
         if (t != (int)blockIdx.x) __syncthreads();
         int mi, ni;
         gemm_tile_of(t, mt, nt, tiles, mi, ni);
-        gemm_tile<T, TA, TB>(As, Bs, mi * GBM, ni * GBN, M, N, K, alpha, A, lda, B,
-                             ldb, beta, C, ldc, vecA, vecB, blas_beta0);
+        gemm_tile<T, TA, TB, GPF>(As, Bs, mi * GBM, ni * GBN, M, N, K, alpha, A, lda, B,
+                                  ldb, beta, C, ldc, vecA, vecB, blas_beta0);
     }
 }
 
@@ -271,13 +302,16 @@ static hipError_t launch_gemm(hipStream_t s, int ta, int tb, int M, int N, int K
     const uintptr_t al = 16;
     int vecA = ((uintptr_t)A % al == 0) && (lda * (long)sizeof(T)) % al == 0;
     int vecB = ((uintptr_t)B % al == 0) && (ldb * (long)sizeof(T)) % al == 0;
-#define FLX_LG(a_, b_)                                                                   \
-    hipLaunchKernelGGL((gemm_mfma_kernel<T, a_, b_>), grid, dim3(GNT), 0, s, M, N, K,    \
+    // short K loops on at most one workgroup per compute unit (the LU's
+    // critical-path updates, K = nb) get the deep prefetch
+    const bool deep = K <= 256 && (int)grid.x <= 256;
+#define FLX_LG(a_, b_, p_)                                                               \
+    hipLaunchKernelGGL((gemm_mfma_kernel<T, a_, b_, p_>), grid, dim3(GNT), 0, s, M, N, K, \
                        alpha, A, lda, B, ldb, beta, C, ldc, vecA, vecB, blas_beta0, mt, tiles)
-    if (!ta && !tb) FLX_LG(0, 0);
-    else if (!ta && tb) FLX_LG(0, 1);
-    else if (ta && !tb) FLX_LG(1, 0);
-    else FLX_LG(1, 1);
+    if (!ta && !tb) { if (deep) FLX_LG(0, 0, 3); else FLX_LG(0, 0, 1); }
+    else if (!ta && tb) { if (deep) FLX_LG(0, 1, 3); else FLX_LG(0, 1, 1); }
+    else if (ta && !tb) { if (deep) FLX_LG(1, 0, 3); else FLX_LG(1, 0, 1); }
+    else { if (deep) FLX_LG(1, 1, 3); else FLX_LG(1, 1, 1); }
 #undef FLX_LG
     return hipGetLastError();
 }

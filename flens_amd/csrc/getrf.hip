@@ -896,7 +896,7 @@ hipError_t getrf_rm_f64(hipStream_t s, LuWork &wk, double *W, long ldw, int m, i
                       wk.ev_r;
     // the composed permutation, when requested, starts as the identity
     if (wk.perm) FLX_CHECK(iota(s, wk.perm, m));
-    bool rest_pending = false;
+    bool rest_pending = false, next_pending = false;
     for (int J = 0; J < mn; J += nb) {
         const int jb = (mn - J < nb) ? mn - J : nb;
         if (look) FLX_CHECK(inv_perm(s, wk.perm, wk.inv, m, wk.mv));
@@ -904,6 +904,12 @@ hipError_t getrf_rm_f64(hipStream_t s, LuWork &wk, double *W, long ldw, int m, i
             const int w = (J + jb - j < ib) ? J + jb - j : ib;
             FLX_CHECK(launch_panel(s, wk, W + (long)j * ldw + j, ldw, m - j, w, piv0 + j, j,
                                    info, ib));
+            // the block's other columns may still be taking the previous
+            // block's update on the side stream
+            if (next_pending) {
+                FLX_CHECK(hipStreamWaitEvent(s, wk.ev_n, 0));
+                next_pending = false;
+            }
             if (look && wk.stage && jb <= LU_STAGE_LD && w == PANEL_W) {
                 // the rest of the panel step in two launches: the stage, then
                 // interchange + U12 + in-block update together
@@ -965,19 +971,32 @@ hipError_t getrf_rm_f64(hipStream_t s, LuWork &wk, double *W, long ldw, int m, i
         FLX_CHECK(defer_moves<double>(s, wk.prof, W, ldw, m, n, J, jb, wk.perm, wk.inv, wk.mv,
                                       wk.moved));
         if (nr > 0) {
-            // the next block's columns first (they feed its panels) ...
+            // The next block's first panel columns on the critical path; its
+            // other columns, then the rest of the trailing matrix, on the side
+            // stream beside that panel (the next block's fused steps wait for
+            // the former, its deferred moves for the latter).
             const int nl = nr < nb ? nr : nb;
-            FLX_CHECK(trsm_rm_llu(s, wk.prof, (const double *)W + (long)J * ldw + J, ldw,
-                                  W + (long)J * ldw + J + jb, ldw, jb, nl));
-            FLX_CHECK(rm_update(s, wk.prof, W, ldw, J + jb, m - J - jb, J + jb, nl, J, jb));
-            // ... the rest on the side stream, beside the next block's panels
+            const bool split = wk.ev_n && nl > PANEL_W;
+            const int nf = split ? PANEL_W : nl;
+            const double *L11 = (const double *)W + (long)J * ldw + J;
             const int c0 = J + jb + nl;
-            if (n > c0) {
+            if (split || n > c0) {
                 FLX_CHECK(hipEventRecord(wk.ev_a, s));
                 FLX_CHECK(hipStreamWaitEvent(wk.side, wk.ev_a, 0));
-                FLX_CHECK(trsm_rm_llu(wk.side, wk.prof, (const double *)W + (long)J * ldw + J,
-                                      ldw, W + (long)J * ldw + c0, ldw, jb, n - c0,
-                                      wk.side_groups));
+            }
+            FLX_CHECK(trsm_rm_llu(s, wk.prof, L11, ldw, W + (long)J * ldw + J + jb, ldw, jb, nf));
+            FLX_CHECK(rm_update(s, wk.prof, W, ldw, J + jb, m - J - jb, J + jb, nf, J, jb));
+            if (split) {
+                FLX_CHECK(trsm_rm_llu(wk.side, wk.prof, L11, ldw, W + (long)J * ldw + J + jb + nf,
+                                      ldw, jb, nl - nf, wk.side_groups));
+                FLX_CHECK(rm_update(wk.side, wk.prof, W, ldw, J + jb, m - J - jb, J + jb + nf,
+                                    nl - nf, J, jb, wk.side_groups));
+                FLX_CHECK(hipEventRecord(wk.ev_n, wk.side));
+                next_pending = true;
+            }
+            if (n > c0) {
+                FLX_CHECK(trsm_rm_llu(wk.side, wk.prof, L11, ldw, W + (long)J * ldw + c0, ldw, jb,
+                                      n - c0, wk.side_groups));
                 FLX_CHECK(rm_update(wk.side, wk.prof, W, ldw, J + jb, m - J - jb, c0, n - c0, J,
                                     jb, wk.side_groups));
                 FLX_CHECK(hipEventRecord(wk.ev_r, wk.side));
@@ -985,6 +1004,8 @@ hipError_t getrf_rm_f64(hipStream_t s, LuWork &wk, double *W, long ldw, int m, i
             }
         }
     }
+    // (columns right of the last block, when n > m, may still be on the side stream)
+    if (next_pending) FLX_CHECK(hipStreamWaitEvent(s, wk.ev_n, 0));
     if (rest_pending) FLX_CHECK(hipStreamWaitEvent(s, wk.ev_r, 0));
     return hipSuccess;
 }
