@@ -227,7 +227,24 @@ FLX_DEV void gemm_tile(T (*As)[GBK][GBM + APAD], T (*Bs)[GBN][GBK + BPAD], int m
                                                  nkt, tid, wm, wn, lrow, lk);
     }
 
-    // Epilogue: see MfmaAcc for which row each accumulator item holds.
+    // Epilogue: see MfmaAcc for which row each accumulator item holds.  The
+    // tile's C entries are all requested before any is used (one memory
+    // latency, not one per entry); generic FLENS order: C*beta first (0*C
+    // for beta=0); BLAS convention (blas_beta0): beta=0 never reads C.
+    const bool readc = !(blas_beta0 && beta == T(0));
+    T cv[GMI][GNJ][4];
+#pragma unroll
+    for (int i = 0; i < GMI; ++i)
+#pragma unroll
+        for (int j = 0; j < GNJ; ++j) {
+            const int col = n0 + wn * (GNJ * 16) + j * 16 + lrow;
+#pragma unroll
+            for (int q = 0; q < 4; ++q) {
+                const int row = m0 + wm * (GMI * 16) + i * 16 + MfmaAcc<T>::row(lk, q);
+                const bool ok = readc && col < N && row < M;
+                cv[i][j][q] = ok ? C[(long)col * ldc + row] : T(0);
+            }
+        }
 #pragma unroll
     for (int i = 0; i < GMI; ++i)
 #pragma unroll
@@ -239,10 +256,8 @@ FLX_DEV void gemm_tile(T (*As)[GBK][GBM + APAD], T (*Bs)[GBN][GBK + BPAD], int m
             for (int q = 0; q < 4; ++q) {
                 const int row = m0 + wm * (GMI * 16) + i * 16 + MfmaAcc<T>::row(lk, q);
                 if (row < M) {
-                    // generic FLENS order: C*beta first (0*C for beta=0);
-                    // BLAS convention (blas_beta0): beta=0 never reads C.
-                    T cv = (blas_beta0 && beta == T(0)) ? T(0) : cp[row] * beta;
-                    cp[row] = (nkt > 0) ? cv + alpha * acc[i][j][q] : cv;
+                    const T c = readc ? cv[i][j][q] * beta : T(0);
+                    cp[row] = (nkt > 0) ? c + alpha * acc[i][j][q] : c;
                 }
             }
         }
@@ -289,6 +304,53 @@ __global__ __launch_bounds__(GNT) void gemm_mfma_kernel(
     }
 }
 
+// Narrow C with short K (the LU's critical-path updates: M <= 128 columns of
+// the row-major matrix, K = nb): the tiled kernel would leave most compute
+// units idle and, for M < 128, most of each tile's MFMAs padding.  One wave
+// per 16 x 16 block of C: every operand element the block needs is requested
+// from global memory at once (no LDS, no barrier), then K / 4 chained
+// v_mfma_f64_16x16x4f64 in the same k order as the tiled kernel, so the
+// result is that kernel's, bit for bit, for K a multiple of 16.
+constexpr int SK_MMAX = 128, SK_KMAX = 128;
+__global__ __launch_bounds__(64) void gemm_skinny_kernel(int M, int N, int K, double alpha,
+                                                         const double *__restrict__ A, long lda,
+                                                         const double *__restrict__ B, long ldb,
+                                                         double beta, double *__restrict__ C,
+                                                         long ldc, int blas_beta0, int mt)
+{
+    const int lane = threadIdx.x, lrow = lane & 15, lk = lane >> 4;
+    const int m0 = (blockIdx.x % mt) * 16, n0 = (blockIdx.x / mt) * 16;
+    const int nks = (alpha == 0.0) ? 0 : (K + 3) / 4;
+    const int m = m0 + lrow, n = n0 + lrow;
+    double a[SK_KMAX / 4], b[SK_KMAX / 4];
+#pragma unroll
+    for (int st = 0; st < SK_KMAX / 4; ++st) {
+        const int k = 4 * st + lk;
+        a[st] = (st < nks && m < M && k < K) ? A[m + (long)k * lda] : 0.0;
+        b[st] = (st < nks && n < N && k < K) ? B[k + (long)n * ldb] : 0.0;
+    }
+    const bool readc = !(blas_beta0 && beta == 0.0);
+    double cv[4];
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+        const int row = m0 + MfmaAcc<double>::row(lk, q);
+        cv[q] = (readc && row < M && n < N) ? C[row + (long)n * ldc] : 0.0;
+    }
+    double4_t acc = {0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+    for (int st = 0; st < SK_KMAX / 4; ++st)
+        if (st < nks) acc = mfma16x16x4(a[st], b[st], acc);
+    if (n >= N) return;
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+        const int row = m0 + MfmaAcc<double>::row(lk, q);
+        if (row < M) {
+            const double c = readc ? cv[q] * beta : 0.0;
+            C[row + (long)n * ldc] = (nks > 0) ? c + alpha * acc[q] : c;
+        }
+    }
+}
+
 template <typename T>
 static hipError_t launch_gemm(hipStream_t s, int ta, int tb, int M, int N, int K, T alpha,
                               const T *A, long lda, const T *B, long ldb, T beta, T *C,
@@ -320,6 +382,14 @@ hipError_t gemm_f64(hipStream_t s, int ta, int tb, int M, int N, int K, double a
                     const double *A, long lda, const double *B, long ldb, double beta,
                     double *C, long ldc, int blas_beta0, int max_groups)
 {
+    if (!ta && !tb && M > 0 && N > 0 && M <= SK_MMAX && K <= SK_KMAX && max_groups <= 0) {
+        const int mt = (M + 15) / 16, nt = (N + 15) / 16;
+        if ((long)mt * nt <= INT_MAX) {
+            hipLaunchKernelGGL(gemm_skinny_kernel, dim3(mt * nt), dim3(64), 0, s, M, N, K, alpha,
+                               A, lda, B, ldb, beta, C, ldc, blas_beta0, mt);
+            return hipGetLastError();
+        }
+    }
     return launch_gemm<double>(s, ta, tb, M, N, K, alpha, A, lda, B, ldb, beta, C, ldc,
                                blas_beta0, max_groups);
 }
