@@ -528,18 +528,21 @@ def selector_section(train_batches: int = 2, train_count: int = 12500, eval_coun
     return res
 
 
-# HBM bytes per launch of the trailing-update GEMM in the default workload
+# HBM bytes per launch of the trailing-update GEMMs in the default workload
 # (n = 8192, nrhs = 16, nb = 128, ib = 32): rocprofv3 --pmc FETCH_SIZE and
-# WRITE_SIZE over 875 launches, averaged, scaled by what the same counters
-# read for a transpose of known size in the same run (fetched = 2.0 x
-# FETCH_SIZE, written = 1.0 x WRITE_SIZE): (2 x 88665.4 + 87862.1) KiB.
-LU8192_GEMM_TRAFFIC_BYTES = (2.0 * 88665.4 + 87862.1) * 1024.0
+# WRITE_SIZE summed over the GEMM kernels of 7 factorisations (188 update
+# launches each: the next block's first panel columns on the one-wave-per-
+# block kernel, its other columns and the rest on the tiled kernel), per
+# launch, scaled by what the same counters read for a transpose of known
+# size in the same run (fetched = 2.0 x FETCH_SIZE, written = 1.0 x
+# WRITE_SIZE): (2 x 42110.9 + 58748.7) KiB.
+LU8192_GEMM_TRAFFIC_BYTES = (2.0 * 42110.9 + 58748.7) * 1024.0
 # HBM bytes per panel launch in the default workload: rocprofv3 --pmc
-# FETCH_SIZE and WRITE_SIZE over 1792 launches, averaged (1732.55 and 2200.30
+# FETCH_SIZE and WRITE_SIZE over 1792 launches, averaged (1769.99 and 2200.30
 # KiB), scaled by what the same counters read for the 1 GiB streaming copy of
-# the peak probe in the same run (fetched = 1.93596 x FETCH_SIZE, written =
-# 0.976495 x WRITE_SIZE).
-LU8192_PANEL_TRAFFIC_BYTES = (1.93596 * 1732.55 + 0.976495 * 2200.30) * 1024.0
+# the peak probe in the same run (fetched = 1.93620 x FETCH_SIZE, written =
+# 0.976462 x WRITE_SIZE).
+LU8192_PANEL_TRAFFIC_BYTES = (1.93620 * 1769.99 + 0.976462 * 2200.30) * 1024.0
 
 
 def main():
@@ -724,7 +727,7 @@ def main():
         "roofline_gemm": {"bound": "mfma", "achieved": g_ach, "peak": PEAK_FP64_MFMA,
                           "unit": "TFLOP/s", "frac": g_ach / PEAK_FP64_MFMA,
                           "traffic": LU8192_GEMM_TRAFFIC_BYTES if headline else None,
-                          "kernel": "gemm_mfma_kernel<double,0,0> (trailing update)",
+                          "kernel": "gemm_mfma_kernel<double,0,0,P> + gemm_skinny_kernel (trailing updates)",
                           "launches": g["launches"], "avg_launch_us": 1e3 * g["ms"] / max(1, g["launches"]),
                           "frac_of_probe": g_ach / peaks["fp64_mfma_tflops"]},
         "peaks_measured": peaks,
