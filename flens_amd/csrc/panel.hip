@@ -549,15 +549,23 @@ __global__ __launch_bounds__(NT) void panel_kernel(
             s_ipos[tid % IO_R] = rid[k] < mp ? pos[k] : -1;
         }
         __syncthreads();
+        // every LDS read first (unconditional, in the staged block), then
+        // the stores: one LDS latency per chunk, not one per pair
+        int dps[IO_PER];
+        double2 pv[IO_PER];
 #pragma unroll
         for (int i = 0; i < IO_PER; ++i) {
             const int p = tid + NT * i, r = p >> 4, c2 = (p & 15) * 2;
-            const int dp = s_ipos[r];
-            if (dp >= 0 && c2 < w) {
-                const double *sr = s_io + r * OUT_LD + c2;
-                double *dst = P + (long)dp * ldw + c2;
-                if (c2 + 1 < w) *(double2 *)dst = double2(sr[0], sr[1]);
-                else dst[0] = sr[0];
+            dps[i] = s_ipos[r];
+            pv[i] = double2(s_io[r * OUT_LD + c2], s_io[r * OUT_LD + c2 + 1]);
+        }
+#pragma unroll
+        for (int i = 0; i < IO_PER; ++i) {
+            const int p = tid + NT * i, c2 = (p & 15) * 2;
+            if (dps[i] >= 0 && c2 < w) {
+                double *dst = P + (long)dps[i] * ldw + c2;
+                if (c2 + 1 < w) *(double2 *)dst = pv[i];
+                else dst[0] = pv[i].x;
             }
         }
         __syncthreads();
