@@ -538,11 +538,11 @@ def selector_section(train_batches: int = 2, train_count: int = 12500, eval_coun
 # WRITE_SIZE): (2 x 42110.9 + 58748.7) KiB.
 LU8192_GEMM_TRAFFIC_BYTES = (2.0 * 42110.9 + 58748.7) * 1024.0
 # HBM bytes per panel launch in the default workload: rocprofv3 --pmc
-# FETCH_SIZE and WRITE_SIZE over 1792 launches, averaged (1769.99 and 2200.30
+# FETCH_SIZE and WRITE_SIZE over 1792 launches, averaged (1292.55 and 1180.30
 # KiB), scaled by what the same counters read for the 1 GiB streaming copy of
-# the peak probe in the same run (fetched = 1.93620 x FETCH_SIZE, written =
-# 0.976462 x WRITE_SIZE).
-LU8192_PANEL_TRAFFIC_BYTES = (1.93620 * 1769.99 + 0.976462 * 2200.30) * 1024.0
+# the peak probe in the same run (fetched = 1.93618 x FETCH_SIZE, written =
+# 0.976469 x WRITE_SIZE).
+LU8192_PANEL_TRAFFIC_BYTES = (1.93618 * 1292.55 + 0.976469 * 1180.30) * 1024.0
 
 
 def main():

@@ -278,7 +278,7 @@ int dgetrf_dev(flx_context *h, int m, int n, double *A, int lda, int *ipiv, int 
     h->wk.mv = (int *)h->lu_mv.p;
     FLX_TRY(ensure(h, h->lu_stage, sizeof(double) * LU_STAGE_DOUBLES));
     FLX_TRY(ensure(h, h->lu_sdst, sizeof(int) * (LU_STAGE_TOP + 1)));
-    FLX_TRY(ensure(h, h->lu_sperm, sizeof(int) * LU_STAGE_TOP));
+    FLX_TRY(ensure(h, h->lu_sperm, sizeof(int) * 2 * LU_STAGE_TOP));
     h->wk.stage = (double *)h->lu_stage.p;
     h->wk.sdst = (int *)h->lu_sdst.p;
     h->wk.sperm = (int *)h->lu_sperm.p;

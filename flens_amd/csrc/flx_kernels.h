@@ -86,7 +86,8 @@ struct LuWork {
     // doubles), their destinations (+ the count at [LU_STAGE_TOP]) and perm
     // entries; null: separate interchange / solve / update launches
     double *stage = nullptr;
-    int *sdst = nullptr, *sperm = nullptr;
+    int *sdst = nullptr;
+    int *sperm = nullptr;       // 2 * LU_STAGE_TOP: moved rows' perm entries, then their sources
 };
 constexpr int LU_BLOCK_MOVES = 2 * 256;   // rows one outer block can move
 constexpr int LU_STAGE_TOP = 128;         // stage rows: list sources, then the top rows
@@ -118,8 +119,10 @@ inline size_t lu_lsc_doubles(int m) { return (size_t)(m > 0 ? m : 1) * 32; }
 inline int panel_groups(int mp) { return (mp + PANEL_ROWS_PER_WG - 1) / PANEL_ROWS_PER_WG; }
 // Panel factorisation of the mp x w row-major panel P (w <= 32); piv gets
 // 0-based absolute pivot rows, the composed interchanges go to wk.list.
+// defer_l: write only the pivot rows; the other rows' multipliers stay in
+// wk.lsc for the fused panel step (lu_apply_kernel) to place
 hipError_t launch_panel(hipStream_t s, LuWork &wk, double *P, long ldw, int mp, int w,
-                        int *piv, int piv_base, int *info, int ib);
+                        int *piv, int piv_base, int *info, int ib, int defer_l = 0);
 // Factor the m x n row-major matrix W (leading dimension ldw) in place.
 // piv0 receives 0-based absolute pivot rows (length min(m,n)); info receives
 // the 1-based first zero pivot column (INT_MAX if none; caller initialises).

@@ -243,6 +243,7 @@ __global__ __launch_bounds__(256) void lu_stage_kernel(const double *__restrict_
         if (t < cnt) {
             sdst[t] = s_dst[t];
             sperm[t] = perm ? perm[s_src[t]] : 0;
+            sperm[LU_STAGE_TOP + t] = s_src[t];   // the moved row's original position
         }
         if (t == 0) sdst[LU_STAGE_TOP] = cnt;
         if (t == 0 && moved) atomicAdd(moved, (unsigned long long)cnt * (jb - w));
@@ -274,7 +275,9 @@ __global__ __launch_bounds__(256) void lu_apply_kernel(double *__restrict__ W, l
                                                        const double *__restrict__ stage,
                                                        const int *__restrict__ sdst,
                                                        const int *__restrict__ sperm,
-                                                       int *__restrict__ perm)
+                                                       int *__restrict__ perm,
+                                                       const double *__restrict__ lsc,
+                                                       int defer_l)
 {
     static_assert(LA_ROWS * CR % 256 == 0 && CR % LA_TPR == 0 && CR <= CR, "apply tile");
     constexpr int NQ = LA_ROWS * CR / 256;   // right-part elements per thread
@@ -284,6 +287,7 @@ __global__ __launch_bounds__(256) void lu_apply_kernel(double *__restrict__ W, l
     __shared__ double s_a[LA_ROWS][CR + 1];   // the tile's right parts
     __shared__ double sx[LA_ROWS][33];           // the tile's L21 rows
     __shared__ int s_slot[LA_ROWS];
+    __shared__ int s_rsrc[LA_ROWS];              // the row's original position (moved rows)
     const int t = threadIdx.x;
     const int jo = j - J;
     const int cr = jb - jo - w;                  // columns right of the panel
@@ -294,6 +298,7 @@ __global__ __launch_bounds__(256) void lu_apply_kernel(double *__restrict__ W, l
     // rows (panel columns, final) and this thread's column of the top rows.
     const int cnt = sdst[LU_STAGE_TOP];
     const int sd = sdst[t < LU_STAGE_TOP ? t : 0];
+    const int ss = sperm[LU_STAGE_TOP + (t < LU_STAGE_TOP ? t : 0)];
     const bool useinv = stage[LU_STAGE_INV + 32 * 32] != 0.0;
     double lv[4], iv[4], lx[4];
 #pragma unroll
@@ -302,7 +307,7 @@ __global__ __launch_bounds__(256) void lu_apply_kernel(double *__restrict__ W, l
         lv[q] = (r < w) ? W[(long)(j + (r < 32 ? r : 0)) * ldw + j + k] : 0.0;
         iv[q] = stage[LU_STAGE_INV + e];
         const int rr = e >> 5, kk = e & 31, ii = i0 + rr;
-        lx[q] = (ii < m && kk < w && ii - j >= w) ? W[(long)ii * ldw + j + kk] : 0.0;
+        lx[q] = (!defer_l && ii < m && kk < w && ii - j >= w) ? W[(long)ii * ldw + j + kk] : 0.0;
     }
     double x[32];
 #pragma unroll
@@ -319,9 +324,28 @@ __global__ __launch_bounds__(256) void lu_apply_kernel(double *__restrict__ W, l
     __syncthreads();
     if (t < cnt) {
         const int d = sd - i0;
-        if (d >= 0 && d < LA_ROWS) s_slot[d] = t;
+        if (d >= 0 && d < LA_ROWS) {
+            s_slot[d] = t;
+            s_rsrc[d] = ss;
+        }
     }
     __syncthreads();
+    if (defer_l) {
+        // the panel left the tile's multiplier rows in lsc (column-major by
+        // original row, mp = m - j rows): row i's are those of the row that
+        // moved into it, or its own
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            const int e = t + 256 * q, rr = e >> 5, kk = e & 31, ii = i0 + rr;
+            const int src = s_slot[rr] >= 0 ? s_rsrc[rr] : ii;
+            lx[q] = (ii < m && kk < w && ii - j >= w) ? lsc[(long)kk * (m - j) + (src - j)] : 0.0;
+        }
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            const int e = t + 256 * q;
+            sx[e >> 5][e & 31] = lx[q];
+        }
+    }
     // the tile's right parts (moved rows from the stage), every load issued
     // before any is used
     {
@@ -396,9 +420,17 @@ __global__ __launch_bounds__(256) void lu_apply_kernel(double *__restrict__ W, l
         }
         if (part == 0 && perm) perm[i] = sperm[slot];
     }
+    const int p = i - j;
+    if (defer_l && p >= w) {
+        // the row's multipliers, which the panel left in lsc
+#pragma unroll
+        for (int q = 0; q < 32 / LA_TPR; ++q) {
+            const int c = part + LA_TPR * q;
+            if (c < w) row[jo + c] = sx[r][c];
+        }
+    }
     if (cr <= 0) return;
     double *rr = row + jo + w;
-    const int p = i - j;
     if (p < w) {
         for (int c = part; c < cr; c += LA_TPR) rr[c] = s12[p][c];
         return;
@@ -902,15 +934,18 @@ hipError_t getrf_rm_f64(hipStream_t s, LuWork &wk, double *W, long ldw, int m, i
         if (look) FLX_CHECK(inv_perm(s, wk.perm, wk.inv, m, wk.mv));
         for (int j = J; j < J + jb; j += ib) {
             const int w = (J + jb - j < ib) ? J + jb - j : ib;
+            // with the fused step, the panel writes only its pivot rows and
+            // the step places the other rows' multipliers
+            const bool fused = look && wk.stage && jb <= LU_STAGE_LD && w == PANEL_W;
             FLX_CHECK(launch_panel(s, wk, W + (long)j * ldw + j, ldw, m - j, w, piv0 + j, j,
-                                   info, ib));
+                                   info, ib, fused ? 1 : 0));
             // the block's other columns may still be taking the previous
             // block's update on the side stream
             if (next_pending) {
                 FLX_CHECK(hipStreamWaitEvent(s, wk.ev_n, 0));
                 next_pending = false;
             }
-            if (look && wk.stage && jb <= LU_STAGE_LD && w == PANEL_W) {
+            if (fused) {
                 // the rest of the panel step in two launches: the stage, then
                 // interchange + U12 + in-block update together
                 {
@@ -928,15 +963,15 @@ hipError_t getrf_rm_f64(hipStream_t s, LuWork &wk, double *W, long ldw, int m, i
                 if (jb - PANEL_W <= 32)
                     hipLaunchKernelGGL(lu_apply_kernel<32>, ag, dim3(256), 0, s, W, ldw, m, J, jb,
                                        j, w, (const double *)wk.stage, (const int *)wk.sdst,
-                                       (const int *)wk.sperm, wk.perm);
+                                       (const int *)wk.sperm, wk.perm, (const double *)wk.lsc, 1);
                 else if (jb - PANEL_W <= 96)
                     hipLaunchKernelGGL(lu_apply_kernel<96>, ag, dim3(256), 0, s, W, ldw, m, J, jb,
                                        j, w, (const double *)wk.stage, (const int *)wk.sdst,
-                                       (const int *)wk.sperm, wk.perm);
+                                       (const int *)wk.sperm, wk.perm, (const double *)wk.lsc, 1);
                 else
                     hipLaunchKernelGGL(lu_apply_kernel<LA_CR>, ag, dim3(256), 0, s, W, ldw, m, J,
                                        jb, j, w, (const double *)wk.stage, (const int *)wk.sdst,
-                                       (const int *)wk.sperm, wk.perm);
+                                       (const int *)wk.sperm, wk.perm, (const double *)wk.lsc, 1);
                 FLX_CHECK(hipGetLastError());
                 continue;
             }

@@ -176,7 +176,8 @@ template <int NT, int RPT, int GMAX>
 __global__ __launch_bounds__(NT) void panel_kernel(
     double *__restrict__ P, long ldw, int mp, int w, uint64_t *__restrict__ ex,
     uint32_t gen, double *__restrict__ lsc, int *__restrict__ piv, int piv_base,
-    int *__restrict__ list, int *__restrict__ info, int info_base, int *__restrict__ err)
+    int *__restrict__ list, int *__restrict__ info, int info_base, int *__restrict__ err,
+    int defer_l)
 {
     constexpr int W = PANEL_W;
     constexpr int NWAVE = NT / 64;
@@ -520,55 +521,79 @@ __global__ __launch_bounds__(NT) void panel_kernel(
     // start + e (a pivot row: start = its column; a row left below the
     // panel's pivots: start = mn); record the moved rows
     int *mylist = list + (size_t)g * PANEL_LIST_SLICE;
-    // the multipliers are loaded first, all at once (unconditional, in
-    // bounds: lsc holds W columns of mp rows), so the epilogue waits for
-    // one memory latency instead of one per column
-    double mv[RPT][W];
+    if (defer_l) {
+        // Only the pivot rows (final positions below mn) are written here;
+        // every other row of the panel is its mn multipliers, which the
+        // caller's next kernel copies from lsc (column-major, by original
+        // row) into the row's final position.  A pivot row's multipliers
+        // are requested at once, then stored with the row's registers.
 #pragma unroll
-    for (int k = 0; k < RPT; ++k) {
-        const long r = rid[k] < mp ? rid[k] : 0;
+        for (int k = 0; k < RPT; ++k) {
+            if (rid[k] < mp && done[k]) {
+                double pm[W];
 #pragma unroll
-        for (int c = 0; c < W; ++c) mv[k][c] = lsc[(long)c * mp + r];
-    }
-    // the final rows leave through LDS as they came in: each thread stages
-    // its row, sixteen lanes store one row as 16-byte pairs
+                for (int c = 0; c < W; ++c) pm[c] = lsc[(long)c * mp + rid[k]];
+                double *dst = P + (long)pos[k] * ldw;
+                const int start = pos[k];
 #pragma unroll
-    for (int q = 0; q < IO_CHUNKS; ++q) {
-        const int k = q / (NT / IO_R), h = q % (NT / IO_R);
-        if (tid / IO_R == h) {
-            // multipliers at [0, W), then the registers at [start, start + W)
-            // over them: unconditional stores, the later ones win
-            // (start <= W: a pivot row's column, mn, or 0 for a slot
-            // beyond the panel, whose row is never stored)
-            double *st = s_io + (tid % IO_R) * OUT_LD;
-            const int start = rid[k] >= mp ? 0 : done[k] ? pos[k] : mn;
+                for (int c = 0; c < W; ++c)
+                    if (c < start) dst[c] = pm[c];
 #pragma unroll
-            for (int c = 0; c < W; ++c) st[c] = mv[k][c];
-#pragma unroll
-            for (int e = 0; e < W; ++e) st[start + e] = row[k][e];
-            s_ipos[tid % IO_R] = rid[k] < mp ? pos[k] : -1;
-        }
-        __syncthreads();
-        // every LDS read first (unconditional, in the staged block), then
-        // the stores: one LDS latency per chunk, not one per pair
-        int dps[IO_PER];
-        double2 pv[IO_PER];
-#pragma unroll
-        for (int i = 0; i < IO_PER; ++i) {
-            const int p = tid + NT * i, r = p >> 4, c2 = (p & 15) * 2;
-            dps[i] = s_ipos[r];
-            pv[i] = double2(s_io[r * OUT_LD + c2], s_io[r * OUT_LD + c2 + 1]);
-        }
-#pragma unroll
-        for (int i = 0; i < IO_PER; ++i) {
-            const int p = tid + NT * i, c2 = (p & 15) * 2;
-            if (dps[i] >= 0 && c2 < w) {
-                double *dst = P + (long)dps[i] * ldw + c2;
-                if (c2 + 1 < w) *(double2 *)dst = pv[i];
-                else dst[0] = pv[i].x;
+                for (int e = 0; e < W; ++e)
+                    if (start + e < w) dst[start + e] = row[k][e];
             }
         }
-        __syncthreads();
+    } else {
+        // the multipliers are loaded first, all at once (unconditional, in
+        // bounds: lsc holds W columns of mp rows), so the epilogue waits for
+        // one memory latency instead of one per column
+        double mv[RPT][W];
+#pragma unroll
+        for (int k = 0; k < RPT; ++k) {
+            const long r = rid[k] < mp ? rid[k] : 0;
+#pragma unroll
+            for (int c = 0; c < W; ++c) mv[k][c] = lsc[(long)c * mp + r];
+        }
+        // the final rows leave through LDS as they came in: each thread stages
+        // its row, sixteen lanes store one row as 16-byte pairs
+#pragma unroll
+        for (int q = 0; q < IO_CHUNKS; ++q) {
+            const int k = q / (NT / IO_R), h = q % (NT / IO_R);
+            if (tid / IO_R == h) {
+                // multipliers at [0, W), then the registers at [start, start + W)
+                // over them: unconditional stores, the later ones win
+                // (start <= W: a pivot row's column, mn, or 0 for a slot
+                // beyond the panel, whose row is never stored)
+                double *st = s_io + (tid % IO_R) * OUT_LD;
+                const int start = rid[k] >= mp ? 0 : done[k] ? pos[k] : mn;
+#pragma unroll
+                for (int c = 0; c < W; ++c) st[c] = mv[k][c];
+#pragma unroll
+                for (int e = 0; e < W; ++e) st[start + e] = row[k][e];
+                s_ipos[tid % IO_R] = rid[k] < mp ? pos[k] : -1;
+            }
+            __syncthreads();
+            // every LDS read first (unconditional, in the staged block), then
+            // the stores: one LDS latency per chunk, not one per pair
+            int dps[IO_PER];
+            double2 pv[IO_PER];
+#pragma unroll
+            for (int i = 0; i < IO_PER; ++i) {
+                const int p = tid + NT * i, r = p >> 4, c2 = (p & 15) * 2;
+                dps[i] = s_ipos[r];
+                pv[i] = double2(s_io[r * OUT_LD + c2], s_io[r * OUT_LD + c2 + 1]);
+            }
+#pragma unroll
+            for (int i = 0; i < IO_PER; ++i) {
+                const int p = tid + NT * i, c2 = (p & 15) * 2;
+                if (dps[i] >= 0 && c2 < w) {
+                    double *dst = P + (long)dps[i] * ldw + c2;
+                    if (c2 + 1 < w) *(double2 *)dst = pv[i];
+                    else dst[0] = pv[i].x;
+                }
+            }
+            __syncthreads();
+        }
     }
 #pragma unroll
     for (int k = 0; k < RPT; ++k) {
@@ -601,7 +626,7 @@ __global__ void zero_u64_kernel(uint64_t *p, size_t n)
 
 hipError_t launch_panel(hipStream_t s, LuWork &wk, double *P, long ldw,
                                int mp, int w, int *piv, int piv_base, int *info,
-                               int ib)
+                               int ib, int defer_l)
 {
     if (mp > PANEL_ROWS_MAX || !wk.lsc) return hipErrorInvalidValue;
     wk.gen = (wk.gen + 1) & 0xffffffu;
@@ -623,17 +648,17 @@ hipError_t launch_panel(hipStream_t s, LuWork &wk, double *P, long ldw,
     const int G = panel_groups(mp);
     if (G <= 8)
         hipLaunchKernelGGL((panel_kernel<NT, RPT, 8>), dim3(G), dim3(NT), 0, s, P, ldw, mp, w,
-                           wk.exch, wk.gen, wk.lsc, piv, piv_base, wk.list, info, piv_base, wk.flags);
+                           wk.exch, wk.gen, wk.lsc, piv, piv_base, wk.list, info, piv_base, wk.flags, defer_l);
     else if (G <= 16)
         hipLaunchKernelGGL((panel_kernel<NT, RPT, 16>), dim3(G), dim3(NT), 0, s, P, ldw, mp, w,
-                           wk.exch, wk.gen, wk.lsc, piv, piv_base, wk.list, info, piv_base, wk.flags);
+                           wk.exch, wk.gen, wk.lsc, piv, piv_base, wk.list, info, piv_base, wk.flags, defer_l);
     else if (G <= 32)
         hipLaunchKernelGGL((panel_kernel<NT, RPT, 32>), dim3(G), dim3(NT), 0, s, P, ldw, mp, w,
-                           wk.exch, wk.gen, wk.lsc, piv, piv_base, wk.list, info, piv_base, wk.flags);
+                           wk.exch, wk.gen, wk.lsc, piv, piv_base, wk.list, info, piv_base, wk.flags, defer_l);
     else
         hipLaunchKernelGGL((panel_kernel<NT, RPT, LU_N_MAX / PANEL_ROWS_PER_WG>), dim3(G),
                            dim3(NT), 0, s, P, ldw, mp, w, wk.exch, wk.gen, wk.lsc, piv,
-                           piv_base, wk.list, info, piv_base, wk.flags);
+                           piv_base, wk.list, info, piv_base, wk.flags, defer_l);
     return hipGetLastError();
 }
 
