@@ -304,51 +304,75 @@ __global__ __launch_bounds__(GNT) void gemm_mfma_kernel(
     }
 }
 
-// Narrow C with short K (the LU's critical-path updates: M <= 128 columns of
-// the row-major matrix, K = nb): the tiled kernel would leave most compute
-// units idle and, for M < 128, most of each tile's MFMAs padding.  One wave
-// per 16 x 16 block of C: every operand element the block needs is requested
+// Narrow C with short K (the LU's critical-path updates: M columns of the
+// row-major matrix, K = nb): the tiled kernel would leave most compute units
+// idle and, for M < 128, most of each tile's MFMAs padding.  One wave per
+// 16 x 16 block of C: every operand element the block needs is requested
 // from global memory at once (no LDS, no barrier), then K / 4 chained
-// v_mfma_f64_16x16x4f64 in the same k order as the tiled kernel, so the
-// result is that kernel's, bit for bit, for K a multiple of 16.
-constexpr int SK_MMAX = 128, SK_KMAX = 128;
-__global__ __launch_bounds__(64) void gemm_skinny_kernel(int M, int N, int K, double alpha,
-                                                         const double *__restrict__ A, long lda,
-                                                         const double *__restrict__ B, long ldb,
-                                                         double beta, double *__restrict__ C,
-                                                         long ldc, int blas_beta0, int mt)
+// 16x16x4 MFMAs in the same k order as the tiled kernel, so the result is
+// that kernel's, bit for bit, for K a multiple of 16.  FP64: M, K <= 128;
+// FP32: M, K <= 256.
+template <typename T>
+struct SkinnyLimits;
+template <> struct SkinnyLimits<double> { static constexpr int M = 128, K = 128; };
+template <> struct SkinnyLimits<float> { static constexpr int M = 256, K = 256; };
+
+template <typename T>
+__global__ __launch_bounds__(64) void gemm_skinny_kernel(int M, int N, int K, T alpha,
+                                                         const T *__restrict__ A, long lda,
+                                                         const T *__restrict__ B, long ldb,
+                                                         T beta, T *__restrict__ C, long ldc,
+                                                         int blas_beta0, int mt)
 {
+    typedef typename MfmaAcc<T>::type acc_t;
+    constexpr int KS = SkinnyLimits<T>::K / 4;   // MFMA steps at most
     const int lane = threadIdx.x, lrow = lane & 15, lk = lane >> 4;
     const int m0 = (blockIdx.x % mt) * 16, n0 = (blockIdx.x / mt) * 16;
-    const int nks = (alpha == 0.0) ? 0 : (K + 3) / 4;
+    const int nks = (alpha == T(0)) ? 0 : (K + 3) / 4;
     const int m = m0 + lrow, n = n0 + lrow;
-    double a[SK_KMAX / 4], b[SK_KMAX / 4];
+    T a[KS], b[KS];
 #pragma unroll
-    for (int st = 0; st < SK_KMAX / 4; ++st) {
+    for (int st = 0; st < KS; ++st) {
         const int k = 4 * st + lk;
-        a[st] = (st < nks && m < M && k < K) ? A[m + (long)k * lda] : 0.0;
-        b[st] = (st < nks && n < N && k < K) ? B[k + (long)n * ldb] : 0.0;
+        a[st] = (st < nks && m < M && k < K) ? A[m + (long)k * lda] : T(0);
+        b[st] = (st < nks && n < N && k < K) ? B[k + (long)n * ldb] : T(0);
     }
-    const bool readc = !(blas_beta0 && beta == 0.0);
-    double cv[4];
+    const bool readc = !(blas_beta0 && beta == T(0));
+    T cv[4];
 #pragma unroll
     for (int q = 0; q < 4; ++q) {
-        const int row = m0 + MfmaAcc<double>::row(lk, q);
-        cv[q] = (readc && row < M && n < N) ? C[row + (long)n * ldc] : 0.0;
+        const int row = m0 + MfmaAcc<T>::row(lk, q);
+        cv[q] = (readc && row < M && n < N) ? C[row + (long)n * ldc] : T(0);
     }
-    double4_t acc = {0.0, 0.0, 0.0, 0.0};
+    acc_t acc = acc_t{0, 0, 0, 0};
 #pragma unroll
-    for (int st = 0; st < SK_KMAX / 4; ++st)
+    for (int st = 0; st < KS; ++st)
         if (st < nks) acc = mfma16x16x4(a[st], b[st], acc);
     if (n >= N) return;
 #pragma unroll
     for (int q = 0; q < 4; ++q) {
-        const int row = m0 + MfmaAcc<double>::row(lk, q);
+        const int row = m0 + MfmaAcc<T>::row(lk, q);
         if (row < M) {
-            const double c = readc ? cv[q] * beta : 0.0;
+            const T c = readc ? cv[q] * beta : T(0);
             C[row + (long)n * ldc] = (nks > 0) ? c + alpha * acc[q] : c;
         }
     }
+}
+
+template <typename T>
+static bool launch_skinny(hipStream_t s, int ta, int tb, int M, int N, int K, T alpha,
+                          const T *A, long lda, const T *B, long ldb, T beta, T *C, long ldc,
+                          int blas_beta0, int max_groups, hipError_t &err)
+{
+    if (ta || tb || M <= 0 || N <= 0 || M > SkinnyLimits<T>::M || K > SkinnyLimits<T>::K ||
+        max_groups > 0)
+        return false;
+    const int mt = (M + 15) / 16, nt = (N + 15) / 16;
+    if ((long)mt * nt > INT_MAX) return false;
+    hipLaunchKernelGGL(gemm_skinny_kernel<T>, dim3(mt * nt), dim3(64), 0, s, M, N, K, alpha, A,
+                       lda, B, ldb, beta, C, ldc, blas_beta0, mt);
+    err = hipGetLastError();
+    return true;
 }
 
 template <typename T>
@@ -382,14 +406,10 @@ hipError_t gemm_f64(hipStream_t s, int ta, int tb, int M, int N, int K, double a
                     const double *A, long lda, const double *B, long ldb, double beta,
                     double *C, long ldc, int blas_beta0, int max_groups)
 {
-    if (!ta && !tb && M > 0 && N > 0 && M <= SK_MMAX && K <= SK_KMAX && max_groups <= 0) {
-        const int mt = (M + 15) / 16, nt = (N + 15) / 16;
-        if ((long)mt * nt <= INT_MAX) {
-            hipLaunchKernelGGL(gemm_skinny_kernel, dim3(mt * nt), dim3(64), 0, s, M, N, K, alpha,
-                               A, lda, B, ldb, beta, C, ldc, blas_beta0, mt);
-            return hipGetLastError();
-        }
-    }
+    hipError_t err = hipSuccess;
+    if (launch_skinny<double>(s, ta, tb, M, N, K, alpha, A, lda, B, ldb, beta, C, ldc,
+                              blas_beta0, max_groups, err))
+        return err;
     return launch_gemm<double>(s, ta, tb, M, N, K, alpha, A, lda, B, ldb, beta, C, ldc,
                                blas_beta0, max_groups);
 }
@@ -398,6 +418,10 @@ hipError_t gemm_f32(hipStream_t s, int ta, int tb, int M, int N, int K, float al
                     const float *A, long lda, const float *B, long ldb, float beta, float *C,
                     long ldc, int blas_beta0, int max_groups)
 {
+    hipError_t err = hipSuccess;
+    if (launch_skinny<float>(s, ta, tb, M, N, K, alpha, A, lda, B, ldb, beta, C, ldc,
+                             blas_beta0, max_groups, err))
+        return err;
     return launch_gemm<float>(s, ta, tb, M, N, K, alpha, A, lda, B, ldb, beta, C, ldc,
                               blas_beta0, max_groups);
 }

@@ -538,51 +538,56 @@ __global__ __launch_bounds__(256) void trsm_rm_llu_kernel(const T *__restrict__ 
 // in range and branch-free, and a zero stands where trsm_rm_llu_kernel's
 // strip holds one, so the arithmetic is that kernel's, operation for
 // operation.
-constexpr int TR_FULLK = 128;
-constexpr int TR_FULLB = TR_FULLK / TR_BS;
-constexpr int TR_LPAD = TR_BS * TR_BS * TR_FULLB * (TR_FULLB + 1) / 2;
+constexpr int TR_FULLK = 128;     // FP64 (KMAX = 256 fits LDS for FP32 at CW = 8)
 
 // CW columns per workgroup, 256 / CW row groups folding the rows below
-template <typename T, int CW>
+template <typename T, int CW, int KMAX>
 __global__ __launch_bounds__(256) void trsm_rm_llu_full_kernel(const T *__restrict__ L,
                                                                long ldl, T *__restrict__ X,
                                                                long ldx, int k, int ncols)
 {
     constexpr int NG = 256 / CW;
-    static_assert(256 % CW == 0 && TR_FULLK % NG == 0, "column chunk");
-    __shared__ T xs[TR_FULLK][CW + 1];
-    __shared__ T lp[TR_LPAD];
+    constexpr int FB = KMAX / TR_BS;                              // 16-row blocks
+    constexpr int LPAD = TR_BS * TR_BS * FB * (FB + 1) / 2;
+    static_assert(256 % CW == 0 && KMAX % NG == 0 && KMAX % TR_BS == 0, "column chunk");
+    __shared__ T xs[KMAX][CW + 1];
+    __shared__ T lp[LPAD];
     const int tid = threadIdx.x;
     const int tc = tid % CW, tg = tid / CW;   // CW columns x NG row groups
-    // block R: 256 (R + 1) entries, R + 1 per thread, all loads issued first
-    StaticFor<0, TR_FULLB>::run([&](auto RC) {
-        constexpr int R = decltype(RC)::value;
-        constexpr int W = TR_BS * (R + 1), BASE = TR_BS * TR_BS * R * (R + 1) / 2;
-        if (R * TR_BS < k) {
-            T v[R + 1];
+    // block R: 256 (R + 1) entries, R + 1 per thread; every block's loads
+    // are issued before any is stored (one memory latency in all)
+    {
+        T lv[FB * (FB + 1) / 2];
+        StaticFor<0, FB>::run([&](auto RC) {
+            constexpr int R = decltype(RC)::value;
+            constexpr int W = TR_BS * (R + 1), V0 = R * (R + 1) / 2;
 #pragma unroll
             for (int u = 0; u <= R; ++u) {
                 const int e = tid + 256 * u, r = R * TR_BS + e / W, c = e % W;
-                v[u] = (r < k && c < r) ? L[(long)r * ldl + c] : T(0);
+                lv[V0 + u] = (r < k && c < r) ? L[(long)r * ldl + c] : T(0);
             }
+        });
+        StaticFor<0, FB>::run([&](auto RC) {
+            constexpr int R = decltype(RC)::value;
+            constexpr int BASE = TR_BS * TR_BS * R * (R + 1) / 2, V0 = R * (R + 1) / 2;
 #pragma unroll
-            for (int u = 0; u <= R; ++u) lp[BASE + tid + 256 * u] = v[u];
-        }
-    });
+            for (int u = 0; u <= R; ++u) lp[BASE + tid + 256 * u] = lv[V0 + u];
+        });
+    }
     // workgroup b solves the column chunks b, b + grid, ...
     for (int c0 = blockIdx.x * CW; c0 < ncols; c0 += gridDim.x * CW) {
         const int cw = min(CW, ncols - c0);
         if (c0 != (int)blockIdx.x * CW) __syncthreads();
         // rows tg, tg + NG, ... of the chunk, all loads issued first
         {
-            T v[TR_FULLK / NG];
+            T v[KMAX / NG];
 #pragma unroll
-            for (int u = 0; u < TR_FULLK / NG; ++u) {
+            for (int u = 0; u < KMAX / NG; ++u) {
                 const int rr = tg + NG * u;
                 v[u] = (rr < k && tc < cw) ? X[(long)rr * ldx + c0 + tc] : T(0);
             }
 #pragma unroll
-            for (int u = 0; u < TR_FULLK / NG; ++u) xs[tg + NG * u][tc] = v[u];
+            for (int u = 0; u < KMAX / NG; ++u) xs[tg + NG * u][tc] = v[u];
         }
         __syncthreads();
         for (int kb = 0; kb < k; kb += TR_BS) {
@@ -726,18 +731,30 @@ hipError_t trsm_rm_llu(hipStream_t s, Profiler *prof, const T *L, long ldl, T *X
                  sizeof(T) * (2.0 * k * ncols + 0.5 * k * k));
     int groups = (ncols + TR_CW - 1) / TR_CW;
     if (max_groups > 0 && groups > max_groups) groups = max_groups;
+    const int g8 = (ncols + 7) / 8;
     if (k <= TR_FULLK && ncols <= 4 * TR_CW) {
         // a narrow solve (the next block's columns): 8-column chunks, so
         // more workgroups share the rows below each diagonal block
-        const int g8 = (ncols + 7) / 8;
-        hipLaunchKernelGGL((trsm_rm_llu_full_kernel<T, 8>), dim3(g8), dim3(256), 0, s, L, ldl, X,
-                           ldx, k, ncols);
-    } else if (k <= TR_FULLK)
-        hipLaunchKernelGGL((trsm_rm_llu_full_kernel<T, TR_CW>), dim3(groups), dim3(256), 0, s, L,
-                           ldl, X, ldx, k, ncols);
-    else
-        hipLaunchKernelGGL(trsm_rm_llu_kernel<T>, dim3(groups), dim3(256), 0, s, L, ldl, X, ldx,
-                           k, ncols);
+        hipLaunchKernelGGL((trsm_rm_llu_full_kernel<T, 8, TR_FULLK>), dim3(g8), dim3(256), 0, s,
+                           L, ldl, X, ldx, k, ncols);
+        return hipGetLastError();
+    }
+    if (k <= TR_FULLK) {
+        hipLaunchKernelGGL((trsm_rm_llu_full_kernel<T, TR_CW, TR_FULLK>), dim3(groups), dim3(256),
+                           0, s, L, ldl, X, ldx, k, ncols);
+        return hipGetLastError();
+    }
+    if constexpr (sizeof(T) == 4) {
+        if (ncols <= 8 * TR_CW) {
+            // FP32 at nb = 256 (the mixed factor's next block): L's 256 rows
+            // fit LDS beside 8-column chunks
+            hipLaunchKernelGGL((trsm_rm_llu_full_kernel<T, 8, 2 * TR_FULLK>), dim3(g8),
+                               dim3(256), 0, s, L, ldl, X, ldx, k, ncols);
+            return hipGetLastError();
+        }
+    }
+    hipLaunchKernelGGL(trsm_rm_llu_kernel<T>, dim3(groups), dim3(256), 0, s, L, ldl, X, ldx, k,
+                       ncols);
     return hipGetLastError();
 }
 template hipError_t trsm_rm_llu<double>(hipStream_t, Profiler *, const double *, long, double *,
