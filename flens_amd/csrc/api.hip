@@ -273,7 +273,7 @@ int dgetrf_dev(flx_context *h, int m, int n, double *A, int lda, int *ipiv, int 
     FLX_TRY(ensure(h, h->perm, sizeof(int) * (size_t)(m > 0 ? m : 1)));
     h->wk.perm = (int *)h->perm.p;
     FLX_TRY(ensure(h, h->lu_inv, sizeof(int) * (size_t)(m > 0 ? m : 1)));
-    FLX_TRY(ensure(h, h->lu_mv, sizeof(int) * (1 + 2 * LU_BLOCK_MOVES)));
+    FLX_TRY(ensure(h, h->lu_mv, sizeof(int) * 2 * (1 + 2 * LU_BLOCK_MOVES)));   // two blocks' lists
     h->wk.inv = (int *)h->lu_inv.p;
     h->wk.mv = (int *)h->lu_mv.p;
     FLX_TRY(ensure(h, h->lu_stage, sizeof(double) * LU_STAGE_DOUBLES));

@@ -904,24 +904,39 @@ hipError_t inv_perm(hipStream_t s, const int *perm, int *inv, int m, int *reset)
     return hipGetLastError();
 }
 
+// the block's move list (rows p >= J whose content moved), mv[0] cleared by
+// the block's inv_perm(..., mv)
+static hipError_t list_moves(hipStream_t s, const int *perm, const int *inv, int J, int m, int *mv)
+{
+    const int rows = m - J;
+    if (rows <= 0) return hipSuccess;
+    hipLaunchKernelGGL(block_moves_kernel, dim3((rows + 255) / 256), dim3(256), 0, s, perm, inv,
+                       J, m, mv);
+    return hipGetLastError();
+}
+
+// the listed moves applied to the columns [0, c1) and [c2, c3)
+template <typename T>
+static hipError_t apply_moves(hipStream_t s, Profiler *prof, T *W, long ldw, int m, int c1, int c2,
+                              int c3, const int *mv, unsigned long long *moved)
+{
+    const int nleft = (c1 + AM_CW - 1) / AM_CW;
+    const int nright = c3 > c2 ? (c3 - c2 + AM_CW - 1) / AM_CW : 0;
+    if (nleft + nright <= 0) return hipSuccess;
+    ProfScope ps(prof, s, PROF_ROWPERM, 0.0, 0.0);
+    hipLaunchKernelGGL(apply_moves_kernel<T>, dim3(nleft + nright), dim3(256), 0, s, W, ldw, m, c1,
+                       c2, c3, mv, moved);
+    return hipGetLastError();
+}
+
 template <typename T>
 hipError_t defer_moves(hipStream_t s, Profiler *prof, T *W, long ldw, int m, int n, int J,
                        int jb, const int *perm, const int *inv, int *mv,
                        unsigned long long *moved)
 {
-    const int rows = m - J;
-    if (rows <= 0) return hipSuccess;
-    // mv[0] was cleared by the block's inv_perm(..., mv)
-    hipLaunchKernelGGL(block_moves_kernel, dim3((rows + 255) / 256), dim3(256), 0, s, perm, inv,
-                       J, m, mv);
-    const int nleft = (J + AM_CW - 1) / AM_CW;
-    const int nright = (n - (J + jb) + AM_CW - 1) / AM_CW;
-    if (nleft + nright > 0) {
-        ProfScope ps(prof, s, PROF_ROWPERM, 0.0, 0.0);
-        hipLaunchKernelGGL(apply_moves_kernel<T>, dim3(nleft + nright), dim3(256), 0, s, W, ldw,
-                           m, J, J + jb, n, (const int *)mv, moved);
-    }
-    return hipGetLastError();
+    if (m - J <= 0) return hipSuccess;
+    FLX_CHECK(list_moves(s, perm, inv, J, m, mv));
+    return apply_moves<T>(s, prof, W, ldw, m, J, J + jb, n, mv, moved);
 }
 template hipError_t defer_moves<double>(hipStream_t, Profiler *, double *, long, int, int, int,
                                         int, const int *, const int *, int *,
@@ -948,7 +963,10 @@ hipError_t getrf_rm_f64(hipStream_t s, LuWork &wk, double *W, long ldw, int m, i
     bool rest_pending = false, next_pending = false;
     for (int J = 0; J < mn; J += nb) {
         const int jb = (mn - J < nb) ? mn - J : nb;
-        if (look) FLX_CHECK(inv_perm(s, wk.perm, wk.inv, m, wk.mv));
+        // the block's move list, double-buffered by block parity: the side
+        // stream may still read the previous block's
+        int *mvb = wk.mv + (size_t)((J / nb) & 1) * (1 + 2 * LU_BLOCK_MOVES);
+        if (look) FLX_CHECK(inv_perm(s, wk.perm, wk.inv, m, mvb));
         for (int j = J; j < J + jb; j += ib) {
             const int w = (J + jb - j < ib) ? J + jb - j : ib;
             // with the fused step, the panel writes only its pivot rows and
@@ -1017,25 +1035,32 @@ hipError_t getrf_rm_f64(hipStream_t s, LuWork &wk, double *W, long ldw, int m, i
             }
             continue;
         }
-        // the previous block's rest update must be done before its columns move
+        // the previous block's side-stream work must be done before its
+        // columns move again
         if (rest_pending) FLX_CHECK(hipStreamWaitEvent(s, wk.ev_r, 0));
         rest_pending = false;
-        FLX_CHECK(defer_moves<double>(s, wk.prof, W, ldw, m, n, J, jb, wk.perm, wk.inv, wk.mv,
-                                      wk.moved));
-        if (nr > 0) {
-            // The next block's first panel columns on the critical path; its
-            // other columns, then the rest of the trailing matrix, on the side
-            // stream beside that panel (the next block's fused steps wait for
-            // the former, its deferred moves for the latter).
-            const int nl = nr < nb ? nr : nb;
-            const bool split = wk.ev_n && nl > PANEL_W;
+        FLX_CHECK(list_moves(s, wk.perm, wk.inv, J, m, mvb));
+        // The block's interchanges reach the next block's columns on the main
+        // stream (its panels need them), every other column on the side
+        // stream ahead of that stream's updates.  Then the next block's first
+        // panel columns are updated on the critical path; its other columns,
+        // then the rest of the trailing matrix, on the side stream beside that
+        // panel (the next block's fused steps wait for the former, its
+        // deferred moves for all of it).
+        const int nl = nr > 0 ? (nr < nb ? nr : nb) : 0;
+        const int c0 = J + jb + nl;
+        FLX_CHECK(apply_moves<double>(s, wk.prof, W, ldw, m, 0, J + jb, c0, mvb, wk.moved));
+        const bool side_moves = J > 0 || n > c0;
+        const bool split = wk.ev_n && nl > PANEL_W;
+        if (side_moves || split) {
+            FLX_CHECK(hipEventRecord(wk.ev_a, s));
+            FLX_CHECK(hipStreamWaitEvent(wk.side, wk.ev_a, 0));
+        }
+        if (side_moves)
+            FLX_CHECK(apply_moves<double>(wk.side, wk.prof, W, ldw, m, J, c0, n, mvb, wk.moved));
+        if (nl > 0) {
             const int nf = split ? PANEL_W : nl;
             const double *L11 = (const double *)W + (long)J * ldw + J;
-            const int c0 = J + jb + nl;
-            if (split || n > c0) {
-                FLX_CHECK(hipEventRecord(wk.ev_a, s));
-                FLX_CHECK(hipStreamWaitEvent(wk.side, wk.ev_a, 0));
-            }
             FLX_CHECK(trsm_rm_llu(s, wk.prof, L11, ldw, W + (long)J * ldw + J + jb, ldw, jb, nf));
             FLX_CHECK(rm_update(s, wk.prof, W, ldw, J + jb, m - J - jb, J + jb, nf, J, jb));
             if (split) {
@@ -1051,9 +1076,11 @@ hipError_t getrf_rm_f64(hipStream_t s, LuWork &wk, double *W, long ldw, int m, i
                                       n - c0, wk.side_groups));
                 FLX_CHECK(rm_update(wk.side, wk.prof, W, ldw, J + jb, m - J - jb, c0, n - c0, J,
                                     jb, wk.side_groups));
-                FLX_CHECK(hipEventRecord(wk.ev_r, wk.side));
-                rest_pending = true;
             }
+        }
+        if (side_moves || split) {
+            FLX_CHECK(hipEventRecord(wk.ev_r, wk.side));
+            rest_pending = true;
         }
     }
     // (columns right of the last block, when n > m, may still be on the side stream)

@@ -36,7 +36,7 @@ def run_dev(F, ta, tb, alpha, a, b, beta, c):
 
 
 SHAPES = [(1, 1, 1), (5, 7, 3), (128, 128, 16), (129, 127, 17), (300, 200, 100),
-          (64, 513, 40), (1000, 33, 257), (17, 400, 0), (256, 256, 256)]
+          (64, 513, 40), (1000, 33, 257), (17, 400, 0), (256, 256, 256), (32, 2000, 128)]
 
 
 @pytest.mark.parametrize("ta,tb", [(0, 0), (0, 1), (1, 0), (1, 1)])
@@ -50,6 +50,21 @@ def test_gemm_vs_oracle_bound(gpu, ta, tb):
             got = run_dev(gpu, ta, tb, alpha, a, b, beta, c0.copy(order="F"))
             err = np.abs(got - want)
             assert np.all(err <= bound(ta, tb, alpha, a, b, beta, c0) + 1e-300), (m, n, k, alpha, beta)
+
+
+def test_narrow_gemm_matches_tiled_bitwise(gpu):
+    """The one-wave-per-block kernel for narrow C (M <= 128, K <= 128, the LU's
+    next-block update) runs the tiled kernel's MFMA sequence: the first 128
+    rows of an M = 129 product (tiled) equal the M = 128 product (narrow)
+    bit for bit, K a multiple of 16."""
+    for (n, k) in [(300, 128), (1000, 64), (37, 16)]:
+        a = O.uniform(129, k, 11 + k)
+        b = O.uniform(k, n, 12 + n)
+        c = O.uniform(129, n, 13 + n + k)
+        tiled = run_dev(gpu, 0, 0, -1.0, a, b, 1.0, c.copy(order="F"))
+        narrow = run_dev(gpu, 0, 0, -1.0, np.asfortranarray(a[:128]), b, 1.0,
+                         np.asfortranarray(c[:128]))
+        assert np.array_equal(tiled[:128].view(np.uint64), narrow.view(np.uint64)), (n, k)
 
 
 def test_gemm_dblat3_grid_dmmch(gpu):
