@@ -98,7 +98,7 @@ extern "C" int flx_destroy(flx_handle h)
     free_buf(h->trsw);
     MixBufs &x = h->mix;
     for (DevBuf *b : {&x.W32, &x.perm, &x.cand, &x.ccount, &x.counters, &x.u11, &x.win, &x.list,
-                      &x.zero, &x.lpmax, &x.la, &x.ub, &x.inv_l, &x.inv_u, &x.y32, &x.y64, &x.flags,
+                      &x.zero, &x.lpmax, &x.la, &x.ub, &x.dinv, &x.y32, &x.y64, &x.flags,
                       &x.err, &x.dr, &x.dc, &x.part, &x.scal, &x.R, &x.V, &x.w, &x.t, &x.zeros,
                       &x.mpart, &x.hbuf, &x.inv, &x.mv, &x.stage, &x.stage_perm, &x.F, &x.rbt})
         free_buf(*b);

@@ -18,7 +18,7 @@ struct MixBufs {
     DevBuf stage, stage_perm;               // rows that move in a panel step
     hipStream_t side = nullptr;             // default priority, ordered by events
     hipEvent_t ev_a = nullptr, ev_r = nullptr;
-    DevBuf inv_l, inv_u, y32, y64, flags, err, dr, dc, part, scal, R, V, w, t, zeros, mpart, hbuf;
+    DevBuf dinv, y32, y64, flags, err, dr, dc, part, scal, R, V, w, t, zeros, mpart, hbuf;
     DevBuf F;                               // FP64 fallback factors (A kept for the residual)
     DevBuf rbt;                             // random butterflies of order rbt_n (4 x n doubles)
     int rbt_n = 0;

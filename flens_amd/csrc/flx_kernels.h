@@ -268,6 +268,7 @@ struct PrecFactor {
     Rbt rb;
     float *y32 = nullptr;      // n
     double *y64 = nullptr;     // n (FP64 application)
+    const double *dinv = nullptr;   // trsv_inv_doubles(n): inverted diagonal blocks of W
     uint64_t *xt = nullptr;    // trsv_words(n) tagged words, zero-initialised once
     unsigned *epoch = nullptr; // host counter paired with xt
     int *err = nullptr;        // device error flag
@@ -281,9 +282,13 @@ hipError_t norm_inf(hipStream_t s, const double *A, long lda, int n, double *par
                     unsigned long long *nrm);
 hipError_t equilibrate(hipStream_t s, const double *A, long lda, int n, double *part,
                        double *dr, double *dc);
+size_t trsv_inv_doubles(int n);
+// the inverted 64 x 64 diagonal blocks of both triangles of W (after every
+// factorisation, before any trsv_rm on it)
+hipError_t trsv_diag_inv(hipStream_t s, const float *W, long ldw, int n, double *dinv);
 template <typename TY>
-hipError_t trsv_rm(hipStream_t s, int upper, const float *W, long ldw, int n, TY *y,
-                   uint64_t *xt, unsigned epoch, int *err);
+hipError_t trsv_rm(hipStream_t s, int upper, const float *W, long ldw, int n, const double *dinv,
+                   TY *y, uint64_t *xt, unsigned epoch, int *err);
 hipError_t zero_slots(hipStream_t s, unsigned long long *p, int k);
 hipError_t vmax(hipStream_t s, const double *v, int n, unsigned long long *nrm);
 hipError_t prec_apply(hipStream_t s, const PrecFactor &f, const double *v, double *z, int add,

@@ -176,19 +176,19 @@ hipError_t equilibrate(hipStream_t s, const double *A, long lda, int n, double *
 // for belongs to a workgroup dispatched before it.
 //   * Finished blocks are published as 64-bit words (epoch << 32 | x bits)
 //     in xt: a word is its own ready flag, so one agent-scope load both
-//     tests readiness and delivers the value.  Dependencies are consumed in
-//     groups of 8 blocks; a block counts once all 64 of its words carry this
-//     launch's epoch; waits are bounded (err |= 2).
+//     tests readiness and delivers the value.  Waits are bounded (err |= 2).
 //   * Thread (rr, q) = (tid / 4, tid % 4) accumulates row rr of its block
-//     over columns q*16..+15 of every finished block.
-//   * The diagonal block is solved by wave 0 by substitution, one row per
-//     lane, the solved entry broadcast by readlane.
+//     over columns q*16..+15 of every finished block: the older blocks in
+//     groups of 8 (wave 0 waits until all 64 words of each carry this
+//     launch's epoch, then the group goes through LDS), the block finished
+//     last on its own -- its factor entries are loaded at kernel start and
+//     every wave polls its words itself, so no barrier sits between its
+//     arrival and the product.
+//   * The diagonal block is applied as its inverse (computed once per
+//     factor by trsv_inv_kernel, in FP64 from the FP32 entries): a 64 x 64
+//     product whose rows are loaded at kernel start, one barrier after the
+//     right-hand side is known, instead of a 64-step substitution chain.
 // ---------------------------------------------------------------------------
-FLX_DEV float readlane_f(float v, int lane)
-{
-    return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), lane));
-}
-
 constexpr int TV = 64;
 constexpr int TV_FETCH = 8;
 constexpr int TV_SPIN_LIMIT = 1 << 22;
@@ -209,7 +209,6 @@ template <> struct TvWords<float> {
         v = __uint_as_float((unsigned)(w & 0xffffffffull));
         return (unsigned)(w >> 32) == e;
     }
-    FLX_DEV float readlane(float v, int l) { return readlane_f(v, l); }
 };
 template <> struct TvWords<double> {
     static constexpr int N = 2;
@@ -228,19 +227,59 @@ template <> struct TvWords<double> {
         v = __longlong_as_double((long long)(((hi & 0xffffffffull) << 32) | (lo & 0xffffffffull)));
         return (unsigned)(hi >> 32) == e && (unsigned)(lo >> 32) == e;
     }
-    FLX_DEV double readlane(double v, int l)
-    {
-        const long long b = __double_as_longlong(v);
-        const int lo = __builtin_amdgcn_readlane((int)(b & 0xffffffffll), l);
-        const int hi = __builtin_amdgcn_readlane((int)(b >> 32), l);
-        return __longlong_as_double(((long long)hi << 32) | (unsigned)lo);
-    }
 };
 
-// TY = float: FP32 substitution (dsgesv's single-precision solve);
-// TY = double: the FP32 factor applied in FP64 arithmetic (GMRES-IR).
+// Inverses of the 64 x 64 diagonal blocks of W's two triangles, FP64,
+// row-major: dinv[(t * nbk + bi) * 4096 + 64 i + j], t = 0 for the unit
+// lower triangle, 1 for the upper.  Rows and columns beyond n are the
+// identity's.  One wave per block; lane j forms column j of the inverse by
+// substitution against the block held in LDS (eight terms of each sum
+// requested at once).
+__global__ __launch_bounds__(64) void trsv_inv_kernel(const float *__restrict__ W, long ldw,
+                                                      int n, double *__restrict__ dinv)
+{
+    __shared__ double s_t[TV][TV + 1];
+    __shared__ double s_x[TV][TV + 1];
+    const int nbk = (n + TV - 1) / TV;
+    const int bi = blockIdx.x, upper = blockIdx.y;
+    const int r0 = bi * TV, rows = min(TV, n - r0);
+    const int j = threadIdx.x;
+    {
+        float v[TV];
+#pragma unroll
+        for (int i = 0; i < TV; ++i)
+            v[i] = (i < rows && j < rows) ? W[(long)(r0 + i) * ldw + r0 + j]
+                                          : (i == j ? 1.0f : 0.0f);
+#pragma unroll
+        for (int i = 0; i < TV; ++i) s_t[i][j] = (double)v[i];
+    }
+    __syncthreads();
+    if (!upper) {
+        for (int i = 0; i < TV; ++i) {
+            double a = (i == j) ? 1.0 : 0.0;
+#pragma unroll 8
+            for (int k = j; k < i; ++k) a = fma(-s_t[i][k], s_x[k][j], a);
+            s_x[i][j] = (i < j) ? 0.0 : a;
+        }
+    } else {
+        for (int i = TV - 1; i >= 0; --i) {
+            double a = (i == j) ? 1.0 : 0.0;
+#pragma unroll 8
+            for (int k = i + 1; k <= j; ++k) a = fma(-s_t[i][k], s_x[k][j], a);
+            s_x[i][j] = (i > j) ? 0.0 : a / s_t[i][i];
+        }
+    }
+    double *out = dinv + ((size_t)upper * nbk + bi) * TV * TV;
+#pragma unroll 8
+    for (int i = 0; i < TV; ++i) out[i * TV + j] = s_x[i][j];
+}
+
+// TY = float: FP32 solve (dsgesv's single-precision solve, the inverse
+// blocks rounded to FP32); TY = double: the FP32 factor applied in FP64
+// arithmetic (GMRES-IR).
 template <typename TY, int UPPER>
 __global__ __launch_bounds__(256) void trsv_rm_kernel(const float *__restrict__ W, long ldw, int n,
+                                                      const double *__restrict__ dinv,
                                                       TY *__restrict__ y,
                                                       uint64_t *__restrict__ xt,
                                                       unsigned epoch, int *__restrict__ err)
@@ -252,38 +291,32 @@ __global__ __launch_bounds__(256) void trsv_rm_kernel(const float *__restrict__ 
     const int g = blockIdx.x;
     const int bi = UPPER ? nbk - 1 - g : g;
     const int r0 = bi * TV, rows = min(TV, n - r0);
-    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int tid = threadIdx.x, lane = tid & 63;
     const int rr = tid >> 2, q = tid & 3;
     const int ndep = UPPER ? nbk - 1 - bi : bi;
     const bool row_ok = rr < rows;
     const float *wrow = W + (long)(r0 + (row_ok ? rr : 0)) * ldw;
     const TY rhs0 = row_ok ? y[r0 + rr] : TY(0);
-    // wave 0: row `lane` of the diagonal block (identity outside the
-    // matrix) and, for U, the reciprocal of the row's diagonal entry, so the
-    // substitution chain multiplies instead of dividing (both ahead of the
-    // dependency wait, off the critical path)
-    float drow[TV];
-    TY dinv = TY(1);
-    if (wave == 0) {
-        const bool lok = lane < rows;
-        const float *dr = W + (long)(r0 + (lok ? lane : 0)) * ldw + r0;
+    // ahead of every wait: the inverse's row rr (columns q*16..+15) and the
+    // factor entries of the block finished last
+    TY iv[16];
+    {
+        const double *ib = dinv + ((size_t)UPPER * nbk + bi) * TV * TV + rr * TV + q * 16;
 #pragma unroll
-        for (int c = 0; c < TV; ++c)
-            drow[c] = (lok && c < rows) ? dr[c] : (lane == c ? 1.0f : 0.0f);
-        if (UPPER) {
-            float dd = 1.0f;
-#pragma unroll
-            for (int c = 0; c < TV; ++c) dd = (lane == c) ? drow[c] : dd;
-            dinv = TY(1) / (TY)dd;
-        }
+        for (int c = 0; c < 16; ++c) iv[c] = (TY)ib[c];
     }
-    // dependency blocks in groups of TV_FETCH: the group's factor entries are
-    // requested first, then wave 0 waits until every block of the group is
-    // published, so the two latencies overlap
+    const int jl = UPPER ? bi + 1 : bi - 1;   // the block finished last (ndep >= 1)
+    float wl[16];
+#pragma unroll
+    for (int c = 0; c < 16; ++c) {
+        const int col = jl * TV + q * 16 + c;
+        wl[c] = (ndep >= 1 && row_ok && col < n) ? wrow[col] : 0.0f;
+    }
     TY acc = TY(0);
     int par = 0, spins = 0;
-    for (int d0 = 0; d0 < ndep; d0 += TV_FETCH, par ^= 1) {
-        const int cnt = min(TV_FETCH, ndep - d0);
+    const int nold = ndep - 1;   // blocks finished before the last one
+    for (int d0 = 0; d0 < nold; d0 += TV_FETCH, par ^= 1) {
+        const int cnt = min(TV_FETCH, nold - d0);
         float wv[TV_FETCH][16];
 #pragma unroll
         for (int b = 0; b < TV_FETCH; ++b) {
@@ -294,7 +327,7 @@ __global__ __launch_bounds__(256) void trsv_rm_kernel(const float *__restrict__ 
             for (int c = 0; c < 16; ++c)
                 wv[b][c] = (b < cnt && row_ok && c0 + c < n) ? wrow[c0 + c] : 0.0f;
         }
-        if (wave == 0) {
+        if (tid < 64) {
             for (;;) {
                 TY v[TV_FETCH];
                 bool all = true;
@@ -323,50 +356,66 @@ __global__ __launch_bounds__(256) void trsv_rm_kernel(const float *__restrict__ 
 #pragma unroll
             for (int c = 0; c < 16; ++c) acc = acc + (TY)wv[b][c] * s_x[par][b][q * 16 + c];
     }
+    if (ndep >= 1) {
+        // every wave polls the last block's 64 words (lane l: row l)
+        TY xl = TY(0);
+        for (;;) {
+            const bool ok = TW::get(&xt[((long)jl * TV + lane) * TW::N], epoch, xl);
+            const bool all = __ballot(ok) == ~0ull;
+            if (all || spins > TV_SPIN_LIMIT) {
+                if (!all && lane == 0) atomicOr(err, 2);
+                break;
+            }
+            ++spins;
+            __builtin_amdgcn_s_sleep(1);
+        }
+#pragma unroll
+        for (int c = 0; c < 16; ++c) acc = acc + (TY)wl[c] * __shfl(xl, q * 16 + c, 64);
+    }
     acc += __shfl_xor(acc, 1, 64);
     acc += __shfl_xor(acc, 2, 64);
     if (q == 0) s_rhs[rr] = row_ok ? rhs0 - acc : TY(0);
     __syncthreads();
-    if (wave == 0) {
-        TY bv = s_rhs[lane];
-        if (!UPPER) {
+    TY x = TY(0);
 #pragma unroll
-            for (int c = 0; c < TV; ++c) {
-                const TY xc = TW::readlane(bv, c);
-                if (lane > c) bv = bv - (TY)drow[c] * xc;
-            }
-        } else {
-#pragma unroll
-            for (int c = TV - 1; c >= 0; --c) {
-                if (lane == c) bv = bv * dinv;
-                const TY xc = TW::readlane(bv, c);
-                if (lane < c) bv = bv - (TY)drow[c] * xc;
-            }
-        }
-        if (lane < rows) y[r0 + lane] = bv;
-        TW::put(&xt[((long)bi * TV + lane) * TW::N], epoch, lane < rows ? bv : TY(0));
+    for (int c = 0; c < 16; ++c) x = x + iv[c] * s_rhs[q * 16 + c];
+    x += __shfl_xor(x, 1, 64);
+    x += __shfl_xor(x, 2, 64);
+    if (q == 0) {
+        if (row_ok) y[r0 + rr] = x;
+        TW::put(&xt[((long)bi * TV + rr) * TW::N], epoch, row_ok ? x : TY(0));
     }
 }
 
+size_t trsv_inv_doubles(int n) { return 2 * (size_t)((n + TV - 1) / TV) * TV * TV; }
+
+hipError_t trsv_diag_inv(hipStream_t s, const float *W, long ldw, int n, double *dinv)
+{
+    const int nbk = (n + TV - 1) / TV;
+    if (nbk <= 0) return hipSuccess;
+    hipLaunchKernelGGL(trsv_inv_kernel, dim3(nbk, 2), dim3(64), 0, s, W, ldw, n, dinv);
+    return hipGetLastError();
+}
+
 template <typename TY>
-hipError_t trsv_rm(hipStream_t s, int upper, const float *W, long ldw, int n, TY *y, uint64_t *xt,
-                   unsigned epoch, int *err)
+hipError_t trsv_rm(hipStream_t s, int upper, const float *W, long ldw, int n, const double *dinv,
+                   TY *y, uint64_t *xt, unsigned epoch, int *err)
 {
     const int nbk = (n + TV - 1) / TV;
     if (nbk <= 0) return hipSuccess;
     ProfScope ps(tl_prof, s, PROF_TRS, (double)n * n, 2.0 * n * (double)n);
     if (upper)
-        hipLaunchKernelGGL((trsv_rm_kernel<TY, 1>), dim3(nbk), dim3(256), 0, s, W, ldw, n, y, xt,
-                           epoch, err);
+        hipLaunchKernelGGL((trsv_rm_kernel<TY, 1>), dim3(nbk), dim3(256), 0, s, W, ldw, n, dinv,
+                           y, xt, epoch, err);
     else
-        hipLaunchKernelGGL((trsv_rm_kernel<TY, 0>), dim3(nbk), dim3(256), 0, s, W, ldw, n, y, xt,
-                           epoch, err);
+        hipLaunchKernelGGL((trsv_rm_kernel<TY, 0>), dim3(nbk), dim3(256), 0, s, W, ldw, n, dinv,
+                           y, xt, epoch, err);
     return hipGetLastError();
 }
-template hipError_t trsv_rm<float>(hipStream_t, int, const float *, long, int, float *, uint64_t *,
-                                   unsigned, int *);
-template hipError_t trsv_rm<double>(hipStream_t, int, const float *, long, int, double *,
-                                    uint64_t *, unsigned, int *);
+template hipError_t trsv_rm<float>(hipStream_t, int, const float *, long, int, const double *,
+                                   float *, uint64_t *, unsigned, int *);
+template hipError_t trsv_rm<double>(hipStream_t, int, const float *, long, int, const double *,
+                                    double *, uint64_t *, unsigned, int *);
 
 size_t trsv_words(int n) { return 2 * (size_t)((n + TV - 1) / TV) * TV; }   // room for doubles
 
@@ -466,8 +515,8 @@ hipError_t prec_apply(hipStream_t s, const PrecFactor &f, const double *v, doubl
     else
         hipLaunchKernelGGL(prec_in_kernel, dim3((n + 255) / 256), dim3(256), 0, s, v,
                            (const int *)f.perm, f.dr, (const unsigned long long *)vnrm, f.y32, n);
-    FLX_CHECK(trsv_rm<float>(s, 0, f.W, f.ldw, n, f.y32, f.xt, next_epoch(s, f), f.err));
-    FLX_CHECK(trsv_rm<float>(s, 1, f.W, f.ldw, n, f.y32, f.xt, next_epoch(s, f), f.err));
+    FLX_CHECK(trsv_rm<float>(s, 0, f.W, f.ldw, n, f.dinv, f.y32, f.xt, next_epoch(s, f), f.err));
+    FLX_CHECK(trsv_rm<float>(s, 1, f.W, f.ldw, n, f.dinv, f.y32, f.xt, next_epoch(s, f), f.err));
     if (f.rbt)
         return rbt_out(s, f.y32, nullptr, f.dc, vnrm, f.rb, z, add, n, znrm);
     hipLaunchKernelGGL(prec_out_kernel, dim3((n + 255) / 256), dim3(256), 0, s,
@@ -504,8 +553,8 @@ hipError_t prec_apply64(hipStream_t s, const PrecFactor &f, const double *v, dou
     else
         hipLaunchKernelGGL(prec_in64_kernel, dim3((n + 255) / 256), dim3(256), 0, s, v,
                            (const int *)f.perm, f.dr, f.y64, n);
-    FLX_CHECK(trsv_rm<double>(s, 0, f.W, f.ldw, n, f.y64, f.xt, next_epoch(s, f), f.err));
-    FLX_CHECK(trsv_rm<double>(s, 1, f.W, f.ldw, n, f.y64, f.xt, next_epoch(s, f), f.err));
+    FLX_CHECK(trsv_rm<double>(s, 0, f.W, f.ldw, n, f.dinv, f.y64, f.xt, next_epoch(s, f), f.err));
+    FLX_CHECK(trsv_rm<double>(s, 1, f.W, f.ldw, n, f.dinv, f.y64, f.xt, next_epoch(s, f), f.err));
     if (f.rbt)
         return rbt_out(s, nullptr, f.y64, f.dc, nullptr, f.rb, z, add, n, nullptr);
     hipLaunchKernelGGL(prec_out64_kernel, dim3((n + 255) / 256), dim3(256), 0, s,

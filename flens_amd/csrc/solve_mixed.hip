@@ -102,6 +102,7 @@ struct Solver {
             FLX_TRY(ensure(h, m.la, 2 * lp_pack_elems(n, o.nb)));
             FLX_TRY(ensure(h, m.ub, 2 * lp_pack_elems(n, o.nb)));
         }
+        FLX_TRY(ensure(h, m.dinv, sizeof(double) * trsv_inv_doubles(n)));
         FLX_TRY(ensure(h, m.y32, sizeof(float) * n));
         FLX_TRY(ensure(h, m.y64, sizeof(double) * n));
         if (m.flags.cap < sizeof(uint64_t) * trsv_words(n)) m.epoch = 0;
@@ -173,6 +174,7 @@ struct Solver {
         }
         f.y32 = (float *)m.y32.p;
         f.y64 = (double *)m.y64.p;
+        f.dinv = (const double *)m.dinv.p;
         f.xt = (uint64_t *)m.flags.p;
         f.epoch = &m.epoch;
         f.err = (int *)m.err.p;
@@ -264,6 +266,7 @@ struct Solver {
         } else {
             FLX_TRY(run_factor(s));
         }
+        FLX_TRY(trsv_diag_inv(s, f.W, ldw, n, (double *)m.dinv.p));
         int flags[4] = {0, 0, 0, 0}, zero = INT_MAX;
         FLX_TRY(hipMemcpyAsync(flags, m.err.p, sizeof flags, hipMemcpyDeviceToHost, s));
         FLX_TRY(hipMemcpyAsync(&zero, mw.zero, sizeof(int), hipMemcpyDeviceToHost, s));
