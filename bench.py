@@ -3,7 +3,9 @@
 Primary workload (`value`): plain FP64 LU with partial pivoting + solve,
 n = 8192, A i.i.d. U[-1,1) (seed 2), nrhs = 16 with B i.i.d. U[-1,1)
 (seed 3), blocked right-looking LU through the library's device entry
-flx_dgesv (getrf + getrs).  One step = restore A and B from resident
+flx_dgesv (getrf with B carried along as extra columns, so its
+interchanges and block updates leave L^-1 P B there, then the U solve of
+getrs).  One step = restore A and B from resident
 pristine copies (device-to-device) + one solve.  The result of the last step
 must reach a normwise backward error <= n*eps64 or the run fails.
 
@@ -709,7 +711,8 @@ def main():
                            "BASELINE.md publishes no number for this metric"),
         "dtype": "f64",
         "data": "synthetic (seeded U[-1,1) generated on the device)",
-        "config": {"workload": f"plain FP64 LU + solve (getrf+getrs), n={n}, nrhs={nrhs}, "
+        "config": {"workload": f"plain FP64 LU + solve (gesv: getrf carrying B, then the U "
+                               f"solve), n={n}, nrhs={nrhs}, "
                                f"A,B ~ U[-1,1), nb={args.nb}, ib={args.ib}",
                    "n": n, "nrhs": nrhs, "nb": args.nb, "ib": args.ib},
         "tflops": fl / elapsed * args.steps / 1e12,

@@ -262,11 +262,18 @@ extern "C" int flx_dgemm(flx_handle h, int transa, int transb, int m, int n, int
 
 // LU of device column-major A via the row-major working copy
 int dgetrf_dev(flx_context *h, int m, int n, double *A, int lda, int *ipiv, int *info,
-               bool keep_perm, double *out, int ldo)
+               bool keep_perm, double *out, int ldo, int nrhs, const double *B, int ldb,
+               double *Y)
 {
     const int mn = m < n ? m : n;
+    if (nrhs > 0 && (m != n || !B || !Y)) return FLX_INVALID_ARG;
+    const int nc = n + (nrhs > 0 ? nrhs : 0);   // columns of the working copy
     FLX_TRY(init_work(h));
-    const long ldw = round_ld(n);
+    // with right-hand sides along, rows padded to a multiple of 256 doubles:
+    // at n = 8192 + 16 a stride of 8448 measured 0.2 ms faster per solve than
+    // 8224 (the trailing updates' tiles and the panels' rows spread over
+    // the memory channels as with 8192)
+    const long ldw = nrhs > 0 ? ((long)nc + 255) / 256 * 256 : round_ld(nc);
     FLX_TRY(ensure(h, h->W, sizeof(double) * (size_t)ldw * (size_t)(m > 0 ? m : 1)));
     FLX_TRY(ensure(h, h->piv0, sizeof(int) * (size_t)(mn > 0 ? mn : 1)));
     (void)keep_perm;   // the composed permutation is always kept (lookahead needs it)
@@ -291,9 +298,11 @@ int dgetrf_dev(flx_context *h, int m, int n, double *A, int lda, int *ipiv, int 
     double *W = (double *)h->W.p;
     FLX_TRY(fill_int(h->stream, info, 1, INT_MAX));
     FLX_TRY(transpose_f64(h->stream, A, lda, W, ldw, m, n));
-    FLX_TRY(getrf_rm_f64(h->stream, h->wk, W, ldw, m, n, (int *)h->piv0.p, info, h->opt));
+    if (nrhs > 0) FLX_TRY(transpose_f64(h->stream, B, ldb, W + n, ldw, n, nrhs));
+    FLX_TRY(getrf_rm_f64(h->stream, h->wk, W, ldw, m, nc, (int *)h->piv0.p, info, h->opt));
     if (out) FLX_TRY(transpose_f64(h->stream, W, ldw, out, ldo, n, m));
     else FLX_TRY(transpose_f64(h->stream, W, ldw, A, lda, n, m));
+    if (nrhs > 0) FLX_TRY(transpose_f64(h->stream, W + n, ldw, Y, n, nrhs, n));
     FLX_TRY(piv0_to_ipiv(h->stream, (int *)h->piv0.p, ipiv, mn, info));
     return FLX_SUCCESS;
 }
@@ -372,14 +381,27 @@ extern "C" int flx_dgesv(flx_handle h, int n, int nrhs, double *A, int lda, int 
     if (n > LU_N_MAX) return FLX_UNSUPPORTED;
     FLX_TRY(ensure(h, h->info, sizeof(int)));
     int *dinfo = (int *)h->info.p;
-    int st = dgetrf_dev(h, n, n, A, lda, ipiv, dinfo, true);
+    // B rides along as extra columns of the factorisation (the forward
+    // solve L^{-1} P B is then done by its interchanges and block updates),
+    // so only the U solve remains; B is written only by that solve
+    double *Y = nullptr;
+    if (nrhs > 0) {
+        FLX_TRY(ensure(h, h->B, sizeof(double) * (size_t)n * (size_t)nrhs));
+        Y = (double *)h->B.p;
+    }
+    int st = dgetrf_dev(h, n, n, A, lda, ipiv, dinfo, true, nullptr, 0, nrhs, B, ldb, Y);
     if (st) return st;
     int info = 0;
     FLX_TRY(hipMemcpyAsync(&info, dinfo, sizeof(int), hipMemcpyDeviceToHost, h->stream));
     FLX_RET(flx_sync(h));   // INFO is only meaningful when every panel wait succeeded
     if (info_host) *info_host = info;
-    if (info == 0 && nrhs > 0)
-        return dgetrs_dev(h, 0, n, nrhs, A, lda, ipiv, (const int *)h->perm.p, B, ldb);
+    if (info == 0 && nrhs > 0) {
+        FLX_TRY(ensure(h, h->tinv, sizeof(double) * trs_inv_doubles(n)));
+        FLX_TRY(ensure_zeroed(h, h->trsw, sizeof(uint64_t) * trs_chain_words(n)));
+        FLX_TRY(trs_chain_f64(h->stream, n, nrhs, A, lda, Y, n, nullptr, B, ldb,
+                              (double *)h->tinv.p, (uint64_t *)h->trsw.p, &h->trs_epoch,
+                              h->wk.flags, h->wk.cus, 0));
+    }
     return FLX_SUCCESS;
 }
 

@@ -178,11 +178,13 @@ hipError_t trsm_cm_f64(hipStream_t s, int uplo_upper, int trans, int unit,
 size_t trs_persist_words(int n);
 // NoTrans solve U^{-1} L^{-1} Bp by the chain-and-helpers kernel: inv holds
 // trs_inv_doubles(n) doubles, w trs_chain_words(n) tagged words (zeroed
-// once), at most cus workgroups.
+// once), at most cus workgroups; lower = 0: Bp is L^{-1} P B already, only
+// the U solve runs.
 size_t trs_chain_words(int n);
 hipError_t trs_chain_f64(hipStream_t s, int n, int nrhs, const double *LU, long lda,
                          const double *Bp, long ldbp, double *T, double *X, long ldx,
-                         double *inv, uint64_t *w, unsigned *epoch, int *err, int cus);
+                         double *inv, uint64_t *w, unsigned *epoch, int *err, int cus,
+                         int lower = 1);
 hipError_t trs_persist_f64(hipStream_t s, int n, int nrhs, const double *LU, long lda,
                            const double *Bp, long ldbp, double *T, double *X, long ldx,
                            uint64_t *xt, unsigned *epoch, int *err, int trans = 0);

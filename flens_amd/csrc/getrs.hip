@@ -136,8 +136,10 @@ __global__ __launch_bounds__(64) void tri_inv_kernel(const double *__restrict__ 
         const int i = FWD ? s : TS_BS - 1 - s;
         double v = (i == t) ? 1.0 : 0.0;
         if (FWD) {
+#pragma unroll 8
             for (int q = 0; q < i; ++q) v = fma(-ts[i][q], xs[q][t], v);
         } else {
+#pragma unroll 8
             for (int q = i + 1; q < TS_BS; ++q) v = fma(-ts[i][q], xs[q][t], v);
         }
         xs[i][t] = UNIT ? v : v / ts[i][i];
@@ -831,10 +833,12 @@ __global__ __launch_bounds__(256) void trs_chain_kernel(const double *__restrict
 size_t trs_chain_words(int n) { return 2 * trs_persist_words(n); }
 
 // NoTrans: X = U^{-1} L^{-1} Bp (Bp already row-permuted), T n x nrhs scratch,
-// inv trs_inv_doubles(n) scratch, w trs_chain_words(n) tagged words.
+// inv trs_inv_doubles(n) scratch, w trs_chain_words(n) tagged words.  With
+// lower = 0 Bp already holds L^{-1} P B: X = U^{-1} Bp (T unused).
 hipError_t trs_chain_f64(hipStream_t s, int n, int nrhs, const double *LU, long lda,
                          const double *Bp, long ldbp, double *T, double *X, long ldx,
-                         double *inv, uint64_t *w, unsigned *epoch, int *err, int cus)
+                         double *inv, uint64_t *w, unsigned *epoch, int *err, int cus,
+                         int lower)
 {
     const int nbk = (n + TC_BS - 1) / TC_BS;
     if (nbk <= 0 || nrhs <= 0) return hipSuccess;
@@ -850,8 +854,8 @@ hipError_t trs_chain_f64(hipStream_t s, int n, int nrhs, const double *LU, long 
     };
     // chain + helpers, all resident: at most one workgroup per CU
     const int grid = 1 + std::max(1, std::min(nbk - 2, cus - 1));
-    hipLaunchKernelGGL((tri_inv_kernel<0, 1, 1>), dim3(nbk), dim3(64), 0, s, LU, lda, n, inv);
-    for (int k0 = 0; k0 < nrhs; k0 += TP_RHS) {
+    if (lower) hipLaunchKernelGGL((tri_inv_kernel<0, 1, 1>), dim3(nbk), dim3(64), 0, s, LU, lda, n, inv);
+    for (int k0 = 0; k0 < (lower ? nrhs : 0); k0 += TP_RHS) {
         const int kc = min(TP_RHS, nrhs - k0);
         ProfScope ps(tl_prof, s, PROF_TRS, 2.0 * (double)n * n * kc,
                      8.0 * ((double)n * n + 4.0 * n * kc));
@@ -867,12 +871,14 @@ hipError_t trs_chain_f64(hipStream_t s, int n, int nrhs, const double *LU, long 
     hipLaunchKernelGGL((tri_inv_kernel<0, 0, 0>), dim3(nbk), dim3(64), 0, s, LU, lda, n, inv);
     for (int k0 = 0; k0 < nrhs; k0 += TP_RHS) {
         const int kc = min(TP_RHS, nrhs - k0);
-        ProfScope ps(tl_prof, s, PROF_TRS, 0.0, 0.0);
-        const double *tsrc = T + (long)k0 * n;
+        ProfScope ps(tl_prof, s, PROF_TRS, lower ? 0.0 : (double)n * n * kc,
+                     lower ? 0.0 : 8.0 * ((double)n * n / 2.0 + 2.0 * n * kc));
+        const double *tsrc = lower ? T + (long)k0 * n : Bp + (long)k0 * ldbp;
+        const long ldt = lower ? (long)n : ldbp;
         double *xdst = X + (long)k0 * ldx;
-        if (kc == 1) FLX_TC(0, 1, tsrc, (long)n, xdst, ldx);
-        else if (kc <= 4) FLX_TC(0, 4, tsrc, (long)n, xdst, ldx);
-        else FLX_TC(0, 16, tsrc, (long)n, xdst, ldx);
+        if (kc == 1) FLX_TC(0, 1, tsrc, ldt, xdst, ldx);
+        else if (kc <= 4) FLX_TC(0, 4, tsrc, ldt, xdst, ldx);
+        else FLX_TC(0, 16, tsrc, ldt, xdst, ldx);
 #undef FLX_TC
     }
     return hipGetLastError();
