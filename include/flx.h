@@ -143,8 +143,12 @@ int flx_dgetrs(flx_handle h, int trans, int n, int nrhs, const double *A,
                int lda, const int *ipiv, double *B, int ldb);
 
 /* trf + trs; *info_host (host int) receives INFO and the solve is skipped
- * when INFO > 0, as in flens::lapack::sv (flens/lapack/ge/sv.tcc:58-77).
- * This entry synchronises the stream to read INFO. */
+ * when INFO > 0 (B left as it was), as in flens::lapack::sv
+ * (flens/lapack/ge/sv.tcc:58-77).  The factors and pivots are flx_dgetrf's;
+ * B is carried through the factorisation as extra columns (L^{-1} P B from
+ * its own interchanges and block updates), then the U solve of flx_dgetrs
+ * runs, so X agrees with trf + trs to rounding (not bit for bit).  This
+ * entry synchronises the stream to read INFO. */
 int flx_dgesv(flx_handle h, int n, int nrhs, double *A, int lda, int *ipiv,
               double *B, int ldb, int *info_host);
 
