@@ -191,6 +191,54 @@ def test_gesv_singular_skips_solve(gpu):
     assert np.array_equal(b, b_before)
 
 
+@pytest.mark.parametrize("n,nrhs", [(300, 3), (1100, 20)])
+def test_device_gesv_padded_leading_dimensions(gpu, n, nrhs):
+    """flx_dgesv on views with lda > n and ldb > n: the factors and pivots
+    equal flx_dgetrf's bit for bit (B is carried through the same
+    factorisation as extra columns), X solves the system (dget02 < 30), and
+    the padding rows are never written; a singular A leaves B untouched."""
+    import ctypes as C
+    import torch
+    lda, ldb = n + 5, n + 3
+    a0 = O.uniform(n, n, 900 + n)
+    b0 = O.uniform(n, nrhs, 901 + n)
+
+    def padded(x, ld, fill):
+        buf = np.full((ld, x.shape[1]), fill, order="F")
+        buf[: x.shape[0], :] = x
+        return torch.from_numpy(np.ascontiguousarray(buf.T)).to("cuda")
+
+    def unpad(t, rows):
+        return np.asfortranarray(t.cpu().numpy().T[:rows, :])
+
+    L = gpu.lib()
+    h = gpu.default_handle()
+    tA, tB = padded(a0, lda, 7.5), padded(b0, ldb, -3.25)
+    ipiv = gpu._int_buffer(n)
+    info = C.c_int(-1)
+    assert L.flx_dgesv(h.ptr, n, nrhs, gpu._vp(tA.data_ptr()), lda, gpu._vp(ipiv.data_ptr()),
+                       gpu._vp(tB.data_ptr()), ldb, C.byref(info)) == 0
+    assert info.value == 0
+    full_a, full_b = tA.cpu().numpy().T, tB.cpu().numpy().T
+    assert np.all(full_a[n:, :] == 7.5) and np.all(full_b[n:, :] == -3.25)
+    x = unpad(tB, n)
+    assert O.dget02(a0, x, b0) < 30
+    # the factors and pivots are flx_dgetrf's
+    A2 = gpu.DeviceMatrix.from_numpy(a0)
+    piv2, info2 = gpu.lapack.trf(A2)
+    assert int(info2.cpu()[0]) == 0
+    assert np.array_equal(ipiv.cpu().numpy()[:n], piv2.cpu().numpy()[:n])
+    assert np.array_equal(unpad(tA, n), A2.numpy())
+    # singular: INFO > 0, B left as it was
+    a1 = a0.copy(order="F")
+    a1[:, 3] = 0.0
+    tA, tB = padded(a1, lda, 0.0), padded(b0, ldb, 0.0)
+    assert L.flx_dgesv(h.ptr, n, nrhs, gpu._vp(tA.data_ptr()), lda, gpu._vp(ipiv.data_ptr()),
+                       gpu._vp(tB.data_ptr()), ldb, C.byref(info)) == 0
+    assert info.value == 4
+    assert np.array_equal(unpad(tB, n), b0)
+
+
 def test_laswp_seam(gpu):
     a0 = O.uniform(30, 7, 5)
     piv = np.array([3, 5, 3, 30, 5, 6, 7, 29], np.int32)
