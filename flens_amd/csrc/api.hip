@@ -62,11 +62,13 @@ hipError_t init_work(flx_context *h)
     if (e != hipSuccess) return e;
     e = hipMemsetAsync(h->wk.moved, 0, sizeof(unsigned long long), h->stream);
     if (e != hipSuccess) return e;
-    // side-stream launches keep a quarter of the compute units free
+    // side-stream launches keep a quarter of the compute units free, an
+    // eighth while the trailing matrix is large
     int cus = 0;
     e = hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, h->device);
     if (e != hipSuccess) return e;
     h->wk.side_groups = cus > 8 ? cus - cus / 4 : cus;
+    h->wk.side_groups_early = cus > 8 ? cus - cus / 8 : cus;
     h->wk.cus = cus;
     return hipMemsetAsync(h->wk.flags, 0, sizeof(int) * 4, h->stream);
 }
@@ -165,6 +167,7 @@ extern "C" int flx_set_lu_side_groups(flx_handle h, int which, int groups)
     int cus = 0;
     FLX_TRY(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, h->device));
     h->wk.side_groups = (groups == 0) ? (cus > 8 ? cus - cus / 4 : cus) : groups;
+    h->wk.side_groups_early = (groups == 0) ? (cus > 8 ? cus - cus / 8 : cus) : groups;
     return FLX_SUCCESS;
 }
 

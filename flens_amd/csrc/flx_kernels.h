@@ -81,6 +81,9 @@ struct LuWork {
     // panels and the small launches of the next block always find free
     // compute units (a trailing update left uncapped fills the device)
     int side_groups = 0;
+    // the same for the first blocks, whose trailing updates outweigh their
+    // panels (more rows left than early_rows_frac of the matrix)
+    int side_groups_early = 0;
     int cus = 0;                // compute units of the device
     // fused panel step (lookahead): staged rows (LU_STAGE_ROWS x LU_STAGE_LD
     // doubles), their destinations (+ the count at [LU_STAGE_TOP]) and perm

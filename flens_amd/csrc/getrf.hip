@@ -1049,6 +1049,14 @@ hipError_t getrf_rm_f64(hipStream_t s, LuWork &wk, double *W, long ldw, int m, i
         // deferred moves for all of it).
         const int nl = nr > 0 ? (nr < nb ? nr : nb) : 0;
         const int c0 = J + jb + nl;
+        // While the trailing matrix is large its update on the side stream
+        // takes longer than the next block's panels, and the block end waits
+        // for it: those blocks' side launches get an eighth of the compute
+        // units more.  Measured: at n = 8192, nb = 128 up to 0.8 n rows
+        // (38.65 -> 38.2 ms per solve); at n = 16384, nb = 256 up to 0.6 n
+        // (142.0 -> 133.2 ms); more workgroups, or for more blocks, is slower.
+        const bool early = (long)(m - J - jb) * 10 > (long)m * (nb >= 256 ? 6 : 8);
+        const int sg = early && wk.side_groups_early > 0 ? wk.side_groups_early : wk.side_groups;
         FLX_CHECK(apply_moves<double>(s, wk.prof, W, ldw, m, 0, J + jb, c0, mvb, wk.moved));
         const bool side_moves = J > 0 || n > c0;
         const bool split = wk.ev_n && nl > PANEL_W;
@@ -1065,17 +1073,17 @@ hipError_t getrf_rm_f64(hipStream_t s, LuWork &wk, double *W, long ldw, int m, i
             FLX_CHECK(rm_update(s, wk.prof, W, ldw, J + jb, m - J - jb, J + jb, nf, J, jb));
             if (split) {
                 FLX_CHECK(trsm_rm_llu(wk.side, wk.prof, L11, ldw, W + (long)J * ldw + J + jb + nf,
-                                      ldw, jb, nl - nf, wk.side_groups));
+                                      ldw, jb, nl - nf, sg));
                 FLX_CHECK(rm_update(wk.side, wk.prof, W, ldw, J + jb, m - J - jb, J + jb + nf,
-                                    nl - nf, J, jb, wk.side_groups));
+                                    nl - nf, J, jb, sg));
                 FLX_CHECK(hipEventRecord(wk.ev_n, wk.side));
                 next_pending = true;
             }
             if (n > c0) {
                 FLX_CHECK(trsm_rm_llu(wk.side, wk.prof, L11, ldw, W + (long)J * ldw + c0, ldw, jb,
-                                      n - c0, wk.side_groups));
+                                      n - c0, sg));
                 FLX_CHECK(rm_update(wk.side, wk.prof, W, ldw, J + jb, m - J - jb, c0, n - c0, J,
-                                    jb, wk.side_groups));
+                                    jb, sg));
             }
         }
         if (side_moves || split) {

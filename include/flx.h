@@ -76,7 +76,9 @@ int flx_set_lu_lookahead(flx_handle h, int depth);
 /* Workgroups the lookahead's second stream may occupy with the rest of a
  * trailing update, so that the next block's panels keep compute units:
  * which = 0 the FP64 LU (0: the default, three quarters of the compute
- * units), which = 1 the mixed-precision factor (0: the default, no cap).
+ * units, seven eighths for the first blocks while more than 0.8 of the rows
+ * remain -- 0.6 at nb >= 256; another value holds for every block),
+ * which = 1 the mixed-precision factor (0: the default, no cap).
  * Tuning only; results do not depend on it.  No counterpart in the
  * reference. */
 int flx_set_lu_side_groups(flx_handle h, int which, int groups);
