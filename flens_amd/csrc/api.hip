@@ -300,8 +300,21 @@ int dgetrf_dev(flx_context *h, int m, int n, double *A, int lda, int *ipiv, int 
     }
     double *W = (double *)h->W.p;
     FLX_TRY(fill_int(h->stream, info, 1, INT_MAX));
-    FLX_TRY(transpose_f64(h->stream, A, lda, W, ldw, m, n));
-    if (nrhs > 0) FLX_TRY(transpose_f64(h->stream, B, ldb, W + n, ldw, n, nrhs));
+    // Into the working copy: the first block's columns on the stream, the
+    // rest (and B) on the side stream beside the first block's panels; the
+    // first block end waits for them (ev_r).
+    const int bw = lu_block_width(h->opt);
+    const int c1 = h->opt.lookahead && n > bw ? bw : n;
+    FLX_TRY(transpose_f64(h->stream, A, lda, W, ldw, m, c1));
+    if (c1 < n || nrhs > 0) {
+        FLX_TRY(hipEventRecord(h->wk.ev_a, h->stream));
+        FLX_TRY(hipStreamWaitEvent(h->wk.side, h->wk.ev_a, 0));
+        if (c1 < n)
+            FLX_TRY(transpose_f64(h->wk.side, A + (size_t)c1 * lda, lda, W + c1, ldw, m, n - c1));
+        if (nrhs > 0) FLX_TRY(transpose_f64(h->wk.side, B, ldb, W + n, ldw, n, nrhs));
+        FLX_TRY(hipEventRecord(h->wk.ev_r, h->wk.side));
+        h->wk.side_pending = true;
+    }
     FLX_TRY(getrf_rm_f64(h->stream, h->wk, W, ldw, m, nc, (int *)h->piv0.p, info, h->opt));
     if (out) FLX_TRY(transpose_f64(h->stream, W, ldw, out, ldo, n, m));
     else FLX_TRY(transpose_f64(h->stream, W, ldw, A, lda, n, m));

@@ -77,6 +77,10 @@ struct LuWork {
     hipStream_t side = nullptr; // default priority, ordered by the events
     hipEvent_t ev_a = nullptr, ev_r = nullptr;
     hipEvent_t ev_n = nullptr;  // optional: the next block's columns done on the side stream
+    // set by the caller when ev_r already marks side-stream work the first
+    // block end must wait for (the working copy's columns right of the first
+    // block, copied in there); cleared by getrf_rm_f64
+    bool side_pending = false;
     // workgroups the side stream's launches may use at once, so that the
     // panels and the small launches of the next block always find free
     // compute units (a trailing update left uncapped fills the device)
@@ -106,6 +110,9 @@ struct LuOptions {
 };
 
 size_t lu_exch_words();
+// the outer block width getrf_rm_f64 uses for opt (nb clamped and rounded
+// to a multiple of the panel width)
+int lu_block_width(const LuOptions &opt);
 
 // Panel kernel geometry: 512 rows per workgroup, at most 64 workgroups; the
 // interchange list has one slice of 1 + 2*64 ints per workgroup:

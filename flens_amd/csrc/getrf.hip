@@ -944,23 +944,40 @@ template hipError_t defer_moves<double>(hipStream_t, Profiler *, double *, long,
 template hipError_t defer_moves<float>(hipStream_t, Profiler *, float *, long, int, int, int, int,
                                        const int *, const int *, int *, unsigned long long *);
 
-hipError_t getrf_rm_f64(hipStream_t s, LuWork &wk, double *W, long ldw, int m, int n,
-                        int *piv0, int *info, const LuOptions &opt)
+int lu_block_width(const LuOptions &opt)
 {
-    const int mn = m < n ? m : n;
-    if (m <= 0 || n <= 0) return hipSuccess;
     int nb = opt.nb, ib = opt.ib;
     if (ib > PANEL_W) ib = PANEL_W;
     if (ib < 1) ib = PANEL_W;
     if (nb < ib) nb = ib;
     if (nb > TR_KMAX) nb = TR_KMAX;
-    nb = (nb / ib) * ib;
-    if (m > PANEL_ROWS_MAX) return hipErrorInvalidValue;
+    return (nb / ib) * ib;
+}
+
+hipError_t getrf_rm_f64(hipStream_t s, LuWork &wk, double *W, long ldw, int m, int n,
+                        int *piv0, int *info, const LuOptions &opt)
+{
+    const int mn = m < n ? m : n;
     const bool look = opt.lookahead && wk.perm && wk.inv && wk.mv && wk.side && wk.ev_a &&
                       wk.ev_r;
+    bool rest_pending = false, next_pending = false;
+    if (wk.side_pending) {
+        // side-stream work the caller queued (ev_r): the first block end
+        // waits for it, or at once without the lookahead (and on every
+        // early return)
+        wk.side_pending = false;
+        const bool run = m > 0 && n > 0 && m <= PANEL_ROWS_MAX;
+        if (look && run) rest_pending = true;
+        else FLX_CHECK(hipStreamWaitEvent(s, wk.ev_r, 0));
+    }
+    if (m <= 0 || n <= 0) return hipSuccess;
+    int ib = opt.ib;
+    if (ib > PANEL_W) ib = PANEL_W;
+    if (ib < 1) ib = PANEL_W;
+    const int nb = lu_block_width(opt);
+    if (m > PANEL_ROWS_MAX) return hipErrorInvalidValue;
     // the composed permutation, when requested, starts as the identity
     if (wk.perm) FLX_CHECK(iota(s, wk.perm, m));
-    bool rest_pending = false, next_pending = false;
     for (int J = 0; J < mn; J += nb) {
         const int jb = (mn - J < nb) ? mn - J : nb;
         // the block's move list, double-buffered by block parity: the side
