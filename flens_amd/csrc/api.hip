@@ -266,10 +266,10 @@ extern "C" int flx_dgemm(flx_handle h, int transa, int transb, int m, int n, int
 // LU of device column-major A via the row-major working copy
 int dgetrf_dev(flx_context *h, int m, int n, double *A, int lda, int *ipiv, int *info,
                bool keep_perm, double *out, int ldo, int nrhs, const double *B, int ldb,
-               double *Y)
+               double *Y, long *w_ld)
 {
     const int mn = m < n ? m : n;
-    if (nrhs > 0 && (m != n || !B || !Y)) return FLX_INVALID_ARG;
+    if (nrhs > 0 && (m != n || !B || (!Y && !w_ld))) return FLX_INVALID_ARG;
     const int nc = n + (nrhs > 0 ? nrhs : 0);   // columns of the working copy
     FLX_TRY(init_work(h));
     // with right-hand sides along, rows padded to a multiple of 256 doubles:
@@ -316,9 +316,18 @@ int dgetrf_dev(flx_context *h, int m, int n, double *A, int lda, int *ipiv, int 
         h->wk.side_pending = true;
     }
     FLX_TRY(getrf_rm_f64(h->stream, h->wk, W, ldw, m, nc, (int *)h->piv0.p, info, h->opt));
-    if (out) FLX_TRY(transpose_f64(h->stream, W, ldw, out, ldo, n, m));
-    else FLX_TRY(transpose_f64(h->stream, W, ldw, A, lda, n, m));
-    if (nrhs > 0) FLX_TRY(transpose_f64(h->stream, W + n, ldw, Y, n, nrhs, n));
+    if (nrhs > 0 && !Y) {
+        // the caller solves from W; the factors go back beside that solve
+        FLX_TRY(hipEventRecord(h->wk.ev_a, h->stream));
+        FLX_TRY(hipStreamWaitEvent(h->wk.side, h->wk.ev_a, 0));
+        FLX_TRY(transpose_f64(h->wk.side, W, ldw, out ? out : A, out ? ldo : lda, n, m));
+        FLX_TRY(hipEventRecord(h->wk.ev_r, h->wk.side));
+        *w_ld = ldw;
+    } else {
+        if (out) FLX_TRY(transpose_f64(h->stream, W, ldw, out, ldo, n, m));
+        else FLX_TRY(transpose_f64(h->stream, W, ldw, A, lda, n, m));
+        if (nrhs > 0) FLX_TRY(transpose_f64(h->stream, W + n, ldw, Y, n, nrhs, n));
+    }
     FLX_TRY(piv0_to_ipiv(h->stream, (int *)h->piv0.p, ipiv, mn, info));
     return FLX_SUCCESS;
 }
@@ -399,26 +408,40 @@ extern "C" int flx_dgesv(flx_handle h, int n, int nrhs, double *A, int lda, int 
     int *dinfo = (int *)h->info.p;
     // B rides along as extra columns of the factorisation (the forward
     // solve L^{-1} P B is then done by its interchanges and block updates),
-    // so only the U solve remains; B is written only by that solve
-    double *Y = nullptr;
-    if (nrhs > 0) {
-        FLX_TRY(ensure(h, h->B, sizeof(double) * (size_t)n * (size_t)nrhs));
-        Y = (double *)h->B.p;
+    // so only the U solve remains, read straight from the row-major working
+    // copy while the factors go back to A on the side stream; B is written
+    // only by that solve
+    if (nrhs == 0) {
+        int st = dgetrf_dev(h, n, n, A, lda, ipiv, dinfo, true);
+        if (st) return st;
+        int info = 0;
+        FLX_TRY(hipMemcpyAsync(&info, dinfo, sizeof(int), hipMemcpyDeviceToHost, h->stream));
+        FLX_RET(flx_sync(h));
+        if (info_host) *info_host = info;
+        return FLX_SUCCESS;
     }
-    int st = dgetrf_dev(h, n, n, A, lda, ipiv, dinfo, true, nullptr, 0, nrhs, B, ldb, Y);
+    FLX_TRY(ensure(h, h->tinv, sizeof(double) * trs_inv_doubles(n)));
+    FLX_TRY(ensure_zeroed(h, h->trsw, sizeof(uint64_t) * trs_chain_words(n)));
+    long wld = 0;
+    int st = dgetrf_dev(h, n, n, A, lda, ipiv, dinfo, true, nullptr, 0, nrhs, B, ldb, nullptr,
+                        &wld);
     if (st) return st;
     int info = 0;
     FLX_TRY(hipMemcpyAsync(&info, dinfo, sizeof(int), hipMemcpyDeviceToHost, h->stream));
-    FLX_RET(flx_sync(h));   // INFO is only meaningful when every panel wait succeeded
-    if (info_host) *info_host = info;
-    if (info == 0 && nrhs > 0) {
-        FLX_TRY(ensure(h, h->tinv, sizeof(double) * trs_inv_doubles(n)));
-        FLX_TRY(ensure_zeroed(h, h->trsw, sizeof(uint64_t) * trs_chain_words(n)));
-        FLX_TRY(trs_chain_f64(h->stream, n, nrhs, A, lda, Y, n, nullptr, B, ldb,
-                              (double *)h->tinv.p, (uint64_t *)h->trsw.p, &h->trs_epoch,
-                              h->wk.flags, h->wk.cus, 0));
+    // INFO is only meaningful when every panel wait succeeded
+    st = flx_sync(h);
+    if (st == FLX_SUCCESS) {
+        if (info_host) *info_host = info;
+        if (info == 0) {
+            const double *W = (const double *)h->W.p;
+            FLX_TRY(trs_chain_f64(h->stream, n, nrhs, W, wld, W + n, wld, nullptr, B, ldb,
+                                  (double *)h->tinv.p, (uint64_t *)h->trsw.p, &h->trs_epoch,
+                                  h->wk.flags, h->wk.cus, 0, 1));
+        }
     }
-    return FLX_SUCCESS;
+    // A holds the factors once the side stream's copy is done
+    FLX_TRY(hipStreamWaitEvent(h->stream, h->wk.ev_r, 0));
+    return st;
 }
 
 extern "C" int flx_dlaswp(flx_handle h, int n, double *A, int lda, int k1, int k2,

@@ -598,7 +598,7 @@ FLX_DEV bool tc_settle(const uint64_t *p, uint64_t &hi, uint64_t &lo, unsigned e
 // the chain's operands of step g into registers: the inverse of its
 // diagonal block and the factor block left of it (columns of block g - 1),
 // IPT entries of each per thread, coalesced; its right-hand side rows
-template <int FWD, int R>
+template <int FWD, int R, int TR>
 FLX_DEV void tc_fetch(int g, int nbk, int n, const double *__restrict__ LU, long lda,
                       const double *__restrict__ inv, const double *__restrict__ Bin, long ldb,
                       int nrhs, double (&inext)[TC_BS * TC_BS / 256],
@@ -614,13 +614,20 @@ FLX_DEV void tc_fetch(int g, int nbk, int n, const double *__restrict__ LU, long
     const int c0 = g > 0 ? (FWD ? g - 1 : nbk - g) * TC_BS : 0;
 #pragma unroll
     for (int u = 0; u < IPT; ++u) {
-        const int q = tid + 256 * u, c = q / TC_BS, r = q % TC_BS;
-        tnext[u] = (g > 0 && r < rows && c0 + c < n) ? LU[(long)(c0 + c) * lda + r0 + r] : 0.0;
+        // consecutive threads read consecutive addresses: down a column of a
+        // column-major factor, along a row of a row-major one (TR)
+        const int q = tid + 256 * u;
+        const int c = TR ? q % TC_BS : q / TC_BS, r = TR ? q / TC_BS : q % TC_BS;
+        tnext[u] = (g > 0 && r < rows && c0 + c < n)
+                       ? (TR ? LU[(long)(r0 + r) * lda + c0 + c] : LU[(long)(c0 + c) * lda + r0 + r])
+                       : 0.0;
     }
 #pragma unroll
     for (int q = 0; q < 4; ++q) {
         const int rr = 16 * wave + lk + 4 * q;
-        bnext[q] = (rr < rows && lr < R && lr < nrhs) ? Bin[(long)lr * ldb + r0 + rr] : 0.0;
+        bnext[q] = (rr < rows && lr < R && lr < nrhs)
+                       ? (TR ? Bin[(long)(r0 + rr) * ldb + lr] : Bin[(long)lr * ldb + r0 + rr])
+                       : 0.0;
     }
 }
 
@@ -631,7 +638,9 @@ FLX_DEV void lds_barrier()
     asm volatile("s_waitcnt lgkmcnt(0)\n\ts_barrier" ::: "memory");
 }
 
-template <int FWD, int R>
+// TR = 1: the factor and the right-hand sides are row-major (element (i, j)
+// at LU[i * lda + j], Bin[i * ldb + j]); the solution is column-major.
+template <int FWD, int R, int TR>
 __global__ __launch_bounds__(256) void trs_chain_kernel(const double *__restrict__ LU, long lda,
                                                         int n, int nrhs,
                                                         const double *__restrict__ inv,
@@ -669,14 +678,15 @@ __global__ __launch_bounds__(256) void trs_chain_kernel(const double *__restrict
         int spins = 0;
         bool bad = false;
         double inext[IPT], tnext[IPT], bnext[4];
-        tc_fetch<FWD, R>(0, nbk, n, LU, lda, inv, Bin, ldb, nrhs, inext, tnext, bnext);
+        tc_fetch<FWD, R, TR>(0, nbk, n, LU, lda, inv, Bin, ldb, nrhs, inext, tnext, bnext);
         for (int g = 0; g < nbk; ++g) {
             const int bi = blk(g), r0 = bi * TC_BS, rows = min(TC_BS, n - r0);
 #pragma unroll
             for (int u = 0; u < IPT; ++u) {
                 const int q = tid + 256 * u;
                 s_inv[q / TC_BS][q % TC_BS] = inext[u];
-                s_t[q % TC_BS][q / TC_BS] = tnext[u];
+                if (TR) s_t[q / TC_BS][q % TC_BS] = tnext[u];
+                else s_t[q % TC_BS][q / TC_BS] = tnext[u];
             }
             double v[4];
 #pragma unroll
@@ -720,7 +730,7 @@ __global__ __launch_bounds__(256) void trs_chain_kernel(const double *__restrict
             // the next step's operands are requested now and arrive during
             // this step's second product and stores
             if (g + 1 < nbk)
-                tc_fetch<FWD, R>(g + 1, nbk, n, LU, lda, inv, Bin, ldb, nrhs, inext, tnext, bnext);
+                tc_fetch<FWD, R, TR>(g + 1, nbk, n, LU, lda, inv, Bin, ldb, nrhs, inext, tnext, bnext);
 #pragma unroll
             for (int q = 0; q < 4; ++q) s_v[16 * wave + lk + 4 * q][lr] = v[q];
             lds_barrier();
@@ -773,7 +783,9 @@ __global__ __launch_bounds__(256) void trs_chain_kernel(const double *__restrict
                 const int c0 = blk(d0 + b) * TC_BS + wave * 16;
 #pragma unroll
                 for (int c = 0; c < 16; ++c)
-                    lv[b][c] = (b < cnt && row_ok && c0 + c < n) ? LU[(long)(c0 + c) * lda + row] : 0.0;
+                    lv[b][c] = (b < cnt && row_ok && c0 + c < n)
+                                   ? (TR ? LU[(long)row * lda + c0 + c] : LU[(long)(c0 + c) * lda + row])
+                                   : 0.0;
             }
             // the group's x values into LDS: every word requested, then
             // each checked against the epoch (a stale one is re-polled)
@@ -838,8 +850,9 @@ size_t trs_chain_words(int n) { return 2 * trs_persist_words(n); }
 hipError_t trs_chain_f64(hipStream_t s, int n, int nrhs, const double *LU, long lda,
                          const double *Bp, long ldbp, double *T, double *X, long ldx,
                          double *inv, uint64_t *w, unsigned *epoch, int *err, int cus,
-                         int lower)
+                         int lower, int rowmajor)
 {
+    if (rowmajor && lower) return hipErrorInvalidValue;   // the U solve only
     const int nbk = (n + TC_BS - 1) / TC_BS;
     if (nbk <= 0 || nrhs <= 0) return hipSuccess;
     const size_t half = trs_persist_words(n);
@@ -862,24 +875,37 @@ hipError_t trs_chain_f64(hipStream_t s, int n, int nrhs, const double *LU, long 
         const double *bsrc = Bp + (long)k0 * ldbp;
         double *tdst = T + (long)k0 * n;
 #define FLX_TC(FW_, R_, IN_, LDI_, OUT_, LDO_)                                                     \
-        hipLaunchKernelGGL((trs_chain_kernel<FW_, R_>), dim3(grid), dim3(256), 0, s, LU, lda, n, \
+        hipLaunchKernelGGL((trs_chain_kernel<FW_, R_, 0>), dim3(grid), dim3(256), 0, s, LU, lda, n, \
+                           kc, inv, IN_, LDI_, OUT_, LDO_, xw, pw, next(), err)
+#define FLX_TCR(R_, IN_, LDI_, OUT_, LDO_)                                                        \
+        hipLaunchKernelGGL((trs_chain_kernel<0, R_, 1>), dim3(grid), dim3(256), 0, s, LU, lda, n, \
                            kc, inv, IN_, LDI_, OUT_, LDO_, xw, pw, next(), err)
         if (kc == 1) FLX_TC(1, 1, bsrc, ldbp, tdst, (long)n);
         else if (kc <= 4) FLX_TC(1, 4, bsrc, ldbp, tdst, (long)n);
         else FLX_TC(1, 16, bsrc, ldbp, tdst, (long)n);
     }
-    hipLaunchKernelGGL((tri_inv_kernel<0, 0, 0>), dim3(nbk), dim3(64), 0, s, LU, lda, n, inv);
+    if (rowmajor)
+        hipLaunchKernelGGL((tri_inv_kernel<1, 0, 0>), dim3(nbk), dim3(64), 0, s, LU, lda, n, inv);
+    else
+        hipLaunchKernelGGL((tri_inv_kernel<0, 0, 0>), dim3(nbk), dim3(64), 0, s, LU, lda, n, inv);
     for (int k0 = 0; k0 < nrhs; k0 += TP_RHS) {
         const int kc = min(TP_RHS, nrhs - k0);
         ProfScope ps(tl_prof, s, PROF_TRS, lower ? 0.0 : (double)n * n * kc,
                      lower ? 0.0 : 8.0 * ((double)n * n / 2.0 + 2.0 * n * kc));
-        const double *tsrc = lower ? T + (long)k0 * n : Bp + (long)k0 * ldbp;
+        const double *tsrc = lower ? T + (long)k0 * n : rowmajor ? Bp + k0 : Bp + (long)k0 * ldbp;
         const long ldt = lower ? (long)n : ldbp;
         double *xdst = X + (long)k0 * ldx;
-        if (kc == 1) FLX_TC(0, 1, tsrc, ldt, xdst, ldx);
-        else if (kc <= 4) FLX_TC(0, 4, tsrc, ldt, xdst, ldx);
-        else FLX_TC(0, 16, tsrc, ldt, xdst, ldx);
+        if (rowmajor) {
+            if (kc == 1) FLX_TCR(1, tsrc, ldt, xdst, ldx);
+            else if (kc <= 4) FLX_TCR(4, tsrc, ldt, xdst, ldx);
+            else FLX_TCR(16, tsrc, ldt, xdst, ldx);
+        } else {
+            if (kc == 1) FLX_TC(0, 1, tsrc, ldt, xdst, ldx);
+            else if (kc <= 4) FLX_TC(0, 4, tsrc, ldt, xdst, ldx);
+            else FLX_TC(0, 16, tsrc, ldt, xdst, ldx);
+        }
 #undef FLX_TC
+#undef FLX_TCR
     }
     return hipGetLastError();
 }
