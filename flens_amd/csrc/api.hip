@@ -317,12 +317,7 @@ int dgetrf_dev(flx_context *h, int m, int n, double *A, int lda, int *ipiv, int 
     }
     FLX_TRY(getrf_rm_f64(h->stream, h->wk, W, ldw, m, nc, (int *)h->piv0.p, info, h->opt));
     if (nrhs > 0 && !Y) {
-        // the caller solves from W; the factors go back beside that solve
-        FLX_TRY(hipEventRecord(h->wk.ev_a, h->stream));
-        FLX_TRY(hipStreamWaitEvent(h->wk.side, h->wk.ev_a, 0));
-        FLX_TRY(transpose_f64(h->wk.side, W, ldw, out ? out : A, out ? ldo : lda, n, m));
-        FLX_TRY(hipEventRecord(h->wk.ev_r, h->wk.side));
-        *w_ld = ldw;
+        *w_ld = ldw;   // the caller solves from W and copies the factors out
     } else {
         if (out) FLX_TRY(transpose_f64(h->stream, W, ldw, out, ldo, n, m));
         else FLX_TRY(transpose_f64(h->stream, W, ldw, A, lda, n, m));
@@ -430,15 +425,20 @@ extern "C" int flx_dgesv(flx_handle h, int n, int nrhs, double *A, int lda, int 
     FLX_TRY(hipMemcpyAsync(&info, dinfo, sizeof(int), hipMemcpyDeviceToHost, h->stream));
     // INFO is only meaningful when every panel wait succeeded
     st = flx_sync(h);
-    if (st == FLX_SUCCESS) {
-        if (info_host) *info_host = info;
-        if (info == 0) {
-            const double *W = (const double *)h->W.p;
-            FLX_TRY(trs_chain_f64(h->stream, n, nrhs, W, wld, W + n, wld, nullptr, B, ldb,
-                                  (double *)h->tinv.p, (uint64_t *)h->trsw.p, &h->trs_epoch,
-                                  h->wk.flags, h->wk.cus, 0, 1));
-        }
+    const double *W = (const double *)h->W.p;
+    if (st == FLX_SUCCESS && info_host) *info_host = info;
+    if (st == FLX_SUCCESS && info == 0) {
+        // the solve's diagonal inverses first; the factors go back to A on
+        // the side stream beside the solve itself
+        FLX_TRY(trs_chain_f64(h->stream, n, nrhs, W, wld, W + n, wld, nullptr, B, ldb,
+                              (double *)h->tinv.p, (uint64_t *)h->trsw.p, &h->trs_epoch,
+                              h->wk.flags, h->wk.cus, 0, 1, h->wk.ev_a));
+    } else {
+        FLX_TRY(hipEventRecord(h->wk.ev_a, h->stream));
     }
+    FLX_TRY(hipStreamWaitEvent(h->wk.side, h->wk.ev_a, 0));
+    FLX_TRY(transpose_f64(h->wk.side, W, wld, A, lda, n, n));
+    FLX_TRY(hipEventRecord(h->wk.ev_r, h->wk.side));
     // A holds the factors once the side stream's copy is done
     FLX_TRY(hipStreamWaitEvent(h->stream, h->wk.ev_r, 0));
     return st;

@@ -90,9 +90,8 @@ inline long round_ld(long n) { return ((n + 31) / 32) * 32; }
 // working copy, so the factorisation's interchanges and updates leave
 // L^{-1} P B in them, copied to Y (n x nrhs, ld n); B itself is not written.
 // With Y null instead, the caller solves from the row-major working copy
-// (h->W, row stride *w_ld, L^{-1} P B at column n): the factors go back to A
-// on the side stream, and the caller's stream must wait for wk.ev_r before
-// A is read.
+// (h->W, row stride *w_ld, L^{-1} P B at column n) and copies the factors
+// back to A itself (nothing is copied out).
 int dgetrf_dev(flx_context *h, int m, int n, double *A, int lda, int *ipiv, int *info,
                bool keep_perm, double *out = nullptr, int ldo = 0, int nrhs = 0,
                const double *B = nullptr, int ldb = 0, double *Y = nullptr,

@@ -190,12 +190,13 @@ size_t trs_persist_words(int n);
 // trs_inv_doubles(n) doubles, w trs_chain_words(n) tagged words (zeroed
 // once), at most cus workgroups; lower = 0: Bp is L^{-1} P B already, only
 // the U solve runs; rowmajor = 1 (with lower = 0): LU and Bp are row-major
-// (the factorisation's working copy), X column-major.
+// (the factorisation's working copy), X column-major.  inv_done (optional)
+// is recorded on s once the U solve's diagonal inverses are launched.
 size_t trs_chain_words(int n);
 hipError_t trs_chain_f64(hipStream_t s, int n, int nrhs, const double *LU, long lda,
                          const double *Bp, long ldbp, double *T, double *X, long ldx,
                          double *inv, uint64_t *w, unsigned *epoch, int *err, int cus,
-                         int lower = 1, int rowmajor = 0);
+                         int lower = 1, int rowmajor = 0, hipEvent_t inv_done = nullptr);
 hipError_t trs_persist_f64(hipStream_t s, int n, int nrhs, const double *LU, long lda,
                            const double *Bp, long ldbp, double *T, double *X, long ldx,
                            uint64_t *xt, unsigned *epoch, int *err, int trans = 0);

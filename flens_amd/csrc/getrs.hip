@@ -850,7 +850,7 @@ size_t trs_chain_words(int n) { return 2 * trs_persist_words(n); }
 hipError_t trs_chain_f64(hipStream_t s, int n, int nrhs, const double *LU, long lda,
                          const double *Bp, long ldbp, double *T, double *X, long ldx,
                          double *inv, uint64_t *w, unsigned *epoch, int *err, int cus,
-                         int lower, int rowmajor)
+                         int lower, int rowmajor, hipEvent_t inv_done)
 {
     if (rowmajor && lower) return hipErrorInvalidValue;   // the U solve only
     const int nbk = (n + TC_BS - 1) / TC_BS;
@@ -888,6 +888,10 @@ hipError_t trs_chain_f64(hipStream_t s, int n, int nrhs, const double *LU, long 
         hipLaunchKernelGGL((tri_inv_kernel<1, 0, 0>), dim3(nbk), dim3(64), 0, s, LU, lda, n, inv);
     else
         hipLaunchKernelGGL((tri_inv_kernel<0, 0, 0>), dim3(nbk), dim3(64), 0, s, LU, lda, n, inv);
+    if (inv_done) {
+        hipError_t e = hipEventRecord(inv_done, s);
+        if (e != hipSuccess) return e;
+    }
     for (int k0 = 0; k0 < nrhs; k0 += TP_RHS) {
         const int kc = min(TP_RHS, nrhs - k0);
         ProfScope ps(tl_prof, s, PROF_TRS, lower ? 0.0 : (double)n * n * kc,
